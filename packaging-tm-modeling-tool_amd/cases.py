@@ -1,0 +1,183 @@
+"""Workload specs: the bare-die FCBGA (48 x 48 x 14 hexes), the lidded
+full-fidelity FCBGA (160 x 160 x 52 hexes, graded) and small test boxes.
+
+Every value the workload statement leaves open is pinned here, once:
+substrate split (build-up 0.2 mm per side, core 0.8 mm), Poisson ratios of
+the build-up, bump and TIM layers (0.3), the die and bump layers present only
+under the die, the BGA field (pitch 1.0 mm, inset 1.0 mm), the grading law of
+the lidded mesh, the lid geometry and the six temperatures.
+"""
+from __future__ import annotations
+
+import math
+
+from .meshspec import PART, linspace_ref
+
+# ---- materials (E in MPa, CTE in ppm/K) -----------------------------------------
+CORE = dict(name="core", E=26000.0, nu=0.2, cte=15.0, part=PART["SUB"])
+BUILDUP = dict(name="buildup", E=5000.0, nu=0.3, cte=40.0, part=PART["SUB"])
+BUMP_UF = dict(name="bump_uf", E=8000.0, nu=0.3, cte=30.0, part=PART["UF_BUMP"])
+SI = dict(name="si", E=131000.0, nu=0.28, cte=2.6, part=PART["DIE"])
+TIM = dict(name="tim", E=10.0, nu=0.3, cte=30.0, part=PART["TIM"])
+LID = dict(name="lid_cu", E=117000.0, nu=0.34, cte=17.0, part=PART["LID"])
+LIDADH = dict(name="lid_adh", E=2000.0, nu=0.3, cte=60.0, part=PART["LIDADH"])
+
+
+def _stack(layers):
+    """zs from (thickness, n_elements) pairs; cumulative sums in float64."""
+    zs = [0.0]
+    for t, n in layers:
+        z0 = zs[-1]
+        for q in range(1, n + 1):
+            zs.append(z0 + t * q / n)
+    return zs
+
+
+def cfg1_spec() -> dict:
+    """Bare-die FCBGA, uniform 48 x 48 in-plane x 14 layers.
+
+    Substrate 40 x 40 x 1.2 mm (build-up 0.2 / core 0.8 / build-up 0.2 mm, 2/4/2
+    layers), bump/underfill 0.08 mm (2 layers) and die 20 x 20 x 0.775 mm
+    (4 layers) only under the die; 150 -> 25 C in one step; PCG to 1e-10.
+    """
+    xs = linspace_ref(-20.0, 20.0, 48)
+    zs = _stack([(0.2, 2), (0.8, 4), (0.2, 2), (0.08, 2), (0.775, 4)])
+    die = [-10.0, 10.0, -10.0, 10.0]
+    return {
+        "name": "cfg1_bare_die_48x48x14",
+        "xs": xs, "ys": list(xs), "zs": zs,
+        "materials": [BUILDUP, CORE, BUMP_UF, SI],
+        "cells": [
+            {"k0": 0, "k1": 2, "mat": 0},
+            {"k0": 2, "k1": 6, "mat": 1},
+            {"k0": 6, "k1": 8, "mat": 0},
+            {"k0": 8, "k1": 10, "mat": 2, "in": die},
+            {"k0": 10, "k1": 14, "mat": 3, "in": die},
+        ],
+        "Ta": 150.0, "Tb": 25.0, "stage": 0,
+        "constraint_span": 8.0,
+        "feature_x": [12, 36], "feature_y": [12, 36],
+        "bga": {"pitch": 1.0, "inset": 1.0, "sx": 40.0, "sy": 40.0},
+        "top_plane": 14,
+        "tol": 1e-10,
+    }
+
+
+def small_box_spec() -> dict:
+    """A 6 x 5 x 4 box exercising every assembly branch: three materials, a void
+    corner, an element scale, two superposed phases in one layer and a group born
+    in a later stage (stiffness x 1e-6, no load)."""
+    xs = linspace_ref(0.0, 6.0, 6)
+    ys = linspace_ref(0.0, 4.0, 5)
+    zs = [0.0, 0.3, 0.5, 0.9, 1.2]
+    mats = [
+        dict(name="A", E=130000.0, nu=0.28, cte=2.6, part=PART["DIE"]),
+        dict(name="B", E=20000.0, nu=0.34, cte=16.4, part=PART["SUB"]),
+        dict(name="C", E=5000.0, nu=0.3, cte=40.0, part=PART["UF_BUMP"]),
+        dict(name="D", E=9000.0, nu=0.35, cte=22.0, part=PART["SOLDER"], column=True),
+        dict(name="E", E=50000.0, nu=0.3, cte=10.0, part=PART["LID"], stage=1),
+    ]
+    return {
+        "name": "small_box",
+        "xs": xs, "ys": ys, "zs": zs,
+        "materials": mats,
+        "cells": [
+            {"k0": 0, "k1": 1, "mat": 1},
+            {"k0": 1, "k1": 2, "mat": 2, "scale": 0.7, "also": 3, "also_scale": 0.3},
+            {"k0": 2, "k1": 3, "mat": 0, "out": [4.5, 6.0, 3.0, 4.0]},
+            {"k0": 3, "k1": 4, "mat": 4, "in": [0.0, 3.0, 0.0, 4.0]},
+        ],
+        "Ta": 125.0, "Tb": 25.0, "stage": 0,
+        "constraint_span": 0.4 * min(3.0, 2.0),
+        "feature_x": [3], "feature_y": [],
+        "bga": {"pitch": 1.0, "inset": 0.5, "sx": 6.0, "sy": 4.0},
+        "top_plane": 3,
+        "tol": 1e-13,
+    }
+
+
+def plate_spec(n: int = 16, nz_sub: int = 2) -> dict:
+    """Bimaterial plate (the Timoshenko check of src/core/15-verify.js:164-181):
+    40 x 40 mm, two 0.5 mm layers, cooled by 100 C."""
+    xs = linspace_ref(-20.0, 20.0, n)
+    zs = _stack([(0.5, nz_sub), (0.5, nz_sub)])
+    return {
+        "name": f"plate_{n}",
+        "xs": xs, "ys": list(xs), "zs": zs,
+        "materials": [dict(name="B", E=110000.0, nu=0.34, cte=16.4, part=PART["SUB"]),
+                      dict(name="A", E=130000.0, nu=0.28, cte=2.6, part=PART["DIE"])],
+        "cells": [{"k0": 0, "k1": nz_sub, "mat": 0}, {"k0": nz_sub, "k1": 2 * nz_sub, "mat": 1}],
+        "Ta": 25.0, "Tb": -75.0, "stage": 0,
+        "constraint_span": 8.0, "feature_x": [], "feature_y": [],
+        "bga": {"pitch": 1.0, "inset": 1.0, "sx": 40.0, "sy": 40.0},
+        "top_plane": 2 * nz_sub, "tol": 1e-10,
+    }
+
+
+# staircase grading of the lidded mesh, in units of 0.005 mm: (size, count) runs
+# from the centre to the die edge, die edge to lid-foot edge, and across the foot
+_RUN_CENTRE_TO_DIE = [(100, 8), (80, 3), (64, 6), (50, 4), (40, 3), (32, 3), (25, 4), (20, 3)]
+_RUN_DIE_TO_FOOT = [(20, 5), (25, 6), (32, 7), (40, 6), (50, 7), (64, 4), (80, 2), (64, 5)]
+_RUN_FOOT = [(50, 4)]
+
+
+def graded_lines() -> list:
+    """161 grid lines on [-20, 20] mm graded toward the die edges (+-10 mm).
+
+    Element sizes take eight values 0.1 .. 0.5 mm, neighbouring sizes differ by
+    at most 1.28x (the reference mesher's growth limit is 1.3), the smallest sit
+    at the die edges and 4 x 0.25 mm span the 1 mm lid foot.  Lines are integer
+    multiples of 0.005 mm, so the die edges (+-10) and the foot edges (+-19)
+    fall exactly on grid lines.
+    """
+    units = [0]
+    for size, count in _RUN_CENTRE_TO_DIE + _RUN_DIE_TO_FOOT + _RUN_FOOT:
+        for _ in range(count):
+            units.append(units[-1] + size)
+    assert units[-1] == 4000 and len(units) == 81
+    right = [u / 200.0 for u in units]
+    return [-v for v in reversed(right[1:])] + right
+
+
+def cfg2_spec() -> dict:
+    """Lidded FCBGA at full fidelity: 160 x 160 in-plane (graded toward the die
+    edges) x 52 layers.
+
+    Stack (bottom to top): build-up 0.2 mm (4 layers), core 0.8 mm (12),
+    build-up 0.2 mm (4); bump/underfill 0.08 mm under the die with the lid
+    adhesive (E 2 GPa, CTE 60 ppm/K) at the same height under the lid foot (4);
+    die 0.775 mm (14); TIM 0.05 mm over the die (2); Cu lid plate 0.5 mm over
+    the whole package (12).  The Cu lid foot is a 1.0 mm ring at the substrate
+    edge spanning the die and TIM layers; between foot and die is the lid
+    cavity (void).  Six temperatures 25..260 C from 150 C stress-free, each
+    solve cold.
+    """
+    xs = graded_lines()
+    zs = _stack([(0.2, 4), (0.8, 12), (0.2, 4), (0.08, 4), (0.775, 14), (0.05, 2), (0.5, 12)])
+    assert len(zs) - 1 == 52
+    die = [-10.0, 10.0, -10.0, 10.0]
+    foot_in = [-19.0, 19.0, -19.0, 19.0]
+    return {
+        "name": "cfg2_lidded_160x160x52",
+        "xs": xs, "ys": list(xs), "zs": zs,
+        "materials": [BUILDUP, CORE, BUMP_UF, SI, TIM, LID, LIDADH],
+        "cells": [
+            {"k0": 0, "k1": 4, "mat": 0},
+            {"k0": 4, "k1": 16, "mat": 1},
+            {"k0": 16, "k1": 20, "mat": 0},
+            {"k0": 20, "k1": 24, "mat": 2, "in": die},
+            {"k0": 20, "k1": 24, "mat": 6, "out": foot_in},
+            {"k0": 24, "k1": 38, "mat": 3, "in": die},
+            {"k0": 24, "k1": 40, "mat": 5, "out": foot_in},
+            {"k0": 38, "k1": 40, "mat": 4, "in": die},
+            {"k0": 40, "k1": 52, "mat": 5},
+        ],
+        "Ta": 150.0, "Tb": 25.0, "stage": 0,
+        "constraint_span": 8.0,
+        "feature_x": [xs.index(v) for v in (-19.0, -10.0, 10.0, 19.0)],
+        "feature_y": [xs.index(v) for v in (-19.0, -10.0, 10.0, 19.0)],
+        "bga": {"pitch": 1.0, "inset": 1.0, "sx": 40.0, "sy": 40.0},
+        "top_plane": 52,
+        "temperatures": [25.0, 72.0, 119.0, 166.0, 213.0, 260.0],
+        "tol": 1e-10,
+    }

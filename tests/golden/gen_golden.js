@@ -1,0 +1,135 @@
+// Golden vectors from the reference JS core (run in the build container only:
+//   node tests/golden/gen_golden.js [/root/reference]
+// ). It loads src/core/*.js the way the reference's own tools/dev-core.mjs does,
+// builds each committed mesh spec (*_spec.json) with the reference mesher's
+// buildStructuredMesh, assembles one thermal step and solves it with the
+// reference LinearSolver at a tight tolerance, then writes:
+//   <name>_u.f64.gz, <name>_rhs.f64.gz  (Float64 little-endian, gzip)
+//   <name>_golden.json                  (solve record, warpage, sizes)
+//   element_golden.json                 (condensed element tables on sampled boxes)
+//   anand_golden.json                   (Anand material-point step sequences)
+// Only data leaves this script; no reference source is copied.
+'use strict';
+const fs = require('fs');
+const path = require('path');
+const zlib = require('zlib');
+
+const REF = process.argv[2] || '/root/reference';
+const HERE = __dirname;
+const dir = path.join(REF, 'src', 'core');
+const src = fs.readdirSync(dir).filter(f => f.endsWith('.js')).sort().map(f => fs.readFileSync(path.join(dir, f), 'utf8')).join('\n');
+const get = new Function(src + '\nreturn (name) => eval(name);')();
+
+const buildStructuredMesh = get('buildStructuredMesh'), constraints321 = get('constraints321');
+const FEModel = get('FEModel'), LinearSolver = get('LinearSolver');
+const shapeMatrices = get('shapeMatrices'), elementK33Fast = get('elementK33Fast'), elementFth = get('elementFth');
+const condense33 = get('condense33'), isoD = get('isoD'), columnD = get('columnD');
+const warpageJEITA = get('warpageJEITA'), bgaSites = get('bgaSites');
+
+const f64gz = (arr, file) => fs.writeFileSync(path.join(HERE, file), zlib.gzipSync(Buffer.from(Float64Array.from(arr).buffer), { level: 9 }));
+const list = a => Array.from(a);
+
+function testMaterial(m) {
+  return { id: m.name, name: m.name, elastic: { type: 'iso', E: { type: 'const', E: { v: m.E / 1000, u: 'GPa' } }, nu: { v: m.nu } }, cte: { type: 'const', a: { v: m.cte } } };
+}
+
+function assignFor(spec) {
+  const xs = spec.xs, ys = spec.ys;
+  const inside = (r, x, y) => x >= r[0] && x <= r[1] && y >= r[2] && y <= r[3];
+  return (i, j, k) => {
+    const xc = (xs[i] + xs[i + 1]) / 2, yc = (ys[j] + ys[j + 1]) / 2;
+    for (const r of spec.cells) {
+      if (!(r.k0 <= k && k < r.k1)) continue;
+      if (r.in && !inside(r.in, xc, yc)) continue;
+      if (r.out && inside(r.out, xc, yc)) continue;
+      const mats = spec.materials;
+      const out = [{ group: r.mat, scale: r.scale == null ? 1 : r.scale, part: mats[r.mat].part }];
+      if (r.also != null) out.push({ group: r.also, scale: r.also_scale == null ? 1 : r.also_scale, part: mats[r.also].part });
+      return out;
+    }
+    return [];
+  };
+}
+
+function solveSpec(name) {
+  const spec = JSON.parse(fs.readFileSync(path.join(HERE, name + '_spec.json'), 'utf8'));
+  const groups = spec.materials.map(m => ({ name: m.name, kind: 'homog', mat: m.name, stage: m.stage || 0, part: m.part, column: !!m.column }));
+  const mesh = buildStructuredMesh(spec.xs, spec.ys, spec.zs, groups, assignFor(spec));
+  mesh.constraints = constraints321(mesh, spec.constraint_span);
+  if (spec.feature_x && spec.feature_x.length) { mesh.featureX = Int32Array.from(spec.feature_x); mesh.featureY = Int32Array.from(spec.feature_y); }
+  const lib = spec.materials.map(testMaterial);
+  const model = new FEModel(mesh, { materials: lib });
+  const solver = new LinearSolver(model, { precond: 'mg', tol: spec.golden_tol || 1e-12 });
+  const rhs = new Float64Array(model.ndof);
+  const t0 = Date.now();
+  model.assembleStep(spec.Ta, spec.Tb, spec.stage || 0, rhs);
+  const rhsCopy = Float64Array.from(rhs);
+  const res = solver.solve(rhs, name);
+  const ms = Date.now() - t0;
+  const cfg = { bga: { pitch: spec.bga.pitch, inset: spec.bga.inset }, substrate: { sx: spec.bga.sx, sy: spec.bga.sy }, limits: {} };
+  const geom = { bga: bgaSites(cfg) };
+  const w = warpageJEITA(mesh, geom, res.u, cfg);
+  // top plane w at every existing node of that plane (i, j order)
+  const top = [], kt = spec.top_plane;
+  for (let i = 0; i < mesh.nx; i++) for (let j = 0; j < mesh.ny; j++) { const n = mesh.nodeId[(i * mesh.ny + j) * mesh.nz + kt]; if (n >= 0) top.push(res.u[3 * n + 2]); }
+  if (model.ndof <= 200000) {
+    f64gz(res.u, name + '_u.f64.gz');
+    f64gz(rhsCopy, name + '_rhs.f64.gz');
+  } else {
+    // large meshes: the bottom and top planes in full plus every 97th node, as (node, ux, uy, uz) rows
+    const rows = [];
+    const put = n => { rows.push(n, res.u[3 * n], res.u[3 * n + 1], res.u[3 * n + 2]); };
+    for (let i = 0; i < mesh.nx; i++) for (let j = 0; j < mesh.ny; j++) for (const k of [0, spec.top_plane]) { const n = mesh.nodeId[(i * mesh.ny + j) * mesh.nz + k]; if (n >= 0) put(n); }
+    for (let n = 0; n < mesh.nn; n += 97) put(n);
+    f64gz(rows, name + '_usample.f64.gz');
+    let s2 = 0; for (let q = 0; q < rhsCopy.length; q++) s2 += rhsCopy[q] * rhsCopy[q];
+    var rhsNorm = Math.sqrt(s2);
+  }
+  const out = {
+    name, nn: mesh.nn, ne: mesh.ne, ndof: model.ndof, constraints: list(mesh.constraints.dofs),
+    iters: res.iters, relres: res.relres, converged: res.converged, tol: solver.opts.tol, ms,
+    mg_levels: solver.pc && solver.pc.info ? solver.pc.info.levels : null,
+    warpage: { signed: w.signed, mag: w.mag, sign: w.sign, plane: w.plane, rmin: w.rmin, rmax: w.rmax, nsites: w.ballW.length },
+    top_w_max: top.reduce((a, b) => Math.max(a, b), -Infinity), top_w_min: top.reduce((a, b) => Math.min(a, b), Infinity),
+    rhs_norm: typeof rhsNorm === 'number' ? rhsNorm : null,
+  };
+  fs.writeFileSync(path.join(HERE, name + '_golden.json'), JSON.stringify(out, null, 1));
+  console.log(name, 'ndof', model.ndof, 'iters', res.iters, 'relres', res.relres.toExponential(3), 'ms', ms, 'warp', w.signed);
+}
+
+function elementCases() {
+  const cases = [];
+  const boxes = [[0.8333333333333357, 0.8333333333333357, 0.1], [0.1, 0.25, 0.05535714285714286], [1.0, 0.5, 2.0], [0.02, 0.5, 0.3]];
+  const mats = [isoD(131000, 0.28, 0.45, new Float64Array(36)), isoD(5000, 0.3, 0.45, new Float64Array(36)), columnD(9000, 0.35, new Float64Array(36)), isoD(26000, 0.2, 0.45, new Float64Array(36))];
+  const dF = [-125 * 2.6e-6, -125 * 40e-6, 100 * 22e-6, -90 * 15e-6];
+  for (let q = 0; q < boxes.length; q++) {
+    const [hx, hy, hz] = boxes[q], D = mats[q];
+    const s0 = new Float64Array(6);
+    for (let r = 0; r < 6; r++) s0[r] = (D[6 * r] + D[6 * r + 1] + D[6 * r + 2]) * dF[q];
+    const M = shapeMatrices(null, D);
+    const K = elementK33Fast(hx, hy, hz, M, new Float64Array(1089));
+    const f = elementFth(hx, hy, hz, null, s0, new Float64Array(33));
+    const c = condense33(K, f);
+    cases.push({ kind: 'homog', hx, hy, hz, D: list(D), s0: list(s0), K33: list(K), f33: list(f), Kc: list(c.Kc), fc: list(c.fc), X: list(c.X), xf: list(c.xf) });
+  }
+  // layered element: three zeta-station weight matrices and thermal-stress weights
+  {
+    const hx = 0.7, hy = 0.4, hz = 0.3;
+    const Ds = [isoD(26000, 0.2, 0.45, new Float64Array(36)), isoD(5000, 0.3, 0.45, new Float64Array(36)), isoD(17000, 0.25, 0.45, new Float64Array(36))];
+    const wz = [0.2, 0.5, 0.3];
+    const Wst = new Float64Array(108), Sst = new Float64Array(18);
+    for (let j = 0; j < 3; j++) for (let q = 0; q < 36; q++) Wst[36 * j + q] = wz[j] * Ds[j][q];
+    for (let j = 0; j < 3; j++) for (let r = 0; r < 6; r++) Sst[6 * j + r] = wz[j] * (Ds[j][6 * r] + Ds[j][6 * r + 1] + Ds[j][6 * r + 2]) * (-1e-3 * (j + 1));
+    const M = shapeMatrices(Wst, null);
+    const K = elementK33Fast(hx, hy, hz, M, new Float64Array(1089));
+    const f = elementFth(hx, hy, hz, Sst, null, new Float64Array(33));
+    const c = condense33(K, f);
+    cases.push({ kind: 'layered', hx, hy, hz, W: list(Wst), V: list(Sst), K33: list(K), f33: list(f), Kc: list(c.Kc), fc: list(c.fc), X: list(c.X), xf: list(c.xf) });
+  }
+  fs.writeFileSync(path.join(HERE, 'element_golden.json'), JSON.stringify(cases));
+  console.log('element cases', cases.length);
+}
+
+const what = process.argv.slice(3);
+if (!what.length || what.includes('element')) elementCases();
+for (const n of ['small_box', 'plate_16', 'cfg1', 'cfg2']) if (!what.length || what.includes(n)) solveSpec(n);
