@@ -1,0 +1,124 @@
+/*
+ * fcbga_hip.h -- C ABI of the MI355X (gfx950) hex8 thermoelastic solver.
+ *
+ * Drop-in for the reference's step solve: "mesh + materials + load step in,
+ * displacements out".  The reference has no FFI; its seam is the JS call
+ * surface inside the worker driver:
+ *   new FEModel(mesh, cfg)                    src/core/09-assembly.js:208-233
+ *   model.assembleStep(Ta, Tb, stage, rhs)    src/core/09-assembly.js:392-424
+ *   solver.solve(rhs, label, key)             src/core/10-solver.js:933-994
+ *   Analysis.solveStep(k)                     src/core/11-analysis.js:189-206
+ * fcbga_model_create replaces the FEModel constructor (mesh contract of
+ * buildStructuredMesh, src/core/06-mesher.js:66-71), and fcbga_step_solve fuses
+ * assembleStep + solve on the device (no assembled matrix) and returns the
+ * step increment du with the reference's solve record fields.
+ *
+ * All functions return 0 on success and a negative code on failure; the
+ * message is available from fcbga_last_error (per calling thread).
+ * Arrays are borrowed for the duration of the call only.
+ */
+#ifndef FCBGA_HIP_H
+#define FCBGA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCBGA_OK 0
+#define FCBGA_ERR_ARG -1
+#define FCBGA_ERR_DEVICE -2
+#define FCBGA_ERR_NOT_CONVERGED -3
+#define FCBGA_ERR_INTERNAL -4
+
+typedef struct fcbga_model fcbga_model;
+
+/* Mesh contract of buildStructuredMesh (src/core/06-mesher.js:14-72) plus the
+ * 3-2-1 constraint DOFs (src/core/06-mesher.js:112-116) and feature lines. */
+typedef struct fcbga_mesh {
+  int32_t nx, ny, nz;          /* grid lines in x, y, z */
+  int32_t nn, ne;              /* nodes, elements */
+  const double *xs, *ys, *zs;  /* nx, ny, nz coordinates (mm) */
+  const int32_t *node_id;      /* nx*ny*nz: compact node id (z fastest, then y, then x) or -1 */
+  const int32_t *cell_elems;   /* 2*(nx-1)*(ny-1)*(nz-1): up to two superposed elements per cell, -1 none */
+  const int32_t *e_group;      /* ne: material group */
+  const int8_t *e_part;        /* ne: part code (src/core/06-mesher.js:3-4) */
+  const double *e_scale;       /* ne: element scale (fill fraction) */
+  const int32_t *e_section;    /* ne or NULL: layered section of an element, -1 homogeneous */
+  int32_t ngroups, nsections;
+  int32_t nconstraints;
+  const int32_t *constraint_dofs; /* constrained DOFs 3*node+component */
+  int32_t nfeature_x, nfeature_y;
+  const int32_t *feature_x, *feature_y; /* grid-line indices kept on every coarse level */
+} fcbga_mesh;
+
+/* One step of the thermal path [Ta, Tb] in a stage: the per-group data of
+ * FEModel.prepareStep (src/core/09-assembly.js:346-365). */
+typedef struct fcbga_step {
+  double Ta, Tb;
+  int32_t stage;
+  const double *D;         /* ngroups*36: mean stiffness over the step (Voigt xx yy zz xy yz zx, MPa) */
+  const double *s0;        /* ngroups*6: thermal stress increment, integral of D dF over the step */
+  const uint8_t *born;     /* ngroups: 1 when the group is born in this stage */
+  const double *secW;      /* nsections*108: zeta-station stiffness weights of layered sections (or NULL) */
+  const double *secV;      /* nsections*18: zeta-station thermal stress weights (or NULL) */
+  const uint8_t *sec_born; /* nsections (or NULL) */
+} fcbga_step;
+
+#define FCBGA_PREC_FP64 0  /* FP64 PCG, FP64 multigrid (the plain comparator) */
+#define FCBGA_PREC_MIXED 1 /* FP64 flexible PCG, FP32 multigrid cycle, FP64 coarsest */
+
+#define FCBGA_PC_MG 0      /* shell-aggregate multigrid (src/core/10-solver.js:620-715) */
+#define FCBGA_PC_COLUMN 1  /* column block Gauss-Seidel sweeps only */
+#define FCBGA_PC_JACOBI 2  /* Jacobi scaling only */
+
+typedef struct fcbga_solve_opts {
+  double tol;         /* relative residual in the Jacobi-scaled norm (src/core/10-solver.js:1014) */
+  int32_t maxit;
+  int32_t precision;  /* FCBGA_PREC_* */
+  int32_t precond;    /* FCBGA_PC_* */
+  int32_t reserved;
+} fcbga_solve_opts;
+
+/* Solve record of LinearSolver.solve (src/core/10-solver.js:988-993) plus timings. */
+typedef struct fcbga_solve_stats {
+  int32_t iters;
+  int32_t converged;
+  double relres;
+  double r0;
+  double ms_total;    /* whole step: assembly + preconditioner set-up + PCG */
+  double ms_assemble; /* element tables, load, diagonal blocks */
+  double ms_setup;    /* multigrid set-up (numeric) */
+  double ms_pcg;      /* PCG iterations */
+  int32_t levels;
+  int32_t coarsest_dof;
+  int64_t ndof;
+  int32_t ntypes;     /* distinct condensed element matrices */
+  int32_t reserved;
+} fcbga_solve_stats;
+
+int fcbga_model_create(const fcbga_mesh *mesh, fcbga_model **out);
+void fcbga_model_destroy(fcbga_model *m);
+int64_t fcbga_model_ndof(const fcbga_model *m);
+
+/* Step solve with host output: du_out has 3*nn doubles in the reference's node order. */
+int fcbga_step_solve(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts, double *du_out,
+                     fcbga_solve_stats *stats);
+/* Same, leaving du in device memory (3*nn doubles at du_dev, written on the library's stream
+ * and complete on return). */
+int fcbga_step_solve_device(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts, double *du_dev,
+                            fcbga_solve_stats *stats);
+
+/* Test hooks: the assembled load f (3*nn, constraints applied) and y = K x with the
+ * step's unconstrained, unscaled stiffness (both in the reference's node order). */
+int fcbga_step_load(fcbga_model *m, const fcbga_step *step, double *f_out);
+int fcbga_step_apply(fcbga_model *m, const fcbga_step *step, const double *x, double *y_out);
+
+int fcbga_last_error(char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1372 @@
+// fcbga_hip.hip -- MI355X (gfx950) matrix-free hex8 thermoelastic step solve:
+// device assembly of condensed element tables, Jacobi-scaled PCG (FP64, or
+// FP64 flexible PCG around an FP32 multigrid cycle) with the shell-aggregate
+// semi-coarsening multigrid, behind the C ABI of include/fcbga_hip.h.
+//
+// Reference behaviour followed (file:line of the reference repository):
+//   element and condensation  src/core/07-element.js:19-120, src/core/09-assembly.js:26-133
+//   step assembly, birth      src/core/09-assembly.js:346-424
+//   constraints               src/core/09-assembly.js:445-464
+//   scaling, PCG              src/core/10-solver.js:852-866, 1001-1042
+//   multigrid                 src/core/10-solver.js:12-715
+// Device layout: see fe_core.h.  No assembled fine matrix exists: every fine
+// operator row is gathered from the condensed element tables.
+#include "../../include/fcbga_hip.h"
+#include "backend.h"
+#include "fe_core.h"
+#include "mg_setup.h"
+
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    p = (T *)be::dmalloc(sizeof(T) * (count ? count : 1));
+    n = count;
+  }
+  void release() {
+    be::dfree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T *src, size_t count, be::stream_t s) {
+    alloc(count);
+    be::h2d(p, src, sizeof(T) * count, s);
+  }
+  void upload(const std::vector<T> &v, be::stream_t s) { upload(v.data(), v.size(), s); }
+  void zero(be::stream_t s) {
+    T *q = p;
+    be::pfor((int64_t)n, s, [=] __host__ __device__(int64_t i) { q[i] = T(0); });
+  }
+};
+
+// ---------------------------------------------------------------------------
+// natural-coordinate integrals and shape matrices (host, per step)
+// ---------------------------------------------------------------------------
+
+struct Univ {
+  double G[3 * 9 * 121], g[3 * 3 * 11], GH[9 * 121], gH[3 * 11];
+};
+
+// Integrals of d_r N_a d_s N_b over the 2x2 in-plane Gauss points at the three
+// zeta stations -1, 0, 1, and their Simpson sum 1/3, 4/3, 1/3 through the
+// thickness (exact for the trilinear + incompatible-mode products; the UNIV
+// table of src/core/09-assembly.js:26-60).
+const Univ &univ() {
+  static Univ U;
+  static bool done = false;
+  if (done) return U;
+  std::memset(&U, 0, sizeof U);
+  const int XI[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, ETA[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+            ZE[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+  const double wz[3] = {1.0 / 3, 4.0 / 3, 1.0 / 3};
+  const double gp = 1 / std::sqrt(3.0);
+  for (int st = 0; st < 3; st++) {
+    double zeta = st - 1;
+    for (int ip = 0; ip < 4; ip++) {
+      double xi = (ip & 1) ? gp : -gp, eta = (ip & 2) ? gp : -gp;
+      double d[3][11];
+      for (int a = 0; a < 8; a++) {
+        d[0][a] = XI[a] * (1 + ETA[a] * eta) * (1 + ZE[a] * zeta) * 0.125;
+        d[1][a] = ETA[a] * (1 + XI[a] * xi) * (1 + ZE[a] * zeta) * 0.125;
+        d[2][a] = ZE[a] * (1 + XI[a] * xi) * (1 + ETA[a] * eta) * 0.125;
+      }
+      for (int r = 0; r < 3; r++)
+        for (int a = 8; a < 11; a++) d[r][a] = 0;
+      d[0][8] = -2 * xi; d[1][9] = -2 * eta; d[2][10] = -2 * zeta;
+      for (int r = 0; r < 3; r++)
+        for (int a = 0; a < 11; a++) {
+          U.g[(st * 3 + r) * 11 + a] += d[r][a];
+          U.gH[r * 11 + a] += wz[st] * d[r][a];
+          for (int s = 0; s < 3; s++)
+            for (int b = 0; b < 11; b++) {
+              double v = d[r][a] * d[s][b];
+              U.G[(st * 9 + r * 3 + s) * 121 + a * 11 + b] += v;
+              U.GH[(r * 3 + s) * 121 + a * 11 + b] += wz[st] * v;
+            }
+        }
+    }
+  }
+  done = true;
+  return U;
+}
+
+// M^m (m = xx yy zz xy yz zx) with K33 = sum_m c_m M^m from a stiffness D (one
+// station, full-box integrals) or three zeta-station weights W (layered); the
+// construction of shapeMatrices, src/core/09-assembly.js:69-93.
+void shape_matrices(const double *W3, const double *D, double *M) {
+  const Univ &U = univ();
+  static const int VI[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+  static const int RS[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {2, 0}};
+  std::memset(M, 0, sizeof(double) * 6 * 1089);
+  int nst = D ? 1 : 3;
+  for (int st = 0; st < nst; st++) {
+    const double *Ds = D ? D : W3 + 36 * st;
+    for (int m = 0; m < 6; m++) {
+      double *Mm = M + m * 1089;
+      int r = RS[m][0], s = RS[m][1];
+      for (int pass = 0; pass < (r == s ? 1 : 2); pass++) {
+        int rr = pass ? s : r, ss = pass ? r : s;
+        const double *Gb = D ? U.GH + (rr * 3 + ss) * 121 : U.G + (st * 9 + rr * 3 + ss) * 121;
+        for (int p = 0; p < 3; p++)
+          for (int q = 0; q < 3; q++) {
+            double dv = Ds[VI[p][rr] * 6 + VI[q][ss]];
+            if (dv == 0) continue;
+            for (int a = 0; a < 11; a++)
+              for (int b = 0; b < 11; b++) Mm[(3 * a + p) * 33 + 3 * b + q] += dv * Gb[a * 11 + b];
+          }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device multigrid level (shell)
+// ---------------------------------------------------------------------------
+
+struct DevShell {
+  int nx = 0, ny = 0;
+  int32_t nagg = 0, ncol = 0, nb = 0;
+  bool coarsest = false;
+  DBuf<int32_t> colStart, rowPtr, col, blockRow;
+  DBuf<double> val64;
+  DBuf<float> val32;
+  DBuf<int64_t> cfPtr;
+  DBuf<double> cf64;
+  DBuf<float> cf32;
+  DBuf<int32_t> colors[4];
+  int32_t ncolor[4] = {0, 0, 0, 0};
+  // level 1: aggregate column and k range
+  DBuf<int32_t> aggCol, k0, k1;
+  // transfer to the next coarser level
+  DBuf<int32_t> pPtr, pIdx;
+  DBuf<double> pW64;
+  DBuf<float> pW32;
+  // children of this level's aggregates in the finer shell level
+  DBuf<int32_t> chPtr, chIdx, chQ;
+  // vectors
+  DBuf<double> b64, x64, r64;
+  DBuf<float> b32, x32, r32;
+  // coarsest: dense inverse (FP64)
+  DBuf<double> inv64, band;
+  int32_t ndense = 0;
+  int64_t cfTotal = 0;
+};
+
+}  // namespace
+
+struct fcbga_model {
+  fe::Grid g{};
+  int32_t nn = 0, ne = 0, ngroups = 0, nsec = 0;
+  std::vector<double> xs, ys, zs;
+  std::vector<int32_t> node_id, cell_elems, e_group, e_section, constraints, fx, fy;
+  std::vector<int8_t> e_part;
+  std::vector<double> e_scale;
+  // element types
+  int32_t ntypes = 0, nslot = 0;
+  std::vector<int32_t> typeSlot;  // material slot of a type
+  std::vector<double> typeH;      // hx hy hz per type
+  std::vector<int32_t> e_type;
+  be::stream_t stream = nullptr;
+  // device: mesh-constant
+  DBuf<int32_t> cellType, cellElem, d_egroup, d_typeSlot;
+  DBuf<double> d_escale, d_typeH;
+  DBuf<uint8_t> dofmask;
+  DBuf<int64_t> gridOf;
+  DBuf<int32_t> fineColors[4];
+  int32_t nFineColor[4] = {0, 0, 0, 0};
+  // device: per step
+  DBuf<double> cellScale64, Kc64, fc64, slotM, slotLoad, tabs;
+  DBuf<float> cellScale32, Kc32;
+  DBuf<uint8_t> slotHasF, slotLayered, typeHasF, d_born;
+  DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64;
+  DBuf<float> w32, sinv32, F32;
+  // solver vectors (grid layout, 3 per node)
+  DBuf<double> x, r, z, zold, p, q, tmp64, dx64;
+  DBuf<float> b32v, x32v, r32v, dx32;
+  DBuf<double> partials, scal;
+  // multigrid
+  bool hierBuilt = false;
+  mgs::Hierarchy H;
+  std::vector<std::unique_ptr<DevShell>> lv;
+  DBuf<int32_t> nodeAgg;
+  DBuf<double> zp64;
+  DBuf<float> zp32;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// model creation
+// ---------------------------------------------------------------------------
+
+std::vector<int32_t> size_classes(const std::vector<double> &c, std::vector<double> &rep) {
+  std::vector<int32_t> cls(c.size() - 1);
+  rep.clear();
+  for (size_t i = 0; i + 1 < c.size(); i++) {
+    double h = c[i + 1] - c[i];
+    int32_t f = -1;
+    for (size_t q = 0; q < rep.size(); q++)
+      if (std::fabs(h - rep[q]) <= 1e-9 * std::fabs(rep[q])) { f = (int32_t)q; break; }
+    if (f < 0) { f = (int32_t)rep.size(); rep.push_back(h); }
+    cls[i] = f;
+  }
+  return cls;
+}
+
+void check_mesh(const fcbga_mesh *m) {
+  if (!m) throw be::Error("mesh is NULL");
+  if (m->nx < 2 || m->ny < 2 || m->nz < 2) throw be::Error("mesh needs at least 2 grid lines per axis");
+  if (!m->xs || !m->ys || !m->zs || !m->node_id || !m->cell_elems || !m->e_group || !m->e_part || !m->e_scale)
+    throw be::Error("mesh arrays missing");
+  if (m->ngroups < 1) throw be::Error("mesh has no material groups");
+  if (m->nsections > 0 && !m->e_section) throw be::Error("layered sections given without e_section");
+}
+
+void model_create(const fcbga_mesh *mm, fcbga_model *M) {
+  check_mesh(mm);
+  fe::Grid &g = M->g;
+  g.nx = mm->nx; g.ny = mm->ny; g.nz = mm->nz;
+  g.ncx = g.nx - 1; g.ncy = g.ny - 1; g.ncz = g.nz - 1;
+  g.G = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t ncells = (int64_t)g.ncx * g.ncy * g.ncz;
+  M->nn = mm->nn; M->ne = mm->ne; M->ngroups = mm->ngroups; M->nsec = mm->nsections;
+  M->xs.assign(mm->xs, mm->xs + g.nx);
+  M->ys.assign(mm->ys, mm->ys + g.ny);
+  M->zs.assign(mm->zs, mm->zs + g.nz);
+  M->node_id.assign(mm->node_id, mm->node_id + g.G);
+  M->cell_elems.assign(mm->cell_elems, mm->cell_elems + 2 * ncells);
+  M->e_group.assign(mm->e_group, mm->e_group + M->ne);
+  M->e_part.assign(mm->e_part, mm->e_part + M->ne);
+  M->e_scale.assign(mm->e_scale, mm->e_scale + M->ne);
+  if (mm->e_section) M->e_section.assign(mm->e_section, mm->e_section + M->ne);
+  else M->e_section.assign(M->ne, -1);
+  M->constraints.assign(mm->constraint_dofs, mm->constraint_dofs + mm->nconstraints);
+  if (mm->nfeature_x) M->fx.assign(mm->feature_x, mm->feature_x + mm->nfeature_x);
+  if (mm->nfeature_y) M->fy.assign(mm->feature_y, mm->feature_y + mm->nfeature_y);
+  // validate
+  int32_t cnt = 0;
+  for (int64_t q = 0; q < g.G; q++)
+    if (M->node_id[q] >= 0) {
+      if (M->node_id[q] != cnt) throw be::Error("node_id must number existing grid nodes 0..nn-1 in grid order");
+      cnt++;
+    }
+  if (cnt != M->nn) throw be::Error("node_id count differs from nn");
+  for (int32_t d : M->constraints)
+    if (d < 0 || d >= 3 * M->nn) throw be::Error("constraint DOF out of range");
+  for (int32_t e = 0; e < M->ne; e++) {
+    if (M->e_group[e] < 0 || M->e_group[e] >= M->ngroups) throw be::Error("element group out of range");
+    if (M->e_section[e] >= M->nsec) throw be::Error("element section out of range");
+  }
+  // cell of every element and its type
+  std::vector<double> hxr, hyr, hzr;
+  std::vector<int32_t> cx = size_classes(M->xs, hxr), cy = size_classes(M->ys, hyr), cz = size_classes(M->zs, hzr);
+  M->nslot = M->ngroups + M->nsec;
+  std::map<std::tuple<int32_t, int32_t, int32_t, int32_t>, int32_t> tmap;
+  M->e_type.assign(M->ne, -1);
+  std::vector<int32_t> cellType(2 * ncells, -1), cellElem(2 * ncells, -1);
+  for (int i = 0; i < g.ncx; i++)
+    for (int j = 0; j < g.ncy; j++)
+      for (int k = 0; k < g.ncz; k++) {
+        int64_t c = ((int64_t)i * g.ncy + j) * g.ncz + k;
+        for (int q = 0; q < 2; q++) {
+          int32_t e = M->cell_elems[2 * c + q];
+          if (e < 0) continue;
+          if (e >= M->ne) throw be::Error("cell_elems entry out of range");
+          int32_t slot = M->e_section[e] >= 0 ? M->ngroups + M->e_section[e] : M->e_group[e];
+          auto key = std::make_tuple(slot, cx[i], cy[j], cz[k]);
+          auto it = tmap.find(key);
+          int32_t t;
+          if (it == tmap.end()) {
+            t = (int32_t)M->typeSlot.size();
+            tmap[key] = t;
+            M->typeSlot.push_back(slot);
+            M->typeH.push_back(hxr[cx[i]]);
+            M->typeH.push_back(hyr[cy[j]]);
+            M->typeH.push_back(hzr[cz[k]]);
+          } else t = it->second;
+          M->e_type[e] = t;
+          cellType[2 * c + q] = t;
+          cellElem[2 * c + q] = e;
+          // every corner of an element's cell must exist
+          for (int a = 0; a < 8; a++) {
+            int64_t gn = ((int64_t)(i + fe::corner_li(a)) * g.ny + j + fe::corner_lj(a)) * g.nz + k + fe::corner_lk(a);
+            if (M->node_id[gn] < 0) throw be::Error("element corner node missing in node_id");
+          }
+        }
+      }
+  M->ntypes = (int32_t)M->typeSlot.size();
+  // DOF mask and compact -> grid map
+  std::vector<uint8_t> mask(3 * g.G, 0);
+  std::vector<int64_t> gridOf(M->nn);
+  for (int64_t q = 0; q < g.G; q++)
+    if (M->node_id[q] >= 0) {
+      gridOf[M->node_id[q]] = q;
+      mask[3 * q] = mask[3 * q + 1] = mask[3 * q + 2] = 1;
+    }
+  for (int32_t d : M->constraints) mask[3 * gridOf[d / 3] + d % 3] = 0;
+#if FCBGA_GPU
+  BE_CHECK(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+#endif
+  be::stream_t s = M->stream;
+  M->cellType.upload(cellType, s);
+  M->cellElem.upload(cellElem, s);
+  M->d_egroup.upload(M->e_group, s);
+  M->d_escale.upload(M->e_scale, s);
+  M->d_typeSlot.upload(M->typeSlot, s);
+  M->d_typeH.upload(M->typeH, s);
+  M->dofmask.upload(mask, s);
+  M->gridOf.upload(gridOf, s);
+  // colour classes of the (i, j) columns: (i mod 2, j mod 2)
+  std::vector<int32_t> cols[4];
+  for (int i = 0; i < g.nx; i++)
+    for (int j = 0; j < g.ny; j++) cols[(i & 1) * 2 + (j & 1)].push_back(i * g.ny + j);
+  for (int c = 0; c < 4; c++) {
+    M->fineColors[c].upload(cols[c], s);
+    M->nFineColor[c] = (int32_t)cols[c].size();
+  }
+  // natural-coordinate tables for the thermal load: g (3*3*11) then gH (3*11)
+  const Univ &U = univ();
+  std::vector<double> tabs(U.g, U.g + 99);
+  tabs.insert(tabs.end(), U.gH, U.gH + 33);
+  M->tabs.upload(tabs, s);
+  be::sync(s);
+}
+
+// ---------------------------------------------------------------------------
+// per-step assembly on the device
+// ---------------------------------------------------------------------------
+
+void check_step(const fcbga_model *M, const fcbga_step *st) {
+  if (!st || !st->D || !st->s0 || !st->born) throw be::Error("step data missing (D, s0, born)");
+  if (M->nsec > 0 && (!st->secW || !st->secV || !st->sec_born))
+    throw be::Error("step data missing for layered sections (secW, secV, sec_born)");
+}
+
+void assemble_step(fcbga_model *M, const fcbga_step *st) {
+  be::stream_t s = M->stream;
+  const fe::Grid g = M->g;
+  const int64_t ncells = (int64_t)g.ncx * g.ncy * g.ncz;
+  // material slots: shape matrices and loads
+  std::vector<double> Mall((size_t)M->nslot * 6 * 1089), load((size_t)M->nslot * 18, 0.0);
+  std::vector<uint8_t> hasF(M->nslot, 0), layered(M->nslot, 0);
+  for (int32_t q = 0; q < M->ngroups; q++) {
+    shape_matrices(nullptr, st->D + 36 * q, Mall.data() + (size_t)6 * 1089 * q);
+    hasF[q] = st->born[q] ? 1 : 0;
+    if (hasF[q])
+      for (int r = 0; r < 6; r++) load[18 * q + r] = st->s0[6 * q + r];
+  }
+  for (int32_t q = 0; q < M->nsec; q++) {
+    int32_t slot = M->ngroups + q;
+    shape_matrices(st->secW + 108 * q, nullptr, Mall.data() + (size_t)6 * 1089 * slot);
+    hasF[slot] = st->sec_born[q] ? 1 : 0;
+    layered[slot] = 1;
+    if (hasF[slot])
+      for (int r = 0; r < 18; r++) load[18 * slot + r] = st->secV[18 * q + r];
+  }
+  M->slotM.upload(Mall, s);
+  M->slotLoad.upload(load, s);
+  M->slotHasF.upload(hasF, s);
+  M->slotLayered.upload(layered, s);
+  std::vector<uint8_t> born(st->born, st->born + M->ngroups);
+  M->d_born.upload(born, s);
+  // condensed element tables, one per type
+  M->Kc64.alloc((size_t)M->ntypes * 576);
+  M->Kc32.alloc((size_t)M->ntypes * 576);
+  M->fc64.alloc((size_t)M->ntypes * 24);
+  M->typeHasF.alloc(M->ntypes);
+  {
+    const int32_t *tslot = M->d_typeSlot.p;
+    const double *tH = M->d_typeH.p, *sM = M->slotM.p, *sL = M->slotLoad.p, *tb = M->tabs.p;
+    const uint8_t *sF = M->slotHasF.p, *sLay = M->slotLayered.p;
+    double *Kc = M->Kc64.p, *fc = M->fc64.p;
+    float *Kc32 = M->Kc32.p;
+    uint8_t *tF = M->typeHasF.p;
+    be::pfor(M->ntypes, s, [=] __host__ __device__(int64_t t) {
+      int32_t sl = tslot[t];
+      fe::element_condense(tH[3 * t], tH[3 * t + 1], tH[3 * t + 2], sM + (int64_t)sl * 6 * 1089, sF[sl], sLay[sl],
+                           sL + 18 * (int64_t)sl, tb, tb + 99, Kc + 576 * t, fc + 24 * t);
+      for (int q = 0; q < 576; q++) Kc32[576 * t + q] = (float)Kc[576 * t + q];
+      tF[t] = sF[sl];
+    }, 64);
+  }
+  // element scales of the step: eScale x (born ? 1 : 1e-6)
+  M->cellScale64.alloc(2 * ncells);
+  M->cellScale32.alloc(2 * ncells);
+  {
+    const int32_t *ce = M->cellElem.p, *eg = M->d_egroup.p;
+    const double *es = M->d_escale.p;
+    const uint8_t *bn = M->d_born.p;
+    double *cs = M->cellScale64.p;
+    float *cs32 = M->cellScale32.p;
+    be::pfor(2 * ncells, s, [=] __host__ __device__(int64_t q) {
+      int32_t e = ce[q];
+      double v = e >= 0 ? es[e] * (bn[eg[e]] ? 1.0 : 1e-6) : 0.0;
+      cs[q] = v;
+      cs32[q] = (float)v;
+    });
+  }
+  // load vector and diagonal / column blocks; Jacobi scaling
+  const int64_t G = g.G;
+  M->bload.alloc(3 * G);
+  M->s64.alloc(3 * G); M->w64.alloc(3 * G); M->sinv64.alloc(3 * G);
+  M->w32.alloc(3 * G); M->sinv32.alloc(3 * G);
+  M->Db.alloc(9 * G); M->Lb.alloc(9 * G);
+  {
+    const int32_t *ct = M->cellType.p;
+    const double *cs = M->cellScale64.p, *Kc = M->Kc64.p, *fc = M->fc64.p;
+    const uint8_t *tF = M->typeHasF.p, *mask = M->dofmask.p;
+    double *bl = M->bload.p, *sv = M->s64.p, *wv = M->w64.p, *si = M->sinv64.p, *Db = M->Db.p, *Lb = M->Lb.p;
+    float *w32 = M->w32.p, *si32 = M->sinv32.p;
+    be::pfor(G, s, [=] __host__ __device__(int64_t gi) {
+      int k = (int)(gi % g.nz);
+      int64_t cj = gi / g.nz;
+      int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+      double f[3], D[9], L[9];
+      fe::node_load(g, i, j, k, ct, cs, fc, tF, f);
+      fe::node_blocks(g, i, j, k, ct, cs, Kc, D, L);
+      bool exists = D[0] != 0 || D[4] != 0 || D[8] != 0;
+      double sc[3];
+      for (int p = 0; p < 3; p++) {
+        double d = D[4 * p];
+        sc[p] = exists ? 1 / sqrt(d > 1e-300 ? d : 1e-300) : 0.0;
+        uint8_t m = mask[3 * gi + p];
+        bl[3 * gi + p] = m ? f[p] : 0.0;
+        sv[3 * gi + p] = sc[p];
+        wv[3 * gi + p] = m ? sc[p] : 0.0;
+        w32[3 * gi + p] = (float)(m ? sc[p] : 0.0);
+        si[3 * gi + p] = exists ? 1 / sc[p] : 0.0;
+        si32[3 * gi + p] = (float)(exists ? 1 / sc[p] : 0.0);
+      }
+      // scaled, constraint-reduced blocks of the column smoother
+      const int64_t gd = gi - 1;  // node below (same column when k > 0)
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) {
+          uint8_t mp = mask[3 * gi + p], mq = mask[3 * gi + q];
+          double v = (mp && mq) ? sc[p] * D[3 * p + q] * sc[q] : 0.0;
+          if (p == q && !mp) v = 1.0;
+          Db[9 * gi + 3 * p + q] = v;
+          double lv = 0.0;
+          if (k > 0) {
+            uint8_t mb = mask[3 * gd + q];
+            // the scale of the node below is applied by the next kernel
+            lv = (mp && mb) ? sc[p] * L[3 * p + q] : 0.0;
+          }
+          Lb[9 * gi + 3 * p + q] = lv;
+        }
+    });
+    // finish the column blocks with the scale of the node below
+    be::pfor(G, s, [=] __host__ __device__(int64_t gi) {
+      int k = (int)(gi % g.nz);
+      if (k == 0) return;
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) Lb[9 * gi + 3 * p + q] *= sv[3 * (gi - 1) + q];
+    });
+  }
+}
+
+// column factors of the fine level (FP64, FP32 copy)
+void factor_fine_columns(fcbga_model *M) {
+  const fe::Grid g = M->g;
+  const int64_t ncol = (int64_t)g.nx * g.ny;
+  M->F64.alloc(18 * g.G);
+  M->F32.alloc(18 * g.G);
+  const double *Db = M->Db.p, *Lb = M->Lb.p;
+  double *F = M->F64.p;
+  float *F32 = M->F32.p;
+  const int nz = g.nz;
+  be::pfor(ncol, M->stream, [=] __host__ __device__(int64_t c) {
+    fe::column_factor(nz, Db + 9 * c * nz, Lb + 9 * c * nz, F + 18 * c * nz);
+    for (int q = 0; q < 18 * nz; q++) F32[18 * c * nz + q] = (float)F[18 * c * nz + q];
+  }, 64);
+}
+
+// ---------------------------------------------------------------------------
+// fine-level operator pieces
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct FineOps {
+  fe::Grid g;
+  const int32_t *cellType;
+  const T *cellScale, *Kc, *w, *sinv;
+  const T *F;
+  const int32_t *nodeAgg;
+  const T *zp;
+};
+
+template <typename T>
+FineOps<T> fine_ops(fcbga_model *M);
+
+template <>
+FineOps<double> fine_ops<double>(fcbga_model *M) {
+  return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->F64.p,
+                         M->nodeAgg.p, M->zp64.p};
+}
+template <>
+FineOps<float> fine_ops<float>(fcbga_model *M) {
+  return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->F32.p,
+                        M->nodeAgg.p, M->zp32.p};
+}
+
+// y = A x on every grid node (A the scaled, constraint-reduced operator)
+template <typename T>
+void fine_apply(const FineOps<T> &o, const T *x, T *y, be::stream_t s) {
+  const fe::Grid g = o.g;
+  const int32_t *ct = o.cellType;
+  const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w;
+  be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
+    int k = (int)(gi % g.nz);
+    int64_t cj = gi / g.nz;
+    int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    T out[3];
+    fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
+    y[3 * gi] = out[0]; y[3 * gi + 1] = out[1]; y[3 * gi + 2] = out[2];
+  });
+}
+
+// res = b - A x on every grid node
+template <typename T>
+void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stream_t s) {
+  const fe::Grid g = o.g;
+  const int32_t *ct = o.cellType;
+  const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w;
+  be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
+    int k = (int)(gi % g.nz);
+    int64_t cj = gi / g.nz;
+    int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    T out[3];
+    fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
+    for (int p = 0; p < 3; p++) res[3 * gi + p] = b[3 * gi + p] - out[p];
+  });
+}
+
+// one colour of a column block Gauss-Seidel sweep: residual of the colour's
+// columns, then their block-tridiagonal solves (columnSweep, src/core/10-solver.js:512-555)
+template <typename T>
+void fine_color_sweep(const FineOps<T> &o, const int32_t *cols, int32_t ncols, const T *b, T *x, T *r, T *dx,
+                      be::stream_t s) {
+  const fe::Grid g = o.g;
+  const int32_t *ct = o.cellType;
+  const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w, *F = o.F;
+  const int nz = g.nz;
+  be::pfor((int64_t)ncols * nz, s, [=] __host__ __device__(int64_t t) {
+    int64_t c = cols[t / nz];
+    int k = (int)(t % nz);
+    int i = (int)(c / g.ny), j = (int)(c % g.ny);
+    int64_t gi = c * nz + k;
+    T out[3];
+    fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
+    for (int p = 0; p < 3; p++) r[3 * gi + p] = b[3 * gi + p] - out[p];
+  });
+  be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
+    int64_t c = cols[t];
+    fe::column_solve<T, T>(nz, F + 18 * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
+    for (int q = 0; q < 3 * nz; q++) x[3 * c * nz + q] += dx[3 * c * nz + q];
+  }, 64);
+}
+
+// ---------------------------------------------------------------------------
+// multigrid numeric set-up
+// ---------------------------------------------------------------------------
+
+void build_hierarchy(fcbga_model *M) {
+  if (M->hierBuilt) return;
+  be::stream_t s = M->stream;
+  mgs::MeshView mv;
+  mv.nx = M->g.nx; mv.ny = M->g.ny; mv.nz = M->g.nz;
+  mv.xs = M->xs.data(); mv.ys = M->ys.data(); mv.zs = M->zs.data();
+  mv.node_id = M->node_id.data();
+  mv.cell_elems = M->cell_elems.data();
+  mv.e_part = M->e_part.data();
+  mv.feature_x = M->fx; mv.feature_y = M->fy;
+  mgs::build_hierarchy(mv, M->H);
+  M->nodeAgg.upload(M->H.nodeAgg, s);
+  M->zp64.upload(M->H.zp, s);
+  {
+    std::vector<float> z32(M->H.zp.begin(), M->H.zp.end());
+    M->zp32.upload(z32, s);
+  }
+  M->lv.clear();
+  const int nl = (int)M->H.L.size();
+  for (int l = 0; l < nl; l++) {
+    const mgs::Level &L = M->H.L[l];
+    auto D = std::make_unique<DevShell>();
+    D->nx = L.nx; D->ny = L.ny; D->nagg = L.nagg; D->ncol = L.nx * L.ny;
+    D->nb = L.rowPtr[L.nagg];
+    D->coarsest = (l == nl - 1);
+    D->colStart.upload(L.colStart, s);
+    D->rowPtr.upload(L.rowPtr, s);
+    D->col.upload(L.col, s);
+    std::vector<int32_t> brow(D->nb);
+    for (int32_t a = 0; a < L.nagg; a++)
+      for (int32_t k = L.rowPtr[a]; k < L.rowPtr[a + 1]; k++) brow[k] = a;
+    D->blockRow.upload(brow, s);
+    D->val64.alloc((size_t)D->nb * 36);
+    D->val32.alloc((size_t)D->nb * 36);
+    std::vector<int64_t> cfp(D->ncol + 1, 0);
+    for (int32_t c = 0; c < D->ncol; c++) {
+      int64_t m = 6 * (int64_t)(L.colStart[c + 1] - L.colStart[c]);
+      cfp[c + 1] = cfp[c] + m * m;
+    }
+    D->cfTotal = cfp[D->ncol];
+    D->cfPtr.upload(cfp, s);
+    D->cf64.alloc(D->cfTotal);
+    D->cf32.alloc(D->cfTotal);
+    std::vector<int32_t> cols[4];
+    for (int i = 0; i < L.nx; i++)
+      for (int j = 0; j < L.ny; j++) cols[(i & 1) * 2 + (j & 1)].push_back(i * L.ny + j);
+    for (int c = 0; c < 4; c++) { D->colors[c].upload(cols[c], s); D->ncolor[c] = (int32_t)cols[c].size(); }
+    if (l == 0) {
+      std::vector<int32_t> aggCol(L.nagg);
+      for (int32_t c = 0; c < D->ncol; c++)
+        for (int32_t a = L.colStart[c]; a < L.colStart[c + 1]; a++) aggCol[a] = c;
+      D->aggCol.upload(aggCol, s);
+      D->k0.upload(L.k0, s);
+      D->k1.upload(L.k1, s);
+    }
+    if (!D->coarsest) {
+      if (l + 1 < nl) {
+        D->pPtr.upload(L.pPtr, s);
+        D->pIdx.upload(L.pIdx, s);
+        D->pW64.upload(L.pW, s);
+        std::vector<float> w32(L.pW.begin(), L.pW.end());
+        D->pW32.upload(w32, s);
+      }
+    }
+    if (l > 0) {
+      D->chPtr.upload(L.chPtr, s);
+      D->chIdx.upload(L.chIdx, s);
+      D->chQ.upload(L.chQ, s);
+    }
+    size_t n = 6 * (size_t)L.nagg;
+    D->b64.alloc(n); D->x64.alloc(n); D->r64.alloc(n);
+    D->b32.alloc(n); D->x32.alloc(n); D->r32.alloc(n);
+    M->lv.push_back(std::move(D));
+  }
+  be::sync(s);
+  M->hierBuilt = true;
+}
+
+// Galerkin product of level 1 from the fine operator (galerkinProduct,
+// src/core/10-solver.js:193-245, with the fine rows gathered from the
+// elements): block (A, B) = sum over nodes i of A and j of B of W_i^T Ks_ij W_j.
+void galerkin_level1(fcbga_model *M) {
+  DevShell &D = *M->lv[0];
+  const fe::Grid g = M->g;
+  const int32_t *brow = D.blockRow.p, *bcol = D.col.p, *aggCol = D.aggCol.p, *k0 = D.k0.p, *k1 = D.k1.p;
+  const int32_t *nodeAgg = M->nodeAgg.p, *ct = M->cellType.p;
+  const double *cs = M->cellScale64.p, *Kc = M->Kc64.p, *sv = M->s64.p, *zp = M->zp64.p;
+  const uint8_t *mask = M->dofmask.p;
+  double *val = D.val64.p;
+  float *val32 = D.val32.p;
+  const int nz = g.nz, ny = g.ny;
+  be::pfor(D.nb, M->stream, [=] __host__ __device__(int64_t kb) {
+    int32_t A = brow[kb], B = bcol[kb];
+    int32_t cA = aggCol[A], cB = aggCol[B];
+    int iA = cA / ny, jA = cA % ny, iB = cB / ny, jB = cB % ny;
+    double acc[36];
+    for (int q = 0; q < 36; q++) acc[q] = 0;
+    for (int ka = k0[A]; ka <= k1[A]; ka++) {
+      int64_t gi = (int64_t)cA * nz + ka;
+      if (nodeAgg[gi] != A) continue;
+      int kb0 = k0[B] > ka - 1 ? k0[B] : ka - 1, kb1 = k1[B] < ka + 1 ? k1[B] : ka + 1;
+      for (int kk = kb0; kk <= kb1; kk++) {
+        int64_t gj = (int64_t)cB * nz + kk;
+        if (nodeAgg[gj] != B) continue;
+        double Kb[9];
+        fe::pair_block(g, iA, jA, ka, iB, jB, kk, ct, cs, Kc, Kb);
+        double Ks[9];
+        for (int p = 0; p < 3; p++)
+          for (int q = 0; q < 3; q++) {
+            uint8_t mp = mask[3 * gi + p], mq = mask[3 * gj + q];
+            double v = (mp && mq) ? sv[3 * gi + p] * Kb[3 * p + q] * sv[3 * gj + q] : 0.0;
+            if (gi == gj && p == q && !mp) v = 1.0;
+            Ks[3 * p + q] = v;
+          }
+        // W (3x6) of a node: diag(1/s) [I | u = t + theta x z'] rows
+        double Wi[18], Wj[18];
+        for (int q = 0; q < 18; q++) { Wi[q] = 0; Wj[q] = 0; }
+        double zi = zp[gi], zj = zp[gj];
+        double si0 = 1 / sv[3 * gi], si1 = 1 / sv[3 * gi + 1], si2 = 1 / sv[3 * gi + 2];
+        double sj0 = 1 / sv[3 * gj], sj1 = 1 / sv[3 * gj + 1], sj2 = 1 / sv[3 * gj + 2];
+        Wi[0] = si0; Wi[4] = zi * si0; Wi[7] = si1; Wi[9] = -zi * si1; Wi[14] = si2; Wi[17] = zi * si2;
+        Wj[0] = sj0; Wj[4] = zj * sj0; Wj[7] = sj1; Wj[9] = -zj * sj1; Wj[14] = sj2; Wj[17] = zj * sj2;
+        double Tm[18];
+        for (int p = 0; p < 3; p++)
+          for (int c = 0; c < 6; c++) {
+            double t = 0;
+            for (int q = 0; q < 3; q++) t += Ks[3 * p + q] * Wj[6 * q + c];
+            Tm[6 * p + c] = t;
+          }
+        for (int a = 0; a < 6; a++)
+          for (int c = 0; c < 6; c++) {
+            double t = 0;
+            for (int p = 0; p < 3; p++) t += Wi[6 * p + a] * Tm[6 * p + c];
+            acc[6 * a + c] += t;
+          }
+      }
+    }
+    for (int q = 0; q < 36; q++) { val[36 * kb + q] = acc[q]; val32[36 * kb + q] = (float)acc[q]; }
+  });
+}
+
+// Galerkin product between shell levels l (fine) and l+1 (coarse)
+void galerkin_shell(fcbga_model *M, int l) {
+  DevShell &F = *M->lv[l];
+  DevShell &C = *M->lv[l + 1];
+  const int32_t *brow = C.blockRow.p, *bcol = C.col.p, *chPtr = C.chPtr.p, *chIdx = C.chIdx.p, *chQ = C.chQ.p;
+  const int32_t *frp = F.rowPtr.p, *fcol = F.col.p, *pPtr = F.pPtr.p, *pIdx = F.pIdx.p;
+  const double *fval = F.val64.p, *W = F.pW64.p;
+  double *val = C.val64.p;
+  float *val32 = C.val32.p;
+  be::pfor(C.nb, M->stream, [=] __host__ __device__(int64_t kb) {
+    int32_t I = brow[kb], J = bcol[kb];
+    double acc[36];
+    for (int q = 0; q < 36; q++) acc[q] = 0;
+    for (int32_t h = chPtr[I]; h < chPtr[I + 1]; h++) {
+      int32_t f = chIdx[h];
+      const double *Wf = W + 36 * (int64_t)chQ[h];
+      for (int32_t k = frp[f]; k < frp[f + 1]; k++) {
+        int32_t gg = fcol[k];
+        for (int32_t pp = pPtr[gg]; pp < pPtr[gg + 1]; pp++) {
+          if (pIdx[pp] != J) continue;
+          const double *Wg = W + 36 * (int64_t)pp, *Kf = fval + 36 * (int64_t)k;
+          double Tm[36];
+          for (int r = 0; r < 6; r++)
+            for (int c = 0; c < 6; c++) {
+              double t = 0;
+              for (int q = 0; q < 6; q++) t += Kf[6 * r + q] * Wg[6 * q + c];
+              Tm[6 * r + c] = t;
+            }
+          for (int a = 0; a < 6; a++)
+            for (int c = 0; c < 6; c++) {
+              double t = 0;
+              for (int r = 0; r < 6; r++) t += Wf[6 * r + a] * Tm[6 * r + c];
+              acc[6 * a + c] += t;
+            }
+        }
+      }
+    }
+    for (int q = 0; q < 36; q++) { val[36 * kb + q] = acc[q]; val32[36 * kb + q] = (float)acc[q]; }
+  });
+}
+
+// dense Cholesky of every column of a shell level (factorShellColumns, src/core/10-solver.js:474-494)
+void factor_shell_columns(fcbga_model *M, int l) {
+  DevShell &D = *M->lv[l];
+  const int32_t *cst = D.colStart.p, *rp = D.rowPtr.p, *col = D.col.p;
+  const int64_t *cfp = D.cfPtr.p;
+  const double *val = D.val64.p;
+  double *cf = D.cf64.p;
+  float *cf32 = D.cf32.p;
+  be::pfor(D.ncol, M->stream, [=] __host__ __device__(int64_t c) {
+    int32_t n0 = cst[c], n1 = cst[c + 1];
+    int m = 6 * (n1 - n0);
+    if (!m) return;
+    double *A = cf + cfp[c];
+    for (int q = 0; q < m * m; q++) A[q] = 0;
+    for (int32_t n = n0; n < n1; n++)
+      for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
+        int32_t j = col[k];
+        if (j < n0 || j >= n1) continue;
+        for (int p = 0; p < 6; p++)
+          for (int q = 0; q < 6; q++) A[((n - n0) * 6 + p) * m + (j - n0) * 6 + q] = val[36 * (int64_t)k + 6 * p + q];
+      }
+    fe::chol_dense(A, m, 1e-12);
+    for (int q = 0; q < m * m; q++) cf32[cfp[c] + q] = (float)A[q];
+  }, 64);
+}
+
+// Coarsest level: banded Cholesky on the host with the jitter rule of
+// factorCoarsest (src/core/10-solver.js:562-597), then the explicit inverse on
+// the device (one banded solve per unit vector) so that each cycle applies it
+// as a dense FP64 product.
+void factor_coarsest(fcbga_model *M) {
+  DevShell &D = *M->lv.back();
+  const mgs::Level &L = M->H.L.back();
+  const int32_t na = L.nagg, n = 6 * na;
+  std::vector<double> val((size_t)D.nb * 36);
+  be::d2h(val.data(), D.val64.p, sizeof(double) * val.size(), M->stream);
+  int bwb = 0;
+  for (int32_t i = 0; i < na; i++)
+    for (int32_t k = L.rowPtr[i]; k < L.rowPtr[i + 1]; k++) bwb = std::max(bwb, std::abs(L.col[k] - i));
+  const int bw = (bwb + 1) * 6 - 1, w = bw + 1;
+  std::vector<double> A((size_t)n * w, 0.0);
+  for (int32_t i = 0; i < na; i++)
+    for (int32_t k = L.rowPtr[i]; k < L.rowPtr[i + 1]; k++) {
+      int32_t j = L.col[k];
+      if (j > i) continue;
+      for (int p = 0; p < 6; p++)
+        for (int q = 0; q < 6; q++) {
+          int32_t r = 6 * i + p, c = 6 * j + q;
+          if (c <= r) A[(size_t)r * w + (c - r + bw)] = val[36 * (size_t)k + 6 * p + q];
+        }
+    }
+  double dmax = 0;
+  for (int32_t i = 0; i < n; i++) dmax = std::max(dmax, std::fabs(A[(size_t)i * w + bw]));
+  const double jitter = 1e-12 * (dmax != 0 ? dmax : 1);
+  for (int32_t j = 0; j < n; j++) {
+    double d = A[(size_t)j * w + bw];
+    int32_t k0 = std::max(0, j - bw);
+    for (int32_t k = k0; k < j; k++) { double l = A[(size_t)j * w + (k - j + bw)]; d -= l * l; }
+    if (!(d > jitter)) d = std::max(std::fabs(d), jitter);
+    double ljj = std::sqrt(d);
+    A[(size_t)j * w + bw] = ljj;
+    double inv = 1 / ljj;
+    int32_t iend = std::min(n, j + bw + 1);
+    for (int32_t i = j + 1; i < iend; i++) {
+      double sum = A[(size_t)i * w + (j - i + bw)];
+      int32_t kk0 = std::max(k0, i - bw);
+      for (int32_t k = kk0; k < j; k++) sum -= A[(size_t)i * w + (k - i + bw)] * A[(size_t)j * w + (k - j + bw)];
+      A[(size_t)i * w + (j - i + bw)] = sum * inv;
+    }
+  }
+  D.band.upload(A, M->stream);
+  D.inv64.alloc((size_t)n * n);
+  D.ndense = n;
+  const double *Ab = D.band.p;
+  double *inv = D.inv64.p;
+  be::pfor(n, M->stream, [=] __host__ __device__(int64_t c) {
+    // column c of the inverse, stored as row c (the inverse is symmetric)
+    double *x = inv + (int64_t)c * n;
+    for (int32_t i = 0; i < n; i++) x[i] = (i == c) ? 1.0 : 0.0;
+    for (int32_t i = (int32_t)c; i < n; i++) {
+      double sum = x[i];
+      int32_t k0 = i - bw > 0 ? i - bw : 0;
+      for (int32_t k = k0; k < i; k++) sum -= Ab[(int64_t)i * w + (k - i + bw)] * x[k];
+      x[i] = sum / Ab[(int64_t)i * w + bw];
+    }
+    for (int32_t i = n - 1; i >= 0; i--) {
+      double sum = x[i];
+      int32_t kend = i + bw + 1 < n ? i + bw + 1 : n;
+      for (int32_t k = i + 1; k < kend; k++) sum -= Ab[(int64_t)k * w + (i - k + bw)] * x[k];
+      x[i] = sum / Ab[(int64_t)i * w + bw];
+    }
+  }, 64);
+}
+
+double mg_setup(fcbga_model *M) {
+  double t0 = now_ms();
+  build_hierarchy(M);
+  galerkin_level1(M);
+  const int nl = (int)M->lv.size();
+  for (int l = 0; l < nl; l++) {
+    if (!M->lv[l]->coarsest) factor_shell_columns(M, l);
+    if (l + 1 < nl) galerkin_shell(M, l);
+  }
+  factor_coarsest(M);
+  be::sync(M->stream);
+  return now_ms() - t0;
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle (MultigridPC._cycleOnce, src/core/10-solver.js:687-712): two
+// forward colour sweeps, residual, restriction, coarse cycle, prolongation,
+// two backward colour sweeps.  Colours replace the sequential column order.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct LevelVec;
+template <>
+struct LevelVec<double> {
+  static double *b(DevShell &D) { return D.b64.p; }
+  static double *x(DevShell &D) { return D.x64.p; }
+  static double *r(DevShell &D) { return D.r64.p; }
+  static const double *val(DevShell &D) { return D.val64.p; }
+  static const double *cf(DevShell &D) { return D.cf64.p; }
+  static const double *pW(DevShell &D) { return D.pW64.p; }
+};
+template <>
+struct LevelVec<float> {
+  static float *b(DevShell &D) { return D.b32.p; }
+  static float *x(DevShell &D) { return D.x32.p; }
+  static float *r(DevShell &D) { return D.r32.p; }
+  static const float *val(DevShell &D) { return D.val32.p; }
+  static const float *cf(DevShell &D) { return D.cf32.p; }
+  static const float *pW(DevShell &D) { return D.pW32.p; }
+};
+
+template <typename T>
+void shell_sweep(DevShell &D, int color, be::stream_t s) {
+  const int32_t *cols = D.colors[color].p, *cst = D.colStart.p, *rp = D.rowPtr.p, *col = D.col.p;
+  const int64_t *cfp = D.cfPtr.p;
+  const T *val = LevelVec<T>::val(D), *cf = LevelVec<T>::cf(D), *b = LevelVec<T>::b(D);
+  T *x = LevelVec<T>::x(D), *r = LevelVec<T>::r(D);
+  be::pfor(D.ncolor[color], s, [=] __host__ __device__(int64_t t) {
+    int32_t c = cols[t];
+    int32_t n0 = cst[c], n1 = cst[c + 1];
+    if (n1 == n0) return;
+    for (int32_t n = n0; n < n1; n++) {
+      T acc[6];
+      for (int p = 0; p < 6; p++) acc[p] = b[6 * (int64_t)n + p];
+      for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
+        const T *v = val + 36 * (int64_t)k, *xj = x + 6 * (int64_t)col[k];
+        for (int p = 0; p < 6; p++) {
+          T sum = 0;
+          for (int q = 0; q < 6; q++) sum += v[6 * p + q] * xj[q];
+          acc[p] -= sum;
+        }
+      }
+      for (int p = 0; p < 6; p++) r[6 * (int64_t)n + p] = acc[p];
+    }
+    int m = 6 * (n1 - n0);
+    fe::chol_solve<T, T>(cf + cfp[c], m, r + 6 * (int64_t)n0);
+    for (int q = 0; q < m; q++) x[6 * (int64_t)n0 + q] += r[6 * (int64_t)n0 + q];
+  }, 64);
+}
+
+template <typename T>
+void shell_residual(DevShell &D, be::stream_t s) {
+  const int32_t *rp = D.rowPtr.p, *col = D.col.p;
+  const T *val = LevelVec<T>::val(D), *b = LevelVec<T>::b(D), *x = LevelVec<T>::x(D);
+  T *r = LevelVec<T>::r(D);
+  be::pfor(D.nagg, s, [=] __host__ __device__(int64_t n) {
+    T acc[6];
+    for (int p = 0; p < 6; p++) acc[p] = b[6 * n + p];
+    for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
+      const T *v = val + 36 * (int64_t)k, *xj = x + 6 * (int64_t)col[k];
+      for (int p = 0; p < 6; p++) {
+        T sum = 0;
+        for (int q = 0; q < 6; q++) sum += v[6 * p + q] * xj[q];
+        acc[p] -= sum;
+      }
+    }
+    for (int p = 0; p < 6; p++) r[6 * n + p] = acc[p];
+  });
+}
+
+template <typename T>
+void zero_vec(T *v, int64_t n, be::stream_t s) {
+  be::pfor(n, s, [=] __host__ __device__(int64_t i) { v[i] = T(0); });
+}
+
+template <typename T>
+void shell_cycle(fcbga_model *M, int l) {
+  DevShell &D = *M->lv[l];
+  be::stream_t s = M->stream;
+  T *x = LevelVec<T>::x(D);
+  const T *b = LevelVec<T>::b(D);
+  if (D.coarsest) {
+    const double *inv = D.inv64.p;
+    const int32_t n = D.ndense;
+    be::pfor(n, s, [=] __host__ __device__(int64_t i) {
+      double acc = 0;
+      const double *row = inv + i * (int64_t)n;
+      for (int32_t j = 0; j < n; j++) acc += row[j] * (double)b[j];
+      x[i] = (T)acc;
+    });
+    return;
+  }
+  zero_vec<T>(x, 6 * (int64_t)D.nagg, s);
+  for (int sw = 0; sw < 2; sw++)
+    for (int c = 0; c < 4; c++) shell_sweep<T>(D, c, s);
+  shell_residual<T>(D, s);
+  DevShell &C = *M->lv[l + 1];
+  {
+    const int32_t *chPtr = C.chPtr.p, *chIdx = C.chIdx.p, *chQ = C.chQ.p;
+    const T *W = LevelVec<T>::pW(D), *res = LevelVec<T>::r(D);
+    T *bc = LevelVec<T>::b(C);
+    be::pfor(C.nagg, s, [=] __host__ __device__(int64_t I) {
+      T acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int32_t h = chPtr[I]; h < chPtr[I + 1]; h++) {
+        const T *w = W + 36 * (int64_t)chQ[h], *rf = res + 6 * (int64_t)chIdx[h];
+        for (int a = 0; a < 6; a++) {
+          T sum = 0;
+          for (int p = 0; p < 6; p++) sum += w[6 * p + a] * rf[p];
+          acc[a] += sum;
+        }
+      }
+      for (int a = 0; a < 6; a++) bc[6 * I + a] = acc[a];
+    });
+  }
+  shell_cycle<T>(M, l + 1);
+  {
+    const int32_t *pPtr = D.pPtr.p, *pIdx = D.pIdx.p;
+    const T *W = LevelVec<T>::pW(D), *ec = LevelVec<T>::x(C);
+    be::pfor(D.nagg, s, [=] __host__ __device__(int64_t f) {
+      for (int32_t q = pPtr[f]; q < pPtr[f + 1]; q++) {
+        const T *w = W + 36 * (int64_t)q, *e = ec + 6 * (int64_t)pIdx[q];
+        for (int p = 0; p < 6; p++) {
+          T sum = 0;
+          for (int a = 0; a < 6; a++) sum += w[6 * p + a] * e[a];
+          x[6 * f + p] += sum;
+        }
+      }
+    });
+  }
+  for (int sw = 0; sw < 2; sw++)
+    for (int c = 3; c >= 0; c--) shell_sweep<T>(D, c, s);
+}
+
+// z = M^-1 r on the fine grid (T precision), r and z are T vectors of 3G
+template <typename T>
+void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) {
+  be::stream_t s = M->stream;
+  FineOps<T> o = fine_ops<T>(M);
+  const int64_t n3 = 3 * M->g.G;
+  zero_vec<T>(x, n3, s);
+  for (int sw = 0; sw < 2; sw++)
+    for (int c = 0; c < 4; c++) fine_color_sweep<T>(o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
+  if (withCoarse) {
+    fine_residual<T>(o, b, x, r, s);
+    DevShell &C = *M->lv[0];
+    const int32_t *aggCol = C.aggCol.p, *k0 = C.k0.p, *k1 = C.k1.p, *nodeAgg = o.nodeAgg;
+    const T *sinv = o.sinv, *zp = o.zp;
+    T *bc = LevelVec<T>::b(C);
+    const int nz = M->g.nz;
+    const T *res = r;
+    be::pfor(C.nagg, s, [=] __host__ __device__(int64_t A) {
+      T acc[6] = {0, 0, 0, 0, 0, 0};
+      int64_t c = aggCol[A];
+      for (int k = k0[A]; k <= k1[A]; k++) {
+        int64_t gi = c * nz + k;
+        if (nodeAgg[gi] != (int32_t)A) continue;
+        T r0 = res[3 * gi] * sinv[3 * gi], r1 = res[3 * gi + 1] * sinv[3 * gi + 1], r2 = res[3 * gi + 2] * sinv[3 * gi + 2];
+        T z = zp[gi];
+        acc[0] += r0; acc[1] += r1; acc[2] += r2;
+        acc[3] -= z * r1; acc[4] += z * r0; acc[5] += z * r2;
+      }
+      for (int a = 0; a < 6; a++) bc[6 * A + a] = acc[a];
+    });
+    shell_cycle<T>(M, 0);
+    const T *ec = LevelVec<T>::x(C);
+    be::pfor(M->g.G, s, [=] __host__ __device__(int64_t gi) {
+      int32_t A = nodeAgg[gi];
+      if (A < 0) return;
+      const T *e = ec + 6 * (int64_t)A;
+      T z = zp[gi];
+      x[3 * gi] += (e[0] + z * e[4]) * sinv[3 * gi];
+      x[3 * gi + 1] += (e[1] - z * e[3]) * sinv[3 * gi + 1];
+      x[3 * gi + 2] += (e[2] + z * e[5]) * sinv[3 * gi + 2];
+    });
+  }
+  for (int sw = 0; sw < 2; sw++)
+    for (int c = 3; c >= 0; c--) fine_color_sweep<T>(o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
+}
+
+// ---------------------------------------------------------------------------
+// PCG (LinearSolver._pcg, src/core/10-solver.js:1001-1042), cold start
+// ---------------------------------------------------------------------------
+
+double dotd(fcbga_model *M, const double *a, const double *b, int slot) {
+  const int64_t n3 = 3 * M->g.G;
+  be::dot_async(n3, a, b, M->partials.p + 1024 * slot, M->scal.p + slot, M->stream);
+  double v;
+  be::d2h(&v, M->scal.p + slot, sizeof(double), M->stream);
+  return v;
+}
+
+void precondition(fcbga_model *M, int precision, int precond) {
+  // z = M^-1 r, then z = 0 on constrained / absent DOFs
+  const int64_t n3 = 3 * M->g.G;
+  be::stream_t s = M->stream;
+  const uint8_t *mask = M->dofmask.p;
+  const double *r = M->r.p;
+  double *z = M->z.p;
+  if (precond == FCBGA_PC_JACOBI) {
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { z[i] = mask[i] ? r[i] : 0.0; });
+    return;
+  }
+  const bool coarse = precond == FCBGA_PC_MG;
+  if (precision == FCBGA_PREC_FP64) {
+    fine_cycle<double>(M, r, z, M->tmp64.p, M->dx64.p, coarse);
+  } else {
+    float *b32 = M->b32v.p, *x32 = M->x32v.p;
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { b32[i] = (float)r[i]; });
+    fine_cycle<float>(M, b32, x32, M->r32v.p, M->dx32.p, coarse);
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { z[i] = (double)x32[i]; });
+  }
+  be::pfor(n3, s, [=] __host__ __device__(int64_t i) { if (!mask[i]) z[i] = 0.0; });
+}
+
+void alloc_vectors(fcbga_model *M, int precision) {
+  const size_t n3 = 3 * (size_t)M->g.G;
+  M->x.alloc(n3); M->r.alloc(n3); M->z.alloc(n3); M->p.alloc(n3); M->q.alloc(n3);
+  M->tmp64.alloc(n3); M->dx64.alloc(n3);
+  if (precision == FCBGA_PREC_MIXED) {
+    M->zold.alloc(n3);
+    M->b32v.alloc(n3); M->x32v.alloc(n3); M->r32v.alloc(n3); M->dx32.alloc(n3);
+  }
+  M->partials.alloc(1024 * 4);
+  M->scal.alloc(4);
+}
+
+void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op, double *du_dev, double *du_host,
+                fcbga_solve_stats *stats) {
+  check_step(M, st);
+  fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_FP64, FCBGA_PC_MG, 0};
+  if (op) o = *op;
+  if (!(o.tol > 0) || o.maxit < 1) throw be::Error("tol must be > 0 and maxit >= 1");
+  if (o.precision != FCBGA_PREC_FP64 && o.precision != FCBGA_PREC_MIXED) throw be::Error("unknown precision");
+  if (o.precond < FCBGA_PC_MG || o.precond > FCBGA_PC_JACOBI) throw be::Error("unknown preconditioner");
+  fcbga_solve_stats S{};
+  S.ndof = 3 * (int64_t)M->nn;
+  S.ntypes = M->ntypes;
+  be::stream_t s = M->stream;
+  const double t0 = now_ms();
+  assemble_step(M, st);
+  if (o.precond != FCBGA_PC_JACOBI) factor_fine_columns(M);
+  alloc_vectors(M, o.precision);
+  be::sync(s);
+  const double t1 = now_ms();
+  S.ms_assemble = t1 - t0;
+  if (o.precond == FCBGA_PC_MG) {
+    S.ms_setup = mg_setup(M);
+    S.levels = 1 + (int32_t)M->lv.size();
+    S.coarsest_dof = M->lv.back()->ndense;
+  } else {
+    S.levels = 1;
+  }
+  const double t2 = now_ms();
+  const int64_t n3 = 3 * M->g.G;
+  FineOps<double> f64 = fine_ops<double>(M);
+  double *x = M->x.p, *r = M->r.p, *z = M->z.p, *p = M->p.p, *q = M->q.p, *zold = M->zold.p;
+  const double *bl = M->bload.p, *sv = M->s64.p;
+  // scaled load; x = 0, r = b
+  be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
+    double bs = bl[i] * sv[i];
+    r[i] = bs;
+    x[i] = 0.0;
+  });
+  const double bnorm = std::sqrt(dotd(M, r, r, 0));
+  int it = 0;
+  bool conv = false;
+  double rnorm = bnorm;
+  if (bnorm == 0) {
+    conv = true;
+    S.r0 = 0;
+  } else {
+    S.r0 = 1.0;
+    if (rnorm / bnorm < o.tol) conv = true;
+    else {
+      precondition(M, o.precision, o.precond);
+      be::d2d(p, z, sizeof(double) * n3, s);
+      double rz = dotd(M, r, z, 1);
+      const bool flex = o.precision == FCBGA_PREC_MIXED;
+      for (it = 1; it <= o.maxit; it++) {
+        fine_apply<double>(f64, p, q, s);
+        double pq = dotd(M, p, q, 2);
+        if (!(pq > 0)) break;
+        const double alpha = rz / pq;
+        be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
+          x[i] += alpha * p[i];
+          r[i] -= alpha * q[i];
+        });
+        rnorm = std::sqrt(dotd(M, r, r, 0));
+        if (rnorm / bnorm < o.tol) { conv = true; break; }
+        if (flex) be::d2d(zold, z, sizeof(double) * n3, s);
+        precondition(M, o.precision, o.precond);
+        double rz2 = dotd(M, r, z, 1);
+        double beta;
+        if (flex) {
+          // flexible (Polak-Ribiere) beta for a preconditioner that varies slightly between applications
+          double rzo = dotd(M, r, zold, 3);
+          beta = (rz2 - rzo) / rz;
+        } else {
+          beta = rz2 / rz;
+        }
+        rz = rz2;
+        be::pfor(n3, s, [=] __host__ __device__(int64_t i) { p[i] = z[i] + beta * p[i]; });
+      }
+      if (it > o.maxit) it = o.maxit;
+    }
+  }
+  // du = S x in the reference's node order
+  {
+    const int64_t *gof = M->gridOf.p;
+    double *out = du_dev;
+    be::pfor(M->nn, s, [=] __host__ __device__(int64_t nidx) {
+      int64_t gi = gof[nidx];
+      for (int c = 0; c < 3; c++) out[3 * nidx + c] = x[3 * gi + c] * sv[3 * gi + c];
+    });
+  }
+  if (du_host) be::d2h(du_host, du_dev, sizeof(double) * 3 * (size_t)M->nn, s);
+  be::sync(s);
+  const double t3 = now_ms();
+  S.ms_pcg = t3 - t2;
+  S.ms_total = t3 - t0;
+  S.iters = it;
+  S.converged = conv ? 1 : 0;
+  S.relres = bnorm > 0 ? rnorm / bnorm : 0;
+  if (stats) *stats = S;
+}
+
+int fail(const std::exception &e, int code) {
+  g_err = e.what();
+  return code;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int fcbga_model_create(const fcbga_mesh *mesh, fcbga_model **out) {
+  if (!out) { g_err = "out is NULL"; return FCBGA_ERR_ARG; }
+  *out = nullptr;
+  fcbga_model *M = new fcbga_model();
+  try {
+    model_create(mesh, M);
+  } catch (const be::Error &e) {
+    delete M;
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    delete M;
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  *out = M;
+  return FCBGA_OK;
+}
+
+void fcbga_model_destroy(fcbga_model *m) {
+  if (!m) return;
+#if FCBGA_GPU
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  hipStream_t s = m->stream;
+  delete m;
+  if (s) (void)hipStreamDestroy(s);
+#else
+  delete m;
+#endif
+}
+
+int64_t fcbga_model_ndof(const fcbga_model *m) { return m ? 3 * (int64_t)m->nn : 0; }
+
+static int solve_common(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts, double *du_dev,
+                        double *du_host, fcbga_solve_stats *stats) {
+  if (!m) { g_err = "model is NULL"; return FCBGA_ERR_ARG; }
+  try {
+    DBuf<double> tmp;
+    double *dst = du_dev;
+    if (!dst) {
+      tmp.alloc(3 * (size_t)m->nn);
+      dst = tmp.p;
+    }
+    fcbga_solve_stats S{};
+    step_solve(m, step, opts, dst, du_host, &S);
+    if (stats) *stats = S;
+    if (!S.converged) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "Linear solver did not converge: relative residual %.2e after %d iterations",
+               S.relres, S.iters);
+      g_err = buf;
+      return FCBGA_ERR_NOT_CONVERGED;
+    }
+  } catch (const be::Error &e) {
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_step_solve(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts, double *du_out,
+                     fcbga_solve_stats *stats) {
+  if (!du_out) { g_err = "du_out is NULL"; return FCBGA_ERR_ARG; }
+  return solve_common(m, step, opts, nullptr, du_out, stats);
+}
+
+int fcbga_step_solve_device(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts, double *du_dev,
+                            fcbga_solve_stats *stats) {
+  if (!du_dev) { g_err = "du_dev is NULL"; return FCBGA_ERR_ARG; }
+  return solve_common(m, step, opts, du_dev, nullptr, stats);
+}
+
+int fcbga_step_load(fcbga_model *m, const fcbga_step *step, double *f_out) {
+  if (!m || !f_out) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    check_step(m, step);
+    assemble_step(m, step);
+    DBuf<double> out;
+    out.alloc(3 * (size_t)m->nn);
+    const int64_t *gof = m->gridOf.p;
+    const double *bl = m->bload.p;
+    double *o = out.p;
+    be::pfor(m->nn, m->stream, [=] __host__ __device__(int64_t n) {
+      for (int c = 0; c < 3; c++) o[3 * n + c] = bl[3 * gof[n] + c];
+    });
+    be::d2h(f_out, out.p, sizeof(double) * 3 * (size_t)m->nn, m->stream);
+  } catch (const be::Error &e) {
+    return fail(e, FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_step_apply(fcbga_model *m, const fcbga_step *step, const double *x, double *y_out) {
+  if (!m || !x || !y_out) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    check_step(m, step);
+    assemble_step(m, step);
+    const fe::Grid g = m->g;
+    DBuf<double> xin, xg, yg, out;
+    xin.upload(x, 3 * (size_t)m->nn, m->stream);
+    xg.alloc(3 * g.G);
+    yg.alloc(3 * g.G);
+    out.alloc(3 * (size_t)m->nn);
+    xg.zero(m->stream);
+    const int64_t *gof = m->gridOf.p;
+    const double *xi = xin.p;
+    double *xgp = xg.p, *ygp = yg.p, *o = out.p;
+    be::pfor(m->nn, m->stream, [=] __host__ __device__(int64_t n) {
+      for (int c = 0; c < 3; c++) xgp[3 * gof[n] + c] = xi[3 * n + c];
+    });
+    const int32_t *ct = m->cellType.p;
+    const double *cs = m->cellScale64.p, *Kc = m->Kc64.p;
+    be::pfor(g.G, m->stream, [=] __host__ __device__(int64_t gi) {
+      int k = (int)(gi % g.nz);
+      int64_t cj = gi / g.nz;
+      int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+      double v[3];
+      fe::node_apply_raw<double>(g, i, j, k, ct, cs, Kc, xgp, v);
+      for (int c = 0; c < 3; c++) ygp[3 * gi + c] = v[c];
+    });
+    be::pfor(m->nn, m->stream, [=] __host__ __device__(int64_t n) {
+      for (int c = 0; c < 3; c++) o[3 * n + c] = ygp[3 * gof[n] + c];
+    });
+    be::d2h(y_out, out.p, sizeof(double) * 3 * (size_t)m->nn, m->stream);
+  } catch (const be::Error &e) {
+    return fail(e, FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_last_error(char *buf, size_t len) {
+  if (!buf || !len) return (int)g_err.size();
+  size_t n = g_err.size() < len - 1 ? g_err.size() : len - 1;
+  std::memcpy(buf, g_err.data(), n);
+  buf[n] = 0;
+  return (int)n;
+}
+
+}  // extern "C"

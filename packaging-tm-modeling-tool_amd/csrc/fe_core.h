@@ -1,0 +1,427 @@
+// fe_core.h -- per-node and per-element arithmetic of the hex8 solver, written
+// once as __host__ __device__ functions: the HIP kernels wrap them, and a host
+// build runs the same code for CPU-side index checks.
+//
+// Layout conventions (device):
+//   grid nodes g = (i * ny + j) * nz + k (z fastest, the reference's node order
+//   with holes for absent nodes); vectors hold 3 DOFs per grid node.
+//   cells c = (i * ncy + j) * ncz + k, two element slots per cell
+//   (src/core/06-mesher.js:27-31); a slot holds a type id (-1 none) and a scale
+//   (element scale x 1 or x 1e-6 for an unborn group, src/core/09-assembly.js:403).
+//   Corner order of a hex: (0,0,0) (1,0,0) (1,1,0) (0,1,0) then the same at +z
+//   (src/core/06-mesher.js:46).
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if !defined(__HIPCC__)
+#ifndef __host__
+#define __host__
+#endif
+#ifndef __device__
+#define __device__
+#endif
+#ifndef __forceinline__
+#define __forceinline__ inline
+#endif
+#endif
+
+#define FE_HD __host__ __device__ __forceinline__
+
+namespace fe {
+
+struct Grid {
+  int nx, ny, nz;     // node lines
+  int ncx, ncy, ncz;  // cells
+  int64_t G;          // grid nodes
+};
+
+// corner index of the local offset (li, lj, lk)
+FE_HD int corner_of(int li, int lj, int lk) { return lk * 4 + (lj ? (li ? 2 : 3) : (li ? 1 : 0)); }
+FE_HD int corner_li(int a) { return (a & 3) == 1 || (a & 3) == 2; }
+FE_HD int corner_lj(int a) { return (a & 3) >= 2; }
+FE_HD int corner_lk(int a) { return a >= 4; }
+
+// ---------------------------------------------------------------------------
+// operator row of one grid node:
+//   y_i = w_i * sum_j K_ij (w_j x_j) + (w_i == 0 ? x_i : 0)
+// with w = s * mask (s the Jacobi scaling diag(K)^-1/2, mask 0 on constrained
+// DOFs and absent nodes).  This is the Jacobi-scaled, constraint-reduced
+// operator of src/core/09-assembly.js:445-464 and src/core/10-solver.js:852-866
+// without the assembled matrix: K is gathered from the condensed element
+// matrices of the (up to 8 x 2) elements around the node.
+// ---------------------------------------------------------------------------
+template <typename T, typename TK>
+FE_HD void node_apply(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
+                      const TK *__restrict__ cellScale, const TK *__restrict__ Kc, const T *__restrict__ w,
+                      const T *__restrict__ x, T out[3]) {
+  // 27-point neighbourhood of w*x
+  T nb[3][3][3][3];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int di = 0; di < 3; di++)
+    for (int dj = 0; dj < 3; dj++)
+      for (int dk = 0; dk < 3; dk++) {
+        int ii = i + di - 1, jj = j + dj - 1, kk = k + dk - 1;
+        bool in = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
+        int64_t gn = in ? ((int64_t)ii * g.ny + jj) * g.nz + kk : 0;
+        for (int p = 0; p < 3; p++) nb[di][dj][dk][p] = in ? w[3 * gn + p] * x[3 * gn + p] : T(0);
+      }
+  T a0 = 0, a1 = 0, a2 = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int ci = 0; ci < 2; ci++)
+    for (int cj = 0; cj < 2; cj++)
+      for (int ck = 0; ck < 2; ck++) {
+        int ii = i - 1 + ci, jj = j - 1 + cj, kk = k - 1 + ck;
+        if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy || kk < 0 || kk >= g.ncz) continue;
+        int64_t c = ((int64_t)ii * g.ncy + jj) * g.ncz + kk;
+        const int a = corner_of(1 - ci, 1 - cj, 1 - ck);
+        for (int q = 0; q < 2; q++) {
+          int32_t t = cellType[2 * c + q];
+          if (t < 0) continue;
+          T sc = (T)cellScale[2 * c + q];
+          const TK *K = Kc + (int64_t)t * 576 + (3 * a) * 24;
+          T s0 = 0, s1 = 0, s2 = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+          for (int b = 0; b < 8; b++) {
+            const T *xv = nb[ci + corner_li(b)][cj + corner_lj(b)][ck + corner_lk(b)];
+            s0 += (T)K[3 * b] * xv[0] + (T)K[3 * b + 1] * xv[1] + (T)K[3 * b + 2] * xv[2];
+            s1 += (T)K[24 + 3 * b] * xv[0] + (T)K[24 + 3 * b + 1] * xv[1] + (T)K[24 + 3 * b + 2] * xv[2];
+            s2 += (T)K[48 + 3 * b] * xv[0] + (T)K[48 + 3 * b + 1] * xv[1] + (T)K[48 + 3 * b + 2] * xv[2];
+          }
+          a0 += sc * s0; a1 += sc * s1; a2 += sc * s2;
+        }
+      }
+  int64_t gi = ((int64_t)i * g.ny + j) * g.nz + k;
+  T acc[3] = {a0, a1, a2};
+  for (int p = 0; p < 3; p++) {
+    T wi = w[3 * gi + p];
+    out[p] = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
+  }
+}
+
+// unscaled K x at a node (no constraints): y_i = sum_j K_ij x_j (test hook and residual check)
+template <typename T>
+FE_HD void node_apply_raw(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
+                          const T *__restrict__ cellScale, const T *__restrict__ Kc, const T *__restrict__ x,
+                          T out[3]) {
+  T a0 = 0, a1 = 0, a2 = 0;
+  for (int ci = 0; ci < 2; ci++)
+    for (int cj = 0; cj < 2; cj++)
+      for (int ck = 0; ck < 2; ck++) {
+        int ii = i - 1 + ci, jj = j - 1 + cj, kk = k - 1 + ck;
+        if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy || kk < 0 || kk >= g.ncz) continue;
+        int64_t c = ((int64_t)ii * g.ncy + jj) * g.ncz + kk;
+        const int a = corner_of(1 - ci, 1 - cj, 1 - ck);
+        for (int q = 0; q < 2; q++) {
+          int32_t t = cellType[2 * c + q];
+          if (t < 0) continue;
+          T sc = cellScale[2 * c + q];
+          const T *K = Kc + (int64_t)t * 576 + (3 * a) * 24;
+          for (int b = 0; b < 8; b++) {
+            int64_t gb = ((int64_t)(ii + corner_li(b)) * g.ny + jj + corner_lj(b)) * g.nz + kk + corner_lk(b);
+            const T *xv = x + 3 * gb;
+            a0 += sc * (K[3 * b] * xv[0] + K[3 * b + 1] * xv[1] + K[3 * b + 2] * xv[2]);
+            a1 += sc * (K[24 + 3 * b] * xv[0] + K[24 + 3 * b + 1] * xv[1] + K[24 + 3 * b + 2] * xv[2]);
+            a2 += sc * (K[48 + 3 * b] * xv[0] + K[48 + 3 * b + 1] * xv[1] + K[48 + 3 * b + 2] * xv[2]);
+          }
+        }
+      }
+  out[0] = a0; out[1] = a1; out[2] = a2;
+}
+
+// thermal load of a node: sum over its elements of scale * fc (src/core/09-assembly.js:416-420)
+FE_HD void node_load(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
+                     const double *__restrict__ cellScale, const double *__restrict__ fc,
+                     const uint8_t *__restrict__ hasF, double out[3]) {
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int ci = 0; ci < 2; ci++)
+    for (int cj = 0; cj < 2; cj++)
+      for (int ck = 0; ck < 2; ck++) {
+        int ii = i - 1 + ci, jj = j - 1 + cj, kk = k - 1 + ck;
+        if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy || kk < 0 || kk >= g.ncz) continue;
+        int64_t c = ((int64_t)ii * g.ncy + jj) * g.ncz + kk;
+        const int a = corner_of(1 - ci, 1 - cj, 1 - ck);
+        for (int q = 0; q < 2; q++) {
+          int32_t t = cellType[2 * c + q];
+          if (t < 0 || !hasF[t]) continue;
+          double sc = cellScale[2 * c + q];
+          a0 += sc * fc[24 * (int64_t)t + 3 * a];
+          a1 += sc * fc[24 * (int64_t)t + 3 * a + 1];
+          a2 += sc * fc[24 * (int64_t)t + 3 * a + 2];
+        }
+      }
+  out[0] = a0; out[1] = a1; out[2] = a2;
+}
+
+// Diagonal block K_nn and the column block K_{n,n-1} (node below) of the
+// unscaled matrix, gathered from the elements (both 3x3 row-major).
+FE_HD void node_blocks(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
+                       const double *__restrict__ cellScale, const double *__restrict__ Kc, double D[9],
+                       double L[9]) {
+  for (int q = 0; q < 9; q++) { D[q] = 0; L[q] = 0; }
+  for (int ci = 0; ci < 2; ci++)
+    for (int cj = 0; cj < 2; cj++)
+      for (int ck = 0; ck < 2; ck++) {
+        int ii = i - 1 + ci, jj = j - 1 + cj, kk = k - 1 + ck;
+        if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy || kk < 0 || kk >= g.ncz) continue;
+        int64_t c = ((int64_t)ii * g.ncy + jj) * g.ncz + kk;
+        const int a = corner_of(1 - ci, 1 - cj, 1 - ck);
+        const int ab = corner_of(1 - ci, 1 - cj, 0);  // same column, one plane down (valid when ck == 0)
+        for (int q = 0; q < 2; q++) {
+          int32_t t = cellType[2 * c + q];
+          if (t < 0) continue;
+          double sc = cellScale[2 * c + q];
+          const double *K = Kc + (int64_t)t * 576;
+          for (int p = 0; p < 3; p++)
+            for (int r = 0; r < 3; r++) D[3 * p + r] += sc * K[(3 * a + p) * 24 + 3 * a + r];
+          if (ck == 0)
+            for (int p = 0; p < 3; p++)
+              for (int r = 0; r < 3; r++) L[3 * p + r] += sc * K[(3 * a + p) * 24 + 3 * ab + r];
+        }
+      }
+}
+
+// chol3 with the jitter rule of src/core/10-solver.js:12-25, 60-64
+FE_HD bool chol3(const double *A, double *L, double jit) {
+  double a00 = A[0] + jit, a10 = A[3], a11 = A[4] + jit, a20 = A[6], a21 = A[7], a22 = A[8] + jit;
+  if (!(a00 > 0)) return false;
+  double l00 = sqrt(a00), l10 = a10 / l00, l20 = a20 / l00;
+  double d11 = a11 - l10 * l10;
+  if (!(d11 > 0)) return false;
+  double l11 = sqrt(d11), l21 = (a21 - l20 * l10) / l11;
+  double d22 = a22 - l20 * l20 - l21 * l21;
+  if (!(d22 > 0)) return false;
+  L[0] = l00; L[1] = 0; L[2] = 0; L[3] = l10; L[4] = l11; L[5] = 0; L[6] = l20; L[7] = l21; L[8] = sqrt(d22);
+  return true;
+}
+
+// Block-tridiagonal Cholesky of one z-column of the scaled, constraint-reduced
+// matrix (factorColumns, src/core/10-solver.js:31-68).  Db/Lb hold the scaled
+// blocks per grid node of the column (k = 0..nz-1); F receives 18 values per
+// node: L_kk (lower 3x3) and L_{k,k-1}.
+FE_HD void column_factor(int nz, const double *Db, const double *Lb, double *F) {
+  double tmp[9], D[9];
+  for (int k = 0; k < nz; k++) {
+    for (int q = 0; q < 9; q++) D[q] = Db[9 * k + q];
+    double *fo = F + 18 * k;
+    bool haveDown = false;
+    if (k > 0) {
+      for (int q = 0; q < 9; q++) haveDown |= Lb[9 * k + q] != 0.0;
+    }
+    if (haveDown) {
+      const double *av = Lb + 9 * k, *lp = F + 18 * (k - 1);
+      for (int r = 0; r < 3; r++) {
+        double b0 = av[3 * r], b1 = av[3 * r + 1], b2 = av[3 * r + 2];
+        double y0 = b0 / lp[0];
+        double y1 = (b1 - lp[3] * y0) / lp[4];
+        double y2 = (b2 - lp[6] * y0 - lp[7] * y1) / lp[8];
+        tmp[3 * r] = y0; tmp[3 * r + 1] = y1; tmp[3 * r + 2] = y2;
+      }
+      for (int q = 0; q < 9; q++) fo[9 + q] = tmp[q];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+          D[3 * r + c] -= tmp[3 * r] * tmp[3 * c] + tmp[3 * r + 1] * tmp[3 * c + 1] + tmp[3 * r + 2] * tmp[3 * c + 2];
+    } else {
+      for (int q = 0; q < 9; q++) fo[9 + q] = 0;
+    }
+    double jit = 0;
+    bool ok = chol3(D, fo, jit);
+    for (int t = 0; t < 8 && !ok; t++) {
+      double mx = fabs(D[0]);
+      if (fabs(D[4]) > mx) mx = fabs(D[4]);
+      if (fabs(D[8]) > mx) mx = fabs(D[8]);
+      if (mx < 1e-300) mx = 1e-300;
+      jit = jit != 0 ? jit * 10 : 1e-10 * mx;
+      ok = chol3(D, fo, jit);
+    }
+    if (!ok) {  // an all-zero block: identity factor keeps the solve finite
+      for (int q = 0; q < 9; q++) fo[q] = (q % 4 == 0) ? 1.0 : 0.0;
+    }
+  }
+}
+
+// Forward/back substitution of one column (solveColumn, src/core/10-solver.js:71-104):
+// x = (L L^T)^-1 r over nodes 0..nz-1 of the column (stride between nodes 3).
+template <typename T, typename TF>
+FE_HD void column_solve(int nz, const TF *__restrict__ F, const T *__restrict__ r, T *__restrict__ x) {
+  for (int n = 0; n < nz; n++) {
+    const TF *fo = F + 18 * n;
+    T b0 = r[3 * n], b1 = r[3 * n + 1], b2 = r[3 * n + 2];
+    if (n > 0) {
+      T p0 = x[3 * n - 3], p1 = x[3 * n - 2], p2 = x[3 * n - 1];
+      b0 -= (T)fo[9] * p0 + (T)fo[10] * p1 + (T)fo[11] * p2;
+      b1 -= (T)fo[12] * p0 + (T)fo[13] * p1 + (T)fo[14] * p2;
+      b2 -= (T)fo[15] * p0 + (T)fo[16] * p1 + (T)fo[17] * p2;
+    }
+    T y0 = b0 / (T)fo[0];
+    T y1 = (b1 - (T)fo[3] * y0) / (T)fo[4];
+    T y2 = (b2 - (T)fo[6] * y0 - (T)fo[7] * y1) / (T)fo[8];
+    x[3 * n] = y0; x[3 * n + 1] = y1; x[3 * n + 2] = y2;
+  }
+  for (int n = nz - 1; n >= 0; n--) {
+    const TF *fo = F + 18 * n;
+    T b0 = x[3 * n], b1 = x[3 * n + 1], b2 = x[3 * n + 2];
+    if (n + 1 < nz) {
+      const TF *fn = F + 18 * (n + 1);
+      T q0 = x[3 * n + 3], q1 = x[3 * n + 4], q2 = x[3 * n + 5];
+      b0 -= (T)fn[9] * q0 + (T)fn[12] * q1 + (T)fn[15] * q2;
+      b1 -= (T)fn[10] * q0 + (T)fn[13] * q1 + (T)fn[16] * q2;
+      b2 -= (T)fn[11] * q0 + (T)fn[14] * q1 + (T)fn[17] * q2;
+    }
+    T z2 = b2 / (T)fo[8];
+    T z1 = (b1 - (T)fo[7] * z2) / (T)fo[4];
+    T z0 = (b0 - (T)fo[3] * z1 - (T)fo[6] * z2) / (T)fo[0];
+    x[3 * n] = z0; x[3 * n + 1] = z1; x[3 * n + 2] = z2;
+  }
+}
+
+// 3x3 block K_ij between grid nodes (i,j,k) and (ii,jj,kk) (|offset| <= 1 per
+// axis), gathered from the elements of the cells that contain both.
+FE_HD void pair_block(const Grid &g, int i, int j, int k, int ii, int jj, int kk, const int32_t *__restrict__ cellType,
+                      const double *__restrict__ cellScale, const double *__restrict__ Kc, double B[9]) {
+  for (int q = 0; q < 9; q++) B[q] = 0;
+  int cx0 = (i > ii ? i : ii) - 1, cx1 = i < ii ? i : ii;
+  int cy0 = (j > jj ? j : jj) - 1, cy1 = j < jj ? j : jj;
+  int cz0 = (k > kk ? k : kk) - 1, cz1 = k < kk ? k : kk;
+  for (int ci = cx0; ci <= cx1; ci++) {
+    if (ci < 0 || ci >= g.ncx) continue;
+    for (int cj = cy0; cj <= cy1; cj++) {
+      if (cj < 0 || cj >= g.ncy) continue;
+      for (int ck = cz0; ck <= cz1; ck++) {
+        if (ck < 0 || ck >= g.ncz) continue;
+        int64_t c = ((int64_t)ci * g.ncy + cj) * g.ncz + ck;
+        int a = corner_of(i - ci, j - cj, k - ck), b = corner_of(ii - ci, jj - cj, kk - ck);
+        for (int q = 0; q < 2; q++) {
+          int32_t t = cellType[2 * c + q];
+          if (t < 0) continue;
+          double sc = cellScale[2 * c + q];
+          const double *K = Kc + (int64_t)t * 576;
+          for (int p = 0; p < 3; p++)
+            for (int r = 0; r < 3; r++) B[3 * p + r] += sc * K[(3 * a + p) * 24 + 3 * b + r];
+        }
+      }
+    }
+  }
+}
+
+// Dense Cholesky with the relative jitter rule of cholDense (src/core/01-util.js:70-91)
+FE_HD int chol_dense(double *A, int n, double rel) {
+  double dmax = 0;
+  for (int i = 0; i < n; i++) dmax = fmax(dmax, fabs(A[i * n + i]));
+  double jitter = rel * (dmax != 0 ? dmax : 1);
+  int fixes = 0;
+  for (int j = 0; j < n; j++) {
+    double *rj = A + j * n;
+    double d = rj[j];
+    for (int k = 0; k < j; k++) d -= rj[k] * rj[k];
+    if (!(d > jitter)) { d = fmax(fabs(d), jitter); fixes++; }
+    double ljj = sqrt(d);
+    rj[j] = ljj;
+    double inv = 1 / ljj;
+    for (int i = j + 1; i < n; i++) {
+      double *ri = A + i * n;
+      double s = ri[j];
+      for (int k = 0; k < j; k++) s -= ri[k] * rj[k];
+      ri[j] = s * inv;
+    }
+  }
+  return fixes;
+}
+
+// L L^T x = b in place (cholSolveDense, src/core/01-util.js:94-107)
+template <typename T, typename TL>
+FE_HD void chol_solve(const TL *L, int n, T *b) {
+  for (int i = 0; i < n; i++) {
+    T s = b[i];
+    for (int k = 0; k < i; k++) s -= (T)L[i * n + k] * b[k];
+    b[i] = s / (T)L[i * n + i];
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    T s = b[i];
+    for (int k = i + 1; k < n; k++) s -= (T)L[k * n + i] * b[k];
+    b[i] = s / (T)L[i * n + i];
+  }
+}
+
+// Condensed element of a box (hx, hy, hz): K33 = sum_m c_m M^m
+// (elementK33Fast, src/core/09-assembly.js:96-108), thermal load from the
+// natural-coordinate integrals (elementFth, src/core/09-assembly.js:115-133),
+// static condensation of the 9 incompatible DOFs with a 1e-14 jittered
+// Cholesky and exact symmetrisation (condense33, src/core/07-element.js:88-120).
+// layered != 0: V holds 3 zeta-station stress weights (18) and gz the per-station
+// integrals g[3][3][11]; else s0 (6) with the full-box integrals gH[3][11].
+FE_HD void element_condense(double hx, double hy, double hz, const double *__restrict__ M, int hasF, int layered,
+                            const double *__restrict__ load, const double *__restrict__ gz,
+                            const double *__restrict__ gH, double *__restrict__ Kc, double *__restrict__ fc) {
+  double K[1089], f[33], Kaa[81], X[216];
+  const double V8 = hx * hy * hz / 8;
+  const double c[6] = {V8 * 4 / (hx * hx), V8 * 4 / (hy * hy), V8 * 4 / (hz * hz),
+                       V8 * 4 / (hx * hy), V8 * 4 / (hy * hz), V8 * 4 / (hz * hx)};
+  for (int i = 0; i < 33; i++)
+    for (int j = i; j < 33; j++) {
+      int o = i * 33 + j;
+      double v = c[0] * M[o] + c[1] * M[1089 + o] + c[2] * M[2178 + o] + c[3] * M[3267 + o] + c[4] * M[4356 + o] +
+                 c[5] * M[5445 + o];
+      K[o] = v;
+      K[j * 33 + i] = v;
+    }
+  for (int q = 0; q < 33; q++) f[q] = 0;
+  if (hasF) {
+    const int VI[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+    const double h[3] = {hx, hy, hz};
+    int nst = layered ? 3 : 1;
+    for (int st = 0; st < nst; st++) {
+      const double *S = load + 6 * st;
+      for (int r = 0; r < 3; r++) {
+        double cr = V8 * (2 / h[r]);
+        const double *gb = layered ? gz + (st * 3 + r) * 11 : gH + r * 11;
+        for (int p = 0; p < 3; p++) {
+          double sv = cr * S[VI[p][r]];
+          if (sv == 0) continue;
+          for (int a = 0; a < 11; a++) f[3 * a + p] += sv * gb[a];
+        }
+      }
+    }
+  }
+  for (int i = 0; i < 9; i++)
+    for (int j = 0; j < 9; j++) Kaa[i * 9 + j] = K[(24 + i) * 33 + 24 + j];
+  chol_dense(Kaa, 9, 1e-14);
+  double col[9];
+  for (int j = 0; j < 24; j++) {
+    for (int i = 0; i < 9; i++) col[i] = K[(24 + i) * 33 + j];
+    chol_solve<double, double>(Kaa, 9, col);
+    for (int i = 0; i < 9; i++) X[i * 24 + j] = col[i];
+  }
+  for (int i = 0; i < 24; i++)
+    for (int j = 0; j < 24; j++) {
+      double s = K[i * 33 + j];
+      for (int q = 0; q < 9; q++) s -= K[i * 33 + 24 + q] * X[q * 24 + j];
+      Kc[i * 24 + j] = s;
+    }
+  for (int i = 0; i < 24; i++)
+    for (int j = i + 1; j < 24; j++) {
+      double v = 0.5 * (Kc[i * 24 + j] + Kc[j * 24 + i]);
+      Kc[i * 24 + j] = v;
+      Kc[j * 24 + i] = v;
+    }
+  if (hasF) {
+    for (int i = 0; i < 9; i++) col[i] = f[24 + i];
+    chol_solve<double, double>(Kaa, 9, col);
+    for (int i = 0; i < 24; i++) {
+      double s = f[i];
+      for (int q = 0; q < 9; q++) s -= K[i * 33 + 24 + q] * col[q];
+      fc[i] = s;
+    }
+  } else {
+    for (int i = 0; i < 24; i++) fc[i] = 0;
+  }
+}
+
+}  // namespace fe

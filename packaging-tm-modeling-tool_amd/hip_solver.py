@@ -1,0 +1,218 @@
+"""ctypes binding of the C ABI in include/fcbga_hip.h.
+
+`Model(mesh)` wraps fcbga_model_create (the FEModel constructor's role,
+src/core/09-assembly.js:208-233) and `Model.solve_step(step, ...)` wraps
+fcbga_step_solve (assembleStep + LinearSolver.solve, src/core/11-analysis.js:189-206).
+The product library is libfcbga_hip.so (HIP, gfx950); loading fails loudly
+when it is missing.  `variant="hostemu"` loads the host-emulation build of the
+same sources, which only the test suite uses to check indexing without a GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBS = {"hip": os.path.join(HERE, "libfcbga_hip.so"), "hostemu": os.path.join(HERE, "libfcbga_hostemu.so")}
+
+PREC_FP64, PREC_MIXED = 0, 1
+PC_MG, PC_COLUMN, PC_JACOBI = 0, 1, 2
+
+_i32p = C.POINTER(C.c_int32)
+_i8p = C.POINTER(C.c_int8)
+_u8p = C.POINTER(C.c_uint8)
+_f64p = C.POINTER(C.c_double)
+
+
+class FcbgaMesh(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nn", C.c_int32), ("ne", C.c_int32),
+                ("xs", _f64p), ("ys", _f64p), ("zs", _f64p), ("node_id", _i32p), ("cell_elems", _i32p),
+                ("e_group", _i32p), ("e_part", _i8p), ("e_scale", _f64p), ("e_section", _i32p),
+                ("ngroups", C.c_int32), ("nsections", C.c_int32), ("nconstraints", C.c_int32),
+                ("constraint_dofs", _i32p), ("nfeature_x", C.c_int32), ("nfeature_y", C.c_int32),
+                ("feature_x", _i32p), ("feature_y", _i32p)]
+
+
+class FcbgaStep(C.Structure):
+    _fields_ = [("Ta", C.c_double), ("Tb", C.c_double), ("stage", C.c_int32), ("D", _f64p), ("s0", _f64p),
+                ("born", _u8p), ("secW", _f64p), ("secV", _f64p), ("sec_born", _u8p)]
+
+
+class FcbgaSolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxit", C.c_int32), ("precision", C.c_int32), ("precond", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class FcbgaSolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double), ("r0", C.c_double),
+                ("ms_total", C.c_double), ("ms_assemble", C.c_double), ("ms_setup", C.c_double),
+                ("ms_pcg", C.c_double), ("levels", C.c_int32), ("coarsest_dof", C.c_int32), ("ndof", C.c_int64),
+                ("ntypes", C.c_int32), ("reserved", C.c_int32)]
+
+
+EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_step_solve",
+           "fcbga_step_solve_device", "fcbga_step_load", "fcbga_step_apply", "fcbga_last_error"]
+
+_loaded = {}
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+def load(variant: str = "hip"):
+    if variant in _loaded:
+        return _loaded[variant]
+    path = LIBS[variant]
+    if not os.path.exists(path):
+        raise SolverError(f"{os.path.basename(path)} is not built: run __graft_entry__.build() first")
+    L = C.CDLL(path)
+    L.fcbga_model_create.argtypes = [C.POINTER(FcbgaMesh), C.POINTER(C.c_void_p)]
+    L.fcbga_model_create.restype = C.c_int
+    L.fcbga_model_destroy.argtypes = [C.c_void_p]
+    L.fcbga_model_ndof.argtypes = [C.c_void_p]
+    L.fcbga_model_ndof.restype = C.c_int64
+    for f in ("fcbga_step_solve", "fcbga_step_solve_device"):
+        getattr(L, f).argtypes = [C.c_void_p, C.POINTER(FcbgaStep), C.POINTER(FcbgaSolveOpts), C.c_void_p,
+                                  C.POINTER(FcbgaSolveStats)]
+        getattr(L, f).restype = C.c_int
+    L.fcbga_step_load.argtypes = [C.c_void_p, C.POINTER(FcbgaStep), _f64p]
+    L.fcbga_step_load.restype = C.c_int
+    L.fcbga_step_apply.argtypes = [C.c_void_p, C.POINTER(FcbgaStep), _f64p, _f64p]
+    L.fcbga_step_apply.restype = C.c_int
+    L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    L.fcbga_last_error.restype = C.c_int
+    _loaded[variant] = L
+    return L
+
+
+def last_error(L) -> str:
+    buf = C.create_string_buffer(1024)
+    L.fcbga_last_error(buf, 1024)
+    return buf.value.decode()
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class Step:
+    """Per-step group data (the prepareStep output, src/core/09-assembly.js:346-365)."""
+
+    def __init__(self, Ta, Tb, stage, D, s0, born, secW=None, secV=None, sec_born=None):
+        self.D = np.ascontiguousarray(D, np.float64).reshape(-1)
+        self.s0 = np.ascontiguousarray(s0, np.float64).reshape(-1)
+        self.born = np.ascontiguousarray(born, np.uint8)
+        self.secW = None if secW is None else np.ascontiguousarray(secW, np.float64).reshape(-1)
+        self.secV = None if secV is None else np.ascontiguousarray(secV, np.float64).reshape(-1)
+        self.sec_born = None if sec_born is None else np.ascontiguousarray(sec_born, np.uint8)
+        st = FcbgaStep()
+        st.Ta, st.Tb, st.stage = float(Ta), float(Tb), int(stage)
+        st.D = _ptr(self.D, _f64p)
+        st.s0 = _ptr(self.s0, _f64p)
+        st.born = _ptr(self.born, _u8p)
+        st.secW = _ptr(self.secW, _f64p)
+        st.secV = _ptr(self.secV, _f64p)
+        st.sec_born = _ptr(self.sec_born, _u8p)
+        self.c = st
+
+
+class Model:
+    def __init__(self, mesh, variant: str = "hip"):
+        self.L = load(variant)
+        self.mesh = mesh
+        a = dict(
+            xs=np.ascontiguousarray(mesh.xs, np.float64), ys=np.ascontiguousarray(mesh.ys, np.float64),
+            zs=np.ascontiguousarray(mesh.zs, np.float64),
+            node_id=np.ascontiguousarray(mesh.node_id, np.int32),
+            cell_elems=np.ascontiguousarray(mesh.cell_elems, np.int32),
+            e_group=np.ascontiguousarray(mesh.e_group, np.int32),
+            e_part=np.ascontiguousarray(mesh.e_part, np.int8),
+            e_scale=np.ascontiguousarray(mesh.e_scale, np.float64),
+            cons=np.ascontiguousarray(mesh.constraint_dofs, np.int32),
+            fx=np.ascontiguousarray(mesh.feature_x, np.int32), fy=np.ascontiguousarray(mesh.feature_y, np.int32))
+        self._keep = a
+        m = FcbgaMesh()
+        m.nx, m.ny, m.nz, m.nn, m.ne = mesh.nx, mesh.ny, mesh.nz, mesh.nn, mesh.ne
+        m.xs, m.ys, m.zs = _ptr(a["xs"], _f64p), _ptr(a["ys"], _f64p), _ptr(a["zs"], _f64p)
+        m.node_id = _ptr(a["node_id"], _i32p)
+        m.cell_elems = _ptr(a["cell_elems"], _i32p)
+        m.e_group = _ptr(a["e_group"], _i32p)
+        m.e_part = _ptr(a["e_part"], _i8p)
+        m.e_scale = _ptr(a["e_scale"], _f64p)
+        m.e_section = None
+        m.ngroups = len(mesh.groups)
+        m.nsections = 0
+        m.nconstraints = len(a["cons"])
+        m.constraint_dofs = _ptr(a["cons"], _i32p)
+        m.nfeature_x, m.nfeature_y = len(a["fx"]), len(a["fy"])
+        m.feature_x, m.feature_y = _ptr(a["fx"], _i32p), _ptr(a["fy"], _i32p)
+        self._cmesh = m
+        h = C.c_void_p()
+        rc = self.L.fcbga_model_create(C.byref(m), C.byref(h))
+        if rc != 0:
+            raise SolverError(f"fcbga_model_create failed ({rc}): {last_error(self.L)}")
+        self.h = h
+        self.ndof = int(self.L.fcbga_model_ndof(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fcbga_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def _opts(tol, maxit, precision, precond):
+        o = FcbgaSolveOpts()
+        o.tol, o.maxit, o.precision, o.precond, o.reserved = tol, maxit, precision, precond, 0
+        return o
+
+    def solve_step(self, step: Step, tol=1e-8, maxit=2000, precision=PREC_FP64, precond=PC_MG, check=True):
+        """du (3*nn, reference node order) and the solve record."""
+        du = np.zeros(self.ndof)
+        st = FcbgaSolveStats()
+        o = self._opts(tol, maxit, precision, precond)
+        rc = self.L.fcbga_step_solve(self.h, C.byref(step.c), C.byref(o), du.ctypes.data_as(C.c_void_p),
+                                     C.byref(st))
+        rec = {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+        if rc != 0 and (check or rc != -3):
+            raise SolverError(f"fcbga_step_solve failed ({rc}): {last_error(self.L)}")
+        return du, rec
+
+    def solve_step_device(self, step: Step, du_dev_ptr: int, tol=1e-8, maxit=2000, precision=PREC_FP64,
+                          precond=PC_MG):
+        st = FcbgaSolveStats()
+        o = self._opts(tol, maxit, precision, precond)
+        rc = self.L.fcbga_step_solve_device(self.h, C.byref(step.c), C.byref(o), C.c_void_p(du_dev_ptr),
+                                            C.byref(st))
+        if rc != 0:
+            raise SolverError(f"fcbga_step_solve_device failed ({rc}): {last_error(self.L)}")
+        return {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+
+    def load(self, step: Step):
+        f = np.zeros(self.ndof)
+        rc = self.L.fcbga_step_load(self.h, C.byref(step.c), f.ctypes.data_as(_f64p))
+        if rc != 0:
+            raise SolverError(last_error(self.L))
+        return f
+
+    def apply(self, step: Step, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.ndof)
+        rc = self.L.fcbga_step_apply(self.h, C.byref(step.c), x.ctypes.data_as(_f64p), y.ctypes.data_as(_f64p))
+        if rc != 0:
+            raise SolverError(last_error(self.L))
+        return y
+
+
+def step_for_spec(spec, Ta=None, Tb=None, stage=None):
+    from .meshspec import group_step_data
+    Ta = spec["Ta"] if Ta is None else Ta
+    Tb = spec["Tb"] if Tb is None else Tb
+    stage = spec.get("stage", 0) if stage is None else stage
+    D, s0, born = group_step_data(spec, Ta, Tb, stage)
+    return Step(Ta, Tb, stage, D, s0, born)
