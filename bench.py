@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Step-solve benchmark of the MI355X hex8 thermoelastic solver.
+
+Workload: the lidded full-fidelity FCBGA (160 x 160 x 52 hexes, graded,
+3,093,159 DOF) solved cold from the 150 C stress-free state to each of six
+temperatures 25..260 C; one "step" is one complete step solve (device
+assembly of the condensed element tables and load, multigrid set-up, PCG to a
+1e-10 scaled relative residual) in mixed precision (FP32 multigrid cycle
+inside FP64 flexible PCG).  The mesh is uploaded before the timed region; the
+displacements stay in HBM.  value = sum(DOF x PCG iterations) / timed seconds.
+
+Beside the timed mixed-precision solves the run also reports the plain FP64
+PCG-MG solve of the same steps (the comparator), the warpage error of each
+mixed solve against it, the dominant kernel's roofline from HIP-event timing,
+and the CPU oracle (a single-threaded C restatement of the reference's FP64
+PCG-MG) on a bounded sample of the same workload.
+
+  python bench.py [--gpus 1] [--steps 6] [--warmup 1] [--config cfg2|cfg1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__ as ge  # noqa: E402
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def incidences_per_node(mesh):
+    """Element-node incidences of every grid node (8 per element in total)."""
+    nx, ny, nz = mesh.nx, mesh.ny, mesh.nz
+    inc = np.zeros((nx, ny, nz), np.int64)
+    slots = (mesh.cell_elems.reshape(nx - 1, ny - 1, nz - 1, 2) >= 0).sum(axis=3)
+    for di in (0, 1):
+        for dj in (0, 1):
+            for dk in (0, 1):
+                inc[di:di + nx - 1, dj:dj + ny - 1, dk:dk + nz - 1] += slots
+    return inc
+
+
+def roofline_for(cls, ms_total, launches, mesh, dtype_cycle):
+    """Algorithmic flops of a node-centric operator launch: 144 flop per
+    (node, element) incidence (3 rows x 24 columns of the condensed matrix, 2 flop
+    per MAC); a full apply is 1152 flop per element."""
+    inc = incidences_per_node(mesh)
+    avg_s = ms_total / launches / 1e3
+    if cls == "apply":
+        flops = 144.0 * inc.sum()
+        peak, unit_dt = PEAK["fp64"], "fp64"
+    elif cls in ("fine_sweep",):
+        # one launch per colour (i mod 2, j mod 2): average incidences of a colour
+        per = [inc[a::2, b::2, :].sum() for a in (0, 1) for b in (0, 1)]
+        flops = 144.0 * float(np.mean(per))
+        peak, unit_dt = PEAK[dtype_cycle], dtype_cycle
+    elif cls == "fine_resid":
+        flops = 144.0 * inc.sum()
+        peak, unit_dt = PEAK[dtype_cycle], dtype_cycle
+    else:
+        return None
+    achieved = flops / avg_s
+    return {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None, "kernel": cls, "dtype": unit_dt,
+            "avg_launch_us": avg_s * 1e6, "launches": launches, "flop_per_launch": flops}
+
+
+def cpu_baseline(spec, mesh, iters=5):
+    """Single-threaded C restatement of the reference FP64 PCG-MG on the same
+    mesh: assembly + multigrid set-up + `iters` PCG iterations (bounded sample)."""
+    from oracle import oracle as orc
+    from fcbga_amd import meshspec
+    if not os.path.exists(orc.LIB_PATH):
+        orc.build()
+    D, s0, born = meshspec.group_step_data(spec, spec["Ta"], spec["Tb"], 0)
+    t0 = time.perf_counter()
+    sysm = orc.System(mesh, D, s0, born)
+    u, st = sysm.solve(tol=1e-30, maxit=iters)
+    dt = time.perf_counter() - t0
+    per_iter = st["ms_pcg"] / 1e3 / max(st["iters"], 1)
+    return {"value": mesh.ndof * st["iters"] / dt, "unit": "DOF*iter/s", "cores": 1, "kind": "port",
+            "sample": f"{spec['name']} full mesh ({mesh.ndof} DOF), one cold step 150->25 C: assembly + "
+                      f"multigrid set-up + {st['iters']} PCG iterations of the oracle (C, FP64, 1 thread)",
+            "seconds": dt, "assemble_s": st["ms_assemble"] / 1e3, "setup_s": st["ms_setup"] / 1e3,
+            "pcg_s_per_iter": per_iter}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2"])
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the solve runs on a single MI355X and this build has no "
+              f"multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    ge.package()
+    from fcbga_amd import cases, meshspec, post
+    from fcbga_amd import hip_solver as hs
+
+    spec = cases.cfg2_spec() if args.config == "cfg2" else cases.cfg1_spec()
+    temps = spec.get("temperatures", [spec["Tb"]])
+    mesh = meshspec.build_mesh(spec)
+    log(f"{spec['name']}: {mesh.ndof} DOF, {mesh.ne} elements")
+    model = hs.Model(mesh, variant="hip")
+    du = torch.empty(mesh.ndof, dtype=torch.float64, device="cuda")
+    steps = [hs.step_for_spec(spec, Ta=spec["Ta"], Tb=temps[k % len(temps)]) for k in range(max(args.steps, 1))]
+
+    def solve(k, prec):
+        return model.solve_step_device(steps[k % len(steps)], du.data_ptr(), tol=args.tol, maxit=2000,
+                                       precision=prec, precond=hs.PC_MG)
+
+    for w in range(args.warmup):
+        rec = solve(w, hs.PREC_MIXED)
+        log(f"warmup {w}: iters {rec['iters']} {rec['ms_total']:.1f} ms")
+    torch.cuda.synchronize()
+    model.set_kernel_timing(True)
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        recs.append(solve(k, hs.PREC_MIXED))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ktimes = model.kernel_times()
+    model.set_kernel_timing(False)
+    elapsed = t1 - t0
+    dof_iters = sum(mesh.ndof * r["iters"] for r in recs)
+    value = dof_iters / elapsed
+    for k, r in enumerate(recs):
+        log(f"step {k} T={temps[k % len(temps)]}: iters {r['iters']} relres {r['relres']:.2e} "
+            f"asm {r['ms_assemble']:.1f} setup {r['ms_setup']:.1f} pcg {r['ms_pcg']:.1f} ms")
+    log("kernel classes (ms total, launches):", json.dumps(ktimes))
+
+    # dominant timed kernel class and its roofline
+    timed = {k: v for k, v in ktimes.items() if v[1] > 0}
+    dom = max(timed, key=lambda k: timed[k][0]) if timed else None
+    roof = None
+    for cls in ([dom] if dom else []) + ["fine_sweep", "apply"]:
+        if cls in timed:
+            roof = roofline_for(cls, timed[cls][0], timed[cls][1], mesh, "fp32")
+            if roof:
+                break
+    if roof and dom and roof["kernel"] != dom:
+        roof["note"] = f"largest timed class is {dom}, whose work is not a single operator launch"
+
+    out = {
+        "metric": "solve s & DOF*iter/s at ~4M DOF: mixed vs FP64 PCG-MG at 1e-6 warpage err",
+        "value": value, "unit": "DOF*iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / max(args.steps, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{spec['name']}: one cold thermoelastic step solve per step (150 C stress-free "
+                               f"to T in {temps}), PCG to {args.tol:g} scaled relative residual",
+                   "ndof": mesh.ndof, "elements": mesh.ne, "precision": "FP32 multigrid in FP64 flexible PCG"},
+        "solve_s_mixed": elapsed / max(args.steps, 1),
+        "iters_mixed": [r["iters"] for r in recs],
+        "setup_ms_mixed": float(np.mean([r["ms_assemble"] + r["ms_setup"] for r in recs])),
+        "roofline": roof,
+    }
+
+    if not args.no_fp64:
+        # FP64 comparator on the same steps; warpage error of each mixed solve
+        bga = meshspec.bga_sites(**spec["bga"])
+        t_fp64, it64, werr = [], [], []
+        for k in range(args.steps):
+            rec = solve(k, hs.PREC_MIXED)
+            u_mixed = du.cpu().numpy().copy()
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            rec64 = solve(k, hs.PREC_FP64)
+            torch.cuda.synchronize()
+            t_fp64.append(time.perf_counter() - ta)
+            it64.append(rec64["iters"])
+            u64 = du.cpu().numpy()
+            w_m = post.warpage_jeita(mesh, u_mixed, bga)["signed"]
+            w_d = post.warpage_jeita(mesh, u64, bga)["signed"]
+            werr.append(abs(w_m - w_d) / abs(w_d))
+        out["solve_s_fp64"] = float(np.mean(t_fp64))
+        out["iters_fp64"] = it64
+        out["speedup_mixed_vs_fp64"] = out["solve_s_fp64"] / out["solve_s_mixed"]
+        out["warpage_rel_err_mixed_vs_fp64_max"] = float(max(werr))
+        log(f"FP64 comparator: {out['solve_s_fp64']:.3f} s/step, warpage err max {max(werr):.2e}")
+
+    if not args.no_cpu:
+        log("CPU oracle sample ...")
+        base = cpu_baseline(spec, mesh)
+        out["cpu_baseline"] = base
+        log(f"CPU: {base['value']:.3e} DOF*iter/s over {base['seconds']:.1f} s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

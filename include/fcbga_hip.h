@@ -116,6 +116,22 @@ int fcbga_step_solve_device(fcbga_model *m, const fcbga_step *step, const fcbga_
 int fcbga_step_load(fcbga_model *m, const fcbga_step *step, double *f_out);
 int fcbga_step_apply(fcbga_model *m, const fcbga_step *step, const double *x, double *y_out);
 
+/* Measurement hook: with timing enabled, launches of the kernel classes below are
+ * bracketed by HIP events on the library's stream and their device time is summed
+ * per class (reset by every fcbga_set_kernel_timing call).  The reference has no
+ * counterpart; its timings are the wall-clock fields of the solve record
+ * (src/core/10-solver.js:988-993). */
+#define FCBGA_KCLASS_APPLY 0        /* FP64 operator apply of the outer PCG (one launch) */
+#define FCBGA_KCLASS_FINE_SWEEP 1   /* fine colour-sweep residual (one launch per colour) */
+#define FCBGA_KCLASS_FINE_RESID 2   /* fine residual before restriction (one launch) */
+#define FCBGA_KCLASS_SHELL1_SWEEP 3 /* first shell level: 2 sweeps x 4 colours (bracketed together) */
+#define FCBGA_KCLASS_SHELL_SWEEP 4  /* coarser shell levels: 2 sweeps x 4 colours (bracketed together) */
+#define FCBGA_KCLASS_COARSEST 5     /* coarsest dense FP64 product (one launch) */
+#define FCBGA_KCLASS_COUNT 8
+int fcbga_set_kernel_timing(fcbga_model *m, int enable);
+int fcbga_kernel_times(const fcbga_model *m, double *ms_total /*[FCBGA_KCLASS_COUNT]*/,
+                       int32_t *launches /*[FCBGA_KCLASS_COUNT]*/);
+
 int fcbga_last_error(char *buf, size_t len);
 
 #ifdef __cplusplus

@@ -28,6 +28,8 @@ namespace {
 
 thread_local std::string g_err;
 
+constexpr int SHELL_MAXM = 512;  // 6 DOFs x up to 85 aggregates in one shell column
+
 double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
@@ -155,7 +157,7 @@ struct DevShell {
   DBuf<double> val64;
   DBuf<float> val32;
   DBuf<int64_t> cfPtr;
-  DBuf<double> cf64;
+  DBuf<double> cf64, cfTmp;
   DBuf<float> cf32;
   DBuf<int32_t> colors[4];
   int32_t ncolor[4] = {0, 0, 0, 0};
@@ -215,7 +217,67 @@ struct fcbga_model {
   DBuf<int32_t> nodeAgg;
   DBuf<double> zp64;
   DBuf<float> zp32;
+  // kernel timing (measurement hook): event pairs per kernel class
+  bool timing = false;
+  double kt_ms[FCBGA_KCLASS_COUNT] = {};
+  int32_t kt_n[FCBGA_KCLASS_COUNT] = {};
+#if FCBGA_GPU
+  std::vector<hipEvent_t> evPool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evUsed;
+#endif
 };
+
+namespace {
+
+// Kernel timing: when enabled, launches of the timed classes are bracketed by
+// HIP events on the model's stream; the elapsed times are summed per class
+// after each solve (fcbga_kernel_times).
+struct KScope {
+  fcbga_model *M;
+  int cls;
+#if FCBGA_GPU
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+#endif
+  KScope(fcbga_model *m, int c) : M(m), cls(c) {
+#if FCBGA_GPU
+    if (!M->timing) return;
+    for (int q = 0; q < 2; q++) {
+      hipEvent_t e;
+      if (M->evPool.empty()) BE_CHECK(hipEventCreate(&e));
+      else { e = M->evPool.back(); M->evPool.pop_back(); }
+      (q ? e1 : e0) = e;
+    }
+    BE_CHECK(hipEventRecord(e0, M->stream));
+#endif
+  }
+  ~KScope() {
+#if FCBGA_GPU
+    if (!M->timing || !e0) return;
+    (void)hipEventRecord(e1, M->stream);
+    M->evUsed.push_back({cls, {e0, e1}});
+#endif
+  }
+};
+
+void ktimer_collect(fcbga_model *M) {
+#if FCBGA_GPU
+  if (M->evUsed.empty()) return;
+  BE_CHECK(hipStreamSynchronize(M->stream));
+  for (auto &u : M->evUsed) {
+    float ms = 0;
+    BE_CHECK(hipEventElapsedTime(&ms, u.second.first, u.second.second));
+    M->kt_ms[u.first] += ms;
+    M->kt_n[u.first] += 1;
+    M->evPool.push_back(u.second.first);
+    M->evPool.push_back(u.second.second);
+  }
+  M->evUsed.clear();
+#else
+  (void)M;
+#endif
+}
+
+}  // namespace
 
 namespace {
 
@@ -567,12 +629,14 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
 // one colour of a column block Gauss-Seidel sweep: residual of the colour's
 // columns, then their block-tridiagonal solves (columnSweep, src/core/10-solver.js:512-555)
 template <typename T>
-void fine_color_sweep(const FineOps<T> &o, const int32_t *cols, int32_t ncols, const T *b, T *x, T *r, T *dx,
-                      be::stream_t s) {
+void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, int32_t ncols, const T *b, T *x, T *r,
+                      T *dx, be::stream_t s) {
   const fe::Grid g = o.g;
   const int32_t *ct = o.cellType;
   const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w, *F = o.F;
   const int nz = g.nz;
+  {
+  KScope ks(M, FCBGA_KCLASS_FINE_SWEEP);
   be::pfor((int64_t)ncols * nz, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t / nz];
     int k = (int)(t % nz);
@@ -582,6 +646,7 @@ void fine_color_sweep(const FineOps<T> &o, const int32_t *cols, int32_t ncols, c
     fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
     for (int p = 0; p < 3; p++) r[3 * gi + p] = b[3 * gi + p] - out[p];
   });
+  }
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t];
     fe::column_solve<T, T>(nz, F + 18 * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
@@ -633,6 +698,8 @@ void build_hierarchy(fcbga_model *M) {
       cfp[c + 1] = cfp[c] + m * m;
     }
     D->cfTotal = cfp[D->ncol];
+    for (int32_t c = 0; c < D->ncol; c++)
+      if (6 * (L.colStart[c + 1] - L.colStart[c]) > SHELL_MAXM) throw be::Error("too many shell aggregates in one column");
     D->cfPtr.upload(cfp, s);
     D->cf64.alloc(D->cfTotal);
     D->cf32.alloc(D->cfTotal);
@@ -775,19 +842,23 @@ void galerkin_shell(fcbga_model *M, int l) {
   });
 }
 
-// dense Cholesky of every column of a shell level (factorShellColumns, src/core/10-solver.js:474-494)
+// Dense Cholesky of every column of a shell level (factorShellColumns,
+// src/core/10-solver.js:474-494, jitter 1e-12) followed by the explicit
+// inverse of the column block, so that a column solve in the cycle is one
+// small dense product spread over a wavefront.
 void factor_shell_columns(fcbga_model *M, int l) {
   DevShell &D = *M->lv[l];
   const int32_t *cst = D.colStart.p, *rp = D.rowPtr.p, *col = D.col.p;
   const int64_t *cfp = D.cfPtr.p;
   const double *val = D.val64.p;
-  double *cf = D.cf64.p;
+  D.cfTmp.alloc(D.cfTotal);
+  double *L = D.cfTmp.p, *cf = D.cf64.p;
   float *cf32 = D.cf32.p;
   be::pfor(D.ncol, M->stream, [=] __host__ __device__(int64_t c) {
     int32_t n0 = cst[c], n1 = cst[c + 1];
     int m = 6 * (n1 - n0);
     if (!m) return;
-    double *A = cf + cfp[c];
+    double *A = L + cfp[c];
     for (int q = 0; q < m * m; q++) A[q] = 0;
     for (int32_t n = n0; n < n1; n++)
       for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
@@ -797,8 +868,130 @@ void factor_shell_columns(fcbga_model *M, int l) {
           for (int q = 0; q < 6; q++) A[((n - n0) * 6 + p) * m + (j - n0) * 6 + q] = val[36 * (int64_t)k + 6 * p + q];
       }
     fe::chol_dense(A, m, 1e-12);
-    for (int q = 0; q < m * m; q++) cf32[cfp[c] + q] = (float)A[q];
+    double *Ai = cf + cfp[c];
+    for (int j = 0; j < m; j++) {
+      double *x = Ai + (int64_t)j * m;  // column j
+      for (int i = 0; i < m; i++) x[i] = (i == j) ? 1.0 : 0.0;
+      fe::chol_solve<double, double>(A, m, x);
+    }
+    for (int q = 0; q < m * m; q++) cf32[cfp[c] + q] = (float)Ai[q];
   }, 64);
+}
+
+#if FCBGA_GPU
+// inverse of a banded Cholesky factor, one 64-lane block per unit vector e_c;
+// the vector lives in LDS and every substitution step is a lane-parallel dot
+__global__ void __launch_bounds__(64) k_band_inverse(int32_t n, int bw, int w, const double *__restrict__ Ab,
+                                                     double *__restrict__ inv) {
+  __shared__ double xs[4096];
+  const int32_t c = blockIdx.x;
+  const int lane = threadIdx.x;
+  for (int32_t i = lane; i < n; i += 64) xs[i] = 0.0;
+  __syncthreads();
+  for (int32_t i = c; i < n; i++) {
+    double part = 0;
+    int32_t k0 = i - bw > c ? i - bw : c;
+    for (int32_t k = k0 + lane; k < i; k += 64) part += Ab[(int64_t)i * w + (k - i + bw)] * xs[k];
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+    if (lane == 0) xs[i] = ((i == c ? 1.0 : 0.0) - part) / Ab[(int64_t)i * w + bw];
+    __syncthreads();
+  }
+  for (int32_t i = n - 1; i >= 0; i--) {
+    double part = 0;
+    int32_t kend = i + bw + 1 < n ? i + bw + 1 : n;
+    for (int32_t k = i + 1 + lane; k < kend; k += 64) part += Ab[(int64_t)k * w + (i - k + bw)] * xs[k];
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+    if (lane == 0) xs[i] = (xs[i] - part) / Ab[(int64_t)i * w + bw];
+    __syncthreads();
+  }
+  for (int32_t i = lane; i < n; i += 64) inv[(int64_t)c * n + i] = xs[i];
+}
+
+// x = inv b, one wavefront per row (coalesced row reads), FP64 accumulation
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_matvec(int32_t n, const double *__restrict__ A, const T *__restrict__ b,
+                                                      T *__restrict__ x) {
+  const int32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double part = 0;
+  const double *Ar = A + (int64_t)row * n;
+  for (int32_t j = lane; j < n; j += 64) part += Ar[j] * (double)b[j];
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+  if (lane == 0) x[row] = (T)part;
+}
+
+// one colour of a shell-level column sweep: one 64-lane block per column;
+// lanes compute the column's residual rows into LDS, then the correction
+// x_col += Ainv_col r_col (Ainv stored by columns, read coalesced)
+template <typename T>
+__global__ void __launch_bounds__(64) k_shell_sweep(const int32_t *__restrict__ cols, const int32_t *__restrict__ cst,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                    const T *__restrict__ val, const T *__restrict__ inv,
+                                                    const int64_t *__restrict__ cfp, const T *__restrict__ b,
+                                                    T *__restrict__ x) {
+  __shared__ T r[SHELL_MAXM];
+  const int32_t c = cols[blockIdx.x];
+  const int32_t n0 = cst[c], n1 = cst[c + 1];
+  const int m = 6 * (n1 - n0);
+  for (int i = threadIdx.x; i < m; i += 64) {
+    const int32_t n = n0 + i / 6;
+    const int p = i % 6;
+    T acc = b[6 * (int64_t)n0 + i];
+    for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
+      const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
+      acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
+    }
+    r[i] = acc;
+  }
+  __syncthreads();
+  const T *A = inv + cfp[c];
+  for (int i = threadIdx.x; i < m; i += 64) {
+    T acc = 0;
+    for (int j = 0; j < m; j++) acc += A[(int64_t)j * m + i] * r[j];
+    x[6 * (int64_t)n0 + i] += acc;
+  }
+}
+#endif
+
+void band_inverse(int32_t n, int bw, int w, const double *Ab, double *inv, be::stream_t s) {
+  if (n > 4096) throw be::Error("coarsest level larger than 4096 DOFs");
+#if FCBGA_GPU
+  hipLaunchKernelGGL(k_band_inverse, dim3(n), dim3(64), 0, s, n, bw, w, Ab, inv);
+  BE_CHECK(hipGetLastError());
+#else
+  (void)s;
+  std::vector<double> x(n);
+  for (int32_t c = 0; c < n; c++) {
+    std::fill(x.begin(), x.end(), 0.0);
+    for (int32_t i = c; i < n; i++) {
+      double part = 0;
+      for (int32_t k = std::max(c, i - bw); k < i; k++) part += Ab[(int64_t)i * w + (k - i + bw)] * x[k];
+      x[i] = ((i == c ? 1.0 : 0.0) - part) / Ab[(int64_t)i * w + bw];
+    }
+    for (int32_t i = n - 1; i >= 0; i--) {
+      double part = 0;
+      for (int32_t k = i + 1; k < std::min(n, i + bw + 1); k++) part += Ab[(int64_t)k * w + (i - k + bw)] * x[k];
+      x[i] = (x[i] - part) / Ab[(int64_t)i * w + bw];
+    }
+    for (int32_t i = 0; i < n; i++) inv[(int64_t)c * n + i] = x[i];
+  }
+#endif
+}
+
+template <typename T>
+void dense_matvec(int32_t n, const double *A, const T *b, T *x, be::stream_t s) {
+#if FCBGA_GPU
+  hipLaunchKernelGGL(k_dense_matvec<T>, dim3((n + 3) / 4), dim3(256), 0, s, n, A, b, x);
+  BE_CHECK(hipGetLastError());
+#else
+  (void)s;
+  for (int32_t i = 0; i < n; i++) {
+    double acc = 0;
+    for (int32_t j = 0; j < n; j++) acc += A[(int64_t)i * n + j] * (double)b[j];
+    x[i] = (T)acc;
+  }
+#endif
 }
 
 // Coarsest level: banded Cholesky on the host with the jitter rule of
@@ -848,25 +1041,7 @@ void factor_coarsest(fcbga_model *M) {
   D.band.upload(A, M->stream);
   D.inv64.alloc((size_t)n * n);
   D.ndense = n;
-  const double *Ab = D.band.p;
-  double *inv = D.inv64.p;
-  be::pfor(n, M->stream, [=] __host__ __device__(int64_t c) {
-    // column c of the inverse, stored as row c (the inverse is symmetric)
-    double *x = inv + (int64_t)c * n;
-    for (int32_t i = 0; i < n; i++) x[i] = (i == c) ? 1.0 : 0.0;
-    for (int32_t i = (int32_t)c; i < n; i++) {
-      double sum = x[i];
-      int32_t k0 = i - bw > 0 ? i - bw : 0;
-      for (int32_t k = k0; k < i; k++) sum -= Ab[(int64_t)i * w + (k - i + bw)] * x[k];
-      x[i] = sum / Ab[(int64_t)i * w + bw];
-    }
-    for (int32_t i = n - 1; i >= 0; i--) {
-      double sum = x[i];
-      int32_t kend = i + bw + 1 < n ? i + bw + 1 : n;
-      for (int32_t k = i + 1; k < kend; k++) sum -= Ab[(int64_t)k * w + (i - k + bw)] * x[k];
-      x[i] = sum / Ab[(int64_t)i * w + bw];
-    }
-  }, 64);
+  band_inverse(n, bw, w, D.band.p, D.inv64.p, M->stream);
 }
 
 double mg_setup(fcbga_model *M) {
@@ -914,29 +1089,37 @@ template <typename T>
 void shell_sweep(DevShell &D, int color, be::stream_t s) {
   const int32_t *cols = D.colors[color].p, *cst = D.colStart.p, *rp = D.rowPtr.p, *col = D.col.p;
   const int64_t *cfp = D.cfPtr.p;
-  const T *val = LevelVec<T>::val(D), *cf = LevelVec<T>::cf(D), *b = LevelVec<T>::b(D);
-  T *x = LevelVec<T>::x(D), *r = LevelVec<T>::r(D);
-  be::pfor(D.ncolor[color], s, [=] __host__ __device__(int64_t t) {
-    int32_t c = cols[t];
-    int32_t n0 = cst[c], n1 = cst[c + 1];
-    if (n1 == n0) return;
-    for (int32_t n = n0; n < n1; n++) {
-      T acc[6];
-      for (int p = 0; p < 6; p++) acc[p] = b[6 * (int64_t)n + p];
-      for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
-        const T *v = val + 36 * (int64_t)k, *xj = x + 6 * (int64_t)col[k];
-        for (int p = 0; p < 6; p++) {
-          T sum = 0;
-          for (int q = 0; q < 6; q++) sum += v[6 * p + q] * xj[q];
-          acc[p] -= sum;
-        }
-      }
-      for (int p = 0; p < 6; p++) r[6 * (int64_t)n + p] = acc[p];
-    }
+  const T *val = LevelVec<T>::val(D), *inv = LevelVec<T>::cf(D), *b = LevelVec<T>::b(D);
+  T *x = LevelVec<T>::x(D);
+  const int32_t ncols = D.ncolor[color];
+  if (!ncols) return;
+#if FCBGA_GPU
+  hipLaunchKernelGGL(k_shell_sweep<T>, dim3(ncols), dim3(64), 0, s, cols, cst, rp, col, val, inv, cfp, b, x);
+  BE_CHECK(hipGetLastError());
+#else
+  (void)s;
+  T r[SHELL_MAXM];
+  for (int32_t t = 0; t < ncols; t++) {
+    int32_t c = cols[t], n0 = cst[c], n1 = cst[c + 1];
     int m = 6 * (n1 - n0);
-    fe::chol_solve<T, T>(cf + cfp[c], m, r + 6 * (int64_t)n0);
-    for (int q = 0; q < m; q++) x[6 * (int64_t)n0 + q] += r[6 * (int64_t)n0 + q];
-  }, 64);
+    for (int i = 0; i < m; i++) {
+      int32_t n = n0 + i / 6;
+      int p = i % 6;
+      T acc = b[6 * (int64_t)n0 + i];
+      for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
+        const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
+        acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
+      }
+      r[i] = acc;
+    }
+    const T *A = inv + cfp[c];
+    for (int i = 0; i < m; i++) {
+      T acc = 0;
+      for (int j = 0; j < m; j++) acc += A[(int64_t)j * m + i] * r[j];
+      x[6 * (int64_t)n0 + i] += acc;
+    }
+  }
+#endif
 }
 
 template <typename T>
@@ -971,19 +1154,16 @@ void shell_cycle(fcbga_model *M, int l) {
   T *x = LevelVec<T>::x(D);
   const T *b = LevelVec<T>::b(D);
   if (D.coarsest) {
-    const double *inv = D.inv64.p;
-    const int32_t n = D.ndense;
-    be::pfor(n, s, [=] __host__ __device__(int64_t i) {
-      double acc = 0;
-      const double *row = inv + i * (int64_t)n;
-      for (int32_t j = 0; j < n; j++) acc += row[j] * (double)b[j];
-      x[i] = (T)acc;
-    });
+    KScope ks(M, FCBGA_KCLASS_COARSEST);
+    dense_matvec<T>(D.ndense, D.inv64.p, b, x, s);
     return;
   }
   zero_vec<T>(x, 6 * (int64_t)D.nagg, s);
-  for (int sw = 0; sw < 2; sw++)
-    for (int c = 0; c < 4; c++) shell_sweep<T>(D, c, s);
+  {
+    KScope ks(M, l == 0 ? FCBGA_KCLASS_SHELL1_SWEEP : FCBGA_KCLASS_SHELL_SWEEP);
+    for (int sw = 0; sw < 2; sw++)
+      for (int c = 0; c < 4; c++) shell_sweep<T>(D, c, s);
+  }
   shell_residual<T>(D, s);
   DevShell &C = *M->lv[l + 1];
   {
@@ -1018,6 +1198,7 @@ void shell_cycle(fcbga_model *M, int l) {
       }
     });
   }
+  KScope ks(M, l == 0 ? FCBGA_KCLASS_SHELL1_SWEEP : FCBGA_KCLASS_SHELL_SWEEP);
   for (int sw = 0; sw < 2; sw++)
     for (int c = 3; c >= 0; c--) shell_sweep<T>(D, c, s);
 }
@@ -1030,9 +1211,12 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
   const int64_t n3 = 3 * M->g.G;
   zero_vec<T>(x, n3, s);
   for (int sw = 0; sw < 2; sw++)
-    for (int c = 0; c < 4; c++) fine_color_sweep<T>(o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
+    for (int c = 0; c < 4; c++) fine_color_sweep<T>(M, o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
   if (withCoarse) {
-    fine_residual<T>(o, b, x, r, s);
+    {
+      KScope ks(M, FCBGA_KCLASS_FINE_RESID);
+      fine_residual<T>(o, b, x, r, s);
+    }
     DevShell &C = *M->lv[0];
     const int32_t *aggCol = C.aggCol.p, *k0 = C.k0.p, *k1 = C.k1.p, *nodeAgg = o.nodeAgg;
     const T *sinv = o.sinv, *zp = o.zp;
@@ -1065,7 +1249,7 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
     });
   }
   for (int sw = 0; sw < 2; sw++)
-    for (int c = 3; c >= 0; c--) fine_color_sweep<T>(o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
+    for (int c = 3; c >= 0; c--) fine_color_sweep<T>(M, o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1168,7 +1352,10 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
       double rz = dotd(M, r, z, 1);
       const bool flex = o.precision == FCBGA_PREC_MIXED;
       for (it = 1; it <= o.maxit; it++) {
-        fine_apply<double>(f64, p, q, s);
+        {
+          KScope ks(M, FCBGA_KCLASS_APPLY);
+          fine_apply<double>(f64, p, q, s);
+        }
         double pq = dotd(M, p, q, 2);
         if (!(pq > 0)) break;
         const double alpha = rz / pq;
@@ -1206,6 +1393,7 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
   }
   if (du_host) be::d2h(du_host, du_dev, sizeof(double) * 3 * (size_t)M->nn, s);
   be::sync(s);
+  ktimer_collect(M);
   const double t3 = now_ms();
   S.ms_pcg = t3 - t2;
   S.ms_total = t3 - t0;
@@ -1249,6 +1437,8 @@ void fcbga_model_destroy(fcbga_model *m) {
   if (!m) return;
 #if FCBGA_GPU
   if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (hipEvent_t e : m->evPool) (void)hipEventDestroy(e);
+  for (auto &u : m->evUsed) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
   hipStream_t s = m->stream;
   delete m;
   if (s) (void)hipStreamDestroy(s);
@@ -1358,6 +1548,19 @@ int fcbga_step_apply(fcbga_model *m, const fcbga_step *step, const double *x, do
   } catch (const std::exception &e) {
     return fail(e, FCBGA_ERR_INTERNAL);
   }
+  return FCBGA_OK;
+}
+
+int fcbga_set_kernel_timing(fcbga_model *m, int enable) {
+  if (!m) { g_err = "model is NULL"; return FCBGA_ERR_ARG; }
+  m->timing = enable != 0;
+  for (int c = 0; c < FCBGA_KCLASS_COUNT; c++) { m->kt_ms[c] = 0; m->kt_n[c] = 0; }
+  return FCBGA_OK;
+}
+
+int fcbga_kernel_times(const fcbga_model *m, double *ms_total, int32_t *launches) {
+  if (!m || !ms_total || !launches) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  for (int c = 0; c < FCBGA_KCLASS_COUNT; c++) { ms_total[c] = m->kt_ms[c]; launches[c] = m->kt_n[c]; }
   return FCBGA_OK;
 }
 

@@ -53,7 +53,12 @@ class FcbgaSolveStats(C.Structure):
 
 
 EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_step_solve",
-           "fcbga_step_solve_device", "fcbga_step_load", "fcbga_step_apply", "fcbga_last_error"]
+           "fcbga_step_solve_device", "fcbga_step_load", "fcbga_step_apply", "fcbga_set_kernel_timing",
+           "fcbga_kernel_times", "fcbga_last_error"]
+
+# kernel classes of fcbga_kernel_times (include/fcbga_hip.h)
+KCLASS = {"apply": 0, "fine_sweep": 1, "fine_resid": 2, "shell1_sweep": 3, "shell_sweep": 4, "coarsest": 5}
+KCLASS_COUNT = 8
 
 _loaded = {}
 
@@ -82,6 +87,10 @@ def load(variant: str = "hip"):
     L.fcbga_step_load.restype = C.c_int
     L.fcbga_step_apply.argtypes = [C.c_void_p, C.POINTER(FcbgaStep), _f64p, _f64p]
     L.fcbga_step_apply.restype = C.c_int
+    L.fcbga_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+    L.fcbga_set_kernel_timing.restype = C.c_int
+    L.fcbga_kernel_times.argtypes = [C.c_void_p, _f64p, _i32p]
+    L.fcbga_kernel_times.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.fcbga_last_error.restype = C.c_int
     _loaded[variant] = L
@@ -192,6 +201,16 @@ class Model:
         if rc != 0:
             raise SolverError(f"fcbga_step_solve_device failed ({rc}): {last_error(self.L)}")
         return {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+
+    def set_kernel_timing(self, enable: bool):
+        self.L.fcbga_set_kernel_timing(self.h, 1 if enable else 0)
+
+    def kernel_times(self):
+        """{class: (total ms, launches)} accumulated since set_kernel_timing."""
+        ms = np.zeros(KCLASS_COUNT)
+        n = np.zeros(KCLASS_COUNT, np.int32)
+        self.L.fcbga_kernel_times(self.h, ms.ctypes.data_as(_f64p), n.ctypes.data_as(_i32p))
+        return {k: (float(ms[v]), int(n[v])) for k, v in KCLASS.items()}
 
     def load(self, step: Step):
         f = np.zeros(self.ndof)
