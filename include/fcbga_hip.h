@@ -79,7 +79,7 @@ typedef struct fcbga_solve_opts {
   int32_t maxit;
   int32_t precision;  /* FCBGA_PREC_* */
   int32_t precond;    /* FCBGA_PC_* */
-  int32_t reserved;
+  int32_t sweeps;     /* pre- and post-smoothing sweeps per level; 0 = the reference's 2 */
 } fcbga_solve_opts;
 
 /* Solve record of LinearSolver.solve (src/core/10-solver.js:988-993) plus timings. */

@@ -29,6 +29,7 @@ namespace {
 thread_local std::string g_err;
 
 constexpr int SHELL_MAXM = 512;  // 6 DOFs x up to 85 aggregates in one shell column
+constexpr int FINE_MAXNZ = 64;  // grid planes in z: one lane per plane in a fine column block
 
 double now_ms() {
   using namespace std::chrono;
@@ -205,7 +206,8 @@ struct fcbga_model {
   DBuf<float> cellScale32, Kc32;
   DBuf<uint8_t> slotHasF, slotLayered, typeHasF, d_born;
   DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64;
-  DBuf<float> w32, sinv32, F32;
+  DBuf<float> w32, sinv32, F32, C32;
+  DBuf<double> C64;
   // solver vectors (grid layout, 3 per node)
   DBuf<double> x, r, z, zold, p, q, tmp64, dx64;
   DBuf<float> b32v, x32v, r32v, dx32;
@@ -219,6 +221,12 @@ struct fcbga_model {
   DBuf<float> zp32;
   // kernel timing (measurement hook): event pairs per kernel class
   bool timing = false;
+  int nsweeps = 2;     // pre- and post-smoothing sweeps per level
+  bool useGraphs = true;
+#if FCBGA_GPU
+  std::map<int, hipGraphExec_t> graphs;
+  double *hostScal = nullptr;  // pinned
+#endif
   double kt_ms[FCBGA_KCLASS_COUNT] = {};
   int32_t kt_n[FCBGA_KCLASS_COUNT] = {};
 #if FCBGA_GPU
@@ -381,6 +389,7 @@ void model_create(const fcbga_mesh *mm, fcbga_model *M) {
         }
       }
   M->ntypes = (int32_t)M->typeSlot.size();
+  if (g.nz > FINE_MAXNZ) throw be::Error("more than 64 grid planes in z");
   // DOF mask and compact -> grid map
   std::vector<uint8_t> mask(3 * g.G, 0);
   std::vector<int64_t> gridOf(M->nn);
@@ -550,19 +559,22 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
   }
 }
 
-// column factors of the fine level (FP64, FP32 copy)
+// column factors of the fine level and the coefficients of the scan form of
+// the column solve (FP64 and FP32 copies)
 void factor_fine_columns(fcbga_model *M) {
   const fe::Grid g = M->g;
   const int64_t ncol = (int64_t)g.nx * g.ny;
   M->F64.alloc(18 * g.G);
-  M->F32.alloc(18 * g.G);
+  M->C64.alloc(27 * g.G);
+  M->C32.alloc(27 * g.G);
   const double *Db = M->Db.p, *Lb = M->Lb.p;
-  double *F = M->F64.p;
-  float *F32 = M->F32.p;
+  double *F = M->F64.p, *C = M->C64.p;
+  float *C32 = M->C32.p;
   const int nz = g.nz;
   be::pfor(ncol, M->stream, [=] __host__ __device__(int64_t c) {
     fe::column_factor(nz, Db + 9 * c * nz, Lb + 9 * c * nz, F + 18 * c * nz);
-    for (int q = 0; q < 18 * nz; q++) F32[18 * c * nz + q] = (float)F[18 * c * nz + q];
+    fe::column_scan_coefs<double>(nz, F + 18 * c * nz, C + 27 * c * nz);
+    for (int q = 0; q < 27 * nz; q++) C32[27 * c * nz + q] = (float)C[27 * c * nz + q];
   }, 64);
 }
 
@@ -585,12 +597,12 @@ FineOps<T> fine_ops(fcbga_model *M);
 
 template <>
 FineOps<double> fine_ops<double>(fcbga_model *M) {
-  return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->F64.p,
+  return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->C64.p,
                          M->nodeAgg.p, M->zp64.p};
 }
 template <>
 FineOps<float> fine_ops<float>(fcbga_model *M) {
-  return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->F32.p,
+  return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->C32.p,
                         M->nodeAgg.p, M->zp32.p};
 }
 
@@ -626,8 +638,85 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
   });
 }
 
-// one colour of a column block Gauss-Seidel sweep: residual of the colour's
-// columns, then their block-tridiagonal solves (columnSweep, src/core/10-solver.js:512-555)
+// One colour of a column block Gauss-Seidel sweep (columnSweep,
+// src/core/10-solver.js:512-555): for every column of the colour, residual
+// r = b - A x over its nodes, block-tridiagonal solve (solveColumn,
+// src/core/10-solver.js:71-104), x += dx.  Columns of one colour share no
+// element, so they update independently.  GPU: one 64-lane block per column;
+// the lanes compute the residual rows and stage the column factor in LDS, one
+// lane runs the substitution chain, and the lanes apply the update.
+#if FCBGA_GPU
+template <typename T>
+__device__ __forceinline__ void scan_round_up(T A[9], T b[3], int s, int lane) {
+  T An[9], bn[3];
+  for (int q = 0; q < 9; q++) An[q] = __shfl_up(A[q], s, 64);
+  for (int p = 0; p < 3; p++) bn[p] = __shfl_up(b[p], s, 64);
+  if (lane >= s) {
+    T R[9], rb[3];
+    for (int i = 0; i < 3; i++) {
+      for (int j = 0; j < 3; j++) R[3 * i + j] = A[3 * i] * An[j] + A[3 * i + 1] * An[3 + j] + A[3 * i + 2] * An[6 + j];
+      rb[i] = A[3 * i] * bn[0] + A[3 * i + 1] * bn[1] + A[3 * i + 2] * bn[2] + b[i];
+    }
+    for (int q = 0; q < 9; q++) A[q] = R[q];
+    for (int p = 0; p < 3; p++) b[p] = rb[p];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void scan_round_down(T A[9], T b[3], int s, int lane, int nz) {
+  T An[9], bn[3];
+  for (int q = 0; q < 9; q++) An[q] = __shfl_down(A[q], s, 64);
+  for (int p = 0; p < 3; p++) bn[p] = __shfl_down(b[p], s, 64);
+  if (lane + s < nz) {
+    T R[9], rb[3];
+    for (int i = 0; i < 3; i++) {
+      for (int j = 0; j < 3; j++) R[3 * i + j] = A[3 * i] * An[j] + A[3 * i + 1] * An[3 + j] + A[3 * i + 2] * An[6 + j];
+      rb[i] = A[3 * i] * bn[0] + A[3 * i + 1] * bn[1] + A[3 * i + 2] * bn[2] + b[i];
+    }
+    for (int q = 0; q < 9; q++) A[q] = R[q];
+    for (int p = 0; p < 3; p++) b[p] = rb[p];
+  }
+}
+
+// lane k = grid plane k of the column (nz <= 64): residual row, then both
+// substitution recurrences as wavefront scans in registers, then x += dx
+template <typename T>
+__global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t *__restrict__ cols,
+                                                       const int32_t *__restrict__ ct, const T *__restrict__ cs,
+                                                       const T *__restrict__ Kc, const T *__restrict__ w,
+                                                       const T *__restrict__ Cf, const T *__restrict__ bvec,
+                                                       T *__restrict__ x) {
+  const int64_t c = cols[blockIdx.x];
+  const int i = (int)(c / g.ny), j = (int)(c % g.ny);
+  const int nz = g.nz;
+  const int k = threadIdx.x;
+  const int64_t gi = c * nz + k;
+  T r[3] = {0, 0, 0};
+  T A[9], b[3];
+  if (k < nz) {
+    T out[3];
+    fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
+    for (int p = 0; p < 3; p++) r[p] = bvec[3 * gi + p] - out[p];
+    const T *o = Cf + 27 * gi;
+    for (int q = 0; q < 9; q++) A[q] = o[9 + q];
+    for (int p = 0; p < 3; p++) b[p] = o[3 * p] * r[0] + o[3 * p + 1] * r[1] + o[3 * p + 2] * r[2];
+  } else {
+    for (int q = 0; q < 9; q++) A[q] = 0;
+    for (int p = 0; p < 3; p++) b[p] = 0;
+  }
+  for (int s = 1; s < nz; s <<= 1) scan_round_up<T>(A, b, s, k);
+  if (k < nz) {
+    const T *o = Cf + 27 * gi;
+    T y0 = b[0], y1 = b[1], y2 = b[2];
+    for (int q = 0; q < 9; q++) A[q] = o[18 + q];
+    for (int p = 0; p < 3; p++) b[p] = o[p] * y0 + o[3 + p] * y1 + o[6 + p] * y2;
+  }
+  for (int s = 1; s < nz; s <<= 1) scan_round_down<T>(A, b, s, k, nz);
+  if (k < nz)
+    for (int p = 0; p < 3; p++) x[3 * gi + p] += b[p];
+}
+#endif
+
 template <typename T>
 void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, int32_t ncols, const T *b, T *x, T *r,
                       T *dx, be::stream_t s) {
@@ -635,8 +724,13 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   const int32_t *ct = o.cellType;
   const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w, *F = o.F;
   const int nz = g.nz;
-  {
+  if (!ncols) return;
   KScope ks(M, FCBGA_KCLASS_FINE_SWEEP);
+#if FCBGA_GPU
+  (void)r; (void)dx;
+  hipLaunchKernelGGL(k_fine_col_sweep<T>, dim3(ncols), dim3(64), 0, s, g, cols, ct, cs, Kc, w, F, b, x);
+  BE_CHECK(hipGetLastError());
+#else
   be::pfor((int64_t)ncols * nz, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t / nz];
     int k = (int)(t % nz);
@@ -646,12 +740,12 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
     fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
     for (int p = 0; p < 3; p++) r[3 * gi + p] = b[3 * gi + p] - out[p];
   });
-  }
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t];
-    fe::column_solve<T, T>(nz, F + 18 * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
+    fe::column_solve_scan<T, T>(nz, F + 27 * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
     for (int q = 0; q < 3 * nz; q++) x[3 * c * nz + q] += dx[3 * c * nz + q];
   }, 64);
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -1127,18 +1221,16 @@ void shell_residual(DevShell &D, be::stream_t s) {
   const int32_t *rp = D.rowPtr.p, *col = D.col.p;
   const T *val = LevelVec<T>::val(D), *b = LevelVec<T>::b(D), *x = LevelVec<T>::x(D);
   T *r = LevelVec<T>::r(D);
-  be::pfor(D.nagg, s, [=] __host__ __device__(int64_t n) {
-    T acc[6];
-    for (int p = 0; p < 6; p++) acc[p] = b[6 * n + p];
+  // one thread per scalar row (aggregate n, DOF p)
+  be::pfor(6 * (int64_t)D.nagg, s, [=] __host__ __device__(int64_t t) {
+    int64_t n = t / 6;
+    int p = (int)(t % 6);
+    T acc = b[t];
     for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
-      const T *v = val + 36 * (int64_t)k, *xj = x + 6 * (int64_t)col[k];
-      for (int p = 0; p < 6; p++) {
-        T sum = 0;
-        for (int q = 0; q < 6; q++) sum += v[6 * p + q] * xj[q];
-        acc[p] -= sum;
-      }
+      const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
+      acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
     }
-    for (int p = 0; p < 6; p++) r[6 * n + p] = acc[p];
+    r[t] = acc;
   });
 }
 
@@ -1161,7 +1253,7 @@ void shell_cycle(fcbga_model *M, int l) {
   zero_vec<T>(x, 6 * (int64_t)D.nagg, s);
   {
     KScope ks(M, l == 0 ? FCBGA_KCLASS_SHELL1_SWEEP : FCBGA_KCLASS_SHELL_SWEEP);
-    for (int sw = 0; sw < 2; sw++)
+    for (int sw = 0; sw < M->nsweeps; sw++)
       for (int c = 0; c < 4; c++) shell_sweep<T>(D, c, s);
   }
   shell_residual<T>(D, s);
@@ -1170,36 +1262,34 @@ void shell_cycle(fcbga_model *M, int l) {
     const int32_t *chPtr = C.chPtr.p, *chIdx = C.chIdx.p, *chQ = C.chQ.p;
     const T *W = LevelVec<T>::pW(D), *res = LevelVec<T>::r(D);
     T *bc = LevelVec<T>::b(C);
-    be::pfor(C.nagg, s, [=] __host__ __device__(int64_t I) {
-      T acc[6] = {0, 0, 0, 0, 0, 0};
+    be::pfor(6 * (int64_t)C.nagg, s, [=] __host__ __device__(int64_t t) {
+      int64_t I = t / 6;
+      int a = (int)(t % 6);
+      T acc = 0;
       for (int32_t h = chPtr[I]; h < chPtr[I + 1]; h++) {
-        const T *w = W + 36 * (int64_t)chQ[h], *rf = res + 6 * (int64_t)chIdx[h];
-        for (int a = 0; a < 6; a++) {
-          T sum = 0;
-          for (int p = 0; p < 6; p++) sum += w[6 * p + a] * rf[p];
-          acc[a] += sum;
-        }
+        const T *w = W + 36 * (int64_t)chQ[h] + a, *rf = res + 6 * (int64_t)chIdx[h];
+        acc += w[0] * rf[0] + w[6] * rf[1] + w[12] * rf[2] + w[18] * rf[3] + w[24] * rf[4] + w[30] * rf[5];
       }
-      for (int a = 0; a < 6; a++) bc[6 * I + a] = acc[a];
+      bc[t] = acc;
     });
   }
   shell_cycle<T>(M, l + 1);
   {
     const int32_t *pPtr = D.pPtr.p, *pIdx = D.pIdx.p;
     const T *W = LevelVec<T>::pW(D), *ec = LevelVec<T>::x(C);
-    be::pfor(D.nagg, s, [=] __host__ __device__(int64_t f) {
+    be::pfor(6 * (int64_t)D.nagg, s, [=] __host__ __device__(int64_t t) {
+      int64_t f = t / 6;
+      int p = (int)(t % 6);
+      T acc = 0;
       for (int32_t q = pPtr[f]; q < pPtr[f + 1]; q++) {
-        const T *w = W + 36 * (int64_t)q, *e = ec + 6 * (int64_t)pIdx[q];
-        for (int p = 0; p < 6; p++) {
-          T sum = 0;
-          for (int a = 0; a < 6; a++) sum += w[6 * p + a] * e[a];
-          x[6 * f + p] += sum;
-        }
+        const T *w = W + 36 * (int64_t)q + 6 * p, *e = ec + 6 * (int64_t)pIdx[q];
+        acc += w[0] * e[0] + w[1] * e[1] + w[2] * e[2] + w[3] * e[3] + w[4] * e[4] + w[5] * e[5];
       }
+      x[t] += acc;
     });
   }
   KScope ks(M, l == 0 ? FCBGA_KCLASS_SHELL1_SWEEP : FCBGA_KCLASS_SHELL_SWEEP);
-  for (int sw = 0; sw < 2; sw++)
+  for (int sw = 0; sw < M->nsweeps; sw++)
     for (int c = 3; c >= 0; c--) shell_sweep<T>(D, c, s);
 }
 
@@ -1210,7 +1300,7 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
   FineOps<T> o = fine_ops<T>(M);
   const int64_t n3 = 3 * M->g.G;
   zero_vec<T>(x, n3, s);
-  for (int sw = 0; sw < 2; sw++)
+  for (int sw = 0; sw < M->nsweeps; sw++)
     for (int c = 0; c < 4; c++) fine_color_sweep<T>(M, o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
   if (withCoarse) {
     {
@@ -1248,7 +1338,7 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
       x[3 * gi + 2] += (e[2] + z * e[5]) * sinv[3 * gi + 2];
     });
   }
-  for (int sw = 0; sw < 2; sw++)
+  for (int sw = 0; sw < M->nsweeps; sw++)
     for (int c = 3; c >= 0; c--) fine_color_sweep<T>(M, o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
 }
 
@@ -1258,7 +1348,7 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
 
 double dotd(fcbga_model *M, const double *a, const double *b, int slot) {
   const int64_t n3 = 3 * M->g.G;
-  be::dot_async(n3, a, b, M->partials.p + 1024 * slot, M->scal.p + slot, M->stream);
+  be::dot_async(n3, a, b, M->partials.p, M->scal.p + slot, M->stream);
   double v;
   be::d2h(&v, M->scal.p + slot, sizeof(double), M->stream);
   return v;
@@ -1287,16 +1377,112 @@ void precondition(fcbga_model *M, int precision, int precond) {
   be::pfor(n3, s, [=] __host__ __device__(int64_t i) { if (!mask[i]) z[i] = 0.0; });
 }
 
+// device scalar slots of the PCG
+enum { SC_RR = 0, SC_RZ = 1, SC_PQ = 2, SC_RZO = 3, SC_RZN = 4, SC_ALPHA = 5, SC_BETA = 6, SC_FLAG = 7, SC_COUNT = 8 };
+
+// One PCG iteration (LinearSolver._pcg, src/core/10-solver.js:1021-1040) with
+// every scalar kept on the device: q = A p, alpha = rz / pq, x += alpha p,
+// r -= alpha q, rr = r.r, z = M^-1 r, beta (Fletcher-Reeves, or Polak-Ribiere
+// for the flexible mixed-precision PCG), p = z + beta p.  A non-positive p.Ap
+// sets the flag and leaves x and r unchanged.
+void iteration_body(fcbga_model *M, const fcbga_solve_opts &o, bool flex) {
+  be::stream_t s = M->stream;
+  const int64_t n3 = 3 * M->g.G;
+  FineOps<double> f64 = fine_ops<double>(M);
+  double *x = M->x.p, *r = M->r.p, *z = M->z.p, *p = M->p.p, *q = M->q.p, *zold = M->zold.p, *sc = M->scal.p;
+  double *part = M->partials.p;
+  {
+    KScope ks(M, FCBGA_KCLASS_APPLY);
+    fine_apply<double>(f64, p, q, s);
+  }
+  be::dot_async(n3, p, q, part, sc + SC_PQ, s);
+  be::pfor(1, s, [=] __host__ __device__(int64_t) {
+    double pq = sc[SC_PQ];
+    bool ok = pq > 0;
+    sc[SC_FLAG] = ok ? 0.0 : 1.0;
+    sc[SC_ALPHA] = ok ? sc[SC_RZ] / pq : 0.0;
+  });
+  be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
+    double a = sc[SC_ALPHA];
+    x[i] += a * p[i];
+    r[i] -= a * q[i];
+  });
+  be::dot_async(n3, r, r, part, sc + SC_RR, s);
+  if (flex) be::pfor(n3, s, [=] __host__ __device__(int64_t i) { zold[i] = z[i]; });
+  precondition(M, o.precision, o.precond);
+  be::dot_async(n3, r, z, part, sc + SC_RZN, s);
+  if (flex) be::dot_async(n3, r, zold, part, sc + SC_RZO, s);
+  be::pfor(1, s, [=] __host__ __device__(int64_t) {
+    double rz = sc[SC_RZ], rzn = sc[SC_RZN];
+    double beta = flex ? (rzn - sc[SC_RZO]) / rz : rzn / rz;
+    sc[SC_BETA] = sc[SC_FLAG] != 0 ? 0.0 : beta;
+    if (sc[SC_FLAG] == 0) sc[SC_RZ] = rzn;
+  });
+  be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
+    if (sc[SC_FLAG] == 0) p[i] = z[i] + sc[SC_BETA] * p[i];
+  });
+}
+
+void run_iteration(fcbga_model *M, const fcbga_solve_opts &o, bool flex, int gkey) {
+#if FCBGA_GPU
+  if (M->useGraphs && !M->timing) {
+    auto it = M->graphs.find(gkey);
+    if (it == M->graphs.end()) {
+      hipGraph_t graph;
+      BE_CHECK(hipStreamBeginCapture(M->stream, hipStreamCaptureModeThreadLocal));
+      iteration_body(M, o, flex);
+      BE_CHECK(hipStreamEndCapture(M->stream, &graph));
+      hipGraphExec_t exec;
+      BE_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      it = M->graphs.emplace(gkey, exec).first;
+    }
+    BE_CHECK(hipGraphLaunch(it->second, M->stream));
+    return;
+  }
+#endif
+  (void)gkey;
+  iteration_body(M, o, flex);
+}
+
+void read_scalars(fcbga_model *M, double h[2]) {
+#if FCBGA_GPU
+  if (!M->hostScal) BE_CHECK(hipHostMalloc((void **)&M->hostScal, sizeof(double) * SC_COUNT, hipHostMallocDefault));
+  BE_CHECK(hipMemcpyAsync(M->hostScal, M->scal.p, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, M->stream));
+  BE_CHECK(hipStreamSynchronize(M->stream));
+  h[0] = M->hostScal[SC_RR];
+  h[1] = M->hostScal[SC_FLAG];
+#else
+  h[0] = M->scal.p[SC_RR];
+  h[1] = M->scal.p[SC_FLAG];
+#endif
+}
+
 void alloc_vectors(fcbga_model *M, int precision) {
   const size_t n3 = 3 * (size_t)M->g.G;
+  const void *before[6] = {M->x.p, M->zold.p, M->b32v.p, M->partials.p, M->scal.p, M->dx64.p};
   M->x.alloc(n3); M->r.alloc(n3); M->z.alloc(n3); M->p.alloc(n3); M->q.alloc(n3);
   M->tmp64.alloc(n3); M->dx64.alloc(n3);
   if (precision == FCBGA_PREC_MIXED) {
     M->zold.alloc(n3);
     M->b32v.alloc(n3); M->x32v.alloc(n3); M->r32v.alloc(n3); M->dx32.alloc(n3);
   }
-  M->partials.alloc(1024 * 4);
-  M->scal.alloc(4);
+  M->partials.alloc(1024);
+  const bool fresh = M->scal.n != SC_COUNT;
+  M->scal.alloc(SC_COUNT);
+  if (fresh) M->scal.zero(M->stream);
+  const void *after[6] = {M->x.p, M->zold.p, M->b32v.p, M->partials.p, M->scal.p, M->dx64.p};
+#if FCBGA_GPU
+  // captured iteration graphs hold these addresses: drop them when a buffer moved
+  for (int q = 0; q < 6; q++)
+    if (before[q] && before[q] != after[q]) {
+      for (auto &g : M->graphs) (void)hipGraphExecDestroy(g.second);
+      M->graphs.clear();
+      break;
+    }
+#else
+  (void)before; (void)after;
+#endif
 }
 
 void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op, double *du_dev, double *du_host,
@@ -1307,6 +1493,8 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
   if (!(o.tol > 0) || o.maxit < 1) throw be::Error("tol must be > 0 and maxit >= 1");
   if (o.precision != FCBGA_PREC_FP64 && o.precision != FCBGA_PREC_MIXED) throw be::Error("unknown precision");
   if (o.precond < FCBGA_PC_MG || o.precond > FCBGA_PC_JACOBI) throw be::Error("unknown preconditioner");
+  if (o.sweeps < 0 || o.sweeps > 8) throw be::Error("sweeps must be 0 (default 2) .. 8");
+  M->nsweeps = o.sweeps ? o.sweeps : 2;
   fcbga_solve_stats S{};
   S.ndof = 3 * (int64_t)M->nn;
   S.ntypes = M->ntypes;
@@ -1327,16 +1515,16 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
   }
   const double t2 = now_ms();
   const int64_t n3 = 3 * M->g.G;
-  FineOps<double> f64 = fine_ops<double>(M);
-  double *x = M->x.p, *r = M->r.p, *z = M->z.p, *p = M->p.p, *q = M->q.p, *zold = M->zold.p;
+  double *x = M->x.p, *r = M->r.p, *z = M->z.p, *p = M->p.p;
   const double *bl = M->bload.p, *sv = M->s64.p;
+  double *sc = M->scal.p;
   // scaled load; x = 0, r = b
   be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
     double bs = bl[i] * sv[i];
     r[i] = bs;
     x[i] = 0.0;
   });
-  const double bnorm = std::sqrt(dotd(M, r, r, 0));
+  const double bnorm = std::sqrt(dotd(M, r, r, SC_RR));
   int it = 0;
   bool conv = false;
   double rnorm = bnorm;
@@ -1348,36 +1536,17 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
     if (rnorm / bnorm < o.tol) conv = true;
     else {
       precondition(M, o.precision, o.precond);
-      be::d2d(p, z, sizeof(double) * n3, s);
-      double rz = dotd(M, r, z, 1);
+      be::pfor(n3, s, [=] __host__ __device__(int64_t i) { p[i] = z[i]; });
+      be::dot_async(n3, r, z, M->partials.p, sc + SC_RZ, s);
       const bool flex = o.precision == FCBGA_PREC_MIXED;
+      const int gkey = o.precision * 64 + o.precond * 8 + M->nsweeps;
       for (it = 1; it <= o.maxit; it++) {
-        {
-          KScope ks(M, FCBGA_KCLASS_APPLY);
-          fine_apply<double>(f64, p, q, s);
-        }
-        double pq = dotd(M, p, q, 2);
-        if (!(pq > 0)) break;
-        const double alpha = rz / pq;
-        be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
-          x[i] += alpha * p[i];
-          r[i] -= alpha * q[i];
-        });
-        rnorm = std::sqrt(dotd(M, r, r, 0));
+        run_iteration(M, o, flex, gkey);
+        double h[2];
+        read_scalars(M, h);  // rr, breakdown flag
+        if (h[1] != 0) { it--; break; }  // p.Ap <= 0: stop before the update, as the reference does
+        rnorm = std::sqrt(h[0]);
         if (rnorm / bnorm < o.tol) { conv = true; break; }
-        if (flex) be::d2d(zold, z, sizeof(double) * n3, s);
-        precondition(M, o.precision, o.precond);
-        double rz2 = dotd(M, r, z, 1);
-        double beta;
-        if (flex) {
-          // flexible (Polak-Ribiere) beta for a preconditioner that varies slightly between applications
-          double rzo = dotd(M, r, zold, 3);
-          beta = (rz2 - rzo) / rz;
-        } else {
-          beta = rz2 / rz;
-        }
-        rz = rz2;
-        be::pfor(n3, s, [=] __host__ __device__(int64_t i) { p[i] = z[i] + beta * p[i]; });
       }
       if (it > o.maxit) it = o.maxit;
     }
@@ -1437,6 +1606,8 @@ void fcbga_model_destroy(fcbga_model *m) {
   if (!m) return;
 #if FCBGA_GPU
   if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (auto &g : m->graphs) (void)hipGraphExecDestroy(g.second);
+  if (m->hostScal) (void)hipHostFree(m->hostScal);
   for (hipEvent_t e : m->evPool) (void)hipEventDestroy(e);
   for (auto &u : m->evUsed) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
   hipStream_t s = m->stream;

@@ -424,4 +424,96 @@ FE_HD void element_condense(double hx, double hy, double hz, const double *__res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Column solve as two linear recurrences (the same solve as column_solve):
+//   forward  L y = r :   y_k = P_k y_{k-1} + Li_k r_k,     P_k = -Li_k L_{k,k-1}
+//   backward L^T x = y:  x_k = Q_k x_{k+1} + Li_k^T y_k,   Q_k = -Li_k^T L_{k+1,k}^T
+// with Li_k = L_kk^-1.  Per node 27 coefficients: Li (9), P (9), Q (9).  A
+// recurrence of affine maps is solved by an inclusive scan in log2(nz) rounds.
+// ---------------------------------------------------------------------------
+FE_HD void inv_lower3(const double *L, double *Li) {
+  // inverse of a lower-triangular 3x3 (row-major)
+  double a = L[0], b = L[3], c = L[4], d = L[6], e = L[7], f = L[8];
+  Li[0] = 1 / a; Li[1] = 0; Li[2] = 0;
+  Li[4] = 1 / c; Li[5] = 0;
+  Li[8] = 1 / f;
+  Li[3] = -b / (a * c);
+  Li[7] = -e / (c * f);
+  Li[6] = (b * e - c * d) / (a * c * f);
+}
+
+// coefficients of one column from its factor F (18 per node) into C (27 per node)
+template <typename TC>
+FE_HD void column_scan_coefs(int nz, const double *F, TC *C) {
+  for (int k = 0; k < nz; k++) {
+    double Li[9];
+    inv_lower3(F + 18 * k, Li);
+    TC *o = C + 27 * k;
+    for (int q = 0; q < 9; q++) o[q] = (TC)Li[q];
+    // P_k = -Li_k L_{k,k-1}
+    const double *Ld = F + 18 * k + 9;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        double v = 0;
+        if (k > 0)
+          for (int t = 0; t < 3; t++) v -= Li[3 * r + t] * Ld[3 * t + c];
+        o[9 + 3 * r + c] = (TC)v;
+      }
+    // Q_k = -Li_k^T L_{k+1,k}^T
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        double v = 0;
+        if (k + 1 < nz) {
+          const double *Lu = F + 18 * (k + 1) + 9;
+          for (int t = 0; t < 3; t++) v -= Li[3 * t + r] * Lu[3 * c + t];
+        }
+        o[18 + 3 * r + c] = (TC)v;
+      }
+  }
+}
+
+// Host form of the scan solve (the GPU kernel does the same rounds with lane
+// shuffles): x = (L L^T)^-1 r for one column, nz <= 128.
+template <typename T, typename TC>
+FE_HD void column_solve_scan(int nz, const TC *C, const T *r, T *x) {
+  T A[128][9], b[128][3], A2[128][9], b2[128][3];
+  // forward: maps y_k = P_k y_{k-1} + Li_k r_k
+  for (int k = 0; k < nz; k++) {
+    const TC *o = C + 27 * k;
+    for (int q = 0; q < 9; q++) A[k][q] = (T)o[9 + q];
+    for (int p = 0; p < 3; p++) b[k][p] = (T)o[3 * p] * r[3 * k] + (T)o[3 * p + 1] * r[3 * k + 1] + (T)o[3 * p + 2] * r[3 * k + 2];
+  }
+  for (int s = 1; s < nz; s <<= 1) {
+    for (int k = 0; k < nz; k++) {
+      if (k < s) { for (int q = 0; q < 9; q++) A2[k][q] = A[k][q]; for (int p = 0; p < 3; p++) b2[k][p] = b[k][p]; continue; }
+      const T *Ak = A[k], *Ap = A[k - s], *bp = b[k - s];
+      for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) A2[k][3 * i + j] = Ak[3 * i] * Ap[j] + Ak[3 * i + 1] * Ap[3 + j] + Ak[3 * i + 2] * Ap[6 + j];
+        b2[k][i] = Ak[3 * i] * bp[0] + Ak[3 * i + 1] * bp[1] + Ak[3 * i + 2] * bp[2] + b[k][i];
+      }
+    }
+    for (int k = 0; k < nz; k++) { for (int q = 0; q < 9; q++) A[k][q] = A2[k][q]; for (int p = 0; p < 3; p++) b[k][p] = b2[k][p]; }
+  }
+  // backward: maps x_k = Q_k x_{k+1} + Li_k^T y_k
+  for (int k = 0; k < nz; k++) {
+    const TC *o = C + 27 * k;
+    T y0 = b[k][0], y1 = b[k][1], y2 = b[k][2];
+    for (int q = 0; q < 9; q++) A[k][q] = (T)o[18 + q];
+    for (int p = 0; p < 3; p++) b[k][p] = (T)o[p] * y0 + (T)o[3 + p] * y1 + (T)o[6 + p] * y2;
+  }
+  for (int s = 1; s < nz; s <<= 1) {
+    for (int k = 0; k < nz; k++) {
+      if (k + s >= nz) { for (int q = 0; q < 9; q++) A2[k][q] = A[k][q]; for (int p = 0; p < 3; p++) b2[k][p] = b[k][p]; continue; }
+      const T *Ak = A[k], *An = A[k + s], *bn = b[k + s];
+      for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) A2[k][3 * i + j] = Ak[3 * i] * An[j] + Ak[3 * i + 1] * An[3 + j] + Ak[3 * i + 2] * An[6 + j];
+        b2[k][i] = Ak[3 * i] * bn[0] + Ak[3 * i + 1] * bn[1] + Ak[3 * i + 2] * bn[2] + b[k][i];
+      }
+    }
+    for (int k = 0; k < nz; k++) { for (int q = 0; q < 9; q++) A[k][q] = A2[k][q]; for (int p = 0; p < 3; p++) b[k][p] = b2[k][p]; }
+  }
+  for (int k = 0; k < nz; k++)
+    for (int p = 0; p < 3; p++) x[3 * k + p] = b[k][p];
+}
+
 }  // namespace fe

@@ -42,7 +42,7 @@ class FcbgaStep(C.Structure):
 
 class FcbgaSolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxit", C.c_int32), ("precision", C.c_int32), ("precond", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("sweeps", C.c_int32)]
 
 
 class FcbgaSolveStats(C.Structure):
@@ -175,16 +175,16 @@ class Model:
         self.close()
 
     @staticmethod
-    def _opts(tol, maxit, precision, precond):
+    def _opts(tol, maxit, precision, precond, sweeps=0):
         o = FcbgaSolveOpts()
-        o.tol, o.maxit, o.precision, o.precond, o.reserved = tol, maxit, precision, precond, 0
+        o.tol, o.maxit, o.precision, o.precond, o.sweeps = tol, maxit, precision, precond, sweeps
         return o
 
-    def solve_step(self, step: Step, tol=1e-8, maxit=2000, precision=PREC_FP64, precond=PC_MG, check=True):
+    def solve_step(self, step: Step, tol=1e-8, maxit=2000, precision=PREC_FP64, precond=PC_MG, check=True, sweeps=0):
         """du (3*nn, reference node order) and the solve record."""
         du = np.zeros(self.ndof)
         st = FcbgaSolveStats()
-        o = self._opts(tol, maxit, precision, precond)
+        o = self._opts(tol, maxit, precision, precond, sweeps)
         rc = self.L.fcbga_step_solve(self.h, C.byref(step.c), C.byref(o), du.ctypes.data_as(C.c_void_p),
                                      C.byref(st))
         rec = {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
@@ -193,9 +193,9 @@ class Model:
         return du, rec
 
     def solve_step_device(self, step: Step, du_dev_ptr: int, tol=1e-8, maxit=2000, precision=PREC_FP64,
-                          precond=PC_MG):
+                          precond=PC_MG, sweeps=0):
         st = FcbgaSolveStats()
-        o = self._opts(tol, maxit, precision, precond)
+        o = self._opts(tol, maxit, precision, precond, sweeps)
         rc = self.L.fcbga_step_solve_device(self.h, C.byref(step.c), C.byref(o), C.c_void_p(du_dev_ptr),
                                             C.byref(st))
         if rc != 0:

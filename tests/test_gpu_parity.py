@@ -110,3 +110,19 @@ def test_error_codes(lib):
     # the model stays usable after an error
     u, rec = model.solve_step(st, tol=1e-12)
     assert rec["converged"]
+
+
+def test_graph_replay_matches_eager(lib):
+    # the PCG iterations replay a captured HIP graph; with kernel timing on, the same
+    # kernels run eagerly: both paths must give the same displacements, replay after replay
+    spec = golden_spec("cfg1")
+    m = meshspec.build_mesh(spec)
+    model = hs.Model(m, variant="hip")
+    st = hs.step_for_spec(spec)
+    model.set_kernel_timing(True)
+    u_eager, r_eager = model.solve_step(st, tol=1e-10, precision=hs.PREC_MIXED)
+    model.set_kernel_timing(False)
+    for _ in range(3):
+        u_graph, r_graph = model.solve_step(st, tol=1e-10, precision=hs.PREC_MIXED)
+        assert r_graph["iters"] == r_eager["iters"]
+        assert np.abs(u_graph - u_eager).max() <= 1e-12 * np.abs(u_eager).max()

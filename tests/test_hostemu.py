@@ -61,7 +61,7 @@ def test_load_and_operator_match_oracle(oracle, name):
 def test_small_box_solve(prec, pc):
     spec, m, M, st = _model("small_box")
     uref = golden_f64("small_box_u.f64.gz")
-    u, rec = M.solve_step(st, tol=1e-12 if prec == hs.PREC_FP64 else 1e-10, maxit=3000, precision=prec, precond=pc)
+    u, rec = M.solve_step(st, tol=1e-12, maxit=3000, precision=prec, precond=pc)
     assert rec["converged"]
     assert np.abs(u - uref).max() <= 1e-8 * np.abs(uref).max()
 
