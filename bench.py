@@ -96,6 +96,39 @@ def cpu_baseline(spec, mesh, iters=5):
             "pcg_s_per_iter": per_iter}
 
 
+def anand_throughput(torch, hs, n=115200, reps=20):
+    """Batched Anand update (FP64) on device-resident states: point updates per second."""
+    P = hs.anand_params(5.87e6, 7460.0, 2.0, 0.0942, 58.3, 0.015, 9350.0, 1.5, 45.9, 0.4)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    eps = (torch.randn(n, 6, generator=g, dtype=torch.float64) * 3e-3).cuda()
+    state = torch.zeros(n, 11, dtype=torch.float64)
+    state[:, 6] = 45.9
+    state = state.cuda()
+    trial = torch.empty_like(state)
+    sig = torch.empty(n, 6, dtype=torch.float64, device="cuda")
+    D = torch.empty(n, 36, dtype=torch.float64, device="cuda")
+    dep = torch.empty(n, dtype=torch.float64, device="cuda")
+    L = hs.load("hip")
+    import ctypes as C
+
+    def run():
+        rc = L.fcbga_anand_update_device(C.byref(P), n, 25.0, 45.0, 46500.0, C.c_void_p(eps.data_ptr()),
+                                         C.c_void_p(state.data_ptr()), C.c_void_p(trial.data_ptr()),
+                                         C.c_void_p(sig.data_ptr()), C.c_void_p(D.data_ptr()),
+                                         C.c_void_p(dep.data_ptr()), None)
+        if rc != 0:
+            raise RuntimeError(hs.last_error(L))
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"n_points": n, "ms_per_update": dt * 1e3, "gp_updates_per_s": n / dt, "dtype": "f64",
+            "note": "AnandPoint.step with consistent tangent, one batch per global Newton iteration"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +236,11 @@ def main():
         out["speedup_mixed_vs_fp64"] = out["solve_s_fp64"] / out["solve_s_mixed"]
         out["warpage_rel_err_mixed_vs_fp64_max"] = float(max(werr))
         log(f"FP64 comparator: {out['solve_s_fp64']:.3f} s/step, warpage err max {max(werr):.2e}")
+
+    # Anand material-point update at the bump-array size (20 x 20 bumps x 36 solder hexes x 8 points)
+    out["anand"] = anand_throughput(torch, hs)
+    log(f"Anand: {out['anand']['gp_updates_per_s']:.3e} point updates/s ({out['anand']['n_points']} points, "
+        f"{out['anand']['ms_per_update']:.3f} ms per batch)")
 
     if not args.no_cpu:
         log("CPU oracle sample ...")

@@ -116,6 +116,25 @@ int fcbga_step_solve_device(fcbga_model *m, const fcbga_step *step, const fcbga_
 int fcbga_step_load(fcbga_model *m, const fcbga_step *step, double *f_out);
 int fcbga_step_apply(fcbga_model *m, const fcbga_step *step, const double *x, double *y_out);
 
+/* Anand viscoplastic material points, batched (AnandPoint.step + commit,
+ * src/core/13-fatigue.js:67-143, 229-236).  Per point: mechanical strain eps (6,
+ * engineering shear) at temperature T (C) over dt (s), modulus E(T) (MPa).
+ * state (FCBGA_ANAND_STATE per point: ep[6] tensor shear, s, ebar, W, sigEq,
+ * lastD) is the committed state; its lastD (the Newton warm start) is updated in
+ * place, as the reference's step does.  trial receives the stepped state (copy it
+ * over state to commit), sig the stress (6), D (36, optional) the consistent
+ * tangent, dep (optional) the equivalent plastic increment. */
+#define FCBGA_ANAND_STATE 11
+typedef struct fcbga_anand_params {
+  double A, QR, xi, m, shat, n, h0, a, s0, nu;
+} fcbga_anand_params;
+int fcbga_anand_update(const fcbga_anand_params *params, int64_t n, double T, double dt, double E, const double *eps,
+                       double *state, double *trial, double *sig, double *D, double *dep);
+/* Same on device arrays, on the given HIP stream (NULL: the null stream); complete on return. */
+int fcbga_anand_update_device(const fcbga_anand_params *params, int64_t n, double T, double dt, double E,
+                              const double *eps, double *state, double *trial, double *sig, double *D, double *dep,
+                              void *stream);
+
 /* Measurement hook: with timing enabled, launches of the kernel classes below are
  * bracketed by HIP events on the library's stream and their device time is summed
  * per class (reset by every fcbga_set_kernel_timing call).  The reference has no

@@ -1484,3 +1484,101 @@ int or_warpage_jeita(int32_t nx, int32_t ny, int32_t nz, const double *xs, const
   free(w);
   return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Anand material point (src/core/13-fatigue.js:32-143, 209-218)             */
+/* ------------------------------------------------------------------------- */
+
+static double sgnd(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
+
+/* s_{n+1} for a plastic increment d: Newton with bisection on
+ * phi(s) = s - s_n - h0 |1 - s/s*|^a sign(1 - s/s*) d (src/core/13-fatigue.js:32-52) */
+static double an_s(const double *P, double sn, double d, double dt, double expQ) {
+  const double A = P[0], shat = P[4], n = P[5], h0 = P[6], a = P[7];
+  if (d <= 0) return sn;
+  double sstar = shat * pow(d / (dt * A) * expQ, n);
+  double lo = fmin(sn, sstar), hi = fmax(sn, sstar);
+  if (hi - lo < 1e-14) return sn;
+  double s = 0.5 * (lo + hi);
+  for (int it = 0; it < 60; it++) {
+    double B = 1 - s / sstar;
+    double f = s - sn - h0 * pow(fabs(B), a) * sgnd(B) * d;
+    if (fabs(f) < 1e-11 * fmax(1, sn)) break;
+    if (f > 0) hi = s; else lo = s;
+    double dphi = 1 + h0 * a * pow(fabs(B), a - 1) * d / sstar;
+    double s2 = s - f / dphi;
+    if (!(s2 > lo && s2 < hi)) s2 = 0.5 * (lo + hi);
+    if (fabs(s2 - s) < 1e-15 * fmax(1, s)) { s = s2; break; }
+    s = s2;
+  }
+  return s;
+}
+
+static double an_dsdd(const double *P, double d, double s, double dt, double expQ) {
+  const double A = P[0], shat = P[4], n = P[5], h0 = P[6], a = P[7];
+  if (d <= 0) return 0;
+  double sstar = shat * pow(d / (dt * A) * expQ, n);
+  double B = 1 - s / sstar;
+  double dps = 1 + h0 * a * pow(fabs(B), a - 1) * d / sstar;
+  double dpd = -h0 * pow(fabs(B), a) * sgnd(B) - h0 * a * pow(fabs(B), a - 1) * s * n / sstar;
+  return -dpd / dps;
+}
+
+/* P = {A, QR, xi, m, shat, n, h0, a, s0, nu}; st (11) committed state with the
+ * warm start st[10] updated in place; tr (11) trial state; returns d. */
+double or_anand_step(const double *P, double E, double T, double dt, const double *eps, double *st, double *tr,
+                     double *sig, double *D) {
+  const double A = P[0], QR = P[1], xi = P[2], m = P[3], nu = P[9];
+  double G = E / (2 * (1 + nu)), K = E / (3 * (1 - 2 * nu));
+  double expQ = exp(QR / (T + 273.15)), rate0 = dt * A / expQ;
+  double vol = (eps[0] + eps[1] + eps[2]) / 3, str[6];
+  for (int i = 0; i < 3; i++) str[i] = 2 * G * (eps[i] - vol - st[i]);
+  for (int i = 3; i < 6; i++) str[i] = 2 * G * (0.5 * eps[i] - st[i]);
+  double nrm = sqrt(str[0] * str[0] + str[1] * str[1] + str[2] * str[2] +
+                    2 * (str[3] * str[3] + str[4] * str[4] + str[5] * str[5]));
+  double sigTr = sqrt(1.5) * nrm, d = 0, s1 = st[6], dDdSig = 0;
+  if (sigTr > 1e-14 && dt > 0) {
+    double dmax = sigTr / (3 * G), lo = 0, hi = dmax;
+    double dd = (st[10] > 0 && st[10] < dmax) ? st[10] : fmin(dmax, rate0 * pow(sinh(xi * sigTr / st[6]), 1 / m));
+    if (!isfinite(dd) || dd <= 0) dd = 0.5 * dmax;
+    double rs = st[6], rx = 0, rsh = 0;
+    for (int it = 0; it < 60; it++) {
+      rs = an_s(P, st[6], dd, dt, expQ);
+      rx = xi * (sigTr - 3 * G * dd) / rs;
+      rsh = sinh(rx);
+      double g = dd - rate0 * pow(rsh, 1 / m);
+      if (!isfinite(g)) { lo = dd; dd = 0.5 * (lo + hi); continue; }
+      if (fabs(g) < 1e-11 * fmax(dmax, 1e-30)) break;
+      if (g > 0) hi = dd; else lo = dd;
+      double fx = rate0 / m * pow(rsh, 1 / m - 1) * cosh(rx);
+      double dg = 1 + fx * xi * (3 * G / rs + (sigTr - 3 * G * dd) * an_dsdd(P, dd, rs, dt, expQ) / (rs * rs));
+      double dn = dd - g / dg;
+      if (!(dn > lo && dn < hi) || !isfinite(dn)) dn = 0.5 * (lo + hi);
+      if (fabs(dn - dd) < 1e-16 * fmax(dmax, 1e-30)) { dd = dn; break; }
+      dd = dn;
+    }
+    d = dd; s1 = rs; st[10] = d;
+    double fx = rate0 / m * pow(rsh, 1 / m - 1) * cosh(rx);
+    double dg = 1 + fx * xi * (3 * G / rs + (sigTr - 3 * G * dd) * an_dsdd(P, dd, rs, dt, expQ) / (rs * rs));
+    dDdSig = (fx * xi / rs) / dg;
+  }
+  double theta = sigTr > 1e-14 ? 1 - 3 * G * d / sigTr : 1, nv[6];
+  for (int i = 0; i < 6; i++) nv[i] = nrm > 0 ? str[i] / nrm : 0;
+  for (int i = 0; i < 6; i++) tr[i] = st[i] + sqrt(1.5) * d * nv[i];
+  double sm = 3 * K * vol, sigEq = sigTr - 3 * G * d;
+  for (int i = 0; i < 6; i++) sig[i] = theta * str[i] + (i < 3 ? sm : 0);
+  tr[6] = s1; tr[7] = st[7] + d; tr[8] = st[8] + 0.5 * (st[9] + sigEq) * d; tr[9] = sigEq; tr[10] = st[10];
+  if (D) {
+    double tb = 2 * G * 1.5 * dDdSig - (1 - theta), c1 = 2 * G * theta, c2 = 2 * G * tb;
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++)
+        D[6 * i + j] = -c2 * nv[i] * nv[j] + ((i < 3 && j < 3) ? K + c1 * ((i == j) - 1.0 / 3) : (i == j ? 0.5 * c1 : 0));
+  }
+  return d;
+}
+
+void or_anand_batch(const double *P, int64_t n, double E, double T, double dt, const double *eps, double *st,
+                    double *tr, double *sig, double *D, double *dep) {
+  for (int64_t q = 0; q < n; q++)
+    dep[q] = or_anand_step(P, E, T, dt, eps + 6 * q, st + 11 * q, tr + 11 * q, sig + 6 * q, D ? D + 36 * q : NULL);
+}

@@ -98,6 +98,13 @@ int or_warpage_jeita(int32_t nx, int32_t ny, int32_t nz, const double *xs, const
                      const double *u, const double *bx, const double *by, int32_t nb, double zx0, double zx1,
                      double zy0, double zy1, or_warpage *out);
 
+/* ---- Anand material point (src/core/13-fatigue.js:32-143) ----
+ * P = {A, QR, xi, m, shat, n, h0, a, s0, nu}; state (11): ep[6], s, ebar, W, sigEq, lastD */
+double or_anand_step(const double *P, double E, double T, double dt, const double *eps, double *st, double *tr,
+                     double *sig, double *D);
+void or_anand_batch(const double *P, int64_t n, double E, double T, double dt, const double *eps, double *st,
+                    double *tr, double *sig, double *D, double *dep);
+
 #ifdef __cplusplus
 }
 #endif

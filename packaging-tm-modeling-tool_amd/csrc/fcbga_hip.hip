@@ -1722,6 +1722,61 @@ int fcbga_step_apply(fcbga_model *m, const fcbga_step *step, const double *x, do
   return FCBGA_OK;
 }
 
+int fcbga_anand_update(const fcbga_anand_params *params, int64_t n, double T, double dt, double E, const double *eps,
+                       double *state, double *trial, double *sig, double *D, double *dep) {
+  if (!params || n < 0 || !eps || !state || !trial || !sig) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  if (n == 0) return FCBGA_OK;
+  try {
+    be::stream_t s = nullptr;
+#if FCBGA_GPU
+    BE_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+#endif
+    DBuf<double> de, dst, dtr, dsig, dD, ddep;
+    de.upload(eps, 6 * (size_t)n, s);
+    dst.upload(state, FCBGA_ANAND_STATE * (size_t)n, s);
+    dtr.alloc(FCBGA_ANAND_STATE * (size_t)n);
+    dsig.alloc(6 * (size_t)n);
+    if (D) dD.alloc(36 * (size_t)n);
+    ddep.alloc((size_t)n);
+    int rc = fcbga_anand_update_device(params, n, T, dt, E, de.p, dst.p, dtr.p, dsig.p, D ? dD.p : nullptr, ddep.p,
+                                       (void *)s);
+    if (rc != FCBGA_OK) return rc;
+    be::d2h(state, dst.p, sizeof(double) * FCBGA_ANAND_STATE * n, s);
+    be::d2h(trial, dtr.p, sizeof(double) * FCBGA_ANAND_STATE * n, s);
+    be::d2h(sig, dsig.p, sizeof(double) * 6 * n, s);
+    if (D) be::d2h(D, dD.p, sizeof(double) * 36 * n, s);
+    if (dep) be::d2h(dep, ddep.p, sizeof(double) * n, s);
+#if FCBGA_GPU
+    (void)hipStreamDestroy(s);
+#endif
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_anand_update_device(const fcbga_anand_params *params, int64_t n, double T, double dt, double E,
+                              const double *eps, double *state, double *trial, double *sig, double *D, double *dep,
+                              void *stream) {
+  if (!params || n < 0 || !eps || !state || !trial || !sig) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  if (!(E > 0) || !(dt >= 0) || !(params->nu < 0.5)) { g_err = "E > 0, dt >= 0 and nu < 0.5 required"; return FCBGA_ERR_ARG; }
+  try {
+    const fe::AnandParams P{params->A, params->QR, params->xi, params->m, params->shat, params->n, params->h0,
+                            params->a, params->s0, params->nu};
+    be::pfor(n, (be::stream_t)stream, [=] __host__ __device__(int64_t q) {
+      double sg[6];
+      double dd = fe::anand_step(P, E, T, dt, eps + 6 * q, state + FCBGA_ANAND_STATE * q,
+                                 trial + FCBGA_ANAND_STATE * q, sg, D ? D + 36 * q : nullptr);
+      for (int i = 0; i < 6; i++) sig[6 * q + i] = sg[i];
+      if (dep) dep[q] = dd;
+    }, 64);
+    be::sync((be::stream_t)stream);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
 int fcbga_set_kernel_timing(fcbga_model *m, int enable) {
   if (!m) { g_err = "model is NULL"; return FCBGA_ERR_ARG; }
   m->timing = enable != 0;

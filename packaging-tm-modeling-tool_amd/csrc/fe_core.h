@@ -516,4 +516,134 @@ FE_HD void column_solve_scan(int nz, const TC *C, const T *r, T *x) {
     for (int p = 0; p < 3; p++) x[3 * k + p] = b[k][p];
 }
 
+// ---------------------------------------------------------------------------
+// Anand viscoplastic material point, backward Euler with J2 radial return
+// (AnandPoint.step, src/core/13-fatigue.js:32-143, 209-218):
+//   ep_dot = A sinh(xi sigma / s)^(1/m) exp(-Q/RT),  s_dot = h0 |B|^a sign(B) ep_dot,
+//   B = 1 - s/s*,  s* = s_hat [(ep_dot/A) exp(Q/RT)]^n.
+// Outer Newton (with bisection safeguard) on the equivalent plastic increment d,
+// inner Newton/bisection for s_{n+1}(d); consistent tangent
+//   D = K 1x1 + 2G theta I_dev - 2G theta_bar n x n  (Voigt, engineering shear).
+// State per point (11): ep[6] (tensor shear), s, ebar, W, sigEq, lastD.
+// ---------------------------------------------------------------------------
+struct AnandParams {
+  double A, QR, xi, m, shat, n, h0, a, s0, nu;
+};
+
+constexpr int ANAND_MAXIT = 60;       // src/core/00-constants.js:123
+constexpr double ANAND_TOL = 1e-11;   // src/core/00-constants.js:124
+constexpr double KELVIN = 273.15;
+
+FE_HD double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
+
+FE_HD double anand_solve_s(const AnandParams &P, double sn, double d, double dt, double expQ) {
+  if (d <= 0) return sn;
+  const double sstar = P.shat * pow(d / (dt * P.A) * expQ, P.n);
+  double lo = fmin(sn, sstar), hi = fmax(sn, sstar);
+  if (hi - lo < 1e-14) return sn;
+  double s = 0.5 * (lo + hi);
+  for (int it = 0; it < ANAND_MAXIT; it++) {
+    const double B0 = 1 - s / sstar;
+    const double f = s - sn - P.h0 * pow(fabs(B0), P.a) * sgn(B0) * d;
+    if (fabs(f) < ANAND_TOL * fmax(1.0, sn)) break;
+    if (f > 0) hi = s; else lo = s;
+    const double dphi = 1 + P.h0 * P.a * pow(fabs(B0), P.a - 1) * d / sstar;
+    double s1 = s - f / dphi;
+    if (!(s1 > lo && s1 < hi)) s1 = 0.5 * (lo + hi);
+    if (fabs(s1 - s) < 1e-15 * fmax(1.0, s)) { s = s1; break; }
+    s = s1;
+  }
+  return s;
+}
+
+FE_HD double anand_dsdd(const AnandParams &P, double d, double s, double dt, double expQ) {
+  if (d <= 0) return 0;
+  const double sstar = P.shat * pow(d / (dt * P.A) * expQ, P.n);
+  const double B = 1 - s / sstar, absB = fabs(B), sg = sgn(B);
+  const double dphi_ds = 1 + P.h0 * P.a * pow(absB, P.a - 1) * d / sstar;
+  const double dphi_dd = -P.h0 * pow(absB, P.a) * sg - P.h0 * P.a * pow(absB, P.a - 1) * s * P.n / sstar;
+  return -dphi_dd / dphi_ds;
+}
+
+// One trial step from the committed state st (st[10], the warm start, is
+// updated in place as the reference does); the trial state goes to out (11).
+// Returns the equivalent plastic strain increment.
+FE_HD double anand_step(const AnandParams &P, double E, double T, double dt, const double eps[6], double *st,
+                        double *out, double sig[6], double *D) {
+  const double G = E / (2 * (1 + P.nu)), K = E / (3 * (1 - 2 * P.nu));
+  const double expQ = exp(P.QR / (T + KELVIN));
+  const double rate0 = dt * P.A / expQ;
+  const double vol = (eps[0] + eps[1] + eps[2]) / 3;
+  double e[6], str[6];
+  e[0] = eps[0] - vol - st[0]; e[1] = eps[1] - vol - st[1]; e[2] = eps[2] - vol - st[2];
+  e[3] = 0.5 * eps[3] - st[3]; e[4] = 0.5 * eps[4] - st[4]; e[5] = 0.5 * eps[5] - st[5];
+  for (int i = 0; i < 6; i++) str[i] = 2 * G * e[i];
+  const double nrm = sqrt(str[0] * str[0] + str[1] * str[1] + str[2] * str[2] +
+                          2 * (str[3] * str[3] + str[4] * str[4] + str[5] * str[5]));
+  const double sigTr = sqrt(1.5) * nrm;
+  const double sn = st[6];
+  double d = 0, s1 = sn, dDdSig = 0;
+  if (sigTr > 1e-14 && dt > 0) {
+    const double dmax = sigTr / (3 * G);
+    double lo = 0, hi = dmax;
+    const double lastD = st[10];
+    double dd = (lastD > 0 && lastD < dmax) ? lastD : fmin(dmax, rate0 * pow(sinh(P.xi * sigTr / sn), 1 / P.m));
+    if (!isfinite(dd) || dd <= 0) dd = 0.5 * dmax;
+    double rs = sn, rx = 0, rsh = 0, rg = 0;
+    for (int it = 0; it < ANAND_MAXIT; it++) {
+      rs = anand_solve_s(P, sn, dd, dt, expQ);
+      rx = P.xi * (sigTr - 3 * G * dd) / rs;
+      rsh = sinh(rx);
+      rg = dd - rate0 * pow(rsh, 1 / P.m);
+      if (!isfinite(rg)) { lo = dd; dd = 0.5 * (lo + hi); continue; }
+      if (fabs(rg) < ANAND_TOL * fmax(dmax, 1e-30)) break;
+      if (rg > 0) hi = dd; else lo = dd;
+      const double fx = rate0 * (1 / P.m) * pow(rsh, 1 / P.m - 1) * cosh(rx);
+      const double dsdd = anand_dsdd(P, dd, rs, dt, expQ);
+      const double sg = sigTr - 3 * G * dd;
+      const double dg = 1 + fx * P.xi * (3 * G / rs + sg * dsdd / (rs * rs));
+      double dn = dd - rg / dg;
+      if (!(dn > lo && dn < hi) || !isfinite(dn)) dn = 0.5 * (lo + hi);
+      if (fabs(dn - dd) < 1e-16 * fmax(dmax, 1e-30)) { dd = dn; break; }
+      dd = dn;
+    }
+    d = dd;
+    s1 = rs;
+    st[10] = d;
+    if (D) {
+      const double fx = rate0 * (1 / P.m) * pow(rsh, 1 / P.m - 1) * cosh(rx);
+      const double dsdd = anand_dsdd(P, dd, rs, dt, expQ);
+      const double sg = sigTr - 3 * G * dd;
+      const double dg = 1 + fx * P.xi * (3 * G / rs + sg * dsdd / (rs * rs));
+      dDdSig = (fx * P.xi / rs) / dg;
+    }
+  }
+  const double theta = sigTr > 1e-14 ? 1 - 3 * G * d / sigTr : 1;
+  double nv[6];
+  for (int i = 0; i < 6; i++) nv[i] = nrm > 0 ? str[i] / nrm : 0.0;
+  const double dgm = sqrt(1.5) * d;
+  for (int i = 0; i < 6; i++) out[i] = st[i] + dgm * nv[i];
+  const double sm = 3 * K * vol;
+  sig[0] = theta * str[0] + sm; sig[1] = theta * str[1] + sm; sig[2] = theta * str[2] + sm;
+  sig[3] = theta * str[3]; sig[4] = theta * str[4]; sig[5] = theta * str[5];
+  const double sigEq = sigTr - 3 * G * d;
+  out[6] = s1;
+  out[7] = st[7] + d;
+  out[8] = st[8] + 0.5 * (st[9] + sigEq) * d;
+  out[9] = sigEq;
+  out[10] = st[10];
+  if (D) {
+    const double thetaBar = 2 * G * 1.5 * dDdSig - (1 - theta);
+    const double c1 = 2 * G * theta, c2 = 2 * G * thetaBar;
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++) {
+        double v = -c2 * nv[i] * nv[j];
+        if (i < 3 && j < 3) v += K + c1 * ((i == j ? 1.0 : 0.0) - 1.0 / 3.0);
+        else if (i == j) v += 0.5 * c1;
+        D[i * 6 + j] = v;
+      }
+  }
+  return d;
+}
+
 }  // namespace fe

@@ -54,7 +54,13 @@ class FcbgaSolveStats(C.Structure):
 
 EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_step_solve",
            "fcbga_step_solve_device", "fcbga_step_load", "fcbga_step_apply", "fcbga_set_kernel_timing",
-           "fcbga_kernel_times", "fcbga_last_error"]
+           "fcbga_kernel_times", "fcbga_anand_update", "fcbga_anand_update_device", "fcbga_last_error"]
+
+ANAND_STATE = 11
+
+
+class FcbgaAnandParams(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0", "nu")]
 
 # kernel classes of fcbga_kernel_times (include/fcbga_hip.h)
 KCLASS = {"apply": 0, "fine_sweep": 1, "fine_resid": 2, "shell1_sweep": 3, "shell_sweep": 4, "coarsest": 5}
@@ -91,6 +97,13 @@ def load(variant: str = "hip"):
     L.fcbga_set_kernel_timing.restype = C.c_int
     L.fcbga_kernel_times.argtypes = [C.c_void_p, _f64p, _i32p]
     L.fcbga_kernel_times.restype = C.c_int
+    L.fcbga_anand_update.argtypes = [C.POINTER(FcbgaAnandParams), C.c_int64, C.c_double, C.c_double, C.c_double,
+                                     _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.fcbga_anand_update.restype = C.c_int
+    L.fcbga_anand_update_device.argtypes = [C.POINTER(FcbgaAnandParams), C.c_int64, C.c_double, C.c_double,
+                                            C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]
+    L.fcbga_anand_update_device.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.fcbga_last_error.restype = C.c_int
     _loaded[variant] = L
@@ -235,3 +248,36 @@ def step_for_spec(spec, Ta=None, Tb=None, stage=None):
     stage = spec.get("stage", 0) if stage is None else stage
     D, s0, born = group_step_data(spec, Ta, Tb, stage)
     return Step(Ta, Tb, stage, D, s0, born)
+
+
+def anand_params(A, QR, xi, m, shat, n, h0, a, s0, nu):
+    p = FcbgaAnandParams()
+    for k, v in dict(A=A, QR=QR, xi=xi, m=m, shat=shat, n=n, h0=h0, a=a, s0=s0, nu=nu).items():
+        setattr(p, k, float(v))
+    return p
+
+
+def anand_initial_state(params, npts):
+    """Committed state of fresh points: zero plastic strain, s = s0 (AnandPoint.reset)."""
+    st = np.zeros((npts, ANAND_STATE))
+    st[:, 6] = params.s0
+    return st
+
+
+def anand_update(params, eps, T, dt, E, state, tangent=True, variant="hip"):
+    """Batched AnandPoint.step: returns (trial state, sig, D or None, dep); the
+    warm-start entry state[:, 10] is updated in place."""
+    L = load(variant)
+    eps = np.ascontiguousarray(eps, np.float64).reshape(-1, 6)
+    n = eps.shape[0]
+    assert state.shape == (n, ANAND_STATE) and state.dtype == np.float64 and state.flags.c_contiguous
+    trial = np.zeros((n, ANAND_STATE))
+    sig = np.zeros((n, 6))
+    D = np.zeros((n, 36)) if tangent else None
+    dep = np.zeros(n)
+    rc = L.fcbga_anand_update(C.byref(params), n, float(T), float(dt), float(E), eps.ctypes.data_as(_f64p),
+                              state.ctypes.data_as(_f64p), trial.ctypes.data_as(_f64p), sig.ctypes.data_as(_f64p),
+                              D.ctypes.data_as(_f64p) if D is not None else None, dep.ctypes.data_as(_f64p))
+    if rc != 0:
+        raise SolverError(last_error(L))
+    return trial, sig, D, dep

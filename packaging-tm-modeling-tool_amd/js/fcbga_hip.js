@@ -1,0 +1,77 @@
+'use strict';
+// JS host side of the MI355X step solve: loads the N-API addon (fcbga_hip.node)
+// and offers the reference's call surface for one thermal-path step.
+//
+//   const fh = require('./fcbga_hip.js');
+//   const solver = new fh.HipStepSolver(mesh, {precision: 'mixed'});
+//   const rec = solver.solveStep(fh.stepFromGroups(Ta, Tb, stage, groups), label);
+//   // rec.u = du of the step, plus the LinearSolver.solve record fields
+//
+// install(Analysis) replaces Analysis.prototype.solveStep (src/core/11-analysis.js:189-206)
+// so that assembleStep + LinearSolver.solve run on the GPU: the per-group step
+// data come from FEModel.prepareStep (src/core/09-assembly.js:346-365), and a
+// solve that does not converge throws the reference's message.  The desktop
+// compute worker installs it in boot() when the addon loads; without it the
+// JS path runs unchanged.
+const path = require('path');
+
+let addon = null;
+function load() {
+  if (!addon) addon = require(path.join(__dirname, '..', 'fcbga_hip.node'));
+  return addon;
+}
+
+/** Step data from per-group {D (36), s0 (6), born} records (the fields prepareStep sets: _D, _s0, _born). */
+function stepFromGroups(Ta, Tb, stage, groups) {
+  const ng = groups.length;
+  const D = new Float64Array(ng * 36), s0 = new Float64Array(ng * 6), born = new Uint8Array(ng);
+  groups.forEach((g, q) => {
+    if (g.kind === 'layered') throw new Error('layered groups are not supported by the GPU step solve');
+    D.set(g._D || g.D, 36 * q);
+    s0.set(g._s0 || g.s0, 6 * q);
+    born[q] = (g._born !== undefined ? g._born : g.born) ? 1 : 0;
+  });
+  return { Ta, Tb, stage, D, s0, born };
+}
+
+class HipStepSolver {
+  constructor(mesh, opts) {
+    this.hip = load();
+    this.model = this.hip.createModel(mesh);
+    this.opts = Object.assign({ tol: 1e-8, maxit: 2000, precision: 'mixed', precond: 'mg' }, opts || {});
+    this.log = [];
+  }
+  solveStep(step, label) {
+    const rec = this.hip.solveStep(this.model, step, this.opts);
+    rec.label = label;
+    this.log.push({ label, iters: rec.iters, relres: rec.relres, converged: rec.converged, ms: rec.ms, pcMs: rec.pcMs, precond: rec.precond });
+    return rec;
+  }
+  destroy() {
+    if (this.model) this.hip.destroyModel(this.model);
+    this.model = null;
+  }
+}
+
+function install(Analysis, opts) {
+  const original = Analysis.prototype.solveStep;
+  Analysis.prototype.solveStep = function (k) {
+    if (this.model && this.model.sections && this.model.sections.length) return original.call(this, k);
+    if (typeof this.checkCancel === 'function') this.checkCancel();
+    if (!this._hip) {
+      const mesh = Object.assign({}, this.mesh, { ngroups: this.model.groups.length });
+      this._hip = new HipStepSolver(mesh, Object.assign({ tol: this.solver.opts.tol, maxit: this.solver.opts.maxit }, opts));
+    }
+    const s = this.path.steps[k];
+    this.model.prepareStep(s.Ta, s.Tb, s.stage);
+    const label = 'step ' + k + ' (' + s.Ta + ' -> ' + s.Tb + ' C)';
+    const res = this._hip.solveStep(stepFromGroups(s.Ta, s.Tb, s.stage, this.model.groups), label);
+    res.T = s.Tb; res.stage = s.stage; res.step = k;
+    if (this.timings) this.timings.solve = (this.timings.solve || 0) + res.ms;
+    if (!res.converged) throw new Error('Linear solver did not converge (' + label + '): relative residual ' + res.relres.toExponential(2));
+    return res;
+  };
+  return original;
+}
+
+module.exports = { load, HipStepSolver, stepFromGroups, install };

@@ -1,0 +1,264 @@
+// fcbga_napi.cc -- Node-API addon over the C ABI of include/fcbga_hip.h.
+//
+// JS surface (the reference's step-solve seam, src/core/11-analysis.js:189-206):
+//   const hip = require('fcbga_hip.node');
+//   const model = hip.createModel(mesh);        // FEModel's mesh contract (src/core/06-mesher.js:66-71)
+//   const rec = hip.solveStep(model, step, opts);
+//     step: {Ta, Tb, stage, D: Float64Array(ng*36), s0: Float64Array(ng*6), born: Uint8Array(ng)}
+//     opts: {tol, maxit, precision: 'mixed'|'fp64', precond: 'mg'|'column'|'jacobi', sweeps}
+//     rec:  {u: Float64Array(ndof), iters, relres, converged, ms, assembleMs, pcMs, pcgMs, levels,
+//            precond, rebuilt, warm}  -- the fields of LinearSolver.solve's record (src/core/10-solver.js:988-993)
+//   hip.destroyModel(model);                   // or left to the garbage collector
+// Argument and device errors throw Error(message); a solve that does not
+// converge returns its record with converged = false (the caller throws, as
+// Analysis.solveStep does).  The addon is context-aware (napi_register_module_v1) and
+// keeps no global state: it can load in the compute worker's worker_thread.
+#include <node_api.h>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/fcbga_hip.h"
+
+namespace {
+
+#define NAPI_OK(call)                                          \
+  do {                                                         \
+    if ((call) != napi_ok) {                                   \
+      napi_throw_error(env, nullptr, "N-API call failed: " #call); \
+      return nullptr;                                          \
+    }                                                          \
+  } while (0)
+
+struct ModelBox {
+  fcbga_model *m = nullptr;
+};
+
+void finalize_model(napi_env, void *data, void *) {
+  ModelBox *b = static_cast<ModelBox *>(data);
+  if (b->m) fcbga_model_destroy(b->m);
+  delete b;
+}
+
+bool get_prop(napi_env env, napi_value obj, const char *name, napi_value *out) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return false;
+  if (napi_get_named_property(env, obj, name, out) != napi_ok) return false;
+  napi_valuetype t;
+  napi_typeof(env, *out, &t);
+  return t != napi_undefined && t != napi_null;
+}
+
+bool get_int(napi_env env, napi_value obj, const char *name, int32_t *v) {
+  napi_value x;
+  if (!get_prop(env, obj, name, &x)) return false;
+  return napi_get_value_int32(env, x, v) == napi_ok;
+}
+
+bool get_double(napi_env env, napi_value obj, const char *name, double *v) {
+  napi_value x;
+  if (!get_prop(env, obj, name, &x)) return false;
+  return napi_get_value_double(env, x, v) == napi_ok;
+}
+
+// typed array of the expected element type; returns its data and length
+bool get_typed(napi_env env, napi_value obj, const char *name, napi_typedarray_type want, void **data, size_t *len) {
+  napi_value x;
+  if (!get_prop(env, obj, name, &x)) return false;
+  bool is = false;
+  napi_is_typedarray(env, x, &is);
+  if (!is) return false;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, x, &t, len, data, &ab, &off) != napi_ok) return false;
+  return t == want;
+}
+
+std::string lib_error() {
+  char buf[512];
+  fcbga_last_error(buf, sizeof buf);
+  return buf;
+}
+
+napi_value throw_msg(napi_env env, const std::string &m) {
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+
+napi_value CreateModel(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 1) return throw_msg(env, "createModel(mesh): mesh missing");
+  napi_value mesh = argv[0];
+  fcbga_mesh m;
+  memset(&m, 0, sizeof m);
+  size_t n;
+  void *p;
+  if (!get_int(env, mesh, "nx", &m.nx) || !get_int(env, mesh, "ny", &m.ny) || !get_int(env, mesh, "nz", &m.nz) ||
+      !get_int(env, mesh, "nn", &m.nn) || !get_int(env, mesh, "ne", &m.ne))
+    return throw_msg(env, "mesh: nx, ny, nz, nn, ne required");
+  if (!get_typed(env, mesh, "xs", napi_float64_array, &p, &n) || (int)n != m.nx) return throw_msg(env, "mesh.xs: Float64Array(nx)");
+  m.xs = (const double *)p;
+  if (!get_typed(env, mesh, "ys", napi_float64_array, &p, &n) || (int)n != m.ny) return throw_msg(env, "mesh.ys: Float64Array(ny)");
+  m.ys = (const double *)p;
+  if (!get_typed(env, mesh, "zs", napi_float64_array, &p, &n) || (int)n != m.nz) return throw_msg(env, "mesh.zs: Float64Array(nz)");
+  m.zs = (const double *)p;
+  const size_t G = (size_t)m.nx * m.ny * m.nz, ncells = (size_t)(m.nx - 1) * (m.ny - 1) * (m.nz - 1);
+  if (!get_typed(env, mesh, "nodeId", napi_int32_array, &p, &n) || n != G) return throw_msg(env, "mesh.nodeId: Int32Array(nx*ny*nz)");
+  m.node_id = (const int32_t *)p;
+  if (!get_typed(env, mesh, "cellElems", napi_int32_array, &p, &n) || n != 2 * ncells)
+    return throw_msg(env, "mesh.cellElems: Int32Array(2*ncells)");
+  m.cell_elems = (const int32_t *)p;
+  if (!get_typed(env, mesh, "eGroup", napi_int32_array, &p, &n) || (int)n != m.ne) return throw_msg(env, "mesh.eGroup: Int32Array(ne)");
+  m.e_group = (const int32_t *)p;
+  if (!get_typed(env, mesh, "ePart", napi_int8_array, &p, &n) || (int)n != m.ne) return throw_msg(env, "mesh.ePart: Int8Array(ne)");
+  m.e_part = (const int8_t *)p;
+  if (!get_typed(env, mesh, "eScale", napi_float64_array, &p, &n) || (int)n != m.ne) return throw_msg(env, "mesh.eScale: Float64Array(ne)");
+  m.e_scale = (const double *)p;
+  napi_value groups;
+  uint32_t ng = 0;
+  if (get_prop(env, mesh, "groups", &groups)) napi_get_array_length(env, groups, &ng);
+  int32_t ngi = 0;
+  if (get_int(env, mesh, "ngroups", &ngi)) ng = (uint32_t)ngi;
+  m.ngroups = (int32_t)ng;
+  napi_value cons;
+  if (get_prop(env, mesh, "constraints", &cons)) {
+    if (!get_typed(env, cons, "dofs", napi_int32_array, &p, &n)) return throw_msg(env, "mesh.constraints.dofs: Int32Array");
+    m.constraint_dofs = (const int32_t *)p;
+    m.nconstraints = (int32_t)n;
+  }
+  if (get_typed(env, mesh, "featureX", napi_int32_array, &p, &n)) { m.feature_x = (const int32_t *)p; m.nfeature_x = (int32_t)n; }
+  if (get_typed(env, mesh, "featureY", napi_int32_array, &p, &n)) { m.feature_y = (const int32_t *)p; m.nfeature_y = (int32_t)n; }
+  fcbga_model *model = nullptr;
+  if (fcbga_model_create(&m, &model) != FCBGA_OK) return throw_msg(env, "createModel: " + lib_error());
+  ModelBox *box = new ModelBox();
+  box->m = model;
+  napi_value ext;
+  if (napi_create_external(env, box, finalize_model, nullptr, &ext) != napi_ok) {
+    fcbga_model_destroy(model);
+    delete box;
+    return throw_msg(env, "createModel: could not wrap the model");
+  }
+  return ext;
+}
+
+ModelBox *unwrap(napi_env env, napi_value v) {
+  void *data = nullptr;
+  if (napi_get_value_external(env, v, &data) != napi_ok) return nullptr;
+  return static_cast<ModelBox *>(data);
+}
+
+napi_value DestroyModel(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  ModelBox *b = argc ? unwrap(env, argv[0]) : nullptr;
+  if (!b) return throw_msg(env, "destroyModel(model): not a model");
+  if (b->m) fcbga_model_destroy(b->m);
+  b->m = nullptr;
+  return nullptr;
+}
+
+void set_num(napi_env env, napi_value obj, const char *k, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, obj, k, x);
+}
+
+napi_value SolveStep(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) return throw_msg(env, "solveStep(model, step, opts)");
+  ModelBox *b = unwrap(env, argv[0]);
+  if (!b || !b->m) return throw_msg(env, "solveStep: not a live model");
+  fcbga_step st;
+  memset(&st, 0, sizeof st);
+  size_t n;
+  void *p;
+  get_double(env, argv[1], "Ta", &st.Ta);
+  get_double(env, argv[1], "Tb", &st.Tb);
+  get_int(env, argv[1], "stage", &st.stage);
+  if (!get_typed(env, argv[1], "D", napi_float64_array, &p, &n)) return throw_msg(env, "step.D: Float64Array(ngroups*36)");
+  st.D = (const double *)p;
+  if (!get_typed(env, argv[1], "s0", napi_float64_array, &p, &n)) return throw_msg(env, "step.s0: Float64Array(ngroups*6)");
+  st.s0 = (const double *)p;
+  if (!get_typed(env, argv[1], "born", napi_uint8_array, &p, &n)) return throw_msg(env, "step.born: Uint8Array(ngroups)");
+  st.born = (const uint8_t *)p;
+  fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
+  if (argc >= 3) {
+    napi_valuetype t;
+    napi_typeof(env, argv[2], &t);
+    if (t == napi_object) {
+      get_double(env, argv[2], "tol", &o.tol);
+      get_int(env, argv[2], "maxit", &o.maxit);
+      get_int(env, argv[2], "sweeps", &o.sweeps);
+      napi_value v;
+      char buf[32];
+      size_t len;
+      if (get_prop(env, argv[2], "precision", &v) &&
+          napi_get_value_string_utf8(env, v, buf, sizeof buf, &len) == napi_ok)
+        o.precision = strcmp(buf, "fp64") == 0 ? FCBGA_PREC_FP64 : FCBGA_PREC_MIXED;
+      if (get_prop(env, argv[2], "precond", &v) && napi_get_value_string_utf8(env, v, buf, sizeof buf, &len) == napi_ok)
+        o.precond = strcmp(buf, "column") == 0 ? FCBGA_PC_COLUMN : (strcmp(buf, "jacobi") == 0 ? FCBGA_PC_JACOBI : FCBGA_PC_MG);
+    }
+  }
+  const int64_t ndof = fcbga_model_ndof(b->m);
+  napi_value ab;
+  void *udata = nullptr;
+  NAPI_OK(napi_create_arraybuffer(env, sizeof(double) * (size_t)ndof, &udata, &ab));
+  fcbga_solve_stats S;
+  memset(&S, 0, sizeof S);
+  int rc = fcbga_step_solve(b->m, &st, &o, (double *)udata, &S);
+  if (rc != FCBGA_OK && rc != FCBGA_ERR_NOT_CONVERGED) return throw_msg(env, "solveStep: " + lib_error());
+  napi_value u, rec;
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)ndof, ab, 0, &u));
+  NAPI_OK(napi_create_object(env, &rec));
+  napi_set_named_property(env, rec, "u", u);
+  set_num(env, rec, "iters", S.iters);
+  set_num(env, rec, "relres", S.relres);
+  napi_value cv;
+  napi_get_boolean(env, S.converged != 0, &cv);
+  napi_set_named_property(env, rec, "converged", cv);
+  set_num(env, rec, "ms", S.ms_total);
+  set_num(env, rec, "assembleMs", S.ms_assemble);
+  set_num(env, rec, "pcMs", S.ms_setup);
+  set_num(env, rec, "pcgMs", S.ms_pcg);
+  set_num(env, rec, "levels", S.levels);
+  set_num(env, rec, "iters0", S.r0);
+  set_num(env, rec, "warm", 0);
+  napi_value t;
+  napi_get_boolean(env, true, &t);
+  napi_set_named_property(env, rec, "rebuilt", t);
+  napi_value pc;
+  napi_create_string_utf8(env, o.precond == FCBGA_PC_MG ? "mg" : (o.precond == FCBGA_PC_COLUMN ? "column" : "jacobi"),
+                          NAPI_AUTO_LENGTH, &pc);
+  napi_set_named_property(env, rec, "precond", pc);
+  return rec;
+}
+
+}  // namespace
+
+static napi_value Init(napi_env env, napi_value exports) {
+  struct { const char *name; napi_callback cb; } fns[] = {
+      {"createModel", CreateModel}, {"destroyModel", DestroyModel}, {"solveStep", SolveStep}};
+  for (auto &f : fns) {
+    napi_value fn;
+    if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||
+        napi_set_named_property(env, exports, f.name, fn) != napi_ok) {
+      napi_throw_error(env, nullptr, "fcbga_hip: could not register the addon functions");
+      return nullptr;
+    }
+  }
+  return exports;
+}
+
+// context-aware registration: Node calls this once per environment (main thread or worker_thread)
+extern "C" NAPI_MODULE_EXPORT napi_value napi_register_module_v1(napi_env env, napi_value exports) {
+  return Init(env, exports);
+}

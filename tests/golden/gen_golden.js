@@ -130,6 +130,46 @@ function elementCases() {
   console.log('element cases', cases.length);
 }
 
+// Anand material point (src/core/13-fatigue.js:13-237) with the SAC305 constants of the
+// viscoplastic cycling workload (s0 45.9 MPa, Q/R 7460 K, A 5.87e6 1/s, xi 2, m 0.0942,
+// h0 9350 MPa, s_hat 58.3 MPa, n 0.015, a 1.5) and the SAC305 elastic table.
+function anandCases() {
+  const AnandPoint = get('AnandPoint'), anandSaturationStress = get('anandSaturationStress'), anandUniaxial = get('anandUniaxial');
+  const mat = {
+    id: 'sac305_cycling', name: 'SAC305 (cycling constants)',
+    elastic: { type: 'iso', E: { type: 'table', T: [-40, 25, 75, 125], v: [57.4, 46.5, 38.2, 30.0], interp: 'linear', extrap: 'linear', u: 'GPa' }, nu: { v: 0.4 } },
+    cte: { type: 'const', a: { v: 22.5 } },
+    anand: { A: { v: 5.87e6 }, QR: { v: 7460 }, xi: { v: 2.0 }, m: { v: 0.0942 }, shat: { v: 58.3 }, n: { v: 0.015 }, h0: { v: 9350 }, a: { v: 1.5 }, s0: { v: 45.9 } },
+  };
+  const params = { A: 5.87e6, QR: 7460, xi: 2.0, m: 0.0942, shat: 58.3, n: 0.015, h0: 9350, a: 1.5, s0: 45.9, nu: 0.4 };
+  const npt = 6, nstep = 36;
+  const points = [];
+  for (let p = 0; p < npt; p++) {
+    const pt = new AnandPoint(mat);
+    const out = { sig: new Float64Array(6), D: new Float64Array(36) };
+    const steps = [];
+    for (let k = 1; k <= nstep; k++) {
+      const ph = 2 * Math.PI * k / 24;
+      const T = 25 + 82.5 * Math.sin(ph + 0.3 * p) - 0.0;
+      const dt = 20 + 10 * ((k + p) % 3);
+      const amp = 2e-3 * (1 + 0.5 * p);
+      const eps = Float64Array.from([amp * Math.sin(ph), -0.4 * amp * Math.sin(ph + 0.2), 0.3 * amp * Math.cos(ph * 0.7 + p),
+        amp * 0.8 * Math.sin(ph + p), 0.5 * amp * Math.cos(ph), (p % 2 ? -1 : 1) * 0.6 * amp * Math.sin(2 * ph)]);
+      const E = pt.E(T);
+      const d = pt.step(eps, T, dt, out, true);
+      pt.commit();
+      steps.push({ eps: Array.from(eps), T, dt, E, d, sig: Array.from(out.sig), D: Array.from(out.D),
+        ep: Array.from(pt.ep), s: pt.s, ebar: pt.ebar, W: pt.W, sigEq: pt.sigEq });
+    }
+    points.push(steps);
+  }
+  // V6 closed form (src/core/15-verify.js:237-248): saturation stress at constant strain rate
+  const sat = [[25, 1e-3], [125, 1e-4], [-40, 1e-2]].map(([T, rate]) => ({ T, rate, closed: anandSaturationStress(mat, T, rate), uniaxial: anandUniaxial(mat, T, rate, 0.05, 400).sigFinal }));
+  fs.writeFileSync(path.join(HERE, 'anand_golden.json'), JSON.stringify({ params, E_table: { T: [-40, 25, 75, 125], v: [57400, 46500, 38200, 30000] }, points, saturation: sat }));
+  console.log('anand points', npt, 'steps', nstep, 'saturation', JSON.stringify(sat));
+}
+
 const what = process.argv.slice(3);
+if (what.includes('anand')) anandCases();
 if (!what.length || what.includes('element')) elementCases();
 for (const n of ['small_box', 'plate_16', 'cfg1', 'cfg2']) if (!what.length || what.includes(n)) solveSpec(n);

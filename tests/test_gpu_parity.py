@@ -126,3 +126,18 @@ def test_graph_replay_matches_eager(lib):
         u_graph, r_graph = model.solve_step(st, tol=1e-10, precision=hs.PREC_MIXED)
         assert r_graph["iters"] == r_eager["iters"]
         assert np.abs(u_graph - u_eager).max() <= 1e-12 * np.abs(u_eager).max()
+
+
+@pytest.mark.parametrize("name,prec", [("plate_16", "mixed"), ("cfg1", "fp64")])
+def test_node_addon_step_solve(lib, name, prec):
+    # the JS host path: node -> Node-API addon -> C ABI -> HIP
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "js", "napi_solve.js")
+    out = subprocess.run([node, script, name, prec], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    assert rec["converged"] and rec["rel_err"] <= 1e-6, rec

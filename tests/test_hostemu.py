@@ -86,3 +86,18 @@ def test_error_paths():
     bad.constraint_dofs = np.array([10 ** 7], np.int32)
     with pytest.raises(hs.SolverError, match="constraint"):
         hs.Model(bad, variant="hostemu")
+
+
+def test_node_addon_loads_and_exports():
+    # the Node-API addon over the C ABI loads in node and registers its functions
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    addon = os.path.join(ROOT, "packaging-tm-modeling-tool_amd", "fcbga_hip.node")
+    assert os.path.exists(addon), "run __graft_entry__.build() first"
+    out = subprocess.run([node, "-e", f"const h=require({addon!r}); console.log(Object.keys(h).sort().join(','))"],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "createModel,destroyModel,solveStep"
