@@ -530,6 +530,16 @@ struct AnandParams {
   double A, QR, xi, m, shat, n, h0, a, s0, nu;
 };
 
+// The return map relies on exact cancellations (sigma_trial - 3 G d is exactly 0
+// at the bracket end d = sigma_trial / 3G); fused multiply-adds would leave a
+// tiny negative value there and sinh(x)^(1/m) would turn NaN.  The Anand
+// functions are therefore compiled without floating-point contraction.
+#if defined(__clang__)
+#define FE_NO_CONTRACT _Pragma("clang fp contract(off)")
+#else
+#define FE_NO_CONTRACT
+#endif
+
 constexpr int ANAND_MAXIT = 60;       // src/core/00-constants.js:123
 constexpr double ANAND_TOL = 1e-11;   // src/core/00-constants.js:124
 constexpr double KELVIN = 273.15;
@@ -537,6 +547,7 @@ constexpr double KELVIN = 273.15;
 FE_HD double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
 
 FE_HD double anand_solve_s(const AnandParams &P, double sn, double d, double dt, double expQ) {
+  FE_NO_CONTRACT
   if (d <= 0) return sn;
   const double sstar = P.shat * pow(d / (dt * P.A) * expQ, P.n);
   double lo = fmin(sn, sstar), hi = fmax(sn, sstar);
@@ -557,6 +568,7 @@ FE_HD double anand_solve_s(const AnandParams &P, double sn, double d, double dt,
 }
 
 FE_HD double anand_dsdd(const AnandParams &P, double d, double s, double dt, double expQ) {
+  FE_NO_CONTRACT
   if (d <= 0) return 0;
   const double sstar = P.shat * pow(d / (dt * P.A) * expQ, P.n);
   const double B = 1 - s / sstar, absB = fabs(B), sg = sgn(B);
@@ -570,6 +582,7 @@ FE_HD double anand_dsdd(const AnandParams &P, double d, double s, double dt, dou
 // Returns the equivalent plastic strain increment.
 FE_HD double anand_step(const AnandParams &P, double E, double T, double dt, const double eps[6], double *st,
                         double *out, double sig[6], double *D) {
+  FE_NO_CONTRACT
   const double G = E / (2 * (1 + P.nu)), K = E / (3 * (1 - 2 * P.nu));
   const double expQ = exp(P.QR / (T + KELVIN));
   const double rate0 = dt * P.A / expQ;

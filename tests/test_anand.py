@@ -144,8 +144,13 @@ def test_anand_full_batch_gpu_vs_oracle(oracle):
     s1, s2 = st.copy(), st.copy()
     tr1, sig1, D1, d1 = oracle.anand_batch(P, eps, -40.0, 45.0, 57400.0, s1)
     tr2, sig2, D2, d2 = hs.anand_update(_params(g), eps, -40.0, 45.0, 57400.0, s2, variant="hip")
+    # points driven to the end of the bracket (d = sigma_trial / 3G) get a non-finite
+    # tangent in the reference algorithm; both sides must agree on which ones
+    bad = ~np.isfinite(D1).all(axis=1)
+    assert np.array_equal(bad, ~np.isfinite(D2).all(axis=1))
+    assert bad.mean() < 0.05
     assert np.abs(sig1 - sig2).max() <= 1e-9 * np.abs(sig1).max()
-    assert np.abs(D1 - D2).max() <= 1e-7 * np.abs(D1).max()
+    assert np.abs(D1[~bad] - D2[~bad]).max() <= 1e-7 * np.abs(D1[~bad]).max()
     assert np.abs(tr1 - tr2).max() <= 1e-9 * np.abs(tr1).max()
     assert np.abs(d1 - d2).max() <= 1e-9 * max(np.abs(d1).max(), 1e-12)
 
@@ -157,5 +162,5 @@ def test_anand_saturation_closed_form_gpu(T, rate):
     P = _params(g)
     tab = g["E_table"]
     closed = [s["closed"] for s in g["saturation"] if s["T"] == T][0]
-    sig = uniaxial(P, lambda t: float(np.interp(t, tab["T"], tab["v"])), T, rate, 0.3, 150, "hip")
+    sig = uniaxial(P, lambda t: float(np.interp(t, tab["T"], tab["v"])), T, rate, 0.3, 300, "hip")
     assert abs(sig / closed - 1) < 0.01
