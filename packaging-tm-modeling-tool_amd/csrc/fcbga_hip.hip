@@ -606,12 +606,24 @@ FineOps<float> fine_ops<float>(fcbga_model *M) {
                         M->nodeAgg.p, M->zp32.p};
 }
 
+#if FCBGA_GPU
+template <typename T, int MODE>
+__global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *__restrict__ ct,
+                                                    const T *__restrict__ cs, const T *__restrict__ Kc,
+                                                    const T *__restrict__ w, const T *__restrict__ x,
+                                                    const T *__restrict__ bvec, T *__restrict__ y);
+#endif
+
 // y = A x on every grid node (A the scaled, constraint-reduced operator)
 template <typename T>
 void fine_apply(const FineOps<T> &o, const T *x, T *y, be::stream_t s) {
   const fe::Grid g = o.g;
   const int32_t *ct = o.cellType;
   const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w;
+#if FCBGA_GPU
+  hipLaunchKernelGGL((k_fine_col_op<T, 0>), dim3(g.nx * g.ny), dim3(64), 0, s, g, ct, cs, Kc, w, x, (const T *)nullptr, y);
+  BE_CHECK(hipGetLastError());
+#else
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
     int k = (int)(gi % g.nz);
     int64_t cj = gi / g.nz;
@@ -620,6 +632,7 @@ void fine_apply(const FineOps<T> &o, const T *x, T *y, be::stream_t s) {
     fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
     y[3 * gi] = out[0]; y[3 * gi + 1] = out[1]; y[3 * gi + 2] = out[2];
   });
+#endif
 }
 
 // res = b - A x on every grid node
@@ -628,6 +641,10 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
   const fe::Grid g = o.g;
   const int32_t *ct = o.cellType;
   const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w;
+#if FCBGA_GPU
+  hipLaunchKernelGGL((k_fine_col_op<T, 1>), dim3(g.nx * g.ny), dim3(64), 0, s, g, ct, cs, Kc, w, x, b, res);
+  BE_CHECK(hipGetLastError());
+#else
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
     int k = (int)(gi % g.nz);
     int64_t cj = gi / g.nz;
@@ -636,6 +653,7 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
     fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
     for (int p = 0; p < 3; p++) res[3 * gi + p] = b[3 * gi + p] - out[p];
   });
+#endif
 }
 
 // One colour of a column block Gauss-Seidel sweep (columnSweep,
@@ -678,25 +696,37 @@ __device__ __forceinline__ void scan_round_down(T A[9], T b[3], int s, int lane,
   }
 }
 
-// lane k = grid plane k of the column (nz <= 64): residual row, then both
-// substitution recurrences as wavefront scans in registers, then x += dx
+// lane k = grid plane k of the column (nz <= 64): the w*x neighbourhood of the
+// column is staged in LDS, then the residual row, both substitution
+// recurrences as wavefront scans in registers, then x += dx
 template <typename T>
 __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t *__restrict__ cols,
                                                        const int32_t *__restrict__ ct, const T *__restrict__ cs,
                                                        const T *__restrict__ Kc, const T *__restrict__ w,
                                                        const T *__restrict__ Cf, const T *__restrict__ bvec,
                                                        T *__restrict__ x) {
-  const int64_t c = cols[blockIdx.x];
-  const int i = (int)(c / g.ny), j = (int)(c % g.ny);
+  __shared__ T nb[9 * fe::NZP * 3];
+  const int c = cols[blockIdx.x];
+  const int i = c / g.ny, j = c % g.ny;
   const int nz = g.nz;
   const int k = threadIdx.x;
-  const int64_t gi = c * nz + k;
+  const int gi = c * nz + k;
+  for (int q = threadIdx.x; q < 9 * 2 * 3; q += 64) {  // padding levels -1 and nz
+    const int col = q / 6, side = (q / 3) % 2, p = q % 3;
+    nb[(col * fe::NZP + (side ? nz + 1 : 0)) * 3 + p] = T(0);
+  }
+  if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
+  __syncthreads();
   T r[3] = {0, 0, 0};
   T A[9], b[3];
   if (k < nz) {
-    T out[3];
-    fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
-    for (int p = 0; p < 3; p++) r[p] = bvec[3 * gi + p] - out[p];
+    T acc[3];
+    fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+    for (int p = 0; p < 3; p++) {
+      const T wi = w[3 * gi + p];
+      const T out = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
+      r[p] = bvec[3 * gi + p] - out;
+    }
     const T *o = Cf + 27 * gi;
     for (int q = 0; q < 9; q++) A[q] = o[9 + q];
     for (int p = 0; p < 3; p++) b[p] = o[3 * p] * r[0] + o[3 * p + 1] * r[1] + o[3 * p + 2] * r[2];
@@ -715,6 +745,35 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
   if (k < nz)
     for (int p = 0; p < 3; p++) x[3 * gi + p] += b[p];
 }
+
+// y = A x (mode 0) or y = b - A x (mode 1) over every node column, one 64-lane
+// block per column with the neighbourhood staged in LDS
+template <typename T, int MODE>
+__global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *__restrict__ ct,
+                                                    const T *__restrict__ cs, const T *__restrict__ Kc,
+                                                    const T *__restrict__ w, const T *__restrict__ x,
+                                                    const T *__restrict__ bvec, T *__restrict__ y) {
+  __shared__ T nb[9 * fe::NZP * 3];
+  const int c = blockIdx.x;
+  const int i = c / g.ny, j = c % g.ny;
+  const int nz = g.nz;
+  const int k = threadIdx.x;
+  const int gi = c * nz + k;
+  for (int q = threadIdx.x; q < 9 * 2 * 3; q += 64) {
+    const int col = q / 6, side = (q / 3) % 2, p = q % 3;
+    nb[(col * fe::NZP + (side ? nz + 1 : 0)) * 3 + p] = T(0);
+  }
+  if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
+  __syncthreads();
+  if (k >= nz) return;
+  T acc[3];
+  fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+  for (int p = 0; p < 3; p++) {
+    const T wi = w[3 * gi + p];
+    const T out = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
+    y[3 * gi + p] = MODE ? bvec[3 * gi + p] - out : out;
+  }
+}
 #endif
 
 template <typename T>
@@ -731,14 +790,21 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   hipLaunchKernelGGL(k_fine_col_sweep<T>, dim3(ncols), dim3(64), 0, s, g, cols, ct, cs, Kc, w, F, b, x);
   BE_CHECK(hipGetLastError());
 #else
-  be::pfor((int64_t)ncols * nz, s, [=] __host__ __device__(int64_t t) {
-    int64_t c = cols[t / nz];
-    int k = (int)(t % nz);
-    int i = (int)(c / g.ny), j = (int)(c % g.ny);
-    int64_t gi = c * nz + k;
-    T out[3];
-    fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
-    for (int p = 0; p < 3; p++) r[3 * gi + p] = b[3 * gi + p] - out[p];
+  be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
+    const int c = cols[t];
+    const int i = c / g.ny, j = c % g.ny;
+    T nb[9 * fe::NZP * 3];
+    for (int q = 0; q < 9 * fe::NZP * 3; q++) nb[q] = T(0);
+    for (int k = 0; k < nz; k++) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
+    for (int k = 0; k < nz; k++) {
+      const int gi = c * nz + k;
+      T acc[3];
+      fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+      for (int p = 0; p < 3; p++) {
+        const T wi = w[3 * gi + p];
+        r[3 * gi + p] = b[3 * gi + p] - (wi != T(0) ? wi * acc[p] : x[3 * gi + p]);
+      }
+    }
   });
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t];

@@ -659,4 +659,62 @@ FE_HD double anand_step(const AnandParams &P, double E, double T, double dt, con
   return d;
 }
 
+// ---------------------------------------------------------------------------
+// Operator rows of one z-column of nodes from a staged neighbourhood.
+// nb holds w*x of the 3 x 3 node columns around column (i, j) at levels
+// -1 .. nz (zero padded), layout nb[(col * NZP + k + 1) * 3 + p] with
+// col = 3 * (di + 1) + (dj + 1).  acc = sum_j K_ij (w x)_j for node k
+// (unscaled row; the caller applies w_i and the identity rows).
+// ---------------------------------------------------------------------------
+constexpr int NZP = 66;  // 64 planes + 2 padding levels
+
+template <typename T>
+FE_HD void fill_neighbourhood(const Grid &g, int i, int j, int k, const T *__restrict__ w, const T *__restrict__ x,
+                              T *nb) {
+  for (int di = 0; di < 3; di++)
+    for (int dj = 0; dj < 3; dj++) {
+      const int ii = i + di - 1, jj = j + dj - 1;
+      const bool in = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny;
+      const int gn = in ? ((ii * g.ny + jj) * g.nz + k) * 3 : 0;
+      T *o = nb + ((di * 3 + dj) * NZP + k + 1) * 3;
+      for (int p = 0; p < 3; p++) o[p] = in ? w[gn + p] * x[gn + p] : T(0);
+    }
+}
+
+template <typename T>
+FE_HD void node_rows_nb(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
+                        const T *__restrict__ cellScale, const T *__restrict__ Kc, const T *nb, T acc[3]) {
+  T a0 = 0, a1 = 0, a2 = 0;
+  for (int ci = 0; ci < 2; ci++) {
+    const int ii = i - 1 + ci;
+    if (ii < 0 || ii >= g.ncx) continue;
+    for (int cj = 0; cj < 2; cj++) {
+      const int jj = j - 1 + cj;
+      if (jj < 0 || jj >= g.ncy) continue;
+      for (int ck = 0; ck < 2; ck++) {
+        const int kk = k - 1 + ck;
+        if (kk < 0 || kk >= g.ncz) continue;
+        const int c = (ii * g.ncy + jj) * g.ncz + kk;
+        const int a = corner_of(1 - ci, 1 - cj, 1 - ck);
+        for (int q = 0; q < 2; q++) {
+          const int32_t t = cellType[2 * c + q];
+          if (t < 0) continue;
+          const T sc = cellScale[2 * c + q];
+          const T *K = Kc + t * 576 + a * 72;
+          T s0 = 0, s1 = 0, s2 = 0;
+          for (int b = 0; b < 8; b++) {
+            const int col = (ci + corner_li(b)) * 3 + (cj + corner_lj(b));
+            const T *xv = nb + (col * NZP + k + ck + corner_lk(b)) * 3;
+            s0 += K[3 * b] * xv[0] + K[3 * b + 1] * xv[1] + K[3 * b + 2] * xv[2];
+            s1 += K[24 + 3 * b] * xv[0] + K[24 + 3 * b + 1] * xv[1] + K[24 + 3 * b + 2] * xv[2];
+            s2 += K[48 + 3 * b] * xv[0] + K[48 + 3 * b + 1] * xv[1] + K[48 + 3 * b + 2] * xv[2];
+          }
+          a0 += sc * s0; a1 += sc * s1; a2 += sc * s2;
+        }
+      }
+    }
+  }
+  acc[0] = a0; acc[1] = a1; acc[2] = a2;
+}
+
 }  // namespace fe
