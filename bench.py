@@ -180,6 +180,12 @@ def main():
     torch.cuda.synchronize()
     ktimes = model.kernel_times()
     model.set_kernel_timing(False)
+    # a plain 256 MiB device copy: the reference point for scaling hardware byte counters
+    src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
     elapsed = t1 - t0
     dof_iters = sum(mesh.ndof * r["iters"] for r in recs)
     value = dof_iters / elapsed

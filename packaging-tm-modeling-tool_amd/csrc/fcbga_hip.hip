@@ -177,6 +177,8 @@ struct DevShell {
   DBuf<double> inv64, band;
   int32_t ndense = 0;
   int64_t cfTotal = 0;
+  int maxm = 0;    // largest column block (6 x aggregates)
+  int maxrow = 0;  // most blocks in one row
 };
 
 }  // namespace
@@ -206,8 +208,7 @@ struct fcbga_model {
   DBuf<float> cellScale32, Kc32;
   DBuf<uint8_t> slotHasF, slotLayered, typeHasF, d_born;
   DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64;
-  DBuf<float> w32, sinv32, F32, C32;
-  DBuf<double> C64;
+  DBuf<float> w32, sinv32, F32;
   // solver vectors (grid layout, 3 per node)
   DBuf<double> x, r, z, zold, p, q, tmp64, dx64;
   DBuf<float> b32v, x32v, r32v, dx32;
@@ -559,22 +560,20 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
   }
 }
 
-// column factors of the fine level and the coefficients of the scan form of
-// the column solve (FP64 and FP32 copies)
+// column factors of the fine level (FP64, FP32 copy); the scan coefficients of
+// the column solve are formed from them on the fly
 void factor_fine_columns(fcbga_model *M) {
   const fe::Grid g = M->g;
   const int64_t ncol = (int64_t)g.nx * g.ny;
   M->F64.alloc(18 * g.G);
-  M->C64.alloc(27 * g.G);
-  M->C32.alloc(27 * g.G);
+  M->F32.alloc(18 * g.G);
   const double *Db = M->Db.p, *Lb = M->Lb.p;
-  double *F = M->F64.p, *C = M->C64.p;
-  float *C32 = M->C32.p;
+  double *F = M->F64.p;
+  float *F32 = M->F32.p;
   const int nz = g.nz;
   be::pfor(ncol, M->stream, [=] __host__ __device__(int64_t c) {
     fe::column_factor(nz, Db + 9 * c * nz, Lb + 9 * c * nz, F + 18 * c * nz);
-    fe::column_scan_coefs<double>(nz, F + 18 * c * nz, C + 27 * c * nz);
-    for (int q = 0; q < 27 * nz; q++) C32[27 * c * nz + q] = (float)C[27 * c * nz + q];
+    for (int q = 0; q < 18 * nz; q++) F32[18 * c * nz + q] = (float)F[18 * c * nz + q];
   }, 64);
 }
 
@@ -597,12 +596,12 @@ FineOps<T> fine_ops(fcbga_model *M);
 
 template <>
 FineOps<double> fine_ops<double>(fcbga_model *M) {
-  return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->C64.p,
+  return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->F64.p,
                          M->nodeAgg.p, M->zp64.p};
 }
 template <>
 FineOps<float> fine_ops<float>(fcbga_model *M) {
-  return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->C32.p,
+  return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->F32.p,
                         M->nodeAgg.p, M->zp32.p};
 }
 
@@ -621,7 +620,8 @@ void fine_apply(const FineOps<T> &o, const T *x, T *y, be::stream_t s) {
   const int32_t *ct = o.cellType;
   const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w;
 #if FCBGA_GPU
-  hipLaunchKernelGGL((k_fine_col_op<T, 0>), dim3(g.nx * g.ny), dim3(64), 0, s, g, ct, cs, Kc, w, x, (const T *)nullptr, y);
+  hipLaunchKernelGGL((k_fine_col_op<T, 0>), dim3(8 * ((g.nx * g.ny + 7) / 8)), dim3(64), 0, s, g, ct, cs, Kc, w, x,
+                     (const T *)nullptr, y);
   BE_CHECK(hipGetLastError());
 #else
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
@@ -642,7 +642,8 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
   const int32_t *ct = o.cellType;
   const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w;
 #if FCBGA_GPU
-  hipLaunchKernelGGL((k_fine_col_op<T, 1>), dim3(g.nx * g.ny), dim3(64), 0, s, g, ct, cs, Kc, w, x, b, res);
+  hipLaunchKernelGGL((k_fine_col_op<T, 1>), dim3(8 * ((g.nx * g.ny + 7) / 8)), dim3(64), 0, s, g, ct, cs, Kc, w, x, b,
+                     res);
   BE_CHECK(hipGetLastError());
 #else
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
@@ -699,14 +700,24 @@ __device__ __forceinline__ void scan_round_down(T A[9], T b[3], int s, int lane,
 // lane k = grid plane k of the column (nz <= 64): the w*x neighbourhood of the
 // column is staged in LDS, then the residual row, both substitution
 // recurrences as wavefront scans in registers, then x += dx
+// Block order across the 8 XCDs: the hardware hands out workgroups round-robin,
+// so block b runs on XCD b % 8.  Giving each XCD a contiguous run of columns
+// keeps neighbouring columns (whose neighbourhoods overlap) in one L2.
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+  const int per = (nblocks + 7) >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
 template <typename T>
-__global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t *__restrict__ cols,
+__global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t *__restrict__ cols, int ncols,
                                                        const int32_t *__restrict__ ct, const T *__restrict__ cs,
                                                        const T *__restrict__ Kc, const T *__restrict__ w,
-                                                       const T *__restrict__ Cf, const T *__restrict__ bvec,
+                                                       const T *__restrict__ F, const T *__restrict__ bvec,
                                                        T *__restrict__ x) {
   __shared__ T nb[9 * fe::NZP * 3];
-  const int c = cols[blockIdx.x];
+  const int t = xcd_block(blockIdx.x, gridDim.x);
+  if (t >= ncols) return;
+  const int c = cols[t];
   const int i = c / g.ny, j = c % g.ny;
   const int nz = g.nz;
   const int k = threadIdx.x;
@@ -718,7 +729,6 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
   if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
   __syncthreads();
   T r[3] = {0, 0, 0};
-  T A[9], b[3];
   if (k < nz) {
     T acc[3];
     fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
@@ -727,19 +737,32 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
       const T out = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
       r[p] = bvec[3 * gi + p] - out;
     }
-    const T *o = Cf + 27 * gi;
-    for (int q = 0; q < 9; q++) A[q] = o[9 + q];
-    for (int p = 0; p < 3; p++) b[p] = o[3 * p] * r[0] + o[3 * p + 1] * r[1] + o[3 * p + 2] * r[2];
-  } else {
-    for (int q = 0; q < 9; q++) A[q] = 0;
-    for (int p = 0; p < 3; p++) b[p] = 0;
   }
+  // scan coefficients from this node's factor and the sub-diagonal block of
+  // the node above (lane k+1): Li = L_kk^-1, P = -Li L_{k,k-1}, Q = -Li^T L_{k+1,k}^T
+  T Fk[18];
+  for (int q = 0; q < 18; q++) Fk[q] = k < nz ? F[18 * gi + q] : T(0);
+  T Ld1[9];
+  for (int q = 0; q < 9; q++) Ld1[q] = __shfl_down(Fk[9 + q], 1, 64);
+  T Li[9], Q[9], A[9], b[3];
+  fe::inv_lower3<T>(Fk, Li);
+  const bool below = k > 0 && k < nz, above = k + 1 < nz;
+  for (int rr = 0; rr < 3; rr++)
+    for (int cc = 0; cc < 3; cc++) {
+      T v = 0, u = 0;
+      for (int q = 0; q < 3; q++) {
+        v -= Li[3 * rr + q] * Fk[9 + 3 * q + cc];
+        u -= Li[3 * q + rr] * Ld1[3 * cc + q];
+      }
+      A[3 * rr + cc] = below ? v : T(0);
+      Q[3 * rr + cc] = above ? u : T(0);
+    }
+  for (int p = 0; p < 3; p++) b[p] = k < nz ? Li[3 * p] * r[0] + Li[3 * p + 1] * r[1] + Li[3 * p + 2] * r[2] : T(0);
   for (int s = 1; s < nz; s <<= 1) scan_round_up<T>(A, b, s, k);
-  if (k < nz) {
-    const T *o = Cf + 27 * gi;
-    T y0 = b[0], y1 = b[1], y2 = b[2];
-    for (int q = 0; q < 9; q++) A[q] = o[18 + q];
-    for (int p = 0; p < 3; p++) b[p] = o[p] * y0 + o[3 + p] * y1 + o[6 + p] * y2;
+  {
+    const T y0 = b[0], y1 = b[1], y2 = b[2];
+    for (int q = 0; q < 9; q++) A[q] = Q[q];
+    for (int p = 0; p < 3; p++) b[p] = k < nz ? Li[p] * y0 + Li[3 + p] * y1 + Li[6 + p] * y2 : T(0);
   }
   for (int s = 1; s < nz; s <<= 1) scan_round_down<T>(A, b, s, k, nz);
   if (k < nz)
@@ -754,7 +777,8 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
                                                     const T *__restrict__ w, const T *__restrict__ x,
                                                     const T *__restrict__ bvec, T *__restrict__ y) {
   __shared__ T nb[9 * fe::NZP * 3];
-  const int c = blockIdx.x;
+  const int c = xcd_block(blockIdx.x, gridDim.x);
+  if (c >= g.nx * g.ny) return;
   const int i = c / g.ny, j = c % g.ny;
   const int nz = g.nz;
   const int k = threadIdx.x;
@@ -787,7 +811,8 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   KScope ks(M, FCBGA_KCLASS_FINE_SWEEP);
 #if FCBGA_GPU
   (void)r; (void)dx;
-  hipLaunchKernelGGL(k_fine_col_sweep<T>, dim3(ncols), dim3(64), 0, s, g, cols, ct, cs, Kc, w, F, b, x);
+  hipLaunchKernelGGL(k_fine_col_sweep<T>, dim3(8 * ((ncols + 7) / 8)), dim3(64), 0, s, g, cols, ncols, ct, cs, Kc, w, F,
+                     b, x);
   BE_CHECK(hipGetLastError());
 #else
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
@@ -808,7 +833,7 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   });
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t];
-    fe::column_solve_scan<T, T>(nz, F + 27 * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
+    fe::column_solve_scan<T, T>(nz, F + 18 * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
     for (int q = 0; q < 3 * nz; q++) x[3 * c * nz + q] += dx[3 * c * nz + q];
   }, 64);
 #endif
@@ -847,8 +872,10 @@ void build_hierarchy(fcbga_model *M) {
     D->rowPtr.upload(L.rowPtr, s);
     D->col.upload(L.col, s);
     std::vector<int32_t> brow(D->nb);
-    for (int32_t a = 0; a < L.nagg; a++)
+    for (int32_t a = 0; a < L.nagg; a++) {
       for (int32_t k = L.rowPtr[a]; k < L.rowPtr[a + 1]; k++) brow[k] = a;
+      D->maxrow = std::max(D->maxrow, (int)(L.rowPtr[a + 1] - L.rowPtr[a]));
+    }
     D->blockRow.upload(brow, s);
     D->val64.alloc((size_t)D->nb * 36);
     D->val32.alloc((size_t)D->nb * 36);
@@ -858,8 +885,11 @@ void build_hierarchy(fcbga_model *M) {
       cfp[c + 1] = cfp[c] + m * m;
     }
     D->cfTotal = cfp[D->ncol];
-    for (int32_t c = 0; c < D->ncol; c++)
-      if (6 * (L.colStart[c + 1] - L.colStart[c]) > SHELL_MAXM) throw be::Error("too many shell aggregates in one column");
+    for (int32_t c = 0; c < D->ncol; c++) {
+      const int m = 6 * (L.colStart[c + 1] - L.colStart[c]);
+      if (m > SHELL_MAXM) throw be::Error("too many shell aggregates in one column");
+      D->maxm = m > D->maxm ? m : D->maxm;
+    }
     D->cfPtr.upload(cfp, s);
     D->cf64.alloc(D->cfTotal);
     D->cf32.alloc(D->cfTotal);
@@ -962,6 +992,75 @@ void galerkin_level1(fcbga_model *M) {
 }
 
 // Galerkin product between shell levels l (fine) and l+1 (coarse)
+#if FCBGA_GPU
+constexpr int GAL_MAXROW = 64;  // blocks of one coarse row held in LDS
+
+// One 64-lane block per coarse row I: the row's blocks accumulate in LDS
+// while the wave walks the (child f, fine block k, parent pp of col k)
+// triples of the row once; lane (r, c) forms Tm = K_f,g W_g entry (r, c) and
+// lane (a, c) adds (W_f^T Tm)(a, c) to the block of column J = parent(pp).
+// Each block entry receives its terms in the same order as the per-block
+// form below.
+__global__ void __launch_bounds__(64) k_galerkin_shell(
+    const int32_t *__restrict__ crp, const int32_t *__restrict__ ccol, const int32_t *__restrict__ chPtr,
+    const int32_t *__restrict__ chIdx, const int32_t *__restrict__ chQ, const int32_t *__restrict__ frp,
+    const int32_t *__restrict__ fcol, const int32_t *__restrict__ pPtr, const int32_t *__restrict__ pIdx,
+    const double *__restrict__ fval, const double *__restrict__ W, double *__restrict__ val,
+    float *__restrict__ val32) {
+  extern __shared__ double acc[];  // maxrow x 36
+  __shared__ int32_t hkey[256];
+  __shared__ int32_t hslot[256];
+  const int32_t I = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int32_t r0 = crp[I], nblk = crp[I + 1] - r0;
+  for (int q = lane; q < nblk * 36; q += 64) acc[q] = 0;
+  for (int q = lane; q < 256; q += 64) hkey[q] = -1;
+  __syncthreads();
+  if (lane == 0)  // open-addressing map column J -> slot of the row
+    for (int32_t sl = 0; sl < nblk; sl++) {
+      const int32_t J = ccol[r0 + sl];
+      int h = J & 255;
+      while (hkey[h] >= 0) h = (h + 1) & 255;
+      hkey[h] = J;
+      hslot[h] = sl;
+    }
+  __syncthreads();
+  const int rr = lane / 6, cc = lane % 6;  // (r, c) for Tm and (a, c) for the product, lanes < 36
+  const bool act = lane < 36;
+  for (int32_t h = chPtr[I]; h < chPtr[I + 1]; h++) {
+    const int32_t f = chIdx[h];
+    const double *Wf = W + 36 * (int64_t)chQ[h];
+    double wfa[6];
+    for (int r = 0; r < 6; r++) wfa[r] = act ? Wf[6 * r + rr] : 0.0;  // W_f(r, a), a = rr
+    for (int32_t k = frp[f]; k < frp[f + 1]; k++) {
+      const int32_t gg = fcol[k];
+      const double *Kf = fval + 36 * (int64_t)k;
+      double kr[6];
+      for (int q = 0; q < 6; q++) kr[q] = act ? Kf[6 * rr + q] : 0.0;
+      for (int32_t pp = pPtr[gg]; pp < pPtr[gg + 1]; pp++) {
+        const int32_t J = pIdx[pp];
+        int hh = J & 255, probes = 0;
+        while (hkey[hh] != J && hkey[hh] >= 0 && probes < 256) { hh = (hh + 1) & 255; probes++; }
+        if (hkey[hh] != J) continue;  // not in the coarse pattern (the per-block form drops it too)
+        const int sl = hslot[hh];
+        const double *Wg = W + 36 * (int64_t)pp;
+        double tm = 0;
+        if (act)
+          for (int q = 0; q < 6; q++) tm += kr[q] * Wg[6 * q + cc];
+        double t = 0;
+        for (int r = 0; r < 6; r++) t += wfa[r] * __shfl(tm, 6 * r + cc, 64);
+        if (act) acc[36 * sl + lane] += t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = lane; q < nblk * 36; q += 64) {
+    val[36 * (int64_t)r0 + q] = acc[q];
+    val32[36 * (int64_t)r0 + q] = (float)acc[q];
+  }
+}
+#endif
+
 void galerkin_shell(fcbga_model *M, int l) {
   DevShell &F = *M->lv[l];
   DevShell &C = *M->lv[l + 1];
@@ -970,6 +1069,15 @@ void galerkin_shell(fcbga_model *M, int l) {
   const double *fval = F.val64.p, *W = F.pW64.p;
   double *val = C.val64.p;
   float *val32 = C.val32.p;
+#if FCBGA_GPU
+  if (C.maxrow <= GAL_MAXROW) {
+    const size_t lds = (size_t)(C.maxrow > 0 ? C.maxrow : 1) * 36 * sizeof(double);
+    hipLaunchKernelGGL(k_galerkin_shell, dim3(C.nagg), dim3(64), lds, M->stream, C.rowPtr.p, bcol, chPtr, chIdx,
+                       chQ, frp, fcol, pPtr, pIdx, fval, W, val, val32);
+    BE_CHECK(hipGetLastError());
+    return;
+  }
+#endif
   be::pfor(C.nb, M->stream, [=] __host__ __device__(int64_t kb) {
     int32_t I = brow[kb], J = bcol[kb];
     double acc[36];
@@ -1006,14 +1114,100 @@ void galerkin_shell(fcbga_model *M, int l) {
 // src/core/10-solver.js:474-494, jitter 1e-12) followed by the explicit
 // inverse of the column block, so that a column solve in the cycle is one
 // small dense product spread over a wavefront.
+#if FCBGA_GPU
+// One 64-lane block per column, m <= 48: the column block and the inverse are
+// staged in LDS.  Every entry is formed with the same operations in the same
+// order as chol_dense / chol_solve (jitter rule included); lanes only split
+// independent rows (factor) and unit vectors (inverse).
+__global__ void __launch_bounds__(64) k_shell_factor(const int32_t *__restrict__ cst, const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ col, const double *__restrict__ val,
+                                                     const int64_t *__restrict__ cfp, double *__restrict__ cf,
+                                                     float *__restrict__ cf32, int mmax) {
+  extern __shared__ double lds[];
+  double *A = lds, *X = lds + mmax * mmax;  // A row-major m x m; X[i * mmax + q] = row i of inverse column q
+  const int32_t c = blockIdx.x;
+  const int32_t n0 = cst[c], n1 = cst[c + 1];
+  const int m = 6 * (n1 - n0);
+  if (!m) return;
+  const int lane = threadIdx.x;
+  for (int q = lane; q < m * m; q += 64) A[q] = 0;
+  __syncthreads();
+  for (int32_t n = n0; n < n1; n++)
+    for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
+      const int32_t j = col[k];
+      if (j < n0 || j >= n1) continue;
+      if (lane < 36) {
+        const int p = lane / 6, q = lane % 6;
+        A[((n - n0) * 6 + p) * m + (j - n0) * 6 + q] = val[36 * (int64_t)k + lane];
+      }
+    }
+  __syncthreads();
+  double dmax = 0;
+  for (int i = 0; i < m; i++) dmax = fmax(dmax, fabs(A[i * m + i]));
+  const double jitter = 1e-12 * (dmax != 0 ? dmax : 1);
+  for (int j = 0; j < m; j++) {
+    const double *rj = A + j * m;
+    double d = rj[j];
+    for (int k = 0; k < j; k++) d -= rj[k] * rj[k];
+    if (!(d > jitter)) d = fmax(fabs(d), jitter);
+    const double ljj = sqrt(d);
+    const double inv = 1 / ljj;
+    const int i = j + 1 + lane;
+    double li = 0;
+    if (i < m) {
+      const double *ri = A + i * m;
+      double sacc = ri[j];
+      for (int k = 0; k < j; k++) sacc -= ri[k] * rj[k];
+      li = sacc * inv;
+    }
+    __syncthreads();
+    if (lane == 0) A[j * m + j] = ljj;
+    if (i < m) A[i * m + j] = li;
+    __syncthreads();
+  }
+  // inverse: lane q solves L L^T x = e_q
+  const int q = lane;
+  if (q < m) {
+    for (int i = 0; i < m; i++) {
+      double sacc = (i == q) ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) sacc -= A[i * m + k] * X[k * mmax + q];
+      X[i * mmax + q] = sacc / A[i * m + i];
+    }
+    for (int i = m - 1; i >= 0; i--) {
+      double sacc = X[i * mmax + q];
+      for (int k = i + 1; k < m; k++) sacc -= A[k * m + i] * X[k * mmax + q];
+      X[i * mmax + q] = sacc / A[i * m + i];
+    }
+  }
+  __syncthreads();
+  double *Ai = cf + cfp[c];
+  float *Ai32 = cf32 + cfp[c];
+  for (int e = lane; e < m * m; e += 64) {  // column-major: Ai[q * m + i]
+    const double v = X[(e % m) * mmax + e / m];
+    Ai[e] = v;
+    Ai32[e] = (float)v;
+  }
+}
+#endif
+
 void factor_shell_columns(fcbga_model *M, int l) {
   DevShell &D = *M->lv[l];
   const int32_t *cst = D.colStart.p, *rp = D.rowPtr.p, *col = D.col.p;
   const int64_t *cfp = D.cfPtr.p;
   const double *val = D.val64.p;
-  D.cfTmp.alloc(D.cfTotal);
-  double *L = D.cfTmp.p, *cf = D.cf64.p;
+  double *cf = D.cf64.p;
   float *cf32 = D.cf32.p;
+#if FCBGA_GPU
+  if (D.maxm <= 48) {
+    const int mmax = D.maxm > 0 ? D.maxm : 1;
+    const size_t lds = 2 * (size_t)mmax * mmax * sizeof(double);
+    hipLaunchKernelGGL(k_shell_factor, dim3(D.ncol), dim3(64), lds, M->stream, cst, rp, col, val, cfp, cf, cf32, mmax);
+    BE_CHECK(hipGetLastError());
+    return;
+  }
+#endif
+  D.cfTmp.alloc(D.cfTotal);
+  double *L = D.cfTmp.p;
   be::pfor(D.ncol, M->stream, [=] __host__ __device__(int64_t c) {
     int32_t n0 = cst[c], n1 = cst[c + 1];
     int m = 6 * (n1 - n0);
@@ -1081,20 +1275,71 @@ __global__ void __launch_bounds__(256) k_dense_matvec(int32_t n, const double *_
   if (lane == 0) x[row] = (T)part;
 }
 
-// one colour of a shell-level column sweep: one 64-lane block per column;
-// lanes compute the column's residual rows into LDS, then the correction
-// x_col += Ainv_col r_col (Ainv stored by columns, read coalesced)
+// one colour of a shell-level column sweep: one 64-lane block per column.
+// Columns hold m = 6 x (1..4) DOFs on the usual meshes, so for m <= 32 the
+// lanes are split into S = 64/m groups: group g takes every S-th block of a
+// row for the residual and every S-th column of the inverse for the
+// correction, and the S partial sums meet in LDS.  Larger columns take one
+// lane per row.  The inverse is stored by columns (read coalesced).
 template <typename T>
-__global__ void __launch_bounds__(64) k_shell_sweep(const int32_t *__restrict__ cols, const int32_t *__restrict__ cst,
+__global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, const int32_t *__restrict__ cst,
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                     const T *__restrict__ val, const T *__restrict__ inv,
                                                     const int64_t *__restrict__ cfp, const T *__restrict__ b,
                                                     T *__restrict__ x) {
   __shared__ T r[SHELL_MAXM];
-  const int32_t c = cols[blockIdx.x];
+  __shared__ T part[64];
+  // colour (ci, cj) holds the columns (ci + 2u, cj + 2v), u-major
+  const int nyc = (ny - cj + 1) >> 1;
+  const int32_t c = (ci + 2 * (int)(blockIdx.x / nyc)) * ny + cj + 2 * (int)(blockIdx.x % nyc);
   const int32_t n0 = cst[c], n1 = cst[c + 1];
   const int m = 6 * (n1 - n0);
-  for (int i = threadIdx.x; i < m; i += 64) {
+  if (m == 0) return;
+  const int lane = threadIdx.x;
+  const T *A = inv + cfp[c];
+  if (m <= 32) {
+    const int S = 64 / m;
+    const int i = lane % m, grp = lane / m;
+    // this lane's share of the inverse (columns grp, grp + S, ...: at most 16)
+    // and of b, loaded before the residual so their latency overlaps it
+    T ainv[16];
+    for (int u = 0; u < 16; u++) {
+      const int j = grp + u * S;
+      ainv[u] = (grp < S && j < m) ? A[(int64_t)j * m + i] : T(0);
+    }
+    const T bl = lane < m ? b[6 * (int64_t)n0 + lane] : T(0);
+    T acc = 0;
+    if (grp < S) {
+      const int32_t n = n0 + i / 6;
+      const int p = i % 6;
+      for (int32_t k = rp[n] + grp; k < rp[n + 1]; k += S) {
+        const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
+        acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
+      }
+    }
+    part[lane] = acc;
+    __syncthreads();
+    if (lane < m) {
+      T rr = bl;
+      for (int g = 0; g < S; g++) rr += part[g * m + lane];
+      r[lane] = rr;
+    }
+    __syncthreads();
+    T dx = 0;
+    for (int u = 0; u < 16; u++) {
+      const int j = grp + u * S;
+      if (grp < S && j < m) dx += ainv[u] * r[j];
+    }
+    part[lane] = dx;
+    __syncthreads();
+    if (lane < m) {
+      T d = 0;
+      for (int g = 0; g < S; g++) d += part[g * m + lane];
+      x[6 * (int64_t)n0 + lane] += d;
+    }
+    return;
+  }
+  for (int i = lane; i < m; i += 64) {
     const int32_t n = n0 + i / 6;
     const int p = i % 6;
     T acc = b[6 * (int64_t)n0 + i];
@@ -1105,8 +1350,7 @@ __global__ void __launch_bounds__(64) k_shell_sweep(const int32_t *__restrict__ 
     r[i] = acc;
   }
   __syncthreads();
-  const T *A = inv + cfp[c];
-  for (int i = threadIdx.x; i < m; i += 64) {
+  for (int i = lane; i < m; i += 64) {
     T acc = 0;
     for (int j = 0; j < m; j++) acc += A[(int64_t)j * m + i] * r[j];
     x[6 * (int64_t)n0 + i] += acc;
@@ -1254,7 +1498,8 @@ void shell_sweep(DevShell &D, int color, be::stream_t s) {
   const int32_t ncols = D.ncolor[color];
   if (!ncols) return;
 #if FCBGA_GPU
-  hipLaunchKernelGGL(k_shell_sweep<T>, dim3(ncols), dim3(64), 0, s, cols, cst, rp, col, val, inv, cfp, b, x);
+  hipLaunchKernelGGL(k_shell_sweep<T>, dim3(ncols), dim3(64), 0, s, color >> 1, color & 1, D.ny, cst, rp, col, val,
+                     inv, cfp, b, x);
   BE_CHECK(hipGetLastError());
 #else
   (void)s;

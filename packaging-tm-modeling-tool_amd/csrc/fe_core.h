@@ -431,57 +431,56 @@ FE_HD void element_condense(double hx, double hy, double hz, const double *__res
 // with Li_k = L_kk^-1.  Per node 27 coefficients: Li (9), P (9), Q (9).  A
 // recurrence of affine maps is solved by an inclusive scan in log2(nz) rounds.
 // ---------------------------------------------------------------------------
-FE_HD void inv_lower3(const double *L, double *Li) {
+template <typename T>
+FE_HD void inv_lower3(const T *L, T *Li) {
   // inverse of a lower-triangular 3x3 (row-major)
-  double a = L[0], b = L[3], c = L[4], d = L[6], e = L[7], f = L[8];
-  Li[0] = 1 / a; Li[1] = 0; Li[2] = 0;
-  Li[4] = 1 / c; Li[5] = 0;
-  Li[8] = 1 / f;
+  const T a = L[0], b = L[3], c = L[4], d = L[6], e = L[7], f = L[8];
+  Li[0] = T(1) / a; Li[1] = 0; Li[2] = 0;
+  Li[4] = T(1) / c; Li[5] = 0;
+  Li[8] = T(1) / f;
   Li[3] = -b / (a * c);
   Li[7] = -e / (c * f);
   Li[6] = (b * e - c * d) / (a * c * f);
 }
 
-// coefficients of one column from its factor F (18 per node) into C (27 per node)
-template <typename TC>
-FE_HD void column_scan_coefs(int nz, const double *F, TC *C) {
-  for (int k = 0; k < nz; k++) {
-    double Li[9];
-    inv_lower3(F + 18 * k, Li);
-    TC *o = C + 27 * k;
-    for (int q = 0; q < 9; q++) o[q] = (TC)Li[q];
-    // P_k = -Li_k L_{k,k-1}
-    const double *Ld = F + 18 * k + 9;
-    for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 3; c++) {
-        double v = 0;
-        if (k > 0)
-          for (int t = 0; t < 3; t++) v -= Li[3 * r + t] * Ld[3 * t + c];
-        o[9 + 3 * r + c] = (TC)v;
-      }
-    // Q_k = -Li_k^T L_{k+1,k}^T
-    for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 3; c++) {
-        double v = 0;
-        if (k + 1 < nz) {
-          const double *Lu = F + 18 * (k + 1) + 9;
-          for (int t = 0; t < 3; t++) v -= Li[3 * t + r] * Lu[3 * c + t];
-        }
-        o[18 + 3 * r + c] = (TC)v;
-      }
-  }
+// Scan coefficients of node k from its factor Fk (18) and the sub-diagonal
+// block of node k+1 (Ld1, the 9 values after Fk+1's diagonal factor; unused
+// at the top of the column): out = Li (9), P (9), Q (9).
+template <typename T>
+FE_HD void node_scan_coefs(const T *Fk, bool hasBelow, bool hasAbove, const T *Ld1, T out[27]) {
+  T Li[9];
+  inv_lower3<T>(Fk, Li);
+  for (int q = 0; q < 9; q++) out[q] = Li[q];
+  const T *Ld = Fk + 9;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      T v = 0;
+      if (hasBelow)
+        for (int t = 0; t < 3; t++) v -= Li[3 * r + t] * Ld[3 * t + c];
+      out[9 + 3 * r + c] = v;
+      T w = 0;
+      if (hasAbove)
+        for (int t = 0; t < 3; t++) w -= Li[3 * t + r] * Ld1[3 * c + t];
+      out[18 + 3 * r + c] = w;
+    }
 }
 
 // Host form of the scan solve (the GPU kernel does the same rounds with lane
-// shuffles): x = (L L^T)^-1 r for one column, nz <= 128.
-template <typename T, typename TC>
-FE_HD void column_solve_scan(int nz, const TC *C, const T *r, T *x) {
-  T A[128][9], b[128][3], A2[128][9], b2[128][3];
+// shuffles): x = (L L^T)^-1 r for one column from its factors F (18 per node), nz <= 128.
+template <typename T, typename TF>
+FE_HD void column_solve_scan(int nz, const TF *F, const T *r, T *x) {
+  T A[128][9], b[128][3], A2[128][9], b2[128][3], C[128][27];
+  for (int k = 0; k < nz; k++) {
+    T Fk[18], Ld1[9];
+    for (int q = 0; q < 18; q++) Fk[q] = (T)F[18 * k + q];
+    for (int q = 0; q < 9; q++) Ld1[q] = k + 1 < nz ? (T)F[18 * (k + 1) + 9 + q] : T(0);
+    node_scan_coefs<T>(Fk, k > 0, k + 1 < nz, Ld1, C[k]);
+  }
   // forward: maps y_k = P_k y_{k-1} + Li_k r_k
   for (int k = 0; k < nz; k++) {
-    const TC *o = C + 27 * k;
-    for (int q = 0; q < 9; q++) A[k][q] = (T)o[9 + q];
-    for (int p = 0; p < 3; p++) b[k][p] = (T)o[3 * p] * r[3 * k] + (T)o[3 * p + 1] * r[3 * k + 1] + (T)o[3 * p + 2] * r[3 * k + 2];
+    const T *o = C[k];
+    for (int q = 0; q < 9; q++) A[k][q] = o[9 + q];
+    for (int p = 0; p < 3; p++) b[k][p] = o[3 * p] * r[3 * k] + o[3 * p + 1] * r[3 * k + 1] + o[3 * p + 2] * r[3 * k + 2];
   }
   for (int s = 1; s < nz; s <<= 1) {
     for (int k = 0; k < nz; k++) {
@@ -496,10 +495,10 @@ FE_HD void column_solve_scan(int nz, const TC *C, const T *r, T *x) {
   }
   // backward: maps x_k = Q_k x_{k+1} + Li_k^T y_k
   for (int k = 0; k < nz; k++) {
-    const TC *o = C + 27 * k;
+    const T *o = C[k];
     T y0 = b[k][0], y1 = b[k][1], y2 = b[k][2];
-    for (int q = 0; q < 9; q++) A[k][q] = (T)o[18 + q];
-    for (int p = 0; p < 3; p++) b[k][p] = (T)o[p] * y0 + (T)o[3 + p] * y1 + (T)o[6 + p] * y2;
+    for (int q = 0; q < 9; q++) A[k][q] = o[18 + q];
+    for (int p = 0; p < 3; p++) b[k][p] = o[p] * y0 + o[3 + p] * y1 + o[6 + p] * y2;
   }
   for (int s = 1; s < nz; s <<= 1) {
     for (int k = 0; k < nz; k++) {
