@@ -96,7 +96,7 @@ typedef struct fcbga_solve_stats {
   int32_t coarsest_dof;
   int64_t ndof;
   int32_t ntypes;     /* distinct condensed element matrices */
-  int32_t reserved;
+  int32_t sym_tables; /* 1: the fine operator ran on the mirror-symmetric compact element tables */
 } fcbga_solve_stats;
 
 int fcbga_model_create(const fcbga_mesh *mesh, fcbga_model **out);

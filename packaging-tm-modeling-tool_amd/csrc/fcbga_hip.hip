@@ -28,7 +28,10 @@ namespace {
 
 thread_local std::string g_err;
 
-constexpr int SHELL_MAXM = 512;  // 6 DOFs x up to 85 aggregates in one shell column
+constexpr int SHELL_MAXM = 512;
+// largest deviation (relative to the largest entry) of a condensed element
+// table from its mirror-symmetric form for the compact tables to be used
+constexpr double SYM_TOL = 1e-11;  // 6 DOFs x up to 85 aggregates in one shell column
 constexpr int FINE_MAXNZ = 64;  // grid planes in z: one lane per plane in a fine column block
 
 double now_ms() {
@@ -206,6 +209,12 @@ struct fcbga_model {
   // device: per step
   DBuf<double> cellScale64, Kc64, fc64, slotM, slotLoad, tabs;
   DBuf<float> cellScale32, Kc32;
+  // corner-0 rows of the mirror-symmetric element tables (72 per type), used
+  // by the fine operator when every type passes the mirror check
+  DBuf<double> R64, symErr;
+  DBuf<float> R32;
+  bool symTables = false;
+  bool sym_allowed = true;  // FCBGA_FULL_TABLES=1 forces the full tables
   DBuf<uint8_t> slotHasF, slotLayered, typeHasF, d_born;
   DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64;
   DBuf<float> w32, sinv32, F32;
@@ -485,6 +494,25 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
       tF[t] = sF[sl];
     }, 64);
   }
+  // mirror-symmetric compact tables (all types must pass, else the full tables)
+  M->R64.alloc((size_t)M->ntypes * 72);
+  M->R32.alloc((size_t)M->ntypes * 72);
+  M->symErr.alloc(M->ntypes);
+  {
+    const double *Kc = M->Kc64.p;
+    double *R = M->R64.p, *err = M->symErr.p;
+    float *R32 = M->R32.p;
+    be::pfor(M->ntypes, s, [=] __host__ __device__(int64_t t) {
+      err[t] = fe::sym_table(Kc + 576 * t, R + 72 * t);
+      for (int q = 0; q < 72; q++) R32[72 * t + q] = (float)R[72 * t + q];
+    }, 64);
+    std::vector<double> e(M->ntypes);
+    be::d2h(e.data(), M->symErr.p, e.size() * sizeof(double), s);
+    be::sync(s);
+    double emax = 0;
+    for (double v : e) emax = v > emax ? v : emax;
+    M->symTables = M->sym_allowed && emax <= SYM_TOL;
+  }
   // element scales of the step: eScale x (born ? 1 : 1e-6)
   M->cellScale64.alloc(2 * ncells);
   M->cellScale32.alloc(2 * ncells);
@@ -589,6 +617,8 @@ struct FineOps {
   const T *F;
   const int32_t *nodeAgg;
   const T *zp;
+  const T *R;  // compact mirror-symmetric tables (72 per type) when sym
+  bool sym;
 };
 
 template <typename T>
@@ -597,20 +627,33 @@ FineOps<T> fine_ops(fcbga_model *M);
 template <>
 FineOps<double> fine_ops<double>(fcbga_model *M) {
   return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->F64.p,
-                         M->nodeAgg.p, M->zp64.p};
+                         M->nodeAgg.p, M->zp64.p, M->R64.p, M->symTables};
 }
 template <>
 FineOps<float> fine_ops<float>(fcbga_model *M) {
   return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->F32.p,
-                        M->nodeAgg.p, M->zp32.p};
+                        M->nodeAgg.p, M->zp32.p, M->R32.p, M->symTables};
 }
 
 #if FCBGA_GPU
-template <typename T, int MODE>
+template <typename T, int MODE, bool SYM>
 __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *__restrict__ ct,
                                                     const T *__restrict__ cs, const T *__restrict__ Kc,
                                                     const T *__restrict__ w, const T *__restrict__ x,
                                                     const T *__restrict__ bvec, T *__restrict__ y);
+
+template <typename T, int MODE>
+void launch_col_op(const FineOps<T> &o, const T *x, const T *b, T *y, be::stream_t s) {
+  const fe::Grid g = o.g;
+  const dim3 grid(8 * ((g.nx * g.ny + 7) / 8));
+  if (o.sym)
+    hipLaunchKernelGGL((k_fine_col_op<T, MODE, true>), grid, dim3(64), 0, s, g, o.cellType, o.cellScale, o.R, o.w, x,
+                       b, y);
+  else
+    hipLaunchKernelGGL((k_fine_col_op<T, MODE, false>), grid, dim3(64), 0, s, g, o.cellType, o.cellScale, o.Kc, o.w,
+                       x, b, y);
+  BE_CHECK(hipGetLastError());
+}
 #endif
 
 // y = A x on every grid node (A the scaled, constraint-reduced operator)
@@ -620,16 +663,20 @@ void fine_apply(const FineOps<T> &o, const T *x, T *y, be::stream_t s) {
   const int32_t *ct = o.cellType;
   const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w;
 #if FCBGA_GPU
-  hipLaunchKernelGGL((k_fine_col_op<T, 0>), dim3(8 * ((g.nx * g.ny + 7) / 8)), dim3(64), 0, s, g, ct, cs, Kc, w, x,
-                     (const T *)nullptr, y);
-  BE_CHECK(hipGetLastError());
+  (void)ct; (void)cs; (void)Kc; (void)w;
+  launch_col_op<T, 0>(o, x, (const T *)nullptr, y, s);
 #else
+  const T *R = o.R;
+  const bool sym = o.sym;
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
     int k = (int)(gi % g.nz);
     int64_t cj = gi / g.nz;
     int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
     T out[3];
-    fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
+    if (sym)
+      fe::node_apply_sym<T>(g, i, j, k, ct, cs, R, w, x, out);
+    else
+      fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
     y[3 * gi] = out[0]; y[3 * gi + 1] = out[1]; y[3 * gi + 2] = out[2];
   });
 #endif
@@ -642,16 +689,20 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
   const int32_t *ct = o.cellType;
   const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w;
 #if FCBGA_GPU
-  hipLaunchKernelGGL((k_fine_col_op<T, 1>), dim3(8 * ((g.nx * g.ny + 7) / 8)), dim3(64), 0, s, g, ct, cs, Kc, w, x, b,
-                     res);
-  BE_CHECK(hipGetLastError());
+  (void)ct; (void)cs; (void)Kc; (void)w;
+  launch_col_op<T, 1>(o, x, b, res, s);
 #else
+  const T *R = o.R;
+  const bool sym = o.sym;
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
     int k = (int)(gi % g.nz);
     int64_t cj = gi / g.nz;
     int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
     T out[3];
-    fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
+    if (sym)
+      fe::node_apply_sym<T>(g, i, j, k, ct, cs, R, w, x, out);
+    else
+      fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
     for (int p = 0; p < 3; p++) res[3 * gi + p] = b[3 * gi + p] - out[p];
   });
 #endif
@@ -708,7 +759,7 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
   return (b & 7) * per + (b >> 3);
 }
 
-template <typename T>
+template <typename T, bool SYM>
 __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t *__restrict__ cols, int ncols,
                                                        const int32_t *__restrict__ ct, const T *__restrict__ cs,
                                                        const T *__restrict__ Kc, const T *__restrict__ w,
@@ -729,15 +780,23 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
   if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
   __syncthreads();
   T r[3] = {0, 0, 0};
-  if (k < nz) {
-    T acc[3];
+  T acc[3] = {0, 0, 0};
+  if (SYM) {
+    T up[3] = {0, 0, 0};
+    if (k < nz) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up);
+    for (int p = 0; p < 3; p++) {
+      const T u = __shfl_up(up[p], 1, 64);
+      acc[p] += k > 0 ? u : T(0);
+    }
+  } else if (k < nz) {
     fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+  }
+  if (k < nz)
     for (int p = 0; p < 3; p++) {
       const T wi = w[3 * gi + p];
       const T out = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
       r[p] = bvec[3 * gi + p] - out;
     }
-  }
   // scan coefficients from this node's factor and the sub-diagonal block of
   // the node above (lane k+1): Li = L_kk^-1, P = -Li L_{k,k-1}, Q = -Li^T L_{k+1,k}^T
   T Fk[18];
@@ -771,7 +830,7 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
 
 // y = A x (mode 0) or y = b - A x (mode 1) over every node column, one 64-lane
 // block per column with the neighbourhood staged in LDS
-template <typename T, int MODE>
+template <typename T, int MODE, bool SYM>
 __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *__restrict__ ct,
                                                     const T *__restrict__ cs, const T *__restrict__ Kc,
                                                     const T *__restrict__ w, const T *__restrict__ x,
@@ -789,9 +848,18 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
   }
   if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
   __syncthreads();
+  T acc[3] = {0, 0, 0};
+  if (SYM) {
+    T up[3] = {0, 0, 0};
+    if (k < nz) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up);
+    for (int p = 0; p < 3; p++) {
+      const T u = __shfl_up(up[p], 1, 64);
+      acc[p] += k > 0 ? u : T(0);
+    }
+  } else if (k < nz) {
+    fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+  }
   if (k >= nz) return;
-  T acc[3];
-  fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
   for (int p = 0; p < 3; p++) {
     const T wi = w[3 * gi + p];
     const T out = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
@@ -811,20 +879,33 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   KScope ks(M, FCBGA_KCLASS_FINE_SWEEP);
 #if FCBGA_GPU
   (void)r; (void)dx;
-  hipLaunchKernelGGL(k_fine_col_sweep<T>, dim3(8 * ((ncols + 7) / 8)), dim3(64), 0, s, g, cols, ncols, ct, cs, Kc, w, F,
-                     b, x);
+  if (o.sym)
+    hipLaunchKernelGGL((k_fine_col_sweep<T, true>), dim3(8 * ((ncols + 7) / 8)), dim3(64), 0, s, g, cols, ncols, ct,
+                       cs, o.R, w, F, b, x);
+  else
+    hipLaunchKernelGGL((k_fine_col_sweep<T, false>), dim3(8 * ((ncols + 7) / 8)), dim3(64), 0, s, g, cols, ncols, ct,
+                       cs, Kc, w, F, b, x);
   BE_CHECK(hipGetLastError());
 #else
+  const T *Rt = o.R;
+  const bool symT = o.sym;
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     const int c = cols[t];
     const int i = c / g.ny, j = c % g.ny;
     T nb[9 * fe::NZP * 3];
     for (int q = 0; q < 9 * fe::NZP * 3; q++) nb[q] = T(0);
     for (int k = 0; k < nz; k++) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
+    T upPrev[3] = {0, 0, 0};
     for (int k = 0; k < nz; k++) {
       const int gi = c * nz + k;
       T acc[3];
-      fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+      if (symT) {
+        T up[3];
+        fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Rt, nb, acc, up);
+        for (int p = 0; p < 3; p++) { acc[p] += upPrev[p]; upPrev[p] = up[p]; }
+      } else {
+        fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+      }
       for (int p = 0; p < 3; p++) {
         const T wi = w[3 * gi + p];
         r[3 * gi + p] = b[3 * gi + p] - (wi != T(0) ? wi * acc[p] : x[3 * gi + p]);
@@ -1812,6 +1893,7 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
   be::stream_t s = M->stream;
   const double t0 = now_ms();
   assemble_step(M, st);
+  S.sym_tables = M->symTables ? 1 : 0;
   if (o.precond != FCBGA_PC_JACOBI) factor_fine_columns(M);
   alloc_vectors(M, o.precision);
   be::sync(s);
@@ -1900,6 +1982,10 @@ int fcbga_model_create(const fcbga_mesh *mesh, fcbga_model **out) {
   if (!out) { g_err = "out is NULL"; return FCBGA_ERR_ARG; }
   *out = nullptr;
   fcbga_model *M = new fcbga_model();
+  {
+    const char *ev = getenv("FCBGA_FULL_TABLES");
+    M->sym_allowed = !(ev && ev[0] == '1');
+  }
   try {
     model_create(mesh, M);
   } catch (const be::Error &e) {

@@ -716,4 +716,152 @@ FE_HD void node_rows_nb(const Grid &g, int i, int j, int k, const int32_t *__res
   acc[0] = a0; acc[1] = a1; acc[2] = a2;
 }
 
+// ---------------------------------------------------------------------------
+// Reflection-symmetric element tables.  A box element whose material has no
+// normal/shear and no shear/shear coupling is invariant under the mirror
+// images through its mid-planes, so with mirror mask m (bits x, y, z) and S_m
+// the matching sign flips of the displacement components,
+//   K_ab = S_a K_{0, a^b} S_a       (a^b: the corner with bits(a) xor bits(b)).
+// The corner-0 rows R_c = K_{0,c} (3 x 24, stored [p][3c+q] like the full
+// table) then carry the whole element.  cell_sym_pair forms, for one cell,
+// the rows of corner (MX, MY, 0) ("own") and of corner (MX, MY, 1) ("up")
+// applied to the cell's corner vectors xb, scaled by sc.
+template <typename T, int MX, int MY>
+FE_HD void cell_sym_pair(const T *__restrict__ R, const T (&xb)[8][3], T sc, T own[3], T up[3]) {
+  const T sx = MX ? T(-1) : T(1), sy = MY ? T(-1) : T(1);
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int p = 0; p < 3; p++) {
+    T zo = 0, zu = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int c = 0; c < 8; c++) {
+      const int co = corner_of(corner_li(c) ^ MX, corner_lj(c) ^ MY, corner_lk(c));
+      const int cu = corner_of(corner_li(c) ^ MX, corner_lj(c) ^ MY, corner_lk(c) ^ 1);
+      const T r0 = R[24 * p + 3 * c], r1 = R[24 * p + 3 * c + 1], r2 = R[24 * p + 3 * c + 2];
+      zo += r0 * (sx * xb[co][0]) + r1 * (sy * xb[co][1]) + r2 * xb[co][2];
+      zu += r0 * (sx * xb[cu][0]) + r1 * (sy * xb[cu][1]) - r2 * xb[cu][2];
+    }
+    const T sp = p == 0 ? sx : p == 1 ? sy : T(1);
+    own[p] += sc * sp * zo;
+    up[p] += sc * (p == 2 ? -sp : sp) * zu;
+  }
+}
+
+// Column form on the LDS neighbourhood: for node (i, j, k) the four cells of
+// level k (above the node).  own = their rows of node (i, j, k), up = their
+// rows of node (i, j, k+1); the node's full row is own_k + up_{k-1}.
+template <typename T, int CI, int CJ>
+FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
+                       const T *__restrict__ cellScale, const T *__restrict__ R, const T *nb, T own[3], T up[3]) {
+  const int ii = i - 1 + CI, jj = j - 1 + CJ;
+  if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy) return;
+  const int c = (ii * g.ncy + jj) * g.ncz + k;
+  for (int q = 0; q < 2; q++) {
+    const int32_t t = cellType[2 * c + q];
+    if (t < 0) continue;
+    const T sc = cellScale[2 * c + q];
+    T xb[8][3];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int b = 0; b < 8; b++) {
+      const int col = (CI + corner_li(b)) * 3 + (CJ + corner_lj(b));
+      const T *xv = nb + (col * NZP + k + corner_lk(b) + 1) * 3;
+      xb[b][0] = xv[0]; xb[b][1] = xv[1]; xb[b][2] = xv[2];
+    }
+    cell_sym_pair<T, 1 - CI, 1 - CJ>(R + 72 * t, xb, sc, own, up);
+  }
+}
+
+template <typename T>
+FE_HD void node_rows_nb_sym(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
+                            const T *__restrict__ cellScale, const T *__restrict__ R, const T *nb, T own[3],
+                            T up[3]) {
+  for (int p = 0; p < 3; p++) { own[p] = 0; up[p] = 0; }
+  if (k >= g.ncz) return;
+  cell_sym_nb<T, 0, 0>(g, i, j, k, cellType, cellScale, R, nb, own, up);
+  cell_sym_nb<T, 0, 1>(g, i, j, k, cellType, cellScale, R, nb, own, up);
+  cell_sym_nb<T, 1, 0>(g, i, j, k, cellType, cellScale, R, nb, own, up);
+  cell_sym_nb<T, 1, 1>(g, i, j, k, cellType, cellScale, R, nb, own, up);
+}
+
+// Host form of one node row through the symmetric tables (same output rule as
+// node_apply: w * (A w x) on free DOFs, x on constrained ones).
+template <typename T, int CI, int CJ>
+FE_HD void cell_sym_direct(const Grid &g, int i, int j, int kk, bool wantUp, const int32_t *__restrict__ cellType,
+                           const T *__restrict__ cellScale, const T *__restrict__ R, const T *__restrict__ w,
+                           const T *__restrict__ x, T acc[3]) {
+  const int ii = i - 1 + CI, jj = j - 1 + CJ;
+  if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy || kk < 0 || kk >= g.ncz) return;
+  const int64_t c = ((int64_t)ii * g.ncy + jj) * g.ncz + kk;
+  for (int q = 0; q < 2; q++) {
+    const int32_t t = cellType[2 * c + q];
+    if (t < 0) continue;
+    const T sc = cellScale[2 * c + q];
+    T xb[8][3];
+    for (int b = 0; b < 8; b++) {
+      const int64_t gn = ((int64_t)(ii + corner_li(b)) * g.ny + jj + corner_lj(b)) * g.nz + kk + corner_lk(b);
+      for (int p = 0; p < 3; p++) xb[b][p] = w[3 * gn + p] * x[3 * gn + p];
+    }
+    T own[3] = {0, 0, 0}, up[3] = {0, 0, 0};
+    cell_sym_pair<T, 1 - CI, 1 - CJ>(R + 72 * t, xb, sc, own, up);
+    for (int p = 0; p < 3; p++) acc[p] += wantUp ? up[p] : own[p];
+  }
+}
+
+template <typename T>
+FE_HD void node_apply_sym(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
+                          const T *__restrict__ cellScale, const T *__restrict__ R, const T *__restrict__ w,
+                          const T *__restrict__ x, T out[3]) {
+  T own[3] = {0, 0, 0}, up[3] = {0, 0, 0};
+  cell_sym_direct<T, 0, 0>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
+  cell_sym_direct<T, 0, 1>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
+  cell_sym_direct<T, 1, 0>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
+  cell_sym_direct<T, 1, 1>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
+  cell_sym_direct<T, 0, 0>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
+  cell_sym_direct<T, 0, 1>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
+  cell_sym_direct<T, 1, 0>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
+  cell_sym_direct<T, 1, 1>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
+  const int64_t gi = ((int64_t)i * g.ny + j) * g.nz + k;
+  for (int p = 0; p < 3; p++) {
+    const T wi = w[3 * gi + p];
+    out[p] = wi != T(0) ? wi * (own[p] + up[p]) : x[3 * gi + p];
+  }
+}
+
+// Corner-0 rows of one full table (576), made to satisfy the mirror relation
+// R_c[q][p] = s_c[p] s_c[q] R_c[p][q] exactly (the operator stays exactly
+// symmetric), and the largest deviation of the full table from the mirror
+// form, relative to its largest entry.
+FE_HD double sym_table(const double *__restrict__ K, double *__restrict__ R) {
+  for (int c = 0; c < 8; c++) {
+    const double s[3] = {corner_li(c) ? -1.0 : 1.0, corner_lj(c) ? -1.0 : 1.0, corner_lk(c) ? -1.0 : 1.0};
+    for (int p = 0; p < 3; p++) {
+      R[24 * p + 3 * c + p] = K[24 * p + 3 * c + p];
+      for (int q = p + 1; q < 3; q++) {
+        const double v = 0.5 * (K[24 * p + 3 * c + q] + s[p] * s[q] * K[24 * q + 3 * c + p]);
+        R[24 * p + 3 * c + q] = v;
+        R[24 * q + 3 * c + p] = s[p] * s[q] * v;
+      }
+    }
+  }
+  double kmax = 0, err = 0;
+  for (int e = 0; e < 576; e++) kmax = fmax(kmax, fabs(K[e]));
+  for (int a = 0; a < 8; a++) {
+    const double sa[3] = {corner_li(a) ? -1.0 : 1.0, corner_lj(a) ? -1.0 : 1.0, corner_lk(a) ? -1.0 : 1.0};
+    for (int b = 0; b < 8; b++) {
+      const int c = corner_of(corner_li(a) ^ corner_li(b), corner_lj(a) ^ corner_lj(b), corner_lk(a) ^ corner_lk(b));
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) {
+          const double pred = sa[p] * sa[q] * R[24 * p + 3 * c + q];
+          err = fmax(err, fabs(K[72 * a + 24 * p + 3 * b + q] - pred));
+        }
+    }
+  }
+  return kmax > 0 ? err / kmax : 0.0;
+}
+
 }  // namespace fe

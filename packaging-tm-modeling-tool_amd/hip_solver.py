@@ -49,7 +49,7 @@ class FcbgaSolveStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double), ("r0", C.c_double),
                 ("ms_total", C.c_double), ("ms_assemble", C.c_double), ("ms_setup", C.c_double),
                 ("ms_pcg", C.c_double), ("levels", C.c_int32), ("coarsest_dof", C.c_int32), ("ndof", C.c_int64),
-                ("ntypes", C.c_int32), ("reserved", C.c_int32)]
+                ("ntypes", C.c_int32), ("sym_tables", C.c_int32)]
 
 
 EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_step_solve",
@@ -200,7 +200,7 @@ class Model:
         o = self._opts(tol, maxit, precision, precond, sweeps)
         rc = self.L.fcbga_step_solve(self.h, C.byref(step.c), C.byref(o), du.ctypes.data_as(C.c_void_p),
                                      C.byref(st))
-        rec = {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+        rec = {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_}
         if rc != 0 and (check or rc != -3):
             raise SolverError(f"fcbga_step_solve failed ({rc}): {last_error(self.L)}")
         return du, rec
@@ -213,7 +213,7 @@ class Model:
                                             C.byref(st))
         if rc != 0:
             raise SolverError(f"fcbga_step_solve_device failed ({rc}): {last_error(self.L)}")
-        return {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+        return {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_}
 
     def set_kernel_timing(self, enable: bool):
         self.L.fcbga_set_kernel_timing(self.h, 1 if enable else 0)

@@ -40,3 +40,20 @@ def oracle():
     if not os.path.exists(orc.LIB_PATH):
         orc.build()
     return orc
+
+
+def coupled_step(spec):
+    """The spec's step with normal/shear coupling added to group 0 (D stays
+    positive definite): an element without mirror symmetry."""
+    D, s0, born = _meshspec().group_step_data(spec, spec["Ta"], spec["Tb"], 0)
+    D = D.copy()
+    d0 = D[0].reshape(6, 6)
+    d0[0, 3] = d0[3, 0] = 0.1 * min(d0[0, 0], d0[3, 3])
+    assert np.linalg.eigvalsh(d0).min() > 0
+    D[0] = d0.reshape(-1)
+    return D, s0, born
+
+
+def _meshspec():
+    from fcbga_amd import meshspec
+    return meshspec

@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, golden_f64, golden_meta, golden_spec
+from conftest import GOLDEN, coupled_step, golden_f64, golden_meta, golden_spec
 from fcbga_amd import cases, meshspec, post
 from fcbga_amd import hip_solver as hs
 
@@ -86,6 +86,7 @@ def test_cfg2_full_size_vs_reference(lib, prec):
     assert m.ndof == meta["ndof"]
     model = hs.Model(m, variant="hip")
     u, rec = model.solve_step(hs.step_for_spec(spec), tol=1e-10, precision=prec)
+    assert rec["sym_tables"] == 1  # every element type of the lidded mesh is mirror-symmetric
     rows = golden_f64("cfg2_usample.f64.gz").reshape(-1, 4)
     idx = rows[:, 0].astype(np.int64)
     us = u.reshape(-1, 3)[idx]
@@ -96,6 +97,25 @@ def test_cfg2_full_size_vs_reference(lib, prec):
     u2, rec2 = model.solve_step(hs.step_for_spec(spec, Tb=260.0), tol=1e-10, precision=prec)
     ratio = (260.0 - 150.0) / (25.0 - 150.0)
     assert np.abs(u2 - ratio * u).max() <= 1e-6 * np.abs(u2).max()
+
+
+@pytest.mark.parametrize("prec", [hs.PREC_FP64, hs.PREC_MIXED])
+def test_full_element_tables_vs_oracle(oracle, lib, monkeypatch, prec):
+    """The full 576-value element tables (the path for elements without mirror
+    symmetry) on the device: forced by FCBGA_FULL_TABLES, and chosen by the
+    library for a material with normal/shear coupling."""
+    monkeypatch.setenv("FCBGA_FULL_TABLES", "1")
+    spec, m, model, u, rec = _solve("small_box", prec, 1e-12)
+    assert rec["sym_tables"] == 0
+    uref = golden_f64("small_box_u.f64.gz")
+    assert np.abs(u - uref).max() <= 1e-9 * np.abs(uref).max()
+    monkeypatch.delenv("FCBGA_FULL_TABLES")
+    D, s0, born = coupled_step(spec)
+    model = hs.Model(m, variant="hip")
+    u, rec = model.solve_step(hs.Step(spec["Ta"], spec["Tb"], 0, D, s0, born), tol=1e-12, precision=prec)
+    assert rec["sym_tables"] == 0
+    uo, ro = oracle.System(m, D, s0, born).solve(tol=1e-13, maxit=3000)
+    assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
 
 
 def test_error_codes(lib):
