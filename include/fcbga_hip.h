@@ -116,6 +116,24 @@ int fcbga_step_solve_device(fcbga_model *m, const fcbga_step *step, const fcbga_
 int fcbga_step_load(fcbga_model *m, const fcbga_step *step, double *f_out);
 int fcbga_step_apply(fcbga_model *m, const fcbga_step *step, const double *x, double *y_out);
 
+/* Stress histories of homogeneous elements (accumulateStep, src/core/09-assembly.js:494-545;
+ * stepRecovery :480-492).  State per element, FCBGA_STATE doubles: q (33: 24 nodal + 9
+ * incompatible-mode DOFs), then per strain monomial [1, xi, eta, zeta, xi eta, eta zeta,
+ * xi zeta] the stress sum S_m (6 each), then the thermal term S_th (6).  One call adds one
+ * step of the path: du (3*nn, the step's increment in the reference's node order) with
+ * the step's group data; q_only != 0 adds only the DOFs (strain histories).  Elements of
+ * layered sections are rejected. */
+#define FCBGA_STATE 81
+int fcbga_state_accumulate(fcbga_model *m, const fcbga_step *step, const double *du, int64_t nelem,
+                           const int32_t *elems, double *states, int32_t q_only);
+/* Stress and total strain at natural points (stateStress, src/core/09-assembly.js:556-571;
+ * the point() of StateView, src/core/11-analysis.js:356-364): point p lies in element
+ * elems[row[p]] whose state is row row[p] of states; xez (3 per point) are its natural
+ * coordinates; sig and eps (6 per point, Voigt, engineering shear) receive the results
+ * (eps may be NULL). */
+int fcbga_state_stress(fcbga_model *m, int64_t nelem, const int32_t *elems, const double *states, int64_t npts,
+                       const int32_t *row, const double *xez, double *sig, double *eps);
+
 /* Anand viscoplastic material points, batched (AnandPoint.step + commit,
  * src/core/13-fatigue.js:67-143, 229-236).  Per point: mechanical strain eps (6,
  * engineering shear) at temperature T (C) over dt (s), modulus E(T) (MPa).

@@ -160,6 +160,53 @@ void or_element_fth(double hx, double hy, double hz, const double *Sst, const do
   }
 }
 
+/* One step of the stress history of a homogeneous box element, evaluated at
+ * natural points (accumulateStep + stateStress, src/core/09-assembly.js:494-571).
+ * Restated through the strain at each point instead of the monomial sums:
+ * sum_m phi_m D E_m(dq) = D eps(dq; point), with eps from the 11 function
+ * derivatives (boxDerivs, src/core/07-element.js:19-31).  dq = (du at the nodes,
+ * dalpha = K_aa^-1 (f_a - K_au du)); q (33) += dq; when born, sig (6 per point)
+ * += D eps - s0 and eps_out (6 per point, optional) += eps. */
+void or_element_step_points(double hx, double hy, double hz, const double *M, const double *D, const double *s0,
+                            int born, const double *du24, double *q, int npts, const double *xez, double *sig,
+                            double *eps_out) {
+  double K[1089], f[33], Kaa[81], dq[33], rhs[9];
+  or_element_k33(hx, hy, hz, M, K);
+  if (born) or_element_fth(hx, hy, hz, NULL, s0, f);
+  else memset(f, 0, sizeof f);
+  for (int r = 0; r < 9; r++)
+    for (int c = 0; c < 9; c++) Kaa[r * 9 + c] = K[(24 + r) * 33 + 24 + c];
+  or_chol_dense(Kaa, 9, 1e-14);
+  for (int t = 0; t < 24; t++) dq[t] = du24[t];
+  for (int r = 0; r < 9; r++) {
+    double v = f[24 + r];
+    for (int t = 0; t < 24; t++) v -= K[(24 + r) * 33 + t] * dq[t];
+    rhs[r] = v;
+  }
+  or_chol_solve_dense(Kaa, 9, rhs);
+  for (int r = 0; r < 9; r++) dq[24 + r] = rhs[r];
+  for (int t = 0; t < 33; t++) q[t] += dq[t];
+  const double sc[3] = {2 / hx, 2 / hy, 2 / hz};
+  for (int p = 0; p < npts; p++) {
+    double d[3][11], e[6] = {0, 0, 0, 0, 0, 0};
+    nat_derivs(xez[3 * p], xez[3 * p + 1], xez[3 * p + 2], d);
+    for (int b = 0; b < 11; b++) {
+      const double gx = d[0][b] * sc[0], gy = d[1][b] * sc[1], gz = d[2][b] * sc[2];
+      const double ux = dq[3 * b], uy = dq[3 * b + 1], uz = dq[3 * b + 2];
+      e[0] += gx * ux; e[1] += gy * uy; e[2] += gz * uz;
+      e[3] += gy * ux + gx * uy; e[4] += gz * uy + gy * uz; e[5] += gz * ux + gx * uz;
+    }
+    if (eps_out)
+      for (int r = 0; r < 6; r++) eps_out[6 * p + r] += e[r];
+    if (!born) continue;
+    for (int r = 0; r < 6; r++) {
+      double v = -s0[r];
+      for (int c = 0; c < 6; c++) v += D[6 * r + c] * e[c];
+      sig[6 * p + r] += v;
+    }
+  }
+}
+
 /* cholDense with jitter (src/core/01-util.js:70-91) */
 int or_chol_dense(double *A, int n, double rel) {
   double dmax = 0;

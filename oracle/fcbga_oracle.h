@@ -23,6 +23,10 @@ void or_shape_matrices(const double *Wst /*108 or NULL*/, const double *D /*36 o
 void or_element_k33(double hx, double hy, double hz, const double *M, double *K /*1089*/);
 void or_element_fth(double hx, double hy, double hz, const double *Sst /*18 or NULL*/, const double *s0 /*6 or NULL*/,
                     double *f /*33*/);
+/* one step of a homogeneous element's stress history at natural points (src/core/09-assembly.js:494-571) */
+void or_element_step_points(double hx, double hy, double hz, const double *M, const double *D, const double *s0,
+                            int born, const double *du24, double *q, int npts, const double *xez, double *sig,
+                            double *eps_out);
 int or_chol_dense(double *A, int n, double rel);
 void or_chol_solve_dense(const double *L, int n, double *b);
 void or_condense33(const double *K, const double *f /*nullable*/, double *Kc, double *fc, double *X, double *xf);

@@ -83,6 +83,8 @@ def lib():
         L.or_element_fth.argtypes = [C.c_double, C.c_double, C.c_double, _f64p, _f64p, _f64p]
         L.or_condense33.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.or_box_gauss_k33.argtypes = [C.c_double, C.c_double, C.c_double, _f64p, _f64p, _f64p, _f64p]
+        L.or_element_step_points.argtypes = [C.c_double, C.c_double, C.c_double, _f64p, _f64p, _f64p, C.c_int,
+                                             _f64p, _f64p, C.c_int, _f64p, _f64p, _f64p]
         L.or_chol_dense.argtypes = [_f64p, C.c_int, C.c_double]
         L.or_chol_dense.restype = C.c_int
         L.or_fit_plane.argtypes = [_f64p, _f64p, _f64p, C.c_int32, _f64p]
@@ -133,6 +135,20 @@ def condense33(K, f=None):
     ff = None if f is None else f64(f)
     lib().or_condense33(_p(K, _f64p), _p(ff, _f64p), _p(Kc, _f64p), _p(fc, _f64p), _p(X, _f64p), _p(xf, _f64p))
     return Kc.reshape(24, 24), fc, X.reshape(9, 24), xf
+
+
+def element_step_points(hx, hy, hz, M, D, s0, born, du24, q, xez, sig=None, eps=None):
+    """One step of a homogeneous element's stress history at natural points
+    xez (n x 3): q (33) is updated in place; returns sig, eps (n x 6)
+    accumulated onto the given arrays (src/core/09-assembly.js:494-571)."""
+    xez = f64(xez).reshape(-1, 3)
+    n = len(xez)
+    sig = np.zeros((n, 6)) if sig is None else sig
+    eps = np.zeros((n, 6)) if eps is None else eps
+    lib().or_element_step_points(hx, hy, hz, _p(f64(M), _f64p), _p(f64(D), _f64p), _p(f64(s0), _f64p),
+                                 1 if born else 0, _p(f64(du24), _f64p), _p(q, _f64p), n, _p(xez, _f64p),
+                                 _p(sig, _f64p), _p(eps, _f64p))
+    return sig, eps
 
 
 def box_gauss_k33(hx, hy, hz, D, s0=None):

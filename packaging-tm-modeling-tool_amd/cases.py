@@ -60,6 +60,11 @@ def cfg1_spec() -> dict:
         "bga": {"pitch": 1.0, "inset": 1.0, "sx": 40.0, "sy": 40.0},
         "top_plane": 14,
         "tol": 1e-10,
+        # stress recovery checks: die top/bottom surface maps; joint loads of a
+        # 39 x 39 field at 0.5 mm pitch (0.1 mm joints) in the smeared bump layer
+        "stress_die": {"rect": die, "ztop": zs[14], "zb": zs[10], "gkey": "si"},
+        "stress_joints": {"nbx": 39, "nby": 39, "bx0": -9.5, "by0": -9.5, "px": 0.5, "py": 0.5,
+                          "zmid": 0.5 * (zs[8] + zs[10]), "part": PART["UF_BUMP"], "A": math.pi * 0.05 ** 2},
     }
 
 
@@ -93,6 +98,11 @@ def small_box_spec() -> dict:
         "bga": {"pitch": 1.0, "inset": 0.5, "sx": 6.0, "sy": 4.0},
         "top_plane": 3,
         "tol": 1e-13,
+        # stress recovery checks: die surface of part A (with the void corner),
+        # solder-phase joint loads in the superposed layer
+        "stress_die": {"rect": [0.0, 6.0, 0.0, 4.0], "ztop": zs[3], "zb": zs[2], "gkey": "A"},
+        "stress_joints": {"nbx": 6, "nby": 4, "bx0": 0.5, "by0": 0.5, "px": 1.0, "py": 1.0,
+                          "zmid": 0.5 * (zs[1] + zs[2]), "part": PART["SOLDER"], "A": 0.05},
     }
 
 

@@ -177,7 +177,7 @@ struct DevShell {
   DBuf<double> b64, x64, r64;
   DBuf<float> b32, x32, r32;
   // coarsest: dense inverse (FP64)
-  DBuf<double> inv64, band;
+  DBuf<double> inv64, band, jit;
   int32_t ndense = 0;
   int64_t cfTotal = 0;
   int maxm = 0;    // largest column block (6 x aggregates)
@@ -212,6 +212,10 @@ struct fcbga_model {
   // corner-0 rows of the mirror-symmetric element tables (72 per type), used
   // by the fine operator when every type passes the mirror check
   DBuf<double> R64, symErr;
+  // stress recovery (built on first use): grid lines, node ids, cell of each element
+  DBuf<double> d_xs, d_ys, d_zs, postM, postD, postS0, postDu;
+  DBuf<int32_t> d_nodeId, d_eCell;
+  DBuf<uint8_t> postBorn;
   DBuf<float> R32;
   bool symTables = false;
   bool sym_allowed = true;  // FCBGA_FULL_TABLES=1 forces the full tables
@@ -1356,12 +1360,57 @@ __global__ void __launch_bounds__(256) k_dense_matvec(int32_t n, const double *_
   if (lane == 0) x[row] = (T)part;
 }
 
-// one colour of a shell-level column sweep: one 64-lane block per column.
-// Columns hold m = 6 x (1..4) DOFs on the usual meshes, so for m <= 32 the
-// lanes are split into S = 64/m groups: group g takes every S-th block of a
-// row for the residual and every S-th column of the inverse for the
-// correction, and the S partial sums meet in LDS.  Larger columns take one
-// lane per row.  The inverse is stored by columns (read coalesced).
+// One column of a shell-level sweep by one wavefront, m = 6 x aggregates <= 32
+// (the usual case: 1..4 aggregates per column).  The lanes are split into
+// S = 64/m groups: group g takes every S-th block of a row for the residual
+// and every S-th column of the inverse for the correction, and the partial
+// sums meet through lane shuffles (no LDS, no barrier).  The inverse is
+// stored by columns (read coalesced) and is loaded, with b, before the
+// residual so that their latency overlaps it.
+template <typename T>
+__device__ __forceinline__ void shell_column_wave(int32_t c, int lane, const int32_t *__restrict__ cst,
+                                                  const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                  const T *__restrict__ val, const T *__restrict__ inv,
+                                                  const int64_t *__restrict__ cfp, const T *__restrict__ b,
+                                                  T *__restrict__ x) {
+  const int32_t n0 = cst[c], n1 = cst[c + 1];
+  const int m = 6 * (n1 - n0);
+  if (m == 0) return;
+  const T *A = inv + cfp[c];
+  const int S = 64 / m;
+  const int i = lane % m, grp = lane / m;
+  T ainv[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int j = grp + u * S;
+    ainv[u] = (grp < S && j < m) ? A[(int64_t)j * m + i] : T(0);
+  }
+  const T bl = b[6 * (int64_t)n0 + i];
+  T acc = 0;
+  if (grp < S) {
+    const int32_t n = n0 + i / 6;
+    const int p = i % 6;
+    for (int32_t k = rp[n] + grp; k < rp[n + 1]; k += S) {
+      const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
+      acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
+    }
+  }
+  T rr = bl;  // every lane ends with r_i of its row i
+  for (int g = 0; g < S; g++) rr += __shfl(acc, g * m + i, 64);
+  T dx = 0;
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int j = grp + u * S;
+    const T rj = __shfl(rr, j < m ? j : 0, 64);
+    if (grp < S && j < m) dx += ainv[u] * rj;
+  }
+  T d = 0;
+  for (int g = 0; g < S; g++) d += __shfl(dx, g * m + i, 64);
+  if (lane < m) x[6 * (int64_t)n0 + lane] += d;
+}
+
+// one colour of a shell-level column sweep, one 64-lane block per column;
+// colour (ci, cj) holds the columns (ci + 2u, cj + 2v), u-major
 template <typename T>
 __global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, const int32_t *__restrict__ cst,
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
@@ -1369,57 +1418,17 @@ __global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, cons
                                                     const int64_t *__restrict__ cfp, const T *__restrict__ b,
                                                     T *__restrict__ x) {
   __shared__ T r[SHELL_MAXM];
-  __shared__ T part[64];
-  // colour (ci, cj) holds the columns (ci + 2u, cj + 2v), u-major
   const int nyc = (ny - cj + 1) >> 1;
   const int32_t c = (ci + 2 * (int)(blockIdx.x / nyc)) * ny + cj + 2 * (int)(blockIdx.x % nyc);
   const int32_t n0 = cst[c], n1 = cst[c + 1];
   const int m = 6 * (n1 - n0);
-  if (m == 0) return;
   const int lane = threadIdx.x;
-  const T *A = inv + cfp[c];
   if (m <= 32) {
-    const int S = 64 / m;
-    const int i = lane % m, grp = lane / m;
-    // this lane's share of the inverse (columns grp, grp + S, ...: at most 16)
-    // and of b, loaded before the residual so their latency overlaps it
-    T ainv[16];
-    for (int u = 0; u < 16; u++) {
-      const int j = grp + u * S;
-      ainv[u] = (grp < S && j < m) ? A[(int64_t)j * m + i] : T(0);
-    }
-    const T bl = lane < m ? b[6 * (int64_t)n0 + lane] : T(0);
-    T acc = 0;
-    if (grp < S) {
-      const int32_t n = n0 + i / 6;
-      const int p = i % 6;
-      for (int32_t k = rp[n] + grp; k < rp[n + 1]; k += S) {
-        const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
-        acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
-      }
-    }
-    part[lane] = acc;
-    __syncthreads();
-    if (lane < m) {
-      T rr = bl;
-      for (int g = 0; g < S; g++) rr += part[g * m + lane];
-      r[lane] = rr;
-    }
-    __syncthreads();
-    T dx = 0;
-    for (int u = 0; u < 16; u++) {
-      const int j = grp + u * S;
-      if (grp < S && j < m) dx += ainv[u] * r[j];
-    }
-    part[lane] = dx;
-    __syncthreads();
-    if (lane < m) {
-      T d = 0;
-      for (int g = 0; g < S; g++) d += part[g * m + lane];
-      x[6 * (int64_t)n0 + lane] += d;
-    }
+    shell_column_wave<T>(c, lane, cst, rp, col, val, inv, cfp, b, x);
     return;
   }
+  // larger columns: one lane per row
+  const T *A = inv + cfp[c];
   for (int i = lane; i < m; i += 64) {
     const int32_t n = n0 + i / 6;
     const int p = i % 6;
@@ -1437,6 +1446,7 @@ __global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, cons
     x[6 * (int64_t)n0 + i] += acc;
   }
 }
+
 #endif
 
 void band_inverse(int32_t n, int bw, int w, const double *Ab, double *inv, be::stream_t s) {
@@ -1479,31 +1489,97 @@ void dense_matvec(int32_t n, const double *A, const T *b, T *x, be::stream_t s) 
 #endif
 }
 
-// Coarsest level: banded Cholesky on the host with the jitter rule of
-// factorCoarsest (src/core/10-solver.js:562-597), then the explicit inverse on
-// the device (one banded solve per unit vector) so that each cycle applies it
-// as a dense FP64 product.
-void factor_coarsest(fcbga_model *M) {
-  DevShell &D = *M->lv.back();
-  const mgs::Level &L = M->H.L.back();
-  const int32_t na = L.nagg, n = 6 * na;
-  std::vector<double> val((size_t)D.nb * 36);
-  be::d2h(val.data(), D.val64.p, sizeof(double) * val.size(), M->stream);
-  int bwb = 0;
-  for (int32_t i = 0; i < na; i++)
-    for (int32_t k = L.rowPtr[i]; k < L.rowPtr[i + 1]; k++) bwb = std::max(bwb, std::abs(L.col[k] - i));
-  const int bw = (bwb + 1) * 6 - 1, w = bw + 1;
-  std::vector<double> A((size_t)n * w, 0.0);
-  for (int32_t i = 0; i < na; i++)
-    for (int32_t k = L.rowPtr[i]; k < L.rowPtr[i + 1]; k++) {
-      int32_t j = L.col[k];
-      if (j > i) continue;
-      for (int p = 0; p < 6; p++)
-        for (int q = 0; q < 6; q++) {
-          int32_t r = 6 * i + p, c = 6 * j + q;
-          if (c <= r) A[(size_t)r * w + (c - r + bw)] = val[36 * (size_t)k + 6 * p + q];
-        }
+#if FCBGA_GPU
+// Banded Cholesky on the device, right-looking by panels of BAND_P columns
+// (band rows stored [i][k - i + bw]).  k_band_jitter: the jitter of
+// factorCoarsest (src/core/10-solver.js:562-597), rel * max|diag|.
+// k_band_panel (one workgroup): the panel's rows j0 .. j0+P+bw-1 staged in
+// LDS, factored column by column (pivot with the jitter rule, scale, rank-1
+// update of the panel's later columns).  k_band_trail: the rank-P update of
+// the trailing window by the panel.
+constexpr int BAND_P = 32;
+
+__global__ void __launch_bounds__(64) k_band_jitter(int32_t n, int bw, int w, const double *__restrict__ A, double rel,
+                                                    double *__restrict__ jit) {
+  double dm = 0;
+  for (int32_t i = threadIdx.x; i < n; i += 64) dm = fmax(dm, fabs(A[(int64_t)i * w + bw]));
+  for (int off = 32; off > 0; off >>= 1) dm = fmax(dm, __shfl_xor(dm, off, 64));
+  if (threadIdx.x == 0) *jit = rel * (dm != 0 ? dm : 1);
+}
+
+template <int P>
+__global__ void __launch_bounds__(1024) k_band_panel(int32_t n, int bw, int w, double *__restrict__ A, int32_t j0,
+                                                     const double *__restrict__ jit) {
+  extern __shared__ double sm[];
+  const int32_t r1 = j0 + P + bw < n ? j0 + P + bw : n;
+  const int R = r1 - j0;
+  constexpr int LD = P + 1;              // padded row stride (no bank conflicts on column reads)
+  double *Lp = sm, *colS = sm + R * LD;  // panel R x P; scaled column, double-buffered
+  const int tid = threadIdx.x;
+  for (int e = tid; e < R * P; e += 1024) {
+    const int r = e / P, c = e % P;
+    const int32_t i = j0 + r, j = j0 + c;
+    Lp[r * LD + c] = (j <= i && i - j <= bw) ? A[(int64_t)i * w + (j - i + bw)] : 0.0;
+  }
+  __syncthreads();
+  const double jitter = *jit;
+  // thread (row group rg, panel column c); one barrier per column: column jj
+  // is scaled into colS while the later columns take the rank-1 update from
+  // the unscaled entries (scaled on the fly), and column jj-1 is copied back
+  const int c = tid % P, rg = tid / P;
+  constexpr int NRG = 1024 / P;
+  const int Pc = n - j0 < P ? n - j0 : P;  // columns of this panel (the last one may be short)
+  for (int jj = 0; jj < Pc; jj++) {
+    double d = Lp[jj * LD + jj];
+    if (!(d > jitter)) d = fmax(fabs(d), jitter);
+    const double ljj = sqrt(d), inv = 1 / ljj;
+    double *cur = colS + (jj & 1) * R;
+    const double *prev = colS + ((jj + 1) & 1) * R;
+    const double lc = c < Pc ? Lp[c * LD + jj] * inv : 0.0;
+    for (int r = rg; r < R; r += NRG) {
+      if (c == jj - 1) {
+        if (r >= c) Lp[r * LD + c] = prev[r];
+      } else if (c == jj) {
+        if (r >= jj) cur[r] = r == jj ? ljj : Lp[r * LD + jj] * inv;
+      } else if (c > jj && c < Pc && r >= c) {
+        Lp[r * LD + c] -= (Lp[r * LD + jj] * inv) * lc;
+      }
     }
+    __syncthreads();
+  }
+  if (c == Pc - 1)
+    for (int r = rg; r < R; r += NRG)
+      if (r >= c) Lp[r * LD + c] = colS[((Pc - 1) & 1) * R + r];
+  __syncthreads();
+  for (int e = tid; e < R * P; e += 1024) {
+    const int r = e / P, cc = e % P;
+    const int32_t i = j0 + r, j = j0 + cc;
+    if (j <= i && i - j <= bw) A[(int64_t)i * w + (j - i + bw)] = Lp[r * LD + cc];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_band_trail(int32_t n, int bw, int w, double *__restrict__ A, int32_t j0,
+                                                    int P) {
+  const int32_t t0 = j0 + P, t1 = j0 + P + bw < n ? j0 + P + bw : n;
+  const int T = t1 - t0;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (T <= 0 || e >= (int64_t)T * T) return;
+  const int ri = (int)(e / T), rk = (int)(e % T);
+  if (rk > ri) return;
+  const int32_t i = t0 + ri, k = t0 + rk;
+  if (i - k > bw) return;
+  double s = 0;
+  for (int jj = 0; jj < P; jj++) {
+    const int32_t j = j0 + jj;
+    if (i - j > bw) continue;
+    s += A[(int64_t)i * w + (j - i + bw)] * A[(int64_t)k * w + (j - k + bw)];
+  }
+  A[(int64_t)i * w + (k - i + bw)] -= s;
+}
+#endif
+
+// serial banded Cholesky (left-looking) with the jitter rule of factorCoarsest
+void band_factor_host(int32_t n, int bw, int w, double *A) {
   double dmax = 0;
   for (int32_t i = 0; i < n; i++) dmax = std::max(dmax, std::fabs(A[(size_t)i * w + bw]));
   const double jitter = 1e-12 * (dmax != 0 ? dmax : 1);
@@ -1523,10 +1599,76 @@ void factor_coarsest(fcbga_model *M) {
       A[(size_t)i * w + (j - i + bw)] = sum * inv;
     }
   }
-  D.band.upload(A, M->stream);
+}
+
+// Coarsest level: the level's matrix scattered into band storage, banded
+// Cholesky with the jitter rule of factorCoarsest (src/core/10-solver.js:
+// 562-597), then the explicit inverse (one banded solve per unit vector) so
+// that each cycle applies it as a dense FP64 product.  All on the device.
+void factor_coarsest(fcbga_model *M) {
+  DevShell &D = *M->lv.back();
+  const mgs::Level &L = M->H.L.back();
+  const int32_t na = L.nagg, n = 6 * na;
+  int bwb = 0;
+  for (int32_t i = 0; i < na; i++)
+    for (int32_t k = L.rowPtr[i]; k < L.rowPtr[i + 1]; k++) bwb = std::max(bwb, std::abs(L.col[k] - i));
+  const int bw = (bwb + 1) * 6 - 1, w = bw + 1;
+  be::stream_t s = M->stream;
+  D.band.alloc((size_t)n * w);
+  double *A = D.band.p;
+  be::pfor((int64_t)n * w, s, [=] __host__ __device__(int64_t q) { A[q] = 0.0; });
+  {
+    const int32_t *brow = D.blockRow.p, *bcol = D.col.p;
+    const double *val = D.val64.p;
+    be::pfor((int64_t)D.nb * 36, s, [=] __host__ __device__(int64_t e) {
+      const int64_t k = e / 36;
+      const int pq = (int)(e % 36), p = pq / 6, q = pq % 6;
+      const int32_t i = brow[k], j = bcol[k];
+      if (j > i) return;
+      const int32_t r = 6 * i + p, c = 6 * j + q;
+      if (c <= r) A[(int64_t)r * w + (c - r + bw)] = val[e];
+    });
+  }
+#if FCBGA_GPU
+  // panel width: 32 columns when the panel (rows P + bw, plus two scaled
+  // columns) fits 64 KB of LDS, else 16, else the host factor below
+  const auto lds_of = [&](int P) { return (size_t)(P + bw) * (P + 3) * sizeof(double); };
+  const int P = lds_of(32) <= 64 * 1024 ? 32 : lds_of(16) <= 64 * 1024 ? 16 : 0;
+  if (P) {
+    D.jit.alloc(1);
+    hipLaunchKernelGGL(k_band_jitter, dim3(1), dim3(64), 0, s, n, bw, w, A, 1e-12, D.jit.p);
+    BE_CHECK(hipGetLastError());
+    for (int32_t j0 = 0; j0 < n; j0 += P) {
+      const int R = (j0 + P + bw < n ? j0 + P + bw : n) - j0;
+      const size_t lds = (size_t)R * (P + 3) * sizeof(double);
+      if (P == 32)
+        hipLaunchKernelGGL(k_band_panel<32>, dim3(1), dim3(1024), lds, s, n, bw, w, A, j0, D.jit.p);
+      else
+        hipLaunchKernelGGL(k_band_panel<16>, dim3(1), dim3(1024), lds, s, n, bw, w, A, j0, D.jit.p);
+      BE_CHECK(hipGetLastError());
+      const int32_t t0 = j0 + P, t1 = j0 + P + bw < n ? j0 + P + bw : n;
+      if (t1 > t0) {
+        const int64_t T = t1 - t0;
+        hipLaunchKernelGGL(k_band_trail, dim3((unsigned)((T * T + 255) / 256)), dim3(256), 0, s, n, bw, w, A, j0, P);
+        BE_CHECK(hipGetLastError());
+      }
+    }
+    D.inv64.alloc((size_t)n * n);
+    D.ndense = n;
+    band_inverse(n, bw, w, D.band.p, D.inv64.p, s);
+    return;
+  }
+  std::vector<double> Ah((size_t)n * w);
+  be::d2h(Ah.data(), A, Ah.size() * sizeof(double), s);
+  be::sync(s);
+  band_factor_host(n, bw, w, Ah.data());
+  D.band.upload(Ah, s);
+#else
+  band_factor_host(n, bw, w, A);
+#endif
   D.inv64.alloc((size_t)n * n);
   D.ndense = n;
-  band_inverse(n, bw, w, D.band.p, D.inv64.p, M->stream);
+  band_inverse(n, bw, w, D.band.p, D.inv64.p, s);
 }
 
 double mg_setup(fcbga_model *M) {
@@ -1965,6 +2107,127 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
   if (stats) *stats = S;
 }
 
+// ---------------------------------------------------------------------------
+// stress recovery: element stress histories and point stresses
+// ---------------------------------------------------------------------------
+void post_tables(fcbga_model *M) {
+  if (M->d_eCell.n) return;
+  be::stream_t s = M->stream;
+  M->d_xs.upload(M->xs, s);
+  M->d_ys.upload(M->ys, s);
+  M->d_zs.upload(M->zs, s);
+  M->d_nodeId.upload(M->node_id, s);
+  std::vector<int32_t> ecell(M->ne, -1);
+  const int64_t ncells = (int64_t)M->g.ncx * M->g.ncy * M->g.ncz;
+  for (int64_t c = 0; c < ncells; c++)
+    for (int q = 0; q < 2; q++) {
+      const int32_t e = M->cell_elems[2 * c + q];
+      if (e >= 0 && ecell[e] < 0) ecell[e] = (int32_t)c;
+    }
+  M->d_eCell.upload(ecell, s);
+}
+
+void check_elems(const fcbga_model *M, int64_t ne, const int32_t *elems) {
+  for (int64_t t = 0; t < ne; t++) {
+    const int32_t e = elems[t];
+    if (e < 0 || e >= M->ne) throw be::Error("element index out of range");
+    if (M->e_section[e] >= 0) throw be::Error("stress histories of layered-section elements are not supported");
+  }
+}
+
+struct ElemGeom {
+  int i, j, k;
+  double hx, hy, hz;
+};
+
+FE_HD ElemGeom elem_geom(const fe::Grid &g, int32_t c, const double *xs, const double *ys, const double *zs) {
+  ElemGeom o;
+  o.k = c % g.ncz;
+  const int32_t cij = c / g.ncz;
+  o.j = cij % g.ncy;
+  o.i = cij / g.ncy;
+  o.hx = xs[o.i + 1] - xs[o.i];
+  o.hy = ys[o.j + 1] - ys[o.j];
+  o.hz = zs[o.k + 1] - zs[o.k];
+  return o;
+}
+
+// accumulateStep (src/core/09-assembly.js:494-545) for a list of homogeneous
+// elements: states (n x STATE_SIZE) += one step with increment du (3 nn, the
+// reference's node order)
+void state_accumulate(fcbga_model *M, const fcbga_step *st, const double *du, int64_t n, const int32_t *elems,
+                      double *states, bool qOnly) {
+  check_step(M, st);
+  check_elems(M, n, elems);
+  post_tables(M);
+  be::stream_t s = M->stream;
+  std::vector<double> Mg((size_t)M->ngroups * 6 * 1089);
+  for (int32_t q = 0; q < M->ngroups; q++) shape_matrices(nullptr, st->D + 36 * q, Mg.data() + (size_t)6 * 1089 * q);
+  M->postM.upload(Mg, s);
+  M->postD.upload(st->D, 36 * (size_t)M->ngroups, s);
+  M->postS0.upload(st->s0, 6 * (size_t)M->ngroups, s);
+  M->postBorn.upload(st->born, (size_t)M->ngroups, s);
+  M->postDu.upload(du, 3 * (size_t)M->nn, s);
+  DBuf<int32_t> de;
+  DBuf<double> dst;
+  de.upload(elems, (size_t)n, s);
+  dst.upload(states, (size_t)n * fe::STATE_SIZE, s);
+  const fe::Grid g = M->g;
+  const double *xs = M->d_xs.p, *ys = M->d_ys.p, *zs = M->d_zs.p, *Mp = M->postM.p, *Dp = M->postD.p,
+               *S0 = M->postS0.p, *dup = M->postDu.p, *gH = M->tabs.p + 99;
+  const int32_t *ep = de.p, *ecell = M->d_eCell.p, *eg = M->d_egroup.p, *nid = M->d_nodeId.p;
+  const uint8_t *born = M->postBorn.p;
+  double *stp = dst.p;
+  const bool qo = qOnly;
+  be::pfor(n, s, [=] __host__ __device__(int64_t t) {
+    const int32_t e = ep[t];
+    const ElemGeom eg_ = elem_geom(g, ecell[e], xs, ys, zs);
+    const int32_t grp = eg[e];
+    double du8[24];
+    for (int a = 0; a < 8; a++) {
+      const int64_t gi = ((int64_t)(eg_.i + fe::corner_li(a)) * g.ny + eg_.j + fe::corner_lj(a)) * g.nz + eg_.k +
+                         fe::corner_lk(a);
+      const int32_t nd = nid[gi];
+      for (int p = 0; p < 3; p++) du8[3 * a + p] = dup[3 * (int64_t)nd + p];
+    }
+    fe::element_recover(eg_.hx, eg_.hy, eg_.hz, Mp + (int64_t)grp * 6 * 1089, Dp + 36 * grp, S0 + 6 * grp,
+                        born[grp] != 0, qo, gH, du8, stp + (int64_t)t * fe::STATE_SIZE);
+  }, 64);
+  be::d2h(states, dst.p, sizeof(double) * (size_t)n * fe::STATE_SIZE, s);
+}
+
+// stateStress (src/core/09-assembly.js:556-571) at natural points: point p of
+// element elems[row[p]] with state row row[p]
+void state_points(fcbga_model *M, int64_t n, const int32_t *elems, const double *states, int64_t npts,
+                  const int32_t *row, const double *xez, double *sig, double *eps) {
+  check_elems(M, n, elems);
+  for (int64_t p = 0; p < npts; p++)
+    if (row[p] < 0 || row[p] >= n) throw be::Error("point row out of range");
+  post_tables(M);
+  be::stream_t s = M->stream;
+  DBuf<int32_t> de, dr;
+  DBuf<double> dst, dx, dsig, deps;
+  de.upload(elems, (size_t)n, s);
+  dst.upload(states, (size_t)n * fe::STATE_SIZE, s);
+  dr.upload(row, (size_t)npts, s);
+  dx.upload(xez, 3 * (size_t)npts, s);
+  dsig.alloc(6 * (size_t)npts);
+  deps.alloc(6 * (size_t)npts);
+  const fe::Grid g = M->g;
+  const double *xs = M->d_xs.p, *ys = M->d_ys.p, *zs = M->d_zs.p, *stp = dst.p, *xp = dx.p;
+  const int32_t *ep = de.p, *rp = dr.p, *ecell = M->d_eCell.p;
+  double *sp = dsig.p, *epp = deps.p;
+  be::pfor(npts, s, [=] __host__ __device__(int64_t p) {
+    const int32_t r = rp[p];
+    const ElemGeom eg_ = elem_geom(g, ecell[ep[r]], xs, ys, zs);
+    fe::state_stress(eg_.hx, eg_.hy, eg_.hz, stp + (int64_t)r * fe::STATE_SIZE, xp[3 * p], xp[3 * p + 1],
+                     xp[3 * p + 2], sp + 6 * p, epp + 6 * p);
+  });
+  be::d2h(sig, dsig.p, sizeof(double) * 6 * (size_t)npts, s);
+  if (eps) be::d2h(eps, deps.p, sizeof(double) * 6 * (size_t)npts, s);
+  be::sync(s);
+}
+
 int fail(const std::exception &e, int code) {
   g_err = e.what();
   return code;
@@ -2055,6 +2318,37 @@ int fcbga_step_solve_device(fcbga_model *m, const fcbga_step *step, const fcbga_
                             fcbga_solve_stats *stats) {
   if (!du_dev) { g_err = "du_dev is NULL"; return FCBGA_ERR_ARG; }
   return solve_common(m, step, opts, du_dev, nullptr, stats);
+}
+
+int fcbga_state_accumulate(fcbga_model *m, const fcbga_step *step, const double *du, int64_t nelem,
+                           const int32_t *elems, double *states, int32_t q_only) {
+  if (!m || !step || !du || nelem < 0 || (nelem && (!elems || !states))) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  if (!nelem) return FCBGA_OK;
+  try {
+    state_accumulate(m, step, du, nelem, elems, states, q_only != 0);
+  } catch (const be::Error &e) {
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_state_stress(fcbga_model *m, int64_t nelem, const int32_t *elems, const double *states, int64_t npts,
+                       const int32_t *row, const double *xez, double *sig, double *eps) {
+  if (!m || nelem < 0 || npts < 0 || (npts && (!elems || !states || !row || !xez || !sig))) {
+    g_err = "NULL argument";
+    return FCBGA_ERR_ARG;
+  }
+  if (!npts) return FCBGA_OK;
+  try {
+    state_points(m, nelem, elems, states, npts, row, xez, sig, eps);
+  } catch (const be::Error &e) {
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  return FCBGA_OK;
 }
 
 int fcbga_step_load(fcbga_model *m, const fcbga_step *step, double *f_out) {

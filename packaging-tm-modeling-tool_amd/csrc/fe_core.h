@@ -864,4 +864,143 @@ FE_HD double sym_table(const double *__restrict__ K, double *__restrict__ R) {
   return kmax > 0 ? err / kmax : 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// Stress histories of homogeneous box elements.  State (STATE_SIZE doubles):
+// q (33: 24 nodal + 9 incompatible-mode DOFs), then per strain monomial
+// phi = [1, xi, eta, zeta, xi eta, eta zeta, xi zeta] the stress S_m (6),
+// then the thermal term S_th (6).
+// ---------------------------------------------------------------------------
+constexpr int STATE_SIZE = 33 + 6 * 8;
+
+constexpr int HEX_XI_[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+constexpr int HEX_ETA_[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+constexpr int HEX_ZETA_[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+
+// strain of the box element as a polynomial in the natural coordinates
+// (boxStrainPoly, src/core/09-assembly.js:167-195): E[6 m + r]
+FE_HD void box_strain_poly(double hx, double hy, double hz, const double *q, double E[42]) {
+  for (int i = 0; i < 42; i++) E[i] = 0;
+  const double sx = 2 / hx, sy = 2 / hy, sz = 2 / hz;
+  for (int a = 0; a < 8; a++) {
+    const double xa = HEX_XI_[a], ya = HEX_ETA_[a], za = HEX_ZETA_[a];
+    const double ux = q[3 * a], uy = q[3 * a + 1], uz = q[3 * a + 2];
+    const double cx = xa * sx / 8, cy = ya * sy / 8, cz = za * sz / 8;
+    // d/dx: monomials 1, eta, zeta, eta zeta; d/dy: 1, xi, zeta, xi zeta; d/dz: 1, xi, eta, xi eta
+    const int mx[4] = {0, 2, 3, 5}, my[4] = {0, 1, 3, 6}, mz[4] = {0, 1, 2, 4};
+    const double vx[4] = {cx, cx * ya, cx * za, cx * ya * za};
+    const double vy[4] = {cy, cy * xa, cy * za, cy * xa * za};
+    const double vz[4] = {cz, cz * xa, cz * ya, cz * xa * ya};
+    for (int t = 0; t < 4; t++) {
+      E[6 * mx[t]] += vx[t] * ux; E[6 * mx[t] + 3] += vx[t] * uy; E[6 * mx[t] + 5] += vx[t] * uz;
+    }
+    for (int t = 0; t < 4; t++) {
+      E[6 * my[t] + 1] += vy[t] * uy; E[6 * my[t] + 3] += vy[t] * ux; E[6 * my[t] + 4] += vy[t] * uz;
+    }
+    for (int t = 0; t < 4; t++) {
+      E[6 * mz[t] + 2] += vz[t] * uz; E[6 * mz[t] + 4] += vz[t] * uy; E[6 * mz[t] + 5] += vz[t] * ux;
+    }
+  }
+  const double *a1 = q + 24, *a2 = q + 27, *a3 = q + 30;
+  E[6] += -2 * sx * a1[0]; E[9] += -2 * sx * a1[1]; E[11] += -2 * sx * a1[2];
+  E[13] += -2 * sy * a2[1]; E[15] += -2 * sy * a2[0]; E[16] += -2 * sy * a2[2];
+  E[20] += -2 * sz * a3[2]; E[22] += -2 * sz * a3[1]; E[23] += -2 * sz * a3[0];
+}
+
+// strain at (xi, eta, zeta) from the 33 element DOFs (boxDerivs + boxStrainAt,
+// src/core/07-element.js:19-31, 127-142)
+FE_HD void box_strain_at(double hx, double hy, double hz, const double *q, double xi, double eta, double zeta,
+                         double eps[6]) {
+  const double sx = 2 / hx, sy = 2 / hy, sz = 2 / hz;
+  double dx[11], dy[11], dz[11];
+  for (int a = 0; a < 8; a++) {
+    const double xa = HEX_XI_[a], ya = HEX_ETA_[a], za = HEX_ZETA_[a];
+    dx[a] = xa * (1 + ya * eta) * (1 + za * zeta) * 0.125 * sx;
+    dy[a] = ya * (1 + xa * xi) * (1 + za * zeta) * 0.125 * sy;
+    dz[a] = za * (1 + xa * xi) * (1 + ya * eta) * 0.125 * sz;
+  }
+  dx[8] = -2 * xi * sx; dy[8] = 0; dz[8] = 0;
+  dx[9] = 0; dy[9] = -2 * eta * sy; dz[9] = 0;
+  dx[10] = 0; dy[10] = 0; dz[10] = -2 * zeta * sz;
+  for (int r = 0; r < 6; r++) eps[r] = 0;
+  for (int b = 0; b < 11; b++) {
+    const double ux = q[3 * b], uy = q[3 * b + 1], uz = q[3 * b + 2];
+    eps[0] += dx[b] * ux; eps[1] += dy[b] * uy; eps[2] += dz[b] * uz;
+    eps[3] += dy[b] * ux + dx[b] * uy;
+    eps[4] += dz[b] * uy + dy[b] * uz;
+    eps[5] += dz[b] * ux + dx[b] * uz;
+  }
+}
+
+// One step added to the state of a homogeneous element (accumulateStep,
+// src/core/09-assembly.js:494-545): dq = (du at the nodes, dalpha =
+// K_aa^-1 (f_a - K_au du)) with the step's shape matrices M (6 x 1089),
+// q += dq; when born (and not qOnly) S_m += D E_m(dq), S_th += s0.
+// du8 holds the 24 nodal increments; gH the full-box thermal integrals.
+FE_HD void element_recover(double hx, double hy, double hz, const double *__restrict__ M,
+                           const double *__restrict__ D, const double *__restrict__ s0, bool born, bool qOnly,
+                           const double *__restrict__ gH, const double *__restrict__ du8, double *__restrict__ state) {
+  const double V8 = hx * hy * hz / 8;
+  const double c[6] = {V8 * 4 / (hx * hx), V8 * 4 / (hy * hy), V8 * 4 / (hz * hz),
+                       V8 * 4 / (hx * hy), V8 * 4 / (hy * hz), V8 * 4 / (hz * hx)};
+  double Kr[9 * 33];
+  for (int i = 0; i < 9; i++)
+    for (int j = 0; j < 33; j++) {
+      const int o = (24 + i) * 33 + j;
+      Kr[i * 33 + j] = c[0] * M[o] + c[1] * M[1089 + o] + c[2] * M[2178 + o] + c[3] * M[3267 + o] +
+                       c[4] * M[4356 + o] + c[5] * M[5445 + o];
+    }
+  // incompatible-mode rows of the thermal load (elementFth, src/core/09-assembly.js:115-133)
+  double fa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (born) {
+    const int VI[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+    const double h[3] = {hx, hy, hz};
+    for (int r = 0; r < 3; r++) {
+      const double cr = V8 * (2 / h[r]);
+      for (int p = 0; p < 3; p++) {
+        const double sv = cr * s0[VI[p][r]];
+        if (sv == 0) continue;
+        for (int a = 8; a < 11; a++) fa[3 * (a - 8) + p] += sv * gH[r * 11 + a];
+      }
+    }
+  }
+  double dq[33];
+  for (int t = 0; t < 24; t++) dq[t] = du8[t];
+  double Kaa[81];
+  for (int r = 0; r < 9; r++)
+    for (int s = 0; s < 9; s++) Kaa[r * 9 + s] = Kr[r * 33 + 24 + s];
+  chol_dense(Kaa, 9, 1e-14);
+  double col[9];
+  for (int r = 0; r < 9; r++) {
+    double sum = fa[r];
+    for (int t = 0; t < 24; t++) sum -= Kr[r * 33 + t] * dq[t];
+    col[r] = sum;
+  }
+  chol_solve<double, double>(Kaa, 9, col);
+  for (int r = 0; r < 9; r++) dq[24 + r] = col[r];
+  for (int t = 0; t < 33; t++) state[t] += dq[t];
+  if (!born || qOnly) return;
+  double E[42];
+  box_strain_poly(hx, hy, hz, dq, E);
+  for (int mo = 0; mo < 7; mo++)
+    for (int r = 0; r < 6; r++) {
+      double v = 0;
+      for (int t = 0; t < 6; t++) v += D[r * 6 + t] * E[6 * mo + t];
+      state[33 + 6 * mo + r] += v;
+    }
+  for (int r = 0; r < 6; r++) state[33 + 42 + r] += s0[r];
+}
+
+// stress and total strain at a natural point from a state (stateStress,
+// src/core/09-assembly.js:556-571): sigma = sum_m phi_m S_m - S_th
+FE_HD void state_stress(double hx, double hy, double hz, const double *__restrict__ state, double xi, double eta,
+                        double zeta, double sig[6], double eps[6]) {
+  box_strain_at(hx, hy, hz, state, xi, eta, zeta, eps);
+  const double phi[7] = {1, xi, eta, zeta, xi * eta, eta * zeta, xi * zeta};
+  for (int r = 0; r < 6; r++) {
+    double v = -state[33 + 42 + r];
+    for (int mo = 0; mo < 7; mo++) v += phi[mo] * state[33 + 6 * mo + r];
+    sig[r] = v;
+  }
+}
+
 }  // namespace fe

@@ -54,9 +54,11 @@ class FcbgaSolveStats(C.Structure):
 
 EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_step_solve",
            "fcbga_step_solve_device", "fcbga_step_load", "fcbga_step_apply", "fcbga_set_kernel_timing",
-           "fcbga_kernel_times", "fcbga_anand_update", "fcbga_anand_update_device", "fcbga_last_error"]
+           "fcbga_kernel_times", "fcbga_anand_update", "fcbga_anand_update_device", "fcbga_state_accumulate",
+           "fcbga_state_stress", "fcbga_last_error"]
 
 ANAND_STATE = 11
+STATE = 81  # stress-history state of one element (FCBGA_STATE)
 
 
 class FcbgaAnandParams(C.Structure):
@@ -104,6 +106,11 @@ def load(variant: str = "hip"):
                                             C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p]
     L.fcbga_anand_update_device.restype = C.c_int
+    L.fcbga_state_accumulate.argtypes = [C.c_void_p, C.POINTER(FcbgaStep), _f64p, C.c_int64, _i32p, _f64p,
+                                         C.c_int32]
+    L.fcbga_state_accumulate.restype = C.c_int
+    L.fcbga_state_stress.argtypes = [C.c_void_p, C.c_int64, _i32p, _f64p, C.c_int64, _i32p, _f64p, _f64p, _f64p]
+    L.fcbga_state_stress.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.fcbga_last_error.restype = C.c_int
     _loaded[variant] = L
@@ -231,6 +238,42 @@ class Model:
         if rc != 0:
             raise SolverError(last_error(self.L))
         return f
+
+    def state_accumulate(self, step: Step, du, elems, states=None, q_only=False):
+        """Add one step (increment du, 3*nn) to the stress-history states of
+        elements `elems` (accumulateStep, src/core/09-assembly.js:494-545);
+        states (len(elems) x STATE) start at zero when not given."""
+        elems = np.ascontiguousarray(elems, np.int32)
+        du = np.ascontiguousarray(du, np.float64)
+        if du.shape != (self.ndof,):
+            raise SolverError("du must have 3*nn entries")
+        st = np.zeros((len(elems), STATE)) if states is None else np.ascontiguousarray(states, np.float64).copy()
+        rc = self.L.fcbga_state_accumulate(self.h, C.byref(step.c), du.ctypes.data_as(_f64p), len(elems),
+                                           elems.ctypes.data_as(_i32p), st.ctypes.data_as(_f64p),
+                                           1 if q_only else 0)
+        if rc != 0:
+            raise SolverError(last_error(self.L))
+        return st
+
+    def state_stress(self, elems, states, row, xez):
+        """Stress and total strain (npts x 6 each) at natural points xez
+        (npts x 3) of elements elems[row] with states[row] (stateStress,
+        src/core/09-assembly.js:556-571)."""
+        elems = np.ascontiguousarray(elems, np.int32)
+        states = np.ascontiguousarray(states, np.float64)
+        row = np.ascontiguousarray(row, np.int32)
+        xez = np.ascontiguousarray(xez, np.float64).reshape(-1, 3)
+        if states.shape != (len(elems), STATE) or len(row) != len(xez):
+            raise SolverError("state_stress: shape mismatch")
+        sig = np.zeros((len(row), 6))
+        eps = np.zeros((len(row), 6))
+        rc = self.L.fcbga_state_stress(self.h, len(elems), elems.ctypes.data_as(_i32p),
+                                       states.ctypes.data_as(_f64p), len(row), row.ctypes.data_as(_i32p),
+                                       xez.ctypes.data_as(_f64p), sig.ctypes.data_as(_f64p),
+                                       eps.ctypes.data_as(_f64p))
+        if rc != 0:
+            raise SolverError(last_error(self.L))
+        return sig, eps
 
     def apply(self, step: Step, x):
         x = np.ascontiguousarray(x, np.float64)

@@ -1,5 +1,5 @@
 // Golden vectors from the reference JS core (run in the build container only:
-//   node tests/golden/gen_golden.js [/root/reference]
+//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|<spec name> ...]
 // ). It loads src/core/*.js the way the reference's own tools/dev-core.mjs does,
 // builds each committed mesh spec (*_spec.json) with the reference mesher's
 // buildStructuredMesh, assembles one thermal step and solves it with the
@@ -8,6 +8,7 @@
 //   <name>_golden.json                  (solve record, warpage, sizes)
 //   element_golden.json                 (condensed element tables on sampled boxes)
 //   anand_golden.json                   (Anand material-point step sequences)
+//   <name>_stress_golden.json           (element states, point stresses, die surface maps, joint loads)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -169,7 +170,59 @@ function anandCases() {
   console.log('anand points', npt, 'steps', nstep, 'saturation', JSON.stringify(sat));
 }
 
+// Stress recovery on the bare-die mesh (src/core/09-assembly.js:480-572, src/core/11-analysis.js:264-368,
+// src/core/12-post.js:180-309): element states and point stresses, the die top/bottom surface maps and the
+// joint loads of a bump field, from the reference's own step solution.  The view and geometry objects are the
+// minimal ones these functions read (view.model, view.T, view.point; geom.siParts; a bump field record).
+function stressCases(name) {
+  const spec = JSON.parse(fs.readFileSync(path.join(HERE, name + '_spec.json'), 'utf8'));
+  const groups = spec.materials.map(m => ({ name: m.name, key: m.name, kind: 'homog', mat: m.name, stage: m.stage || 0, part: m.part, column: !!m.column }));
+  const mesh = buildStructuredMesh(spec.xs, spec.ys, spec.zs, groups, assignFor(spec));
+  mesh.constraints = constraints321(mesh, spec.constraint_span);
+  if (spec.feature_x && spec.feature_x.length) { mesh.featureX = Int32Array.from(spec.feature_x); mesh.featureY = Int32Array.from(spec.feature_y); }
+  mesh.groupIndex = {};  // as the package mesher records it (src/core/06-mesher.js:529-530)
+  groups.forEach((g, i) => { mesh.groupIndex[g.key] = i; });
+  const model = new FEModel(mesh, { materials: spec.materials.map(testMaterial) });
+  const solver = new LinearSolver(model, { precond: 'mg', tol: 1e-12 });
+  const rhs = new Float64Array(model.ndof);
+  model.assembleStep(spec.Ta, spec.Tb, spec.stage || 0, rhs);
+  const res = solver.solve(rhs, name);
+  const rec = model.stepRecovery(spec.Ta, spec.Tb, spec.stage || 0);
+  const cache = new Map();
+  const state = e => { let st = cache.get(e); if (!st) { st = new Float64Array(model.stateSize(e)); model.accumulateStep(e, rec, res.u, st); cache.set(e, st); } return st; };
+  const view = { model, T: spec.Tb, point: (e, xi, eta, zeta, out) => model.stateStress(e, state(e), xi, eta, zeta, out) };
+  const dieSurfaceStress = get('dieSurfaceStress'), jointLoads = get('jointLoads'), PART = get('PART');
+  const out = { name, T: spec.Tb, elements: [] };
+  // element-level: states and stresses at three natural points of sampled elements
+  const pts = [[-0.577, 0.577, 0.577], [0.3, -0.9, -0.2], [1, 1, -1]];
+  const picks = spec.stress_elements || [0, Math.floor(mesh.ne / 3), Math.floor(mesh.ne / 2), mesh.ne - 1];
+  for (const e of picks) {
+    const st = state(e), o = { eps: new Float64Array(6), sig: new Float64Array(6) };
+    const rows = pts.map(p => { model.stateStress(e, st, p[0], p[1], p[2], o); return { xez: p, sig: list(o.sig), eps: list(o.eps) }; });
+    out.elements.push({ e, state: list(st), points: rows });
+  }
+  if (spec.stress_die) {
+    const d = spec.stress_die;
+    const geom = { siParts: [{ name: 'die', rect: { x0: d.rect[0], x1: d.rect[1], y0: d.rect[2], y1: d.rect[3] }, ztop: d.ztop, zb: d.zb, gkey: d.gkey, part: PART.DIE }] };
+    for (const which of ['top', 'bottom']) {
+      const r = dieSurfaceStress(view, geom, 0, which);
+      out['die_' + which] = { nx: r.nx, ny: r.ny, xs: list(r.xs), ys: list(r.ys), smax: list(r.smax), sxx: list(r.sxx), syy: list(r.syy),
+        peakInterior: r.peakInterior, peakX: r.peakX, peakY: r.peakY, gpPeakInterior: r.gpPeakInterior, gpPeakAll: r.gpPeakAll,
+        corners: r.corners.map(c => c.value), midEdges: r.midEdges.map(c => c.value) };
+    }
+  }
+  if (spec.stress_joints) {
+    const j = spec.stress_joints;
+    const d = { name: 'die', nbx: j.nbx, nby: j.nby, bx0: j.bx0, by0: j.by0, px: j.px, py: j.py, peripheral: false, rows: 0 };
+    const r = jointLoads(view, d, j.zmid, j.part, j.A);
+    out.joints = { N: list(r.N), V: list(r.V), szz: list(r.szz), worstN: r.worstN, worstV: r.worstV, worstNidx: r.worstNidx, worstVidx: r.worstVidx };
+  }
+  fs.writeFileSync(path.join(HERE, name + '_stress_golden.json'), JSON.stringify(out));
+  console.log(name, 'stress golden: elements', out.elements.length, out.die_top ? 'die peak ' + out.die_top.peakInterior : '', out.joints ? 'worst N ' + out.joints.worstN : '');
+}
+
 const what = process.argv.slice(3);
+if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(n);
 if (what.includes('anand')) anandCases();
 if (!what.length || what.includes('element')) elementCases();
 for (const n of ['small_box', 'plate_16', 'cfg1', 'cfg2']) if (!what.length || what.includes(n)) solveSpec(n);
