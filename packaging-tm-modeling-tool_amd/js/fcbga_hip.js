@@ -6,6 +6,8 @@
 //   const solver = new fh.HipStepSolver(mesh, {precision: 'mixed'});
 //   const rec = solver.solveStep(fh.stepFromGroups(Ta, Tb, stage, groups), label);
 //   // rec.u = du of the step, plus the LinearSolver.solve record fields
+//   const view = new fh.HipStateView(solver, feModel, [{step, du: rec.u}], Tb);
+//   dieSurfaceStress(view, geom, 0, 'top');   // the reference's post functions, unchanged
 //
 // install(Analysis) replaces Analysis.prototype.solveStep (src/core/11-analysis.js:189-206)
 // so that assembleStep + LinearSolver.solve run on the GPU: the per-group step
@@ -47,9 +49,51 @@ class HipStepSolver {
     this.log.push({ label, iters: rec.iters, relres: rec.relres, converged: rec.converged, ms: rec.ms, pcMs: rec.pcMs, precond: rec.precond });
     return rec;
   }
+  /** One step of the stress histories of elements (accumulateStep, src/core/09-assembly.js:494-545). */
+  stateAccumulate(step, du, elems, states, qOnly) {
+    return this.hip.stateAccumulate(this.model, step, du, Int32Array.from(elems), states || null, !!qOnly);
+  }
+  /** Stress and strain at natural points of elements[rows] (stateStress, src/core/09-assembly.js:556-571). */
+  stateStress(elems, states, rows, xez) {
+    return this.hip.stateStress(this.model, Int32Array.from(elems), states, Int32Array.from(rows), Float64Array.from(xez));
+  }
   destroy() {
     if (this.model) this.hip.destroyModel(this.model);
     this.model = null;
+  }
+}
+
+const STATE = 81;
+
+/**
+ * Drop-in for the reference's StateView (src/core/11-analysis.js:336-365) in the
+ * post-processing functions, which read view.model, view.T and view.point
+ * (dieSurfaceStress, jointLoads, src/core/12-post.js:180-309): each element's
+ * stress-history state is accumulated on the device over the steps that lead to
+ * T ({step, du} in path order) and cached; prefetch(elems) batches the states of
+ * many elements into one call per step.
+ */
+class HipStateView {
+  constructor(solver, model, steps, T) {
+    this.solver = solver; this.model = model; this.steps = steps; this.T = T;
+    this.cache = new Map();
+  }
+  prefetch(elems) {
+    const need = Array.from(new Set(Array.from(elems))).filter(e => !this.cache.has(e));
+    if (!need.length) return;
+    let st = null;
+    for (const { step, du } of this.steps) st = this.solver.stateAccumulate(step, du, need, st);
+    need.forEach((e, q) => this.cache.set(e, st.slice(q * STATE, (q + 1) * STATE)));
+  }
+  state(e) {
+    if (!this.cache.has(e)) this.prefetch([e]);
+    return this.cache.get(e);
+  }
+  point(e, xi, eta, zeta, out) {
+    out = out || { eps: new Float64Array(6), sig: new Float64Array(6) };
+    const r = this.solver.stateStress([e], this.state(e), [0], [xi, eta, zeta]);
+    out.sig.set(r.sig); out.eps.set(r.eps);
+    return out;
   }
 }
 
@@ -74,4 +118,4 @@ function install(Analysis, opts) {
   return original;
 }
 
-module.exports = { load, HipStepSolver, stepFromGroups, install };
+module.exports = { load, HipStepSolver, HipStateView, stepFromGroups, install, STATE };

@@ -8,6 +8,8 @@
 //     opts: {tol, maxit, precision: 'mixed'|'fp64', precond: 'mg'|'column'|'jacobi', sweeps}
 //     rec:  {u: Float64Array(ndof), iters, relres, converged, ms, assembleMs, pcMs, pcgMs, levels,
 //            precond, rebuilt, warm}  -- the fields of LinearSolver.solve's record (src/core/10-solver.js:988-993)
+//   const st = hip.stateAccumulate(model, step, du, elems[, states[, qOnly]]);  // Float64Array(n * 81)
+//   const {sig, eps} = hip.stateStress(model, elems, st, rows, xez);       // stresses at natural points
 //   hip.destroyModel(model);                   // or left to the garbage collector
 // Argument and device errors throw Error(message); a solve that does not
 // converge returns its record with converged = false (the caller throws, as
@@ -242,11 +244,114 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
   return rec;
 }
 
+// step record {Ta, Tb, stage, D, s0, born} -> fcbga_step (the arrays are borrowed)
+bool read_step(napi_env env, napi_value v, fcbga_step *st, std::string *err) {
+  memset(st, 0, sizeof *st);
+  size_t n;
+  void *p;
+  get_double(env, v, "Ta", &st->Ta);
+  get_double(env, v, "Tb", &st->Tb);
+  get_int(env, v, "stage", &st->stage);
+  if (!get_typed(env, v, "D", napi_float64_array, &p, &n)) { *err = "step.D: Float64Array(ngroups*36)"; return false; }
+  st->D = (const double *)p;
+  if (!get_typed(env, v, "s0", napi_float64_array, &p, &n)) { *err = "step.s0: Float64Array(ngroups*6)"; return false; }
+  st->s0 = (const double *)p;
+  if (!get_typed(env, v, "born", napi_uint8_array, &p, &n)) { *err = "step.born: Uint8Array(ngroups)"; return false; }
+  st->born = (const uint8_t *)p;
+  return true;
+}
+
+// typed array argument (not a property)
+bool arg_typed(napi_env env, napi_value x, napi_typedarray_type want, void **data, size_t *len) {
+  bool is = false;
+  napi_is_typedarray(env, x, &is);
+  if (!is) return false;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, x, &t, len, data, &ab, &off) != napi_ok) return false;
+  return t == want;
+}
+
+// stateAccumulate(model, step, du, elems[, states[, qOnly]]) -> states (Float64Array(n * 81)):
+// one step of the stress histories of elements (accumulateStep, src/core/09-assembly.js:494-545);
+// a given states array is updated in place and returned
+napi_value StateAccumulate(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 4) return throw_msg(env, "stateAccumulate(model, step, du, elems[, states[, qOnly]])");
+  ModelBox *b = unwrap(env, argv[0]);
+  if (!b || !b->m) return throw_msg(env, "stateAccumulate: not a live model");
+  fcbga_step st;
+  std::string err;
+  if (!read_step(env, argv[1], &st, &err)) return throw_msg(env, "stateAccumulate: " + err);
+  void *du, *el;
+  size_t ndu, ne;
+  if (!arg_typed(env, argv[2], napi_float64_array, &du, &ndu) || (int64_t)ndu != fcbga_model_ndof(b->m))
+    return throw_msg(env, "stateAccumulate: du must be a Float64Array(ndof)");
+  if (!arg_typed(env, argv[3], napi_int32_array, &el, &ne)) return throw_msg(env, "stateAccumulate: elems must be an Int32Array");
+  napi_value states;
+  void *sp = nullptr;
+  size_t ns = 0;
+  napi_valuetype t = napi_undefined;
+  if (argc >= 5) napi_typeof(env, argv[4], &t);
+  if (t != napi_undefined && t != napi_null) {
+    if (!arg_typed(env, argv[4], napi_float64_array, &sp, &ns) || ns != ne * FCBGA_STATE)
+      return throw_msg(env, "stateAccumulate: states must be a Float64Array(elems.length * 81)");
+    states = argv[4];
+  } else {
+    napi_value ab;
+    NAPI_OK(napi_create_arraybuffer(env, sizeof(double) * ne * FCBGA_STATE, &sp, &ab));
+    memset(sp, 0, sizeof(double) * ne * FCBGA_STATE);
+    NAPI_OK(napi_create_typedarray(env, napi_float64_array, ne * FCBGA_STATE, ab, 0, &states));
+  }
+  bool qo = false;
+  if (argc >= 6) napi_get_value_bool(env, argv[5], &qo);
+  if (fcbga_state_accumulate(b->m, &st, (const double *)du, (int64_t)ne, (const int32_t *)el, (double *)sp,
+                             qo ? 1 : 0) != FCBGA_OK)
+    return throw_msg(env, "stateAccumulate: " + lib_error());
+  return states;
+}
+
+// stateStress(model, elems, states, rows, xez) -> {sig, eps} (Float64Array(6 * npts) each):
+// stress and strain at natural points (stateStress, src/core/09-assembly.js:556-571)
+napi_value StateStress(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 5) return throw_msg(env, "stateStress(model, elems, states, rows, xez)");
+  ModelBox *b = unwrap(env, argv[0]);
+  if (!b || !b->m) return throw_msg(env, "stateStress: not a live model");
+  void *el, *sp, *rw, *xz;
+  size_t ne, ns, nr, nx;
+  if (!arg_typed(env, argv[1], napi_int32_array, &el, &ne)) return throw_msg(env, "stateStress: elems must be an Int32Array");
+  if (!arg_typed(env, argv[2], napi_float64_array, &sp, &ns) || ns != ne * FCBGA_STATE)
+    return throw_msg(env, "stateStress: states must be a Float64Array(elems.length * 81)");
+  if (!arg_typed(env, argv[3], napi_int32_array, &rw, &nr)) return throw_msg(env, "stateStress: rows must be an Int32Array");
+  if (!arg_typed(env, argv[4], napi_float64_array, &xz, &nx) || nx != 3 * nr)
+    return throw_msg(env, "stateStress: xez must be a Float64Array(3 * rows.length)");
+  napi_value abs, abe, sig, eps, out;
+  void *ps, *pe;
+  NAPI_OK(napi_create_arraybuffer(env, sizeof(double) * 6 * nr, &ps, &abs));
+  NAPI_OK(napi_create_arraybuffer(env, sizeof(double) * 6 * nr, &pe, &abe));
+  if (fcbga_state_stress(b->m, (int64_t)ne, (const int32_t *)el, (const double *)sp, (int64_t)nr, (const int32_t *)rw,
+                         (const double *)xz, (double *)ps, (double *)pe) != FCBGA_OK)
+    return throw_msg(env, "stateStress: " + lib_error());
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, 6 * nr, abs, 0, &sig));
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, 6 * nr, abe, 0, &eps));
+  NAPI_OK(napi_create_object(env, &out));
+  napi_set_named_property(env, out, "sig", sig);
+  napi_set_named_property(env, out, "eps", eps);
+  return out;
+}
+
 }  // namespace
 
 static napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback cb; } fns[] = {
-      {"createModel", CreateModel}, {"destroyModel", DestroyModel}, {"solveStep", SolveStep}};
+      {"createModel", CreateModel}, {"destroyModel", DestroyModel}, {"solveStep", SolveStep},
+      {"stateAccumulate", StateAccumulate}, {"stateStress", StateStress}};
   for (auto &f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||

@@ -1,9 +1,7 @@
 'use strict';
 // Step solve through the Node-API addon for committed mesh specs, checked
-// against the reference's golden displacements.  Builds the mesh contract of
-// buildStructuredMesh (src/core/06-mesher.js:14-72) from a spec the same way
-// packaging-tm-modeling-tool_amd/meshspec.py does, and the per-group step data
-// of T-independent isotropic materials (isoD, src/core/03-materials.js:222-254).
+// against the reference's golden displacements (mesh and step data from
+// spec_mesh.js).
 //   node tests/js/napi_solve.js <spec name> [mixed|fp64]   -> one JSON line
 const fs = require('fs');
 const path = require('path');
@@ -12,72 +10,7 @@ const fh = require('../../packaging-tm-modeling-tool_amd/js/fcbga_hip.js');
 
 const GOLDEN = path.join(__dirname, '..', 'golden');
 
-function buildMesh(spec) {
-  const xs = Float64Array.from(spec.xs), ys = Float64Array.from(spec.ys), zs = Float64Array.from(spec.zs);
-  const nx = xs.length, ny = ys.length, nz = zs.length, ncy = ny - 1, ncz = nz - 1;
-  const ncells = (nx - 1) * ncy * ncz;
-  const cellElems = new Int32Array(2 * ncells).fill(-1);
-  const eGroup = [], ePart = [], eScale = [];
-  const exist = new Uint8Array(nx * ny * nz);
-  const inside = (r, x, y) => x >= r[0] && x <= r[1] && y >= r[2] && y <= r[3];
-  for (let i = 0; i < nx - 1; i++) for (let j = 0; j < ncy; j++) for (let k = 0; k < ncz; k++) {
-    const xc = (xs[i] + xs[i + 1]) / 2, yc = (ys[j] + ys[j + 1]) / 2;
-    let rule = null;
-    for (const r of spec.cells) {
-      if (!(r.k0 <= k && k < r.k1)) continue;
-      if (r.in && !inside(r.in, xc, yc)) continue;
-      if (r.out && inside(r.out, xc, yc)) continue;
-      rule = r; break;
-    }
-    if (!rule) continue;
-    const c = (i * ncy + j) * ncz + k;
-    const list = [[rule.mat, rule.scale == null ? 1 : rule.scale]];
-    if (rule.also != null) list.push([rule.also, rule.also_scale == null ? 1 : rule.also_scale]);
-    list.forEach(([g, sc], q) => { cellElems[2 * c + q] = eGroup.length; eGroup.push(g); ePart.push(spec.materials[g].part); eScale.push(sc); });
-    for (let a = 0; a < 2; a++) for (let b = 0; b < 2; b++) for (let d = 0; d < 2; d++) exist[((i + a) * ny + j + b) * nz + k + d] = 1;
-  }
-  const nodeId = new Int32Array(nx * ny * nz).fill(-1);
-  let nn = 0;
-  for (let q = 0; q < exist.length; q++) if (exist[q]) nodeId[q] = nn++;
-  const nearest = (x, y) => {
-    let best = -1, bd = Infinity;
-    for (let i = 0; i < nx; i++) for (let j = 0; j < ny; j++) {
-      const id = nodeId[(i * ny + j) * nz];
-      if (id < 0) continue;
-      const d = Math.hypot(xs[i] - x, ys[j] - y);
-      if (d < bd - 1e-12) { bd = d; best = id; }
-    }
-    return best;
-  };
-  const a = spec.constraint_span;
-  const n0 = nearest(0, 0), n1 = nearest(a, 0), n2 = nearest(0, a);
-  return {
-    nx, ny, nz, nn, ne: eGroup.length, xs, ys, zs, nodeId, cellElems,
-    eGroup: Int32Array.from(eGroup), ePart: Int8Array.from(ePart), eScale: Float64Array.from(eScale),
-    ngroups: spec.materials.length,
-    constraints: { dofs: Int32Array.from([3 * n0, 3 * n0 + 1, 3 * n0 + 2, 3 * n1 + 1, 3 * n1 + 2, 3 * n2 + 2]) },
-    featureX: Int32Array.from(spec.feature_x || []), featureY: Int32Array.from(spec.feature_y || []),
-  };
-}
-
-function isoD(E, nu) {
-  nu = Math.min(nu, 0.45);
-  const D = new Float64Array(36), f = E / ((1 + nu) * (1 - 2 * nu));
-  for (let i = 0; i < 3; i++) for (let j = 0; j < 3; j++) D[6 * i + j] = f * (i === j ? 1 - nu : nu);
-  D[21] = D[28] = D[35] = E / (2 * (1 + nu));
-  return D;
-}
-
-function groupsFor(spec) {
-  const dT = spec.Tb - spec.Ta;
-  return spec.materials.map(m => {
-    if (m.column) throw new Error('column phase not handled in this check');
-    const D = isoD(m.E, m.nu), born = (m.stage || 0) <= (spec.stage || 0);
-    const s0 = new Float64Array(6), eps = m.cte * 1e-6 * dT;
-    if (born) for (let r = 0; r < 6; r++) s0[r] = (D[6 * r] + D[6 * r + 1] + D[6 * r + 2]) * eps;
-    return { D, s0, born };
-  });
-}
+const { buildMesh, groupsFor } = require('./spec_mesh.js');
 
 const name = process.argv[2] || 'plate_16';
 const precision = process.argv[3] || 'mixed';

@@ -206,3 +206,21 @@ def test_gpu_solve_to_die_peak_within_tenth_percent(prec):
     ref = g["joints"]
     assert abs(r["worstN"] - ref["worstN"]) <= 1e-3 * abs(ref["worstN"])
     assert abs(r["worstV"] - ref["worstV"]) <= 1e-3 * abs(ref["worstV"])
+
+
+@pytest.mark.gpu
+def test_node_addon_state_view():
+    """The JS host path of the stress histories: node -> HipStateView ->
+    Node-API addon -> C ABI -> HIP, against the reference's element states and
+    point stresses."""
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "js", "napi_stress.js")
+    out = subprocess.run([node, script, "cfg1"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    assert rec["points"] == 12
+    assert rec["q_rel_err"] <= 1e-11 and rec["sig_rel_err"] <= 1e-9 and rec["eps_rel_err"] <= 1e-9, rec
