@@ -134,6 +134,31 @@ int fcbga_state_accumulate(fcbga_model *m, const fcbga_step *step, const double 
 int fcbga_state_stress(fcbga_model *m, int64_t nelem, const int32_t *elems, const double *states, int64_t npts,
                        const int32_t *row, const double *xez, double *sig, double *eps);
 
+/* Adjoint sensitivities of one thermal step: the forward/backward pair (autograd.Function
+ * semantics) that replaces the finite-difference tornado of the sensitivity study
+ * (runSensitivityCase / tornadoFromCases, src/core/14-sensitivity.js:8-172).  Objective
+ * components of the step solution u:
+ *   J[0] = sum over the existing nodes of grid plane k_top of (w - mean w)^2  (warpage proxy)
+ *   J[1] = (sum over the 8 Gauss points of the elements vm_elems of sigma_vm^p)^(1/p)
+ *          (peak von Mises as a p-norm; point stresses as stateStress, src/core/09-assembly.js:556-571)
+ * fcbga_adjoint_forward solves the step (as fcbga_step_solve, du_out optional), evaluates J
+ * and keeps the solution on the model as the pair's saved context.  fcbga_adjoint_backward
+ * takes the upstream gradients gJ (2) and returns, from one adjoint PCG solve with the same
+ * operator and preconditioner, the log-derivatives of gJ . J: dlnE and dlnA (ngroups) per
+ * group's E and CTE (the stiffness and thermal stress increment scale with E, the thermal
+ * stress increment with CTE), dlnA_elem (ne, optional) per element's CTE.  Divide by E or
+ * CTE for plain derivatives.  Elements of layered sections are not supported in vm_elems. */
+typedef struct fcbga_objective {
+  int32_t k_top;             /* grid plane of the warpage nodes */
+  int32_t n_vm;              /* elements of the von Mises p-norm (0: J[1] = 0) */
+  const int32_t *vm_elems;
+  double p;                  /* p-norm exponent, >= 1 */
+} fcbga_objective;
+int fcbga_adjoint_forward(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts,
+                          const fcbga_objective *obj, double *J, double *du_out, fcbga_solve_stats *stats);
+int fcbga_adjoint_backward(fcbga_model *m, const double *gJ, double *dlnE, double *dlnA, double *dlnA_elem,
+                           fcbga_solve_stats *stats);
+
 /* Anand viscoplastic material points, batched (AnandPoint.step + commit,
  * src/core/13-fatigue.js:67-143, 229-236).  Per point: mechanical strain eps (6,
  * engineering shear) at temperature T (C) over dt (s), modulus E(T) (MPa).

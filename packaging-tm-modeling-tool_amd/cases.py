@@ -191,3 +191,42 @@ def cfg2_spec() -> dict:
         "temperatures": [25.0, 72.0, 119.0, 166.0, 213.0, 260.0],
         "tol": 1e-10,
     }
+
+
+def cfg4_spec() -> dict:
+    """Adjoint sensitivity workload: the lidded mesh of cfg2_spec with its
+    layers split into 18 material groups (same materials, so the same
+    solution).  Objectives: the warpage proxy on the lid top plane (k = 52) and
+    the von Mises p-norm (p = 8, over Gauss points, unweighted) of the die
+    elements (groups die_lower and die_upper)."""
+    sp = cfg2_spec()
+    die = [-10.0, 10.0, -10.0, 10.0]
+    foot_in = [-19.0, 19.0, -19.0, 19.0]
+
+    def m(base, name):
+        return dict(base, name=name)
+
+    mats = [m(BUILDUP, "buildup_bot_outer"), m(BUILDUP, "buildup_bot_inner"), m(CORE, "core_bot"),
+            m(CORE, "core_mid"), m(CORE, "core_top"), m(BUILDUP, "buildup_top_inner"),
+            m(BUILDUP, "buildup_top_outer"), m(BUMP_UF, "bump_uf_lower"), m(BUMP_UF, "bump_uf_upper"),
+            m(LIDADH, "lid_adh"), m(SI, "die_lower"), m(SI, "die_upper"), m(TIM, "tim"),
+            m(LID, "lid_foot_lower"), m(LID, "lid_foot_upper"), m(LID, "lid_plate_lower"),
+            m(LID, "lid_plate_mid"), m(LID, "lid_plate_upper")]
+    sp.update({
+        "name": "cfg4_lidded_18_groups",
+        "materials": mats,
+        "cells": [
+            {"k0": 0, "k1": 2, "mat": 0}, {"k0": 2, "k1": 4, "mat": 1},
+            {"k0": 4, "k1": 8, "mat": 2}, {"k0": 8, "k1": 12, "mat": 3}, {"k0": 12, "k1": 16, "mat": 4},
+            {"k0": 16, "k1": 18, "mat": 5}, {"k0": 18, "k1": 20, "mat": 6},
+            {"k0": 20, "k1": 22, "mat": 7, "in": die}, {"k0": 22, "k1": 24, "mat": 8, "in": die},
+            {"k0": 20, "k1": 24, "mat": 9, "out": foot_in},
+            {"k0": 24, "k1": 31, "mat": 10, "in": die}, {"k0": 31, "k1": 38, "mat": 11, "in": die},
+            {"k0": 24, "k1": 32, "mat": 13, "out": foot_in}, {"k0": 32, "k1": 40, "mat": 14, "out": foot_in},
+            {"k0": 38, "k1": 40, "mat": 12, "in": die},
+            {"k0": 40, "k1": 44, "mat": 15}, {"k0": 44, "k1": 48, "mat": 16}, {"k0": 48, "k1": 52, "mat": 17},
+        ],
+        "temperatures": [25.0],
+        "objective": {"k_top": 52, "vm_groups": [10, 11], "p": 8.0},
+    })
+    return sp

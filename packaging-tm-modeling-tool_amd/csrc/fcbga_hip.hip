@@ -186,6 +186,21 @@ struct DevShell {
 
 }  // namespace
 
+// saved context of the adjoint forward/backward pair
+struct AdjTape {
+  bool valid = false;
+  uint64_t gen = 0;  // prepGen of the forward's prepared step
+  double Ta = 0, Tb = 0;
+  int32_t stage = 0;
+  std::vector<double> D, s0, secW, secV;
+  std::vector<uint8_t> born, secBorn;
+  fcbga_solve_opts opts{};
+  double p = 8, wmean = 0, S = 0, J[2] = {0, 0};
+  int32_t ntop = 0, nvm = 0;
+  DBuf<double> u, lam, rhs, rhsGrid, work, ones, states, statesTh, sig, sigTh, dJds, gvm, dE, dA, red;
+  DBuf<int32_t> topNodes, vmElems, vmSlot;
+};
+
 struct fcbga_model {
   fe::Grid g{};
   int32_t nn = 0, ne = 0, ngroups = 0, nsec = 0;
@@ -219,6 +234,10 @@ struct fcbga_model {
   DBuf<float> R32;
   bool symTables = false;
   bool sym_allowed = true;  // FCBGA_FULL_TABLES=1 forces the full tables
+  // generation of the prepared step (operator + preconditioner), bumped by every prepare_step
+  uint64_t prepGen = 0;
+  int prepPrecond = -1, prepPrecision = -1;
+  AdjTape tape;
   DBuf<uint8_t> slotHasF, slotLayered, typeHasF, d_born;
   DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64;
   DBuf<float> w32, sinv32, F32;
@@ -2019,17 +2038,21 @@ void alloc_vectors(fcbga_model *M, int precision) {
 #endif
 }
 
-void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op, double *du_dev, double *du_host,
-                fcbga_solve_stats *stats) {
-  check_step(M, st);
+fcbga_solve_opts check_opts(const fcbga_solve_opts *op) {
   fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_FP64, FCBGA_PC_MG, 0};
   if (op) o = *op;
   if (!(o.tol > 0) || o.maxit < 1) throw be::Error("tol must be > 0 and maxit >= 1");
   if (o.precision != FCBGA_PREC_FP64 && o.precision != FCBGA_PREC_MIXED) throw be::Error("unknown precision");
   if (o.precond < FCBGA_PC_MG || o.precond > FCBGA_PC_JACOBI) throw be::Error("unknown preconditioner");
   if (o.sweeps < 0 || o.sweeps > 8) throw be::Error("sweeps must be 0 (default 2) .. 8");
+  return o;
+}
+
+// assembly, column factors and multigrid set-up of a step (the operator and
+// preconditioner every later pcg_run of this step uses)
+void prepare_step(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts &o, fcbga_solve_stats &S) {
+  check_step(M, st);
   M->nsweeps = o.sweeps ? o.sweeps : 2;
-  fcbga_solve_stats S{};
   S.ndof = 3 * (int64_t)M->nn;
   S.ntypes = M->ntypes;
   be::stream_t s = M->stream;
@@ -2048,10 +2071,21 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
   } else {
     S.levels = 1;
   }
+  M->prepGen++;
+  M->prepPrecond = o.precond;
+  M->prepPrecision = o.precision;
+}
+
+// PCG from x = 0 on the prepared step: right-hand side the step's load, or
+// rhs_grid (grid layout, unscaled, zero on constrained DOFs); du_dev receives
+// the solution in the reference's node order
+void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, double *du_dev, double *du_host,
+             fcbga_solve_stats &S) {
+  be::stream_t s = M->stream;
   const double t2 = now_ms();
   const int64_t n3 = 3 * M->g.G;
   double *x = M->x.p, *r = M->r.p, *z = M->z.p, *p = M->p.p;
-  const double *bl = M->bload.p, *sv = M->s64.p;
+  const double *bl = rhs_grid ? rhs_grid : M->bload.p, *sv = M->s64.p;
   double *sc = M->scal.p;
   // scaled load; x = 0, r = b
   be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
@@ -2100,10 +2134,19 @@ void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op
   ktimer_collect(M);
   const double t3 = now_ms();
   S.ms_pcg = t3 - t2;
-  S.ms_total = t3 - t0;
   S.iters = it;
   S.converged = conv ? 1 : 0;
   S.relres = bnorm > 0 ? rnorm / bnorm : 0;
+}
+
+void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op, double *du_dev, double *du_host,
+                fcbga_solve_stats *stats) {
+  const fcbga_solve_opts o = check_opts(op);
+  fcbga_solve_stats S{};
+  const double t0 = now_ms();
+  prepare_step(M, st, o, S);
+  pcg_run(M, o, nullptr, du_dev, du_host, S);
+  S.ms_total = now_ms() - t0;
   if (stats) *stats = S;
 }
 
@@ -2155,11 +2198,9 @@ FE_HD ElemGeom elem_geom(const fe::Grid &g, int32_t c, const double *xs, const d
 // accumulateStep (src/core/09-assembly.js:494-545) for a list of homogeneous
 // elements: states (n x STATE_SIZE) += one step with increment du (3 nn, the
 // reference's node order)
-void state_accumulate(fcbga_model *M, const fcbga_step *st, const double *du, int64_t n, const int32_t *elems,
-                      double *states, bool qOnly) {
-  check_step(M, st);
-  check_elems(M, n, elems);
-  post_tables(M);
+// per-group shape matrices, stiffness, thermal stress increment and birth of a
+// step on the device (the data of stepRecovery, src/core/09-assembly.js:480-492)
+void upload_step_groups(fcbga_model *M, const fcbga_step *st) {
   be::stream_t s = M->stream;
   std::vector<double> Mg((size_t)M->ngroups * 6 * 1089);
   for (int32_t q = 0; q < M->ngroups; q++) shape_matrices(nullptr, st->D + 36 * q, Mg.data() + (size_t)6 * 1089 * q);
@@ -2167,19 +2208,21 @@ void state_accumulate(fcbga_model *M, const fcbga_step *st, const double *du, in
   M->postD.upload(st->D, 36 * (size_t)M->ngroups, s);
   M->postS0.upload(st->s0, 6 * (size_t)M->ngroups, s);
   M->postBorn.upload(st->born, (size_t)M->ngroups, s);
-  M->postDu.upload(du, 3 * (size_t)M->nn, s);
-  DBuf<int32_t> de;
-  DBuf<double> dst;
-  de.upload(elems, (size_t)n, s);
-  dst.upload(states, (size_t)n * fe::STATE_SIZE, s);
+}
+
+// one step added to the states of n elements (device arrays; du in the
+// reference's node order) with the group data of upload_step_groups
+void recover_states(fcbga_model *M, int64_t n, const int32_t *elems_dev, const double *du_dev, double *states_dev,
+                    bool qOnly) {
+  post_tables(M);
   const fe::Grid g = M->g;
   const double *xs = M->d_xs.p, *ys = M->d_ys.p, *zs = M->d_zs.p, *Mp = M->postM.p, *Dp = M->postD.p,
-               *S0 = M->postS0.p, *dup = M->postDu.p, *gH = M->tabs.p + 99;
-  const int32_t *ep = de.p, *ecell = M->d_eCell.p, *eg = M->d_egroup.p, *nid = M->d_nodeId.p;
+               *S0 = M->postS0.p, *dup = du_dev, *gH = M->tabs.p + 99;
+  const int32_t *ep = elems_dev, *ecell = M->d_eCell.p, *eg = M->d_egroup.p, *nid = M->d_nodeId.p;
   const uint8_t *born = M->postBorn.p;
-  double *stp = dst.p;
+  double *stp = states_dev;
   const bool qo = qOnly;
-  be::pfor(n, s, [=] __host__ __device__(int64_t t) {
+  be::pfor(n, M->stream, [=] __host__ __device__(int64_t t) {
     const int32_t e = ep[t];
     const ElemGeom eg_ = elem_geom(g, ecell[e], xs, ys, zs);
     const int32_t grp = eg[e];
@@ -2188,11 +2231,29 @@ void state_accumulate(fcbga_model *M, const fcbga_step *st, const double *du, in
       const int64_t gi = ((int64_t)(eg_.i + fe::corner_li(a)) * g.ny + eg_.j + fe::corner_lj(a)) * g.nz + eg_.k +
                          fe::corner_lk(a);
       const int32_t nd = nid[gi];
-      for (int p = 0; p < 3; p++) du8[3 * a + p] = dup[3 * (int64_t)nd + p];
+      for (int p = 0; p < 3; p++) du8[3 * a + p] = dup ? dup[3 * (int64_t)nd + p] : 0.0;
     }
     fe::element_recover(eg_.hx, eg_.hy, eg_.hz, Mp + (int64_t)grp * 6 * 1089, Dp + 36 * grp, S0 + 6 * grp,
                         born[grp] != 0, qo, gH, du8, stp + (int64_t)t * fe::STATE_SIZE);
   }, 64);
+}
+
+// accumulateStep (src/core/09-assembly.js:494-545) for a list of homogeneous
+// elements: states (n x STATE_SIZE) += one step with increment du (3 nn, the
+// reference's node order)
+void state_accumulate(fcbga_model *M, const fcbga_step *st, const double *du, int64_t n, const int32_t *elems,
+                      double *states, bool qOnly) {
+  check_step(M, st);
+  check_elems(M, n, elems);
+  post_tables(M);
+  be::stream_t s = M->stream;
+  upload_step_groups(M, st);
+  M->postDu.upload(du, 3 * (size_t)M->nn, s);
+  DBuf<int32_t> de;
+  DBuf<double> dst;
+  de.upload(elems, (size_t)n, s);
+  dst.upload(states, (size_t)n * fe::STATE_SIZE, s);
+  recover_states(M, n, de.p, M->postDu.p, dst.p, qOnly);
   be::d2h(states, dst.p, sizeof(double) * (size_t)n * fe::STATE_SIZE, s);
 }
 
@@ -2226,6 +2287,370 @@ void state_points(fcbga_model *M, int64_t n, const int32_t *elems, const double 
   be::d2h(sig, dsig.p, sizeof(double) * 6 * (size_t)npts, s);
   if (eps) be::d2h(eps, deps.p, sizeof(double) * 6 * (size_t)npts, s);
   be::sync(s);
+}
+
+#if FCBGA_GPU
+// per-group sums of two element arrays, one block per group (fixed order:
+// deterministic)
+__global__ void __launch_bounds__(256) k_group_sum(int64_t ne, const int32_t *__restrict__ eg,
+                                                   const double *__restrict__ a, const double *__restrict__ b,
+                                                   int ng, double *__restrict__ out) {
+  __shared__ double sa[256], sb[256];
+  const int grp = blockIdx.x, tid = threadIdx.x;
+  double x = 0, y = 0;
+  for (int64_t e = tid; e < ne; e += 256)
+    if (eg[e] == grp) {
+      x += a[e];
+      y += b[e];
+    }
+  sa[tid] = x;
+  sb[tid] = y;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off) {
+      sa[tid] += sa[tid + off];
+      sb[tid] += sb[tid + off];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[grp] = sa[0];
+    out[ng + grp] = sb[0];
+  }
+}
+#endif
+
+void group_sums(fcbga_model *M, const double *a, const double *b, double *outA, double *outB) {
+  const int ng = M->ngroups;
+  std::vector<double> h(2 * (size_t)ng, 0.0);
+#if FCBGA_GPU
+  DBuf<double> d;
+  d.alloc(2 * (size_t)ng);
+  hipLaunchKernelGGL(k_group_sum, dim3(ng), dim3(256), 0, M->stream, (int64_t)M->ne, M->d_egroup.p, a, b, ng, d.p);
+  BE_CHECK(hipGetLastError());
+  be::d2h(h.data(), d.p, sizeof(double) * h.size(), M->stream);
+  be::sync(M->stream);
+#else
+  for (int64_t e = 0; e < M->ne; e++) {
+    h[M->e_group[e]] += a[e];
+    h[ng + M->e_group[e]] += b[e];
+  }
+#endif
+  for (int q = 0; q < ng; q++) {
+    outA[q] = h[q];
+    outB[q] = h[ng + q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint sensitivities: forward (solve + objective) and backward (one adjoint
+// PCG solve + per-element contractions)
+// ---------------------------------------------------------------------------
+double dev_dot(fcbga_model *M, int64_t n, const double *a, const double *b) {
+  AdjTape &T = M->tape;
+  T.red.alloc(1);
+  be::dot_async(n, a, b, M->partials.p, T.red.p, M->stream);
+  double v;
+  be::d2h(&v, T.red.p, sizeof(double), M->stream);
+  be::sync(M->stream);
+  return v;
+}
+
+void fill(double *v, int64_t n, double x, be::stream_t s) {
+  be::pfor(n, s, [=] __host__ __device__(int64_t i) { v[i] = x; });
+}
+
+fcbga_step tape_step(const AdjTape &T) {
+  fcbga_step st{};
+  st.Ta = T.Ta; st.Tb = T.Tb; st.stage = T.stage;
+  st.D = T.D.data(); st.s0 = T.s0.data(); st.born = T.born.data();
+  st.secW = T.secW.empty() ? nullptr : T.secW.data();
+  st.secV = T.secV.empty() ? nullptr : T.secV.data();
+  st.sec_born = T.secBorn.empty() ? nullptr : T.secBorn.data();
+  return st;
+}
+
+constexpr double GPN = 0.57735026918962576451;  // 1 / sqrt(3)
+
+// Gauss points of the box element, ordered like the nodes (src/core/12-post.js:201-205)
+FE_HD void gauss_point(int g, double *x) {
+  x[0] = fe::HEX_XI_[g] * GPN; x[1] = fe::HEX_ETA_[g] * GPN; x[2] = fe::HEX_ZETA_[g] * GPN;
+}
+
+void adjoint_forward(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op, const fcbga_objective *obj,
+                     double *J, double *du_host, fcbga_solve_stats *stats) {
+  if (!obj) throw be::Error("objective is NULL");
+  if (obj->k_top < 0 || obj->k_top >= M->g.nz) throw be::Error("k_top outside the grid");
+  if (obj->n_vm < 0 || (obj->n_vm && !obj->vm_elems)) throw be::Error("vm_elems missing");
+  if (obj->n_vm && !(obj->p >= 1)) throw be::Error("p-norm exponent must be >= 1");
+  check_elems(M, obj->n_vm, obj->vm_elems);
+  const fcbga_solve_opts o = check_opts(op);
+  AdjTape &T = M->tape;
+  T.valid = false;
+  be::stream_t s = M->stream;
+  fcbga_solve_stats S{};
+  const double t0 = now_ms();
+  prepare_step(M, st, o, S);
+  T.u.alloc(3 * (size_t)M->nn);
+  pcg_run(M, o, nullptr, T.u.p, du_host, S);
+  S.ms_total = now_ms() - t0;
+  // the step, to re-prepare the same operator in backward if another solve ran in between
+  T.Ta = st->Ta; T.Tb = st->Tb; T.stage = st->stage;
+  T.D.assign(st->D, st->D + 36 * (size_t)M->ngroups);
+  T.s0.assign(st->s0, st->s0 + 6 * (size_t)M->ngroups);
+  T.born.assign(st->born, st->born + M->ngroups);
+  T.secW.clear(); T.secV.clear(); T.secBorn.clear();
+  if (M->nsec) {
+    T.secW.assign(st->secW, st->secW + 108 * (size_t)M->nsec);
+    T.secV.assign(st->secV, st->secV + 18 * (size_t)M->nsec);
+    T.secBorn.assign(st->sec_born, st->sec_born + M->nsec);
+  }
+  T.opts = o;
+  T.gen = M->prepGen;
+  T.p = obj->p;
+  // warpage proxy: nodes of plane k_top in (i, j) order
+  std::vector<int32_t> top;
+  for (int i = 0; i < M->g.nx; i++)
+    for (int j = 0; j < M->g.ny; j++) {
+      const int32_t nd = M->node_id[((int64_t)i * M->g.ny + j) * M->g.nz + obj->k_top];
+      if (nd >= 0) top.push_back(nd);
+    }
+  if (top.empty()) throw be::Error("no nodes on plane k_top");
+  T.ntop = (int32_t)top.size();
+  T.topNodes.upload(top, s);
+  T.work.alloc(std::max<size_t>(top.size(), 8 * (size_t)std::max(obj->n_vm, 1)));
+  T.ones.alloc(T.work.n);
+  fill(T.ones.p, (int64_t)T.ones.n, 1.0, s);
+  {
+    const int32_t *tn = T.topNodes.p;
+    const double *u = T.u.p;
+    double *w = T.work.p;
+    be::pfor(T.ntop, s, [=] __host__ __device__(int64_t t) { w[t] = u[3 * (int64_t)tn[t] + 2]; });
+    T.wmean = dev_dot(M, T.ntop, w, T.ones.p) / T.ntop;
+    const double wm = T.wmean;
+    be::pfor(T.ntop, s, [=] __host__ __device__(int64_t t) { w[t] -= wm; });
+    T.J[0] = dev_dot(M, T.ntop, w, w);
+  }
+  // die von Mises p-norm over the Gauss points of vm_elems
+  T.nvm = obj->n_vm;
+  T.J[1] = 0;
+  T.S = 0;
+  if (T.nvm) {
+    const int64_t n = T.nvm;
+    T.vmElems.upload(obj->vm_elems, (size_t)n, s);
+    std::vector<int32_t> slot(M->ne, -1);
+    for (int64_t q = 0; q < n; q++) {
+      if (slot[obj->vm_elems[q]] >= 0) throw be::Error("vm_elems repeats an element");
+      slot[obj->vm_elems[q]] = (int32_t)q;
+    }
+    T.vmSlot.upload(slot, s);
+    upload_step_groups(M, st);
+    T.states.alloc(n * fe::STATE_SIZE);
+    T.statesTh.alloc(n * fe::STATE_SIZE);
+    fill(T.states.p, n * fe::STATE_SIZE, 0.0, s);
+    fill(T.statesTh.p, n * fe::STATE_SIZE, 0.0, s);
+    recover_states(M, n, T.vmElems.p, T.u.p, T.states.p, false);
+    recover_states(M, n, T.vmElems.p, nullptr, T.statesTh.p, false);
+    T.sig.alloc(n * 48);
+    T.sigTh.alloc(n * 48);
+    const fe::Grid g = M->g;
+    const double *xs = M->d_xs.p, *ys = M->d_ys.p, *zs = M->d_zs.p, *stp = T.states.p, *stth = T.statesTh.p;
+    const int32_t *ep = T.vmElems.p, *ecell = M->d_eCell.p;
+    double *sg = T.sig.p, *sgt = T.sigTh.p, *w = T.work.p;
+    const double p = T.p;
+    be::pfor(n * 8, s, [=] __host__ __device__(int64_t q) {
+      const int64_t t = q / 8;
+      const int gpi = (int)(q % 8);
+      const ElemGeom eg_ = elem_geom(g, ecell[ep[t]], xs, ys, zs);
+      double x[3], eps[6];
+      gauss_point(gpi, x);
+      fe::state_stress(eg_.hx, eg_.hy, eg_.hz, stp + t * fe::STATE_SIZE, x[0], x[1], x[2], sg + 6 * q, eps);
+      fe::state_stress(eg_.hx, eg_.hy, eg_.hz, stth + t * fe::STATE_SIZE, x[0], x[1], x[2], sgt + 6 * q, eps);
+      w[q] = pow(fe::von_mises(sg + 6 * q), p);
+    });
+    T.S = dev_dot(M, n * 8, w, T.ones.p);
+    T.J[1] = T.S > 0 ? pow(T.S, 1.0 / p) : 0.0;
+  }
+  T.valid = true;
+  J[0] = T.J[0];
+  J[1] = T.J[1];
+  if (stats) *stats = S;
+}
+
+void adjoint_backward(fcbga_model *M, const double *gJ, double *dlnE, double *dlnA, double *dlnAe,
+                      fcbga_solve_stats *stats) {
+  AdjTape &T = M->tape;
+  if (!T.valid) throw be::Error("adjoint backward without a forward on this model");
+  be::stream_t s = M->stream;
+  fcbga_solve_stats S{};
+  const double t0 = now_ms();
+  const fcbga_step st = tape_step(T);
+  if (M->prepGen != T.gen) {  // another solve replaced the operator: rebuild the forward's
+    prepare_step(M, &st, T.opts, S);
+    T.gen = M->prepGen;
+  }
+  const int64_t nn = M->nn, n3 = 3 * nn;
+  T.rhs.alloc(n3);
+  fill(T.rhs.p, n3, 0.0, s);
+  // dJ_w/du: 2 (w - mean w) on the z components of the plane's nodes
+  {
+    const int32_t *tn = T.topNodes.p;
+    const double *u = T.u.p;
+    double *r = T.rhs.p;
+    const double wm = T.wmean, g0 = gJ[0];
+    be::pfor(T.ntop, s, [=] __host__ __device__(int64_t t) {
+      const int64_t nd = tn[t];
+      r[3 * nd + 2] = g0 * 2 * (u[3 * nd + 2] - wm);
+    });
+  }
+  const int64_t nvm = T.nvm;
+  const bool vm = nvm > 0 && gJ[1] != 0 && T.S > 0;
+  if (vm) {
+    upload_step_groups(M, &st);
+    post_tables(M);
+    // dJ_vm/dsigma at each Gauss point: S^(1/p - 1) sigma_vm^(p-1) dsigma_vm/dsigma
+    T.dJds.alloc(nvm * 48);
+    {
+      const double *sg = T.sig.p;
+      double *d = T.dJds.p;
+      const double p = T.p, fac = gJ[1] * pow(T.S, 1.0 / p - 1.0);
+      be::pfor(nvm * 8, s, [=] __host__ __device__(int64_t q) {
+        const double *x = sg + 6 * q;
+        const double v = fe::von_mises(x);
+        double *o = d + 6 * q;
+        if (!(v > 0)) {
+          for (int c = 0; c < 6; c++) o[c] = 0;
+          return;
+        }
+        const double f = fac * pow(v, p - 1) / v;
+        o[0] = f * (2 * x[0] - x[1] - x[2]) * 0.5;
+        o[1] = f * (2 * x[1] - x[2] - x[0]) * 0.5;
+        o[2] = f * (2 * x[2] - x[0] - x[1]) * 0.5;
+        o[3] = f * 3 * x[3];
+        o[4] = f * 3 * x[4];
+        o[5] = f * 3 * x[5];
+      });
+    }
+    // per element: dJ/du_e (24)
+    T.gvm.alloc(nvm * 24);
+    {
+      const fe::Grid g = M->g;
+      const double *xs = M->d_xs.p, *ys = M->d_ys.p, *zs = M->d_zs.p, *Mp = M->postM.p, *Dp = M->postD.p,
+                   *d = T.dJds.p;
+      const int32_t *ep = T.vmElems.p, *ecell = M->d_eCell.p, *eg = M->d_egroup.p;
+      double *gv = T.gvm.p;
+      be::pfor(nvm, s, [=] __host__ __device__(int64_t t) {
+        const int32_t e = ep[t];
+        const ElemGeom eg_ = elem_geom(g, ecell[e], xs, ys, zs);
+        const int32_t grp = eg[e];
+        double xez[24];
+        for (int q = 0; q < 8; q++) gauss_point(q, xez + 3 * q);
+        fe::element_stress_pullback(eg_.hx, eg_.hy, eg_.hz, Mp + (int64_t)grp * 6 * 1089, Dp + 36 * grp, 8, xez,
+                                    d + 48 * t, gv + 24 * t);
+      }, 64);
+    }
+    // gathered by node (fixed order over the node's cells: deterministic)
+    {
+      const fe::Grid g = M->g;
+      const int64_t *gof = M->gridOf.p;
+      const int32_t *ce = M->cellElem.p, *slot = T.vmSlot.p;
+      const double *gv = T.gvm.p;
+      double *r = T.rhs.p;
+      be::pfor(nn, s, [=] __host__ __device__(int64_t nd) {
+        const int64_t gi = gof[nd];
+        const int k = (int)(gi % g.nz);
+        const int64_t cij = gi / g.nz;
+        const int j = (int)(cij % g.ny), i = (int)(cij / g.ny);
+        double acc[3] = {0, 0, 0};
+        for (int ci = i - 1; ci <= i; ci++)
+          for (int cj = j - 1; cj <= j; cj++)
+            for (int ck = k - 1; ck <= k; ck++) {
+              if (ci < 0 || cj < 0 || ck < 0 || ci >= g.ncx || cj >= g.ncy || ck >= g.ncz) continue;
+              const int64_t c = ((int64_t)ci * g.ncy + cj) * g.ncz + ck;
+              for (int q = 0; q < 2; q++) {
+                const int32_t e = ce[2 * c + q];
+                if (e < 0 || slot[e] < 0) continue;
+                const int a = fe::corner_of(i - ci, j - cj, k - ck);
+                for (int p = 0; p < 3; p++) acc[p] += gv[24 * (int64_t)slot[e] + 3 * a + p];
+              }
+            }
+        for (int p = 0; p < 3; p++) r[3 * nd + p] += acc[p];
+      });
+    }
+  }
+  // adjoint solve K lambda = dJ/du with the forward's operator and preconditioner
+  {
+    const int64_t G3 = 3 * M->g.G;
+    T.rhsGrid.alloc(G3);
+    fill(T.rhsGrid.p, G3, 0.0, s);
+    const int64_t *gof = M->gridOf.p;
+    const uint8_t *mask = M->dofmask.p;
+    const double *r = T.rhs.p;
+    double *rg = T.rhsGrid.p;
+    be::pfor(nn, s, [=] __host__ __device__(int64_t nd) {
+      const int64_t gi = gof[nd];
+      for (int p = 0; p < 3; p++) rg[3 * gi + p] = mask[3 * gi + p] ? r[3 * nd + p] : 0.0;
+    });
+    T.lam.alloc(n3);
+    pcg_run(M, T.opts, T.rhsGrid.p, T.lam.p, nullptr, S);
+  }
+  // per element: d/dlnE = lambda.(f_e - K_e u_e), d/dlnCTE = lambda.f_e (K_e and f_e scale with E,
+  // f_e with CTE; the element tables are the step's condensed ones)
+  post_tables(M);
+  const int64_t ne = M->ne;
+  T.dE.alloc(ne);
+  T.dA.alloc(ne);
+  {
+    const fe::Grid g = M->g;
+    const int32_t *ecell = M->d_eCell.p, *ce = M->cellElem.p, *ct = M->cellType.p, *nid = M->d_nodeId.p;
+    const double *cs = M->cellScale64.p, *Kc = M->Kc64.p, *fc = M->fc64.p, *u = T.u.p, *lam = T.lam.p;
+    double *dE = T.dE.p, *dA = T.dA.p;
+    be::pfor(ne, s, [=] __host__ __device__(int64_t e) {
+      const int32_t c = ecell[e];
+      const int q = ce[2 * (int64_t)c] == (int32_t)e ? 0 : 1;
+      const int32_t t = ct[2 * (int64_t)c + q];
+      const double sc = cs[2 * (int64_t)c + q];
+      const int k = c % g.ncz;
+      const int32_t cij = c / g.ncz;
+      const int j = cij % g.ncy, i = cij / g.ncy;
+      double ue[24], le[24];
+      for (int a = 0; a < 8; a++) {
+        const int64_t gi = ((int64_t)(i + fe::corner_li(a)) * g.ny + j + fe::corner_lj(a)) * g.nz + k + fe::corner_lk(a);
+        const int64_t nd = nid[gi];
+        for (int p = 0; p < 3; p++) { ue[3 * a + p] = u[3 * nd + p]; le[3 * a + p] = lam[3 * nd + p]; }
+      }
+      const double *K = Kc + 576 * (int64_t)t, *f = fc + 24 * (int64_t)t;
+      double ku = 0, lf = 0;
+      for (int r = 0; r < 24; r++) {
+        double row = 0;
+        for (int q2 = 0; q2 < 24; q2++) row += K[24 * r + q2] * ue[q2];
+        ku += le[r] * row;
+        lf += le[r] * f[r];
+      }
+      dE[e] = sc * (lf - ku);
+      dA[e] = sc * lf;
+    });
+  }
+  // explicit terms of the p-norm at fixed u: dsigma/dlnE = sigma, dsigma/dlnCTE = sigma at u = 0
+  if (vm) {
+    const int32_t *ep = T.vmElems.p;
+    const double *d = T.dJds.p, *sg = T.sig.p, *sgt = T.sigTh.p;
+    double *dE = T.dE.p, *dA = T.dA.p;
+    be::pfor(nvm, s, [=] __host__ __device__(int64_t t) {
+      double aE = 0, aA = 0;
+      for (int q = 0; q < 48; q++) {
+        aE += d[48 * t + q] * sg[48 * t + q];
+        aA += d[48 * t + q] * sgt[48 * t + q];
+      }
+      dE[ep[t]] += aE;
+      dA[ep[t]] += aA;
+    });
+  }
+  group_sums(M, T.dE.p, T.dA.p, dlnE, dlnA);
+  if (dlnAe) be::d2h(dlnAe, T.dA.p, sizeof(double) * (size_t)ne, s);
+  be::sync(s);
+  S.ms_total = now_ms() - t0;
+  if (stats) *stats = S;
 }
 
 int fail(const std::exception &e, int code) {
@@ -2318,6 +2743,44 @@ int fcbga_step_solve_device(fcbga_model *m, const fcbga_step *step, const fcbga_
                             fcbga_solve_stats *stats) {
   if (!du_dev) { g_err = "du_dev is NULL"; return FCBGA_ERR_ARG; }
   return solve_common(m, step, opts, du_dev, nullptr, stats);
+}
+
+int fcbga_adjoint_forward(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts,
+                          const fcbga_objective *obj, double *J, double *du_out, fcbga_solve_stats *stats) {
+  if (!m || !step || !obj || !J) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    fcbga_solve_stats S{};
+    adjoint_forward(m, step, opts, obj, J, du_out, &S);
+    if (stats) *stats = S;
+    if (!S.converged) {
+      g_err = "not converged";
+      return FCBGA_ERR_NOT_CONVERGED;
+    }
+  } catch (const be::Error &e) {
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_adjoint_backward(fcbga_model *m, const double *gJ, double *dlnE, double *dlnA, double *dlnA_elem,
+                           fcbga_solve_stats *stats) {
+  if (!m || !gJ || !dlnE || !dlnA) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    fcbga_solve_stats S{};
+    adjoint_backward(m, gJ, dlnE, dlnA, dlnA_elem, &S);
+    if (stats) *stats = S;
+    if (!S.converged) {
+      g_err = "adjoint solve not converged";
+      return FCBGA_ERR_NOT_CONVERGED;
+    }
+  } catch (const be::Error &e) {
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  return FCBGA_OK;
 }
 
 int fcbga_state_accumulate(fcbga_model *m, const fcbga_step *step, const double *du, int64_t nelem,

@@ -1003,4 +1003,70 @@ FE_HD void state_stress(double hx, double hy, double hz, const double *__restric
   }
 }
 
+// Adjoint of the recovered point stresses of one element: given dJ/dsigma at
+// natural points (npts x 6), the derivative of J with respect to the element's
+// 24 nodal displacements at fixed material data.  sigma = D B(point) q with
+// q = (u, alpha), alpha = K_aa^-1 (f_a - K_au u) (accumulateStep's recovery), so
+// dJ/du = w_u - K_ua K_aa^-1 w_a with w = sum_points B^T D dJ/dsigma (33).
+FE_HD void element_stress_pullback(double hx, double hy, double hz, const double *__restrict__ M,
+                                   const double *__restrict__ D, int npts, const double *__restrict__ xez,
+                                   const double *__restrict__ dJds, double g[24]) {
+  const double sx = 2 / hx, sy = 2 / hy, sz = 2 / hz;
+  double w[33];
+  for (int t = 0; t < 33; t++) w[t] = 0;
+  for (int p = 0; p < npts; p++) {
+    const double xi = xez[3 * p], eta = xez[3 * p + 1], zeta = xez[3 * p + 2];
+    double v[6];
+    for (int r = 0; r < 6; r++) {
+      double acc = 0;
+      for (int c = 0; c < 6; c++) acc += D[6 * r + c] * dJds[6 * p + c];
+      v[r] = acc;
+    }
+    for (int b = 0; b < 11; b++) {
+      double dx, dy, dz;
+      if (b < 8) {
+        const double xa = HEX_XI_[b], ya = HEX_ETA_[b], za = HEX_ZETA_[b];
+        dx = xa * (1 + ya * eta) * (1 + za * zeta) * 0.125 * sx;
+        dy = ya * (1 + xa * xi) * (1 + za * zeta) * 0.125 * sy;
+        dz = za * (1 + xa * xi) * (1 + ya * eta) * 0.125 * sz;
+      } else {
+        dx = b == 8 ? -2 * xi * sx : 0.0;
+        dy = b == 9 ? -2 * eta * sy : 0.0;
+        dz = b == 10 ? -2 * zeta * sz : 0.0;
+      }
+      w[3 * b] += dx * v[0] + dy * v[3] + dz * v[5];
+      w[3 * b + 1] += dy * v[1] + dx * v[3] + dz * v[4];
+      w[3 * b + 2] += dz * v[2] + dy * v[4] + dx * v[5];
+    }
+  }
+  const double V8 = hx * hy * hz / 8;
+  const double c[6] = {V8 * 4 / (hx * hx), V8 * 4 / (hy * hy), V8 * 4 / (hz * hz),
+                       V8 * 4 / (hx * hy), V8 * 4 / (hy * hz), V8 * 4 / (hz * hx)};
+  double Kr[9 * 33];
+  for (int i = 0; i < 9; i++)
+    for (int j = 0; j < 33; j++) {
+      const int o = (24 + i) * 33 + j;
+      Kr[i * 33 + j] = c[0] * M[o] + c[1] * M[1089 + o] + c[2] * M[2178 + o] + c[3] * M[3267 + o] +
+                       c[4] * M[4356 + o] + c[5] * M[5445 + o];
+    }
+  double Kaa[81], y[9];
+  for (int r = 0; r < 9; r++)
+    for (int q = 0; q < 9; q++) Kaa[r * 9 + q] = Kr[r * 33 + 24 + q];
+  chol_dense(Kaa, 9, 1e-14);
+  for (int r = 0; r < 9; r++) y[r] = w[24 + r];
+  chol_solve<double, double>(Kaa, 9, y);
+  for (int t = 0; t < 24; t++) {
+    double acc = w[t];
+    for (int r = 0; r < 9; r++) acc -= Kr[r * 33 + t] * y[r];
+    g[t] = acc;
+  }
+}
+
+// von Mises stress of a Voigt tensor (engineering shear order xx yy zz xy yz zx)
+FE_HD double von_mises(const double *s) {
+  const double a = s[0] - s[1], b = s[1] - s[2], c = s[2] - s[0];
+  const double v = 0.5 * (a * a + b * b + c * c) + 3 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+  return sqrt(v > 0 ? v : 0.0);
+}
+
 }  // namespace fe

@@ -55,10 +55,14 @@ class FcbgaSolveStats(C.Structure):
 EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_step_solve",
            "fcbga_step_solve_device", "fcbga_step_load", "fcbga_step_apply", "fcbga_set_kernel_timing",
            "fcbga_kernel_times", "fcbga_anand_update", "fcbga_anand_update_device", "fcbga_state_accumulate",
-           "fcbga_state_stress", "fcbga_last_error"]
+           "fcbga_state_stress", "fcbga_adjoint_forward", "fcbga_adjoint_backward", "fcbga_last_error"]
 
 ANAND_STATE = 11
 STATE = 81  # stress-history state of one element (FCBGA_STATE)
+
+
+class FcbgaObjective(C.Structure):
+    _fields_ = [("k_top", C.c_int32), ("n_vm", C.c_int32), ("vm_elems", _i32p), ("p", C.c_double)]
 
 
 class FcbgaAnandParams(C.Structure):
@@ -111,6 +115,11 @@ def load(variant: str = "hip"):
     L.fcbga_state_accumulate.restype = C.c_int
     L.fcbga_state_stress.argtypes = [C.c_void_p, C.c_int64, _i32p, _f64p, C.c_int64, _i32p, _f64p, _f64p, _f64p]
     L.fcbga_state_stress.restype = C.c_int
+    L.fcbga_adjoint_forward.argtypes = [C.c_void_p, C.POINTER(FcbgaStep), C.POINTER(FcbgaSolveOpts),
+                                        C.POINTER(FcbgaObjective), _f64p, _f64p, C.POINTER(FcbgaSolveStats)]
+    L.fcbga_adjoint_forward.restype = C.c_int
+    L.fcbga_adjoint_backward.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p, C.POINTER(FcbgaSolveStats)]
+    L.fcbga_adjoint_backward.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.fcbga_last_error.restype = C.c_int
     _loaded[variant] = L
@@ -274,6 +283,43 @@ class Model:
         if rc != 0:
             raise SolverError(last_error(self.L))
         return sig, eps
+
+    def adjoint_forward(self, step: Step, k_top, vm_elems=(), p=8.0, tol=1e-10, maxit=2000, precision=PREC_MIXED,
+                        precond=PC_MG, check=True):
+        """Forward of the sensitivity pair: solve the step and evaluate J =
+        [warpage proxy on plane k_top, von Mises p-norm over vm_elems]; returns
+        (J (2), du, solve record).  The solution stays on the model for
+        adjoint_backward."""
+        vm = np.ascontiguousarray(vm_elems, np.int32)
+        obj = FcbgaObjective()
+        obj.k_top, obj.n_vm, obj.p = int(k_top), len(vm), float(p)
+        obj.vm_elems = vm.ctypes.data_as(_i32p) if len(vm) else None
+        J = np.zeros(2)
+        du = np.zeros(self.ndof)
+        st = FcbgaSolveStats()
+        o = self._opts(tol, maxit, precision, precond)
+        rc = self.L.fcbga_adjoint_forward(self.h, C.byref(step.c), C.byref(o), C.byref(obj), J.ctypes.data_as(_f64p),
+                                          du.ctypes.data_as(_f64p), C.byref(st))
+        if rc != 0 and (check or rc != -3):
+            raise SolverError(f"fcbga_adjoint_forward failed ({rc}): {last_error(self.L)}")
+        return J, du, {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_}
+
+    def adjoint_backward(self, gJ, elem_cte=True):
+        """Backward of the pair: log-derivatives of gJ . J with respect to each
+        group's E and CTE and (elem_cte) each element's CTE, from one adjoint
+        solve; returns (dlnE, dlnA, dlnA_elem or None, solve record)."""
+        g = np.ascontiguousarray(gJ, np.float64).reshape(2)
+        ng = int(self.mesh.e_group.max()) + 1 if len(self.mesh.groups) == 0 else len(self.mesh.groups)
+        dE = np.zeros(ng)
+        dA = np.zeros(ng)
+        de = np.zeros(self.mesh.ne) if elem_cte else None
+        st = FcbgaSolveStats()
+        rc = self.L.fcbga_adjoint_backward(self.h, g.ctypes.data_as(_f64p), dE.ctypes.data_as(_f64p),
+                                           dA.ctypes.data_as(_f64p),
+                                           de.ctypes.data_as(_f64p) if de is not None else None, C.byref(st))
+        if rc != 0:
+            raise SolverError(f"fcbga_adjoint_backward failed ({rc}): {last_error(self.L)}")
+        return dE, dA, de, {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_}
 
     def apply(self, step: Step, x):
         x = np.ascontiguousarray(x, np.float64)
