@@ -97,6 +97,32 @@ class HipStateView {
   }
 }
 
+/**
+ * The sensitivity op with autograd.Function semantics over the addon pair:
+ * forward(step, objective) solves the step and returns J = [warpage proxy on
+ * plane kTop, von Mises p-norm over objective.vmElems], keeping the context on
+ * the model; backward(gJ) returns the log-derivatives of gJ . J for every
+ * group's E and CTE and every element's CTE from one adjoint solve.  It takes
+ * the place of the finite-difference tornado of the sensitivity study
+ * (runSensitivityCase / tornadoFromCases, src/core/14-sensitivity.js:8-172):
+ * dJ/dE_g = dlnE[g] / E_g, dJ/dCTE_g = dlnCte[g] / CTE_g.
+ */
+class HipSensitivity {
+  constructor(solver) { this.solver = solver; this.saved = false; }
+  forward(step, objective, opts) {
+    const r = this.solver.hip.adjointForward(this.solver.model, step, objective, Object.assign({}, this.solver.opts, opts || {}));
+    if (!r.converged) throw new Error('Linear solver did not converge (sensitivity forward): relative residual ' + r.relres.toExponential(2));
+    this.saved = true;
+    return r;
+  }
+  backward(gJ, elemCte) {
+    if (!this.saved) throw new Error('sensitivity backward before forward');
+    const r = this.solver.hip.adjointBackward(this.solver.model, Float64Array.from(gJ), elemCte !== false);
+    if (!r.converged) throw new Error('Linear solver did not converge (sensitivity adjoint): relative residual ' + r.relres.toExponential(2));
+    return r;
+  }
+}
+
 function install(Analysis, opts) {
   const original = Analysis.prototype.solveStep;
   Analysis.prototype.solveStep = function (k) {
@@ -118,4 +144,4 @@ function install(Analysis, opts) {
   return original;
 }
 
-module.exports = { load, HipStepSolver, HipStateView, stepFromGroups, install, STATE };
+module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, stepFromGroups, install, STATE };

@@ -10,6 +10,8 @@
 //            precond, rebuilt, warm}  -- the fields of LinearSolver.solve's record (src/core/10-solver.js:988-993)
 //   const st = hip.stateAccumulate(model, step, du, elems[, states[, qOnly]]);  // Float64Array(n * 81)
 //   const {sig, eps} = hip.stateStress(model, elems, st, rows, xez);       // stresses at natural points
+//   const f = hip.adjointForward(model, step, {kTop, vmElems, p}, opts);   // {J, u, ...}: saves the context
+//   const g = hip.adjointBackward(model, Float64Array.of(1, 0));          // {dlnE, dlnCte, dlnCteElem}
 //   hip.destroyModel(model);                   // or left to the garbage collector
 // Argument and device errors throw Error(message); a solve that does not
 // converge returns its record with converged = false (the caller throws, as
@@ -37,6 +39,7 @@ namespace {
 
 struct ModelBox {
   fcbga_model *m = nullptr;
+  int32_t ngroups = 0, ne = 0;
 };
 
 void finalize_model(napi_env, void *data, void *) {
@@ -140,6 +143,8 @@ napi_value CreateModel(napi_env env, napi_callback_info info) {
   if (fcbga_model_create(&m, &model) != FCBGA_OK) return throw_msg(env, "createModel: " + lib_error());
   ModelBox *box = new ModelBox();
   box->m = model;
+  box->ngroups = m.ngroups;
+  box->ne = m.ne;
   napi_value ext;
   if (napi_create_external(env, box, finalize_model, nullptr, &ext) != napi_ok) {
     fcbga_model_destroy(model);
@@ -346,12 +351,117 @@ napi_value StateStress(napi_env env, napi_callback_info info) {
   return out;
 }
 
+napi_value make_f64(napi_env env, size_t n, double **data) {
+  napi_value ab, arr;
+  void *p = nullptr;
+  if (napi_create_arraybuffer(env, sizeof(double) * n, &p, &ab) != napi_ok) return nullptr;
+  if (napi_create_typedarray(env, napi_float64_array, n, ab, 0, &arr) != napi_ok) return nullptr;
+  *data = (double *)p;
+  return arr;
+}
+
+bool read_opts(napi_env env, napi_value v, fcbga_solve_opts *o) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t != napi_object) return true;
+  get_double(env, v, "tol", &o->tol);
+  get_int(env, v, "maxit", &o->maxit);
+  get_int(env, v, "sweeps", &o->sweeps);
+  napi_value x;
+  char buf[32];
+  size_t len;
+  if (get_prop(env, v, "precision", &x) && napi_get_value_string_utf8(env, x, buf, sizeof buf, &len) == napi_ok)
+    o->precision = strcmp(buf, "fp64") == 0 ? FCBGA_PREC_FP64 : FCBGA_PREC_MIXED;
+  return true;
+}
+
+// adjointForward(model, step, objective, opts) -> {J: Float64Array(2), u, iters, relres, converged}
+//   objective: {kTop, vmElems: Int32Array, p}
+// the forward of the sensitivity pair (fcbga_adjoint_forward); the model keeps the context
+napi_value AdjointForward(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 3) return throw_msg(env, "adjointForward(model, step, objective[, opts])");
+  ModelBox *b = unwrap(env, argv[0]);
+  if (!b || !b->m) return throw_msg(env, "adjointForward: not a live model");
+  fcbga_step st;
+  std::string err;
+  if (!read_step(env, argv[1], &st, &err)) return throw_msg(env, "adjointForward: " + err);
+  fcbga_objective obj;
+  memset(&obj, 0, sizeof obj);
+  obj.p = 8;
+  if (!get_int(env, argv[2], "kTop", &obj.k_top)) return throw_msg(env, "adjointForward: objective.kTop required");
+  void *p;
+  size_t n;
+  if (get_typed(env, argv[2], "vmElems", napi_int32_array, &p, &n)) {
+    obj.vm_elems = (const int32_t *)p;
+    obj.n_vm = (int32_t)n;
+  }
+  get_double(env, argv[2], "p", &obj.p);
+  fcbga_solve_opts o{1e-10, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
+  if (argc >= 4) read_opts(env, argv[3], &o);
+  const int64_t ndof = fcbga_model_ndof(b->m);
+  double *J, *u;
+  napi_value jv = make_f64(env, 2, &J), uv = make_f64(env, (size_t)ndof, &u);
+  if (!jv || !uv) return throw_msg(env, "adjointForward: allocation failed");
+  fcbga_solve_stats S;
+  memset(&S, 0, sizeof S);
+  const int rc = fcbga_adjoint_forward(b->m, &st, &o, &obj, J, u, &S);
+  if (rc != FCBGA_OK && rc != FCBGA_ERR_NOT_CONVERGED) return throw_msg(env, "adjointForward: " + lib_error());
+  napi_value rec, cv;
+  NAPI_OK(napi_create_object(env, &rec));
+  napi_set_named_property(env, rec, "J", jv);
+  napi_set_named_property(env, rec, "u", uv);
+  set_num(env, rec, "iters", S.iters);
+  set_num(env, rec, "relres", S.relres);
+  set_num(env, rec, "ms", S.ms_total);
+  napi_get_boolean(env, S.converged != 0, &cv);
+  napi_set_named_property(env, rec, "converged", cv);
+  return rec;
+}
+
+// adjointBackward(model, gJ: Float64Array(2)[, elemCte]) -> {dlnE, dlnCte, dlnCteElem, iters, relres, converged}
+napi_value AdjointBackward(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) return throw_msg(env, "adjointBackward(model, gJ[, elemCte])");
+  ModelBox *b = unwrap(env, argv[0]);
+  if (!b || !b->m) return throw_msg(env, "adjointBackward: not a live model");
+  void *g;
+  size_t n;
+  if (!arg_typed(env, argv[1], napi_float64_array, &g, &n) || n != 2)
+    return throw_msg(env, "adjointBackward: gJ must be a Float64Array(2)");
+  bool elem = true;
+  if (argc >= 3) napi_get_value_bool(env, argv[2], &elem);
+  const int32_t ng = b->ngroups, ne = b->ne;
+  double *dE, *dA, *de = nullptr;
+  napi_value ev = make_f64(env, (size_t)ng, &dE), av = make_f64(env, (size_t)ng, &dA), elv = nullptr;
+  if (elem) elv = make_f64(env, (size_t)ne, &de);
+  fcbga_solve_stats S;
+  memset(&S, 0, sizeof S);
+  const int rc = fcbga_adjoint_backward(b->m, (const double *)g, dE, dA, de, &S);
+  if (rc != FCBGA_OK && rc != FCBGA_ERR_NOT_CONVERGED) return throw_msg(env, "adjointBackward: " + lib_error());
+  napi_value rec, cv;
+  NAPI_OK(napi_create_object(env, &rec));
+  napi_set_named_property(env, rec, "dlnE", ev);
+  napi_set_named_property(env, rec, "dlnCte", av);
+  if (elv) napi_set_named_property(env, rec, "dlnCteElem", elv);
+  set_num(env, rec, "iters", S.iters);
+  set_num(env, rec, "relres", S.relres);
+  napi_get_boolean(env, S.converged != 0, &cv);
+  napi_set_named_property(env, rec, "converged", cv);
+  return rec;
+}
+
 }  // namespace
 
 static napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback cb; } fns[] = {
       {"createModel", CreateModel}, {"destroyModel", DestroyModel}, {"solveStep", SolveStep},
-      {"stateAccumulate", StateAccumulate}, {"stateStress", StateStress}};
+      {"stateAccumulate", StateAccumulate}, {"stateStress", StateStress},
+      {"adjointForward", AdjointForward}, {"adjointBackward", AdjointBackward}};
   for (auto &f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||

@@ -170,3 +170,21 @@ def test_gpu_cfg4_adjoint_vs_central_differences():
     fE, fA = _fd(model, spec, data, obj, hs.PREC_FP64, 1e-12, h=4e-3, richardson=True)
     _check(dE, {g: float(gJ @ v) for g, v in fE.items()}, 1e-4, floor=1e-2)
     _check(dA, {g: float(gJ @ v) for g, v in fA.items()}, 1e-4, floor=1e-2)
+
+
+@pytest.mark.gpu
+def test_node_addon_sensitivity_pair():
+    """The JS host path of the pair: node -> HipSensitivity -> Node-API addon
+    -> C ABI -> HIP, against central differences of its own forward."""
+    import json
+    import os
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "js", "napi_adjoint.js")
+    out = subprocess.run([node, script, "plate_16"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    assert rec["rel_err"] <= 1e-5 and rec["elem_sum_rel_err"] <= 1e-10, rec
