@@ -51,20 +51,45 @@ def incidences_per_node(mesh):
     return inc
 
 
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/<kernel>_cfg2_hbm_traffic.txt: FETCH_SIZE and WRITE_SIZE per
+    dispatch, scaled by the factors a plain copy of known size reads)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_cfg2_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def roofline_for(cls, ms_total, launches, mesh, dtype_cycle):
-    """Algorithmic flops of a node-centric operator launch: 144 flop per
-    (node, element) incidence (3 rows x 24 columns of the condensed matrix, 2 flop
-    per MAC); a full apply is 1152 flop per element."""
+    """Roofline of a kernel class from its HIP-event time per launch.
+    fine_sweep (the z-line smoother, one launch per colour): HBM-bound, 36 B per
+    swept DOF (FP32 column factors 24, x read/write 8, b 4) plus the 4 B FP32 x
+    of every DOF that the colour's residual reads.  apply / fine_resid (the
+    operator): compute-bound, 144 flop per (node, element) incidence (3 rows x
+    24 columns of the condensed matrix, 2 flop per MAC), 1152 per element."""
     inc = incidences_per_node(mesh)
     avg_s = ms_total / launches / 1e3
+    if cls == "fine_sweep":
+        exist = mesh.node_id.reshape(mesh.nx, mesh.ny, mesh.nz) >= 0
+        per = [exist[a::2, b::2, :].sum() for a in (0, 1) for b in (0, 1)]
+        nbytes = 3 * (36.0 * float(np.mean(per)) + 4.0 * mesh.nn)
+        achieved = nbytes / avg_s
+        tr = traffic_from_profile("k_fine_col_sweep")
+        return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
+                "frac": achieved / PEAK["hbm"], "traffic": tr, "kernel": cls, "dtype": dtype_cycle,
+                "avg_launch_us": avg_s * 1e6, "launches": launches, "bytes_per_launch": nbytes}
     if cls == "apply":
         flops = 144.0 * inc.sum()
         peak, unit_dt = PEAK["fp64"], "fp64"
-    elif cls in ("fine_sweep",):
-        # one launch per colour (i mod 2, j mod 2): average incidences of a colour
-        per = [inc[a::2, b::2, :].sum() for a in (0, 1) for b in (0, 1)]
-        flops = 144.0 * float(np.mean(per))
-        peak, unit_dt = PEAK[dtype_cycle], dtype_cycle
     elif cls == "fine_resid":
         flops = 144.0 * inc.sum()
         peak, unit_dt = PEAK[dtype_cycle], dtype_cycle
@@ -129,6 +154,37 @@ def anand_throughput(torch, hs, n=115200, reps=20):
             "note": "AnandPoint.step with consistent tangent, one batch per global Newton iteration"}
 
 
+def adjoint_timing(torch, hs, cases, meshspec, tol, reps=3):
+    """Sensitivity pair on the lidded mesh with 18 groups: one forward (solve +
+    objectives) and one backward (adjoint solve + per-element contractions)
+    give the gradients for 36 group parameters and every element's CTE; central
+    differences on the 36 group parameters take 72 forward solves."""
+    spec = cases.cfg4_spec()
+    mesh = meshspec.build_mesh(spec)
+    model = hs.Model(mesh, variant="hip")
+    st = hs.step_for_spec(spec)
+    obj = spec["objective"]
+    vm = np.nonzero(np.isin(mesh.e_group, obj["vm_groups"]))[0].astype(np.int32)
+    gJ = np.array([1.0, 1e-3])
+    tf, tb, itf, itb = [], [], [], []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J, _, rf = model.adjoint_forward(st, obj["k_top"], vm, obj["p"], tol=tol, precision=hs.PREC_MIXED)
+        t1 = time.perf_counter()
+        _, _, _, rb = model.adjoint_backward(gJ)
+        t2 = time.perf_counter()
+        if r:
+            tf.append(t1 - t0); tb.append(t2 - t1); itf.append(rf["iters"]); itb.append(rb["iters"])
+    fwd, bwd = float(np.mean(tf)), float(np.mean(tb))
+    nparam = 2 * len(mesh.groups)
+    return {"workload": f"{spec['name']}: warpage proxy + die von Mises p-norm (p = {obj['p']:g})",
+            "params": nparam, "elem_params": mesh.ne, "forward_s": fwd, "backward_s": bwd,
+            "forward_iters": itf[-1], "adjoint_iters": itb[-1], "fd_solves": 2 * nparam,
+            "fd_equivalent_s": 2 * nparam * fwd, "speedup_vs_fd": 2 * nparam * fwd / (fwd + bwd),
+            "J": [float(J[0]), float(J[1])]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +195,7 @@ def main():
     ap.add_argument("--sweeps", type=int, default=0, help="smoothing sweeps per level (0 = the reference's 2)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solve runs on a single MI355X and this build has no "
@@ -242,6 +299,12 @@ def main():
         out["speedup_mixed_vs_fp64"] = out["solve_s_fp64"] / out["solve_s_mixed"]
         out["warpage_rel_err_mixed_vs_fp64_max"] = float(max(werr))
         log(f"FP64 comparator: {out['solve_s_fp64']:.3f} s/step, warpage err max {max(werr):.2e}")
+
+    if not args.no_adjoint and args.config == "cfg2":
+        out["adjoint"] = adjoint_timing(torch, hs, cases, meshspec, args.tol)
+        a = out["adjoint"]
+        log(f"adjoint ({a['params']} parameters): forward {a['forward_s']:.3f} s + backward {a['backward_s']:.3f} s "
+            f"vs {a['fd_solves']} central-difference solves {a['fd_equivalent_s']:.2f} s")
 
     # Anand material-point update at the bump-array size (20 x 20 bumps x 36 solder hexes x 8 points)
     out["anand"] = anand_throughput(torch, hs)
