@@ -80,6 +80,10 @@ typedef struct fcbga_solve_opts {
   int32_t precision;  /* FCBGA_PREC_* */
   int32_t precond;    /* FCBGA_PC_* */
   int32_t sweeps;     /* pre- and post-smoothing sweeps per level; 0 = the reference's 2 */
+  int32_t warm;       /* warm start (LinearSolver._warmStart/_storeSolution, src/core/10-solver.js:885-926):
+                         size of the basis of previous solutions the load is projected onto, and that a
+                         converged solution joins (the reference's default is 20); 0 = cold, the basis is
+                         neither used nor extended */
 } fcbga_solve_opts;
 
 /* Solve record of LinearSolver.solve (src/core/10-solver.js:988-993) plus timings. */
@@ -87,7 +91,7 @@ typedef struct fcbga_solve_stats {
   int32_t iters;
   int32_t converged;
   double relres;
-  double r0;
+  double r0;          /* relative residual of the initial guess (iters0: 1 cold, below 1 warm) */
   double ms_total;    /* whole step: assembly + preconditioner set-up + PCG */
   double ms_assemble; /* element tables, load, diagonal blocks */
   double ms_setup;    /* multigrid set-up (numeric) */
@@ -97,10 +101,14 @@ typedef struct fcbga_solve_stats {
   int64_t ndof;
   int32_t ntypes;     /* distinct condensed element matrices */
   int32_t sym_tables; /* 1: the fine operator ran on the mirror-symmetric compact element tables */
+  int32_t warm;       /* warm-start basis size after the solve (the record's warm field) */
+  int32_t reserved;
 } fcbga_solve_stats;
 
 int fcbga_model_create(const fcbga_mesh *mesh, fcbga_model **out);
 void fcbga_model_destroy(fcbga_model *m);
+/* Empties the warm-start basis (a new path, or a stage with another constraint set). */
+int fcbga_warm_reset(fcbga_model *m);
 int64_t fcbga_model_ndof(const fcbga_model *m);
 
 /* Step solve with host output: du_out has 3*nn doubles in the reference's node order. */

@@ -238,6 +238,9 @@ struct fcbga_model {
   uint64_t prepGen = 0;
   int prepPrecond = -1, prepPrecision = -1;
   AdjTape tape;
+  // warm start: orthonormal previous solutions (grid layout, unscaled), oldest first
+  std::vector<std::unique_ptr<DBuf<double>>> basis;
+  DBuf<double> warmTmp, warmKv;
   DBuf<uint8_t> slotHasF, slotLayered, typeHasF, d_born;
   DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64;
   DBuf<float> w32, sinv32, F32;
@@ -2038,6 +2041,116 @@ void alloc_vectors(fcbga_model *M, int precision) {
 #endif
 }
 
+// Gauss-Jordan inverse with partial pivoting, then x = A^-1 b (solveDense /
+// invDense, src/core/01-util.js:111-142); false when singular
+bool solve_dense_gj(std::vector<double> A, int n, const std::vector<double> &b, std::vector<double> &x) {
+  const int w = 2 * n;
+  std::vector<double> Mx((size_t)n * w, 0.0);
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) Mx[(size_t)i * w + j] = A[(size_t)i * n + j];
+    Mx[(size_t)i * w + n + i] = 1;
+  }
+  for (int c = 0; c < n; c++) {
+    int p = c;
+    double best = std::fabs(Mx[(size_t)c * w + c]);
+    for (int r = c + 1; r < n; r++) {
+      const double v = std::fabs(Mx[(size_t)r * w + c]);
+      if (v > best) { best = v; p = r; }
+    }
+    if (best == 0) return false;
+    if (p != c)
+      for (int j = 0; j < w; j++) std::swap(Mx[(size_t)c * w + j], Mx[(size_t)p * w + j]);
+    const double inv = 1 / Mx[(size_t)c * w + c];
+    for (int j = 0; j < w; j++) Mx[(size_t)c * w + j] *= inv;
+    for (int r = 0; r < n; r++) {
+      if (r == c) continue;
+      const double f = Mx[(size_t)r * w + c];
+      if (f == 0) continue;
+      for (int j = 0; j < w; j++) Mx[(size_t)r * w + j] -= f * Mx[(size_t)c * w + j];
+    }
+  }
+  x.assign(n, 0.0);
+  for (int i = 0; i < n; i++) {
+    double acc = 0;
+    for (int j = 0; j < n; j++) acc += Mx[(size_t)i * w + n + j] * b[j];
+    x[i] = acc;
+  }
+  return true;
+}
+
+// Galerkin projection of the load onto the stored solutions (_warmStart,
+// src/core/10-solver.js:885-915), in the unscaled space: A_ij = v_i.K v_j
+// (symmetrised), b_i = v_i.f, x0 = sum c_i v_i; M->x receives x0 / s (the
+// PCG's scaled space).  f is the unscaled load (grid layout).  False (x left
+// 0) without a basis or when the projection is singular or not finite.
+bool warm_start(fcbga_model *M, const double *f) {
+  const int m = (int)M->basis.size();
+  if (!m) return false;
+  be::stream_t s = M->stream;
+  const int64_t n3 = 3 * M->g.G;
+  const double *si = M->sinv64.p;
+  M->warmTmp.alloc(n3);
+  M->warmKv.alloc((size_t)m * n3);
+  double *tmp = M->warmTmp.p;
+  const FineOps<double> ops = fine_ops<double>(M);
+  std::vector<double> A((size_t)m * m), b(m);
+  for (int i = 0; i < m; i++) {
+    const double *v = M->basis[i]->p;
+    double *kv = M->warmKv.p + (size_t)i * n3;
+    // K v = S^-1 Ks S^-1 v
+    be::pfor(n3, s, [=] __host__ __device__(int64_t q) { tmp[q] = v[q] * si[q]; });
+    fine_apply<double>(ops, tmp, kv, s);
+    be::pfor(n3, s, [=] __host__ __device__(int64_t q) { kv[q] *= si[q]; });
+  }
+  for (int i = 0; i < m; i++) {
+    b[i] = dotd(M, M->basis[i]->p, f, SC_RR);
+    for (int j = 0; j < m; j++) A[(size_t)i * m + j] = dotd(M, M->basis[i]->p, M->warmKv.p + (size_t)j * n3, SC_RR);
+  }
+  for (int i = 0; i < m; i++)
+    for (int j = i + 1; j < m; j++) {
+      const double v = 0.5 * (A[(size_t)i * m + j] + A[(size_t)j * m + i]);
+      A[(size_t)i * m + j] = A[(size_t)j * m + i] = v;
+    }
+  std::vector<double> c;
+  if (!solve_dense_gj(A, m, b, c)) return false;
+  for (int i = 0; i < m; i++)
+    if (!std::isfinite(c[i])) return false;
+  double *x = M->x.p;
+  be::pfor(n3, s, [=] __host__ __device__(int64_t q) { x[q] = 0.0; });
+  for (int i = 0; i < m; i++) {
+    const double *v = M->basis[i]->p;
+    const double ci = c[i];
+    be::pfor(n3, s, [=] __host__ __device__(int64_t q) { x[q] += ci * v[q]; });
+  }
+  be::pfor(n3, s, [=] __host__ __device__(int64_t q) { x[q] *= si[q]; });
+  return true;
+}
+
+// a converged solution joins the basis (_storeSolution, src/core/10-solver.js:917-926):
+// two passes of modified Gram-Schmidt, normalised, the oldest dropped beyond nmax
+void store_solution(fcbga_model *M, int nmax) {
+  be::stream_t s = M->stream;
+  const int64_t n3 = 3 * M->g.G;
+  const double *x = M->x.p, *sv = M->s64.p;
+  auto v = std::make_unique<DBuf<double>>();
+  v->alloc(n3);
+  double *vp = v->p;
+  be::pfor(n3, s, [=] __host__ __device__(int64_t q) { vp[q] = x[q] * sv[q]; });
+  const double nu = std::sqrt(dotd(M, vp, vp, SC_RR));
+  for (int pass = 0; pass < 2; pass++)
+    for (auto &bv : M->basis) {
+      const double d = dotd(M, bv->p, vp, SC_RR);
+      const double *bp = bv->p;
+      be::pfor(n3, s, [=] __host__ __device__(int64_t q) { vp[q] -= d * bp[q]; });
+    }
+  const double nv = std::sqrt(dotd(M, vp, vp, SC_RR));
+  if (!(nv > 1e-14 * (nu + 1e-300))) return;
+  const double inv = 1 / nv;
+  be::pfor(n3, s, [=] __host__ __device__(int64_t q) { vp[q] *= inv; });
+  M->basis.push_back(std::move(v));
+  while ((int)M->basis.size() > nmax) M->basis.erase(M->basis.begin());
+}
+
 fcbga_solve_opts check_opts(const fcbga_solve_opts *op) {
   fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_FP64, FCBGA_PC_MG, 0};
   if (op) o = *op;
@@ -2045,6 +2158,7 @@ fcbga_solve_opts check_opts(const fcbga_solve_opts *op) {
   if (o.precision != FCBGA_PREC_FP64 && o.precision != FCBGA_PREC_MIXED) throw be::Error("unknown precision");
   if (o.precond < FCBGA_PC_MG || o.precond > FCBGA_PC_JACOBI) throw be::Error("unknown preconditioner");
   if (o.sweeps < 0 || o.sweeps > 8) throw be::Error("sweeps must be 0 (default 2) .. 8");
+  if (o.warm < 0 || o.warm > 64) throw be::Error("warm must be 0 .. 64");
   return o;
 }
 
@@ -2094,14 +2208,21 @@ void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, 
     x[i] = 0.0;
   });
   const double bnorm = std::sqrt(dotd(M, r, r, SC_RR));
+  const bool warm = o.warm > 0 && !rhs_grid;
+  if (warm && bnorm != 0 && warm_start(M, bl)) {
+    // r = b - A x for the projected initial guess
+    fine_apply<double>(fine_ops<double>(M), x, M->q.p, s);
+    const double *q = M->q.p;
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { r[i] -= q[i]; });
+  }
   int it = 0;
   bool conv = false;
-  double rnorm = bnorm;
+  double rnorm = bnorm == 0 ? 0 : std::sqrt(dotd(M, r, r, SC_RR));
   if (bnorm == 0) {
     conv = true;
     S.r0 = 0;
   } else {
-    S.r0 = 1.0;
+    S.r0 = rnorm / bnorm;
     if (rnorm / bnorm < o.tol) conv = true;
     else {
       precondition(M, o.precision, o.precond);
@@ -2130,6 +2251,7 @@ void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, 
     });
   }
   if (du_host) be::d2h(du_host, du_dev, sizeof(double) * 3 * (size_t)M->nn, s);
+  if (warm && conv && bnorm != 0) store_solution(M, o.warm);
   be::sync(s);
   ktimer_collect(M);
   const double t3 = now_ms();
@@ -2137,6 +2259,7 @@ void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, 
   S.iters = it;
   S.converged = conv ? 1 : 0;
   S.relres = bnorm > 0 ? rnorm / bnorm : 0;
+  S.warm = (int32_t)M->basis.size();
 }
 
 void step_solve(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts *op, double *du_dev, double *du_host,
@@ -2730,6 +2853,12 @@ static int solve_common(fcbga_model *m, const fcbga_step *step, const fcbga_solv
   } catch (const std::exception &e) {
     return fail(e, FCBGA_ERR_INTERNAL);
   }
+  return FCBGA_OK;
+}
+
+int fcbga_warm_reset(fcbga_model *m) {
+  if (!m) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  m->basis.clear();
   return FCBGA_OK;
 }
 

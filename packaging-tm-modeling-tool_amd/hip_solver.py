@@ -42,17 +42,17 @@ class FcbgaStep(C.Structure):
 
 class FcbgaSolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxit", C.c_int32), ("precision", C.c_int32), ("precond", C.c_int32),
-                ("sweeps", C.c_int32)]
+                ("sweeps", C.c_int32), ("warm", C.c_int32)]
 
 
 class FcbgaSolveStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double), ("r0", C.c_double),
                 ("ms_total", C.c_double), ("ms_assemble", C.c_double), ("ms_setup", C.c_double),
                 ("ms_pcg", C.c_double), ("levels", C.c_int32), ("coarsest_dof", C.c_int32), ("ndof", C.c_int64),
-                ("ntypes", C.c_int32), ("sym_tables", C.c_int32)]
+                ("ntypes", C.c_int32), ("sym_tables", C.c_int32), ("warm", C.c_int32), ("reserved", C.c_int32)]
 
 
-EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_step_solve",
+EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_warm_reset", "fcbga_step_solve",
            "fcbga_step_solve_device", "fcbga_step_load", "fcbga_step_apply", "fcbga_set_kernel_timing",
            "fcbga_kernel_times", "fcbga_anand_update", "fcbga_anand_update_device", "fcbga_state_accumulate",
            "fcbga_state_stress", "fcbga_adjoint_forward", "fcbga_adjoint_backward", "fcbga_last_error"]
@@ -120,6 +120,8 @@ def load(variant: str = "hip"):
     L.fcbga_adjoint_forward.restype = C.c_int
     L.fcbga_adjoint_backward.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p, C.POINTER(FcbgaSolveStats)]
     L.fcbga_adjoint_backward.restype = C.c_int
+    L.fcbga_warm_reset.argtypes = [C.c_void_p]
+    L.fcbga_warm_reset.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.fcbga_last_error.restype = C.c_int
     _loaded[variant] = L
@@ -204,19 +206,25 @@ class Model:
         self.close()
 
     @staticmethod
-    def _opts(tol, maxit, precision, precond, sweeps=0):
+    def _opts(tol, maxit, precision, precond, sweeps=0, warm=0):
         o = FcbgaSolveOpts()
-        o.tol, o.maxit, o.precision, o.precond, o.sweeps = tol, maxit, precision, precond, sweeps
+        o.tol, o.maxit, o.precision, o.precond, o.sweeps, o.warm = tol, maxit, precision, precond, sweeps, warm
         return o
 
-    def solve_step(self, step: Step, tol=1e-8, maxit=2000, precision=PREC_FP64, precond=PC_MG, check=True, sweeps=0):
-        """du (3*nn, reference node order) and the solve record."""
+    def warm_reset(self):
+        """Empty the warm-start basis."""
+        self.L.fcbga_warm_reset(self.h)
+
+    def solve_step(self, step: Step, tol=1e-8, maxit=2000, precision=PREC_FP64, precond=PC_MG, check=True, sweeps=0,
+                   warm=0):
+        """du (3*nn, reference node order) and the solve record; warm > 0
+        projects the load onto (and keeps) up to `warm` previous solutions."""
         du = np.zeros(self.ndof)
         st = FcbgaSolveStats()
-        o = self._opts(tol, maxit, precision, precond, sweeps)
+        o = self._opts(tol, maxit, precision, precond, sweeps, warm)
         rc = self.L.fcbga_step_solve(self.h, C.byref(step.c), C.byref(o), du.ctypes.data_as(C.c_void_p),
                                      C.byref(st))
-        rec = {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_}
+        rec = {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
         if rc != 0 and (check or rc != -3):
             raise SolverError(f"fcbga_step_solve failed ({rc}): {last_error(self.L)}")
         return du, rec
@@ -229,7 +237,7 @@ class Model:
                                             C.byref(st))
         if rc != 0:
             raise SolverError(f"fcbga_step_solve_device failed ({rc}): {last_error(self.L)}")
-        return {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_}
+        return {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
 
     def set_kernel_timing(self, enable: bool):
         self.L.fcbga_set_kernel_timing(self.h, 1 if enable else 0)
@@ -302,7 +310,7 @@ class Model:
                                           du.ctypes.data_as(_f64p), C.byref(st))
         if rc != 0 and (check or rc != -3):
             raise SolverError(f"fcbga_adjoint_forward failed ({rc}): {last_error(self.L)}")
-        return J, du, {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_}
+        return J, du, {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
 
     def adjoint_backward(self, gJ, elem_cte=True):
         """Backward of the pair: log-derivatives of gJ . J with respect to each
@@ -319,7 +327,7 @@ class Model:
                                            de.ctypes.data_as(_f64p) if de is not None else None, C.byref(st))
         if rc != 0:
             raise SolverError(f"fcbga_adjoint_backward failed ({rc}): {last_error(self.L)}")
-        return dE, dA, de, {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_}
+        return dE, dA, de, {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
 
     def apply(self, step: Step, x):
         x = np.ascontiguousarray(x, np.float64)

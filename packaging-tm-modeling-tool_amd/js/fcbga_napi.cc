@@ -205,6 +205,7 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
       get_double(env, argv[2], "tol", &o.tol);
       get_int(env, argv[2], "maxit", &o.maxit);
       get_int(env, argv[2], "sweeps", &o.sweeps);
+      get_int(env, argv[2], "warm", &o.warm);
       napi_value v;
       char buf[32];
       size_t len;
@@ -238,7 +239,7 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
   set_num(env, rec, "pcgMs", S.ms_pcg);
   set_num(env, rec, "levels", S.levels);
   set_num(env, rec, "iters0", S.r0);
-  set_num(env, rec, "warm", 0);
+  set_num(env, rec, "warm", S.warm);
   napi_value t;
   napi_get_boolean(env, true, &t);
   napi_set_named_property(env, rec, "rebuilt", t);

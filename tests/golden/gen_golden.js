@@ -1,5 +1,5 @@
 // Golden vectors from the reference JS core (run in the build container only:
-//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|<spec name> ...]
+//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|<spec name> ...]
 // ). It loads src/core/*.js the way the reference's own tools/dev-core.mjs does,
 // builds each committed mesh spec (*_spec.json) with the reference mesher's
 // buildStructuredMesh, assembles one thermal step and solves it with the
@@ -9,6 +9,7 @@
 //   element_golden.json                 (condensed element tables on sampled boxes)
 //   anand_golden.json                   (Anand material-point step sequences)
 //   <name>_stress_golden.json           (element states, point stresses, die surface maps, joint loads)
+//   <name>_warm_golden.json, <name>_warm_u.f64.gz (warm-started solves along a path with T-dependent moduli)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -221,7 +222,41 @@ function stressCases(name) {
   console.log(name, 'stress golden: elements', out.elements.length, out.die_top ? 'die peak ' + out.die_top.peakInterior : '', out.joints ? 'worst N ' + out.joints.worstN : '');
 }
 
+// Warm start across the steps of a path (LinearSolver._warmStart/_storeSolution,
+// src/core/10-solver.js:885-926): temperature-dependent moduli (every material but the
+// die falls linearly from 1.2 E at -55 C to 0.6 E at 260 C) so that each step's matrix
+// differs; a fresh preconditioner per step (the step index as the solve key), the default
+// basis of 20 solutions.  Per step: the group data of prepareStep, the solve record and u.
+function warmCases(name) {
+  const spec = JSON.parse(fs.readFileSync(path.join(HERE, name + '_spec.json'), 'utf8'));
+  const groups = spec.materials.map(m => ({ name: m.name, key: m.name, kind: 'homog', mat: m.name, stage: m.stage || 0, part: m.part, column: !!m.column }));
+  const mesh = buildStructuredMesh(spec.xs, spec.ys, spec.zs, groups, assignFor(spec));
+  mesh.constraints = constraints321(mesh, spec.constraint_span);
+  if (spec.feature_x && spec.feature_x.length) { mesh.featureX = Int32Array.from(spec.feature_x); mesh.featureY = Int32Array.from(spec.feature_y); }
+  const mats = spec.materials.map(m => {
+    const t = testMaterial(m);
+    if (m.part !== 4) t.elastic.E = { type: 'table', T: [-55, 260], v: [1.2 * m.E / 1000, 0.6 * m.E / 1000], interp: 'linear', extrap: 'linear', u: 'GPa' };
+    return t;
+  });
+  const model = new FEModel(mesh, { materials: mats });
+  const solver = new LinearSolver(model, { precond: 'mg', tol: 1e-12 });
+  const temps = [150, 110, 70, 40, 25];
+  const steps = [], us = [];
+  for (let k = 0; k + 1 < temps.length; k++) {
+    const rhs = new Float64Array(model.ndof);
+    model.assembleStep(temps[k], temps[k + 1], 0, rhs);
+    const gd = model.groups.map(g => ({ D: list(g._D), s0: list(g._s0), born: !!g._born }));
+    const res = solver.solve(rhs, 'step ' + k, k);
+    steps.push({ Ta: temps[k], Tb: temps[k + 1], groups: gd, iters: res.iters, iters0: res.iters0, relres: res.relres, warm: res.warm });
+    us.push(...res.u);
+  }
+  fs.writeFileSync(path.join(HERE, name + '_warm_golden.json'), JSON.stringify({ name, tol: 1e-12, steps }));
+  f64gz(us, name + '_warm_u.f64.gz');
+  console.log(name, 'warm steps', steps.map(s => s.iters + '/' + s.iters0.toExponential(2)).join(' '));
+}
+
 const what = process.argv.slice(3);
+if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
 if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(n);
 if (what.includes('anand')) anandCases();
 if (!what.length || what.includes('element')) elementCases();
