@@ -1375,6 +1375,117 @@ static double dotv(const double *a, const double *b, int64_t n) {
 }
 
 /* LinearSolver.solve with a cold start (src/core/10-solver.js:933-994) and _pcg (:1001-1042) */
+/* ---- IC(0) with the Manteuffel shift (scalarUpperCSR, ICPC: src/core/10-solver.js:718-800) ---- */
+typedef struct icpc {
+  int64_t n;
+  int64_t *rp;
+  int32_t *col;
+  double *v;
+  double shift;
+} icpc;
+
+static int ic_factor(icpc *U) {
+  const int64_t n = U->n;
+  const int64_t *rp = U->rp;
+  const int32_t *col = U->col;
+  double *v = U->v;
+  for (int64_t i = 0; i < n; i++) {
+    const double d = v[rp[i]];
+    if (!(d > 0)) return 0;
+    const double uii = sqrt(d);
+    v[rp[i]] = uii;
+    for (int64_t k = rp[i] + 1; k < rp[i + 1]; k++) v[k] /= uii;
+    for (int64_t k1 = rp[i] + 1; k1 < rp[i + 1]; k1++) {
+      const int32_t j1 = col[k1];
+      const double u1 = v[k1];
+      int64_t kk = rp[j1];
+      const int64_t kend = rp[j1 + 1];
+      for (int64_t k2 = k1; k2 < rp[i + 1]; k2++) {
+        const int32_t j2 = col[k2];
+        while (kk < kend && col[kk] < j2) kk++;
+        if (kk < kend && col[kk] == j2) v[kk] -= u1 * v[k2];
+      }
+    }
+  }
+  return 1;
+}
+
+/* scalar upper CSR of the (scaled) block matrix, then the factor with the
+ * shift schedule 0, 1e-3, 1e-2, ... capped at 1 (8 attempts); NULL on failure */
+static icpc *ic_build(const or_sys *S) {
+  const int32_t nn = S->nn;
+  icpc *U = calloc(1, sizeof *U);
+  U->n = 3 * (int64_t)nn;
+  U->rp = malloc(sizeof(int64_t) * (U->n + 1));
+  U->rp[0] = 0;
+  for (int32_t i = 0; i < nn; i++)
+    for (int p = 0; p < 3; p++) {
+      int64_t c = 0;
+      for (int32_t k = S->rowPtr[i]; k < S->rowPtr[i + 1]; k++) {
+        const int32_t j = S->col[k];
+        if (j > i) c += 3;
+        else if (j == i) c += 3 - p;
+      }
+      U->rp[3 * i + p + 1] = U->rp[3 * i + p] + c;
+    }
+  const int64_t nnz = U->rp[U->n];
+  U->col = malloc(sizeof(int32_t) * nnz);
+  double *v0 = malloc(sizeof(double) * nnz);
+  for (int32_t i = 0; i < nn; i++)
+    for (int p = 0; p < 3; p++) {
+      int64_t w = U->rp[3 * i + p];
+      for (int32_t k = S->rowPtr[i]; k < S->rowPtr[i + 1]; k++) {
+        const int32_t j = S->col[k];
+        if (j < i) continue;
+        for (int q = (j == i ? p : 0); q < 3; q++) {
+          U->col[w] = 3 * j + q;
+          v0[w] = S->val[9 * (int64_t)k + 3 * p + q];
+          w++;
+        }
+      }
+    }
+  U->v = malloc(sizeof(double) * nnz);
+  double shift = 0;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    memcpy(U->v, v0, sizeof(double) * nnz);
+    if (shift > 0)
+      for (int64_t i = 0; i < U->n; i++) U->v[U->rp[i]] *= 1 + shift;
+    if (ic_factor(U)) {
+      U->shift = shift;
+      free(v0);
+      return U;
+    }
+    shift = shift == 0 ? 1e-3 : shift * 10; /* IC_SHIFT0, src/core/00-constants.js:83 */
+    if (shift > 1) shift = 1;               /* IC_SHIFT_MAX, src/core/00-constants.js:84 */
+  }
+  free(v0);
+  free(U->rp); free(U->col); free(U->v); free(U);
+  return NULL;
+}
+
+static void ic_free(icpc *U) {
+  if (!U) return;
+  free(U->rp); free(U->col); free(U->v); free(U);
+}
+
+/* z = (U^T U)^-1 r (ICPC.apply) */
+static void ic_apply(const icpc *U, const double *r, double *z) {
+  const int64_t n = U->n, *rp = U->rp;
+  const int32_t *col = U->col;
+  const double *v = U->v;
+  memcpy(z, r, sizeof(double) * n);
+  for (int64_t i = 0; i < n; i++) {
+    const double yi = z[i] / v[rp[i]];
+    z[i] = yi;
+    for (int64_t k = rp[i] + 1; k < rp[i + 1]; k++) z[col[k]] -= v[k] * yi;
+  }
+  for (int64_t i = n - 1; i >= 0; i--) {
+    double sum = z[i];
+    for (int64_t k = rp[i] + 1; k < rp[i + 1]; k++) sum -= v[k] * z[col[k]];
+    z[i] = sum / v[rp[i]];
+  }
+}
+
 int or_solve(or_sys *S, double tol, int32_t maxit, int32_t precond, double *u, or_stats *st) {
   double t0 = now_ms();
   memset(st, 0, sizeof *st);
@@ -1410,19 +1521,28 @@ int or_solve(or_sys *S, double tol, int32_t maxit, int32_t precond, double *u, o
   }
   double t1 = now_ms();
   mg *M = precond == 0 ? mg_build(S, s) : NULL;
+  icpc *IC = precond == 2 ? ic_build(S) : NULL;
+  if (precond == 2 && !IC) { st->ms_total = now_ms() - t0; free(s); free(b); free(x); free(r); free(z); free(p); free(q); return -1; }
   st->ms_setup = now_ms() - t1;
   if (M) { st->levels = M->nl; st->coarsest_dof = M->L[M->nl - 1]->denseN; }
+  if (IC) st->ic_shift = IC->shift;
   double t2 = now_ms();
   const uint8_t *isC = S->isC;
-  or_matvec(S, x, r);
-  for (int64_t i = 0; i < n; i++) r[i] = b[i] - r[i];
-  for (int64_t i = 0; i < n; i++) if (isC[i]) { r[i] = 0; x[i] = 0; }
-  double rnorm = sqrt(dotv(r, r, n));
-  st->r0 = rnorm / bnorm;
-  int it = 0, conv = 0;
-  if (rnorm / bnorm < tol) conv = 1;
-  else {
-    if (M) mg_cycle(M, 0, r, z); else memcpy(z, r, sizeof(double) * n);
+  int it = 0, conv = 0, total = 0;
+  double rnorm = 0;
+  /* _pcg from the current x; after a failed run with one preconditioner the
+   * reference rebuilds the other once and continues from the same iterate
+   * (LinearSolver.solve, src/core/10-solver.js:960-972) */
+  for (int attempt = 0; attempt < 2; attempt++) {
+    or_matvec(S, x, r);
+    for (int64_t i = 0; i < n; i++) r[i] = b[i] - r[i];
+    for (int64_t i = 0; i < n; i++) if (isC[i]) { r[i] = 0; x[i] = 0; }
+    rnorm = sqrt(dotv(r, r, n));
+    if (attempt == 0) st->r0 = rnorm / bnorm;
+    it = 0;
+    if (rnorm / bnorm < tol) { conv = 1; break; }
+#define OR_PC(rr, zz) do { if (M) mg_cycle(M, 0, rr, zz); else if (IC) ic_apply(IC, rr, zz); else memcpy(zz, rr, sizeof(double) * n); } while (0)
+    OR_PC(r, z);
     for (int64_t i = 0; i < n; i++) if (isC[i]) z[i] = 0;
     memcpy(p, z, sizeof(double) * n);
     double rz = dotv(r, z, n);
@@ -1434,21 +1554,32 @@ int or_solve(or_sys *S, double tol, int32_t maxit, int32_t precond, double *u, o
       for (int64_t i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
       rnorm = sqrt(dotv(r, r, n));
       if (rnorm / bnorm < tol) { conv = 1; break; }
-      if (M) mg_cycle(M, 0, r, z); else memcpy(z, r, sizeof(double) * n);
+      OR_PC(r, z);
       for (int64_t i = 0; i < n; i++) if (isC[i]) z[i] = 0;
       double rz2 = dotv(r, z, n);
       double beta = rz2 / rz;
       rz = rz2;
       for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
     }
+#undef OR_PC
     if (it > maxit) it = maxit;
+    total += it;
+    if (conv || precond == 1 || attempt == 1) break;
+    /* fallback: the other preconditioner */
+    st->fallback_iters = total;
+    st->fallback_relres = rnorm / bnorm;
+    if (M) { mg_free(M); M = NULL; IC = ic_build(S); if (IC) st->ic_shift = IC->shift; }
+    else { ic_free(IC); IC = NULL; M = mg_build(S, s); }
+    if (!M && !IC) break;
   }
+  it = total;
   st->ms_pcg = now_ms() - t2;
   st->iters = it;
   st->converged = conv;
   st->relres = rnorm / bnorm;
   for (int64_t i = 0; i < n; i++) u[i] = x[i] * s[i];
   if (M) mg_free(M);
+  ic_free(IC);
   free(s); free(b); free(x); free(r); free(z); free(p); free(q);
   st->ms_total = now_ms() - t0;
   return 0;

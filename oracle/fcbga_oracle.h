@@ -74,6 +74,10 @@ typedef struct or_stats {
   double ms_pcg;
   int32_t levels;
   int32_t coarsest_dof;
+  int32_t fallback_iters;   /* iterations of the first preconditioner when it failed (0: no fallback) */
+  int32_t pad;
+  double fallback_relres;
+  double ic_shift;          /* Manteuffel shift of the IC(0) factor */
 } or_stats;
 
 or_sys *or_build(const or_model *m);             /* pattern + assembled full BCSR + rhs (constraints applied) */
@@ -88,7 +92,10 @@ double or_sys_assemble_ms(const or_sys *s);
 int32_t or_sys_ntypes(const or_sys *s);
 void or_matvec(const or_sys *s, const double *x, double *y); /* y = K x, full BCSR */
 /* Jacobi-scaled PCG (src/core/10-solver.js:852-866, 1001-1042); precond 0 = multigrid
- * (src/core/10-solver.js:620-715), 1 = none (Jacobi only). The values are scaled in place. */
+ * (src/core/10-solver.js:620-715), 1 = none (Jacobi only), 2 = IC(0) with the Manteuffel
+ * shift (src/core/10-solver.js:718-800).  A multigrid or IC(0) run that does not converge
+ * continues once with the other preconditioner from the same iterate (src/core/10-solver.js:
+ * 960-972); iters counts both.  The values are scaled in place.  -1: IC(0) failed at every shift. */
 int or_solve(or_sys *s, double tol, int32_t maxit, int32_t precond, double *u, or_stats *st);
 
 /* ---- post (src/core/12-post.js:30-99, src/core/01-util.js:176-190) ---- */

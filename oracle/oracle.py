@@ -36,7 +36,9 @@ class OrModel(C.Structure):
 class OrStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double), ("r0", C.c_double),
                 ("ms_total", C.c_double), ("ms_assemble", C.c_double), ("ms_setup", C.c_double),
-                ("ms_pcg", C.c_double), ("levels", C.c_int32), ("coarsest_dof", C.c_int32)]
+                ("ms_pcg", C.c_double), ("levels", C.c_int32), ("coarsest_dof", C.c_int32),
+                ("fallback_iters", C.c_int32), ("pad", C.c_int32), ("fallback_relres", C.c_double),
+                ("ic_shift", C.c_double)]
 
 
 class OrWarpage(C.Structure):
@@ -236,7 +238,9 @@ class System:
     def solve(self, tol=1e-10, maxit=2000, precond=0):
         u = np.zeros(self.ndof)
         st = OrStats()
-        lib().or_solve(self.h, tol, maxit, precond, _p(u, _f64p), C.byref(st))
+        rc = lib().or_solve(self.h, tol, maxit, precond, _p(u, _f64p), C.byref(st))
+        if rc != 0:
+            raise RuntimeError("IC(0) failed even with the maximum diagonal shift")
         self.solved = True
         return u, {f: getattr(st, f) for f, _ in OrStats._fields_}
 

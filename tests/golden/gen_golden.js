@@ -1,5 +1,5 @@
 // Golden vectors from the reference JS core (run in the build container only:
-//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|<spec name> ...]
+//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|ic|<spec name> ...]
 // ). It loads src/core/*.js the way the reference's own tools/dev-core.mjs does,
 // builds each committed mesh spec (*_spec.json) with the reference mesher's
 // buildStructuredMesh, assembles one thermal step and solves it with the
@@ -10,6 +10,8 @@
 //   anand_golden.json                   (Anand material-point step sequences)
 //   <name>_stress_golden.json           (element states, point stresses, die surface maps, joint loads)
 //   <name>_warm_golden.json, <name>_warm_u.f64.gz (warm-started solves along a path with T-dependent moduli)
+//   <name>_ic_golden.json, <name>_ic_u.f64.gz     (IC(0)-preconditioned solves)
+//   fallback_golden.json                (the fallback to the other preconditioner)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -255,7 +257,48 @@ function warmCases(name) {
   console.log(name, 'warm steps', steps.map(s => s.iters + '/' + s.iters0.toExponential(2)).join(' '));
 }
 
+// IC(0)-preconditioned solves (ICPC, src/core/10-solver.js:740-800, the Manteuffel
+// shift schedule included) of the committed specs: iterations, shift, solution.
+function icCases(name) {
+  const spec = JSON.parse(fs.readFileSync(path.join(HERE, name + '_spec.json'), 'utf8'));
+  const groups = spec.materials.map(m => ({ name: m.name, kind: 'homog', mat: m.name, stage: m.stage || 0, part: m.part, column: !!m.column }));
+  const mesh = buildStructuredMesh(spec.xs, spec.ys, spec.zs, groups, assignFor(spec));
+  mesh.constraints = constraints321(mesh, spec.constraint_span);
+  const model = new FEModel(mesh, { materials: spec.materials.map(testMaterial) });
+  const solver = new LinearSolver(model, { precond: 'ic', tol: 1e-10, warm: 0 });
+  const rhs = new Float64Array(model.ndof);
+  model.assembleStep(spec.Ta, spec.Tb, spec.stage || 0, rhs);
+  const res = solver.solve(rhs, name);
+  const out = { name, tol: 1e-10, iters: res.iters, relres: res.relres, converged: res.converged, precond: res.precond,
+    shift: res.precond === 'ic' ? solver.pc.shift : null, fallback: res.fallback };
+  fs.writeFileSync(path.join(HERE, name + '_ic_golden.json'), JSON.stringify(out));
+  if (model.ndof <= 20000) f64gz(res.u, name + '_ic_u.f64.gz');
+  console.log(name, 'IC(0): iters', res.iters, 'precond', res.precond, 'shift', out.shift, 'fallback', JSON.stringify(res.fallback), 'relres', res.relres.toExponential(2));
+}
+
+// The fallback to the other preconditioner after a failed run (LinearSolver.solve,
+// src/core/10-solver.js:960-972), probed with small iteration caps.
+function fallbackCases() {
+  const out = [];
+  for (const [name, maxit, precond] of [['plate_16', 50, 'ic'], ['plate_16', 5, 'mg'], ['cfg1', 40, 'ic']]) {
+    const spec = JSON.parse(fs.readFileSync(path.join(HERE, name + '_spec.json'), 'utf8'));
+    const groups = spec.materials.map(m => ({ name: m.name, kind: 'homog', mat: m.name, stage: m.stage || 0, part: m.part, column: !!m.column }));
+    const mesh = buildStructuredMesh(spec.xs, spec.ys, spec.zs, groups, assignFor(spec));
+    mesh.constraints = constraints321(mesh, spec.constraint_span);
+    if (spec.feature_x && spec.feature_x.length) { mesh.featureX = Int32Array.from(spec.feature_x); mesh.featureY = Int32Array.from(spec.feature_y); }
+    const model = new FEModel(mesh, { materials: spec.materials.map(testMaterial) });
+    const solver = new LinearSolver(model, { precond, tol: 1e-10, warm: 0, maxit });
+    const rhs = new Float64Array(model.ndof);
+    model.assembleStep(spec.Ta, spec.Tb, spec.stage || 0, rhs);
+    const res = solver.solve(rhs, name);
+    out.push({ name, maxit, first: precond, iters: res.iters, relres: res.relres, converged: res.converged, precond: res.precond, fallback: res.fallback });
+  }
+  fs.writeFileSync(path.join(HERE, 'fallback_golden.json'), JSON.stringify(out, null, 1));
+  console.log('fallback cases', JSON.stringify(out.map(o => [o.name, o.first, o.iters, o.precond])));
+}
+
 const what = process.argv.slice(3);
+if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }
 if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
 if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(n);
 if (what.includes('anand')) anandCases();

@@ -185,3 +185,43 @@ def test_warpage_dome_sign_and_magnitude(oracle):
         # residual range of the sampled bilinear field over the BGA sites
         wb = sgn * 1e-4 * (400 - (bx ** 2 + by ** 2))
         assert abs(w["mag"] - (wb.max() - wb.min())) < 0.02 * w["mag"]
+
+
+def _ic_golden(name):
+    with open(os.path.join(GOLDEN, name + "_ic_golden.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("name", ["small_box", "plate_16"])
+def test_oracle_ic0_matches_reference(oracle, name):
+    """IC(0) with the Manteuffel shift (ICPC, src/core/10-solver.js:740-800):
+    the same shift and the reference's iteration count (to rounding over a slow
+    1000-iteration run) and solution."""
+    spec = golden_spec(name)
+    m = meshspec.build_mesh(spec)
+    D, s0, born = meshspec.group_step_data(spec, spec["Ta"], spec["Tb"], 0)
+    g = _ic_golden(name)
+    u, st = oracle.System(m, D, s0, born).solve(tol=g["tol"], maxit=2000, precond=2)
+    assert st["converged"] and st["fallback_iters"] == 0
+    assert st["ic_shift"] == g["shift"]
+    assert abs(st["iters"] - g["iters"]) <= 0.01 * g["iters"]
+    uref = golden_f64(name + "_ic_u.f64.gz")
+    assert np.abs(u - uref).max() <= 1e-7 * np.abs(uref).max()
+
+
+def test_oracle_fallback_matches_reference(oracle):
+    """After a run that does not converge the other preconditioner continues
+    from the same iterate (src/core/10-solver.js:960-972): the reference's
+    iteration counts and residuals on short capped runs."""
+    with open(os.path.join(GOLDEN, "fallback_golden.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        spec = golden_spec(c["name"])
+        m = meshspec.build_mesh(spec)
+        D, s0, born = meshspec.group_step_data(spec, spec["Ta"], spec["Tb"], 0)
+        pc = {"mg": 0, "ic": 2}[c["first"]]
+        u, st = oracle.System(m, D, s0, born).solve(tol=1e-10, maxit=c["maxit"], precond=pc)
+        assert st["iters"] == c["iters"] and bool(st["converged"]) == c["converged"], (c, st)
+        assert st["fallback_iters"] == c["fallback"]["iters"]
+        assert abs(st["fallback_relres"] - c["fallback"]["relres"]) <= 1e-6 * c["fallback"]["relres"]
+        assert abs(st["relres"] - c["relres"]) <= 1e-4 * c["relres"]
