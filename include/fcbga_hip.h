@@ -73,6 +73,11 @@ typedef struct fcbga_step {
 #define FCBGA_PC_MG 0      /* shell-aggregate multigrid (src/core/10-solver.js:620-715) */
 #define FCBGA_PC_COLUMN 1  /* column block Gauss-Seidel sweeps only */
 #define FCBGA_PC_JACOBI 2  /* Jacobi scaling only */
+#define FCBGA_PC_IC 3      /* IC(0) with the Manteuffel shift (ICPC, src/core/10-solver.js:718-800), FP64 */
+
+/* MG and IC(0) runs that do not converge continue once with the other one from
+ * the same iterate (LinearSolver.solve, src/core/10-solver.js:965-973); the
+ * record's precond is then the second one and fallback_* describe the first. */
 
 typedef struct fcbga_solve_opts {
   double tol;         /* relative residual in the Jacobi-scaled norm (src/core/10-solver.js:1014) */
@@ -102,7 +107,11 @@ typedef struct fcbga_solve_stats {
   int32_t ntypes;     /* distinct condensed element matrices */
   int32_t sym_tables; /* 1: the fine operator ran on the mirror-symmetric compact element tables */
   int32_t warm;       /* warm-start basis size after the solve (the record's warm field) */
-  int32_t reserved;
+  int32_t precond;    /* preconditioner of the final PCG run (the record's precond) */
+  int32_t fallback_first;  /* -1: no fallback; else the preconditioner of the run that did not converge */
+  int32_t fallback_iters;  /* iterations before the fallback (the record's fallback.iters) */
+  double fallback_relres;  /* relative residual when the fallback started (fallback.relres) */
+  double ic_shift;         /* Manteuffel shift of the IC(0) factor in use (ICPC.info.shift), 0 without IC */
 } fcbga_solve_stats;
 
 int fcbga_model_create(const fcbga_mesh *mesh, fcbga_model **out);

@@ -255,6 +255,14 @@ struct fcbga_model {
   DBuf<int32_t> nodeAgg;
   DBuf<double> zp64;
   DBuf<float> zp32;
+  // IC(0): upper-pattern slot bits per grid node, nodes by dependency level, factor (14 3x3 slots per grid node)
+  bool icPattern = false;
+  DBuf<uint16_t> icMask;
+  DBuf<int32_t> icNodes, icFail;
+  std::vector<int64_t> icLvPtr;
+  DBuf<double> icU;
+  double icShift = 0;
+  bool colsFactored = false;  // fine column factors of the prepared step exist
   // kernel timing (measurement hook): event pairs per kernel class
   bool timing = false;
   int nsweeps = 2;     // pre- and post-smoothing sweeps per level
@@ -1899,6 +1907,415 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
 }
 
 // ---------------------------------------------------------------------------
+// IC(0) with the Manteuffel shift (ICPC + scalarUpperCSR, src/core/10-solver.js:
+// 718-800).  Node order is grid order (z fastest) and the reference's scalar
+// row 3n+p, so the upper pattern of node n is itself and those of its 26 grid
+// neighbours with a larger index that share an element with it: at most 14
+// "slots" of 3x3, slot t at the grid offset ic_off(t), in increasing index
+// order.  The factor U is kept in that layout (126 doubles per grid node; the
+// self slot uses its upper triangle).  The factor is left-looking: entry (a, b)
+// of row a receives -u_ia u_ib from every earlier row i holding both columns,
+// in ascending i, then the pivot's square root and the division, which is the
+// sequence of updates the reference's right-looking loop applies.  Nodes are
+// processed level by level over the dependency graph (a node depends on its
+// lower neighbours), one wavefront per node; both triangular solves run the
+// same levels, forward ascending and backward descending.
+// ---------------------------------------------------------------------------
+constexpr int IC_SLOTS = 14, IC_STRIDE = 9 * IC_SLOTS, IC_WAVES = 4;
+
+FE_HD void ic_off(int t, int &di, int &dj, int &dk) {
+  if (t == 0) { di = 0; dj = 0; dk = 0; }
+  else if (t == 1) { di = 0; dj = 0; dk = 1; }
+  else if (t <= 4) { di = 0; dj = 1; dk = t - 3; }
+  else { di = 1; dj = (t - 5) / 3 - 1; dk = (t - 5) % 3 - 1; }
+}
+
+// slot of an offset in [-1, 1]^3 that points to a larger grid index (or 0 for
+// the node itself); -1 otherwise
+FE_HD int ic_slot(int di, int dj, int dk) {
+  if (di == 0 && dj == 0) return dk == 0 ? 0 : (dk == 1 ? 1 : -1);
+  if (di == 0) return dj == 1 ? 3 + dk : -1;
+  if (di == 1) return 5 + (dj + 1) * 3 + (dk + 1);
+  return -1;
+}
+
+// Lower-neighbour updates of the entries (n p, nb q), p = 0..2, of node
+// (i, j, k), nb in slot t: acc[p] -= u_{(m,r),(n,p)} u_{(m,r),(nb,q)} over the
+// lower neighbours m in ascending order (descending slot of n in m) and their
+// rows r = 0..2.
+FE_HD void ic_lower_updates(const fe::Grid &g, int i, int j, int k, int t, int q, const uint16_t *mask,
+                                   const double *U, double acc[3]) {
+  int di, dj, dk;
+  ic_off(t, di, dj, dk);
+  for (int tl = IC_SLOTS - 1; tl >= 1; tl--) {
+    int ei, ej, ek;
+    ic_off(tl, ei, ej, ek);
+    const int mi = i - ei, mj = j - ej, mk = k - ek;
+    if (mi < 0 || mj < 0 || mj >= g.ny || mk < 0 || mk >= g.nz) continue;
+    const int64_t gm = ((int64_t)mi * g.ny + mj) * g.nz + mk;
+    const uint32_t mm = mask[gm];
+    if (!((mm >> tl) & 1u)) continue;
+    const int si = di + ei, sj = dj + ej, sk = dk + ek;  // offset of nb from m
+    if (si > 1 || sj < -1 || sj > 1 || sk < -1 || sk > 1) continue;
+    const int sb = ic_slot(si, sj, sk);
+    if (sb <= 0 || !((mm >> sb) & 1u)) continue;
+    const double *Um = U + gm * IC_STRIDE;
+    for (int r = 0; r < 3; r++) {
+      const double ub = Um[sb * 9 + 3 * r + q];
+      for (int p = 0; p < 3; p++) acc[p] -= Um[tl * 9 + 3 * r + p] * ub;
+    }
+  }
+}
+
+// forward solve U^T y = r, lane (slot tl of node n in its lower neighbour m, row r of m):
+// s[p] = u_{(m,r),(n,p)} y_{(m,r)}
+FE_HD void ic_fwd_lane(const fe::Grid &g, int i, int j, int k, int tl, int r, const uint16_t *mask,
+                              const double *U, const double *y, double s[3]) {
+  int ei, ej, ek;
+  ic_off(tl, ei, ej, ek);
+  const int mi = i - ei, mj = j - ej, mk = k - ek;
+  if (mi < 0 || mj < 0 || mj >= g.ny || mk < 0 || mk >= g.nz) return;
+  const int64_t gm = ((int64_t)mi * g.ny + mj) * g.nz + mk;
+  if (!((mask[gm] >> tl) & 1u)) return;
+  const double ym = y[3 * gm + r];
+  const double *u = U + gm * IC_STRIDE + tl * 9 + 3 * r;
+  for (int p = 0; p < 3; p++) s[p] = u[p] * ym;
+}
+
+// the node's own rows: y_p = (r_p - s_p - sum_{r<p} u_rp y_r) / u_pp
+FE_HD void ic_fwd_finish(const double *Un, const double *rn, const double s[3], double *yn) {
+  const double y0 = (rn[0] - s[0]) / Un[0];
+  const double y1 = (rn[1] - s[1] - Un[1] * y0) / Un[4];
+  const double y2 = (rn[2] - s[2] - Un[2] * y0 - Un[5] * y1) / Un[8];
+  yn[0] = y0; yn[1] = y1; yn[2] = y2;
+}
+
+// backward solve U z = y, lane (upper slot t >= 1, column q): s[p] = u_{(n,p),(nb,q)} z_{(nb,q)}
+FE_HD void ic_bwd_lane(const fe::Grid &g, int64_t gi, int i, int j, int k, int t, int q, uint32_t mk,
+                              const double *U, const double *z, double s[3]) {
+  if (!((mk >> t) & 1u)) return;
+  int di, dj, dk;
+  ic_off(t, di, dj, dk);
+  const int64_t gb = ((int64_t)(i + di) * g.ny + (j + dj)) * g.nz + (k + dk);
+  const double zb = z[3 * gb + q];
+  const double *u = U + gi * IC_STRIDE + t * 9 + q;
+  for (int p = 0; p < 3; p++) s[p] = u[3 * p] * zb;
+}
+
+// zn holds y on entry
+FE_HD void ic_bwd_finish(const double *Un, const double s[3], double *zn) {
+  const double z2 = (zn[2] - s[2]) / Un[8];
+  const double z1 = (zn[1] - s[1] - Un[5] * z2) / Un[4];
+  const double z0 = (zn[0] - s[0] - Un[1] * z1 - Un[2] * z2) / Un[0];
+  zn[0] = z0; zn[1] = z1; zn[2] = z2;
+}
+
+#if FCBGA_GPU
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// one level of the factor: wavefront w owns node nodes[w]; lane 3t+q (< 42)
+// owns the column (slot t, q) of the node's three rows
+__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_factor(fe::Grid g, const int32_t *__restrict__ nodes,
+                                                             int32_t cnt, const uint16_t *__restrict__ mask,
+                                                             double *U, int32_t *fail) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
+  if (w >= cnt) return;
+  const int64_t gi = nodes[w];
+  const int k = (int)(gi % g.nz);
+  const int64_t cj = gi / g.nz;
+  const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+  const uint32_t mk = mask[gi];
+  const int t = lane / 3, q = lane - 3 * (lane / 3);
+  const bool act = lane < 3 * IC_SLOTS && ((mk >> t) & 1u);
+  double *Un = U + gi * IC_STRIDE;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (act) {
+    for (int p = 0; p < 3; p++) acc[p] = Un[t * 9 + 3 * p + q];
+    ic_lower_updates(g, i, j, k, t, q, mask, U, acc);
+  }
+  // the node's own rows, in order; lane p holds the diagonal of row p
+  const double d0 = __shfl(acc[0], 0);
+  const double u00 = sqrt(d0);
+  const double u0 = lane == 0 ? u00 : acc[0] / u00;
+  const double u01 = __shfl(u0, 1), u02 = __shfl(u0, 2);
+  acc[1] -= u01 * u0;
+  const double d1 = __shfl(acc[1], 1);
+  const double u11 = sqrt(d1);
+  const double u1 = lane == 1 ? u11 : acc[1] / u11;
+  const double u12 = __shfl(u1, 2);
+  acc[2] -= u02 * u0;
+  acc[2] -= u12 * u1;
+  const double d2 = __shfl(acc[2], 2);
+  const double u22 = sqrt(d2);
+  const double u2 = lane == 2 ? u22 : acc[2] / u22;
+  if (act) {
+    Un[t * 9 + q] = u0;
+    if (t > 0 || q >= 1) Un[t * 9 + 3 + q] = u1;
+    if (t > 0 || q >= 2) Un[t * 9 + 6 + q] = u2;
+  }
+  if (lane == 0 && !(d0 > 0 && d1 > 0 && d2 > 0)) fail[0] = 1;
+}
+
+__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_forward(fe::Grid g, const int32_t *__restrict__ nodes,
+                                                              int32_t cnt, const uint16_t *__restrict__ mask,
+                                                              const double *__restrict__ U,
+                                                              const double *__restrict__ r, double *z) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
+  if (w >= cnt) return;
+  const int64_t gi = nodes[w];
+  const int k = (int)(gi % g.nz);
+  const int64_t cj = gi / g.nz;
+  const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+  double s[3] = {0.0, 0.0, 0.0};
+  if (lane >= 3 && lane < 3 * IC_SLOTS) ic_fwd_lane(g, i, j, k, lane / 3, lane % 3, mask, U, z, s);
+  for (int p = 0; p < 3; p++) s[p] = wave_sum(s[p]);
+  if (lane == 0) ic_fwd_finish(U + gi * IC_STRIDE, r + 3 * gi, s, z + 3 * gi);
+}
+
+__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_backward(fe::Grid g, const int32_t *__restrict__ nodes,
+                                                               int32_t cnt, const uint16_t *__restrict__ mask,
+                                                               const double *__restrict__ U, double *z) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
+  if (w >= cnt) return;
+  const int64_t gi = nodes[w];
+  const int k = (int)(gi % g.nz);
+  const int64_t cj = gi / g.nz;
+  const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+  double s[3] = {0.0, 0.0, 0.0};
+  if (lane >= 3 && lane < 3 * IC_SLOTS) ic_bwd_lane(g, gi, i, j, k, lane / 3, lane % 3, mask[gi], U, z, s);
+  for (int p = 0; p < 3; p++) s[p] = wave_sum(s[p]);
+  if (lane == 0) ic_bwd_finish(U + gi * IC_STRIDE, s, z + 3 * gi);
+}
+#endif
+
+// pattern slots and dependency levels (once per model)
+void ic_pattern(fcbga_model *M) {
+  if (M->icPattern) return;
+  const fe::Grid &g = M->g;
+  std::vector<uint16_t> mask(g.G, 0);
+  for (int64_t q = 0; q < g.G; q++)
+    if (M->node_id[q] >= 0) mask[q] = 1;
+  for (int ci = 0; ci < g.ncx; ci++)
+    for (int cj = 0; cj < g.ncy; cj++)
+      for (int ck = 0; ck < g.ncz; ck++) {
+        const int64_t c = ((int64_t)ci * g.ncy + cj) * g.ncz + ck;
+        if (M->cell_elems[2 * c] < 0 && M->cell_elems[2 * c + 1] < 0) continue;
+        for (int a = 0; a < 8; a++) {
+          const int ai = ci + fe::corner_li(a), aj = cj + fe::corner_lj(a), ak = ck + fe::corner_lk(a);
+          const int64_t ga = ((int64_t)ai * g.ny + aj) * g.nz + ak;
+          for (int b = 0; b < 8; b++) {
+            const int bi = ci + fe::corner_li(b), bj = cj + fe::corner_lj(b), bk = ck + fe::corner_lk(b);
+            const int t = ic_slot(bi - ai, bj - aj, bk - ak);
+            if (t > 0) mask[ga] |= (uint16_t)(1u << t);
+          }
+        }
+      }
+  // level of a node: one more than the deepest of its lower neighbours
+  std::vector<int32_t> level(g.G, -1);
+  int32_t nl = 0;
+  for (int i = 0; i < g.nx; i++)
+    for (int j = 0; j < g.ny; j++)
+      for (int k = 0; k < g.nz; k++) {
+        const int64_t gi = ((int64_t)i * g.ny + j) * g.nz + k;
+        if (!(mask[gi] & 1)) continue;
+        int32_t L = 0;
+        for (int tl = 1; tl < IC_SLOTS; tl++) {
+          int ei, ej, ek;
+          ic_off(tl, ei, ej, ek);
+          const int mi = i - ei, mj = j - ej, mk = k - ek;
+          if (mi < 0 || mj < 0 || mj >= g.ny || mk < 0 || mk >= g.nz) continue;
+          const int64_t gm = ((int64_t)mi * g.ny + mj) * g.nz + mk;
+          if ((mask[gm] >> tl) & 1u) L = std::max(L, level[gm] + 1);
+        }
+        level[gi] = L;
+        nl = std::max(nl, L + 1);
+      }
+  std::vector<int64_t> ptr(nl + 1, 0);
+  for (int64_t q = 0; q < g.G; q++)
+    if (level[q] >= 0) ptr[level[q] + 1]++;
+  for (int32_t l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+  std::vector<int32_t> nodes(ptr[nl]);
+  std::vector<int64_t> fillp(ptr.begin(), ptr.end() - 1);
+  for (int64_t q = 0; q < g.G; q++)
+    if (level[q] >= 0) nodes[fillp[level[q]]++] = (int32_t)q;
+  M->icMask.upload(mask, M->stream);
+  M->icNodes.upload(nodes, M->stream);
+  M->icLvPtr = ptr;
+  M->icFail.alloc(1);
+  M->icPattern = true;
+}
+
+// values of the scaled, constraint-reduced matrix in the upper pattern with
+// the diagonal times (1 + shift), then the factor level by level; false on a
+// non-positive pivot
+bool ic_try(fcbga_model *M, double shift) {
+  const fe::Grid g = M->g;
+  be::stream_t s = M->stream;
+  M->icU.alloc((size_t)IC_STRIDE * g.G);
+  const uint16_t *mask = M->icMask.p;
+  const uint8_t *dm = M->dofmask.p;
+  const int32_t *ct = M->cellType.p;
+  const double *cs = M->cellScale64.p, *Kc = M->Kc64.p, *sv = M->s64.p;
+  double *U = M->icU.p;
+  int32_t *fail = M->icFail.p;
+  be::pfor(1, s, [=] __host__ __device__(int64_t) { fail[0] = 0; });
+  be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
+    const uint32_t mk = mask[gi];
+    if (!(mk & 1u)) return;
+    const int k = (int)(gi % g.nz);
+    const int64_t cj = gi / g.nz;
+    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    double *Un = U + gi * IC_STRIDE;
+    for (int t = 0; t < IC_SLOTS; t++) {
+      if (!((mk >> t) & 1u)) continue;
+      int di, dj, dk;
+      ic_off(t, di, dj, dk);
+      const int64_t gj = ((int64_t)(i + di) * g.ny + (j + dj)) * g.nz + (k + dk);
+      double Kb[9];
+      fe::pair_block(g, i, j, k, i + di, j + dj, k + dk, ct, cs, Kc, Kb);
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) {
+          double v = 0.0;
+          if (t == 0 && q < p) { Un[3 * p + q] = 0.0; continue; }
+          // applyConstraints keeps a constrained DOF's diagonal (1 when it is 0), then _scale
+          if (dm[3 * gi + p] && dm[3 * gj + q]) v = Kb[3 * p + q] * (sv[3 * gi + p] * sv[3 * gj + q]);
+          else if (t == 0 && p == q) v = Kb[3 * p + p] != 0.0 ? Kb[3 * p + p] * (sv[3 * gi + p] * sv[3 * gi + p]) : 1.0;
+          if (t == 0 && p == q) v *= 1.0 + shift;
+          Un[t * 9 + 3 * p + q] = v;
+        }
+    }
+  });
+  const int64_t nl = (int64_t)M->icLvPtr.size() - 1;
+  const int32_t *nodes = M->icNodes.p;
+#if FCBGA_GPU
+  for (int64_t l = 0; l < nl; l++) {
+    const int64_t a = M->icLvPtr[l], cnt = M->icLvPtr[l + 1] - a;
+    if (cnt <= 0) continue;
+    hipLaunchKernelGGL(k_ic_factor, dim3((unsigned)((cnt + IC_WAVES - 1) / IC_WAVES)), dim3(64 * IC_WAVES), 0, s, g,
+                       nodes + a, (int32_t)cnt, mask, U, fail);
+    BE_CHECK(hipGetLastError());
+  }
+  int32_t h = 0;
+  be::d2h(&h, fail, sizeof(int32_t), s);
+  return h == 0;
+#else
+  bool ok = true;
+  for (int64_t w = 0; w < M->icLvPtr[nl]; w++) {
+    const int64_t gi = nodes[w];
+    const int k = (int)(gi % g.nz);
+    const int64_t cj = gi / g.nz;
+    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    const uint32_t mk = mask[gi];
+    double *Un = U + gi * IC_STRIDE;
+    double acc[3][3 * IC_SLOTS];
+    for (int l = 0; l < 3 * IC_SLOTS; l++) {
+      const int t = l / 3, q = l % 3;
+      double a3[3] = {0.0, 0.0, 0.0};
+      if ((mk >> t) & 1u) {
+        for (int p = 0; p < 3; p++) a3[p] = Un[t * 9 + 3 * p + q];
+        ic_lower_updates(g, i, j, k, t, q, mask, U, a3);
+      }
+      for (int p = 0; p < 3; p++) acc[p][l] = a3[p];
+    }
+    for (int p = 0; p < 3; p++) {
+      for (int r = 0; r < p; r++)
+        for (int l = p; l < 3 * IC_SLOTS; l++) acc[p][l] -= acc[r][p] * acc[r][l];
+      const double d = acc[p][p];
+      ok = ok && d > 0;
+      const double u = std::sqrt(d);
+      acc[p][p] = u;
+      for (int l = p + 1; l < 3 * IC_SLOTS; l++) acc[p][l] /= u;
+    }
+    for (int l = 0; l < 3 * IC_SLOTS; l++) {
+      const int t = l / 3, q = l % 3;
+      if (!((mk >> t) & 1u)) continue;
+      for (int p = 0; p < 3; p++)
+        if (t > 0 || q >= p) Un[t * 9 + 3 * p + q] = acc[p][l];
+    }
+  }
+  return ok;
+#endif
+}
+
+// ICPC.build: shifts 0, 1e-3, 1e-2, ... capped at 1, eight attempts
+// (IC_SHIFT0, IC_SHIFT_MAX: src/core/00-constants.js:83-84); returns ms
+double ic_setup(fcbga_model *M) {
+  const double t0 = now_ms();
+  ic_pattern(M);
+  double shift = 0;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    if (ic_try(M, shift)) {
+      M->icShift = shift;
+      be::sync(M->stream);
+      return now_ms() - t0;
+    }
+    shift = shift == 0 ? 1e-3 : shift * 10;
+    if (shift > 1) shift = 1;
+  }
+  throw be::Error("IC(0) failed even with the maximum diagonal shift");
+}
+
+// z = (U^T U)^-1 r (ICPC.apply); z at absent nodes is left as it was
+void ic_apply(fcbga_model *M, const double *r, double *z) {
+  const fe::Grid g = M->g;
+  const int64_t nl = (int64_t)M->icLvPtr.size() - 1;
+  const int32_t *nodes = M->icNodes.p;
+  const uint16_t *mask = M->icMask.p;
+  const double *U = M->icU.p;
+#if FCBGA_GPU
+  be::stream_t s = M->stream;
+  for (int64_t l = 0; l < nl; l++) {
+    const int64_t a = M->icLvPtr[l], cnt = M->icLvPtr[l + 1] - a;
+    if (cnt <= 0) continue;
+    hipLaunchKernelGGL(k_ic_forward, dim3((unsigned)((cnt + IC_WAVES - 1) / IC_WAVES)), dim3(64 * IC_WAVES), 0, s,
+                       g, nodes + a, (int32_t)cnt, mask, U, r, z);
+    BE_CHECK(hipGetLastError());
+  }
+  for (int64_t l = nl - 1; l >= 0; l--) {
+    const int64_t a = M->icLvPtr[l], cnt = M->icLvPtr[l + 1] - a;
+    if (cnt <= 0) continue;
+    hipLaunchKernelGGL(k_ic_backward, dim3((unsigned)((cnt + IC_WAVES - 1) / IC_WAVES)), dim3(64 * IC_WAVES), 0, s,
+                       g, nodes + a, (int32_t)cnt, mask, U, z);
+    BE_CHECK(hipGetLastError());
+  }
+#else
+  const int64_t nn = M->icLvPtr[nl];
+  for (int64_t w = 0; w < nn; w++) {
+    const int64_t gi = nodes[w];
+    const int k = (int)(gi % g.nz);
+    const int64_t cj = gi / g.nz;
+    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int l = 3; l < 3 * IC_SLOTS; l++) {
+      double t3[3] = {0.0, 0.0, 0.0};
+      ic_fwd_lane(g, i, j, k, l / 3, l % 3, mask, U, z, t3);
+      for (int p = 0; p < 3; p++) s[p] += t3[p];
+    }
+    ic_fwd_finish(U + gi * IC_STRIDE, r + 3 * gi, s, z + 3 * gi);
+  }
+  for (int64_t w = nn - 1; w >= 0; w--) {
+    const int64_t gi = nodes[w];
+    const int k = (int)(gi % g.nz);
+    const int64_t cj = gi / g.nz;
+    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int l = 3; l < 3 * IC_SLOTS; l++) {
+      double t3[3] = {0.0, 0.0, 0.0};
+      ic_bwd_lane(g, gi, i, j, k, l / 3, l % 3, mask[gi], U, z, t3);
+      for (int p = 0; p < 3; p++) s[p] += t3[p];
+    }
+    ic_bwd_finish(U + gi * IC_STRIDE, s, z + 3 * gi);
+  }
+#endif
+}
+
+// ---------------------------------------------------------------------------
 // PCG (LinearSolver._pcg, src/core/10-solver.js:1001-1042), cold start
 // ---------------------------------------------------------------------------
 
@@ -1919,6 +2336,11 @@ void precondition(fcbga_model *M, int precision, int precond) {
   double *z = M->z.p;
   if (precond == FCBGA_PC_JACOBI) {
     be::pfor(n3, s, [=] __host__ __device__(int64_t i) { z[i] = mask[i] ? r[i] : 0.0; });
+    return;
+  }
+  if (precond == FCBGA_PC_IC) {
+    ic_apply(M, r, z);
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { if (!mask[i]) z[i] = 0.0; });
     return;
   }
   const bool coarse = precond == FCBGA_PC_MG;
@@ -2156,51 +2578,100 @@ fcbga_solve_opts check_opts(const fcbga_solve_opts *op) {
   if (op) o = *op;
   if (!(o.tol > 0) || o.maxit < 1) throw be::Error("tol must be > 0 and maxit >= 1");
   if (o.precision != FCBGA_PREC_FP64 && o.precision != FCBGA_PREC_MIXED) throw be::Error("unknown precision");
-  if (o.precond < FCBGA_PC_MG || o.precond > FCBGA_PC_JACOBI) throw be::Error("unknown preconditioner");
+  if (o.precond < FCBGA_PC_MG || o.precond > FCBGA_PC_IC) throw be::Error("unknown preconditioner");
   if (o.sweeps < 0 || o.sweeps > 8) throw be::Error("sweeps must be 0 (default 2) .. 8");
   if (o.warm < 0 || o.warm > 64) throw be::Error("warm must be 0 .. 64");
   return o;
 }
 
-// assembly, column factors and multigrid set-up of a step (the operator and
-// preconditioner every later pcg_run of this step uses)
+// set-up of a preconditioner on the prepared operator (_buildPC, src/core/10-solver.js:868-883); ms
+double build_pc(fcbga_model *M, int kind, fcbga_solve_stats &S) {
+  const double t0 = now_ms();
+  if ((kind == FCBGA_PC_MG || kind == FCBGA_PC_COLUMN) && !M->colsFactored) {
+    factor_fine_columns(M);
+    M->colsFactored = true;
+  }
+  if (kind == FCBGA_PC_MG) {
+    mg_setup(M);
+    S.levels = 1 + (int32_t)M->lv.size();
+    S.coarsest_dof = M->lv.back()->ndense;
+  } else if (kind == FCBGA_PC_IC) {
+    ic_setup(M);
+    S.ic_shift = M->icShift;
+  }
+  be::sync(M->stream);
+  return now_ms() - t0;
+}
+
+// assembly, column factors and preconditioner set-up of a step (the operator
+// and preconditioner every later pcg_run of this step uses)
 void prepare_step(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts &o, fcbga_solve_stats &S) {
   check_step(M, st);
   M->nsweeps = o.sweeps ? o.sweeps : 2;
   S.ndof = 3 * (int64_t)M->nn;
   S.ntypes = M->ntypes;
+  S.levels = 1;
   be::stream_t s = M->stream;
   const double t0 = now_ms();
   assemble_step(M, st);
+  M->colsFactored = false;
   S.sym_tables = M->symTables ? 1 : 0;
-  if (o.precond != FCBGA_PC_JACOBI) factor_fine_columns(M);
   alloc_vectors(M, o.precision);
   be::sync(s);
-  const double t1 = now_ms();
-  S.ms_assemble = t1 - t0;
-  if (o.precond == FCBGA_PC_MG) {
-    S.ms_setup = mg_setup(M);
-    S.levels = 1 + (int32_t)M->lv.size();
-    S.coarsest_dof = M->lv.back()->ndense;
-  } else {
-    S.levels = 1;
-  }
+  S.ms_assemble = now_ms() - t0;
+  S.ms_setup = build_pc(M, o.precond, S);
   M->prepGen++;
   M->prepPrecond = o.precond;
   M->prepPrecision = o.precision;
 }
 
-// PCG from x = 0 on the prepared step: right-hand side the step's load, or
-// rhs_grid (grid layout, unscaled, zero on constrained DOFs); du_dev receives
-// the solution in the reference's node order
+// r = b - A x on the scaled system (b = the scaled load bl * s)
+void residual_from_x(fcbga_model *M, const double *bl) {
+  be::stream_t s = M->stream;
+  const int64_t n3 = 3 * M->g.G;
+  fine_apply<double>(fine_ops<double>(M), M->x.p, M->q.p, s);
+  const double *q = M->q.p, *sv = M->s64.p;
+  double *r = M->r.p;
+  be::pfor(n3, s, [=] __host__ __device__(int64_t i) { r[i] = bl[i] * sv[i] - q[i]; });
+}
+
+// PCG iterations from the current x with r = b - A x (_pcg, src/core/10-solver.js:1001-1042);
+// returns the iteration count, rnorm and conv updated
+int pcg_iterate(fcbga_model *M, const fcbga_solve_opts &o, double bnorm, double &rnorm, bool &conv) {
+  be::stream_t s = M->stream;
+  const int64_t n3 = 3 * M->g.G;
+  double *r = M->r.p, *z = M->z.p, *p = M->p.p, *sc = M->scal.p;
+  if (rnorm / bnorm < o.tol) { conv = true; return 0; }
+  precondition(M, o.precision, o.precond);
+  be::pfor(n3, s, [=] __host__ __device__(int64_t i) { p[i] = z[i]; });
+  be::dot_async(n3, r, z, M->partials.p, sc + SC_RZ, s);
+  // the FP32 cycles need the flexible form; IC(0) runs in FP64 whatever the precision
+  const bool flex = o.precision == FCBGA_PREC_MIXED && o.precond != FCBGA_PC_IC;
+  const int gkey = o.precision * 64 + o.precond * 8 + M->nsweeps;
+  int it;
+  for (it = 1; it <= o.maxit; it++) {
+    run_iteration(M, o, flex, gkey);
+    double h[2];
+    read_scalars(M, h);  // rr, breakdown flag
+    if (h[1] != 0) break;  // p.Ap <= 0: stop before the update; the iteration counts, as in the reference
+    rnorm = std::sqrt(h[0]);
+    if (rnorm / bnorm < o.tol) { conv = true; break; }
+  }
+  return it > o.maxit ? o.maxit : it;
+}
+
+// PCG on the prepared step: right-hand side the step's load, or rhs_grid
+// (grid layout, unscaled, zero on constrained DOFs); du_dev receives the
+// solution in the reference's node order.  A run with MG or IC(0) that does
+// not converge continues once with the other from the same iterate
+// (LinearSolver.solve, src/core/10-solver.js:965-973).
 void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, double *du_dev, double *du_host,
              fcbga_solve_stats &S) {
   be::stream_t s = M->stream;
   const double t2 = now_ms();
   const int64_t n3 = 3 * M->g.G;
-  double *x = M->x.p, *r = M->r.p, *z = M->z.p, *p = M->p.p;
+  double *x = M->x.p, *r = M->r.p;
   const double *bl = rhs_grid ? rhs_grid : M->bload.p, *sv = M->s64.p;
-  double *sc = M->scal.p;
   // scaled load; x = 0, r = b
   be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
     double bs = bl[i] * sv[i];
@@ -2209,36 +2680,30 @@ void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, 
   });
   const double bnorm = std::sqrt(dotd(M, r, r, SC_RR));
   const bool warm = o.warm > 0 && !rhs_grid;
-  if (warm && bnorm != 0 && warm_start(M, bl)) {
-    // r = b - A x for the projected initial guess
-    fine_apply<double>(fine_ops<double>(M), x, M->q.p, s);
-    const double *q = M->q.p;
-    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { r[i] -= q[i]; });
-  }
+  if (warm && bnorm != 0 && warm_start(M, bl)) residual_from_x(M, bl);
   int it = 0;
   bool conv = false;
   double rnorm = bnorm == 0 ? 0 : std::sqrt(dotd(M, r, r, SC_RR));
+  S.precond = o.precond;
+  S.fallback_first = -1;
   if (bnorm == 0) {
     conv = true;
     S.r0 = 0;
   } else {
     S.r0 = rnorm / bnorm;
-    if (rnorm / bnorm < o.tol) conv = true;
-    else {
-      precondition(M, o.precision, o.precond);
-      be::pfor(n3, s, [=] __host__ __device__(int64_t i) { p[i] = z[i]; });
-      be::dot_async(n3, r, z, M->partials.p, sc + SC_RZ, s);
-      const bool flex = o.precision == FCBGA_PREC_MIXED;
-      const int gkey = o.precision * 64 + o.precond * 8 + M->nsweeps;
-      for (it = 1; it <= o.maxit; it++) {
-        run_iteration(M, o, flex, gkey);
-        double h[2];
-        read_scalars(M, h);  // rr, breakdown flag
-        if (h[1] != 0) { it--; break; }  // p.Ap <= 0: stop before the update, as the reference does
-        rnorm = std::sqrt(h[0]);
-        if (rnorm / bnorm < o.tol) { conv = true; break; }
-      }
-      if (it > o.maxit) it = o.maxit;
+    it = pcg_iterate(M, o, bnorm, rnorm, conv);
+    if (!conv && (o.precond == FCBGA_PC_MG || o.precond == FCBGA_PC_IC)) {
+      fcbga_solve_opts o2 = o;
+      o2.precond = o.precond == FCBGA_PC_MG ? FCBGA_PC_IC : FCBGA_PC_MG;
+      S.fallback_first = o.precond;
+      S.fallback_iters = it;
+      S.fallback_relres = rnorm / bnorm;
+      S.ms_setup += build_pc(M, o2.precond, S);
+      residual_from_x(M, bl);
+      rnorm = std::sqrt(dotd(M, r, r, SC_RR));
+      S.r0 = rnorm / bnorm;  // the record's iters0 is that of the last run
+      it += pcg_iterate(M, o2, bnorm, rnorm, conv);
+      S.precond = o2.precond;
     }
   }
   // du = S x in the reference's node order

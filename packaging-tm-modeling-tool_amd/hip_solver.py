@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBS = {"hip": os.path.join(HERE, "libfcbga_hip.so"), "hostemu": os.path.join(HERE, "libfcbga_hostemu.so")}
 
 PREC_FP64, PREC_MIXED = 0, 1
-PC_MG, PC_COLUMN, PC_JACOBI = 0, 1, 2
+PC_MG, PC_COLUMN, PC_JACOBI, PC_IC = 0, 1, 2, 3
+PC_NAMES = {PC_MG: "mg", PC_COLUMN: "column", PC_JACOBI: "jacobi", PC_IC: "ic"}
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -49,7 +50,22 @@ class FcbgaSolveStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double), ("r0", C.c_double),
                 ("ms_total", C.c_double), ("ms_assemble", C.c_double), ("ms_setup", C.c_double),
                 ("ms_pcg", C.c_double), ("levels", C.c_int32), ("coarsest_dof", C.c_int32), ("ndof", C.c_int64),
-                ("ntypes", C.c_int32), ("sym_tables", C.c_int32), ("warm", C.c_int32), ("reserved", C.c_int32)]
+                ("ntypes", C.c_int32), ("sym_tables", C.c_int32), ("warm", C.c_int32), ("precond", C.c_int32),
+                ("fallback_first", C.c_int32), ("fallback_iters", C.c_int32), ("fallback_relres", C.c_double),
+                ("ic_shift", C.c_double)]
+
+
+def _record(st):
+    """The solve record: the stats fields, precond as the reference's name and
+    fallback None or {first, iters, relres} (LinearSolver.solve,
+    src/core/10-solver.js:965-993)."""
+    rec = {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_
+           if f not in ("fallback_first", "fallback_iters", "fallback_relres")}
+    rec["precond"] = PC_NAMES.get(st.precond, str(st.precond))
+    rec["fallback"] = None if st.fallback_first < 0 else {
+        "first": PC_NAMES.get(st.fallback_first, str(st.fallback_first)), "iters": st.fallback_iters,
+        "relres": st.fallback_relres}
+    return rec
 
 
 EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_warm_reset", "fcbga_step_solve",
@@ -224,7 +240,7 @@ class Model:
         o = self._opts(tol, maxit, precision, precond, sweeps, warm)
         rc = self.L.fcbga_step_solve(self.h, C.byref(step.c), C.byref(o), du.ctypes.data_as(C.c_void_p),
                                      C.byref(st))
-        rec = {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+        rec = _record(st)
         if rc != 0 and (check or rc != -3):
             raise SolverError(f"fcbga_step_solve failed ({rc}): {last_error(self.L)}")
         return du, rec
@@ -237,7 +253,7 @@ class Model:
                                             C.byref(st))
         if rc != 0:
             raise SolverError(f"fcbga_step_solve_device failed ({rc}): {last_error(self.L)}")
-        return {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+        return _record(st)
 
     def set_kernel_timing(self, enable: bool):
         self.L.fcbga_set_kernel_timing(self.h, 1 if enable else 0)
@@ -310,7 +326,7 @@ class Model:
                                           du.ctypes.data_as(_f64p), C.byref(st))
         if rc != 0 and (check or rc != -3):
             raise SolverError(f"fcbga_adjoint_forward failed ({rc}): {last_error(self.L)}")
-        return J, du, {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+        return J, du, _record(st)
 
     def adjoint_backward(self, gJ, elem_cte=True):
         """Backward of the pair: log-derivatives of gJ . J with respect to each
@@ -327,7 +343,7 @@ class Model:
                                            de.ctypes.data_as(_f64p) if de is not None else None, C.byref(st))
         if rc != 0:
             raise SolverError(f"fcbga_adjoint_backward failed ({rc}): {last_error(self.L)}")
-        return dE, dA, de, {f: getattr(st, f) for f, _ in FcbgaSolveStats._fields_ if f != "reserved"}
+        return dE, dA, de, _record(st)
 
     def apply(self, step: Step, x):
         x = np.ascontiguousarray(x, np.float64)

@@ -5,9 +5,9 @@
 //   const model = hip.createModel(mesh);        // FEModel's mesh contract (src/core/06-mesher.js:66-71)
 //   const rec = hip.solveStep(model, step, opts);
 //     step: {Ta, Tb, stage, D: Float64Array(ng*36), s0: Float64Array(ng*6), born: Uint8Array(ng)}
-//     opts: {tol, maxit, precision: 'mixed'|'fp64', precond: 'mg'|'column'|'jacobi', sweeps}
+//     opts: {tol, maxit, precision: 'mixed'|'fp64', precond: 'mg'|'ic'|'column'|'jacobi', sweeps, warm}
 //     rec:  {u: Float64Array(ndof), iters, relres, converged, ms, assembleMs, pcMs, pcgMs, levels,
-//            precond, rebuilt, warm}  -- the fields of LinearSolver.solve's record (src/core/10-solver.js:988-993)
+//            precond, rebuilt, warm, fallback, icShift}  -- the fields of LinearSolver.solve's record (src/core/10-solver.js:988-993)
 //   const st = hip.stateAccumulate(model, step, du, elems[, states[, qOnly]]);  // Float64Array(n * 81)
 //   const {sig, eps} = hip.stateStress(model, elems, st, rows, xez);       // stresses at natural points
 //   const f = hip.adjointForward(model, step, {kTop, vmElems, p}, opts);   // {J, u, ...}: saves the context
@@ -177,6 +177,60 @@ void set_num(napi_env env, napi_value obj, const char *k, double v) {
   napi_set_named_property(env, obj, k, x);
 }
 
+const char *pc_name(int pc) {
+  switch (pc) {
+    case FCBGA_PC_MG: return "mg";
+    case FCBGA_PC_COLUMN: return "column";
+    case FCBGA_PC_JACOBI: return "jacobi";
+    case FCBGA_PC_IC: return "ic";
+  }
+  return "?";
+}
+
+// opts {tol, maxit, sweeps, warm, precision: 'mixed'|'fp64', precond: 'mg'|'ic'|'column'|'jacobi'};
+// false for an unknown precond name
+bool read_opts(napi_env env, napi_value v, fcbga_solve_opts *o) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t != napi_object) return true;
+  get_double(env, v, "tol", &o->tol);
+  get_int(env, v, "maxit", &o->maxit);
+  get_int(env, v, "sweeps", &o->sweeps);
+  get_int(env, v, "warm", &o->warm);
+  napi_value x;
+  char buf[32];
+  size_t len;
+  if (get_prop(env, v, "precision", &x) && napi_get_value_string_utf8(env, x, buf, sizeof buf, &len) == napi_ok)
+    o->precision = strcmp(buf, "fp64") == 0 ? FCBGA_PREC_FP64 : FCBGA_PREC_MIXED;
+  if (get_prop(env, v, "precond", &x) && napi_get_value_string_utf8(env, x, buf, sizeof buf, &len) == napi_ok) {
+    int pc = -1;
+    for (int q = FCBGA_PC_MG; q <= FCBGA_PC_IC; q++)
+      if (strcmp(buf, pc_name(q)) == 0) pc = q;
+    if (pc < 0) return false;
+    o->precond = pc;
+  }
+  return true;
+}
+
+// precond of the final run, fallback {first, iters, relres} or null, and the IC(0) shift
+void set_pc_fields(napi_env env, napi_value rec, const fcbga_solve_stats &S) {
+  napi_value pc, fb;
+  napi_create_string_utf8(env, pc_name(S.precond), NAPI_AUTO_LENGTH, &pc);
+  napi_set_named_property(env, rec, "precond", pc);
+  if (S.fallback_first >= 0) {
+    napi_create_object(env, &fb);
+    napi_value first;
+    napi_create_string_utf8(env, pc_name(S.fallback_first), NAPI_AUTO_LENGTH, &first);
+    napi_set_named_property(env, fb, "first", first);
+    set_num(env, fb, "iters", S.fallback_iters);
+    set_num(env, fb, "relres", S.fallback_relres);
+  } else {
+    napi_get_null(env, &fb);
+  }
+  napi_set_named_property(env, rec, "fallback", fb);
+  set_num(env, rec, "icShift", S.ic_shift);
+}
+
 napi_value SolveStep(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
@@ -198,24 +252,7 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
   if (!get_typed(env, argv[1], "born", napi_uint8_array, &p, &n)) return throw_msg(env, "step.born: Uint8Array(ngroups)");
   st.born = (const uint8_t *)p;
   fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
-  if (argc >= 3) {
-    napi_valuetype t;
-    napi_typeof(env, argv[2], &t);
-    if (t == napi_object) {
-      get_double(env, argv[2], "tol", &o.tol);
-      get_int(env, argv[2], "maxit", &o.maxit);
-      get_int(env, argv[2], "sweeps", &o.sweeps);
-      get_int(env, argv[2], "warm", &o.warm);
-      napi_value v;
-      char buf[32];
-      size_t len;
-      if (get_prop(env, argv[2], "precision", &v) &&
-          napi_get_value_string_utf8(env, v, buf, sizeof buf, &len) == napi_ok)
-        o.precision = strcmp(buf, "fp64") == 0 ? FCBGA_PREC_FP64 : FCBGA_PREC_MIXED;
-      if (get_prop(env, argv[2], "precond", &v) && napi_get_value_string_utf8(env, v, buf, sizeof buf, &len) == napi_ok)
-        o.precond = strcmp(buf, "column") == 0 ? FCBGA_PC_COLUMN : (strcmp(buf, "jacobi") == 0 ? FCBGA_PC_JACOBI : FCBGA_PC_MG);
-    }
-  }
+  if (argc >= 3 && !read_opts(env, argv[2], &o)) return throw_msg(env, "solveStep: unknown opts.precond");
   const int64_t ndof = fcbga_model_ndof(b->m);
   napi_value ab;
   void *udata = nullptr;
@@ -243,10 +280,7 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
   napi_value t;
   napi_get_boolean(env, true, &t);
   napi_set_named_property(env, rec, "rebuilt", t);
-  napi_value pc;
-  napi_create_string_utf8(env, o.precond == FCBGA_PC_MG ? "mg" : (o.precond == FCBGA_PC_COLUMN ? "column" : "jacobi"),
-                          NAPI_AUTO_LENGTH, &pc);
-  napi_set_named_property(env, rec, "precond", pc);
+  set_pc_fields(env, rec, S);
   return rec;
 }
 
@@ -361,20 +395,6 @@ napi_value make_f64(napi_env env, size_t n, double **data) {
   return arr;
 }
 
-bool read_opts(napi_env env, napi_value v, fcbga_solve_opts *o) {
-  napi_valuetype t;
-  napi_typeof(env, v, &t);
-  if (t != napi_object) return true;
-  get_double(env, v, "tol", &o->tol);
-  get_int(env, v, "maxit", &o->maxit);
-  get_int(env, v, "sweeps", &o->sweeps);
-  napi_value x;
-  char buf[32];
-  size_t len;
-  if (get_prop(env, v, "precision", &x) && napi_get_value_string_utf8(env, x, buf, sizeof buf, &len) == napi_ok)
-    o->precision = strcmp(buf, "fp64") == 0 ? FCBGA_PREC_FP64 : FCBGA_PREC_MIXED;
-  return true;
-}
 
 // adjointForward(model, step, objective, opts) -> {J: Float64Array(2), u, iters, relres, converged}
 //   objective: {kTop, vmElems: Int32Array, p}
@@ -401,7 +421,7 @@ napi_value AdjointForward(napi_env env, napi_callback_info info) {
   }
   get_double(env, argv[2], "p", &obj.p);
   fcbga_solve_opts o{1e-10, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
-  if (argc >= 4) read_opts(env, argv[3], &o);
+  if (argc >= 4 && !read_opts(env, argv[3], &o)) return throw_msg(env, "adjointForward: unknown opts.precond");
   const int64_t ndof = fcbga_model_ndof(b->m);
   double *J, *u;
   napi_value jv = make_f64(env, 2, &J), uv = make_f64(env, (size_t)ndof, &u);
@@ -419,6 +439,7 @@ napi_value AdjointForward(napi_env env, napi_callback_info info) {
   set_num(env, rec, "ms", S.ms_total);
   napi_get_boolean(env, S.converged != 0, &cv);
   napi_set_named_property(env, rec, "converged", cv);
+  set_pc_fields(env, rec, S);
   return rec;
 }
 
@@ -453,6 +474,7 @@ napi_value AdjointBackward(napi_env env, napi_callback_info info) {
   set_num(env, rec, "relres", S.relres);
   napi_get_boolean(env, S.converged != 0, &cv);
   napi_set_named_property(env, rec, "converged", cv);
+  set_pc_fields(env, rec, S);
   return rec;
 }
 
