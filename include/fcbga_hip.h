@@ -206,6 +206,8 @@ int fcbga_anand_update_device(const fcbga_anand_params *params, int64_t n, doubl
 #define FCBGA_KCLASS_SHELL1_SWEEP 3 /* first shell level: 2 sweeps x 4 colours (bracketed together) */
 #define FCBGA_KCLASS_SHELL_SWEEP 4  /* coarser shell levels: 2 sweeps x 4 colours (bracketed together) */
 #define FCBGA_KCLASS_COARSEST 5     /* coarsest dense FP64 product (one launch) */
+#define FCBGA_KCLASS_IC_APPLY 6     /* IC(0) forward + backward solve (one launch per level each, bracketed together) */
+#define FCBGA_KCLASS_IC_FACTOR 7    /* IC(0) factor, one attempt (one launch per level, bracketed together) */
 #define FCBGA_KCLASS_COUNT 8
 int fcbga_set_kernel_timing(fcbga_model *m, int enable);
 int fcbga_kernel_times(const fcbga_model *m, double *ms_total /*[FCBGA_KCLASS_COUNT]*/,

@@ -2194,12 +2194,15 @@ bool ic_try(fcbga_model *M, double shift) {
   const int64_t nl = (int64_t)M->icLvPtr.size() - 1;
   const int32_t *nodes = M->icNodes.p;
 #if FCBGA_GPU
-  for (int64_t l = 0; l < nl; l++) {
-    const int64_t a = M->icLvPtr[l], cnt = M->icLvPtr[l + 1] - a;
-    if (cnt <= 0) continue;
-    hipLaunchKernelGGL(k_ic_factor, dim3((unsigned)((cnt + IC_WAVES - 1) / IC_WAVES)), dim3(64 * IC_WAVES), 0, s, g,
-                       nodes + a, (int32_t)cnt, mask, U, fail);
-    BE_CHECK(hipGetLastError());
+  {
+    KScope ks(M, FCBGA_KCLASS_IC_FACTOR);
+    for (int64_t l = 0; l < nl; l++) {
+      const int64_t a = M->icLvPtr[l], cnt = M->icLvPtr[l + 1] - a;
+      if (cnt <= 0) continue;
+      hipLaunchKernelGGL(k_ic_factor, dim3((unsigned)((cnt + IC_WAVES - 1) / IC_WAVES)), dim3(64 * IC_WAVES), 0, s,
+                         g, nodes + a, (int32_t)cnt, mask, U, fail);
+      BE_CHECK(hipGetLastError());
+    }
   }
   int32_t h = 0;
   be::d2h(&h, fail, sizeof(int32_t), s);
@@ -2339,7 +2342,10 @@ void precondition(fcbga_model *M, int precision, int precond) {
     return;
   }
   if (precond == FCBGA_PC_IC) {
-    ic_apply(M, r, z);
+    {
+      KScope ks(M, FCBGA_KCLASS_IC_APPLY);
+      ic_apply(M, r, z);
+    }
     be::pfor(n3, s, [=] __host__ __device__(int64_t i) { if (!mask[i]) z[i] = 0.0; });
     return;
   }

@@ -85,7 +85,8 @@ class FcbgaAnandParams(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0", "nu")]
 
 # kernel classes of fcbga_kernel_times (include/fcbga_hip.h)
-KCLASS = {"apply": 0, "fine_sweep": 1, "fine_resid": 2, "shell1_sweep": 3, "shell_sweep": 4, "coarsest": 5}
+KCLASS = {"apply": 0, "fine_sweep": 1, "fine_resid": 2, "shell1_sweep": 3, "shell_sweep": 4, "coarsest": 5,
+          "ic_apply": 6, "ic_factor": 7}
 KCLASS_COUNT = 8
 
 _loaded = {}

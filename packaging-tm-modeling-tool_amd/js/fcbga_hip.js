@@ -46,7 +46,7 @@ class HipStepSolver {
   solveStep(step, label) {
     const rec = this.hip.solveStep(this.model, step, this.opts);
     rec.label = label;
-    this.log.push({ label, iters: rec.iters, relres: rec.relres, converged: rec.converged, ms: rec.ms, pcMs: rec.pcMs, precond: rec.precond });
+    this.log.push({ label, iters: rec.iters, relres: rec.relres, converged: rec.converged, ms: rec.ms, pcMs: rec.pcMs, precond: rec.precond, fallback: rec.fallback });
     return rec;
   }
   /** One step of the stress histories of elements (accumulateStep, src/core/09-assembly.js:494-545). */

@@ -130,3 +130,21 @@ def test_gpu_ic_cfg1_falls_back_to_mg():
     assert fb is not None and fb["first"] == "ic" and fb["iters"] == 2000
     assert 0.5 * c["fallback"]["relres"] <= fb["relres"] <= 2 * c["fallback"]["relres"], fb
     assert rec["converged"] and rec["precond"] == "mg"
+
+
+@pytest.mark.gpu
+def test_node_addon_ic_precond():
+    """opts.precond 'ic' through node -> HipStepSolver -> Node-API addon -> C ABI:
+    the record names the preconditioner and the shift, and the solution is the
+    reference's."""
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "js", "napi_solve.js")
+    out = subprocess.run([node, script, "plate_16", "fp64", "ic"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    assert rec["precond"] == "ic" and rec["fallback"] is None and rec["icShift"] == 0.01, rec
+    assert rec["converged"] and rec["rel_err"] <= 1e-7, rec
