@@ -1760,3 +1760,129 @@ void or_anand_batch(const double *P, int64_t n, double E, double T, double dt, c
   for (int64_t q = 0; q < n; q++)
     dep[q] = or_anand_step(P, E, T, dt, eps + 6 * q, st + 11 * q, tr + 11 * q, sig + 6 * q, D ? D + 36 * q : NULL);
 }
+
+/* ------------------------------------------------------------------------- */
+/* bump screening (screenJoints, src/core/13-fatigue.js:375-455)             */
+/* ------------------------------------------------------------------------- */
+
+/* inverse of a 3x3 by cofactors (inv3, src/core/07-element.js:146-153) */
+static void inv3c(const double *J, double *o) {
+  const double a = J[0], b = J[1], c = J[2], d = J[3], e = J[4], f = J[5], g = J[6], h = J[7], i = J[8];
+  const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+  o[0] = (e * i - f * h) / det; o[1] = (c * h - b * i) / det; o[2] = (b * f - c * e) / det;
+  o[3] = (f * g - d * i) / det; o[4] = (a * i - c * g) / det; o[5] = (c * d - a * f) / det;
+  o[6] = (d * h - e * g) / det; o[7] = (b * g - a * h) / det; o[8] = (a * e - b * d) / det;
+}
+
+/* AnandPoint.stepColumn (src/core/13-fatigue.js:153-166): Newton on the
+ * in-plane strains (xx, yy, xy) until the in-plane stresses vanish */
+static void col_step(const double *P, double E, double T, double dt, double *eps, double *st, double *tr) {
+  static const int Pi[3] = {0, 1, 3};
+  double sig[6], D[36], A[9], Ai[9];
+  for (int it = 0; it < 20; it++) { /* COLUMN_NEWTON_MAXIT, src/core/00-constants.js:121 */
+    or_anand_step(P, E, T, dt, eps, st, tr, sig, D);
+    const double r0 = sig[0], r1 = sig[1], r2 = sig[3];
+    const double ref = fmax(1.0, fmax(fabs(sig[2]), fmax(fabs(sig[4]), fabs(sig[5]))));
+    if (fmax(fabs(r0), fmax(fabs(r1), fabs(r2))) < 1e-3 * ref) break; /* COLUMN_STRESS_TOL */
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) A[3 * r + c] = D[6 * Pi[r] + Pi[c]];
+    inv3c(A, Ai);
+    for (int r = 0; r < 3; r++) eps[Pi[r]] -= Ai[3 * r] * r0 + Ai[3 * r + 1] * r1 + Ai[3 * r + 2] * r2;
+  }
+}
+
+/* AnandPoint.stepSeriesColumn (src/core/13-fatigue.js:178-206): solder cap in
+ * series with an elastic pillar; Newton on the six cap strains */
+static void series_step(const double *P, double E, double T, double dt, double *eps, const double *tz, double fs,
+                        const double *Cp, double *st, double *tr) {
+  static const int Pi[3] = {0, 1, 3}, Z[3] = {2, 4, 5}, ord[6] = {0, 1, 3, 2, 4, 5};
+  const double fp = 1 - fs;
+  double sig[6], D[36], J[36], R[6];
+  for (int it = 0; it < 20; it++) {
+    or_anand_step(P, E, T, dt, eps, st, tr, sig, D);
+    for (int r = 0; r < 3; r++) {
+      R[r] = sig[Pi[r]];
+      R[3 + r] = fs * eps[Z[r]] + fp * Cp[r] * sig[Z[r]] - tz[r];
+    }
+    const double ref = fmax(1.0, fmax(fabs(sig[2]), fmax(fabs(sig[4]), fabs(sig[5]))));
+    double worst = 0;
+    for (int r = 0; r < 3; r++) worst = fmax(worst, fmax(fabs(R[r]), fabs(R[3 + r]) * E));
+    if (worst < 1e-3 * ref) break;
+    for (int r = 0; r < 6; r++)
+      for (int c = 0; c < 6; c++) {
+        const double Dv = D[6 * ord[r] + ord[c]];
+        J[6 * r + c] = r < 3 ? Dv : fp * Cp[r - 3] * Dv + (ord[r] == ord[c] ? fs : 0);
+      }
+    for (int c = 0; c < 6; c++) {
+      int pr = c;
+      for (int r = c + 1; r < 6; r++)
+        if (fabs(J[6 * r + c]) > fabs(J[6 * pr + c])) pr = r;
+      if (pr != c) {
+        for (int q = 0; q < 6; q++) { const double t = J[6 * c + q]; J[6 * c + q] = J[6 * pr + q]; J[6 * pr + q] = t; }
+        const double t = R[c]; R[c] = R[pr]; R[pr] = t;
+      }
+      const double piv = J[7 * c];
+      for (int r = c + 1; r < 6; r++) {
+        const double f = J[6 * r + c] / piv;
+        if (f == 0) continue;
+        for (int q = c; q < 6; q++) J[6 * r + q] -= f * J[6 * c + q];
+        R[r] -= f * R[c];
+      }
+    }
+    for (int c = 5; c >= 0; c--) {
+      double v = R[c];
+      for (int q = c + 1; q < 6; q++) v -= J[6 * c + q] * R[q];
+      R[c] = v / J[7 * c];
+    }
+    for (int c = 0; c < 6; c++) eps[ord[c]] -= R[c];
+  }
+}
+
+void or_screen_joints(const double *P, const or_screen_steps *h, int64_t nsites, int32_t ntemps, int32_t i25,
+                      const double *table, double *dW, double *dE, double *stab) {
+  static const int IP[3] = {0, 1, 3};
+  const int nc = h->cycles;
+  double *Wc = malloc(sizeof(double) * (nc + 1)), *Ec = malloc(sizeof(double) * (nc + 1));
+  double e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0}, s1[6] = {0}, s2[6] = {0};
+  const double nu = P[9];
+  for (int64_t q = 0; q < nsites; q++) {
+    const double *base = table + q * ntemps * 6;
+    dW[q] = dE[q] = stab[q] = NAN;
+    if (!isfinite(base[0])) continue;
+    double st[11] = {0, 0, 0, 0, 0, 0, P[8], 0, 0, 0, 0}, tr[11], eps[6] = {0, 0, 0, 0, 0, 0}, e25[6];
+    for (int c = 0; c < 6; c++) e25[c] = i25 >= 0 ? base[i25 * 6 + c] : 0;
+    for (int c = 0; c <= nc; c++) Wc[c] = Ec[c] = 0;
+    for (int k = 1; k < h->nsteps; k++) {
+      const double *a = base + h->wi[k] * 6, *b = a + 6;
+      const double t = h->wt[k];
+      double lay[6];
+      for (int c = 0; c < 6; c++) lay[c] = a[c] + t * (b[c] - a[c]) - e25[c];
+      const int ext = k > 2 && h->same[k];
+      if (h->Cp) {
+        const double tz[3] = {lay[2] - h->fs * h->Fth[k] - (1 - h->fs) * h->Fp[k], lay[4], lay[5]};
+        if (ext)
+          for (int c = 0; c < 6; c++) eps[c] = 2 * s1[c] - s2[c];
+        series_step(P, h->E[k], h->T[k], h->dt[k], eps, tz, h->fs, h->Cp + 3 * k, st, tr);
+        for (int c = 0; c < 6; c++) { s2[c] = s1[c]; s1[c] = eps[c]; }
+      } else {
+        const double ez = lay[2] - h->Fth[k];
+        if (ext) {
+          for (int r = 0; r < 3; r++) eps[IP[r]] = 2 * e1[r] - e2[r];
+        } else {
+          eps[0] -= nu * (ez - eps[2]);
+          eps[1] -= nu * (ez - eps[2]);
+        }
+        eps[2] = ez; eps[4] = lay[4]; eps[5] = lay[5];
+        col_step(P, h->E[k], h->T[k], h->dt[k], eps, st, tr);
+      }
+      for (int c = 0; c < 10; c++) st[c] = tr[c]; /* commit (the warm start st[10] is the step's) */
+      for (int r = 0; r < 3; r++) { e2[r] = e1[r]; e1[r] = eps[IP[r]]; }
+      Wc[h->cycle[k]] = st[8];
+      Ec[h->cycle[k]] = st[7];
+    }
+    dW[q] = Wc[nc] - Wc[nc - 1];
+    dE[q] = Ec[nc] - Ec[nc - 1];
+    stab[q] = nc >= 2 ? fabs((Wc[nc] - Wc[nc - 1]) - (Wc[nc - 1] - Wc[nc - 2])) / fmax(Wc[nc] - Wc[nc - 1], 1e-30) : NAN;
+  }
+  free(Wc); free(Ec);
+}

@@ -116,6 +116,18 @@ double or_anand_step(const double *P, double E, double T, double dt, const doubl
 void or_anand_batch(const double *P, int64_t n, double E, double T, double dt, const double *eps, double *st,
                     double *tr, double *sig, double *D, double *dep);
 
+/* ---- bump screening (screenJoints, src/core/13-fatigue.js:375-455) ----
+ * per-step data of the cycle history (step 0 is the start); Cp/Fp NULL without a pillar */
+typedef struct or_screen_steps {
+  int32_t nsteps, cycles;
+  const double *T, *dt, *E, *Fth, *wt, *Cp, *Fp;
+  const int32_t *wi, *cycle;
+  const uint8_t *same;
+  double fs;
+} or_screen_steps;
+void or_screen_joints(const double *P, const or_screen_steps *h, int64_t nsites, int32_t ntemps, int32_t i25,
+                      const double *table, double *dW, double *dE, double *stab);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,8 @@ def lib():
                                        C.POINTER(OrWarpage)]
         L.or_anand_batch.argtypes = [_f64p, C.c_int64, C.c_double, C.c_double, C.c_double, _f64p, _f64p, _f64p,
                                      _f64p, _f64p, _f64p]
+        L.or_screen_joints.argtypes = [_f64p, C.POINTER(OrScreenSteps), C.c_int64, C.c_int32, C.c_int32, _f64p,
+                                       _f64p, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -276,3 +278,38 @@ def anand_batch(P, eps, T, dt, E, state, tangent=True):
     lib().or_anand_batch(_p(Pa, _f64p), n, E, T, dt, _p(eps, _f64p), _p(state, _f64p), _p(tr, _f64p),
                          _p(sig, _f64p), _p(D, _f64p), _p(dep, _f64p))
     return tr, sig, D, dep
+
+
+class OrScreenSteps(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("cycles", C.c_int32)] + \
+        [(k, _f64p) for k in ("T", "dt", "E", "Fth", "wt", "Cp", "Fp")] + \
+        [("wi", _i32p), ("cycle", _i32p), ("same", C.POINTER(C.c_uint8)), ("fs", C.c_double)]
+
+
+def screen_joints(P, steps, table, ntemps, i25):
+    """Oracle screening loop (screenJoints, src/core/13-fatigue.js:416-451) over
+    per-step data `steps` (keys of the product's step_data) and a strain table
+    (nsites x ntemps x 6): per-site dW, dE, stab."""
+    Pa = f64(P)
+    tab = f64(table).reshape(-1)
+    n = len(tab) // (6 * ntemps)
+    keep = {}
+
+    def arr(k, dt):
+        v = steps[k]
+        if v is None:
+            return None
+        keep[k] = np.ascontiguousarray(v, dt)
+        return keep[k]
+
+    h = OrScreenSteps()
+    h.nsteps, h.cycles, h.fs = steps["nsteps"], steps["cycles"], steps["fs"]
+    for k in ("T", "dt", "E", "Fth", "wt", "Cp", "Fp"):
+        setattr(h, k, _p(arr(k, np.float64), _f64p))
+    h.wi = _p(arr("wi", np.int32), _i32p)
+    h.cycle = _p(arr("cycle", np.int32), _i32p)
+    h.same = _p(arr("same", np.uint8), C.POINTER(C.c_uint8))
+    dW, dE, stab = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().or_screen_joints(_p(Pa, _f64p), C.byref(h), n, ntemps, i25, _p(tab, _f64p), _p(dW, _f64p),
+                           _p(dE, _f64p), _p(stab, _f64p))
+    return dW, dE, stab

@@ -1,5 +1,5 @@
 // Golden vectors from the reference JS core (run in the build container only:
-//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|ic|<spec name> ...]
+//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|ic|screen|<spec name> ...]
 // ). It loads src/core/*.js the way the reference's own tools/dev-core.mjs does,
 // builds each committed mesh spec (*_spec.json) with the reference mesher's
 // buildStructuredMesh, assembles one thermal step and solves it with the
@@ -12,6 +12,7 @@
 //   <name>_warm_golden.json, <name>_warm_u.f64.gz (warm-started solves along a path with T-dependent moduli)
 //   <name>_ic_golden.json, <name>_ic_u.f64.gz     (IC(0)-preconditioned solves)
 //   fallback_golden.json                (the fallback to the other preconditioner)
+//   screen_golden.json, screen_<case>_table.f64.gz (bump screening on synthetic strain tables, cycle histories, site subsampling)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -297,10 +298,68 @@ function fallbackCases() {
   console.log('fallback cases', JSON.stringify(out.map(o => [o.name, o.first, o.iters, o.precond])));
 }
 
+// Bump screening (screenJoints, cycleHistory, screeningSites: src/core/13-fatigue.js:
+// 275-455): the reference's own loop over sites and cycle steps (AnandPoint.stepColumn /
+// stepSeriesColumn) on a synthetic bump-layer strain table that stands in for
+// siteStrainTable (the table is written out with the results), plus cycle histories and
+// the site subsampling above the screening cap.
+function screenCases() {
+  const screenJoints = get('screenJoints'), cycleHistory = get('cycleHistory'), screeningSites = get('screeningSites');
+  const solder = {
+    id: 'sac305_cycling', name: 'SAC305 (cycling constants)',
+    elastic: { type: 'iso', E: { type: 'table', T: [-40, 25, 75, 125], v: [57.4, 46.5, 38.2, 30.0], interp: 'linear', extrap: 'linear', u: 'GPa' }, nu: { v: 0.4 } },
+    cte: { type: 'const', a: { v: 22.5 } },
+    anand: { A: { v: 5.87e6 }, QR: { v: 7460 }, xi: { v: 2.0 }, m: { v: 0.0942 }, shat: { v: 58.3 }, n: { v: 0.015 }, h0: { v: 9350 }, a: { v: 1.5 }, s0: { v: 45.9 } },
+  };
+  const cu = { id: 'cu', name: 'Cu pillar', elastic: { type: 'iso', E: { type: 'table', T: [-55, 25, 150], v: [123, 120, 115], interp: 'linear', extrap: 'linear', u: 'GPa' }, nu: { v: 0.34 } }, cte: { type: 'const', a: { v: 17 } } };
+  const synth = (sites, temps) => {
+    const n = sites.x.length, nt = temps.length, out = new Float64Array(n * nt * 6);
+    for (let q = 0; q < n; q++) {
+      const x = sites.x[q], y = sites.y[q], miss = q % 37 === 5;
+      for (let ti = 0; ti < nt; ti++) {
+        const dT = temps[ti] - 25;
+        const v = [1.6e-5 * dT * (1 + 0.2 * Math.sin(x)), 1.3e-5 * dT * (1 + 0.2 * Math.cos(y)), (2.4e-5 + 2e-6 * x) * dT,
+          3e-6 * dT * Math.sin(x + y), 6e-6 * dT * (y - 1.5), 6e-6 * dT * (x - 2.0)];
+        for (let c = 0; c < 6; c++) out[(q * nt + ti) * 6 + c] = miss ? NaN : v[c];
+      }
+    }
+    return out;
+  };
+  let lastTable = null;
+  globalThis.__screenTable = (sites, temps) => (lastTable = synth(sites, temps));
+  get('siteStrainTable = function (an, dieIdx, sites, temps, layer) { return globalThis.__screenTable(sites, temps); }');
+  const field = { name: 'die', nbx: 18, nby: 14, bx0: 0.3, by0: 0.25, px: 0.2, py: 0.2, peripheral: false, rows: 0 };
+  const ring = { name: 'die', nbx: 20, nby: 16, bx0: 0.2, by0: 0.2, px: 0.18, py: 0.18, peripheral: true, rows: 2 };
+  const defs = [
+    { name: 'bump3', d: field, cycling: { Tmin: -40, Tmax: 125, ramp: 10, dwell: 15, cycles: 3 }, temps: [-40, 0, 25, 75, 125], pillar: null },
+    { name: 'bump2', d: field, cycling: { Tmin: -55, Tmax: 125, ramp: 15, dwell: 10, cycles: 2 }, temps: [-55, -20, 25, 60, 125], pillar: null },
+    { name: 'ddi2', d: field, cycling: { Tmin: -40, Tmax: 125, ramp: 10, dwell: 15, cycles: 2 }, temps: [-40, 25, 125], pillar: { mat: 'cu', fs: 0.35 } },
+    { name: 'ring1', d: ring, cycling: { Tmin: 0, Tmax: 100, ramp: 20, dwell: 5, cycles: 1 }, temps: [0, 25, 100], pillar: null },
+  ];
+  const cases = [];
+  for (const c of defs) {
+    const an = { cfg: { materials: [solder, cu], cycling: c.cycling, bump: { solder: solder.id } }, temps: { cycling: c.temps }, mesh: { geom: {} } };
+    const r = screenJoints(an, c.d, c.pillar ? { zmid: 0, part: 0 } : null, solder.id, c.pillar);
+    f64gz(lastTable, 'screen_' + c.name + '_table.f64.gz');
+    const nn = a => list(a).map(v => (isFinite(v) ? v : null));
+    cases.push({ name: c.name, d: c.d, cycling: c.cycling, temps: c.temps, pillar: c.pillar, nSites: r.nSites, cycles: r.cycles,
+      dW: nn(r.dW), dE: nn(r.dE), stab: nn(r.stab), top: r.top.map(t => t.q) });
+    console.log('screen', c.name, 'sites', r.nSites, 'top dW', r.top[0].dW, 'ms', r.ms.toFixed(0));
+  }
+  const hists = [{ Tmin: -40, Tmax: 125, ramp: 10, dwell: 15, cycles: 3 }, { Tmin: -55, Tmax: 125, ramp: 15, dwell: 10, cycles: 2 },
+    { Tmin: 0, Tmax: 100, ramp: 7, dwell: 5, cycles: 1 }].map(cy => { const h = cycleHistory(cy); return { cycling: cy, t: list(h.t), T: list(h.T), cycle: list(h.cycle), phase: h.phase }; });
+  const big = { nbx: 160, nby: 150, bx0: 0.1, by0: 0.2, px: 0.15, py: 0.15, peripheral: false, rows: 0 };
+  const per = { nbx: 60, nby: 50, bx0: 0.1, by0: 0.1, px: 0.15, py: 0.15, peripheral: true, rows: 5 };
+  const subs = [[big, null], [per, 300]].map(([d, cap]) => { const r = screeningSites(d, cap); return { d, cap, subsampled: r.subsampled, n: r.sites.x.length, idx: r.idx ? list(r.idx) : null }; });
+  fs.writeFileSync(path.join(HERE, 'screen_golden.json'), JSON.stringify({ solder: { params: { A: 5.87e6, QR: 7460, xi: 2.0, m: 0.0942, shat: 58.3, n: 0.015, h0: 9350, a: 1.5, s0: 45.9, nu: 0.4 }, E_table: { T: [-40, 25, 75, 125], v: [57400, 46500, 38200, 30000] }, cte: 22.5 },
+    pillar_mat: { E_table: { T: [-55, 25, 150], v: [123000, 120000, 115000] }, nu: 0.34, cte: 17 }, cases, histories: hists, subsample: subs }));
+}
+
 const what = process.argv.slice(3);
 if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }
 if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
 if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(n);
 if (what.includes('anand')) anandCases();
+if (what.includes('screen')) screenCases();
 if (!what.length || what.includes('element')) elementCases();
 for (const n of ['small_box', 'plate_16', 'cfg1', 'cfg2']) if (!what.length || what.includes(n)) solveSpec(n);
