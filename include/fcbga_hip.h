@@ -195,6 +195,35 @@ int fcbga_anand_update_device(const fcbga_anand_params *params, int64_t n, doubl
                               const double *eps, double *state, double *trial, double *sig, double *D, double *dep,
                               void *stream);
 
+/* Bump screening through the temperature cycle (screenJoints, src/core/13-fatigue.js:375-455):
+ * every joint a free-standing solder column (AnandPoint.stepColumn, :153-166) or a solder
+ * cap on an elastic pillar in series (stepSeriesColumn, :178-206), stepped through the
+ * cycle history (cycleHistory, :275-293), one device thread per joint.  The per-step
+ * arrays are the reference's precomputation (:394-415); step 0 is the start of the
+ * history and only its cycle index is read. */
+typedef struct fcbga_screen_steps {
+  int32_t nsteps;         /* history points, >= 2 */
+  int32_t cycles;         /* >= 1; cycle[k] in 0..cycles */
+  const double *T;        /* temperature of step k (C) */
+  const double *dt;       /* t[k] - t[k-1] (s) */
+  const double *E;        /* solder modulus at T[k] on the 1 C property grid (MPa) */
+  const double *Fth;      /* solder thermal strain F(T[k]) - F(25) */
+  const double *wt;       /* weight of T[k] in its table interval, clamped to [0, 1] */
+  const double *Cp;       /* pillar compliances (1/E_p, 1/G_p, 1/G_p) per step (3 per step), or NULL */
+  const double *Fp;       /* pillar thermal strain F_p(T[k]) - F_p(25), or NULL (with Cp) */
+  const int32_t *wi;      /* interval of T[k] in the table temperatures (findInterval) */
+  const int32_t *cycle;   /* cycle index of step k */
+  const uint8_t *same;    /* 1 when step k continues the ramp or dwell of step k-1 */
+  double fs;              /* solder fraction of the joint height (1 without a pillar) */
+} fcbga_screen_steps;
+/* table: nsites x ntemps x 6 bump-layer strains (engineering, siteStrainTable,
+ * src/core/13-fatigue.js:318-345; NaN rows for sites without a bump element); i25 the
+ * row of 25 C or -1.  Out per site: dW (last cycle's inelastic work density), dE (its
+ * equivalent plastic strain), stab (|dW - dW of the cycle before| / dW, NaN below two
+ * cycles); NaN for sites without data. */
+int fcbga_screen_joints(const fcbga_anand_params *solder, const fcbga_screen_steps *steps, int64_t nsites,
+                        int32_t ntemps, int32_t i25, const double *table, double *dW, double *dE, double *stab);
+
 /* Measurement hook: with timing enabled, launches of the kernel classes below are
  * bracketed by HIP events on the library's stream and their device time is summed
  * per class (reset by every fcbga_set_kernel_timing call).  The reference has no

@@ -3531,6 +3531,64 @@ int fcbga_anand_update_device(const fcbga_anand_params *params, int64_t n, doubl
   return FCBGA_OK;
 }
 
+int fcbga_screen_joints(const fcbga_anand_params *solder, const fcbga_screen_steps *st, int64_t nsites,
+                        int32_t ntemps, int32_t i25, const double *table, double *dW, double *dE, double *stab) {
+  if (!solder || !st || nsites < 0 || !table || !dW || !dE || !stab) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  if (nsites == 0) return FCBGA_OK;
+  try {
+    const int32_t n = st->nsteps;
+    if (n < 2 || st->cycles < 1 || ntemps < 2 || i25 < -1 || i25 >= ntemps)
+      throw be::Error("screening needs nsteps >= 2, cycles >= 1, ntemps >= 2 and i25 in -1..ntemps-1");
+    if (!st->T || !st->dt || !st->E || !st->Fth || !st->wt || !st->wi || !st->cycle || !st->same)
+      throw be::Error("screening step arrays missing");
+    if ((st->Cp == nullptr) != (st->Fp == nullptr)) throw be::Error("pillar needs both Cp and Fp");
+    if (!(st->fs > 0 && st->fs <= 1)) throw be::Error("solder height fraction fs must be in (0, 1]");
+    if (!(solder->nu < 0.5)) throw be::Error("nu < 0.5 required");
+    for (int32_t k = 0; k < n; k++) {
+      if (st->wi[k] < 0 || st->wi[k] > ntemps - 2) throw be::Error("table interval wi out of range");
+      if (st->cycle[k] < 0 || st->cycle[k] > st->cycles) throw be::Error("cycle index out of range");
+      if (k > 0 && !(st->E[k] > 0 && st->dt[k] >= 0)) throw be::Error("E > 0 and dt >= 0 required");
+    }
+    be::stream_t s = nullptr;
+#if FCBGA_GPU
+    BE_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+#endif
+    DBuf<double> dT, ddt, dEm, dF, dwt, dCp, dFp, dtab, dout;
+    DBuf<int32_t> dwi, dcy;
+    DBuf<uint8_t> dsame;
+    dT.upload(st->T, n, s); ddt.upload(st->dt, n, s); dEm.upload(st->E, n, s); dF.upload(st->Fth, n, s);
+    dwt.upload(st->wt, n, s); dwi.upload(st->wi, n, s); dcy.upload(st->cycle, n, s); dsame.upload(st->same, n, s);
+    if (st->Cp) { dCp.upload(st->Cp, 3 * (size_t)n, s); dFp.upload(st->Fp, n, s); }
+    dtab.upload(table, (size_t)nsites * ntemps * 6, s);
+    dout.alloc(3 * (size_t)nsites);
+    fe::ScreenSteps h{n, st->cycles, dT.p, ddt.p, dEm.p, dF.p, dwt.p, st->Cp ? dCp.p : nullptr,
+                      st->Cp ? dFp.p : nullptr, dwi.p, dcy.p, dsame.p, st->fs};
+    const fe::AnandParams P{solder->A, solder->QR, solder->xi, solder->m, solder->shat, solder->n, solder->h0,
+                            solder->a, solder->s0, solder->nu};
+    const double *tab = dtab.p;
+    double *out = dout.p;
+    const int64_t stride = (int64_t)ntemps * 6;
+    be::pfor(nsites, s, [=] __host__ __device__(int64_t q) {
+      double r[3];
+      fe::screen_site(P, h, i25, tab + q * stride, r);
+      out[q] = r[0];
+      out[nsites + q] = r[1];
+      out[2 * nsites + q] = r[2];
+    }, 64);
+    be::d2h(dW, out, sizeof(double) * nsites, s);
+    be::d2h(dE, out + nsites, sizeof(double) * nsites, s);
+    be::d2h(stab, out + 2 * nsites, sizeof(double) * nsites, s);
+#if FCBGA_GPU
+    (void)hipStreamDestroy(s);
+#endif
+  } catch (const be::Error &e) {
+    return fail(e, FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
 int fcbga_set_kernel_timing(fcbga_model *m, int enable) {
   if (!m) { g_err = "model is NULL"; return FCBGA_ERR_ARG; }
   m->timing = enable != 0;

@@ -659,6 +659,159 @@ FE_HD double anand_step(const AnandParams &P, double E, double T, double dt, con
 }
 
 // ---------------------------------------------------------------------------
+// Bump screening (screenJoints, src/core/13-fatigue.js:375-455): each joint a
+// free-standing solder column whose axial and transverse shear strains follow
+// the bump layer while its in-plane stresses vanish (AnandPoint.stepColumn,
+// :153-166), or a solder cap on an elastic pillar in series
+// (stepSeriesColumn, :178-206), stepped through the cycle history; per joint
+// the last cycle's inelastic work density dW and equivalent strain range dE,
+// and the change of dW from the cycle before relative to dW (stab).
+// ---------------------------------------------------------------------------
+struct ScreenSteps {
+  int32_t nsteps, cycles;
+  const double *T, *dt, *E, *Fth, *wt, *Cp, *Fp;  // Cp, Fp: NULL without a pillar
+  const int32_t *wi, *cycle;
+  const uint8_t *same;
+  double fs;
+};
+
+constexpr int COLUMN_NEWTON_MAXIT = 20;     // src/core/00-constants.js:121
+constexpr double COLUMN_STRESS_TOL = 1e-3;  // src/core/00-constants.js:122
+
+// 3x3 inverse by cofactors (inv3, src/core/07-element.js:146-153)
+FE_HD void inv3_cof(const double *J, double *o) {
+  FE_NO_CONTRACT
+  const double a = J[0], b = J[1], c = J[2], d = J[3], e = J[4], f = J[5], g = J[6], h = J[7], i = J[8];
+  const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+  o[0] = (e * i - f * h) / det; o[1] = (c * h - b * i) / det; o[2] = (b * f - c * e) / det;
+  o[3] = (f * g - d * i) / det; o[4] = (a * i - c * g) / det; o[5] = (c * d - a * f) / det;
+  o[6] = (d * h - e * g) / det; o[7] = (b * g - a * h) / det; o[8] = (a * e - b * d) / det;
+}
+
+// Newton on the in-plane strains (xx, yy, xy) with the consistent tangent
+// until the in-plane stresses are below 1e-3 of the axial/shear stress
+FE_HD void step_column(const AnandParams &P, double E, double T, double dt, double eps[6], double *st, double *tr) {
+  FE_NO_CONTRACT
+  const int ip[3] = {0, 1, 3};
+  double sig[6], D[36], A[9], Ai[9];
+  for (int it = 0; it < COLUMN_NEWTON_MAXIT; it++) {
+    anand_step(P, E, T, dt, eps, st, tr, sig, D);
+    const double r0 = sig[0], r1 = sig[1], r2 = sig[3];
+    const double ref = fmax(1.0, fmax(fabs(sig[2]), fmax(fabs(sig[4]), fabs(sig[5]))));
+    if (fmax(fabs(r0), fmax(fabs(r1), fabs(r2))) < COLUMN_STRESS_TOL * ref) break;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) A[3 * r + c] = D[6 * ip[r] + ip[c]];
+    inv3_cof(A, Ai);
+    for (int r = 0; r < 3; r++) eps[ip[r]] -= Ai[3 * r] * r0 + Ai[3 * r + 1] * r1 + Ai[3 * r + 2] * r2;
+  }
+}
+
+// cap in series with a pillar of compliances Cp: the column-average strains tz
+// are prescribed (fs eps_Z + (1 - fs) Cp sigma_Z = tz) and the cap's in-plane
+// stresses vanish; Newton on the six cap strains, Gaussian elimination with
+// partial pivoting
+FE_HD void step_series_column(const AnandParams &P, double E, double T, double dt, double eps[6], const double tz[3],
+                              double fs, const double *Cp, double *st, double *tr) {
+  FE_NO_CONTRACT
+  const int ip[3] = {0, 1, 3}, iz[3] = {2, 4, 5}, ord[6] = {0, 1, 3, 2, 4, 5};
+  const double fp = 1 - fs;
+  double sig[6], D[36], J[36], R[6];
+  for (int it = 0; it < COLUMN_NEWTON_MAXIT; it++) {
+    anand_step(P, E, T, dt, eps, st, tr, sig, D);
+    for (int r = 0; r < 3; r++) {
+      R[r] = sig[ip[r]];
+      R[3 + r] = fs * eps[iz[r]] + fp * Cp[r] * sig[iz[r]] - tz[r];
+    }
+    const double ref = fmax(1.0, fmax(fabs(sig[2]), fmax(fabs(sig[4]), fabs(sig[5]))));
+    double worst = 0;
+    for (int r = 0; r < 3; r++) worst = fmax(worst, fmax(fabs(R[r]), fabs(R[3 + r]) * E));
+    if (worst < COLUMN_STRESS_TOL * ref) break;
+    for (int r = 0; r < 6; r++)
+      for (int c = 0; c < 6; c++) {
+        const double Dv = D[6 * ord[r] + ord[c]];
+        J[6 * r + c] = r < 3 ? Dv : fp * Cp[r - 3] * Dv + (ord[r] == ord[c] ? fs : 0.0);
+      }
+    for (int c = 0; c < 6; c++) {
+      int pr = c;
+      for (int r = c + 1; r < 6; r++)
+        if (fabs(J[6 * r + c]) > fabs(J[6 * pr + c])) pr = r;
+      if (pr != c) {
+        for (int q = 0; q < 6; q++) {
+          const double t = J[6 * c + q];
+          J[6 * c + q] = J[6 * pr + q];
+          J[6 * pr + q] = t;
+        }
+        const double t = R[c];
+        R[c] = R[pr];
+        R[pr] = t;
+      }
+      const double piv = J[7 * c];
+      for (int r = c + 1; r < 6; r++) {
+        const double f = J[6 * r + c] / piv;
+        if (f == 0) continue;
+        for (int q = c; q < 6; q++) J[6 * r + q] -= f * J[6 * c + q];
+        R[r] -= f * R[c];
+      }
+    }
+    for (int c = 5; c >= 0; c--) {
+      double v = R[c];
+      for (int q = c + 1; q < 6; q++) v -= J[6 * c + q] * R[q];
+      R[c] = v / J[7 * c];
+    }
+    for (int c = 0; c < 6; c++) eps[ord[c]] -= R[c];
+  }
+}
+
+// One joint through the cycle history: base = its strain-table rows (ntemps x 6,
+// NaN when the site has no bump element), i25 the row of 25 C (-1: none).
+// out = {dW, dE, stab} (NaN for a site without data; stab NaN below 2 cycles).
+FE_HD void screen_site(const AnandParams &P, const ScreenSteps &h, int32_t i25, const double *base, double out[3]) {
+  FE_NO_CONTRACT
+  const double nan = __builtin_nan("");
+  out[0] = out[1] = out[2] = nan;
+  if (!isfinite(base[0])) return;
+  const int ip[3] = {0, 1, 3};
+  const int nc = h.cycles;
+  double st[11] = {0, 0, 0, 0, 0, 0, P.s0, 0, 0, 0, 0}, tr[11];
+  double eps[6] = {0, 0, 0, 0, 0, 0}, e25[6], e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0};
+  double s1[6] = {0, 0, 0, 0, 0, 0}, s2[6] = {0, 0, 0, 0, 0, 0};
+  double W3[3] = {0, 0, 0}, E3[3] = {0, 0, 0};  // W and ebar at the end of cycles nc-2, nc-1, nc
+  for (int c = 0; c < 6; c++) e25[c] = i25 >= 0 ? base[6 * i25 + c] : 0.0;
+  for (int k = 1; k < h.nsteps; k++) {
+    const double *a = base + 6 * h.wi[k], *b = a + 6;
+    const double t = h.wt[k];
+    double lay[6];
+    for (int c = 0; c < 6; c++) lay[c] = a[c] + t * (b[c] - a[c]) - e25[c];
+    const bool ext = k > 2 && h.same[k];
+    if (h.Cp) {
+      const double tz[3] = {lay[2] - h.fs * h.Fth[k] - (1 - h.fs) * h.Fp[k], lay[4], lay[5]};
+      if (ext)
+        for (int c = 0; c < 6; c++) eps[c] = 2 * s1[c] - s2[c];
+      step_series_column(P, h.E[k], h.T[k], h.dt[k], eps, tz, h.fs, h.Cp + 3 * k, st, tr);
+      for (int c = 0; c < 6; c++) { s2[c] = s1[c]; s1[c] = eps[c]; }
+    } else {
+      const double ez = lay[2] - h.Fth[k];
+      if (ext) {
+        for (int r = 0; r < 3; r++) eps[ip[r]] = 2 * e1[r] - e2[r];
+      } else {
+        // the last strains plus the elastic lateral contraction of the axial increment
+        eps[0] -= P.nu * (ez - eps[2]);
+        eps[1] -= P.nu * (ez - eps[2]);
+      }
+      eps[2] = ez; eps[4] = lay[4]; eps[5] = lay[5];
+      step_column(P, h.E[k], h.T[k], h.dt[k], eps, st, tr);
+    }
+    for (int c = 0; c < 10; c++) st[c] = tr[c];  // commit (st[10], the Newton warm start, is the step's)
+    for (int r = 0; r < 3; r++) { e2[r] = e1[r]; e1[r] = eps[ip[r]]; }
+    const int slot = h.cycle[k] - (nc - 2);
+    if (slot >= 0 && slot < 3) { W3[slot] = st[8]; E3[slot] = st[7]; }
+  }
+  out[0] = W3[2] - W3[1];
+  out[1] = E3[2] - E3[1];
+  out[2] = nc >= 2 ? fabs((W3[2] - W3[1]) - (W3[1] - W3[0])) / fmax(W3[2] - W3[1], 1e-30) : nan;
+}
+
+// ---------------------------------------------------------------------------
 // Operator rows of one z-column of nodes from a staged neighbourhood.
 // nb holds w*x of the 3 x 3 node columns around column (i, j) at levels
 // -1 .. nz (zero padded), layout nb[(col * NZP + k + 1) * 3 + p] with

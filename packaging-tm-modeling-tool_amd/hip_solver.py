@@ -68,10 +68,17 @@ def _record(st):
     return rec
 
 
+class FcbgaScreenSteps(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("cycles", C.c_int32)] + \
+        [(k, _f64p) for k in ("T", "dt", "E", "Fth", "wt", "Cp", "Fp")] + \
+        [("wi", _i32p), ("cycle", _i32p), ("same", C.POINTER(C.c_uint8)), ("fs", C.c_double)]
+
+
 EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcbga_warm_reset", "fcbga_step_solve",
            "fcbga_step_solve_device", "fcbga_step_load", "fcbga_step_apply", "fcbga_set_kernel_timing",
            "fcbga_kernel_times", "fcbga_anand_update", "fcbga_anand_update_device", "fcbga_state_accumulate",
-           "fcbga_state_stress", "fcbga_adjoint_forward", "fcbga_adjoint_backward", "fcbga_last_error"]
+           "fcbga_state_stress", "fcbga_adjoint_forward", "fcbga_adjoint_backward", "fcbga_screen_joints",
+           "fcbga_last_error"]
 
 ANAND_STATE = 11
 STATE = 81  # stress-history state of one element (FCBGA_STATE)
@@ -137,6 +144,9 @@ def load(variant: str = "hip"):
     L.fcbga_adjoint_forward.restype = C.c_int
     L.fcbga_adjoint_backward.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p, C.POINTER(FcbgaSolveStats)]
     L.fcbga_adjoint_backward.restype = C.c_int
+    L.fcbga_screen_joints.argtypes = [C.POINTER(FcbgaAnandParams), C.POINTER(FcbgaScreenSteps), C.c_int64,
+                                      C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _f64p]
+    L.fcbga_screen_joints.restype = C.c_int
     L.fcbga_warm_reset.argtypes = [C.c_void_p]
     L.fcbga_warm_reset.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
@@ -395,3 +405,35 @@ def anand_update(params, eps, T, dt, E, state, tangent=True, variant="hip"):
     if rc != 0:
         raise SolverError(last_error(L))
     return trial, sig, D, dep
+
+
+def screen_joints(params, steps, table, i25, variant="hip"):
+    """Bump screening loop on the device (fcbga_screen_joints): params the
+    solder's FcbgaAnandParams, steps the per-step arrays of
+    screening.step_data, table (nsites, ntemps, 6) bump-layer strains, i25 the
+    row of 25 C (-1: none).  Returns per-site dW, dE, stab."""
+    L = load(variant)
+    tab = np.ascontiguousarray(table, np.float64)
+    nsites, ntemps = tab.shape[0], tab.shape[1]
+    keep = {}
+
+    def ptr(k, dt, t):
+        v = steps[k]
+        if v is None:
+            return None
+        keep[k] = np.ascontiguousarray(v, dt)
+        return keep[k].ctypes.data_as(t)
+
+    h = FcbgaScreenSteps()
+    h.nsteps, h.cycles, h.fs = int(steps["nsteps"]), int(steps["cycles"]), float(steps["fs"])
+    for k in ("T", "dt", "E", "Fth", "wt", "Cp", "Fp"):
+        setattr(h, k, ptr(k, np.float64, _f64p))
+    h.wi = ptr("wi", np.int32, _i32p)
+    h.cycle = ptr("cycle", np.int32, _i32p)
+    h.same = ptr("same", np.uint8, C.POINTER(C.c_uint8))
+    dW, dE, stab = np.zeros(nsites), np.zeros(nsites), np.zeros(nsites)
+    rc = L.fcbga_screen_joints(C.byref(params), C.byref(h), nsites, ntemps, int(i25), tab.ctypes.data_as(_f64p),
+                               dW.ctypes.data_as(_f64p), dE.ctypes.data_as(_f64p), stab.ctypes.data_as(_f64p))
+    if rc != 0:
+        raise SolverError(last_error(L))
+    return dW, dE, stab

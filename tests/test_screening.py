@@ -87,3 +87,46 @@ def test_oracle_screening_matches_reference(oracle, case):
     P, steps, table, i25 = case_inputs(g, c)
     dW, dE, stab = oracle.screen_joints(P, steps, table, len(c["temps"]), i25)
     check_results(c, dW, dE, stab, 1e-9)
+
+
+def anand_params(g):
+    from fcbga_amd import hip_solver as hs
+    p = g["solder"]["params"]
+    P = hs.FcbgaAnandParams()
+    for k in ("A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0", "nu"):
+        setattr(P, k, p[k])
+    return P
+
+
+VARIANTS = ["hostemu", pytest.param("hip", marks=pytest.mark.gpu)]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("case", ["bump3", "bump2", "ddi2", "ring1"])
+def test_screen_joints_matches_reference(variant, case):
+    """The C ABI (one device thread per joint) against the reference's
+    screenJoints: dW, dE and stab per site (1e-9; the device evaluates the same
+    FP64 operations without contraction, so the difference is the libm's last
+    bits) and the same top-20 ranking."""
+    from fcbga_amd import hip_solver as hs
+    g = golden()
+    c = next(c for c in g["cases"] if c["name"] == case)
+    _, steps, table, i25 = case_inputs(g, c)
+    dW, dE, stab = hs.screen_joints(anand_params(g), steps, table, i25, variant=variant)
+    check_results(c, dW, dE, stab, 1e-9)
+
+
+def test_screen_joints_rejects_bad_steps():
+    from fcbga_amd import hip_solver as hs
+    g = golden()
+    c = g["cases"][0]
+    _, steps, table, i25 = case_inputs(g, c)
+    bad = dict(steps, wi=np.full(steps["nsteps"], len(c["temps"]) - 1, np.int32))
+    with pytest.raises(hs.SolverError):
+        hs.screen_joints(anand_params(g), bad, table, i25, variant="hostemu")
+    with pytest.raises(hs.SolverError):
+        hs.screen_joints(anand_params(g), dict(steps, fs=0.0), table, i25, variant="hostemu")
+    with pytest.raises(hs.SolverError):
+        hs.screen_joints(anand_params(g), steps, table, len(c["temps"]), variant="hostemu")
+    dW, dE, stab = hs.screen_joints(anand_params(g), steps, table[:0], i25, variant="hostemu")
+    assert len(dW) == 0
