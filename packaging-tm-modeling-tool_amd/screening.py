@@ -209,3 +209,85 @@ def rank(dW, n_top=20):
     q = np.nonzero(np.isfinite(dW))[0]
     order = q[np.argsort(-dW[q], kind="stable")]
     return order[:n_top]
+
+
+def site_strain_table(model, chain, up, down, temps, sites, zmid, part):
+    """Bump-layer total strain (engineering) at every site for every cycling
+    temperature (siteStrainTable + Analysis.qAlong, src/core/13-fatigue.js:
+    318-345, src/core/11-analysis.js:298-316): the element DOFs q (24 nodal +
+    9 incompatible) of each site's bump element are accumulated on the device
+    (q only) along `chain`, the (Step, du) pairs from the stress-free state to
+    25 C, then along the branches `up` and `down` from 25 C towards the highest
+    and the lowest of `temps`, recorded at every step whose end temperature is
+    in `temps`; the strain is the box strain of q at the site's natural
+    coordinates in the layer at height zmid.  Returns (nsites, ntemps, 6), NaN
+    for sites without an element of `part` and temperatures not reached."""
+    from .post import _interval, element_in_cell
+    mesh = model.mesh
+    xs, ys, zs = (np.asarray(a, np.float64) for a in (mesh.xs, mesh.ys, mesh.zs))
+    sx, sy = np.asarray(sites["x"]), np.asarray(sites["y"])
+    n, nt = len(sx), len(temps)
+    out = np.full((n, nt, 6), np.nan)
+    kb = int(_interval(zs, zmid))
+    site_e = np.full(n, -1, np.int64)
+    xez = np.zeros((n, 3))
+    for q in range(n):
+        i, j = int(_interval(xs, sx[q])), int(_interval(ys, sy[q]))
+        e = element_in_cell(mesh, i, j, kb, part)
+        if e < 0:
+            continue
+        site_e[q] = e
+        ei, ej, ek = (int(v) for v in mesh.e_ijk[e])
+        # naturalCoords (src/core/12-post.js:4-11)
+        xez[q] = [np.clip(2 * (sx[q] - xs[ei]) / (xs[ei + 1] - xs[ei]) - 1, -1, 1),
+                  np.clip(2 * (sy[q] - ys[ej]) / (ys[ej + 1] - ys[ej]) - 1, -1, 1),
+                  np.clip(2 * (zmid - zs[ek]) / (zs[ek + 1] - zs[ek]) - 1, -1, 1)]
+    has = np.nonzero(site_e >= 0)[0]
+    if not len(has):
+        return out
+    elems, rows = np.unique(site_e[has], return_inverse=True)
+    elems = elems.astype(np.int32)
+    base = np.zeros((len(elems), 81))
+    for step, du in chain:
+        base = model.state_accumulate(step, du, elems, base, q_only=True)
+    want = [float(T) for T in temps]
+    qs = {}
+    if 25.0 in want:
+        qs[25.0] = base
+    hi, lo = max(want), min(want)
+    for br, end in ((up, hi), (down, lo)):
+        if end == 25.0:
+            continue
+        st = base.copy()
+        for step, du in br:
+            st = model.state_accumulate(step, du, elems, st, q_only=True)
+            Tb = float(step.c.Tb)
+            if Tb in want:
+                qs[Tb] = st.copy()
+            if Tb == end:
+                break
+    for ti, T in enumerate(want):
+        if T not in qs:
+            continue
+        _, eps = model.state_stress(elems, qs[T], rows.astype(np.int32), xez[has])
+        out[has, ti] = eps
+    return out
+
+
+def screen_joints(model, chain, up, down, temps, d, zmid, part, solder_params, solder: MaterialTables, cycling,
+                  pillar: MaterialTables | None = None, fs=1.0, variant="hip"):
+    """screenJoints (src/core/13-fatigue.js:375-455) from a solved thermal
+    path: sites kept for screening, their bump-layer strain table, the cycle
+    history and its per-step data on the host, the per-joint loop on the
+    device; dW, dE, stab per site and the top 20 by dW."""
+    from . import hip_solver as hs
+    sites, idx = screening_sites(d, SCREEN_MAX_DDI if pillar is not None else None)
+    table = site_strain_table(model, chain, up, down, temps, sites, zmid, part)
+    hist = cycle_history(cycling)
+    steps = step_data(hist, temps, solder, pillar, fs)
+    want = [float(T) for T in temps]
+    i25 = want.index(25.0) if 25.0 in want else -1
+    dW, dE, stab = hs.screen_joints(solder_params, steps, table, i25, variant=variant)
+    return {"x": sites["x"], "y": sites["y"], "ix": sites["ix"], "iy": sites["iy"], "dW": dW, "dE": dE,
+            "stab": stab, "subsampled": idx is not None, "nSites": len(sites["x"]), "top": rank(dW),
+            "cycles": cycling["cycles"], "table": table}

@@ -130,3 +130,110 @@ def test_screen_joints_rejects_bad_steps():
         hs.screen_joints(anand_params(g), steps, table, len(c["temps"]), variant="hostemu")
     dW, dE, stab = hs.screen_joints(anand_params(g), steps, table[:0], i25, variant="hostemu")
     assert len(dW) == 0
+
+
+def test_site_strain_table_vs_oracle_recovery(oracle):
+    """siteStrainTable from a solved path (chain 150 -> 25 C, branches
+    25 -> 75 -> 125 C and 25 -> -40 C on the box): the device q-only state
+    accumulation and point strains against the oracle's element recovery of
+    each step (the strain is linear in q, so the strains of the steps' dq add
+    up along the path), at every site of the box's joint field."""
+    from conftest import golden_spec
+    from fcbga_amd import hip_solver as hs
+    from fcbga_amd import meshspec, post
+    spec = golden_spec("small_box")
+    mesh = meshspec.build_mesh(spec)
+    model = hs.Model(mesh, variant="hostemu")
+
+    def solve(Ta, Tb):
+        st = hs.step_for_spec(spec, Ta, Tb, 0)
+        u, rec = model.solve_step(st, tol=1e-12)
+        assert rec["converged"]
+        return st, u, meshspec.group_step_data(spec, Ta, Tb, 0)
+
+    chain, up, down = [solve(150, 25)], [solve(25, 75), solve(75, 125)], [solve(25, -40)]
+    temps = [-40, 25, 75, 125]
+    sj = spec["stress_joints"]
+    d = dict(nbx=sj["nbx"], nby=sj["nby"], bx0=sj["bx0"], by0=sj["by0"], px=sj["px"], py=sj["py"],
+             peripheral=False, rows=0)
+    sites = scr.bump_sites(d)
+    tab = scr.site_strain_table(model, [(s, u) for s, u, _ in chain], [(s, u) for s, u, _ in up],
+                                [(s, u) for s, u, _ in down], temps, sites, sj["zmid"], sj["part"])
+    xs, ys, zs = mesh.xs, mesh.ys, mesh.zs
+    kb = int(np.searchsorted(zs, sj["zmid"], side="right") - 1)
+    found = 0
+    for q in range(len(sites["x"])):
+        i = int(np.searchsorted(xs, sites["x"][q], side="right") - 1)
+        j = int(np.searchsorted(ys, sites["y"][q], side="right") - 1)
+        e = post.element_in_cell(mesh, min(i, len(xs) - 2), min(j, len(ys) - 2), kb, sj["part"])
+        if e < 0:
+            assert np.isnan(tab[q]).all()
+            continue
+        found += 1
+        ei, ej, ek = (int(v) for v in mesh.e_ijk[e])
+        hx, hy, hz = xs[ei + 1] - xs[ei], ys[ej + 1] - ys[ej], zs[ek + 1] - zs[ek]
+        xez = np.array([[2 * (sites["x"][q] - xs[ei]) / hx - 1, 2 * (sites["y"][q] - ys[ej]) / hy - 1,
+                         2 * (sj["zmid"] - zs[ek]) / hz - 1]])
+        grp = int(mesh.e_group[e])
+
+        def run(eps0, steps):
+            # the oracle returns the strain of each step's increment dq; strains add up along the path
+            eps = eps0.copy()
+            for _, du, (D, s0, born) in steps:
+                du24 = du.reshape(-1, 3)[mesh.conn[e]].reshape(-1)
+                _, de = oracle.element_step_points(hx, hy, hz, oracle.shape_matrices(D=D[grp]), D[grp], s0[grp],
+                                                   bool(born[grp]), du24, np.zeros(33), xez)
+                eps += de[0]
+            return eps
+
+        e25 = run(np.zeros(6), chain)
+        e75 = run(e25, up[:1])
+        e125 = run(e75, up[1:])
+        em40 = run(e25, down)
+        for ti, ref in zip(range(4), (em40, e25, e75, e125)):
+            assert np.abs(tab[q, ti] - ref).max() <= 1e-10 * np.abs(ref).max(), (q, ti)
+    assert found > 0
+
+
+@pytest.mark.gpu
+def test_gpu_screening_from_solved_path(oracle):
+    """End to end on the bare-die mesh: device solves along a path (150 -> 25 C,
+    25 -> 75 -> 125 C, 25 -> -40 C), the bump-layer strain table from the
+    device's q-only states, the device screening loop; the table against the
+    host build's from the same solutions, dW and dE against the oracle's loop
+    on that table."""
+    from conftest import golden_spec
+    from fcbga_amd import hip_solver as hs
+    from fcbga_amd import meshspec
+    g = golden()
+    spec = golden_spec("cfg1")
+    mesh = meshspec.build_mesh(spec)
+    model = hs.Model(mesh, variant="hip")
+    emu = hs.Model(mesh, variant="hostemu")
+
+    def solve(Ta, Tb):
+        st = hs.step_for_spec(spec, Ta, Tb, 0)
+        u, rec = model.solve_step(st, tol=1e-10, precision=hs.PREC_MIXED)
+        assert rec["converged"]
+        return st, u
+
+    chain, up, down = [solve(150, 25)], [solve(25, 75), solve(75, 125)], [solve(25, -40)]
+    temps = [-40, 25, 75, 125]
+    sj = spec["stress_joints"]
+    d = dict(nbx=sj["nbx"], nby=sj["nby"], bx0=sj["bx0"], by0=sj["by0"], px=sj["px"], py=sj["py"],
+             peripheral=False, rows=0)
+    s = g["solder"]
+    solder = scr.MaterialTables(s["E_table"]["T"], [v / 1000 for v in s["E_table"]["v"]], s["cte"], s["params"]["nu"])
+    cyc = {"Tmin": -40, "Tmax": 125, "ramp": 10, "dwell": 15, "cycles": 2}
+    r = scr.screen_joints(model, chain, up, down, temps, d, sj["zmid"], sj["part"], anand_params(g), solder, cyc)
+    ref_tab = scr.site_strain_table(emu, chain, up, down, temps, scr.bump_sites(d), sj["zmid"], sj["part"])
+    ok = np.isfinite(ref_tab)
+    assert np.array_equal(ok, np.isfinite(r["table"])) and ok.any()
+    assert np.abs(r["table"][ok] - ref_tab[ok]).max() <= 1e-9 * np.abs(ref_tab[ok]).max()
+    P = [s["params"][k] for k in ("A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0", "nu")]
+    steps = scr.step_data(scr.cycle_history(cyc), temps, solder)
+    oW, oE, _ = oracle.screen_joints(P, steps, r["table"], len(temps), 1)
+    f = np.isfinite(oW)
+    assert np.array_equal(f, np.isfinite(r["dW"])) and f.any()
+    assert np.abs(r["dW"][f] - oW[f]).max() <= 1e-9 * np.abs(oW[f]).max()
+    assert np.abs(r["dE"][f] - oE[f]).max() <= 1e-9 * np.abs(oE[f]).max()
