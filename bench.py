@@ -154,6 +154,42 @@ def anand_throughput(torch, hs, n=115200, reps=20):
             "note": "AnandPoint.step with consistent tangent, one batch per global Newton iteration"}
 
 
+def screening_case(scr):
+    """Bump screening at the README's size: a 132 x 132 die bump field (17,424
+    joints), the reference's default cycle (-40..125 C, 10 C/min, 15 min
+    dwells, 3 cycles: 245 steps), SAC305 Anand constants, and a synthetic
+    bump-layer strain table at five temperatures."""
+    import numpy as np
+    solder = scr.MaterialTables([-40, 25, 75, 125], [57.4, 46.5, 38.2, 30.0], 22.5, 0.4)
+    d = {"nbx": 132, "nby": 132, "bx0": 0.1, "by0": 0.1, "px": 0.15, "py": 0.15, "peripheral": False, "rows": 0}
+    sites, _ = scr.screening_sites(d)
+    temps = [-40, 0, 25, 75, 125]
+    x, y = sites["x"][:, None], sites["y"][:, None]
+    dT = np.array(temps, np.float64)[None, :] - 25.0
+    table = np.stack([1.6e-5 * dT * (1 + 0.2 * np.sin(x)), 1.3e-5 * dT * (1 + 0.2 * np.cos(y)),
+                      (2.4e-5 + 2e-6 * x) * dT, 3e-6 * dT * np.sin(x + y), 6e-6 * dT * (y - 10),
+                      6e-6 * dT * (x - 10)], axis=-1)
+    hist = scr.cycle_history({"Tmin": -40, "Tmax": 125, "ramp": 10, "dwell": 15, "cycles": 3})
+    return scr.step_data(hist, temps, solder), table, temps.index(25)
+
+
+def screening_timing(hs, scr, reps=5):
+    """fcbga_screen_joints on the screening case (host arrays in and out)."""
+    P = hs.anand_params(5.87e6, 7460.0, 2.0, 0.0942, 58.3, 0.015, 9350.0, 1.5, 45.9, 0.4)
+    steps, table, i25 = screening_case(scr)
+    hs.screen_joints(P, steps, table, i25)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dW, _, _ = hs.screen_joints(P, steps, table, i25)
+    dt = (time.perf_counter() - t0) / reps
+    n = table.shape[0]
+    return {"joints": n, "steps": steps["nsteps"] - 1, "cycles": steps["cycles"], "ms": dt * 1e3,
+            "joint_steps_per_s": n * (steps["nsteps"] - 1) / dt, "dtype": "f64",
+            "reference_s": 15.0, "reference_note": "README.md:378, 17,424 bumps, JS",
+            "note": "one thread per joint through the cycle (stepColumn Newton around the Anand return map); "
+                    "includes the host copies of the strain table and results"}
+
+
 def adjoint_timing(torch, hs, cases, meshspec, tol, reps=3):
     """Sensitivity pair on the lidded mesh with 18 groups: one forward (solve +
     objectives) and one backward (adjoint solve + per-element contractions)
@@ -311,11 +347,26 @@ def main():
     log(f"Anand: {out['anand']['gp_updates_per_s']:.3e} point updates/s ({out['anand']['n_points']} points, "
         f"{out['anand']['ms_per_update']:.3f} ms per batch)")
 
+    from fcbga_amd import screening as scr
+    out["screening"] = screening_timing(hs, scr)
+    log(f"screening: {out['screening']['joints']} joints x {out['screening']['steps']} steps in "
+        f"{out['screening']['ms']:.1f} ms")
+
     if not args.no_cpu:
         log("CPU oracle sample ...")
         base = cpu_baseline(spec, mesh)
+        # the same screening loop in the oracle (C, 1 thread) on every 97th joint
+        from oracle import oracle as orc
+        steps, table, i25 = screening_case(scr)
+        sub = table[::97]
+        t0 = time.perf_counter()
+        orc.screen_joints([5.87e6, 7460.0, 2.0, 0.0942, 58.3, 0.015, 9350.0, 1.5, 45.9, 0.4], steps, sub,
+                          table.shape[1], i25)
+        base["screening_s_full_estimate"] = (time.perf_counter() - t0) * table.shape[0] / len(sub)
+        base["screening_sample"] = f"{len(sub)} of {table.shape[0]} joints, 1 thread"
         out["cpu_baseline"] = base
-        log(f"CPU: {base['value']:.3e} DOF*iter/s over {base['seconds']:.1f} s")
+        log(f"CPU: {base['value']:.3e} DOF*iter/s over {base['seconds']:.1f} s; screening "
+            f"{base['screening_s_full_estimate']:.1f} s for all joints")
     print(json.dumps(out), flush=True)
 
 

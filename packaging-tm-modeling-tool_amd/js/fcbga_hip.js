@@ -123,6 +123,57 @@ class HipSensitivity {
   }
 }
 
+/**
+ * screenJoints (src/core/13-fatigue.js:375-455) with the per-joint loop on the
+ * device: the sites, the bump-layer strain table, the cycle history and the
+ * per-step material data come from the reference's own functions, passed in
+ * `core` ({screeningSites, siteStrainTable, findMaterial, cycleHistory,
+ * AnandPoint, materialTables, gridAt, findInterval, clamp, CONST}); every
+ * joint's AnandPoint.stepColumn / stepSeriesColumn march through the cycle runs
+ * in one device thread (hip.screenJoints).  Returns the reference's record.
+ */
+function makeScreenJoints(core) {
+  const hip = load();
+  return function screenJoints(an, d, layer, solderId, pillar) {
+    const t0 = Date.now();
+    const { sites, subsampled } = core.screeningSites(d, pillar ? core.CONST.SCREEN_MAX_DDI : null);
+    const temps = an.temps.cycling;
+    const table = core.siteStrainTable(an, null, sites, temps, layer);
+    const solder = core.findMaterial(an.cfg.materials, solderId);
+    const hist = core.cycleHistory(an.cfg.cycling);
+    const n = sites.x.length, nsteps = hist.t.length;
+    const pt = new core.AnandPoint(solder);
+    const st = {
+      nsteps, cycles: hist.cycles, nsites: n, fs: pillar ? pillar.fs : 1,
+      T: Float64Array.from(hist.T), dt: new Float64Array(nsteps), E: new Float64Array(nsteps), Fth: new Float64Array(nsteps),
+      wt: new Float64Array(nsteps), wi: new Int32Array(nsteps), cycle: Int32Array.from(hist.cycle), same: new Uint8Array(nsteps),
+    };
+    for (let k = 0; k < nsteps; k++) {
+      const i = core.findInterval(temps, hist.T[k]);
+      st.wi[k] = i;
+      st.wt[k] = core.clamp((hist.T[k] - temps[i]) / (temps[i + 1] - temps[i]), 0, 1);
+      st.Fth[k] = pt.thermalStrain(hist.T[k]) - pt.thermalStrain(25);
+      st.E[k] = pt.E(hist.T[k]);
+      st.dt[k] = k > 0 ? hist.t[k] - hist.t[k - 1] : 0;
+      st.same[k] = k > 0 && hist.phase[k] === hist.phase[k - 1] ? 1 : 0;
+    }
+    if (pillar) {
+      const pm = core.findMaterial(an.cfg.materials, pillar.mat), tb = core.materialTables(pm);
+      st.Cp = new Float64Array(3 * nsteps); st.Fp = new Float64Array(nsteps);
+      for (let k = 0; k < nsteps; k++) {
+        const Ep = core.gridAt(tb.E, hist.T[k]), Gp = Ep / (2 * (1 + tb.nu));
+        st.Cp[3 * k] = 1 / Ep; st.Cp[3 * k + 1] = st.Cp[3 * k + 2] = 1 / Gp;
+        st.Fp[k] = core.gridAt(tb.F, hist.T[k]) - core.gridAt(tb.F, 25);
+      }
+    }
+    const params = { A: pt.A, QR: pt.QR, xi: pt.xi, m: pt.m, shat: pt.shat, n: pt.n, h0: pt.h0, a: pt.a, s0: pt.s0, nu: pt.nu };
+    const r = hip.screenJoints(params, st, Float64Array.from(table), temps.indexOf(25));
+    const order = Array.from({ length: n }, (_, q) => q).filter(q => isFinite(r.dW[q])).sort((a, b) => r.dW[b] - r.dW[a]);
+    return { die: d.name, x: sites.x, y: sites.y, ix: sites.ix, iy: sites.iy, dW: r.dW, dE: r.dE, stab: r.stab, subsampled, nSites: n,
+      top: order.slice(0, 20).map(q => ({ q, x: sites.x[q], y: sites.y[q], dW: r.dW[q], dE: r.dE[q] })), ms: Date.now() - t0, cycles: hist.cycles };
+  };
+}
+
 function install(Analysis, opts) {
   const original = Analysis.prototype.solveStep;
   Analysis.prototype.solveStep = function (k) {
@@ -144,4 +195,4 @@ function install(Analysis, opts) {
   return original;
 }
 
-module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, stepFromGroups, install, STATE };
+module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, makeScreenJoints, stepFromGroups, install, STATE };

@@ -12,6 +12,7 @@
 //   const {sig, eps} = hip.stateStress(model, elems, st, rows, xez);       // stresses at natural points
 //   const f = hip.adjointForward(model, step, {kTop, vmElems, p}, opts);   // {J, u, ...}: saves the context
 //   const g = hip.adjointBackward(model, Float64Array.of(1, 0));          // {dlnE, dlnCte, dlnCteElem}
+//   const s = hip.screenJoints(solder, steps, table, i25);                 // {dW, dE, stab} per joint
 //   hip.destroyModel(model);                   // or left to the garbage collector
 // Argument and device errors throw Error(message); a solve that does not
 // converge returns its record with converged = false (the caller throws, as
@@ -478,13 +479,83 @@ napi_value AdjointBackward(napi_env env, napi_callback_info info) {
   return rec;
 }
 
+// screenJoints(solder, steps, table, i25) -> {dW, dE, stab}  (fcbga_screen_joints)
+//   solder: {A, QR, xi, m, shat, n, h0, a, s0, nu}
+//   steps:  {nsteps, cycles, T, dt, E, Fth, wt: Float64Array, wi, cycle: Int32Array, same: Uint8Array,
+//            Cp, Fp: Float64Array (pillar, optional), fs}
+//           nsites (number of joints)
+//   table:  Float64Array(nsites * ntemps * 6) bump-layer strains; i25 the row of 25 C or -1
+napi_value ScreenJoints(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 4) return throw_msg(env, "screenJoints(solder, steps, table, i25)");
+  fcbga_anand_params P;
+  const char *names[10] = {"A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0", "nu"};
+  double *pv[10] = {&P.A, &P.QR, &P.xi, &P.m, &P.shat, &P.n, &P.h0, &P.a, &P.s0, &P.nu};
+  for (int q = 0; q < 10; q++)
+    if (!get_double(env, argv[0], names[q], pv[q])) return throw_msg(env, std::string("screenJoints: solder.") + names[q]);
+  fcbga_screen_steps st;
+  memset(&st, 0, sizeof st);
+  int32_t nsites = 0;
+  if (!get_int(env, argv[1], "nsteps", &st.nsteps) || !get_int(env, argv[1], "cycles", &st.cycles) ||
+      !get_int(env, argv[1], "nsites", &nsites))
+    return throw_msg(env, "screenJoints: steps.nsteps, steps.cycles and steps.nsites required");
+  st.fs = 1;
+  get_double(env, argv[1], "fs", &st.fs);
+  const size_t n = (size_t)(st.nsteps > 0 ? st.nsteps : 0);
+  struct { const char *k; const double **p; size_t len; } f64[] = {
+      {"T", &st.T, n}, {"dt", &st.dt, n}, {"E", &st.E, n}, {"Fth", &st.Fth, n}, {"wt", &st.wt, n}};
+  void *p;
+  size_t len;
+  for (auto &a : f64) {
+    if (!get_typed(env, argv[1], a.k, napi_float64_array, &p, &len) || len != a.len)
+      return throw_msg(env, std::string("screenJoints: steps.") + a.k + ": Float64Array(nsteps)");
+    *a.p = (const double *)p;
+  }
+  if (!get_typed(env, argv[1], "wi", napi_int32_array, &p, &len) || len != n)
+    return throw_msg(env, "screenJoints: steps.wi: Int32Array(nsteps)");
+  st.wi = (const int32_t *)p;
+  if (!get_typed(env, argv[1], "cycle", napi_int32_array, &p, &len) || len != n)
+    return throw_msg(env, "screenJoints: steps.cycle: Int32Array(nsteps)");
+  st.cycle = (const int32_t *)p;
+  if (!get_typed(env, argv[1], "same", napi_uint8_array, &p, &len) || len != n)
+    return throw_msg(env, "screenJoints: steps.same: Uint8Array(nsteps)");
+  st.same = (const uint8_t *)p;
+  if (get_typed(env, argv[1], "Cp", napi_float64_array, &p, &len)) {
+    if (len != 3 * n) return throw_msg(env, "screenJoints: steps.Cp: Float64Array(3 * nsteps)");
+    st.Cp = (const double *)p;
+    if (!get_typed(env, argv[1], "Fp", napi_float64_array, &p, &len) || len != n)
+      return throw_msg(env, "screenJoints: steps.Fp: Float64Array(nsteps) with Cp");
+    st.Fp = (const double *)p;
+  }
+  void *tp;
+  size_t tlen;
+  if (!arg_typed(env, argv[2], napi_float64_array, &tp, &tlen) || nsites <= 0 || tlen % (6 * (size_t)nsites) != 0)
+    return throw_msg(env, "screenJoints: table: Float64Array(nsites * ntemps * 6)");
+  const int32_t ntemps = (int32_t)(tlen / (6 * (size_t)nsites));
+  int32_t i25 = -1;
+  napi_get_value_int32(env, argv[3], &i25);
+  double *dW, *dE, *stab;
+  napi_value vW = make_f64(env, nsites, &dW), vE = make_f64(env, nsites, &dE), vS = make_f64(env, nsites, &stab);
+  if (!vW || !vE || !vS) return throw_msg(env, "screenJoints: out of memory");
+  if (fcbga_screen_joints(&P, &st, nsites, ntemps, i25, (const double *)tp, dW, dE, stab) != FCBGA_OK)
+    return throw_msg(env, "screenJoints: " + lib_error());
+  napi_value rec;
+  NAPI_OK(napi_create_object(env, &rec));
+  napi_set_named_property(env, rec, "dW", vW);
+  napi_set_named_property(env, rec, "dE", vE);
+  napi_set_named_property(env, rec, "stab", vS);
+  return rec;
+}
+
 }  // namespace
 
 static napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback cb; } fns[] = {
       {"createModel", CreateModel}, {"destroyModel", DestroyModel}, {"solveStep", SolveStep},
       {"stateAccumulate", StateAccumulate}, {"stateStress", StateStress},
-      {"adjointForward", AdjointForward}, {"adjointBackward", AdjointBackward}};
+      {"adjointForward", AdjointForward}, {"adjointBackward", AdjointBackward}, {"screenJoints", ScreenJoints}};
   for (auto &f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||

@@ -237,3 +237,20 @@ def test_gpu_screening_from_solved_path(oracle):
     assert np.array_equal(f, np.isfinite(r["dW"])) and f.any()
     assert np.abs(r["dW"][f] - oW[f]).max() <= 1e-9 * np.abs(oW[f]).max()
     assert np.abs(r["dE"][f] - oE[f]).max() <= 1e-9 * np.abs(oE[f]).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["bump3", "ddi2"])
+def test_node_addon_screening(case):
+    """The JS host path: node -> makeScreenJoints -> Node-API addon -> C ABI ->
+    device, against the reference's dW and ranking."""
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "js", "napi_screen.js")
+    out = subprocess.run([node, script, case], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    assert rec["dW_rel_err"] <= 1e-9 and rec["nan_ok"] and rec["top_ok"], rec
