@@ -224,6 +224,52 @@ typedef struct fcbga_screen_steps {
 int fcbga_screen_joints(const fcbga_anand_params *solder, const fcbga_screen_steps *steps, int64_t nsites,
                         int32_t ntemps, int32_t i25, const double *table, double *dW, double *dE, double *stab);
 
+/* Bump submodel (BumpSubmodel, src/core/13-fatigue.js:514-1105): one pitch cell around a
+ * bump on a structured grid of B-bar hexahedra (one element per cell), the nodes on the six
+ * faces prescribed.  The engine does the time steps (stepTo, :924-961: Newton with the
+ * algorithmic tangent and IC(0)-PCG inner solves), the commit, the cutback snapshot and the
+ * layer averages on the device; the caller keeps the cycle and the adaptive sub-stepping
+ * (run, :963-1090) and the material data of each step. */
+#define FCBGA_SUB_SUBSTRATE 0 /* homogenized top of the substrate (elastic, 36 + 6 per step) */
+#define FCBGA_SUB_SOLDER 1    /* Anand points */
+#define FCBGA_SUB_UNDERFILL 2
+#define FCBGA_SUB_SILICON 3
+typedef struct fcbga_submodel fcbga_submodel;
+typedef struct fcbga_submodel_mesh {
+  int32_t nx, ny, nz;           /* grid lines per axis (nodes), z fastest */
+  const double *xs, *ys, *zs;
+  const int8_t *cell_mat;       /* per cell ((i*(ny-1) + j)*(nz-1) + k): FCBGA_SUB_* */
+  const uint8_t *cell_layer;    /* per cell: bit 0 die-side averaging layer, bit 1 substrate-side layer */
+} fcbga_submodel_mesh;
+typedef struct fcbga_submodel_step_in {
+  double T, dt;                 /* end temperature of the step (C) and its duration (s) */
+  double E;                     /* solder modulus E(T) (MPa) */
+  double fsolder;               /* solder thermal strain F(T) - F(25) */
+  const double *elas_D;         /* 4 x 36: mean stiffness of each material over the step from the committed
+                                   temperature to T (the solder row is unused) */
+  const double *elas_sth;       /* 4 x 6: thermal stress increment over the same step */
+  const double *ub;             /* 3 per boundary node (face nodes in grid order): u(T) - u(25) */
+} fcbga_submodel_step_in;
+typedef struct fcbga_submodel_step_out {
+  int32_t converged, newton;    /* the reference's {converged, newton, dMax} */
+  int32_t pcg_iters, ic_builds;
+  double dmax;                  /* largest equivalent plastic increment over the solder points */
+} fcbga_submodel_step_out;
+int fcbga_submodel_create(const fcbga_submodel_mesh *mesh, const fcbga_anand_params *solder, fcbga_submodel **out);
+void fcbga_submodel_destroy(fcbga_submodel *s);
+int32_t fcbga_submodel_nboundary(const fcbga_submodel *s);
+int fcbga_submodel_step(fcbga_submodel *s, const fcbga_submodel_step_in *in, fcbga_submodel_step_out *out);
+/* committed state for a cutback (_snapshot / _restore, :1069-1079) */
+int fcbga_submodel_save(fcbga_submodel *s);
+int fcbga_submodel_restore(fcbga_submodel *s);
+/* volume-weighted means over the solder points of the two layers: {W die, ebar die, W sub, ebar sub} */
+int fcbga_submodel_layer_means(fcbga_submodel *s, double *out4);
+/* hysteresis sample: {tau die, gamma die, tau sub, gamma sub} along (cphi, sphi) */
+int fcbga_submodel_loop_sample(fcbga_submodel *s, double T, double E, double fsolder, double cphi, double sphi,
+                               double *out4);
+/* displacement (3 per grid node) */
+int fcbga_submodel_displacement(fcbga_submodel *s, double *u);
+
 /* Measurement hook: with timing enabled, launches of the kernel classes below are
  * bracketed by HIP events on the library's stream and their device time is summed
  * per class (reset by every fcbga_set_kernel_timing call).  The reference has no

@@ -2016,15 +2016,10 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// one level of the factor: wavefront w owns node nodes[w]; lane 3t+q (< 42)
-// owns the column (slot t, q) of the node's three rows
-__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_factor(fe::Grid g, const int32_t *__restrict__ nodes,
-                                                             int32_t cnt, const uint16_t *__restrict__ mask,
-                                                             double *U, int32_t *fail) {
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
-  if (w >= cnt) return;
-  const int64_t gi = nodes[w];
+// factor rows of node gi (one wavefront; lane 3t+q < 42 owns the column (slot
+// t, q) of the node's three rows); a non-positive pivot sets fail[0]
+__device__ inline void ic_factor_node(const fe::Grid &g, int64_t gi, const uint16_t *__restrict__ mask, double *U,
+                                      int32_t *fail, int lane) {
   const int k = (int)(gi % g.nz);
   const int64_t cj = gi / g.nz;
   const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
@@ -2060,14 +2055,10 @@ __global__ void __launch_bounds__(64 * IC_WAVES) k_ic_factor(fe::Grid g, const i
   if (lane == 0 && !(d0 > 0 && d1 > 0 && d2 > 0)) fail[0] = 1;
 }
 
-__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_forward(fe::Grid g, const int32_t *__restrict__ nodes,
-                                                              int32_t cnt, const uint16_t *__restrict__ mask,
-                                                              const double *__restrict__ U,
-                                                              const double *__restrict__ r, double *z) {
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
-  if (w >= cnt) return;
-  const int64_t gi = nodes[w];
+// forward solve of node gi (one wavefront): z_n from r_n and the lower neighbours' z
+__device__ inline void ic_forward_node(const fe::Grid &g, int64_t gi, const uint16_t *__restrict__ mask,
+                                       const double *__restrict__ U, const double *__restrict__ r, double *z,
+                                       int lane) {
   const int k = (int)(gi % g.nz);
   const int64_t cj = gi / g.nz;
   const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
@@ -2077,13 +2068,9 @@ __global__ void __launch_bounds__(64 * IC_WAVES) k_ic_forward(fe::Grid g, const 
   if (lane == 0) ic_fwd_finish(U + gi * IC_STRIDE, r + 3 * gi, s, z + 3 * gi);
 }
 
-__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_backward(fe::Grid g, const int32_t *__restrict__ nodes,
-                                                               int32_t cnt, const uint16_t *__restrict__ mask,
-                                                               const double *__restrict__ U, double *z) {
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
-  if (w >= cnt) return;
-  const int64_t gi = nodes[w];
+// backward solve of node gi (one wavefront): z_n from y_n (in z) and the upper neighbours' z
+__device__ inline void ic_backward_node(const fe::Grid &g, int64_t gi, const uint16_t *__restrict__ mask,
+                                        const double *__restrict__ U, double *z, int lane) {
   const int k = (int)(gi % g.nz);
   const int64_t cj = gi / g.nz;
   const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
@@ -2092,7 +2079,67 @@ __global__ void __launch_bounds__(64 * IC_WAVES) k_ic_backward(fe::Grid g, const
   for (int p = 0; p < 3; p++) s[p] = wave_sum(s[p]);
   if (lane == 0) ic_bwd_finish(U + gi * IC_STRIDE, s, z + 3 * gi);
 }
+
+// one dependency level per launch, one wavefront per node
+__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_factor(fe::Grid g, const int32_t *__restrict__ nodes,
+                                                             int32_t cnt, const uint16_t *__restrict__ mask,
+                                                             double *U, int32_t *fail) {
+  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
+  if (w >= cnt) return;
+  ic_factor_node(g, nodes[w], mask, U, fail, threadIdx.x & 63);
+}
+
+__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_forward(fe::Grid g, const int32_t *__restrict__ nodes,
+                                                              int32_t cnt, const uint16_t *__restrict__ mask,
+                                                              const double *__restrict__ U,
+                                                              const double *__restrict__ r, double *z) {
+  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
+  if (w >= cnt) return;
+  ic_forward_node(g, nodes[w], mask, U, r, z, threadIdx.x & 63);
+}
+
+__global__ void __launch_bounds__(64 * IC_WAVES) k_ic_backward(fe::Grid g, const int32_t *__restrict__ nodes,
+                                                               int32_t cnt, const uint16_t *__restrict__ mask,
+                                                               const double *__restrict__ U, double *z) {
+  const int64_t w = (int64_t)blockIdx.x * IC_WAVES + (threadIdx.x >> 6);
+  if (w >= cnt) return;
+  ic_backward_node(g, nodes[w], mask, U, z, threadIdx.x & 63);
+}
 #endif
+
+// dependency levels of the nodes present in mask (bit 0): a node's level is one
+// more than the deepest of its lower neighbours; nodes grouped by level, in
+// grid order within a level (ptr: level starts, nl + 1 entries)
+void ic_levels(const fe::Grid &g, const std::vector<uint16_t> &mask, std::vector<int32_t> &nodes,
+               std::vector<int64_t> &ptr) {
+  std::vector<int32_t> level(g.G, -1);
+  int32_t nl = 0;
+  for (int i = 0; i < g.nx; i++)
+    for (int j = 0; j < g.ny; j++)
+      for (int k = 0; k < g.nz; k++) {
+        const int64_t gi = ((int64_t)i * g.ny + j) * g.nz + k;
+        if (!(mask[gi] & 1)) continue;
+        int32_t L = 0;
+        for (int tl = 1; tl < IC_SLOTS; tl++) {
+          int ei, ej, ek;
+          ic_off(tl, ei, ej, ek);
+          const int mi = i - ei, mj = j - ej, mk = k - ek;
+          if (mi < 0 || mj < 0 || mj >= g.ny || mk < 0 || mk >= g.nz) continue;
+          const int64_t gm = ((int64_t)mi * g.ny + mj) * g.nz + mk;
+          if ((mask[gm] >> tl) & 1u) L = std::max(L, level[gm] + 1);
+        }
+        level[gi] = L;
+        nl = std::max(nl, L + 1);
+      }
+  ptr.assign(nl + 1, 0);
+  for (int64_t q = 0; q < g.G; q++)
+    if (level[q] >= 0) ptr[level[q] + 1]++;
+  for (int32_t l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+  nodes.assign(ptr[nl], 0);
+  std::vector<int64_t> fillp(ptr.begin(), ptr.end() - 1);
+  for (int64_t q = 0; q < g.G; q++)
+    if (level[q] >= 0) nodes[fillp[level[q]]++] = (int32_t)q;
+}
 
 // pattern slots and dependency levels (once per model)
 void ic_pattern(fcbga_model *M) {
@@ -2116,40 +2163,87 @@ void ic_pattern(fcbga_model *M) {
           }
         }
       }
-  // level of a node: one more than the deepest of its lower neighbours
-  std::vector<int32_t> level(g.G, -1);
-  int32_t nl = 0;
-  for (int i = 0; i < g.nx; i++)
-    for (int j = 0; j < g.ny; j++)
-      for (int k = 0; k < g.nz; k++) {
-        const int64_t gi = ((int64_t)i * g.ny + j) * g.nz + k;
-        if (!(mask[gi] & 1)) continue;
-        int32_t L = 0;
-        for (int tl = 1; tl < IC_SLOTS; tl++) {
-          int ei, ej, ek;
-          ic_off(tl, ei, ej, ek);
-          const int mi = i - ei, mj = j - ej, mk = k - ek;
-          if (mi < 0 || mj < 0 || mj >= g.ny || mk < 0 || mk >= g.nz) continue;
-          const int64_t gm = ((int64_t)mi * g.ny + mj) * g.nz + mk;
-          if ((mask[gm] >> tl) & 1u) L = std::max(L, level[gm] + 1);
-        }
-        level[gi] = L;
-        nl = std::max(nl, L + 1);
-      }
-  std::vector<int64_t> ptr(nl + 1, 0);
-  for (int64_t q = 0; q < g.G; q++)
-    if (level[q] >= 0) ptr[level[q] + 1]++;
-  for (int32_t l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
-  std::vector<int32_t> nodes(ptr[nl]);
-  std::vector<int64_t> fillp(ptr.begin(), ptr.end() - 1);
-  for (int64_t q = 0; q < g.G; q++)
-    if (level[q] >= 0) nodes[fillp[level[q]]++] = (int32_t)q;
+  std::vector<int32_t> nodes;
+  std::vector<int64_t> ptr;
+  ic_levels(g, mask, nodes, ptr);
   M->icMask.upload(mask, M->stream);
   M->icNodes.upload(nodes, M->stream);
   M->icLvPtr = ptr;
   M->icFail.alloc(1);
   M->icPattern = true;
 }
+
+#if !FCBGA_GPU
+// host builds: the factor node by node in level order (the device's
+// per-node computation with the lanes as a loop); false on a non-positive pivot
+bool ic_factor_host(const fe::Grid &g, const int32_t *nodes, int64_t nn, const uint16_t *mask, double *U) {
+  bool ok = true;
+  for (int64_t w = 0; w < nn; w++) {
+    const int64_t gi = nodes[w];
+    const int k = (int)(gi % g.nz);
+    const int64_t cj = gi / g.nz;
+    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    const uint32_t mk = mask[gi];
+    double *Un = U + gi * IC_STRIDE;
+    double acc[3][3 * IC_SLOTS];
+    for (int l = 0; l < 3 * IC_SLOTS; l++) {
+      const int t = l / 3, q = l % 3;
+      double a3[3] = {0.0, 0.0, 0.0};
+      if ((mk >> t) & 1u) {
+        for (int p = 0; p < 3; p++) a3[p] = Un[t * 9 + 3 * p + q];
+        ic_lower_updates(g, i, j, k, t, q, mask, U, a3);
+      }
+      for (int p = 0; p < 3; p++) acc[p][l] = a3[p];
+    }
+    for (int p = 0; p < 3; p++) {
+      for (int r = 0; r < p; r++)
+        for (int l = p; l < 3 * IC_SLOTS; l++) acc[p][l] -= acc[r][p] * acc[r][l];
+      const double d = acc[p][p];
+      ok = ok && d > 0;
+      const double u = std::sqrt(d);
+      acc[p][p] = u;
+      for (int l = p + 1; l < 3 * IC_SLOTS; l++) acc[p][l] /= u;
+    }
+    for (int l = 0; l < 3 * IC_SLOTS; l++) {
+      const int t = l / 3, q = l % 3;
+      if (!((mk >> t) & 1u)) continue;
+      for (int p = 0; p < 3; p++)
+        if (t > 0 || q >= p) Un[t * 9 + 3 * p + q] = acc[p][l];
+    }
+  }
+  return ok;
+}
+
+void ic_apply_host(const fe::Grid &g, const int32_t *nodes, int64_t nn, const uint16_t *mask, const double *U,
+                   const double *r, double *z) {
+  for (int64_t w = 0; w < nn; w++) {
+    const int64_t gi = nodes[w];
+    const int k = (int)(gi % g.nz);
+    const int64_t cj = gi / g.nz;
+    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int l = 3; l < 3 * IC_SLOTS; l++) {
+      double t3[3] = {0.0, 0.0, 0.0};
+      ic_fwd_lane(g, i, j, k, l / 3, l % 3, mask, U, z, t3);
+      for (int p = 0; p < 3; p++) s[p] += t3[p];
+    }
+    ic_fwd_finish(U + gi * IC_STRIDE, r + 3 * gi, s, z + 3 * gi);
+  }
+  for (int64_t w = nn - 1; w >= 0; w--) {
+    const int64_t gi = nodes[w];
+    const int k = (int)(gi % g.nz);
+    const int64_t cj = gi / g.nz;
+    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int l = 3; l < 3 * IC_SLOTS; l++) {
+      double t3[3] = {0.0, 0.0, 0.0};
+      ic_bwd_lane(g, gi, i, j, k, l / 3, l % 3, mask[gi], U, z, t3);
+      for (int p = 0; p < 3; p++) s[p] += t3[p];
+    }
+    ic_bwd_finish(U + gi * IC_STRIDE, s, z + 3 * gi);
+  }
+}
+#endif
 
 // values of the scaled, constraint-reduced matrix in the upper pattern with
 // the diagonal times (1 + shift), then the factor level by level; false on a
@@ -2208,41 +2302,7 @@ bool ic_try(fcbga_model *M, double shift) {
   be::d2h(&h, fail, sizeof(int32_t), s);
   return h == 0;
 #else
-  bool ok = true;
-  for (int64_t w = 0; w < M->icLvPtr[nl]; w++) {
-    const int64_t gi = nodes[w];
-    const int k = (int)(gi % g.nz);
-    const int64_t cj = gi / g.nz;
-    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
-    const uint32_t mk = mask[gi];
-    double *Un = U + gi * IC_STRIDE;
-    double acc[3][3 * IC_SLOTS];
-    for (int l = 0; l < 3 * IC_SLOTS; l++) {
-      const int t = l / 3, q = l % 3;
-      double a3[3] = {0.0, 0.0, 0.0};
-      if ((mk >> t) & 1u) {
-        for (int p = 0; p < 3; p++) a3[p] = Un[t * 9 + 3 * p + q];
-        ic_lower_updates(g, i, j, k, t, q, mask, U, a3);
-      }
-      for (int p = 0; p < 3; p++) acc[p][l] = a3[p];
-    }
-    for (int p = 0; p < 3; p++) {
-      for (int r = 0; r < p; r++)
-        for (int l = p; l < 3 * IC_SLOTS; l++) acc[p][l] -= acc[r][p] * acc[r][l];
-      const double d = acc[p][p];
-      ok = ok && d > 0;
-      const double u = std::sqrt(d);
-      acc[p][p] = u;
-      for (int l = p + 1; l < 3 * IC_SLOTS; l++) acc[p][l] /= u;
-    }
-    for (int l = 0; l < 3 * IC_SLOTS; l++) {
-      const int t = l / 3, q = l % 3;
-      if (!((mk >> t) & 1u)) continue;
-      for (int p = 0; p < 3; p++)
-        if (t > 0 || q >= p) Un[t * 9 + 3 * p + q] = acc[p][l];
-    }
-  }
-  return ok;
+  return ic_factor_host(g, nodes, M->icLvPtr[nl], mask, U);
 #endif
 }
 
@@ -2288,33 +2348,7 @@ void ic_apply(fcbga_model *M, const double *r, double *z) {
     BE_CHECK(hipGetLastError());
   }
 #else
-  const int64_t nn = M->icLvPtr[nl];
-  for (int64_t w = 0; w < nn; w++) {
-    const int64_t gi = nodes[w];
-    const int k = (int)(gi % g.nz);
-    const int64_t cj = gi / g.nz;
-    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int l = 3; l < 3 * IC_SLOTS; l++) {
-      double t3[3] = {0.0, 0.0, 0.0};
-      ic_fwd_lane(g, i, j, k, l / 3, l % 3, mask, U, z, t3);
-      for (int p = 0; p < 3; p++) s[p] += t3[p];
-    }
-    ic_fwd_finish(U + gi * IC_STRIDE, r + 3 * gi, s, z + 3 * gi);
-  }
-  for (int64_t w = nn - 1; w >= 0; w--) {
-    const int64_t gi = nodes[w];
-    const int k = (int)(gi % g.nz);
-    const int64_t cj = gi / g.nz;
-    const int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int l = 3; l < 3 * IC_SLOTS; l++) {
-      double t3[3] = {0.0, 0.0, 0.0};
-      ic_bwd_lane(g, gi, i, j, k, l / 3, l % 3, mask[gi], U, z, t3);
-      for (int p = 0; p < 3; p++) s[p] += t3[p];
-    }
-    ic_bwd_finish(U + gi * IC_STRIDE, s, z + 3 * gi);
-  }
+  ic_apply_host(g, nodes, M->icLvPtr[nl], mask, U, r, z);
 #endif
 }
 
@@ -3254,6 +3288,8 @@ int fail(const std::exception &e, int code) {
 
 }  // namespace
 
+#include "submodel.h"
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -3608,6 +3644,105 @@ int fcbga_last_error(char *buf, size_t len) {
   std::memcpy(buf, g_err.data(), n);
   buf[n] = 0;
   return (int)n;
+}
+
+int fcbga_submodel_create(const fcbga_submodel_mesh *mesh, const fcbga_anand_params *solder, fcbga_submodel **out) {
+  if (!out) { g_err = "out is NULL"; return FCBGA_ERR_ARG; }
+  *out = nullptr;
+  fcbga_submodel *S = new fcbga_submodel();
+  try {
+    sub_create(mesh, solder, S);
+  } catch (const be::Error &e) {
+    fcbga_submodel_destroy(S);
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    fcbga_submodel_destroy(S);
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  *out = S;
+  return FCBGA_OK;
+}
+
+void fcbga_submodel_destroy(fcbga_submodel *s) {
+  if (!s) return;
+#if FCBGA_GPU
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+#endif
+  be::stream_t st = s->stream;
+  delete s;
+#if FCBGA_GPU
+  if (st) (void)hipStreamDestroy(st);
+#else
+  (void)st;
+#endif
+}
+
+int32_t fcbga_submodel_nboundary(const fcbga_submodel *s) { return s ? s->nb : -1; }
+
+int fcbga_submodel_step(fcbga_submodel *s, const fcbga_submodel_step_in *in, fcbga_submodel_step_out *out) {
+  if (!s || !in || !out) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    sub_step(s, *in, *out);
+  } catch (const be::Error &e) {
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_submodel_save(fcbga_submodel *s) {
+  if (!s) { g_err = "submodel is NULL"; return FCBGA_ERR_ARG; }
+  try {
+    sub_save(s);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_submodel_restore(fcbga_submodel *s) {
+  if (!s) { g_err = "submodel is NULL"; return FCBGA_ERR_ARG; }
+  try {
+    sub_restore(s);
+  } catch (const be::Error &e) {
+    return fail(e, FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_submodel_layer_means(fcbga_submodel *s, double *out4) {
+  if (!s || !out4) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    sub_layer_means(s, out4);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_submodel_loop_sample(fcbga_submodel *s, double T, double E, double fsolder, double cphi, double sphi,
+                               double *out4) {
+  if (!s || !out4) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  if (!(E > 0)) { g_err = "E > 0 required"; return FCBGA_ERR_ARG; }
+  try {
+    sub_loop_sample(s, T, E, fsolder, cphi, sphi, out4);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_submodel_displacement(fcbga_submodel *s, double *u) {
+  if (!s || !u) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    be::d2h(u, s->u.p, sizeof(double) * s->ndof, s->stream);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
 }
 
 }  // extern "C"

@@ -659,6 +659,42 @@ FE_HD double anand_step(const AnandParams &P, double E, double T, double dt, con
 }
 
 // ---------------------------------------------------------------------------
+// B-bar (mean dilatation) strain-displacement matrix of a box element at a
+// point (bbarMatrix, src/core/13-fatigue.js:474-490): 6 x 24 row-major, rows
+// xx yy zz xy yz zx (engineering shear), columns 3a + {x, y, z} for corners
+// a = corner_of(li, lj, lk); the normal rows take the volumetric part of the
+// element centre's derivatives.
+// ---------------------------------------------------------------------------
+FE_HD void box_corner_derivs(double hx, double hy, double hz, double xi, double eta, double zeta, double dx[8],
+                             double dy[8], double dz[8]) {
+  const double sx = 2 / hx, sy = 2 / hy, sz = 2 / hz;
+  for (int a = 0; a < 8; a++) {
+    const double xa = 2 * corner_li(a) - 1, ya = 2 * corner_lj(a) - 1, za = 2 * corner_lk(a) - 1;
+    dx[a] = xa * (1 + ya * eta) * (1 + za * zeta) * 0.125 * sx;
+    dy[a] = ya * (1 + xa * xi) * (1 + za * zeta) * 0.125 * sy;
+    dz[a] = za * (1 + xa * xi) * (1 + ya * eta) * 0.125 * sz;
+  }
+}
+
+FE_HD void bbar_matrix(double hx, double hy, double hz, double xi, double eta, double zeta, double *B) {
+  FE_NO_CONTRACT
+  double dx[8], dy[8], dz[8], cx[8], cy[8], cz[8];
+  box_corner_derivs(hx, hy, hz, xi, eta, zeta, dx, dy, dz);
+  box_corner_derivs(hx, hy, hz, 0, 0, 0, cx, cy, cz);
+  for (int q = 0; q < 144; q++) B[q] = 0;
+  for (int a = 0; a < 8; a++) {
+    const int c = 3 * a;
+    const double ddx = (cx[a] - dx[a]) / 3, ddy = (cy[a] - dy[a]) / 3, ddz = (cz[a] - dz[a]) / 3;
+    B[c] = dx[a] + ddx; B[c + 1] = ddy; B[c + 2] = ddz;
+    B[24 + c] = ddx; B[24 + c + 1] = dy[a] + ddy; B[24 + c + 2] = ddz;
+    B[48 + c] = ddx; B[48 + c + 1] = ddy; B[48 + c + 2] = dz[a] + ddz;
+    B[72 + c] = dy[a]; B[72 + c + 1] = dx[a];
+    B[96 + c + 1] = dz[a]; B[96 + c + 2] = dy[a];
+    B[120 + c] = dz[a]; B[120 + c + 2] = dx[a];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Bump screening (screenJoints, src/core/13-fatigue.js:375-455): each joint a
 // free-standing solder column whose axial and transverse shear strains follow
 // the bump layer while its in-plane stresses vanish (AnandPoint.stepColumn,

@@ -1,5 +1,5 @@
 // Golden vectors from the reference JS core (run in the build container only:
-//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|ic|screen|<spec name> ...]
+//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|ic|screen|submodel|<spec name> ...]
 // ). It loads src/core/*.js the way the reference's own tools/dev-core.mjs does,
 // builds each committed mesh spec (*_spec.json) with the reference mesher's
 // buildStructuredMesh, assembles one thermal step and solves it with the
@@ -13,6 +13,7 @@
 //   <name>_ic_golden.json, <name>_ic_u.f64.gz     (IC(0)-preconditioned solves)
 //   fallback_golden.json                (the fallback to the other preconditioner)
 //   screen_golden.json, screen_<case>_table.f64.gz (bump screening on synthetic strain tables, cycle histories, site subsampling)
+//   submodel_golden.json, submodel_*.f64.gz (a bump submodel of the bare-die preset: set-up data and result)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -355,11 +356,63 @@ function screenCases() {
     pillar_mat: { E_table: { T: [-55, 25, 150], v: [123000, 120000, 115000] }, nu: 0.34, cte: 17 }, cases, histories: hists, subsample: subs }));
 }
 
+// Bump submodel (BumpSubmodel, src/core/13-fatigue.js:514-1105): the reference's own global
+// analysis of the bare-die preset at the Draft mesh (2 cycles) and one corner bump's
+// submodel (8 x 8 x 8 elements). Written: the submodel's set-up data (planes, element
+// materials, boundary nodes, the material grids of its elastic parts, the prescribed
+// boundary displacement at every sample temperature of the cycle), the first steps' Newton
+// records and displacement, and the result record.
+function submodelCases() {
+  const db = JSON.parse(fs.readFileSync(path.join(REF, 'src', 'materials-db.json'), 'utf8'));
+  const presetConfig = get('presetConfig'), Analysis = get('Analysis'), BumpSubmodel = get('BumpSubmodel');
+  const dieBumpSites = get('dieBumpSites'), materialTables = get('materialTables');
+  const cfg = presetConfig(db, 'bare');
+  cfg.mesh.preset = 'draft';
+  cfg.cycling.cycles = 2;
+  const an = new Analysis(cfg, { preset: 'draft', evaluations: 'full' });
+  an.run();
+  const d = an.mesh.geom.dies[0], g = an.mesh.geom;
+  const sites = dieBumpSites(d);
+  const q = 0, site = { x: sites.x[q], y: sites.y[q], ix: sites.ix[q], iy: sites.iy[q] };
+  const sm = new BumpSubmodel(an, 0, site, { nxy: 8, nz: 8 });
+  const m = sm.mesh;
+  const cyc = cfg.cycling;
+  const Tb = an.temps.all.filter(T => T >= cyc.Tmin - 1e-9 && T <= cyc.Tmax + 1e-9);
+  const ub = new Float64Array(sm.ndof), bvals = [];
+  for (const T of Tb) { sm.prescribe(T, ub); for (const n of sm.bNodes) bvals.push(ub[3 * n], ub[3 * n + 1], ub[3 * n + 2]); }
+  const first = [];
+  const orig = sm.stepTo.bind(sm);
+  let u1 = null;
+  sm.stepTo = (T, dt) => { const r = orig(T, dt); if (first.length < 6) first.push({ T, dt, converged: r.converged, newton: r.newton, dMax: r.dMax }); if (!u1 && r.converged) u1 = Float64Array.from(sm.u); return r; };
+  const st = materialTables(sm.solderMat), ut = sm.ufTables;
+  const a = sm.solderMat.anand, pv = x => (typeof x === 'object' ? x.v : x);
+  const setup = {
+    xs: list(m.xs), ys: list(m.ys), zs: list(m.zs), eGroup: list(m.eGroup), eIJK: list(m.eIJK), nn: m.nn, ne: m.ne,
+    bNodes: list(sm.bNodes), zt: g.zt, zaf: g.zaf, die: { cx: d.cx, cy: d.cy, rt: d.profile.rt, rb: d.profile.rb }, site,
+    solder: { A: pv(a.A), QR: pv(a.QR), xi: pv(a.xi), m: pv(a.m), shat: pv(a.shat), n: pv(a.n), h0: pv(a.h0), a: pv(a.a), s0: pv(a.s0), nu: st.nu },
+    ufNu: ut.nu, fatigue: cfg.fatigue, cycling: cyc, tempsAll: an.temps.all, boundaryTemps: Tb,
+  };
+  f64gz(st.E, 'submodel_solderE.f64.gz'); f64gz(st.F, 'submodel_solderF.f64.gz');
+  f64gz(ut.E, 'submodel_ufE.f64.gz'); f64gz(ut.F, 'submodel_ufF.f64.gz');
+  f64gz(sm.subD, 'submodel_subD.f64.gz'); f64gz(sm.subF, 'submodel_subF.f64.gz');
+  f64gz(sm.siD, 'submodel_siD.f64.gz'); f64gz(sm.siF, 'submodel_siF.f64.gz');
+  f64gz(bvals, 'submodel_boundary.f64.gz');
+  const t0 = Date.now();
+  const r = sm.run();
+  f64gz(u1, 'submodel_u1.f64.gz');
+  const rec = { dW: r.dW, dWdie: r.dWdie, dWsub: r.dWsub, dE: r.dE, dEdie: r.dEdie, dEsub: r.dEsub, critical: r.critical,
+    stabChange: r.stabChange, perCycle: r.perCycle, steps: r.steps, cutbacks: r.cutbacks, newtonPerStep: r.newtonPerStep,
+    dJoint: r.dJoint, life: r.life, loop: r.loop, ndof: r.ndof, nElem: r.nElem };
+  fs.writeFileSync(path.join(HERE, 'submodel_golden.json'), JSON.stringify({ setup, first, result: rec }));
+  console.log('submodel', r.ndof, 'DOF; dW', r.dW, 'steps', r.steps, 'cutbacks', r.cutbacks, 'newton/step', r.newtonPerStep.toFixed(2), 'ms', Date.now() - t0);
+}
+
 const what = process.argv.slice(3);
 if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }
 if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
 if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(n);
 if (what.includes('anand')) anandCases();
 if (what.includes('screen')) screenCases();
+if (what.includes('submodel')) submodelCases();
 if (!what.length || what.includes('element')) elementCases();
 for (const n of ['small_box', 'plate_16', 'cfg1', 'cfg2']) if (!what.length || what.includes(n)) solveSpec(n);

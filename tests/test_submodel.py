@@ -1,0 +1,73 @@
+"""Bump submodel (BumpSubmodel, src/core/13-fatigue.js:514-1105) on the device
+engine against the reference's own run of one corner bump of the bare-die
+preset (Draft global analysis, 8 x 8 x 8 elements, two cycles), set up from
+the reference's data (tests/golden/gen_golden.js 'submodel').
+
+The first steps pin the engine step by step: the same Newton iteration counts,
+the same largest plastic increment (1e-9) and the reference's displacement
+after the first step; the adaptive sub-stepping must take the same
+sub-division decisions.  The full cycle pins the result record: work
+densities of both averaging layers, the strain range, the hysteresis loop,
+the step and cutback counts."""
+import math
+
+import numpy as np
+import pytest
+
+from submodel_case import build, f64
+
+VARIANTS = ["hostemu", pytest.param("hip", marks=pytest.mark.gpu)]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_submodel_first_step_displacement(variant):
+    g, m, _ = build(variant)
+    res = m.step(30.0, 30.0)
+    ref = g["first"][0]
+    assert res["converged"] and res["newton"] == ref["newton"]
+    assert abs(res["dMax"] - ref["dMax"]) <= 1e-9 * ref["dMax"]
+    u = m.displacement()
+    u1 = f64("submodel_u1.f64.gz")
+    assert np.abs(u - u1).max() <= 1e-9 * np.abs(u1).max()
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_submodel_adaptive_steps_match_reference(variant):
+    """Five history steps: the first six engine steps with the reference's sub-divisions
+    (30, 35, 40 C at 30 s, then 42.5, 45 at 15 s and 46.25 at 7.5 s)."""
+    g, m, _ = build(variant)
+    r = m.run(max_steps=5)
+    assert len(r["first"]) == len(g["first"]) == 6
+    for a, b in zip(r["first"], g["first"]):
+        assert (a["T"], a["dt"], a["converged"], a["newton"]) == (b["T"], b["dt"], b["converged"], b["newton"])
+        assert abs(a["dMax"] - b["dMax"]) <= 1e-9 * b["dMax"]
+
+
+def test_submodel_rejects_bad_input():
+    from fcbga_amd import hip_solver as hs
+    g, m, _ = build("hostemu")
+    with pytest.raises(hs.SolverError):
+        m.restore()  # nothing saved
+    with pytest.raises(hs.SolverError):
+        m.step(30.0, -1.0)
+
+
+@pytest.mark.gpu
+def test_gpu_submodel_two_cycles_match_reference():
+    """The whole run: 686 steps with 4 cutbacks in the reference.  Work density
+    per cycle, strain range, stabilisation and the loop to 1e-6 of the
+    reference (the rounding of the device's reductions moves Newton's stopping
+    point by far less), step and cutback counts exactly."""
+    g, m, _ = build("hip")
+    r = m.run()
+    ref = g["result"]
+    assert (r["steps"], r["cutbacks"]) == (ref["steps"], ref["cutbacks"])
+    for k in ("dW", "dWdie", "dWsub", "dE", "dEdie", "dEsub", "stabChange", "dJoint", "newtonPerStep"):
+        assert abs(r[k] - ref[k]) <= 1e-6 * abs(ref[k]), (k, r[k], ref[k])
+    assert r["critical"] == ref["critical"]
+    for c, (a, b) in enumerate(zip(r["perCycle"], ref["perCycle"])):
+        assert abs(a["dWdie"] - b["dWdie"]) <= 1e-6 * abs(b["dWdie"]) and abs(a["dWsub"] - b["dWsub"]) <= 1e-6 * abs(b["dWsub"])
+    for key in ("tau", "gamma", "tauSub", "gammaSub"):
+        a, b = np.array(r["loop"][key]), np.array(ref["loop"][key], dtype=np.float64)
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max(), key
+    assert math.isclose(r["life"]["syedEnergy"], ref["life"]["syedEnergy"], rel_tol=1e-6)
