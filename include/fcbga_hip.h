@@ -267,6 +267,9 @@ int fcbga_submodel_layer_means(fcbga_submodel *s, double *out4);
 /* hysteresis sample: {tau die, gamma die, tau sub, gamma sub} along (cphi, sphi) */
 int fcbga_submodel_loop_sample(fcbga_submodel *s, double T, double E, double fsolder, double cphi, double sphi,
                                double *out4);
+/* stress change from 25 C at the committed state, extrapolated to the corners of every element
+ * (_stressSnapshot, :1036-1062): 48 floats per element (8 corners x 6), E and fsolder at T */
+int fcbga_submodel_section(fcbga_submodel *s, double E, double fsolder, float *out);
 /* displacement (3 per grid node) */
 int fcbga_submodel_displacement(fcbga_submodel *s, double *u);
 

@@ -3735,6 +3735,17 @@ int fcbga_submodel_loop_sample(fcbga_submodel *s, double T, double E, double fso
   return FCBGA_OK;
 }
 
+int fcbga_submodel_section(fcbga_submodel *s, double E, double fsolder, float *out) {
+  if (!s || !out) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  if (!(E > 0)) { g_err = "E > 0 required"; return FCBGA_ERR_ARG; }
+  try {
+    sub_section(s, E, fsolder, out);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
 int fcbga_submodel_displacement(fcbga_submodel *s, double *u) {
   if (!s || !u) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
   try {

@@ -981,6 +981,73 @@ FE_HD void sub_loop_element(const SubView &v, const fe::AnandParams &P, int64_t 
   }
 }
 
+// stresses of one element at the committed state extrapolated to its corners
+// (_stressSnapshot, src/core/13-fatigue.js:1036-1062): elastic parts the
+// committed Gauss-point stresses, solder the Anand stress at the committed
+// plastic strain (stressAt, :221-227) for the current strain; corner a gets
+// sum over points of GP_TO_NODE[a][gq] sigma_gq (the trilinear functions of the
+// points at sqrt(3) x the corner's natural coordinates)
+FE_HD void sub_section_element(const SubView &v, const fe::AnandParams &P, int64_t e, double E, double fsol,
+                               const double *u, const double *st, const double *sigE, float *out) {
+  const fe::Grid &g = v.g;
+  int i, j, k;
+  sub_cell(g, e, i, j, k);
+  double gpS[48];
+  if (v.cellMat[e] != SUB_SOLDER) {
+    for (int q = 0; q < 48; q++) gpS[q] = sigE[48 * e + q];
+  } else {
+    const double hx = v.xs[i + 1] - v.xs[i], hy = v.ys[j + 1] - v.ys[j], hz = v.zs[k + 1] - v.zs[k];
+    double ue[24], B[144];
+    for (int a = 0; a < 8; a++) {
+      const int64_t n = sub_node(g, i + fe::corner_li(a), j + fe::corner_lj(a), k + fe::corner_lk(a));
+      for (int c = 0; c < 3; c++) ue[3 * a + c] = u[3 * n + c];
+    }
+    const double G = E / (2 * (1 + P.nu)), K = E / (3 * (1 - 2 * P.nu));
+    const double gp = 0x1.279a74590331dp-1;
+    for (int gq = 0; gq < 8; gq++) {
+      fe::bbar_matrix(hx, hy, hz, (2 * fe::corner_li(gq) - 1) * gp, (2 * fe::corner_lj(gq) - 1) * gp,
+                      (2 * fe::corner_lk(gq) - 1) * gp, B);
+      double eps[6];
+      for (int r = 0; r < 6; r++) {
+        double s2 = 0;
+        for (int c = 0; c < 24; c++) s2 += B[r * 24 + c] * ue[c];
+        eps[r] = s2;
+      }
+      eps[0] -= fsol; eps[1] -= fsol; eps[2] -= fsol;
+      const double *ep = st + 11 * (8 * e + gq);
+      const double vol = (eps[0] + eps[1] + eps[2]) / 3, sm = 3 * K * vol;
+      for (int c = 0; c < 3; c++) gpS[6 * gq + c] = 2 * G * (eps[c] - vol - ep[c]) + sm;
+      for (int c = 3; c < 6; c++) gpS[6 * gq + c] = 2 * G * (0.5 * eps[c] - ep[c]);
+    }
+  }
+  const double s3 = 1.7320508075688772;  // sqrt(3)
+  for (int a = 0; a < 8; a++) {
+    const double x = (2 * fe::corner_li(a) - 1) * s3, y = (2 * fe::corner_lj(a) - 1) * s3,
+                 z = (2 * fe::corner_lk(a) - 1) * s3;
+    double N[8];
+    for (int gq = 0; gq < 8; gq++)
+      N[gq] = 0.125 * (1 + (2 * fe::corner_li(gq) - 1) * x) * (1 + (2 * fe::corner_lj(gq) - 1) * y) *
+              (1 + (2 * fe::corner_lk(gq) - 1) * z);
+    for (int c = 0; c < 6; c++) {
+      double acc = 0;
+      for (int gq = 0; gq < 8; gq++) acc += N[gq] * gpS[6 * gq + c];
+      out[48 * e + 6 * a + c] = (float)acc;
+    }
+  }
+}
+
+void sub_section(fcbga_submodel *S, double E, double fsol, float *host_out) {
+  be::stream_t s = S->stream;
+  const SubView v = sub_view(S);
+  const fe::AnandParams P = S->P;
+  const double *u = S->u.p, *st = S->st.p, *sigE = S->sigE.p;
+  DBuf<float> o;
+  o.alloc(48 * S->ne);
+  float *op = o.p;
+  be::pfor(S->ne, s, [=] __host__ __device__(int64_t e) { sub_section_element(v, P, e, E, fsol, u, st, sigE, op); });
+  be::d2h(host_out, op, sizeof(float) * 48 * S->ne, s);
+}
+
 void sub_loop_sample(fcbga_submodel *S, double T, double E, double fsol, double cphi, double sphi, double out[4]) {
   be::stream_t s = S->stream;
   const SubView v = sub_view(S);

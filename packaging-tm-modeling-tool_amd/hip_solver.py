@@ -80,7 +80,8 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_state_stress", "fcbga_adjoint_forward", "fcbga_adjoint_backward", "fcbga_screen_joints",
            "fcbga_submodel_create", "fcbga_submodel_destroy", "fcbga_submodel_nboundary", "fcbga_submodel_step",
            "fcbga_submodel_save", "fcbga_submodel_restore", "fcbga_submodel_layer_means",
-           "fcbga_submodel_loop_sample", "fcbga_submodel_displacement", "fcbga_last_error"]
+           "fcbga_submodel_loop_sample", "fcbga_submodel_section", "fcbga_submodel_displacement",
+           "fcbga_last_error"]
 
 ANAND_STATE = 11
 STATE = 81  # stress-history state of one element (FCBGA_STATE)

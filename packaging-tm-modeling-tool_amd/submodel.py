@@ -143,6 +143,8 @@ def _bind(L):
     L.fcbga_submodel_layer_means.restype = C.c_int
     L.fcbga_submodel_loop_sample.argtypes = [C.c_void_p] + [C.c_double] * 5 + [_f64p]
     L.fcbga_submodel_loop_sample.restype = C.c_int
+    L.fcbga_submodel_section.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_float)]
+    L.fcbga_submodel_section.restype = C.c_int
     L.fcbga_submodel_displacement.argtypes = [C.c_void_p, _f64p]
     L.fcbga_submodel_displacement.restype = C.c_int
     L._sub_bound = True
@@ -283,6 +285,15 @@ class Submodel:
             raise hs.SolverError(hs.last_error(self.L))
         return out
 
+    def section(self, T):
+        """Corner stresses of every element at the committed state (48 float32 per element)."""
+        out = np.zeros(48 * len(self.cell_mat), np.float32)
+        E = grid_at(self.solder.E, T)
+        f = grid_at(self.solder.F, T) - grid_at(self.solder.F, 25.0)
+        if self.L.fcbga_submodel_section(self.h, E, f, out.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+            raise hs.SolverError(hs.last_error(self.L))
+        return out
+
     def displacement(self):
         u = np.zeros(3 * len(self.xs) * len(self.ys) * len(self.zs))
         if self.L.fcbga_submodel_displacement(self.h, u.ctypes.data_as(_f64p)) != 0:
@@ -290,9 +301,11 @@ class Submodel:
         return u
 
     # -- the cycle ----------------------------------------------------------------------
-    def run(self, max_steps=None):
+    def run(self, max_steps=None, sections=True):
         """The cycle with adaptive sub-stepping (BumpSubmodel.run): returns the
-        result record.  max_steps stops after that many history steps (tests)."""
+        result record, with the corner stresses at the ends of the last cycle's
+        ramps and dwells when sections.  max_steps stops after that many
+        history steps (tests)."""
         st = self.setup
         hist = cycle_history(st["cycling"])
         nsteps = len(hist["t"]) if max_steps is None else min(len(hist["t"]), max_steps + 1)
@@ -304,6 +317,7 @@ class Submodel:
         total_newton = cutbacks = steps = 0
         nsub_next = 1
         trace = []
+        snaps = []
 
         def sample(T, t):
             s = self.loop_sample(T, cphi, sphi)
@@ -343,6 +357,8 @@ class Submodel:
                 raise hs.SolverError(f"Submodel step failed to converge after cutbacks at T = {T1:.1f} C")
             sample(T1, t1s)
             c = int(hist["cycle"][k])
+            if sections and c == nc and (k == len(hist["t"]) - 1 or hist["phase"][k + 1] != hist["phase"][k]):
+                snaps.append({"T": T1, "t": t1s, "cycle": c, "phase": hist["phase"][k], "sig": self.section(T1)})
             m = self.layer_means()
             Wd[c], Ed[c], Ws[c], Es[c] = m[0], m[1], m[2], m[3]
         dWdie, dWsub = Wd[nc] - Wd[nc - 1], Ws[nc] - Ws[nc - 1]
@@ -362,4 +378,5 @@ class Submodel:
                 "perCycle": [{"cycle": c, "dWdie": Wd[c] - Wd[c - 1], "dWsub": Ws[c] - Ws[c - 1]}
                              for c in range(1, nc + 1)],
                 "loop": loop, "life": life, "dJoint": d_joint, "steps": steps, "cutbacks": cutbacks,
-                "newtonPerStep": total_newton / max(steps, 1), "first": trace}
+                "newtonPerStep": total_newton / max(steps, 1), "first": trace,
+                "section": {"xs": self.xs, "ys": self.ys, "zs": self.zs, "grp": self.cell_mat, "snaps": snaps}}

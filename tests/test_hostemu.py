@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "fcbga_hip.h")
 
 def _declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|int64_t)\s+(fcbga_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|int64_t|int32_t)\s+(fcbga_\w+)\s*\(", txt, re.M)))
 
 
 def test_header_matches_binding_exports():
