@@ -174,6 +174,112 @@ function makeScreenJoints(core) {
   };
 }
 
+/**
+ * BumpSubmodel.run (src/core/13-fatigue.js:963-1090) with every time step on
+ * the device engine (hip.submodelCreate / submodelCall): `sm` is the
+ * reference's BumpSubmodel instance (its mesh, boundary, material tables and
+ * global fields are set up by the reference's constructor), `core` the core's
+ * {cycleHistory, AnandPoint, lifeEstimates, CONST}.  The step material data
+ * come from the instance's own _elastic and the boundary from its prescribe;
+ * the engine does stepTo, the commit, the cutback snapshot, the layer means,
+ * the hysteresis samples and the section snapshots.  Returns the reference's
+ * record.  Install with
+ *   BumpSubmodel.prototype.run = function () { return fh.runSubmodel(this, core); };
+ * (the homogenized-layer and elastic-solder check modes keep the JS run).
+ */
+function runSubmodel(sm, core) {
+  const hip = load();
+  const t0 = Date.now();
+  const m = sm.mesh, MAT = sm.MAT, g = sm.an.mesh.geom, C = core.CONST;
+  const nx = m.xs.length, ny = m.ys.length, nz = m.zs.length, ncell = (nx - 1) * (ny - 1) * (nz - 1);
+  const cellMat = new Int8Array(ncell), cellLayer = new Uint8Array(ncell);
+  const kind = [MAT.SUB, MAT.SOLDER, MAT.UF, MAT.SI];
+  for (let e = 0; e < m.ne; e++) {
+    const i = m.eIJK[3 * e], j = m.eIJK[3 * e + 1], k = m.eIJK[3 * e + 2], c = (i * (ny - 1) + j) * (nz - 1) + k;
+    cellMat[c] = kind.indexOf(m.eGroup[e]);
+    if (m.eGroup[e] === MAT.SOLDER) {
+      const zc = 0.5 * (m.zs[k] + m.zs[k + 1]);
+      cellLayer[c] = (zc > g.zaf - C.SUB_T_AVG ? 1 : 0) | (zc < g.zt + C.SUB_T_AVG ? 2 : 0);
+    }
+  }
+  const pt = new core.AnandPoint(sm.solderMat);
+  const h = hip.submodelCreate({ nx, ny, nz, xs: Float64Array.from(m.xs), ys: Float64Array.from(m.ys), zs: Float64Array.from(m.zs), cellMat, cellLayer },
+    { A: pt.A, QR: pt.QR, xi: pt.xi, m: pt.m, shat: pt.shat, n: pt.n, h0: pt.h0, a: pt.a, s0: pt.s0, nu: pt.nu });
+  try {
+    const nb = sm.bNodes.length, u = new Float64Array(sm.ndof);
+    let Tn = 25;
+    const stepTo = (T, dt) => {
+      const elasD = new Float64Array(144), elasSth = new Float64Array(24);
+      for (let q = 0; q < 4; q++) sm._elastic(kind[q], Tn, T, elasD.subarray(36 * q, 36 * q + 36), elasSth.subarray(6 * q, 6 * q + 6));
+      sm.prescribe(T, u);
+      const ub = new Float64Array(3 * nb);
+      for (let q = 0; q < nb; q++) { const n = sm.bNodes[q]; ub[3 * q] = u[3 * n]; ub[3 * q + 1] = u[3 * n + 1]; ub[3 * q + 2] = u[3 * n + 2]; }
+      const r = hip.submodelCall(h, 'step', { T, dt, E: pt.E(T), fsolder: pt.thermalStrain(T) - pt.thermalStrain(25), elasD, elasSth, ub });
+      if (r.converged) Tn = T;
+      return r;
+    };
+    const hist = core.cycleHistory(sm.cfg.cycling), nsteps = hist.t.length, nc = hist.cycles;
+    const phi = Math.atan2(sm.site.y - sm.die.cy, sm.site.x - sm.die.cx), cphi = Math.cos(phi), sphi = Math.sin(phi);
+    const loop = { t: [], T: [], tau: [], gamma: [], tauSub: [], gammaSub: [] };
+    const Wd = new Float64Array(nc + 1), Ws = new Float64Array(nc + 1), Ed = new Float64Array(nc + 1), Es = new Float64Array(nc + 1);
+    const sample = (T, t) => {
+      const s = hip.submodelCall(h, 'loopSample', T, pt.E(T), pt.thermalStrain(T) - pt.thermalStrain(25), cphi, sphi);
+      loop.t.push(t); loop.T.push(T); loop.tau.push(s[0]); loop.gamma.push(s[1]); loop.tauSub.push(s[2]); loop.gammaSub.push(s[3]);
+    };
+    let totalNewton = 0, cutbacks = 0, steps = 0, nsubNext = 1;
+    const snaps = [], snapOn = !sm.opts.uniformMaterial;
+    sample(25, 0);
+    for (let k = 1; k < nsteps; k++) {
+      if (sm.opts.cancelled && sm.opts.cancelled()) throw new Error('cancelled');
+      const T0 = hist.T[k - 1], T1 = hist.T[k], t0s = hist.t[k - 1], t1s = hist.t[k];
+      let nsub = nsubNext, done = false, dLast = 0;
+      for (let attempt = 0; attempt < 8 && !done; attempt++) {
+        hip.submodelCall(h, 'save');
+        const TnSave = Tn;
+        let ok = true;
+        for (let s = 1; s <= nsub; s++) {
+          const res = stepTo(T0 + (T1 - T0) * s / nsub, (t1s - t0s) / nsub);
+          totalNewton += res.newton; steps++;
+          dLast = res.dMax;
+          if (!res.converged || res.dMax > C.INELASTIC_STEP_MAX) { ok = false; break; }
+        }
+        if (ok) done = true;
+        else { hip.submodelCall(h, 'restore'); Tn = TnSave; nsub *= 2; cutbacks++; }
+      }
+      nsubNext = dLast > 0.5 * C.INELASTIC_STEP_MAX ? Math.min(nsub * 2, 64) : (dLast < 0.2 * C.INELASTIC_STEP_MAX ? Math.max(1, nsub >> 1) : nsub);
+      if (!done) throw new Error('Submodel step failed to converge after cutbacks at T = ' + T1.toFixed(1) + ' °C');
+      sample(T1, t1s);
+      const c = hist.cycle[k];
+      if (snapOn && c === nc && (k === nsteps - 1 || hist.phase[k + 1] !== hist.phase[k]))
+        snaps.push({ T: T1, t: t1s, cycle: c, phase: hist.phase[k], sig: hip.submodelCall(h, 'section', pt.E(T1), pt.thermalStrain(T1) - pt.thermalStrain(25)) });
+      const lm = hip.submodelCall(h, 'layerMeans');
+      Wd[c] = lm[0]; Ed[c] = lm[1]; Ws[c] = lm[2]; Es[c] = lm[3];
+      if (sm.opts.progress) sm.opts.progress(k / (nsteps - 1), 'Submodel step ' + k + '/' + (nsteps - 1) + ' at ' + T1.toFixed(0) + ' °C');
+    }
+    const dWdie = Wd[nc] - Wd[nc - 1], dWsub = Ws[nc] - Ws[nc - 1], dEdie = Ed[nc] - Ed[nc - 1], dEsub = Es[nc] - Es[nc - 1];
+    const dW = Math.max(dWdie, dWsub), dE = dWdie >= dWsub ? dEdie : dEsub;
+    const prev = nc >= 2 ? Math.max(Wd[nc - 1] - Wd[nc - 2], Ws[nc - 1] - Ws[nc - 2]) : NaN;
+    const stabChange = isFinite(prev) && prev > 0 ? Math.abs(dW - prev) / prev : NaN;
+    const perCycle = [];
+    for (let c = 1; c <= nc; c++) perCycle.push({ cycle: c, dWdie: Wd[c] - Wd[c - 1], dWsub: Ws[c] - Ws[c - 1] });
+    const prof = sm.die.profile, dJoint = dWdie >= dWsub ? prof.rt * 2 : prof.rb * 2;
+    return {
+      die: sm.die.name, site: { x: sm.site.x, y: sm.site.y, ix: sm.site.ix, iy: sm.site.iy }, nxy: sm.opts.nxy, nz: sm.opts.nz, nElem: m.ne, ndof: sm.ndof,
+      dW, dWdie, dWsub, dE, dEdie, dEsub, critical: dWdie >= dWsub ? 'die side' : 'substrate side', stabChange, stabWarn: isFinite(stabChange) && stabChange > C.STABILIZATION_WARN,
+      perCycle, loop, life: core.lifeEstimates(dW, dE, sm.cfg.fatigue, dJoint), dJoint, steps, cutbacks, newtonPerStep: totalNewton / Math.max(steps, 1), ms: Date.now() - t0,
+      section: snapOn ? {
+        xs: Float64Array.from(m.xs), ys: Float64Array.from(m.ys), zs: Float64Array.from(m.zs), grp: Int8Array.from(m.eGroup), cx: sm.site.x, cy: sm.site.y, zt: g.zt, zaf: g.zaf,
+        dieCx: sm.die.cx, dieCy: sm.die.cy, radii: { top: prof.rt, bottom: prof.rb, max: prof.rm },
+        groups: [{ name: 'Substrate slice', material: 'top ' + Math.round(C.SUB_MARGIN * 1000) + ' µm, homogenized' }, { name: 'Solder', material: sm.solderMat.name },
+          { name: 'Underfill', material: sm.ufMat ? sm.ufMat.name : '' }, { name: 'Silicon', material: '' }],
+        snaps,
+      } : null,
+    };
+  } finally {
+    hip.submodelCall(h, 'destroy');
+  }
+}
+
 function install(Analysis, opts) {
   const original = Analysis.prototype.solveStep;
   Analysis.prototype.solveStep = function (k) {
@@ -195,4 +301,4 @@ function install(Analysis, opts) {
   return original;
 }
 
-module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, makeScreenJoints, stepFromGroups, install, STATE };
+module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, makeScreenJoints, runSubmodel, stepFromGroups, install, STATE };

@@ -13,6 +13,7 @@
 //   const f = hip.adjointForward(model, step, {kTop, vmElems, p}, opts);   // {J, u, ...}: saves the context
 //   const g = hip.adjointBackward(model, Float64Array.of(1, 0));          // {dlnE, dlnCte, dlnCteElem}
 //   const s = hip.screenJoints(solder, steps, table, i25);                 // {dW, dE, stab} per joint
+//   const h = hip.submodelCreate(mesh, solder); hip.submodelCall(h, 'step', data); ...  // bump submodel engine
 //   hip.destroyModel(model);                   // or left to the garbage collector
 // Argument and device errors throw Error(message); a solve that does not
 // converge returns its record with converged = false (the caller throws, as
@@ -38,10 +39,26 @@ namespace {
     }                                                          \
   } while (0)
 
+constexpr uint32_t MODEL_TAG = 0x4d4f444c, SUBMODEL_TAG = 0x5355424d;
+
 struct ModelBox {
+  uint32_t tag = MODEL_TAG;
   fcbga_model *m = nullptr;
   int32_t ngroups = 0, ne = 0;
 };
+
+struct SubBox {
+  uint32_t tag = SUBMODEL_TAG;
+  fcbga_submodel *s = nullptr;
+  int32_t nb = 0;
+  int64_t ne = 0, nn = 0;
+};
+
+void finalize_submodel(napi_env, void *data, void *) {
+  SubBox *b = static_cast<SubBox *>(data);
+  if (b->s) fcbga_submodel_destroy(b->s);
+  delete b;
+}
 
 void finalize_model(napi_env, void *data, void *) {
   ModelBox *b = static_cast<ModelBox *>(data);
@@ -157,8 +174,16 @@ napi_value CreateModel(napi_env env, napi_callback_info info) {
 
 ModelBox *unwrap(napi_env env, napi_value v) {
   void *data = nullptr;
-  if (napi_get_value_external(env, v, &data) != napi_ok) return nullptr;
-  return static_cast<ModelBox *>(data);
+  if (napi_get_value_external(env, v, &data) != napi_ok || !data) return nullptr;
+  ModelBox *b = static_cast<ModelBox *>(data);
+  return b->tag == MODEL_TAG ? b : nullptr;
+}
+
+SubBox *unwrap_sub(napi_env env, napi_value v) {
+  void *data = nullptr;
+  if (napi_get_value_external(env, v, &data) != napi_ok || !data) return nullptr;
+  SubBox *b = static_cast<SubBox *>(data);
+  return b->tag == SUBMODEL_TAG && b->s ? b : nullptr;
 }
 
 napi_value DestroyModel(napi_env env, napi_callback_info info) {
@@ -549,13 +574,159 @@ napi_value ScreenJoints(napi_env env, napi_callback_info info) {
   return rec;
 }
 
+// submodelCreate({nx, ny, nz, xs, ys, zs: Float64Array, cellMat: Int8Array, cellLayer: Uint8Array},
+//                solder {A, QR, xi, m, shat, n, h0, a, s0, nu}) -> handle (fcbga_submodel_create)
+napi_value SubmodelCreate(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) return throw_msg(env, "submodelCreate(mesh, solder)");
+  fcbga_submodel_mesh m;
+  memset(&m, 0, sizeof m);
+  if (!get_int(env, argv[0], "nx", &m.nx) || !get_int(env, argv[0], "ny", &m.ny) || !get_int(env, argv[0], "nz", &m.nz))
+    return throw_msg(env, "submodelCreate: mesh.nx, ny, nz required");
+  void *p;
+  size_t n;
+  const int64_t ncell = (int64_t)(m.nx - 1) * (m.ny - 1) * (m.nz - 1);
+  if (!get_typed(env, argv[0], "xs", napi_float64_array, &p, &n) || (int64_t)n != m.nx) return throw_msg(env, "submodelCreate: mesh.xs");
+  m.xs = (const double *)p;
+  if (!get_typed(env, argv[0], "ys", napi_float64_array, &p, &n) || (int64_t)n != m.ny) return throw_msg(env, "submodelCreate: mesh.ys");
+  m.ys = (const double *)p;
+  if (!get_typed(env, argv[0], "zs", napi_float64_array, &p, &n) || (int64_t)n != m.nz) return throw_msg(env, "submodelCreate: mesh.zs");
+  m.zs = (const double *)p;
+  if (!get_typed(env, argv[0], "cellMat", napi_int8_array, &p, &n) || (int64_t)n != ncell)
+    return throw_msg(env, "submodelCreate: mesh.cellMat: Int8Array(cells)");
+  m.cell_mat = (const int8_t *)p;
+  if (!get_typed(env, argv[0], "cellLayer", napi_uint8_array, &p, &n) || (int64_t)n != ncell)
+    return throw_msg(env, "submodelCreate: mesh.cellLayer: Uint8Array(cells)");
+  m.cell_layer = (const uint8_t *)p;
+  fcbga_anand_params P;
+  const char *names[10] = {"A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0", "nu"};
+  double *pv[10] = {&P.A, &P.QR, &P.xi, &P.m, &P.shat, &P.n, &P.h0, &P.a, &P.s0, &P.nu};
+  for (int q = 0; q < 10; q++)
+    if (!get_double(env, argv[1], names[q], pv[q])) return throw_msg(env, std::string("submodelCreate: solder.") + names[q]);
+  fcbga_submodel *sm = nullptr;
+  if (fcbga_submodel_create(&m, &P, &sm) != FCBGA_OK) return throw_msg(env, "submodelCreate: " + lib_error());
+  SubBox *box = new SubBox();
+  box->s = sm;
+  box->nb = fcbga_submodel_nboundary(sm);
+  box->ne = ncell;
+  box->nn = (int64_t)m.nx * m.ny * m.nz;
+  napi_value ext;
+  if (napi_create_external(env, box, finalize_submodel, nullptr, &ext) != napi_ok) {
+    fcbga_submodel_destroy(sm);
+    delete box;
+    return throw_msg(env, "submodelCreate: could not wrap the submodel");
+  }
+  return ext;
+}
+
+// submodelCall(handle, op, ...) for the step-level engine:
+//   'step', {T, dt, E, fsolder, elasD: Float64Array(144), elasSth: Float64Array(24), ub: Float64Array(3 nb)}
+//        -> {converged, newton, dMax, pcgIters, icBuilds}
+//   'save' | 'restore' | 'destroy' -> undefined
+//   'nboundary' -> number
+//   'layerMeans' -> Float64Array(4) {W die, ebar die, W sub, ebar sub}
+//   'loopSample', T, E, fsolder, cphi, sphi -> Float64Array(4) {tau, gamma die; tau, gamma sub}
+//   'section', E, fsolder -> Float32Array(48 per element)
+//   'displacement' -> Float64Array(3 per node)
+napi_value SubmodelCall(napi_env env, napi_callback_info info) {
+  size_t argc = 7;
+  napi_value argv[7];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) return throw_msg(env, "submodelCall(handle, op, ...)");
+  SubBox *b = unwrap_sub(env, argv[0]);
+  if (!b) return throw_msg(env, "submodelCall: not a live submodel");
+  char op[32];
+  size_t len;
+  if (napi_get_value_string_utf8(env, argv[1], op, sizeof op, &len) != napi_ok) return throw_msg(env, "submodelCall: op");
+  auto num = [&](size_t q, double *v) { return q < argc && napi_get_value_double(env, argv[q], v) == napi_ok; };
+  if (!strcmp(op, "step")) {
+    if (argc < 3) return throw_msg(env, "submodelCall(h, 'step', data)");
+    fcbga_submodel_step_in in;
+    memset(&in, 0, sizeof in);
+    if (!get_double(env, argv[2], "T", &in.T) || !get_double(env, argv[2], "dt", &in.dt) ||
+        !get_double(env, argv[2], "E", &in.E) || !get_double(env, argv[2], "fsolder", &in.fsolder))
+      return throw_msg(env, "submodel step: T, dt, E, fsolder required");
+    void *p;
+    size_t n;
+    if (!get_typed(env, argv[2], "elasD", napi_float64_array, &p, &n) || n != 144) return throw_msg(env, "submodel step: elasD: Float64Array(144)");
+    in.elas_D = (const double *)p;
+    if (!get_typed(env, argv[2], "elasSth", napi_float64_array, &p, &n) || n != 24) return throw_msg(env, "submodel step: elasSth: Float64Array(24)");
+    in.elas_sth = (const double *)p;
+    if (!get_typed(env, argv[2], "ub", napi_float64_array, &p, &n) || n != 3 * (size_t)b->nb)
+      return throw_msg(env, "submodel step: ub: Float64Array(3 * boundary nodes)");
+    in.ub = (const double *)p;
+    fcbga_submodel_step_out out;
+    if (fcbga_submodel_step(b->s, &in, &out) != FCBGA_OK) return throw_msg(env, "submodel step: " + lib_error());
+    napi_value rec, cv;
+    NAPI_OK(napi_create_object(env, &rec));
+    napi_get_boolean(env, out.converged != 0, &cv);
+    napi_set_named_property(env, rec, "converged", cv);
+    set_num(env, rec, "newton", out.newton);
+    set_num(env, rec, "dMax", out.dmax);
+    set_num(env, rec, "pcgIters", out.pcg_iters);
+    set_num(env, rec, "icBuilds", out.ic_builds);
+    return rec;
+  }
+  if (!strcmp(op, "save") || !strcmp(op, "restore")) {
+    const int rc = op[0] == 's' ? fcbga_submodel_save(b->s) : fcbga_submodel_restore(b->s);
+    if (rc != FCBGA_OK) return throw_msg(env, std::string("submodel ") + op + ": " + lib_error());
+    return nullptr;
+  }
+  if (!strcmp(op, "destroy")) {
+    fcbga_submodel_destroy(b->s);
+    b->s = nullptr;
+    return nullptr;
+  }
+  if (!strcmp(op, "nboundary")) {
+    napi_value v;
+    napi_create_int32(env, b->nb, &v);
+    return v;
+  }
+  if (!strcmp(op, "layerMeans") || !strcmp(op, "loopSample")) {
+    double *o;
+    napi_value arr = make_f64(env, 4, &o);
+    if (!arr) return throw_msg(env, "submodel: out of memory");
+    int rc;
+    if (op[0] == 'l' && op[1] == 'a') rc = fcbga_submodel_layer_means(b->s, o);
+    else {
+      double T, E, f, c, sn;
+      if (!num(2, &T) || !num(3, &E) || !num(4, &f) || !num(5, &c) || !num(6, &sn))
+        return throw_msg(env, "submodelCall(h, 'loopSample', T, E, fsolder, cphi, sphi)");
+      rc = fcbga_submodel_loop_sample(b->s, T, E, f, c, sn, o);
+    }
+    if (rc != FCBGA_OK) return throw_msg(env, std::string("submodel ") + op + ": " + lib_error());
+    return arr;
+  }
+  if (!strcmp(op, "section")) {
+    double E, f;
+    if (!num(2, &E) || !num(3, &f)) return throw_msg(env, "submodelCall(h, 'section', E, fsolder)");
+    napi_value ab, arr;
+    void *data = nullptr;
+    NAPI_OK(napi_create_arraybuffer(env, sizeof(float) * 48 * (size_t)b->ne, &data, &ab));
+    NAPI_OK(napi_create_typedarray(env, napi_float32_array, 48 * (size_t)b->ne, ab, 0, &arr));
+    if (fcbga_submodel_section(b->s, E, f, (float *)data) != FCBGA_OK) return throw_msg(env, "submodel section: " + lib_error());
+    return arr;
+  }
+  if (!strcmp(op, "displacement")) {
+    double *o;
+    napi_value arr = make_f64(env, 3 * (size_t)b->nn, &o);
+    if (!arr) return throw_msg(env, "submodel: out of memory");
+    if (fcbga_submodel_displacement(b->s, o) != FCBGA_OK) return throw_msg(env, "submodel displacement: " + lib_error());
+    return arr;
+  }
+  return throw_msg(env, std::string("submodelCall: unknown op ") + op);
+}
+
 }  // namespace
 
 static napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback cb; } fns[] = {
       {"createModel", CreateModel}, {"destroyModel", DestroyModel}, {"solveStep", SolveStep},
       {"stateAccumulate", StateAccumulate}, {"stateStress", StateStress},
-      {"adjointForward", AdjointForward}, {"adjointBackward", AdjointBackward}, {"screenJoints", ScreenJoints}};
+      {"adjointForward", AdjointForward}, {"adjointBackward", AdjointBackward}, {"screenJoints", ScreenJoints},
+      {"submodelCreate", SubmodelCreate}, {"submodelCall", SubmodelCall}};
   for (auto &f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||

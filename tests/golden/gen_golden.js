@@ -402,7 +402,10 @@ function submodelCases() {
   f64gz(u1, 'submodel_u1.f64.gz');
   const rec = { dW: r.dW, dWdie: r.dWdie, dWsub: r.dWsub, dE: r.dE, dEdie: r.dEdie, dEsub: r.dEsub, critical: r.critical,
     stabChange: r.stabChange, perCycle: r.perCycle, steps: r.steps, cutbacks: r.cutbacks, newtonPerStep: r.newtonPerStep,
-    dJoint: r.dJoint, life: r.life, loop: r.loop, ndof: r.ndof, nElem: r.nElem };
+    dJoint: r.dJoint, life: r.life, loop: r.loop, ndof: r.ndof, nElem: r.nElem,
+    snaps: r.section.snaps.map(q => ({ T: q.T, t: q.t, cycle: q.cycle, phase: q.phase })) };
+  const last = r.section.snaps[r.section.snaps.length - 1];
+  fs.writeFileSync(path.join(HERE, 'submodel_section.f32.gz'), zlib.gzipSync(Buffer.from(Float32Array.from(last.sig).buffer), { level: 9 }));
   fs.writeFileSync(path.join(HERE, 'submodel_golden.json'), JSON.stringify({ setup, first, result: rec }));
   console.log('submodel', r.ndof, 'DOF; dW', r.dW, 'steps', r.steps, 'cutbacks', r.cutbacks, 'newton/step', r.newtonPerStep.toFixed(2), 'ms', Date.now() - t0);
 }

@@ -129,4 +129,4 @@ def test_node_addon_loads_and_exports():
                          capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip() == ("adjointBackward,adjointForward,createModel,destroyModel,screenJoints,solveStep,"
-                                  "stateAccumulate,stateStress")
+                                  "stateAccumulate,stateStress,submodelCall,submodelCreate")

@@ -71,3 +71,34 @@ def test_gpu_submodel_two_cycles_match_reference():
         a, b = np.array(r["loop"][key]), np.array(ref["loop"][key], dtype=np.float64)
         assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max(), key
     assert math.isclose(r["life"]["syedEnergy"], ref["life"]["syedEnergy"], rel_tol=1e-6)
+    # section view: corner stresses at the ends of the last cycle's ramps and dwells
+    snaps = r["section"]["snaps"]
+    assert [(q["T"], q["t"], q["phase"]) for q in snaps] == [(q["T"], q["t"], q["phase"]) for q in ref["snaps"]]
+    import gzip
+    import os
+    from conftest import GOLDEN
+    with gzip.open(os.path.join(GOLDEN, "submodel_section.f32.gz")) as f:
+        sec = np.frombuffer(f.read(), np.float32)
+    assert np.abs(snaps[-1]["sig"] - sec).max() <= 1e-5 * np.abs(sec).max()
+
+
+@pytest.mark.gpu
+def test_node_addon_submodel_run():
+    """The JS host path of the submodel: node -> runSubmodel -> Node-API addon ->
+    C ABI -> device, with the step material data and the boundary from the
+    instance's own _elastic and prescribe (stand-ins here), against the
+    reference's record."""
+    import json
+    import os
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "js", "napi_submodel.js")
+    out = subprocess.run([node, script], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    assert (rec["steps"], rec["cutbacks"], rec["snaps"]) == (rec["ref_steps"], rec["ref_cutbacks"], rec["ref_snaps"]), rec
+    for k in ("dW_rel_err", "dE_rel_err", "dWsub_rel_err", "stab_rel_err"):
+        assert rec[k] <= 1e-6, rec
