@@ -260,9 +260,9 @@ class Model:
         return du, rec
 
     def solve_step_device(self, step: Step, du_dev_ptr: int, tol=1e-8, maxit=2000, precision=PREC_FP64,
-                          precond=PC_MG, sweeps=0):
+                          precond=PC_MG, sweeps=0, warm=0):
         st = FcbgaSolveStats()
-        o = self._opts(tol, maxit, precision, precond, sweeps)
+        o = self._opts(tol, maxit, precision, precond, sweeps, warm)
         rc = self.L.fcbga_step_solve_device(self.h, C.byref(step.c), C.byref(o), C.c_void_p(du_dev_ptr),
                                             C.byref(st))
         if rc != 0:

@@ -1,5 +1,5 @@
 // Golden vectors from the reference JS core (run in the build container only:
-//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|ic|screen|submodel|<spec name> ...]
+//   node tests/golden/gen_golden.js [/root/reference] [anand|element|stress|warm|ic|screen|submodel|path|<spec name> ...]
 // ). It loads src/core/*.js the way the reference's own tools/dev-core.mjs does,
 // builds each committed mesh spec (*_spec.json) with the reference mesher's
 // buildStructuredMesh, assembles one thermal step and solves it with the
@@ -14,6 +14,7 @@
 //   fallback_golden.json                (the fallback to the other preconditioner)
 //   screen_golden.json, screen_<case>_table.f64.gz (bump screening on synthetic strain tables, cycle histories, site subsampling)
 //   submodel_golden.json, submodel_*.f64.gz (a bump submodel of the bare-die preset: set-up data and result)
+//   path_golden.json                    (thermal path plans, evaluation temperatures and step sums)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -410,6 +411,58 @@ function submodelCases() {
   console.log('submodel', r.ndof, 'DOF; dW', r.dW, 'steps', r.steps, 'cutbacks', r.cutbacks, 'newton/step', r.newtonPerStep.toFixed(2), 'ms', Date.now() - t0);
 }
 
+// Thermal path planning and summation (temperatureSamples, thermalPathPoints,
+// Analysis.evaluationTemperatures / buildPath / finishPath: src/core/11-analysis.js:29-261)
+// on configuration records of several kinds (bare, lidded, stacked modules; full and preview
+// evaluations; Tg windows): the reference's own methods called on plain objects that hold
+// the fields they read.
+function pathCases() {
+  const temperatureSamples = get('temperatureSamples'), thermalPathPoints = get('thermalPathPoints'), Analysis = get('Analysis');
+  const samples = [
+    [-40, 125, [], [25]], [-55, 150, [120], [25]], [-40, 125, [105, 160], [25, 0]], [0, 100, [50.5], []],
+  ].map(([a, b, tgs, ex]) => ({ args: [a, b, tgs, ex], out: temperatureSamples(a, b, tgs, ex) }));
+  const points = [
+    [217, 165, [], []], [165, 25, [120], [150, 217]], [25, -40, [], [-40, -30, 0, 10]], [25, 260, [160, 120], [100, 217, 218, 250]],
+    [150, 150, [], []], [165, 25, [30], [25.5]],
+  ].map(([a, b, tgs, st]) => ({ args: [a, b, tgs, st], out: thermalPathPoints(a, b, tgs, st) }));
+  const plans = [];
+  const cfgs = [
+    { name: 'bare', packageType: 'bare', mods: false, tgs: [], stageTgs: { 0: [], 1: [], 2: [], 3: [] }, evaluations: 'full' },
+    { name: 'lid', packageType: 'lid', mods: false, tgs: [160, 120], stageTgs: { 0: [160], 1: [160], 2: [160, 120], 3: [160, 120] }, evaluations: 'full' },
+    { name: 'lid_sens', packageType: 'lid', mods: false, tgs: [160, 120], stageTgs: { 0: [160], 1: [160], 2: [160, 120], 3: [160, 120] }, evaluations: 'sensitivity' },
+    { name: 'preview', packageType: 'stiffener', mods: false, tgs: [140], stageTgs: { 0: [140], 1: [140], 2: [140], 3: [140] }, evaluations: 'preview' },
+    { name: 'modules_wcuf', packageType: 'lid', mods: 'wcuf', tgs: [150], stageTgs: { '-3': [], '-2': [150], '-1': [150], 0: [150], 1: [150], 2: [150], 3: [150] }, evaluations: 'full' },
+    { name: 'modules_mold', packageType: 'bare', mods: 'mold', tgs: [], stageTgs: { '-3': [], '-2': [], '-1': [], 0: [], 1: [], 2: [], 3: [] }, evaluations: 'full' },
+  ];
+  for (const c of cfgs) {
+    const cfg = {
+      packageType: c.packageType, process: { Tsub: 217, Tjoin: 217, Tuf: 165, Tattach: 150 },
+      reflow: { peak: 260, reportTemps: [25, 150, 220, 260] }, cycling: { Tmin: -40, Tmax: 125 },
+      stack: { fill: c.mods === 'wcuf' ? 'wcuf' : 'mold', process: { Tddi: 240, Twcuf: 150, Tmold: 175 } },
+    };
+    const an = Object.create(Analysis.prototype);
+    an.cfg = cfg; an.opts = { evaluations: c.evaluations };
+    an.finalStage = c.packageType === 'bare' ? 2 : 3;
+    an.hasModules = !!c.mods;
+    an.tgs = c.tgs;
+    an.stageTgs = new Map(Object.entries(c.stageTgs).map(([k, v]) => [Number(k), v]));
+    an.solidi = c.mods ? [217, 221] : [217];
+    const steps = an.buildPath();
+    // summation with synthetic increments: du_k = (k + 1) * [1, -0.5, 0.25] + 1e-3 * [0, k, k * k]
+    an.model = { ndof: 3 };
+    an.du = steps.map((s, k) => Float64Array.from([(k + 1), -0.5 * (k + 1) + 1e-3 * k, 0.25 * (k + 1) + 1e-3 * k * k]));
+    an.timings = {}; an.progress = () => {};
+    an.finishPath();
+    const fields = {};
+    for (const [T, u] of an.fields) fields[T] = list(u);
+    plans.push({ name: c.name, cfg, evaluations: c.evaluations, finalStage: an.finalStage, hasModules: an.hasModules, tgs: c.tgs, stageTgs: c.stageTgs,
+      solidi: an.solidi, temps: an.temps, steps, path: { chain: an.path.chain, chainJoin: an.path.chainJoin, cpi: an.path.cpi, up: an.path.up, down: an.path.down,
+        ddiJoin: an.path.ddiJoin, ddi: an.path.ddi }, fields, uCPI: an.uCPI ? list(an.uCPI) : null, uDDI: an.uDDI ? list(an.uDDI) : null });
+    console.log('path', c.name, 'steps', steps.length, 'temps', an.temps.all.length);
+  }
+  fs.writeFileSync(path.join(HERE, 'path_golden.json'), JSON.stringify({ samples, points, plans }));
+}
+
 const what = process.argv.slice(3);
 if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }
 if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
@@ -417,5 +470,6 @@ if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(
 if (what.includes('anand')) anandCases();
 if (what.includes('screen')) screenCases();
 if (what.includes('submodel')) submodelCases();
+if (what.includes('path')) pathCases();
 if (!what.length || what.includes('element')) elementCases();
 for (const n of ['small_box', 'plate_16', 'cfg1', 'cfg2']) if (!what.length || what.includes(n)) solveSpec(n);
