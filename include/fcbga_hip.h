@@ -273,6 +273,18 @@ int fcbga_submodel_section(fcbga_submodel *s, double E, double fsolder, float *o
 /* displacement (3 per grid node) */
 int fcbga_submodel_displacement(fcbga_submodel *s, double *u);
 
+/* Thermal path on the device (Analysis.run / finishPath, src/core/11-analysis.js:212-261): the
+ * increment of each solved step stays in device memory (FP64); fields are sums of increments.
+ * fcbga_path_solve solves step k of the plan (as fcbga_step_solve, opts->warm carrying the
+ * warm-start basis along the path) and keeps its increment; fcbga_path_sum writes the sum of the
+ * increments of steps ids[0..n-1], added in that order (3*nn doubles, host); fcbga_path_du the
+ * FP32 increment of step k, the form the reference keeps for the stress histories. */
+int fcbga_path_reset(fcbga_model *m);
+int fcbga_path_solve(fcbga_model *m, int32_t k, const fcbga_step *step, const fcbga_solve_opts *opts,
+                     fcbga_solve_stats *stats);
+int fcbga_path_sum(fcbga_model *m, int32_t n, const int32_t *ids, double *out);
+int fcbga_path_du(fcbga_model *m, int32_t k, float *out);
+
 /* Measurement hook: with timing enabled, launches of the kernel classes below are
  * bracketed by HIP events on the library's stream and their device time is summed
  * per class (reset by every fcbga_set_kernel_timing call).  The reference has no

@@ -240,6 +240,9 @@ struct fcbga_model {
   AdjTape tape;
   // warm start: orthonormal previous solutions (grid layout, unscaled), oldest first
   std::vector<std::unique_ptr<DBuf<double>>> basis;
+  // thermal path: the increment of every solved step (FP64, reference node order)
+  std::vector<std::unique_ptr<DBuf<double>>> pathDu;
+  DBuf<double> pathSum;
   DBuf<double> warmTmp, warmKv;
   DBuf<uint8_t> slotHasF, slotLayered, typeHasF, d_born;
   DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64;
@@ -3379,6 +3382,69 @@ int fcbga_step_solve_device(fcbga_model *m, const fcbga_step *step, const fcbga_
                             fcbga_solve_stats *stats) {
   if (!du_dev) { g_err = "du_dev is NULL"; return FCBGA_ERR_ARG; }
   return solve_common(m, step, opts, du_dev, nullptr, stats);
+}
+
+int fcbga_path_reset(fcbga_model *m) {
+  if (!m) { g_err = "model is NULL"; return FCBGA_ERR_ARG; }
+  m->pathDu.clear();
+  return FCBGA_OK;
+}
+
+int fcbga_path_solve(fcbga_model *m, int32_t k, const fcbga_step *step, const fcbga_solve_opts *opts,
+                     fcbga_solve_stats *stats) {
+  if (!m || k < 0) { g_err = "model is NULL or k < 0"; return FCBGA_ERR_ARG; }
+  try {
+    if ((size_t)k >= m->pathDu.size()) m->pathDu.resize(k + 1);
+    if (!m->pathDu[k]) m->pathDu[k] = std::make_unique<DBuf<double>>();
+    m->pathDu[k]->alloc(3 * (size_t)m->nn);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return solve_common(m, step, opts, m->pathDu[k]->p, nullptr, stats);
+}
+
+int fcbga_path_sum(fcbga_model *m, int32_t n, const int32_t *ids, double *out) {
+  if (!m || n < 0 || (n && !ids) || !out) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    for (int32_t q = 0; q < n; q++)
+      if (ids[q] < 0 || (size_t)ids[q] >= m->pathDu.size() || !m->pathDu[ids[q]])
+        throw be::Error("path step not solved");
+    const int64_t n3 = 3 * (int64_t)m->nn;
+    be::stream_t s = m->stream;
+    m->pathSum.alloc(n3);
+    double *acc = m->pathSum.p;
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { acc[i] = 0.0; });
+    // in list order, as the reference adds the increments
+    for (int32_t q = 0; q < n; q++) {
+      const double *d = m->pathDu[ids[q]]->p;
+      be::pfor(n3, s, [=] __host__ __device__(int64_t i) { acc[i] += d[i]; });
+    }
+    be::d2h(out, acc, sizeof(double) * n3, s);
+  } catch (const be::Error &e) {
+    return fail(e, FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_path_du(fcbga_model *m, int32_t k, float *out) {
+  if (!m || !out || k < 0 || (size_t)k >= m->pathDu.size() || !m->pathDu[k]) {
+    g_err = "path step not solved";
+    return FCBGA_ERR_ARG;
+  }
+  try {
+    const int64_t n3 = 3 * (int64_t)m->nn;
+    DBuf<float> f;
+    f.alloc(n3);
+    const double *d = m->pathDu[k]->p;
+    float *fp = f.p;
+    be::pfor(n3, m->stream, [=] __host__ __device__(int64_t i) { fp[i] = (float)d[i]; });
+    be::d2h(out, fp, sizeof(float) * n3, m->stream);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
 }
 
 int fcbga_adjoint_forward(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts,

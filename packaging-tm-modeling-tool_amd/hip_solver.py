@@ -81,7 +81,7 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_submodel_create", "fcbga_submodel_destroy", "fcbga_submodel_nboundary", "fcbga_submodel_step",
            "fcbga_submodel_save", "fcbga_submodel_restore", "fcbga_submodel_layer_means",
            "fcbga_submodel_loop_sample", "fcbga_submodel_section", "fcbga_submodel_displacement",
-           "fcbga_last_error"]
+           "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_last_error"]
 
 ANAND_STATE = 11
 STATE = 81  # stress-history state of one element (FCBGA_STATE)
@@ -150,6 +150,15 @@ def load(variant: str = "hip"):
     L.fcbga_screen_joints.argtypes = [C.POINTER(FcbgaAnandParams), C.POINTER(FcbgaScreenSteps), C.c_int64,
                                       C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _f64p]
     L.fcbga_screen_joints.restype = C.c_int
+    L.fcbga_path_reset.argtypes = [C.c_void_p]
+    L.fcbga_path_reset.restype = C.c_int
+    L.fcbga_path_solve.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FcbgaStep), C.POINTER(FcbgaSolveOpts),
+                                   C.POINTER(FcbgaSolveStats)]
+    L.fcbga_path_solve.restype = C.c_int
+    L.fcbga_path_sum.argtypes = [C.c_void_p, C.c_int32, _i32p, _f64p]
+    L.fcbga_path_sum.restype = C.c_int
+    L.fcbga_path_du.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float)]
+    L.fcbga_path_du.restype = C.c_int
     L.fcbga_warm_reset.argtypes = [C.c_void_p]
     L.fcbga_warm_reset.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
@@ -268,6 +277,35 @@ class Model:
         if rc != 0:
             raise SolverError(f"fcbga_step_solve_device failed ({rc}): {last_error(self.L)}")
         return _record(st)
+
+    def path_reset(self):
+        self.L.fcbga_path_reset(self.h)
+
+    def path_solve(self, k, step: Step, tol=1e-8, maxit=2000, precision=PREC_MIXED, precond=PC_MG, warm=20):
+        """Solve step k of a thermal path, keeping its increment on the device."""
+        st = FcbgaSolveStats()
+        o = self._opts(tol, maxit, precision, precond, 0, warm)
+        rc = self.L.fcbga_path_solve(self.h, int(k), C.byref(step.c), C.byref(o), C.byref(st))
+        rec = _record(st)
+        if rc != 0 and rc != -3:
+            raise SolverError(f"fcbga_path_solve failed ({rc}): {last_error(self.L)}")
+        return rec
+
+    def path_sum(self, ids):
+        """Sum of the kept increments of steps ids, added in that order (host FP64)."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        out = np.zeros(self.ndof)
+        rc = self.L.fcbga_path_sum(self.h, len(ids), ids.ctypes.data_as(_i32p), out.ctypes.data_as(_f64p))
+        if rc != 0:
+            raise SolverError(last_error(self.L))
+        return out
+
+    def path_du(self, k):
+        out = np.zeros(self.ndof, np.float32)
+        rc = self.L.fcbga_path_du(self.h, int(k), out.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            raise SolverError(last_error(self.L))
+        return out
 
     def set_kernel_timing(self, enable: bool):
         self.L.fcbga_set_kernel_timing(self.h, 1 if enable else 0)

@@ -152,13 +152,12 @@ def build_path(cfg, tgs, stage_tgs, solidi, evaluations="full"):
     return steps, path, temps
 
 
-def finish_path(steps, path, du, temps_all, zeros=None, copy=None):
+def finish_path(steps, path, du, temps_all):
     """Fields at every evaluation temperature and the chip-join / chiplet-join
-    states from the step increments du[k] (Analysis.finishPath,
-    src/core/11-analysis.js:229-261).  zeros(n) / copy(v) allocate (numpy by
-    default; device arrays for a device path)."""
-    zeros = zeros or (lambda n: np.zeros(n))
-    copy = copy or (lambda v: v.copy())
+    states from host step increments du[k] (Analysis.finishPath,
+    src/core/11-analysis.js:229-261); run_path does the same on the device."""
+    zeros = np.zeros
+    copy = np.copy
     n = len(du[0])
     want = set(temps_all)
     u = zeros(n)
@@ -183,36 +182,30 @@ def finish_path(steps, path, du, temps_all, zeros=None, copy=None):
     return fields, u_cpi, u_ddi
 
 
-def run_path(model, steps, path, temps_all, step_of, tol=1e-8, precision=None, warm=20, device=True):
+def run_path(model, steps, path, temps_all, step_of, tol=1e-8, precision=None, warm=20):
     """Solve every step in plan order on the device with one warm-start basis
-    along the path (Analysis.run, src/core/11-analysis.js:212-216) and sum the
-    fields.  step_of(step) gives the hip_solver.Step of a plan step.  With
-    device=True the increments stay in device memory (torch tensors; each kept
-    in FP32 as the reference keeps them after the summation) and the fields
-    are summed there.  Returns (fields, u_cpi, u_ddi, records)."""
+    along the path (Analysis.run, src/core/11-analysis.js:212-216); the
+    increments stay in device memory and every field is summed there from the
+    increments that lead to it, added in the reference's order
+    (finishPath).  step_of(step) gives the hip_solver.Step of a plan step.
+    Returns (fields, u_cpi, u_ddi, records)."""
     from . import hip_solver as hs
     precision = hs.PREC_MIXED if precision is None else precision
-    recs, du = [], []
+    recs = []
     model.warm_reset()
-    if device:
-        import torch
-        for s in steps:
-            buf = torch.empty(model.ndof, dtype=torch.float64, device="cuda")
-            rec = model.solve_step_device(step_of(s), buf.data_ptr(), tol=tol, precision=precision, warm=warm)
-            if not rec["converged"]:
-                raise hs.SolverError(f"Linear solver did not converge (step {len(recs)}): relative residual "
-                                     f"{rec['relres']:.2e}")
-            recs.append(rec)
-            du.append(buf)
-        fields, u_cpi, u_ddi = finish_path(
-            steps, path, du, temps_all,
-            zeros=lambda n: torch.zeros(n, dtype=torch.float64, device="cuda"), copy=lambda v: v.clone())
-        for k in range(len(du)):
-            du[k] = du[k].float()
-        return fields, u_cpi, u_ddi, recs
-    for s in steps:
-        u, rec = model.solve_step(step_of(s), tol=tol, precision=precision, warm=warm)
+    model.path_reset()
+    for k, s in enumerate(steps):
+        rec = model.path_solve(k, step_of(s), tol=tol, precision=precision, warm=warm)
+        if not rec["converged"]:
+            raise hs.SolverError(f"Linear solver did not converge (step {k}: {s['Ta']} -> {s['Tb']} C): "
+                                 f"relative residual {rec['relres']:.2e}")
         recs.append(rec)
-        du.append(u)
-    fields, u_cpi, u_ddi = finish_path(steps, path, du, temps_all)
+    want = set(temps_all)
+    fields = {25: model.path_sum(path["chain"])}
+    for br in (path["up"], path["down"]):
+        for q, k in enumerate(br):
+            if steps[k]["Tb"] in want:
+                fields[steps[k]["Tb"]] = model.path_sum(path["chain"] + br[:q + 1])
+    u_cpi = model.path_sum(path["chainJoin"] + path["cpi"]) if path["hasCPI"] else None
+    u_ddi = model.path_sum(path["ddiJoin"] + path["ddi"]) if path["hasDDI"] else None
     return fields, u_cpi, u_ddi, recs

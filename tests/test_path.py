@@ -78,11 +78,13 @@ def test_path_run_sums_to_single_solves(variant):
     stress-free temperature to T (the reference's own consistency of the
     incremental path, cfg1's 'their sum equals one solve')."""
     spec, model, steps, path, temps, step_of = _plate_path(variant)
-    fields, _, _, recs = P.run_path(model, steps, path, temps["all"], step_of, tol=1e-12,
-                                    precision=hs.PREC_FP64, device=variant == "hip")
+    fields, _, _, recs = P.run_path(model, steps, path, temps["all"], step_of, tol=1e-12, precision=hs.PREC_FP64)
     assert all(r["converged"] for r in recs)
     assert recs[-1]["warm"] > 0 and min(r["r0"] for r in recs[1:]) < 1
     for T in (25, -40, 125, 260):
         ref, _ = model.solve_step(hs.step_for_spec(spec, 217, T, 0), tol=1e-13, precision=hs.PREC_FP64)
-        got = fields[T].cpu().numpy() if variant == "hip" else fields[T]
-        assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max(), T
+        assert np.abs(fields[T] - ref).max() <= 1e-9 * np.abs(ref).max(), T
+    # the FP32 increments the stress histories use, and the same sums on the host
+    du = [model.path_du(k).astype(np.float64) for k in range(len(steps))]
+    f32, _, _ = P.finish_path(steps, path, du, temps["all"])
+    assert np.abs(f32[125] - fields[125]).max() <= 1e-6 * np.abs(fields[125]).max()
