@@ -215,6 +215,26 @@ def path_timing(torch, hs, model, spec, tol):
                     "projection onto the stored solutions already meets the tolerance on most steps"}
 
 
+def submodel_timing():
+    """One bump submodel through two temperature cycles (BumpSubmodel.run):
+    the corner bump of the bare-die preset set up by the reference's own code
+    (8 x 8 x 12 B-bar elements, 3159 DOF, committed case data), 686 steps with
+    4 cutbacks.  The reference's run of the same case took 61.5 s in Node on
+    the build host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from submodel_case import build
+    g, m, _ = build("hip")
+    t0 = time.perf_counter()
+    r = m.run()
+    dt = time.perf_counter() - t0
+    m.close()
+    ref = g["result"]
+    return {"ndof": ref["ndof"], "steps": r["steps"], "cutbacks": r["cutbacks"], "s": dt,
+            "ms_per_step": 1e3 * dt / r["steps"], "newton_per_step": r["newtonPerStep"],
+            "dW_rel_err_vs_reference": abs(r["dW"] - ref["dW"]) / abs(ref["dW"]), "reference_s": 61.5,
+            "note": "Newton + IC(0)-PCG per step, the CG in one 1024-thread workgroup (latency-bound at this size)"}
+
+
 def adjoint_timing(torch, hs, cases, meshspec, tol, reps=3):
     """Sensitivity pair on the lidded mesh with 18 groups: one forward (solve +
     objectives) and one backward (adjoint solve + per-element contractions)
@@ -258,6 +278,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-path", action="store_true")
+    ap.add_argument("--no-submodel", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solve runs on a single MI355X and this build has no "
@@ -377,6 +398,11 @@ def main():
         out["path"] = path_timing(torch, hs, model, spec, args.tol)
         log(f"thermal path: {out['path']['steps']} steps in {out['path']['s']:.2f} s "
             f"({out['path']['iters_mean']:.1f} iterations per step, warm start)")
+
+    if not args.no_submodel:
+        out["submodel"] = submodel_timing()
+        log(f"submodel: {out['submodel']['steps']} steps in {out['submodel']['s']:.1f} s "
+            f"(dW vs reference {out['submodel']['dW_rel_err_vs_reference']:.1e})")
 
     from fcbga_amd import screening as scr
     out["screening"] = screening_timing(hs, scr)
