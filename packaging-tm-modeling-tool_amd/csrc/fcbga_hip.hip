@@ -245,8 +245,8 @@ struct fcbga_model {
   DBuf<double> pathSum;
   DBuf<double> warmTmp, warmKv;
   DBuf<uint8_t> slotHasF, slotLayered, typeHasF, d_born;
-  DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64;
-  DBuf<float> w32, sinv32, F32;
+  DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64, C64;
+  DBuf<float> w32, sinv32, C32;
   // solver vectors (grid layout, 3 per node)
   DBuf<double> x, r, z, zold, p, q, tmp64, dx64;
   DBuf<float> b32v, x32v, r32v, dx32;
@@ -625,20 +625,23 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
   }
 }
 
-// column factors of the fine level (FP64, FP32 copy); the scan coefficients of
-// the column solve are formed from them on the fly
+// column factors of the fine level (FP64) and the scan coefficients of the
+// column solve formed from them (FP64, FP32 copy)
 void factor_fine_columns(fcbga_model *M) {
   const fe::Grid g = M->g;
   const int64_t ncol = (int64_t)g.nx * g.ny;
+  constexpr int NC = fe::SCAN_COEFS;
   M->F64.alloc(18 * g.G);
-  M->F32.alloc(18 * g.G);
+  M->C64.alloc(NC * g.G);
+  M->C32.alloc(NC * g.G);
   const double *Db = M->Db.p, *Lb = M->Lb.p;
-  double *F = M->F64.p;
-  float *F32 = M->F32.p;
+  double *F = M->F64.p, *C = M->C64.p;
+  float *C32 = M->C32.p;
   const int nz = g.nz;
   be::pfor(ncol, M->stream, [=] __host__ __device__(int64_t c) {
     fe::column_factor(nz, Db + 9 * c * nz, Lb + 9 * c * nz, F + 18 * c * nz);
-    for (int q = 0; q < 18 * nz; q++) F32[18 * c * nz + q] = (float)F[18 * c * nz + q];
+    fe::column_scan_coefs(nz, F + 18 * c * nz, C + NC * c * nz);
+    for (int q = 0; q < NC * nz; q++) C32[NC * c * nz + q] = (float)C[NC * c * nz + q];
   }, 64);
 }
 
@@ -651,7 +654,7 @@ struct FineOps {
   fe::Grid g;
   const int32_t *cellType;
   const T *cellScale, *Kc, *w, *sinv;
-  const T *F;
+  const T *C;  // column scan coefficients, fe::SCAN_COEFS per node
   const int32_t *nodeAgg;
   const T *zp;
   const T *R;  // compact mirror-symmetric tables (72 per type) when sym
@@ -663,12 +666,12 @@ FineOps<T> fine_ops(fcbga_model *M);
 
 template <>
 FineOps<double> fine_ops<double>(fcbga_model *M) {
-  return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->F64.p,
+  return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->C64.p,
                          M->nodeAgg.p, M->zp64.p, M->R64.p, M->symTables};
 }
 template <>
 FineOps<float> fine_ops<float>(fcbga_model *M) {
-  return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->F32.p,
+  return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->C32.p,
                         M->nodeAgg.p, M->zp32.p, M->R32.p, M->symTables};
 }
 
@@ -753,35 +756,51 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
 // the lanes compute the residual rows and stage the column factor in LDS, one
 // lane runs the substitution chain, and the lanes apply the update.
 #if FCBGA_GPU
-template <typename T>
-__device__ __forceinline__ void scan_round_up(T A[9], T b[3], int s, int lane) {
+// One round of an inclusive scan of affine maps (A, b) at distance s: lane k
+// composes its map with that of lane k - s (up) or k + s (down).  Every lane
+// updates: a lane without a partner at distance s holds a composition that
+// includes the column end's zero map (P at the bottom node, Q at the top one,
+// zero coefficients past the column), so its A is exactly zero and the update
+// (with the finite values the shuffle returns) leaves its b unchanged.  The
+// maps' products are only needed by later rounds (needA false in the last).
+template <typename T, bool UP>
+__device__ __forceinline__ void scan_round(T A[9], T b[3], int s, bool needA) {
   T An[9], bn[3];
-  for (int q = 0; q < 9; q++) An[q] = __shfl_up(A[q], s, 64);
-  for (int p = 0; p < 3; p++) bn[p] = __shfl_up(b[p], s, 64);
-  if (lane >= s) {
-    T R[9], rb[3];
-    for (int i = 0; i < 3; i++) {
+  for (int p = 0; p < 3; p++) bn[p] = UP ? __shfl_up(b[p], s, 64) : __shfl_down(b[p], s, 64);
+  T rb[3];
+  for (int i = 0; i < 3; i++) rb[i] = A[3 * i] * bn[0] + A[3 * i + 1] * bn[1] + A[3 * i + 2] * bn[2] + b[i];
+  if (needA) {
+    for (int q = 0; q < 9; q++) An[q] = UP ? __shfl_up(A[q], s, 64) : __shfl_down(A[q], s, 64);
+    T R[9];
+    for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) R[3 * i + j] = A[3 * i] * An[j] + A[3 * i + 1] * An[3 + j] + A[3 * i + 2] * An[6 + j];
-      rb[i] = A[3 * i] * bn[0] + A[3 * i + 1] * bn[1] + A[3 * i + 2] * bn[2] + b[i];
-    }
     for (int q = 0; q < 9; q++) A[q] = R[q];
-    for (int p = 0; p < 3; p++) b[p] = rb[p];
   }
+  for (int p = 0; p < 3; p++) b[p] = rb[p];
 }
 
+// the SCAN_COEFS coefficients of one node (16-byte aligned records)
 template <typename T>
-__device__ __forceinline__ void scan_round_down(T A[9], T b[3], int s, int lane, int nz) {
-  T An[9], bn[3];
-  for (int q = 0; q < 9; q++) An[q] = __shfl_down(A[q], s, 64);
-  for (int p = 0; p < 3; p++) bn[p] = __shfl_down(b[p], s, 64);
-  if (lane + s < nz) {
-    T R[9], rb[3];
-    for (int i = 0; i < 3; i++) {
-      for (int j = 0; j < 3; j++) R[3 * i + j] = A[3 * i] * An[j] + A[3 * i + 1] * An[3 + j] + A[3 * i + 2] * An[6 + j];
-      rb[i] = A[3 * i] * bn[0] + A[3 * i + 1] * bn[1] + A[3 * i + 2] * bn[2] + b[i];
-    }
-    for (int q = 0; q < 9; q++) A[q] = R[q];
-    for (int p = 0; p < 3; p++) b[p] = rb[p];
+__device__ __forceinline__ void load_scan_coefs(const T *__restrict__ C, int gi, bool valid, T out[fe::SCAN_COEFS]);
+template <>
+__device__ __forceinline__ void load_scan_coefs<float>(const float *__restrict__ C, int gi, bool valid,
+                                                        float out[fe::SCAN_COEFS]) {
+  const float4 *p = reinterpret_cast<const float4 *>(C + (int64_t)fe::SCAN_COEFS * (valid ? gi : 0));
+#pragma unroll
+  for (int v = 0; v < fe::SCAN_COEFS / 4; v++) {
+    const float4 q = p[v];
+    out[4 * v] = valid ? q.x : 0.f; out[4 * v + 1] = valid ? q.y : 0.f;
+    out[4 * v + 2] = valid ? q.z : 0.f; out[4 * v + 3] = valid ? q.w : 0.f;
+  }
+}
+template <>
+__device__ __forceinline__ void load_scan_coefs<double>(const double *__restrict__ C, int gi, bool valid,
+                                                         double out[fe::SCAN_COEFS]) {
+  const double2 *p = reinterpret_cast<const double2 *>(C + (int64_t)fe::SCAN_COEFS * (valid ? gi : 0));
+#pragma unroll
+  for (int v = 0; v < fe::SCAN_COEFS / 2; v++) {
+    const double2 q = p[v];
+    out[2 * v] = valid ? q.x : 0.0; out[2 * v + 1] = valid ? q.y : 0.0;
   }
 }
 
@@ -800,7 +819,7 @@ template <typename T, bool SYM>
 __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t *__restrict__ cols, int ncols,
                                                        const int32_t *__restrict__ ct, const T *__restrict__ cs,
                                                        const T *__restrict__ Kc, const T *__restrict__ w,
-                                                       const T *__restrict__ F, const T *__restrict__ bvec,
+                                                       const T *__restrict__ C, const T *__restrict__ bvec,
                                                        T *__restrict__ x) {
   __shared__ T nb[9 * fe::NZP * 3];
   const int t = xcd_block(blockIdx.x, gridDim.x);
@@ -809,60 +828,55 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
   const int i = c / g.ny, j = c % g.ny;
   const int nz = g.nz;
   const int k = threadIdx.x;
-  const int gi = c * nz + k;
+  const bool valid = k < nz;
+  const int gi = c * nz + (valid ? k : 0);
+  // this node's own data first: their latency overlaps the staging below
+  T Ck[fe::SCAN_COEFS], wv[3], xv[3], bv[3];
+  load_scan_coefs<T>(C, gi, valid, Ck);
+  for (int p = 0; p < 3; p++) {
+    wv[p] = w[3 * gi + p];
+    xv[p] = x[3 * gi + p];
+    bv[p] = bvec[3 * gi + p];
+  }
   for (int q = threadIdx.x; q < 9 * 2 * 3; q += 64) {  // padding levels -1 and nz
     const int col = q / 6, side = (q / 3) % 2, p = q % 3;
     nb[(col * fe::NZP + (side ? nz + 1 : 0)) * 3 + p] = T(0);
   }
-  if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
+  if (valid) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
   __syncthreads();
   T r[3] = {0, 0, 0};
   T acc[3] = {0, 0, 0};
   if (SYM) {
     T up[3] = {0, 0, 0};
-    if (k < nz) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up);
+    if (valid) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up);
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);
       acc[p] += k > 0 ? u : T(0);
     }
-  } else if (k < nz) {
+  } else if (valid) {
     fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
   }
-  if (k < nz)
-    for (int p = 0; p < 3; p++) {
-      const T wi = w[3 * gi + p];
-      const T out = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
-      r[p] = bvec[3 * gi + p] - out;
-    }
-  // scan coefficients from this node's factor and the sub-diagonal block of
-  // the node above (lane k+1): Li = L_kk^-1, P = -Li L_{k,k-1}, Q = -Li^T L_{k+1,k}^T
-  T Fk[18];
-  for (int q = 0; q < 18; q++) Fk[q] = k < nz ? F[18 * gi + q] : T(0);
-  T Ld1[9];
-  for (int q = 0; q < 9; q++) Ld1[q] = __shfl_down(Fk[9 + q], 1, 64);
-  T Li[9], Q[9], A[9], b[3];
-  fe::inv_lower3<T>(Fk, Li);
-  const bool below = k > 0 && k < nz, above = k + 1 < nz;
-  for (int rr = 0; rr < 3; rr++)
-    for (int cc = 0; cc < 3; cc++) {
-      T v = 0, u = 0;
-      for (int q = 0; q < 3; q++) {
-        v -= Li[3 * rr + q] * Fk[9 + 3 * q + cc];
-        u -= Li[3 * q + rr] * Ld1[3 * cc + q];
-      }
-      A[3 * rr + cc] = below ? v : T(0);
-      Q[3 * rr + cc] = above ? u : T(0);
-    }
-  for (int p = 0; p < 3; p++) b[p] = k < nz ? Li[3 * p] * r[0] + Li[3 * p + 1] * r[1] + Li[3 * p + 2] * r[2] : T(0);
-  for (int s = 1; s < nz; s <<= 1) scan_round_up<T>(A, b, s, k);
+  if (valid)
+    for (int p = 0; p < 3; p++) r[p] = bv[p] - (wv[p] != T(0) ? wv[p] * acc[p] : xv[p]);
+  // forward recurrence y_k = P_k y_{k-1} + Li_k r_k, then the backward one
+  // dx_k = Q_k dx_{k+1} + Li_k^T y_k, each as a scan over the 64 lanes in six
+  // rounds (the coefficients of lanes past the column are zero; rounds at
+  // distances beyond nz change nothing)
+  const T *Li = Ck;
+  T A[9], b[3];
+  for (int q = 0; q < 9; q++) A[q] = Ck[9 + q];
+  for (int p = 0; p < 3; p++) b[p] = Li[3 * p] * r[0] + Li[3 * p + 1] * r[1] + Li[3 * p + 2] * r[2];
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) scan_round<T, true>(A, b, s, s < 32);
   {
     const T y0 = b[0], y1 = b[1], y2 = b[2];
-    for (int q = 0; q < 9; q++) A[q] = Q[q];
-    for (int p = 0; p < 3; p++) b[p] = k < nz ? Li[p] * y0 + Li[3 + p] * y1 + Li[6 + p] * y2 : T(0);
+    for (int q = 0; q < 9; q++) A[q] = Ck[18 + q];
+    for (int p = 0; p < 3; p++) b[p] = Li[p] * y0 + Li[3 + p] * y1 + Li[6 + p] * y2;
   }
-  for (int s = 1; s < nz; s <<= 1) scan_round_down<T>(A, b, s, k, nz);
-  if (k < nz)
-    for (int p = 0; p < 3; p++) x[3 * gi + p] += b[p];
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) scan_round<T, false>(A, b, s, s < 32);
+  if (valid)
+    for (int p = 0; p < 3; p++) x[3 * gi + p] = xv[p] + b[p];
 }
 
 // y = A x (mode 0) or y = b - A x (mode 1) over every node column, one 64-lane
@@ -910,7 +924,7 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
                       T *dx, be::stream_t s) {
   const fe::Grid g = o.g;
   const int32_t *ct = o.cellType;
-  const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w, *F = o.F;
+  const T *cs = o.cellScale, *Kc = o.Kc, *w = o.w, *C = o.C;
   const int nz = g.nz;
   if (!ncols) return;
   KScope ks(M, FCBGA_KCLASS_FINE_SWEEP);
@@ -918,10 +932,10 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   (void)r; (void)dx;
   if (o.sym)
     hipLaunchKernelGGL((k_fine_col_sweep<T, true>), dim3(8 * ((ncols + 7) / 8)), dim3(64), 0, s, g, cols, ncols, ct,
-                       cs, o.R, w, F, b, x);
+                       cs, o.R, w, C, b, x);
   else
     hipLaunchKernelGGL((k_fine_col_sweep<T, false>), dim3(8 * ((ncols + 7) / 8)), dim3(64), 0, s, g, cols, ncols, ct,
-                       cs, Kc, w, F, b, x);
+                       cs, Kc, w, C, b, x);
   BE_CHECK(hipGetLastError());
 #else
   const T *Rt = o.R;
@@ -951,7 +965,7 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   });
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t];
-    fe::column_solve_scan<T, T>(nz, F + 18 * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
+    fe::column_solve_scan<T, T>(nz, C + fe::SCAN_COEFS * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
     for (int q = 0; q < 3 * nz; q++) x[3 * c * nz + q] += dx[3 * c * nz + q];
   }, 64);
 #endif
@@ -1350,6 +1364,28 @@ void factor_shell_columns(fcbga_model *M, int l) {
   }, 64);
 }
 
+// acc -= sum over the entries k = k0, k0 + S, ... < k1 of row p of block k
+// (val, 36 per block) times x at block column col[k], in that order.  The
+// entries are taken four at a time: their column indices are loaded
+// together, then their blocks and x values, so that one pass costs two
+// memory latencies for four entries (an entry past k1 re-reads entry kb and
+// adds zero).
+template <typename T>
+FE_HD void row_sub_blocks(T &acc, int32_t k0, int32_t k1, int32_t S, int p, const int32_t *__restrict__ col,
+                          const T *__restrict__ val, const T *__restrict__ x) {
+  constexpr int U = 4;
+  for (int32_t kb = k0; kb < k1; kb += U * S) {
+    int32_t kc[U], cj[U];
+    for (int u = 0; u < U; u++) kc[u] = kb + u * S < k1 ? kb + u * S : kb;
+    for (int u = 0; u < U; u++) cj[u] = col[kc[u]];
+    for (int u = 0; u < U; u++) {
+      const T *v = val + 36 * (int64_t)kc[u] + 6 * p, *xj = x + 6 * (int64_t)cj[u];
+      const T d = v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
+      acc -= kb + u * S < k1 ? d : T(0);  // exact: a zero term changes nothing
+    }
+  }
+}
+
 #if FCBGA_GPU
 // inverse of a banded Cholesky factor, one 64-lane block per unit vector e_c;
 // the vector lives in LDS and every substitution step is a lane-parallel dot
@@ -1422,11 +1458,7 @@ __device__ __forceinline__ void shell_column_wave(int32_t c, int lane, const int
   T acc = 0;
   if (grp < S) {
     const int32_t n = n0 + i / 6;
-    const int p = i % 6;
-    for (int32_t k = rp[n] + grp; k < rp[n + 1]; k += S) {
-      const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
-      acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
-    }
+    row_sub_blocks<T>(acc, rp[n] + grp, rp[n + 1], S, i % 6, col, val, x);
   }
   T rr = bl;  // every lane ends with r_i of its row i
   for (int g = 0; g < S; g++) rr += __shfl(acc, g * m + i, 64);
@@ -1464,12 +1496,8 @@ __global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, cons
   const T *A = inv + cfp[c];
   for (int i = lane; i < m; i += 64) {
     const int32_t n = n0 + i / 6;
-    const int p = i % 6;
     T acc = b[6 * (int64_t)n0 + i];
-    for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
-      const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
-      acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
-    }
+    row_sub_blocks<T>(acc, rp[n], rp[n + 1], 1, i % 6, col, val, x);
     r[i] = acc;
   }
   __syncthreads();
@@ -1793,10 +1821,7 @@ void shell_residual(DevShell &D, be::stream_t s) {
     int64_t n = t / 6;
     int p = (int)(t % 6);
     T acc = b[t];
-    for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
-      const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
-      acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
-    }
+    row_sub_blocks<T>(acc, rp[n], rp[n + 1], 1, p, col, val, x);
     r[t] = acc;
   });
 }
@@ -1833,9 +1858,16 @@ void shell_cycle(fcbga_model *M, int l) {
       int64_t I = t / 6;
       int a = (int)(t % 6);
       T acc = 0;
-      for (int32_t h = chPtr[I]; h < chPtr[I + 1]; h++) {
-        const T *w = W + 36 * (int64_t)chQ[h] + a, *rf = res + 6 * (int64_t)chIdx[h];
-        acc += w[0] * rf[0] + w[6] * rf[1] + w[12] * rf[2] + w[18] * rf[3] + w[24] * rf[4] + w[30] * rf[5];
+      const int32_t h1 = chPtr[I + 1];
+      for (int32_t hb = chPtr[I]; hb < h1; hb += 4) {  // four children per pass, loads batched
+        int32_t hc[4], qv[4], fv[4];
+        for (int u = 0; u < 4; u++) hc[u] = hb + u < h1 ? hb + u : hb;
+        for (int u = 0; u < 4; u++) { qv[u] = chQ[hc[u]]; fv[u] = chIdx[hc[u]]; }
+        for (int u = 0; u < 4; u++) {
+          const T *w = W + 36 * (int64_t)qv[u] + a, *rf = res + 6 * (int64_t)fv[u];
+          const T d = w[0] * rf[0] + w[6] * rf[1] + w[12] * rf[2] + w[18] * rf[3] + w[24] * rf[4] + w[30] * rf[5];
+          acc += hb + u < h1 ? d : T(0);
+        }
       }
       bc[t] = acc;
     });
@@ -1848,9 +1880,16 @@ void shell_cycle(fcbga_model *M, int l) {
       int64_t f = t / 6;
       int p = (int)(t % 6);
       T acc = 0;
-      for (int32_t q = pPtr[f]; q < pPtr[f + 1]; q++) {
-        const T *w = W + 36 * (int64_t)q + 6 * p, *e = ec + 6 * (int64_t)pIdx[q];
-        acc += w[0] * e[0] + w[1] * e[1] + w[2] * e[2] + w[3] * e[3] + w[4] * e[4] + w[5] * e[5];
+      const int32_t q1 = pPtr[f + 1];
+      for (int32_t qb = pPtr[f]; qb < q1; qb += 4) {  // four coarse aggregates per pass, loads batched
+        int32_t qc[4], ev[4];
+        for (int u = 0; u < 4; u++) qc[u] = qb + u < q1 ? qb + u : qb;
+        for (int u = 0; u < 4; u++) ev[u] = pIdx[qc[u]];
+        for (int u = 0; u < 4; u++) {
+          const T *w = W + 36 * (int64_t)qc[u] + 6 * p, *e = ec + 6 * (int64_t)ev[u];
+          const T d = w[0] * e[0] + w[1] * e[1] + w[2] * e[2] + w[3] * e[3] + w[4] * e[4] + w[5] * e[5];
+          acc += qb + u < q1 ? d : T(0);
+        }
       }
       x[t] += acc;
     });

@@ -465,17 +465,29 @@ FE_HD void node_scan_coefs(const T *Fk, bool hasBelow, bool hasAbove, const T *L
     }
 }
 
-// Host form of the scan solve (the GPU kernel does the same rounds with lane
-// shuffles): x = (L L^T)^-1 r for one column from its factors F (18 per node), nz <= 128.
-template <typename T, typename TF>
-FE_HD void column_solve_scan(int nz, const TF *F, const T *r, T *x) {
-  T A[128][9], b[128][3], A2[128][9], b2[128][3], C[128][27];
+// Scan coefficients of every node of one column from its factor F (18 per
+// node), SCAN_COEFS per node: Li (9), P (9), Q (9) and one zero pad value (so
+// that a node's record is 16-byte aligned).  Formed once per factorisation in
+// FP64; the sweeps read them instead of inverting the diagonal factors.
+constexpr int SCAN_COEFS = 28;
+
+FE_HD void column_scan_coefs(int nz, const double *F, double *C) {
   for (int k = 0; k < nz; k++) {
-    T Fk[18], Ld1[9];
-    for (int q = 0; q < 18; q++) Fk[q] = (T)F[18 * k + q];
-    for (int q = 0; q < 9; q++) Ld1[q] = k + 1 < nz ? (T)F[18 * (k + 1) + 9 + q] : T(0);
-    node_scan_coefs<T>(Fk, k > 0, k + 1 < nz, Ld1, C[k]);
+    double Ld1[9];
+    for (int q = 0; q < 9; q++) Ld1[q] = k + 1 < nz ? F[18 * (k + 1) + 9 + q] : 0.0;
+    node_scan_coefs<double>(F + 18 * k, k > 0, k + 1 < nz, Ld1, C + SCAN_COEFS * k);
+    C[SCAN_COEFS * k + 27] = 0.0;
   }
+}
+
+// Host form of the scan solve (the GPU kernel does the same rounds with lane
+// shuffles): x = (L L^T)^-1 r for one column from its scan coefficients C
+// (SCAN_COEFS per node), nz <= 128.
+template <typename T, typename TF>
+FE_HD void column_solve_scan(int nz, const TF *Cc, const T *r, T *x) {
+  T A[128][9], b[128][3], A2[128][9], b2[128][3], C[128][27];
+  for (int k = 0; k < nz; k++)
+    for (int q = 0; q < 27; q++) C[k][q] = (T)Cc[SCAN_COEFS * k + q];
   // forward: maps y_k = P_k y_{k-1} + Li_k r_k
   for (int k = 0; k < nz; k++) {
     const T *o = C[k];
@@ -859,14 +871,26 @@ constexpr int NZP = 66;  // 64 planes + 2 padding levels
 template <typename T>
 FE_HD void fill_neighbourhood(const Grid &g, int i, int j, int k, const T *__restrict__ w, const T *__restrict__ x,
                               T *nb) {
-  for (int di = 0; di < 3; di++)
-    for (int dj = 0; dj < 3; dj++) {
-      const int ii = i + di - 1, jj = j + dj - 1;
-      const bool in = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny;
-      const int gn = in ? ((ii * g.ny + jj) * g.nz + k) * 3 : 0;
-      T *o = nb + ((di * 3 + dj) * NZP + k + 1) * 3;
-      for (int p = 0; p < 3; p++) o[p] = in ? w[gn + p] * x[gn + p] : T(0);
-    }
+  // every load is issued unconditionally (columns outside the grid read the
+  // centre column and are zeroed afterwards), so that their latencies overlap
+  T v[9][3];
+  bool in[9];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int q = 0; q < 9; q++) {
+    const int ii = i + q / 3 - 1, jj = j + q % 3 - 1;
+    in[q] = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny;
+    const int gn = ((in[q] ? ii * g.ny + jj : i * g.ny + j) * g.nz + k) * 3;
+    for (int p = 0; p < 3; p++) v[q][p] = w[gn + p] * x[gn + p];
+  }
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int q = 0; q < 9; q++) {
+    T *o = nb + (q * NZP + k + 1) * 3;
+    for (int p = 0; p < 3; p++) o[p] = in[q] ? v[q][p] : T(0);
+  }
 }
 
 template <typename T>
@@ -930,8 +954,13 @@ FE_HD void cell_sym_pair(const T *__restrict__ R, const T (&xb)[8][3], T sc, T o
       const int co = corner_of(corner_li(c) ^ MX, corner_lj(c) ^ MY, corner_lk(c));
       const int cu = corner_of(corner_li(c) ^ MX, corner_lj(c) ^ MY, corner_lk(c) ^ 1);
       const T r0 = R[24 * p + 3 * c], r1 = R[24 * p + 3 * c + 1], r2 = R[24 * p + 3 * c + 2];
-      zo += r0 * (sx * xb[co][0]) + r1 * (sy * xb[co][1]) + r2 * xb[co][2];
-      zu += r0 * (sx * xb[cu][0]) + r1 * (sy * xb[cu][1]) - r2 * xb[cu][2];
+      // one multiply-add per table entry (the sign flips are exact)
+      zo += r0 * (sx * xb[co][0]);
+      zo += r1 * (sy * xb[co][1]);
+      zo += r2 * xb[co][2];
+      zu += r0 * (sx * xb[cu][0]);
+      zu += r1 * (sy * xb[cu][1]);
+      zu -= r2 * xb[cu][2];
     }
     const T sp = p == 0 ? sx : p == 1 ? sy : T(1);
     own[p] += sc * sp * zo;
