@@ -224,7 +224,7 @@ struct fcbga_model {
   // device: per step
   DBuf<double> cellScale64, Kc64, fc64, slotM, slotLoad, tabs;
   DBuf<float> cellScale32, Kc32;
-  // corner-0 rows of the mirror-symmetric element tables (72 per type), used
+  // compact mirror-symmetric element tables (fe::SYM_TABLE per type), used
   // by the fine operator when every type passes the mirror check
   DBuf<double> R64, symErr;
   // stress recovery (built on first use): grid lines, node ids, cell of each element
@@ -532,16 +532,18 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
     }, 64);
   }
   // mirror-symmetric compact tables (all types must pass, else the full tables)
-  M->R64.alloc((size_t)M->ntypes * 72);
-  M->R32.alloc((size_t)M->ntypes * 72);
+  M->R64.alloc((size_t)M->ntypes * fe::SYM_TABLE);
+  M->R32.alloc((size_t)M->ntypes * fe::SYM_TABLE);
   M->symErr.alloc(M->ntypes);
   {
     const double *Kc = M->Kc64.p;
     double *R = M->R64.p, *err = M->symErr.p;
     float *R32 = M->R32.p;
     be::pfor(M->ntypes, s, [=] __host__ __device__(int64_t t) {
-      err[t] = fe::sym_table(Kc + 576 * t, R + 72 * t);
-      for (int q = 0; q < 72; q++) R32[72 * t + q] = (float)R[72 * t + q];
+      double Rf[72];
+      err[t] = fe::sym_table(Kc + 576 * t, Rf);
+      fe::sym_compact(Rf, R + fe::SYM_TABLE * t);
+      for (int q = 0; q < fe::SYM_TABLE; q++) R32[fe::SYM_TABLE * t + q] = (float)R[fe::SYM_TABLE * t + q];
     }, 64);
     std::vector<double> e(M->ntypes);
     be::d2h(e.data(), M->symErr.p, e.size() * sizeof(double), s);
@@ -657,7 +659,7 @@ struct FineOps {
   const T *C;  // column scan coefficients, fe::SCAN_COEFS per node
   const int32_t *nodeAgg;
   const T *zp;
-  const T *R;  // compact mirror-symmetric tables (72 per type) when sym
+  const T *R;  // compact mirror-symmetric tables (fe::SYM_TABLE per type) when sym
   bool sym;
 };
 
@@ -831,8 +833,9 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
   const bool valid = k < nz;
   const int gi = c * nz + (valid ? k : 0);
   // this node's own data first: their latency overlaps the staging below
+  // (the scan coefficients are loaded after the residual, which needs the
+  // registers)
   T Ck[fe::SCAN_COEFS], wv[3], xv[3], bv[3];
-  load_scan_coefs<T>(C, gi, valid, Ck);
   for (int p = 0; p < 3; p++) {
     wv[p] = w[3 * gi + p];
     xv[p] = x[3 * gi + p];
@@ -862,6 +865,7 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
   // dx_k = Q_k dx_{k+1} + Li_k^T y_k, each as a scan over the 64 lanes in six
   // rounds (the coefficients of lanes past the column are zero; rounds at
   // distances beyond nz change nothing)
+  load_scan_coefs<T>(C, gi, valid, Ck);
   const T *Li = Ck;
   T A[9], b[3];
   for (int q = 0; q < 9; q++) A[q] = Ck[9 + q];
@@ -1366,14 +1370,12 @@ void factor_shell_columns(fcbga_model *M, int l) {
 
 // acc -= sum over the entries k = k0, k0 + S, ... < k1 of row p of block k
 // (val, 36 per block) times x at block column col[k], in that order.  The
-// entries are taken four at a time: their column indices are loaded
-// together, then their blocks and x values, so that one pass costs two
-// memory latencies for four entries (an entry past k1 re-reads entry kb and
-// adds zero).
-template <typename T>
+// entries are taken U at a time: their column indices are loaded together,
+// then their blocks and x values, so that one pass costs two memory
+// latencies for U entries (an entry past k1 re-reads entry kb and adds zero).
+template <typename T, int U = 4>
 FE_HD void row_sub_blocks(T &acc, int32_t k0, int32_t k1, int32_t S, int p, const int32_t *__restrict__ col,
                           const T *__restrict__ val, const T *__restrict__ x) {
-  constexpr int U = 4;
   for (int32_t kb = k0; kb < k1; kb += U * S) {
     int32_t kc[U], cj[U];
     for (int u = 0; u < U; u++) kc[u] = kb + u * S < k1 ? kb + u * S : kb;
@@ -1436,7 +1438,7 @@ __global__ void __launch_bounds__(256) k_dense_matvec(int32_t n, const double *_
 // sums meet through lane shuffles (no LDS, no barrier).  The inverse is
 // stored by columns (read coalesced) and is loaded, with b, before the
 // residual so that their latency overlaps it.
-template <typename T>
+template <typename T, int U>
 __device__ __forceinline__ void shell_column_wave(int32_t c, int lane, const int32_t *__restrict__ cst,
                                                   const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                   const T *__restrict__ val, const T *__restrict__ inv,
@@ -1458,7 +1460,7 @@ __device__ __forceinline__ void shell_column_wave(int32_t c, int lane, const int
   T acc = 0;
   if (grp < S) {
     const int32_t n = n0 + i / 6;
-    row_sub_blocks<T>(acc, rp[n] + grp, rp[n + 1], S, i % 6, col, val, x);
+    row_sub_blocks<T, U>(acc, rp[n] + grp, rp[n + 1], S, i % 6, col, val, x);
   }
   T rr = bl;  // every lane ends with r_i of its row i
   for (int g = 0; g < S; g++) rr += __shfl(acc, g * m + i, 64);
@@ -1476,7 +1478,7 @@ __device__ __forceinline__ void shell_column_wave(int32_t c, int lane, const int
 
 // one colour of a shell-level column sweep, one 64-lane block per column;
 // colour (ci, cj) holds the columns (ci + 2u, cj + 2v), u-major
-template <typename T>
+template <typename T, int U>
 __global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, const int32_t *__restrict__ cst,
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                     const T *__restrict__ val, const T *__restrict__ inv,
@@ -1489,7 +1491,7 @@ __global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, cons
   const int m = 6 * (n1 - n0);
   const int lane = threadIdx.x;
   if (m <= 32) {
-    shell_column_wave<T>(c, lane, cst, rp, col, val, inv, cfp, b, x);
+    shell_column_wave<T, U>(c, lane, cst, rp, col, val, inv, cfp, b, x);
     return;
   }
   // larger columns: one lane per row
@@ -1782,8 +1784,14 @@ void shell_sweep(DevShell &D, int color, be::stream_t s) {
   const int32_t ncols = D.ncolor[color];
   if (!ncols) return;
 #if FCBGA_GPU
-  hipLaunchKernelGGL(k_shell_sweep<T>, dim3(ncols), dim3(64), 0, s, color >> 1, color & 1, D.ny, cst, rp, col, val,
-                     inv, cfp, b, x);
+  // entries in flight per lane: two where the colour's columns fill the chip
+  // (more registers would cut the waves resident per SIMD), four elsewhere
+  if (ncols > 4096)
+    hipLaunchKernelGGL((k_shell_sweep<T, 2>), dim3(ncols), dim3(64), 0, s, color >> 1, color & 1, D.ny, cst, rp, col,
+                       val, inv, cfp, b, x);
+  else
+    hipLaunchKernelGGL((k_shell_sweep<T, 4>), dim3(ncols), dim3(64), 0, s, color >> 1, color & 1, D.ny, cst, rp, col,
+                       val, inv, cfp, b, x);
   BE_CHECK(hipGetLastError());
 #else
   (void)s;

@@ -935,36 +935,53 @@ FE_HD void node_rows_nb(const Grid &g, int i, int j, int k, const int32_t *__res
 // images through its mid-planes, so with mirror mask m (bits x, y, z) and S_m
 // the matching sign flips of the displacement components,
 //   K_ab = S_a K_{0, a^b} S_a       (a^b: the corner with bits(a) xor bits(b)).
-// The corner-0 rows R_c = K_{0,c} (3 x 24, stored [p][3c+q] like the full
-// table) then carry the whole element.  cell_sym_pair forms, for one cell,
-// the rows of corner (MX, MY, 0) ("own") and of corner (MX, MY, 1) ("up")
-// applied to the cell's corner vectors xb, scaled by sc.
+// The corner-0 rows R_c = K_{0,c} (3 x 24) then carry the whole element, and
+// each 3 x 3 block R_c satisfies R_c[q][p] = s_c[p] s_c[q] R_c[p][q] (s_c the
+// sign flips of corner c), so its 6 entries on and above the diagonal carry
+// it: the compact table holds, per corner c, [R_c00, R_c11, R_c22, R_c01,
+// R_c02, R_c12] (SYM_TABLE = 48 values per element type).  cell_sym_pair
+// forms, for one cell, the rows of corner (MX, MY, 0) ("own") and of corner
+// (MX, MY, 1) ("up") applied to the cell's corner vectors xb, scaled by sc.
+constexpr int SYM_TABLE = 48;
+
 template <typename T, int MX, int MY>
 FE_HD void cell_sym_pair(const T *__restrict__ R, const T (&xb)[8][3], T sc, T own[3], T up[3]) {
   const T sx = MX ? T(-1) : T(1), sy = MY ? T(-1) : T(1);
+  // corner by corner (each corner's 6 table values serve the three rows);
+  // every row still sums its terms in corner order, one multiply-add per
+  // table entry (the sign flips are exact)
+  T zo[3] = {0, 0, 0}, zu[3] = {0, 0, 0};
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int c = 0; c < 8; c++) {
+    const int co = corner_of(corner_li(c) ^ MX, corner_lj(c) ^ MY, corner_lk(c));
+    const int cu = corner_of(corner_li(c) ^ MX, corner_lj(c) ^ MY, corner_lk(c) ^ 1);
+    const T *Rc = R + 6 * c;
+    const T s01 = (corner_li(c) ^ corner_lj(c)) ? T(-1) : T(1);
+    const T s02 = (corner_li(c) ^ corner_lk(c)) ? T(-1) : T(1);
+    const T s12 = (corner_lj(c) ^ corner_lk(c)) ? T(-1) : T(1);
+    const T d0 = Rc[0], d1 = Rc[1], d2 = Rc[2], o01 = Rc[3], o02 = Rc[4], o12 = Rc[5];
+    const T rows[3][3] = {{d0, o01, o02}, {s01 * o01, d1, o12}, {s02 * o02, s12 * o12, d2}};
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int p = 0; p < 3; p++) {
+      zo[p] += rows[p][0] * (sx * xb[co][0]);
+      zo[p] += rows[p][1] * (sy * xb[co][1]);
+      zo[p] += rows[p][2] * xb[co][2];
+      zu[p] += rows[p][0] * (sx * xb[cu][0]);
+      zu[p] += rows[p][1] * (sy * xb[cu][1]);
+      zu[p] -= rows[p][2] * xb[cu][2];
+    }
+  }
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
   for (int p = 0; p < 3; p++) {
-    T zo = 0, zu = 0;
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
-    for (int c = 0; c < 8; c++) {
-      const int co = corner_of(corner_li(c) ^ MX, corner_lj(c) ^ MY, corner_lk(c));
-      const int cu = corner_of(corner_li(c) ^ MX, corner_lj(c) ^ MY, corner_lk(c) ^ 1);
-      const T r0 = R[24 * p + 3 * c], r1 = R[24 * p + 3 * c + 1], r2 = R[24 * p + 3 * c + 2];
-      // one multiply-add per table entry (the sign flips are exact)
-      zo += r0 * (sx * xb[co][0]);
-      zo += r1 * (sy * xb[co][1]);
-      zo += r2 * xb[co][2];
-      zu += r0 * (sx * xb[cu][0]);
-      zu += r1 * (sy * xb[cu][1]);
-      zu -= r2 * xb[cu][2];
-    }
     const T sp = p == 0 ? sx : p == 1 ? sy : T(1);
-    own[p] += sc * sp * zo;
-    up[p] += sc * (p == 2 ? -sp : sp) * zu;
+    own[p] += sc * sp * zo[p];
+    up[p] += sc * (p == 2 ? -sp : sp) * zu[p];
   }
 }
 
@@ -990,7 +1007,7 @@ FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__rest
       const T *xv = nb + (col * NZP + k + corner_lk(b) + 1) * 3;
       xb[b][0] = xv[0]; xb[b][1] = xv[1]; xb[b][2] = xv[2];
     }
-    cell_sym_pair<T, 1 - CI, 1 - CJ>(R + 72 * t, xb, sc, own, up);
+    cell_sym_pair<T, 1 - CI, 1 - CJ>(R + SYM_TABLE * t, xb, sc, own, up);
   }
 }
 
@@ -1025,7 +1042,7 @@ FE_HD void cell_sym_direct(const Grid &g, int i, int j, int kk, bool wantUp, con
       for (int p = 0; p < 3; p++) xb[b][p] = w[3 * gn + p] * x[3 * gn + p];
     }
     T own[3] = {0, 0, 0}, up[3] = {0, 0, 0};
-    cell_sym_pair<T, 1 - CI, 1 - CJ>(R + 72 * t, xb, sc, own, up);
+    cell_sym_pair<T, 1 - CI, 1 - CJ>(R + SYM_TABLE * t, xb, sc, own, up);
     for (int p = 0; p < 3; p++) acc[p] += wantUp ? up[p] : own[p];
   }
 }
@@ -1080,6 +1097,15 @@ FE_HD double sym_table(const double *__restrict__ K, double *__restrict__ R) {
     }
   }
   return kmax > 0 ? err / kmax : 0.0;
+}
+
+// compact table (SYM_TABLE values) of the corner-0 rows R (72, [p][3c+q])
+FE_HD void sym_compact(const double *__restrict__ R, double *__restrict__ Rs) {
+  for (int c = 0; c < 8; c++) {
+    double *o = Rs + 6 * c;
+    o[0] = R[3 * c]; o[1] = R[24 + 3 * c + 1]; o[2] = R[48 + 3 * c + 2];
+    o[3] = R[3 * c + 1]; o[4] = R[3 * c + 2]; o[5] = R[24 + 3 * c + 2];
+  }
 }
 
 // ---------------------------------------------------------------------------
