@@ -688,11 +688,12 @@ template <typename T, int MODE>
 void launch_col_op(const FineOps<T> &o, const T *x, const T *b, T *y, be::stream_t s) {
   const fe::Grid g = o.g;
   const dim3 grid(8 * ((g.nx * g.ny + 7) / 8));
+  const size_t lds = (size_t)9 * (g.nz + 2) * 3 * sizeof(T);  // the 3 x 3 column neighbourhood
   if (o.sym)
-    hipLaunchKernelGGL((k_fine_col_op<T, MODE, true>), grid, dim3(64), 0, s, g, o.cellType, o.cellScale, o.R, o.w, x,
+    hipLaunchKernelGGL((k_fine_col_op<T, MODE, true>), grid, dim3(64), lds, s, g, o.cellType, o.cellScale, o.R, o.w, x,
                        b, y);
   else
-    hipLaunchKernelGGL((k_fine_col_op<T, MODE, false>), grid, dim3(64), 0, s, g, o.cellType, o.cellScale, o.Kc, o.w,
+    hipLaunchKernelGGL((k_fine_col_op<T, MODE, false>), grid, dim3(64), lds, s, g, o.cellType, o.cellScale, o.Kc, o.w,
                        x, b, y);
   BE_CHECK(hipGetLastError());
 }
@@ -823,7 +824,7 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
                                                        const T *__restrict__ Kc, const T *__restrict__ w,
                                                        const T *__restrict__ C, const T *__restrict__ bvec,
                                                        T *__restrict__ x) {
-  __shared__ T nb[9 * fe::NZP * 3];
+  __shared__ T nb[9 * fe::NZP * 3];  // fixed 64-plane stride
   const int t = xcd_block(blockIdx.x, gridDim.x);
   if (t >= ncols) return;
   const int c = cols[t];
@@ -845,19 +846,19 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
     const int col = q / 6, side = (q / 3) % 2, p = q % 3;
     nb[(col * fe::NZP + (side ? nz + 1 : 0)) * 3 + p] = T(0);
   }
-  if (valid) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
+  if (valid) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, fe::NZP);
   __syncthreads();
   T r[3] = {0, 0, 0};
   T acc[3] = {0, 0, 0};
   if (SYM) {
     T up[3] = {0, 0, 0};
-    if (valid) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up);
+    if (valid) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up, fe::NZP);
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);
       acc[p] += k > 0 ? u : T(0);
     }
   } else if (valid) {
-    fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+    fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc, fe::NZP);
   }
   if (valid)
     for (int p = 0; p < 3; p++) r[p] = bv[p] - (wv[p] != T(0) ? wv[p] * acc[p] : xv[p]);
@@ -890,7 +891,8 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
                                                     const T *__restrict__ cs, const T *__restrict__ Kc,
                                                     const T *__restrict__ w, const T *__restrict__ x,
                                                     const T *__restrict__ bvec, T *__restrict__ y) {
-  __shared__ T nb[9 * fe::NZP * 3];
+  extern __shared__ double nb_lds[];  // 9 * (nz + 2) * 3 values of T
+  T *nb = reinterpret_cast<T *>(nb_lds);
   const int c = xcd_block(blockIdx.x, gridDim.x);
   if (c >= g.nx * g.ny) return;
   const int i = c / g.ny, j = c % g.ny;
@@ -899,20 +901,20 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
   const int gi = c * nz + k;
   for (int q = threadIdx.x; q < 9 * 2 * 3; q += 64) {
     const int col = q / 6, side = (q / 3) % 2, p = q % 3;
-    nb[(col * fe::NZP + (side ? nz + 1 : 0)) * 3 + p] = T(0);
+    nb[(col * (nz + 2) + (side ? nz + 1 : 0)) * 3 + p] = T(0);
   }
-  if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
+  if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, nz + 2);
   __syncthreads();
   T acc[3] = {0, 0, 0};
   if (SYM) {
     T up[3] = {0, 0, 0};
-    if (k < nz) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up);
+    if (k < nz) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up, nz + 2);
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);
       acc[p] += k > 0 ? u : T(0);
     }
   } else if (k < nz) {
-    fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+    fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc, nz + 2);
   }
   if (k >= nz) return;
   for (int p = 0; p < 3; p++) {
@@ -934,11 +936,12 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   KScope ks(M, FCBGA_KCLASS_FINE_SWEEP);
 #if FCBGA_GPU
   (void)r; (void)dx;
+  const size_t lds = 0;
   if (o.sym)
-    hipLaunchKernelGGL((k_fine_col_sweep<T, true>), dim3(8 * ((ncols + 7) / 8)), dim3(64), 0, s, g, cols, ncols, ct,
+    hipLaunchKernelGGL((k_fine_col_sweep<T, true>), dim3(8 * ((ncols + 7) / 8)), dim3(64), lds, s, g, cols, ncols, ct,
                        cs, o.R, w, C, b, x);
   else
-    hipLaunchKernelGGL((k_fine_col_sweep<T, false>), dim3(8 * ((ncols + 7) / 8)), dim3(64), 0, s, g, cols, ncols, ct,
+    hipLaunchKernelGGL((k_fine_col_sweep<T, false>), dim3(8 * ((ncols + 7) / 8)), dim3(64), lds, s, g, cols, ncols, ct,
                        cs, Kc, w, C, b, x);
   BE_CHECK(hipGetLastError());
 #else
@@ -949,17 +952,17 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
     const int i = c / g.ny, j = c % g.ny;
     T nb[9 * fe::NZP * 3];
     for (int q = 0; q < 9 * fe::NZP * 3; q++) nb[q] = T(0);
-    for (int k = 0; k < nz; k++) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb);
+    for (int k = 0; k < nz; k++) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, fe::NZP);
     T upPrev[3] = {0, 0, 0};
     for (int k = 0; k < nz; k++) {
       const int gi = c * nz + k;
       T acc[3];
       if (symT) {
         T up[3];
-        fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Rt, nb, acc, up);
+        fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Rt, nb, acc, up, fe::NZP);
         for (int p = 0; p < 3; p++) { acc[p] += upPrev[p]; upPrev[p] = up[p]; }
       } else {
-        fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc);
+        fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc, fe::NZP);
       }
       for (int p = 0; p < 3; p++) {
         const T wi = w[3 * gi + p];

@@ -862,15 +862,15 @@ FE_HD void screen_site(const AnandParams &P, const ScreenSteps &h, int32_t i25, 
 // ---------------------------------------------------------------------------
 // Operator rows of one z-column of nodes from a staged neighbourhood.
 // nb holds w*x of the 3 x 3 node columns around column (i, j) at levels
-// -1 .. nz (zero padded), layout nb[(col * NZP + k + 1) * 3 + p] with
+// -1 .. nz (zero padded), layout nb[(col * nzp + k + 1) * 3 + p] (nzp >= nz + 2) with
 // col = 3 * (di + 1) + (dj + 1).  acc = sum_j K_ij (w x)_j for node k
 // (unscaled row; the caller applies w_i and the identity rows).
 // ---------------------------------------------------------------------------
-constexpr int NZP = 66;  // 64 planes + 2 padding levels
+constexpr int NZP = 66;  // largest stride: 64 planes + 2 padding levels
 
 template <typename T>
 FE_HD void fill_neighbourhood(const Grid &g, int i, int j, int k, const T *__restrict__ w, const T *__restrict__ x,
-                              T *nb) {
+                              T *nb, int nzp) {
   // every load is issued unconditionally (columns outside the grid read the
   // centre column and are zeroed afterwards), so that their latencies overlap
   T v[9][3];
@@ -888,14 +888,14 @@ FE_HD void fill_neighbourhood(const Grid &g, int i, int j, int k, const T *__res
 #pragma unroll
 #endif
   for (int q = 0; q < 9; q++) {
-    T *o = nb + (q * NZP + k + 1) * 3;
+    T *o = nb + (q * nzp + k + 1) * 3;
     for (int p = 0; p < 3; p++) o[p] = in[q] ? v[q][p] : T(0);
   }
 }
 
 template <typename T>
 FE_HD void node_rows_nb(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
-                        const T *__restrict__ cellScale, const T *__restrict__ Kc, const T *nb, T acc[3]) {
+                        const T *__restrict__ cellScale, const T *__restrict__ Kc, const T *nb, T acc[3], int nzp) {
   T a0 = 0, a1 = 0, a2 = 0;
   for (int ci = 0; ci < 2; ci++) {
     const int ii = i - 1 + ci;
@@ -916,7 +916,7 @@ FE_HD void node_rows_nb(const Grid &g, int i, int j, int k, const int32_t *__res
           T s0 = 0, s1 = 0, s2 = 0;
           for (int b = 0; b < 8; b++) {
             const int col = (ci + corner_li(b)) * 3 + (cj + corner_lj(b));
-            const T *xv = nb + (col * NZP + k + ck + corner_lk(b)) * 3;
+            const T *xv = nb + (col * nzp + k + ck + corner_lk(b)) * 3;
             s0 += K[3 * b] * xv[0] + K[3 * b + 1] * xv[1] + K[3 * b + 2] * xv[2];
             s1 += K[24 + 3 * b] * xv[0] + K[24 + 3 * b + 1] * xv[1] + K[24 + 3 * b + 2] * xv[2];
             s2 += K[48 + 3 * b] * xv[0] + K[48 + 3 * b + 1] * xv[1] + K[48 + 3 * b + 2] * xv[2];
@@ -990,7 +990,8 @@ FE_HD void cell_sym_pair(const T *__restrict__ R, const T (&xb)[8][3], T sc, T o
 // rows of node (i, j, k+1); the node's full row is own_k + up_{k-1}.
 template <typename T, int CI, int CJ>
 FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
-                       const T *__restrict__ cellScale, const T *__restrict__ R, const T *nb, T own[3], T up[3]) {
+                       const T *__restrict__ cellScale, const T *__restrict__ R, const T *nb, T own[3], T up[3],
+                       int nzp) {
   const int ii = i - 1 + CI, jj = j - 1 + CJ;
   if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy) return;
   const int c = (ii * g.ncy + jj) * g.ncz + k;
@@ -1004,7 +1005,7 @@ FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__rest
 #endif
     for (int b = 0; b < 8; b++) {
       const int col = (CI + corner_li(b)) * 3 + (CJ + corner_lj(b));
-      const T *xv = nb + (col * NZP + k + corner_lk(b) + 1) * 3;
+      const T *xv = nb + (col * nzp + k + corner_lk(b) + 1) * 3;
       xb[b][0] = xv[0]; xb[b][1] = xv[1]; xb[b][2] = xv[2];
     }
     cell_sym_pair<T, 1 - CI, 1 - CJ>(R + SYM_TABLE * t, xb, sc, own, up);
@@ -1014,13 +1015,13 @@ FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__rest
 template <typename T>
 FE_HD void node_rows_nb_sym(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
                             const T *__restrict__ cellScale, const T *__restrict__ R, const T *nb, T own[3],
-                            T up[3]) {
+                            T up[3], int nzp) {
   for (int p = 0; p < 3; p++) { own[p] = 0; up[p] = 0; }
   if (k >= g.ncz) return;
-  cell_sym_nb<T, 0, 0>(g, i, j, k, cellType, cellScale, R, nb, own, up);
-  cell_sym_nb<T, 0, 1>(g, i, j, k, cellType, cellScale, R, nb, own, up);
-  cell_sym_nb<T, 1, 0>(g, i, j, k, cellType, cellScale, R, nb, own, up);
-  cell_sym_nb<T, 1, 1>(g, i, j, k, cellType, cellScale, R, nb, own, up);
+  cell_sym_nb<T, 0, 0>(g, i, j, k, cellType, cellScale, R, nb, own, up, nzp);
+  cell_sym_nb<T, 0, 1>(g, i, j, k, cellType, cellScale, R, nb, own, up, nzp);
+  cell_sym_nb<T, 1, 0>(g, i, j, k, cellType, cellScale, R, nb, own, up, nzp);
+  cell_sym_nb<T, 1, 1>(g, i, j, k, cellType, cellScale, R, nb, own, up, nzp);
 }
 
 // Host form of one node row through the symmetric tables (same output rule as
