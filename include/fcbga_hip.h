@@ -284,6 +284,18 @@ int fcbga_path_solve(fcbga_model *m, int32_t k, const fcbga_step *step, const fc
                      fcbga_solve_stats *stats);
 int fcbga_path_sum(fcbga_model *m, int32_t n, const int32_t *ids, double *out);
 int fcbga_path_du(fcbga_model *m, int32_t k, float *out);
+/* Analysis.restorePath (src/core/11-analysis.js:220-229): step k's increment from a results file
+ * (3*nn FP32, as exportAnalysisState saves them, src/core/16-results-file.js:121-132) replaces its
+ * solve; fcbga_path_sum then sums the restored increments as finishPath does. */
+int fcbga_path_load(fcbga_model *m, int32_t k, const float *du);
+
+/* meshFingerprint (src/core/16-results-file.js:104-119): the 14-hex-digit hash a results file
+ * carries of the mesh its increments belong to, over the bytes of xs, ys, zs (FP64), eGroup
+ * (int32), ePart (int8) and eScale (FP64), seeded with nn and ne.  Host only; out receives 14
+ * lower-case hex digits and a NUL. */
+int fcbga_mesh_fingerprint(int32_t nn, int32_t ne, const double *xs, int32_t nx, const double *ys, int32_t ny,
+                           const double *zs, int32_t nz, const int32_t *e_group, const int8_t *e_part,
+                           const double *e_scale, char out[15]);
 
 /* Measurement hook: with timing enabled, launches of the kernel classes below are
  * bracketed by HIP events on the library's stream and their device time is summed

@@ -3497,6 +3497,57 @@ int fcbga_path_du(fcbga_model *m, int32_t k, float *out) {
   return FCBGA_OK;
 }
 
+int fcbga_path_load(fcbga_model *m, int32_t k, const float *du) {
+  if (!m || !du || k < 0) { g_err = "model or du is NULL, or k < 0"; return FCBGA_ERR_ARG; }
+  try {
+    if ((size_t)k >= m->pathDu.size()) m->pathDu.resize(k + 1);
+    if (!m->pathDu[k]) m->pathDu[k] = std::make_unique<DBuf<double>>();
+    const int64_t n3 = 3 * (int64_t)m->nn;
+    m->pathDu[k]->alloc(n3);
+    DBuf<float> f;
+    f.alloc(n3);
+    be::h2d(f.p, du, sizeof(float) * n3, m->stream);
+    const float *fp = f.p;
+    double *d = m->pathDu[k]->p;
+    // exact: every FP32 value is an FP64 value
+    be::pfor(n3, m->stream, [=] __host__ __device__(int64_t i) { d[i] = (double)fp[i]; });
+    be::sync(m->stream);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_mesh_fingerprint(int32_t nn, int32_t ne, const double *xs, int32_t nx, const double *ys, int32_t ny,
+                           const double *zs, int32_t nz, const int32_t *e_group, const int8_t *e_part,
+                           const double *e_scale, char out[15]) {
+  if (!out || nx < 0 || ny < 0 || nz < 0 || ne < 0 || (nx && !xs) || (ny && !ys) || (nz && !zs) ||
+      (ne && (!e_group || !e_part || !e_scale))) {
+    g_err = "NULL argument or negative size";
+    return FCBGA_ERR_ARG;
+  }
+  // two 32-bit multiplicative hashes over the bytes (Math.imul is the low 32 bits of the product)
+  uint32_t h1 = 0xdeadbeefu ^ (uint32_t)nn, h2 = 0x41c6ce57u ^ (uint32_t)ne;
+  const auto feed = [&](const void *p, size_t nbytes) {
+    const uint8_t *b = static_cast<const uint8_t *>(p);
+    for (size_t i = 0; i < nbytes; i++) {
+      h1 = (h1 ^ b[i]) * 2654435761u;
+      h2 = (h2 ^ b[i]) * 1597334677u;
+    }
+  };
+  feed(xs, sizeof(double) * (size_t)nx);
+  feed(ys, sizeof(double) * (size_t)ny);
+  feed(zs, sizeof(double) * (size_t)nz);
+  feed(e_group, sizeof(int32_t) * (size_t)ne);
+  feed(e_part, sizeof(int8_t) * (size_t)ne);
+  feed(e_scale, sizeof(double) * (size_t)ne);
+  h1 = ((h1 ^ (h1 >> 16)) * 2246822507u) ^ ((h2 ^ (h2 >> 13)) * 3266489909u);
+  h2 = ((h2 ^ (h2 >> 16)) * 2246822507u) ^ ((h1 ^ (h1 >> 13)) * 3266489909u);
+  const unsigned long long v = ((unsigned long long)(h2 & 0x1fffffu) << 32) | h1;
+  snprintf(out, 15, "%014llx", v);
+  return FCBGA_OK;
+}
+
 int fcbga_adjoint_forward(fcbga_model *m, const fcbga_step *step, const fcbga_solve_opts *opts,
                           const fcbga_objective *obj, double *J, double *du_out, fcbga_solve_stats *stats) {
   if (!m || !step || !obj || !J) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }

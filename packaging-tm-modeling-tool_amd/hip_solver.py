@@ -81,7 +81,8 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_submodel_create", "fcbga_submodel_destroy", "fcbga_submodel_nboundary", "fcbga_submodel_step",
            "fcbga_submodel_save", "fcbga_submodel_restore", "fcbga_submodel_layer_means",
            "fcbga_submodel_loop_sample", "fcbga_submodel_section", "fcbga_submodel_displacement",
-           "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_last_error"]
+           "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
+           "fcbga_mesh_fingerprint", "fcbga_last_error"]
 
 ANAND_STATE = 11
 STATE = 81  # stress-history state of one element (FCBGA_STATE)
@@ -159,6 +160,11 @@ def load(variant: str = "hip"):
     L.fcbga_path_sum.restype = C.c_int
     L.fcbga_path_du.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float)]
     L.fcbga_path_du.restype = C.c_int
+    L.fcbga_path_load.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float)]
+    L.fcbga_path_load.restype = C.c_int
+    L.fcbga_mesh_fingerprint.argtypes = [C.c_int32, C.c_int32, _f64p, C.c_int32, _f64p, C.c_int32, _f64p, C.c_int32,
+                                         _i32p, C.POINTER(C.c_int8), _f64p, C.c_char_p]
+    L.fcbga_mesh_fingerprint.restype = C.c_int
     L.fcbga_warm_reset.argtypes = [C.c_void_p]
     L.fcbga_warm_reset.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
@@ -301,11 +307,21 @@ class Model:
         return out
 
     def path_du(self, k):
+        """FP32 increment of step k (as the reference keeps it after finishPath)."""
         out = np.zeros(self.ndof, np.float32)
         rc = self.L.fcbga_path_du(self.h, int(k), out.ctypes.data_as(C.POINTER(C.c_float)))
         if rc != 0:
             raise SolverError(last_error(self.L))
         return out
+
+    def path_load(self, k, du):
+        """Restore step k's FP32 increment (Analysis.restorePath) in place of its solve."""
+        du = np.ascontiguousarray(du, np.float32)
+        if du.shape != (self.ndof,):
+            raise ValueError(f"increment of {du.shape} values for a model of {self.ndof} DOF")
+        rc = self.L.fcbga_path_load(self.h, int(k), du.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            raise SolverError(last_error(self.L))
 
     def set_kernel_timing(self, enable: bool):
         self.L.fcbga_set_kernel_timing(self.h, 1 if enable else 0)

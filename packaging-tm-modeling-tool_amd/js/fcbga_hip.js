@@ -57,6 +57,24 @@ class HipStepSolver {
   stateStress(elems, states, rows, xez) {
     return this.hip.stateStress(this.model, Int32Array.from(elems), states, Int32Array.from(rows), Float64Array.from(xez));
   }
+  /**
+   * The thermal path kept on the device (Analysis.run / finishPath / restorePath,
+   * src/core/11-analysis.js:212-261): pathSolve keeps step k's increment in device
+   * memory, pathSum adds kept increments in the given order (a field, or the FP64
+   * payload of a partial {k, du} message with ids = [k]), pathDu gives step k's
+   * increment in single precision (what exportAnalysisState saves) and pathLoad
+   * puts a saved one back in place of the solve.
+   */
+  pathSolve(k, step, label) {
+    const rec = this.hip.pathCall(this.model, 'solve', k, step, this.opts);
+    rec.label = label;
+    this.log.push({ label, iters: rec.iters, relres: rec.relres, converged: rec.converged, ms: rec.ms, pcMs: rec.pcMs, precond: rec.precond, fallback: rec.fallback });
+    return rec;
+  }
+  pathSum(ids) { return this.hip.pathCall(this.model, 'sum', Int32Array.from(ids)); }
+  pathDu(k) { return this.hip.pathCall(this.model, 'du', k); }
+  pathLoad(k, du) { this.hip.pathCall(this.model, 'load', k, du); }
+  pathReset() { this.hip.pathCall(this.model, 'reset'); }
   destroy() {
     if (this.model) this.hip.destroyModel(this.model);
     this.model = null;

@@ -26,6 +26,7 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/fcbga_hip.h"
 
@@ -257,6 +258,8 @@ void set_pc_fields(napi_env env, napi_value rec, const fcbga_solve_stats &S) {
   set_num(env, rec, "icShift", S.ic_shift);
 }
 
+napi_value solve_record(napi_env env, const fcbga_solve_stats &S);
+
 napi_value SolveStep(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
@@ -287,10 +290,17 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
   memset(&S, 0, sizeof S);
   int rc = fcbga_step_solve(b->m, &st, &o, (double *)udata, &S);
   if (rc != FCBGA_OK && rc != FCBGA_ERR_NOT_CONVERGED) return throw_msg(env, "solveStep: " + lib_error());
-  napi_value u, rec;
+  napi_value u;
   NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)ndof, ab, 0, &u));
-  NAPI_OK(napi_create_object(env, &rec));
+  napi_value rec = solve_record(env, S);
   napi_set_named_property(env, rec, "u", u);
+  return rec;
+}
+
+// the solve record of the reference (src/core/10-solver.js:988-993) without u
+napi_value solve_record(napi_env env, const fcbga_solve_stats &S) {
+  napi_value rec;
+  napi_create_object(env, &rec);
   set_num(env, rec, "iters", S.iters);
   set_num(env, rec, "relres", S.relres);
   napi_value cv;
@@ -721,12 +731,99 @@ napi_value SubmodelCall(napi_env env, napi_callback_info info) {
 
 }  // namespace
 
+// pathCall(model, op, ...): the thermal path kept on the device (Analysis.run / restorePath /
+// finishPath, src/core/11-analysis.js:212-261):
+//   'reset'                  drop every kept increment
+//   'solve', k, step[, opts] solve step k of the plan and keep its increment -> solve record
+//   'sum', ids               sum of the kept increments of steps ids, in that order -> Float64Array
+//   'du', k                  step k's increment in single precision -> Float32Array
+//   'load', k, du            restore step k's increment from a results file (Float32Array)
+napi_value PathCall(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) return throw_msg(env, "pathCall(model, op, ...)");
+  ModelBox *b = unwrap(env, argv[0]);
+  if (!b || !b->m) return throw_msg(env, "pathCall: not a live model");
+  char op[16] = {0};
+  size_t ol = 0;
+  if (napi_get_value_string_utf8(env, argv[1], op, sizeof op, &ol) != napi_ok) return throw_msg(env, "pathCall: op");
+  const std::string o(op);
+  const int64_t ndof = fcbga_model_ndof(b->m);
+  napi_value undef;
+  napi_get_undefined(env, &undef);
+  int32_t k = -1;
+  if (o != "reset" && o != "sum" && (argc < 3 || napi_get_value_int32(env, argv[2], &k) != napi_ok))
+    return throw_msg(env, "pathCall: step index");
+  if (o == "reset") {
+    if (fcbga_path_reset(b->m) != FCBGA_OK) return throw_msg(env, "pathCall reset: " + lib_error());
+    return undef;
+  }
+  if (o == "solve") {
+    if (argc < 4) return throw_msg(env, "pathCall(model, 'solve', k, step[, opts])");
+    fcbga_step st;
+    std::string err;
+    if (!read_step(env, argv[3], &st, &err)) return throw_msg(env, err);
+    fcbga_solve_opts so{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
+    if (argc >= 5 && !read_opts(env, argv[4], &so)) return throw_msg(env, "pathCall solve: unknown opts.precond");
+    fcbga_solve_stats S;
+    memset(&S, 0, sizeof S);
+    const int rc = fcbga_path_solve(b->m, k, &st, &so, &S);
+    if (rc != FCBGA_OK && rc != FCBGA_ERR_NOT_CONVERGED) return throw_msg(env, "pathCall solve: " + lib_error());
+    return solve_record(env, S);
+  }
+  if (o == "sum") {
+    if (argc < 3) return throw_msg(env, "pathCall(model, 'sum', ids)");
+    std::vector<int32_t> ids;
+    bool isArr = false;
+    napi_is_array(env, argv[2], &isArr);
+    void *p;
+    size_t n;
+    if (isArr) {
+      uint32_t len = 0;
+      napi_get_array_length(env, argv[2], &len);
+      ids.resize(len);
+      for (uint32_t q = 0; q < len; q++) {
+        napi_value e;
+        napi_get_element(env, argv[2], q, &e);
+        if (napi_get_value_int32(env, e, &ids[q]) != napi_ok) return throw_msg(env, "pathCall sum: ids");
+      }
+    } else if (arg_typed(env, argv[2], napi_int32_array, &p, &n)) {
+      ids.assign((const int32_t *)p, (const int32_t *)p + n);
+    } else {
+      return throw_msg(env, "pathCall sum: ids must be an Array or Int32Array");
+    }
+    double *out;
+    napi_value arr = make_f64(env, (size_t)ndof, &out);
+    if (fcbga_path_sum(b->m, (int32_t)ids.size(), ids.data(), out) != FCBGA_OK)
+      return throw_msg(env, "pathCall sum: " + lib_error());
+    return arr;
+  }
+  if (o == "du") {
+    napi_value ab, arr;
+    void *data = nullptr;
+    NAPI_OK(napi_create_arraybuffer(env, sizeof(float) * (size_t)ndof, &data, &ab));
+    NAPI_OK(napi_create_typedarray(env, napi_float32_array, (size_t)ndof, ab, 0, &arr));
+    if (fcbga_path_du(b->m, k, (float *)data) != FCBGA_OK) return throw_msg(env, "pathCall du: " + lib_error());
+    return arr;
+  }
+  if (o == "load") {
+    void *p;
+    size_t n;
+    if (argc < 4 || !arg_typed(env, argv[3], napi_float32_array, &p, &n) || (int64_t)n != ndof)
+      return throw_msg(env, "pathCall load: du must be a Float32Array of ndof values");
+    if (fcbga_path_load(b->m, k, (const float *)p) != FCBGA_OK) return throw_msg(env, "pathCall load: " + lib_error());
+    return undef;
+  }
+  return throw_msg(env, "pathCall: unknown op " + o);
+}
+
 static napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback cb; } fns[] = {
       {"createModel", CreateModel}, {"destroyModel", DestroyModel}, {"solveStep", SolveStep},
       {"stateAccumulate", StateAccumulate}, {"stateStress", StateStress},
       {"adjointForward", AdjointForward}, {"adjointBackward", AdjointBackward}, {"screenJoints", ScreenJoints},
-      {"submodelCreate", SubmodelCreate}, {"submodelCall", SubmodelCall}};
+      {"submodelCreate", SubmodelCreate}, {"submodelCall", SubmodelCall}, {"pathCall", PathCall}};
   for (auto &f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||

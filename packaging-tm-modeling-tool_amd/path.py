@@ -200,6 +200,14 @@ def run_path(model, steps, path, temps_all, step_of, tol=1e-8, precision=None, w
             raise hs.SolverError(f"Linear solver did not converge (step {k}: {s['Ta']} -> {s['Tb']} C): "
                                  f"relative residual {rec['relres']:.2e}")
         recs.append(rec)
+    fields, u_cpi, u_ddi = finish_path_device(model, steps, path, temps_all)
+    return fields, u_cpi, u_ddi, recs
+
+
+def finish_path_device(model, steps, path, temps_all):
+    """finishPath (src/core/11-analysis.js:236-255) over the increments kept on
+    the device: every field is the sum of the increments that lead to it, added
+    in the reference's order.  Returns (fields, u_cpi, u_ddi)."""
     want = set(temps_all)
     fields = {25: model.path_sum(path["chain"])}
     for br in (path["up"], path["down"]):
@@ -208,4 +216,4 @@ def run_path(model, steps, path, temps_all, step_of, tol=1e-8, precision=None, w
                 fields[steps[k]["Tb"]] = model.path_sum(path["chain"] + br[:q + 1])
     u_cpi = model.path_sum(path["chainJoin"] + path["cpi"]) if path["hasCPI"] else None
     u_ddi = model.path_sum(path["ddiJoin"] + path["ddi"]) if path["hasDDI"] else None
-    return fields, u_cpi, u_ddi, recs
+    return fields, u_cpi, u_ddi

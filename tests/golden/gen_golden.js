@@ -463,6 +463,69 @@ function pathCases() {
   fs.writeFileSync(path.join(HERE, 'path_golden.json'), JSON.stringify({ samples, points, plans }));
 }
 
+function resultsCases() {
+  const encodeResultsFile = get('encodeResultsFile'), joinParts = get('joinParts'), decodeResultsFile = get('decodeResultsFile');
+  const meshFingerprint = get('meshFingerprint'), resultsChunkSteps = get('resultsChunkSteps');
+  const exportAnalysisState = get('exportAnalysisState'), Analysis = get('Analysis');
+  const gz = (bytes, file) => fs.writeFileSync(path.join(HERE, file), zlib.gzipSync(Buffer.from(bytes), { level: 9 }));
+  // 1. a value tree with every kind of value the format carries
+  const shared = Float32Array.from([1.5, -2.25, 3e-8]);
+  const tree = {
+    cfg: { name: 'r\u00e9sum\u00e9 \u2713', packageType: 'lid', nested: [1, 0.1, -0, 1e21, 1.5e-7, 123456789012, 2 ** 53, -3.25e-300, 5e-324, 1e-6, 2.5e-7, 1e20],
+      flags: [true, false, null], b: 1, 2: 'two', a: 'x', 1: 'one', quote: 'a"b\\c\n\t\u0001' },
+    nums: [NaN, Infinity, -Infinity, 0],
+    fields: new Map([[25, Float64Array.from([0.1, 0.2])], [-40, Float64Array.from([1, 2, 3])]]),
+    temps: new Set([25, 150, 'x']),
+    arrays: [Float64Array.from([Math.PI]), shared, shared, Int32Array.from([-1, 2]), Uint32Array.from([4294967295]),
+      Int16Array.from([-32768]), Uint16Array.from([65535]), Int8Array.from([-128, 127]), Uint8Array.from([0, 255, 7])],
+    empty: new Float32Array(0),
+    skip: undefined, fn: () => 1,
+  };
+  let enc = encodeResultsFile(tree);
+  gz(new Uint8Array(joinParts(enc.parts, enc.size)), 'results_tree.fcbgares.gz');
+  // 2. mesh fingerprints of two golden meshes
+  const fingerprints = {};
+  for (const name of ['small_box', 'cfg1']) {
+    const spec = JSON.parse(fs.readFileSync(path.join(HERE, name + '_spec.json'), 'utf8'));
+    const groups = spec.materials.map(m => ({ name: m.name, kind: 'homog', mat: m.name, stage: m.stage || 0, part: m.part, column: !!m.column }));
+    const mesh = buildStructuredMesh(spec.xs, spec.ys, spec.zs, groups, assignFor(spec));
+    fingerprints[name] = { fingerprint: meshFingerprint(mesh), nn: mesh.nn, ne: mesh.ne };
+  }
+  // 3. an analysis state over the small_box mesh: the preview plan of a stiffener package and
+  //    synthetic single-precision increments du_k[i] = fround((k + 1) * 0.001 * ((i % 97) - 48))
+  const spec = JSON.parse(fs.readFileSync(path.join(HERE, 'small_box_spec.json'), 'utf8'));
+  const groups = spec.materials.map(m => ({ name: m.name, kind: 'homog', mat: m.name, stage: m.stage || 0, part: m.part, column: !!m.column }));
+  const mesh = buildStructuredMesh(spec.xs, spec.ys, spec.zs, groups, assignFor(spec));
+  const cfg = { packageType: 'stiffener', process: { Tsub: 217, Tjoin: 217, Tuf: 165, Tattach: 150 },
+    reflow: { peak: 260, reportTemps: [25, 150, 220, 260] }, cycling: { Tmin: -40, Tmax: 125 },
+    stack: { fill: 'mold', process: { Tddi: 240, Twcuf: 150, Tmold: 175 } } };
+  const an = Object.create(Analysis.prototype);
+  an.cfg = cfg; an.opts = { preset: 'draft', evaluations: 'preview', precond: 'mg', hmin: 0.05 };
+  an.finalStage = 3; an.hasModules = false; an.tgs = [140];
+  an.stageTgs = new Map([[0, [140]], [1, [140]], [2, [140]], [3, [140]]]);
+  an.solidi = [217];
+  an.mesh = mesh;
+  const ndof = 3 * mesh.nn;
+  an.model = { ndof };
+  const steps = an.buildPath();
+  an.du = steps.map((s, k) => { const d = new Float32Array(ndof); for (let i = 0; i < ndof; i++) d[i] = Math.fround((k + 1) * 0.001 * ((i % 97) - 48)); return d; });
+  const st = exportAnalysisState(an);
+  enc = encodeResultsFile(st);
+  const stBytes = new Uint8Array(joinParts(enc.parts, enc.size));
+  gz(stBytes, 'results_state.fcbgares.gz');
+  // restore: the decoded increments summed by finishPath
+  const back = decodeResultsFile(stBytes.buffer);
+  an.du = back.du; an.timings = {}; an.progress = () => {};
+  an.finishPath();
+  const fields = {};
+  for (const [T, u] of an.fields) fields[T] = list(u);
+  const out = { fingerprints, state: { cfg, opts: an.opts, ndof, nsteps: steps.length, fingerprint: st.fingerprint, plan: st.plan },
+    restored: { fields, uCPI: an.uCPI ? list(an.uCPI) : null },
+    chunkSteps: [1, 570, 76077, 3093159, 20000000].map(n => [n, resultsChunkSteps(n)]) };
+  fs.writeFileSync(path.join(HERE, 'results_golden.json'), JSON.stringify(out));
+  console.log('results: state', stBytes.length, 'bytes,', steps.length, 'steps; fingerprints', JSON.stringify(fingerprints));
+}
+
 const what = process.argv.slice(3);
 if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }
 if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
@@ -471,5 +534,6 @@ if (what.includes('anand')) anandCases();
 if (what.includes('screen')) screenCases();
 if (what.includes('submodel')) submodelCases();
 if (what.includes('path')) pathCases();
+if (what.includes('results')) resultsCases();
 if (!what.length || what.includes('element')) elementCases();
 for (const n of ['small_box', 'plate_16', 'cfg1', 'cfg2']) if (!what.length || what.includes(n)) solveSpec(n);

@@ -128,5 +128,5 @@ def test_node_addon_loads_and_exports():
     out = subprocess.run([node, "-e", f"const h=require({addon!r}); console.log(Object.keys(h).sort().join(','))"],
                          capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
-    assert out.stdout.strip() == ("adjointBackward,adjointForward,createModel,destroyModel,screenJoints,solveStep,"
+    assert out.stdout.strip() == ("adjointBackward,adjointForward,createModel,destroyModel,pathCall,screenJoints,solveStep,"
                                   "stateAccumulate,stateStress,submodelCall,submodelCreate")
