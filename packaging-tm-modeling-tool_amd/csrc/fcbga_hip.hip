@@ -642,9 +642,14 @@ void factor_fine_columns(fcbga_model *M) {
   const int nz = g.nz;
   be::pfor(ncol, M->stream, [=] __host__ __device__(int64_t c) {
     fe::column_factor(nz, Db + 9 * c * nz, Lb + 9 * c * nz, F + 18 * c * nz);
-    fe::column_scan_coefs(nz, F + 18 * c * nz, C + NC * c * nz);
-    for (int q = 0; q < NC * nz; q++) C32[NC * c * nz + q] = (float)C[NC * c * nz + q];
   }, 64);
+  // the coefficients of every node at once (each needs its own factor and the
+  // sub-diagonal block of the node above)
+  be::pfor(g.G, M->stream, [=] __host__ __device__(int64_t gi) {
+    const int k = (int)(gi % nz);
+    fe::node_scan_coefs_at(k, nz, F + 18 * gi, C + NC * gi);
+    for (int q = 0; q < NC; q++) C32[NC * gi + q] = (float)C[NC * gi + q];
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1392,29 +1397,92 @@ FE_HD void row_sub_blocks(T &acc, int32_t k0, int32_t k1, int32_t S, int p, cons
 }
 
 #if FCBGA_GPU
-// inverse of a banded Cholesky factor, one 64-lane block per unit vector e_c;
-// the vector lives in LDS and every substitution step is a lane-parallel dot
+// inverse of a banded Cholesky factor, one wavefront per unit vector e_c,
+// the vector in LDS.  Both substitutions go by blocks of 64 rows, lane l on
+// row l of the block: first the lane sums its row's terms over the rows
+// already solved (its own sequential dot), then the block's triangle is
+// solved row by row, the new entry taken from its lane (readlane; times the
+// row's reciprocal pivot, so the step's critical path is one multiply and
+// one multiply-add) and the triangle's coefficients read from LDS, where
+// they were staged first.
+constexpr int BAND_INV_B = 64;  // rows per block = lanes
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
 __global__ void __launch_bounds__(64) k_band_inverse(int32_t n, int bw, int w, const double *__restrict__ Ab,
                                                      double *__restrict__ inv) {
-  __shared__ double xs[4096];
+  extern __shared__ double band_lds[];
+  double *tri = band_lds;                          // tri[j * 64 + l]: row l's coefficient of block row j
+  double *xs = band_lds + BAND_INV_B * BAND_INV_B;  // the vector (n values)
   const int32_t c = blockIdx.x;
   const int lane = threadIdx.x;
   for (int32_t i = lane; i < n; i += 64) xs[i] = 0.0;
   __syncthreads();
-  for (int32_t i = c; i < n; i++) {
-    double part = 0;
-    int32_t k0 = i - bw > c ? i - bw : c;
-    for (int32_t k = k0 + lane; k < i; k += 64) part += Ab[(int64_t)i * w + (k - i + bw)] * xs[k];
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
-    if (lane == 0) xs[i] = ((i == c ? 1.0 : 0.0) - part) / Ab[(int64_t)i * w + bw];
+  // forward, L y = e_c: rows c .. n - 1
+  for (int32_t i0 = c; i0 < n; i0 += BAND_INV_B) {
+    const int32_t r = i0 + lane;
+    const bool live = r < n;
+    const int jn = n - i0 < BAND_INV_B ? n - i0 : BAND_INV_B;
+    // L(r, i0 + j) for i0 + j < r within the band; every load is issued (an
+    // entry outside reads the first one) so that their latencies overlap
+#pragma unroll 16
+    for (int j = 0; j < jn; j++) {
+      const int32_t k = i0 + j;
+      const bool in = live && k < r && k >= r - bw;
+      const double v = Ab[in ? (int64_t)r * w + (k - r + bw) : 0];
+      tri[j * BAND_INV_B + lane] = in ? v : 0.0;
+    }
+    double rhs = r == c ? 1.0 : 0.0;
+    const double rd = live ? 1.0 / Ab[(int64_t)r * w + bw] : 1.0;  // the row's reciprocal pivot
+    if (live) {
+      double part = 0;
+      const double *row = Ab + (int64_t)r * w + bw - r;  // row[k] = L(r, k)
+#pragma unroll 8
+      for (int32_t k = (r - bw > c ? r - bw : c); k < i0; k++) part += row[k] * xs[k];
+      rhs -= part;
+    }
+    __syncthreads();
+    for (int j = 0; j < jn; j++) {
+      const int32_t rj = i0 + j;
+      const double xj = readlane_f64(rhs, j) * readlane_f64(rd, j);
+      if (lane == j) xs[rj] = xj;
+      rhs -= tri[j * BAND_INV_B + lane] * xj;
+    }
     __syncthreads();
   }
-  for (int32_t i = n - 1; i >= 0; i--) {
-    double part = 0;
-    int32_t kend = i + bw + 1 < n ? i + bw + 1 : n;
-    for (int32_t k = i + 1 + lane; k < kend; k += 64) part += Ab[(int64_t)k * w + (i - k + bw)] * xs[k];
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
-    if (lane == 0) xs[i] = (xs[i] - part) / Ab[(int64_t)i * w + bw];
+  // backward, L^T x = y: rows n - 1 .. 0, lane l on row i1 - l
+  for (int32_t i1 = n - 1; i1 >= 0; i1 -= BAND_INV_B) {
+    const int32_t r = i1 - lane;
+    const bool live = r >= 0;
+    const int jn = i1 + 1 < BAND_INV_B ? i1 + 1 : BAND_INV_B;
+    // L(i1 - j, r) for i1 - j > r within the band
+#pragma unroll 16
+    for (int j = 0; j < jn; j++) {
+      const int32_t k = i1 - j;
+      const bool in = live && k > r && k <= r + bw;
+      const double v = Ab[in ? (int64_t)k * w + (r - k + bw) : 0];
+      tri[j * BAND_INV_B + lane] = in ? v : 0.0;
+    }
+    double rhs = live ? xs[r] : 0.0;
+    const double rd = live ? 1.0 / Ab[(int64_t)r * w + bw] : 1.0;
+    if (live) {
+      double part = 0;
+      const int32_t kend = r + bw < n - 1 ? r + bw : n - 1;
+#pragma unroll 8
+      for (int32_t k = i1 + 1; k <= kend; k++) part += Ab[(int64_t)k * w + (r - k + bw)] * xs[k];
+      rhs -= part;
+    }
+    __syncthreads();
+    for (int j = 0; j < jn; j++) {
+      const int32_t rj = i1 - j;
+      const double xj = readlane_f64(rhs, j) * readlane_f64(rd, j);
+      if (lane == j) xs[rj] = xj;
+      rhs -= tri[j * BAND_INV_B + lane] * xj;
+    }
     __syncthreads();
   }
   for (int32_t i = lane; i < n; i += 64) inv[(int64_t)c * n + i] = xs[i];
@@ -1515,13 +1583,8 @@ __global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, cons
 
 #endif
 
-void band_inverse(int32_t n, int bw, int w, const double *Ab, double *inv, be::stream_t s) {
-  if (n > 4096) throw be::Error("coarsest level larger than 4096 DOFs");
-#if FCBGA_GPU
-  hipLaunchKernelGGL(k_band_inverse, dim3(n), dim3(64), 0, s, n, bw, w, Ab, inv);
-  BE_CHECK(hipGetLastError());
-#else
-  (void)s;
+// host form: the same substitutions column by column
+void band_inverse_host(int32_t n, int bw, int w, const double *Ab, double *inv) {
   std::vector<double> x(n);
   for (int32_t c = 0; c < n; c++) {
     std::fill(x.begin(), x.end(), 0.0);
@@ -1537,6 +1600,17 @@ void band_inverse(int32_t n, int bw, int w, const double *Ab, double *inv, be::s
     }
     for (int32_t i = 0; i < n; i++) inv[(int64_t)c * n + i] = x[i];
   }
+}
+
+void band_inverse(int32_t n, int bw, int w, const double *Ab, double *inv, be::stream_t s) {
+  if (n > 4096) throw be::Error("coarsest level larger than 4096 DOFs");
+#if FCBGA_GPU
+  const size_t lds = sizeof(double) * ((size_t)BAND_INV_B * BAND_INV_B + n);
+  hipLaunchKernelGGL(k_band_inverse, dim3(n), dim3(64), lds, s, n, bw, w, Ab, inv);
+  BE_CHECK(hipGetLastError());
+#else
+  (void)s;
+  band_inverse_host(n, bw, w, Ab, inv);
 #endif
 }
 

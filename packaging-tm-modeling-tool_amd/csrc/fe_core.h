@@ -465,19 +465,19 @@ FE_HD void node_scan_coefs(const T *Fk, bool hasBelow, bool hasAbove, const T *L
     }
 }
 
-// Scan coefficients of every node of one column from its factor F (18 per
-// node), SCAN_COEFS per node: Li (9), P (9), Q (9) and one zero pad value (so
-// that a node's record is 16-byte aligned).  Formed once per factorisation in
-// FP64; the sweeps read them instead of inverting the diagonal factors.
+// Scan coefficients of a node from the column's factor (18 per node),
+// SCAN_COEFS per node: Li (9), P (9), Q (9) and one zero pad value (so that a
+// node's record is 16-byte aligned).  Formed once per factorisation in FP64;
+// the sweeps read them instead of inverting the diagonal factors.
 constexpr int SCAN_COEFS = 28;
 
-FE_HD void column_scan_coefs(int nz, const double *F, double *C) {
-  for (int k = 0; k < nz; k++) {
-    double Ld1[9];
-    for (int q = 0; q < 9; q++) Ld1[q] = k + 1 < nz ? F[18 * (k + 1) + 9 + q] : 0.0;
-    node_scan_coefs<double>(F + 18 * k, k > 0, k + 1 < nz, Ld1, C + SCAN_COEFS * k);
-    C[SCAN_COEFS * k + 27] = 0.0;
-  }
+// node k of a column of nz nodes: Fk its factor record (the next node's
+// follows it), out its SCAN_COEFS coefficients
+FE_HD void node_scan_coefs_at(int k, int nz, const double *Fk, double *out) {
+  double Ld1[9];
+  for (int q = 0; q < 9; q++) Ld1[q] = k + 1 < nz ? Fk[18 + 9 + q] : 0.0;
+  node_scan_coefs<double>(Fk, k > 0, k + 1 < nz, Ld1, out);
+  out[27] = 0.0;
 }
 
 // Host form of the scan solve (the GPU kernel does the same rounds with lane
