@@ -1075,6 +1075,103 @@ void build_hierarchy(fcbga_model *M) {
 // Galerkin product of level 1 from the fine operator (galerkinProduct,
 // src/core/10-solver.js:193-245, with the fine rows gathered from the
 // elements): block (A, B) = sum over nodes i of A and j of B of W_i^T Ks_ij W_j.
+// The term of one node pair (i at plane ka of column (iA, jA), j at plane kk
+// of column (iB, jB)); t receives W_i^T Ks_ij W_j (6 x 6).
+FE_HD void galerkin_pair_term(const fe::Grid &g, int iA, int jA, int ka, int iB, int jB, int kk, int64_t gi,
+                              int64_t gj, const int32_t *__restrict__ ct, const double *__restrict__ cs,
+                              const double *__restrict__ Kc, const uint8_t *__restrict__ mask,
+                              const double *__restrict__ sv, const double *__restrict__ zp, double t[36]) {
+  double Kb[9];
+  fe::pair_block(g, iA, jA, ka, iB, jB, kk, ct, cs, Kc, Kb);
+  double Ks[9];
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) {
+      uint8_t mp = mask[3 * gi + p], mq = mask[3 * gj + q];
+      double v = (mp && mq) ? sv[3 * gi + p] * Kb[3 * p + q] * sv[3 * gj + q] : 0.0;
+      if (gi == gj && p == q && !mp) v = 1.0;
+      Ks[3 * p + q] = v;
+    }
+  // W (3x6) of a node: diag(1/s) [I | u = t + theta x z'] rows
+  double Wi[18], Wj[18];
+  for (int q = 0; q < 18; q++) { Wi[q] = 0; Wj[q] = 0; }
+  double zi = zp[gi], zj = zp[gj];
+  double si0 = 1 / sv[3 * gi], si1 = 1 / sv[3 * gi + 1], si2 = 1 / sv[3 * gi + 2];
+  double sj0 = 1 / sv[3 * gj], sj1 = 1 / sv[3 * gj + 1], sj2 = 1 / sv[3 * gj + 2];
+  Wi[0] = si0; Wi[4] = zi * si0; Wi[7] = si1; Wi[9] = -zi * si1; Wi[14] = si2; Wi[17] = zi * si2;
+  Wj[0] = sj0; Wj[4] = zj * sj0; Wj[7] = sj1; Wj[9] = -zj * sj1; Wj[14] = sj2; Wj[17] = zj * sj2;
+  double Tm[18];
+  for (int p = 0; p < 3; p++)
+    for (int c = 0; c < 6; c++) {
+      double v = 0;
+      for (int q = 0; q < 3; q++) v += Ks[3 * p + q] * Wj[6 * q + c];
+      Tm[6 * p + c] = v;
+    }
+  for (int a = 0; a < 6; a++)
+    for (int c = 0; c < 6; c++) {
+      double v = 0;
+      for (int p = 0; p < 3; p++) v += Wi[6 * p + a] * Tm[6 * p + c];
+      t[6 * a + c] = v;
+    }
+}
+
+#if FCBGA_GPU
+// One wavefront per block (A, B).  The block's node pairs are taken in the
+// host loop's order, pair l = (plane k0[A] + l / 3, that plane + l % 3 - 1):
+// lane l of a 64-pair chunk forms its pair's term (zero where the pair is not
+// in the block), and the chunk's terms pass through LDS in two halves of 18
+// entries, lanes 0..17 adding each half entry by entry in pair order, so
+// every entry is the host loop's sum.  (Half the terms at a time keeps the
+// LDS per wavefront small enough for four wavefronts per SIMD.)
+constexpr int GAL1_STRIDE = 19;  // LDS words per pair and half term (18, padded)
+
+__global__ void __launch_bounds__(64) k_galerkin_level1(fe::Grid g, const int32_t *__restrict__ brow,
+                                                        const int32_t *__restrict__ bcol,
+                                                        const int32_t *__restrict__ aggCol,
+                                                        const int32_t *__restrict__ k0,
+                                                        const int32_t *__restrict__ k1,
+                                                        const int32_t *__restrict__ nodeAgg,
+                                                        const int32_t *__restrict__ ct, const double *__restrict__ cs,
+                                                        const double *__restrict__ Kc,
+                                                        const uint8_t *__restrict__ mask,
+                                                        const double *__restrict__ sv, const double *__restrict__ zp,
+                                                        double *__restrict__ val, float *__restrict__ val32) {
+  __shared__ double term[64 * GAL1_STRIDE];
+  const int64_t kb = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int32_t A = brow[kb], B = bcol[kb];
+  const int32_t cA = aggCol[A], cB = aggCol[B];
+  const int iA = cA / g.ny, jA = cA % g.ny, iB = cB / g.ny, jB = cB % g.ny;
+  const int a0 = k0[A], npair = 3 * (k1[A] - a0 + 1), b0 = k0[B], b1 = k1[B];
+  double acc0 = 0, acc1 = 0;  // entries lane and lane + 18 (lanes 0..17)
+  for (int base = 0; base < npair; base += 64) {
+    const int l = base + lane;
+    const int ka = a0 + l / 3, kk = ka + l % 3 - 1;
+    const int64_t gi = (int64_t)cA * g.nz + ka, gj = (int64_t)cB * g.nz + kk;
+    bool ok = l < npair && kk >= b0 && kk <= b1;
+    ok = ok && nodeAgg[gi] == A && nodeAgg[gj] == B;
+    double t[36];
+    if (ok) galerkin_pair_term(g, iA, jA, ka, iB, jB, kk, gi, gj, ct, cs, Kc, mask, sv, zp, t);
+    const int nq = npair - base < 64 ? npair - base : 64;
+    for (int half = 0; half < 2; half++) {
+      for (int e = 0; e < 18; e++) term[lane * GAL1_STRIDE + e] = ok ? t[18 * half + e] : 0.0;
+      __syncthreads();
+      if (lane < 18) {
+        double a = half ? acc1 : acc0;
+        for (int q = 0; q < nq; q++) a += term[q * GAL1_STRIDE + lane];
+        if (half) acc1 = a; else acc0 = a;
+      }
+      __syncthreads();
+    }
+  }
+  if (lane < 18) {
+    val[36 * kb + lane] = acc0;
+    val32[36 * kb + lane] = (float)acc0;
+    val[36 * kb + 18 + lane] = acc1;
+    val32[36 * kb + 18 + lane] = (float)acc1;
+  }
+}
+#endif
+
 void galerkin_level1(fcbga_model *M) {
   DevShell &D = *M->lv[0];
   const fe::Grid g = M->g;
@@ -1084,6 +1181,13 @@ void galerkin_level1(fcbga_model *M) {
   const uint8_t *mask = M->dofmask.p;
   double *val = D.val64.p;
   float *val32 = D.val32.p;
+#if FCBGA_GPU
+  if (D.nb > 0) {
+    hipLaunchKernelGGL(k_galerkin_level1, dim3((unsigned)D.nb), dim3(64), 0, M->stream, g, brow, bcol, aggCol, k0, k1,
+                       nodeAgg, ct, cs, Kc, mask, sv, zp, val, val32);
+    BE_CHECK(hipGetLastError());
+  }
+#else
   const int nz = g.nz, ny = g.ny;
   be::pfor(D.nb, M->stream, [=] __host__ __device__(int64_t kb) {
     int32_t A = brow[kb], B = bcol[kb];
@@ -1098,41 +1202,14 @@ void galerkin_level1(fcbga_model *M) {
       for (int kk = kb0; kk <= kb1; kk++) {
         int64_t gj = (int64_t)cB * nz + kk;
         if (nodeAgg[gj] != B) continue;
-        double Kb[9];
-        fe::pair_block(g, iA, jA, ka, iB, jB, kk, ct, cs, Kc, Kb);
-        double Ks[9];
-        for (int p = 0; p < 3; p++)
-          for (int q = 0; q < 3; q++) {
-            uint8_t mp = mask[3 * gi + p], mq = mask[3 * gj + q];
-            double v = (mp && mq) ? sv[3 * gi + p] * Kb[3 * p + q] * sv[3 * gj + q] : 0.0;
-            if (gi == gj && p == q && !mp) v = 1.0;
-            Ks[3 * p + q] = v;
-          }
-        // W (3x6) of a node: diag(1/s) [I | u = t + theta x z'] rows
-        double Wi[18], Wj[18];
-        for (int q = 0; q < 18; q++) { Wi[q] = 0; Wj[q] = 0; }
-        double zi = zp[gi], zj = zp[gj];
-        double si0 = 1 / sv[3 * gi], si1 = 1 / sv[3 * gi + 1], si2 = 1 / sv[3 * gi + 2];
-        double sj0 = 1 / sv[3 * gj], sj1 = 1 / sv[3 * gj + 1], sj2 = 1 / sv[3 * gj + 2];
-        Wi[0] = si0; Wi[4] = zi * si0; Wi[7] = si1; Wi[9] = -zi * si1; Wi[14] = si2; Wi[17] = zi * si2;
-        Wj[0] = sj0; Wj[4] = zj * sj0; Wj[7] = sj1; Wj[9] = -zj * sj1; Wj[14] = sj2; Wj[17] = zj * sj2;
-        double Tm[18];
-        for (int p = 0; p < 3; p++)
-          for (int c = 0; c < 6; c++) {
-            double t = 0;
-            for (int q = 0; q < 3; q++) t += Ks[3 * p + q] * Wj[6 * q + c];
-            Tm[6 * p + c] = t;
-          }
-        for (int a = 0; a < 6; a++)
-          for (int c = 0; c < 6; c++) {
-            double t = 0;
-            for (int p = 0; p < 3; p++) t += Wi[6 * p + a] * Tm[6 * p + c];
-            acc[6 * a + c] += t;
-          }
+        double t[36];
+        galerkin_pair_term(g, iA, jA, ka, iB, jB, kk, gi, gj, ct, cs, Kc, mask, sv, zp, t);
+        for (int q = 0; q < 36; q++) acc[q] += t[q];
       }
     }
     for (int q = 0; q < 36; q++) { val[36 * kb + q] = acc[q]; val32[36 * kb + q] = (float)acc[q]; }
   });
+#endif
 }
 
 // Galerkin product between shell levels l (fine) and l+1 (coarse)
@@ -2006,14 +2083,26 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
     const T *res = r;
     be::pfor(C.nagg, s, [=] __host__ __device__(int64_t A) {
       T acc[6] = {0, 0, 0, 0, 0, 0};
-      int64_t c = aggCol[A];
-      for (int k = k0[A]; k <= k1[A]; k++) {
-        int64_t gi = c * nz + k;
-        if (nodeAgg[gi] != (int32_t)A) continue;
-        T r0 = res[3 * gi] * sinv[3 * gi], r1 = res[3 * gi + 1] * sinv[3 * gi + 1], r2 = res[3 * gi + 2] * sinv[3 * gi + 2];
-        T z = zp[gi];
-        acc[0] += r0; acc[1] += r1; acc[2] += r2;
-        acc[3] -= z * r1; acc[4] += z * r0; acc[5] += z * r2;
+      const int64_t c = aggCol[A];
+      const int ka = k0[A], kb = k1[A];
+      // the aggregate's nodes four at a time: their loads are issued together
+      // (a node past kb re-reads node ka), then added in node order
+      for (int k = ka; k <= kb; k += 4) {
+        int32_t ag[4];
+        T rv[4][3], sv[4][3], zv[4];
+        for (int u = 0; u < 4; u++) {
+          const int64_t gi = c * nz + (k + u <= kb ? k + u : ka);
+          ag[u] = k + u <= kb ? nodeAgg[gi] : -1;
+          for (int p = 0; p < 3; p++) { rv[u][p] = res[3 * gi + p]; sv[u][p] = sinv[3 * gi + p]; }
+          zv[u] = zp[gi];
+        }
+        for (int u = 0; u < 4; u++) {
+          if (ag[u] != (int32_t)A) continue;
+          const T r0 = rv[u][0] * sv[u][0], r1 = rv[u][1] * sv[u][1], r2 = rv[u][2] * sv[u][2];
+          const T z = zv[u];
+          acc[0] += r0; acc[1] += r1; acc[2] += r2;
+          acc[3] -= z * r1; acc[4] += z * r0; acc[5] += z * r2;
+        }
       }
       for (int a = 0; a < 6; a++) bc[6 * A + a] = acc[a];
     });
