@@ -142,10 +142,10 @@ inline void pfor(int64_t n, stream_t s, F f, int block = 256) {
 // ---- FP64 dot products --------------------------------------------------------
 // Two-pass reduction in a fixed order (deterministic for a given n).
 
-#if FCBGA_GPU
 constexpr int RED_BLOCK = 256;
-constexpr int RED_GRID = 1024;
+constexpr int RED_GRID = 1024;  // blocks of the first pass: partial sums per dot
 
+#if FCBGA_GPU
 template <typename TA, typename TB>
 __global__ void __launch_bounds__(RED_BLOCK) k_dot_partial(int64_t n, const TA *__restrict__ a,
                                                            const TB *__restrict__ b, double *__restrict__ part) {

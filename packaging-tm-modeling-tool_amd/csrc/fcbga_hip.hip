@@ -248,7 +248,7 @@ struct fcbga_model {
   DBuf<double> s64, w64, sinv64, bload, Db, Lb, F64, C64;
   DBuf<float> w32, sinv32, C32;
   // solver vectors (grid layout, 3 per node)
-  DBuf<double> x, r, z, zold, p, q, tmp64, dx64;
+  DBuf<double> x, r, z, p, q, tmp64, dx64;
   DBuf<float> b32v, x32v, r32v, dx32;
   DBuf<double> partials, scal;
   // multigrid
@@ -2614,6 +2614,69 @@ void precondition(fcbga_model *M, int precision, int precond) {
 // device scalar slots of the PCG
 enum { SC_RR = 0, SC_RZ = 1, SC_PQ = 2, SC_RZO = 3, SC_RZN = 4, SC_ALPHA = 5, SC_BETA = 6, SC_FLAG = 7, SC_COUNT = 8 };
 
+#if FCBGA_GPU
+// x += alpha p, r -= alpha q, and per block the partial sums of r.r and
+// r.z over the updated r (the grid-stride order and block tree of
+// be::dot_async, so the sums are the same as two separate dots)
+__global__ void __launch_bounds__(be::RED_BLOCK) k_cg_update(int64_t n, const double *__restrict__ sc,
+                                                            double *__restrict__ x, double *__restrict__ r,
+                                                            const double *__restrict__ p,
+                                                            const double *__restrict__ q,
+                                                            const double *__restrict__ z, double *__restrict__ part) {
+  __shared__ double s1[be::RED_BLOCK], s2[be::RED_BLOCK];
+  const double a = sc[SC_ALPHA];
+  double rr = 0, rz = 0;
+  for (int64_t i = (int64_t)blockIdx.x * be::RED_BLOCK + threadIdx.x; i < n; i += (int64_t)be::RED_GRID * be::RED_BLOCK) {
+    x[i] += a * p[i];
+    r[i] -= a * q[i];
+    const double ri = r[i];
+    rr += ri * ri;
+    if (z) rz += ri * z[i];
+  }
+  s1[threadIdx.x] = rr;
+  s2[threadIdx.x] = rz;
+  __syncthreads();
+  for (int w = be::RED_BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1[0];
+    part[be::RED_GRID + blockIdx.x] = s2[0];
+  }
+}
+#endif
+
+void cg_update(int64_t n3, double *sc, double *x, double *r, const double *p, const double *q, const double *z,
+               double *part, be::stream_t s) {
+#if FCBGA_GPU
+  hipLaunchKernelGGL(k_cg_update, dim3(be::RED_GRID), dim3(be::RED_BLOCK), 0, s, n3, sc, x, r, p, q, z, part);
+  BE_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(be::k_sum_partials, dim3(1), dim3(be::RED_BLOCK), 0, s, 1, be::RED_GRID, part, sc + SC_RR);
+  BE_CHECK(hipGetLastError());
+  if (z) {
+    hipLaunchKernelGGL(be::k_sum_partials, dim3(1), dim3(be::RED_BLOCK), 0, s, 1, be::RED_GRID, part + be::RED_GRID,
+                       sc + SC_RZO);
+    BE_CHECK(hipGetLastError());
+  }
+#else
+  (void)part; (void)s;
+  const double a = sc[SC_ALPHA];
+  double rr = 0, rz = 0;
+  for (int64_t i = 0; i < n3; i++) {
+    x[i] += a * p[i];
+    r[i] -= a * q[i];
+    rr += r[i] * r[i];
+    if (z) rz += r[i] * z[i];
+  }
+  sc[SC_RR] = rr;
+  if (z) sc[SC_RZO] = rz;
+#endif
+}
+
 // One PCG iteration (LinearSolver._pcg, src/core/10-solver.js:1021-1040) with
 // every scalar kept on the device: q = A p, alpha = rz / pq, x += alpha p,
 // r -= alpha q, rr = r.r, z = M^-1 r, beta (Fletcher-Reeves, or Polak-Ribiere
@@ -2623,7 +2686,7 @@ void iteration_body(fcbga_model *M, const fcbga_solve_opts &o, bool flex) {
   be::stream_t s = M->stream;
   const int64_t n3 = 3 * M->g.G;
   FineOps<double> f64 = fine_ops<double>(M);
-  double *x = M->x.p, *r = M->r.p, *z = M->z.p, *p = M->p.p, *q = M->q.p, *zold = M->zold.p, *sc = M->scal.p;
+  double *x = M->x.p, *r = M->r.p, *z = M->z.p, *p = M->p.p, *q = M->q.p, *sc = M->scal.p;
   double *part = M->partials.p;
   {
     KScope ks(M, FCBGA_KCLASS_APPLY);
@@ -2636,16 +2699,11 @@ void iteration_body(fcbga_model *M, const fcbga_solve_opts &o, bool flex) {
     sc[SC_FLAG] = ok ? 0.0 : 1.0;
     sc[SC_ALPHA] = ok ? sc[SC_RZ] / pq : 0.0;
   });
-  be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
-    double a = sc[SC_ALPHA];
-    x[i] += a * p[i];
-    r[i] -= a * q[i];
-  });
-  be::dot_async(n3, r, r, part, sc + SC_RR, s);
-  if (flex) be::pfor(n3, s, [=] __host__ __device__(int64_t i) { zold[i] = z[i]; });
+  // x += alpha p, r -= alpha q with r.r and (flexible) r.z_old of the new r
+  // in the same pass, before the preconditioner overwrites z
+  cg_update(n3, sc, x, r, p, q, flex ? z : nullptr, part, s);
   precondition(M, o.precision, o.precond);
   be::dot_async(n3, r, z, part, sc + SC_RZN, s);
-  if (flex) be::dot_async(n3, r, zold, part, sc + SC_RZO, s);
   be::pfor(1, s, [=] __host__ __device__(int64_t) {
     double rz = sc[SC_RZ], rzn = sc[SC_RZN];
     double beta = flex ? (rzn - sc[SC_RZO]) / rz : rzn / rz;
@@ -2694,18 +2752,17 @@ void read_scalars(fcbga_model *M, double h[2]) {
 
 void alloc_vectors(fcbga_model *M, int precision) {
   const size_t n3 = 3 * (size_t)M->g.G;
-  const void *before[6] = {M->x.p, M->zold.p, M->b32v.p, M->partials.p, M->scal.p, M->dx64.p};
+  const void *before[6] = {M->x.p, M->z.p, M->b32v.p, M->partials.p, M->scal.p, M->dx64.p};
   M->x.alloc(n3); M->r.alloc(n3); M->z.alloc(n3); M->p.alloc(n3); M->q.alloc(n3);
   M->tmp64.alloc(n3); M->dx64.alloc(n3);
   if (precision == FCBGA_PREC_MIXED) {
-    M->zold.alloc(n3);
     M->b32v.alloc(n3); M->x32v.alloc(n3); M->r32v.alloc(n3); M->dx32.alloc(n3);
   }
-  M->partials.alloc(1024);
+  M->partials.alloc(2 * be::RED_GRID);  // two partial sums per pass (cg_update)
   const bool fresh = M->scal.n != SC_COUNT;
   M->scal.alloc(SC_COUNT);
   if (fresh) M->scal.zero(M->stream);
-  const void *after[6] = {M->x.p, M->zold.p, M->b32v.p, M->partials.p, M->scal.p, M->dx64.p};
+  const void *after[6] = {M->x.p, M->z.p, M->b32v.p, M->partials.p, M->scal.p, M->dx64.p};
 #if FCBGA_GPU
   // captured iteration graphs hold these addresses: drop them when a buffer moved
   for (int q = 0; q < 6; q++)
