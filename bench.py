@@ -22,6 +22,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -229,10 +230,44 @@ def submodel_timing():
     dt = time.perf_counter() - t0
     m.close()
     ref = g["result"]
+    # the batch form in a process of its own (its streams then have the device's
+    # hardware queues to themselves): eight copies of the submodel at once
+    nb = 8
+    env = dict(os.environ, GPU_MAX_HW_QUEUES="16")
+    res = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--submodel-batch", str(nb)],
+                         capture_output=True, text=True, env=env, timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError("submodel batch run failed: " + res.stderr[-2000:])
+    b = json.loads(res.stdout.strip().splitlines()[-1])
+    b["vs_one_at_a_time"] = nb * dt / b["s"]
+    b["same_records"] = bool(b["same_records"] and b["dW"] == float(r["dW"]))
     return {"ndof": ref["ndof"], "steps": r["steps"], "cutbacks": r["cutbacks"], "s": dt,
             "ms_per_step": 1e3 * dt / r["steps"], "newton_per_step": r["newtonPerStep"],
             "dW_rel_err_vs_reference": abs(r["dW"] - ref["dW"]) / abs(ref["dW"]), "reference_s": 61.5,
-            "note": "Newton + IC(0)-PCG per step, the CG in one 1024-thread workgroup (latency-bound at this size)"}
+            "note": "Newton + IC(0)-PCG per step, the CG in one 1024-thread workgroup (latency-bound at this size)",
+            "batch": b}
+
+
+def submodel_batch_child(nb):
+    """The batch measurement's own process: nb copies of the corner-bump
+    submodel run at once (submodel.run_batch, one host thread and device stream
+    each; the parent sets GPU_MAX_HW_QUEUES=16 so that the streams do not share
+    hardware queues).  Prints one JSON line."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    ge.package()
+    from submodel_case import build
+    from fcbga_amd.submodel import run_batch
+    ms = [build("hip")[1] for _ in range(nb)]
+    t0 = time.perf_counter()
+    recs = run_batch(ms, workers=nb)
+    dtb = time.perf_counter() - t0
+    for x in ms:
+        x.close()
+    print(json.dumps({"submodels": nb, "s": dtb, "submodels_per_s": nb / dtb, "dW": float(recs[0]["dW"]),
+                      "same_records": bool(all(x["dW"] == recs[0]["dW"] and x["steps"] == recs[0]["steps"]
+                                               for x in recs)),
+                      "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+                      "note": "one host thread and device stream per submodel, in a process of its own"}))
 
 
 def adjoint_timing(torch, hs, cases, meshspec, tol, reps=3):
@@ -279,7 +314,11 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-path", action="store_true")
     ap.add_argument("--no-submodel", action="store_true")
+    ap.add_argument("--submodel-batch", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.submodel_batch:
+        submodel_batch_child(args.submodel_batch)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solve runs on a single MI355X and this build has no "
               f"multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
@@ -401,8 +440,10 @@ def main():
 
     if not args.no_submodel:
         out["submodel"] = submodel_timing()
+        b = out["submodel"]["batch"]
         log(f"submodel: {out['submodel']['steps']} steps in {out['submodel']['s']:.1f} s "
-            f"(dW vs reference {out['submodel']['dW_rel_err_vs_reference']:.1e})")
+            f"(dW vs reference {out['submodel']['dW_rel_err_vs_reference']:.1e}); "
+            f"{b['submodels']} at once in {b['s']:.1f} s ({b['vs_one_at_a_time']:.1f}x one at a time)")
 
     from fcbga_amd import screening as scr
     out["screening"] = screening_timing(hs, scr)

@@ -380,3 +380,19 @@ class Submodel:
                 "loop": loop, "life": life, "dJoint": d_joint, "steps": steps, "cutbacks": cutbacks,
                 "newtonPerStep": total_newton / max(steps, 1), "first": trace,
                 "section": {"xs": self.xs, "ys": self.ys, "zs": self.zs, "grp": self.cell_mat, "snaps": snaps}}
+
+
+def run_batch(models, workers=4, **run_kw):
+    """Several submodels at once (the batch form submodelBatch(jobs) -> records
+    of the many-bump workload; the reference runs one BumpSubmodel per helper
+    worker, src/worker.js:96-104, 230, 291).  Each Submodel owns a device
+    stream, and each run's host steps go in a thread of their own (the library
+    calls release the interpreter lock), so the runs overlap on the device.
+    How many overlap is bounded by the process's hardware queues (HIP's
+    GPU_MAX_HW_QUEUES, 4 by default, read when the process starts HIP).
+    Returns the records in the order of models."""
+    from concurrent.futures import ThreadPoolExecutor
+    if not models:
+        return []
+    with ThreadPoolExecutor(max(1, min(int(workers), len(models)))) as ex:
+        return list(ex.map(lambda m: m.run(**run_kw), models))

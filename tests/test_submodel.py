@@ -16,6 +16,8 @@ import pytest
 
 from submodel_case import build, f64
 
+from fcbga_amd import submodel as sm
+
 VARIANTS = ["hostemu", pytest.param("hip", marks=pytest.mark.gpu)]
 
 
@@ -102,3 +104,23 @@ def test_node_addon_submodel_run():
     assert (rec["steps"], rec["cutbacks"], rec["snaps"]) == (rec["ref_steps"], rec["ref_cutbacks"], rec["ref_snaps"]), rec
     for k in ("dW_rel_err", "dE_rel_err", "dWsub_rel_err", "stab_rel_err"):
         assert rec[k] <= 1e-6, rec
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_batch_runs_overlap_and_match_single_runs(variant):
+    """Three submodels run at once (one host thread and device stream each)
+    give the record of the same submodel run alone."""
+    n = 12 if variant == "hip" else 2
+    g, alone, _ = build(variant)
+    ref = alone.run(max_steps=n, sections=False)
+    alone.close()
+    models = [build(variant)[1] for _ in range(3)]
+    try:
+        recs = sm.run_batch(models, workers=3, max_steps=n, sections=False)
+    finally:
+        for m in models:
+            m.close()
+    for r in recs:
+        assert r["steps"] == ref["steps"] and r["cutbacks"] == ref["cutbacks"]
+        assert r["dW"] == ref["dW"] and r["dE"] == ref["dE"]
+        assert np.array_equal(np.array(r["loop"]["tau"]), np.array(ref["loop"]["tau"]))
