@@ -148,6 +148,28 @@ def test_graph_replay_matches_eager(lib):
         assert np.abs(u_graph - u_eager).max() <= 1e-12 * np.abs(u_eager).max()
 
 
+@pytest.mark.gpu
+def test_solves_in_flight_on_model_instances_match(lib):
+    # several model instances solved at once from host threads (each its own
+    # stream and graphs): every solve gives what it gives alone
+    from concurrent.futures import ThreadPoolExecutor
+    spec = golden_spec("cfg1")
+    m = meshspec.build_mesh(spec)
+    models = [hs.Model(m, variant="hip") for _ in range(3)]
+    steps = [hs.step_for_spec(spec, Tb=T) for T in (-40.0, 25.0, 125.0, 260.0, 100.0, 0.0)]
+    alone = [models[0].solve_step(st, tol=1e-10, precision=hs.PREC_MIXED) for st in steps]
+
+    def run(mi):
+        return [(k, models[mi].solve_step(steps[k], tol=1e-10, precision=hs.PREC_MIXED)) for k in range(mi, 6, 3)]
+
+    with ThreadPoolExecutor(3) as ex:
+        got = dict(kr for lst in ex.map(run, range(3)) for kr in lst)
+    for k in range(6):
+        u, rec = got[k]
+        assert rec["iters"] == alone[k][1]["iters"]
+        assert np.array_equal(u, alone[k][0]), k
+
+
 @pytest.mark.parametrize("name,prec", [("plate_16", "mixed"), ("cfg1", "fp64")])
 def test_node_addon_step_solve(lib, name, prec):
     # the JS host path: node -> Node-API addon -> C ABI -> HIP
