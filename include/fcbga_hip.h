@@ -288,6 +288,13 @@ int fcbga_path_du(fcbga_model *m, int32_t k, float *out);
  * (3*nn FP32, as exportAnalysisState saves them, src/core/16-results-file.js:121-132) replaces its
  * solve; fcbga_path_sum then sums the restored increments as finishPath does. */
 int fcbga_path_load(fcbga_model *m, int32_t k, const float *du);
+/* The reference's worker farm (runSweep / evalSteps, src/worker.js:76-133) solves blocks of a
+ * plan's steps on several models and sends each step's FP64 increment back (the partial {k, du}
+ * message, src/worker.js:89), which finishPath then sums.  fcbga_path_load64 keeps such an
+ * increment (3*nn FP64, host) as step k; fcbga_path_copy copies step k's kept increment from
+ * another model of the same mesh on the same device (device to device). */
+int fcbga_path_load64(fcbga_model *m, int32_t k, const double *du);
+int fcbga_path_copy(fcbga_model *dst, const fcbga_model *src, int32_t k);
 
 /* meshFingerprint (src/core/16-results-file.js:104-119): the 14-hex-digit hash a results file
  * carries of the mesh its increments belong to, over the bytes of xs, ys, zs (FP64), eGroup

@@ -3738,6 +3738,43 @@ int fcbga_path_load(fcbga_model *m, int32_t k, const float *du) {
   return FCBGA_OK;
 }
 
+int fcbga_path_load64(fcbga_model *m, int32_t k, const double *du) {
+  if (!m || !du || k < 0) { g_err = "model or du is NULL, or k < 0"; return FCBGA_ERR_ARG; }
+  try {
+    if ((size_t)k >= m->pathDu.size()) m->pathDu.resize(k + 1);
+    if (!m->pathDu[k]) m->pathDu[k] = std::make_unique<DBuf<double>>();
+    const int64_t n3 = 3 * (int64_t)m->nn;
+    m->pathDu[k]->alloc(n3);
+    be::h2d(m->pathDu[k]->p, du, sizeof(double) * n3, m->stream);
+    be::sync(m->stream);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_path_copy(fcbga_model *dst, const fcbga_model *src, int32_t k) {
+  if (!dst || !src || k < 0 || (size_t)k >= src->pathDu.size() || !src->pathDu[k]) {
+    g_err = "model is NULL or the source has not kept step k";
+    return FCBGA_ERR_ARG;
+  }
+  if (dst->nn != src->nn) { g_err = "models of different meshes"; return FCBGA_ERR_ARG; }
+  if (dst == src) return FCBGA_OK;
+  try {
+    if ((size_t)k >= dst->pathDu.size()) dst->pathDu.resize(k + 1);
+    if (!dst->pathDu[k]) dst->pathDu[k] = std::make_unique<DBuf<double>>();
+    const int64_t n3 = 3 * (int64_t)dst->nn;
+    dst->pathDu[k]->alloc(n3);
+    // the source's kernels for step k are finished before its bytes are read
+    be::sync(src->stream);
+    be::d2d(dst->pathDu[k]->p, src->pathDu[k]->p, sizeof(double) * n3, dst->stream);
+    be::sync(dst->stream);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
 int fcbga_mesh_fingerprint(int32_t nn, int32_t ne, const double *xs, int32_t nx, const double *ys, int32_t ny,
                            const double *zs, int32_t nz, const int32_t *e_group, const int8_t *e_part,
                            const double *e_scale, char out[15]) {

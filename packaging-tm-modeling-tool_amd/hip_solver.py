@@ -82,6 +82,7 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_submodel_save", "fcbga_submodel_restore", "fcbga_submodel_layer_means",
            "fcbga_submodel_loop_sample", "fcbga_submodel_section", "fcbga_submodel_displacement",
            "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
+           "fcbga_path_load64", "fcbga_path_copy",
            "fcbga_mesh_fingerprint", "fcbga_last_error"]
 
 ANAND_STATE = 11
@@ -162,6 +163,10 @@ def load(variant: str = "hip"):
     L.fcbga_path_du.restype = C.c_int
     L.fcbga_path_load.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float)]
     L.fcbga_path_load.restype = C.c_int
+    L.fcbga_path_load64.argtypes = [C.c_void_p, C.c_int32, _f64p]
+    L.fcbga_path_load64.restype = C.c_int
+    L.fcbga_path_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+    L.fcbga_path_copy.restype = C.c_int
     L.fcbga_mesh_fingerprint.argtypes = [C.c_int32, C.c_int32, _f64p, C.c_int32, _f64p, C.c_int32, _f64p, C.c_int32,
                                          _i32p, C.POINTER(C.c_int8), _f64p, C.c_char_p]
     L.fcbga_mesh_fingerprint.restype = C.c_int
@@ -315,11 +320,25 @@ class Model:
         return out
 
     def path_load(self, k, du):
-        """Restore step k's FP32 increment (Analysis.restorePath) in place of its solve."""
-        du = np.ascontiguousarray(du, np.float32)
+        """Keep a given increment as step k: FP32 from a results file
+        (Analysis.restorePath) or FP64 from another model's solve (the partial
+        {k, du} message of the worker farm), by the array's type."""
+        du = np.asarray(du)
         if du.shape != (self.ndof,):
             raise ValueError(f"increment of {du.shape} values for a model of {self.ndof} DOF")
-        rc = self.L.fcbga_path_load(self.h, int(k), du.ctypes.data_as(C.POINTER(C.c_float)))
+        if du.dtype == np.float64:
+            du = np.ascontiguousarray(du)
+            rc = self.L.fcbga_path_load64(self.h, int(k), du.ctypes.data_as(_f64p))
+        else:
+            du = np.ascontiguousarray(du, np.float32)
+            rc = self.L.fcbga_path_load(self.h, int(k), du.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            raise SolverError(last_error(self.L))
+
+    def path_copy_from(self, other, k):
+        """Keep step k's increment as kept by another model of the same mesh
+        (device to device)."""
+        rc = self.L.fcbga_path_copy(self.h, other.h, int(k))
         if rc != 0:
             raise SolverError(last_error(self.L))
 

@@ -737,7 +737,8 @@ napi_value SubmodelCall(napi_env env, napi_callback_info info) {
 //   'solve', k, step[, opts] solve step k of the plan and keep its increment -> solve record
 //   'sum', ids               sum of the kept increments of steps ids, in that order -> Float64Array
 //   'du', k                  step k's increment in single precision -> Float32Array
-//   'load', k, du            restore step k's increment from a results file (Float32Array)
+//   'load', k, du            keep du as step k's increment: a results file's (Float32Array) or a
+//                            helper's partial message (Float64Array)
 napi_value PathCall(napi_env env, napi_callback_info info) {
   size_t argc = 5;
   napi_value argv[5];
@@ -807,13 +808,19 @@ napi_value PathCall(napi_env env, napi_callback_info info) {
     if (fcbga_path_du(b->m, k, (float *)data) != FCBGA_OK) return throw_msg(env, "pathCall du: " + lib_error());
     return arr;
   }
-  if (o == "load") {
+  if (o == "load") {  // FP32 from a results file, FP64 from a helper's partial message
     void *p;
     size_t n;
-    if (argc < 4 || !arg_typed(env, argv[3], napi_float32_array, &p, &n) || (int64_t)n != ndof)
-      return throw_msg(env, "pathCall load: du must be a Float32Array of ndof values");
-    if (fcbga_path_load(b->m, k, (const float *)p) != FCBGA_OK) return throw_msg(env, "pathCall load: " + lib_error());
-    return undef;
+    if (argc >= 4 && arg_typed(env, argv[3], napi_float32_array, &p, &n) && (int64_t)n == ndof) {
+      if (fcbga_path_load(b->m, k, (const float *)p) != FCBGA_OK) return throw_msg(env, "pathCall load: " + lib_error());
+      return undef;
+    }
+    if (argc >= 4 && arg_typed(env, argv[3], napi_float64_array, &p, &n) && (int64_t)n == ndof) {
+      if (fcbga_path_load64(b->m, k, (const double *)p) != FCBGA_OK)
+        return throw_msg(env, "pathCall load: " + lib_error());
+      return undef;
+    }
+    return throw_msg(env, "pathCall load: du must be a Float32Array or Float64Array of ndof values");
   }
   return throw_msg(env, "pathCall: unknown op " + o);
 }

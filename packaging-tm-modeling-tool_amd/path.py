@@ -217,3 +217,60 @@ def finish_path_device(model, steps, path, temps_all):
     u_cpi = model.path_sum(path["chainJoin"] + path["cpi"]) if path["hasCPI"] else None
     u_ddi = model.path_sum(path["ddiJoin"] + path["ddi"]) if path["hasDDI"] else None
     return fields, u_cpi, u_ddi
+
+
+SWEEP_BLOCKS_PER_WORKER = 3  # src/core/00-constants.js:65
+SWEEP_BLOCK_MIN = 2          # :66
+
+
+def path_blocks(n, workers):
+    """Blocks of consecutive steps of a plan of n steps for the given number
+    of workers (runSweep, src/worker.js:114-117)."""
+    size = max(SWEEP_BLOCK_MIN, math.ceil(n / (SWEEP_BLOCKS_PER_WORKER * workers)))
+    return [list(range(k, min(n, k + size))) for k in range(0, n, size)]
+
+
+def run_path_blocks(models, steps, path, temps_all, step_of, tol=1e-8, precision=None, warm=20):
+    """The plan's steps over several model instances of the same mesh, the
+    reference's worker farm (runSweep / evalSteps, src/worker.js:76-133):
+    blocks of consecutive steps are taken from one queue by one host thread
+    per instance, each instance solving its blocks in order with its own
+    warm-start basis (its own stream, so the solves overlap on the device);
+    the increments then meet in the first instance (device to device) and
+    the fields are summed there in finishPath's order.  With one instance
+    this is run_path.  Returns (fields, u_cpi, u_ddi, records)."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    from . import hip_solver as hs
+    precision = hs.PREC_MIXED if precision is None else precision
+    blocks = path_blocks(len(steps), len(models))
+    lock = threading.Lock()
+    nxt = [0]
+    recs = [None] * len(steps)
+    owner = [0] * len(steps)
+
+    def worker(mi):
+        m = models[mi]
+        m.warm_reset()
+        m.path_reset()
+        while True:
+            with lock:
+                if nxt[0] >= len(blocks):
+                    return
+                b = blocks[nxt[0]]
+                nxt[0] += 1
+            for k in b:
+                rec = m.path_solve(k, step_of(steps[k]), tol=tol, precision=precision, warm=warm)
+                if not rec["converged"]:
+                    raise hs.SolverError(f"Linear solver did not converge (step {k}: {steps[k]['Ta']} -> "
+                                         f"{steps[k]['Tb']} C): relative residual {rec['relres']:.2e}")
+                recs[k], owner[k] = rec, mi
+
+    with ThreadPoolExecutor(len(models)) as ex:
+        list(ex.map(worker, range(len(models))))
+    for k, mi in enumerate(owner):
+        if mi:
+            models[0].path_copy_from(models[mi], k)
+    fields, u_cpi, u_ddi = finish_path_device(models[0], steps, path, temps_all)
+    return fields, u_cpi, u_ddi, recs

@@ -35,7 +35,11 @@ solver.pathLoad(1, d1);
 const back = solver.pathSum([0, 1]);
 let exact = true;
 for (let q = 0; q < back.length; q++) exact = exact && back[q] === (0 + d0[q]) + d1[q];
+// an FP64 increment (a helper's partial message) kept as it is
+solver.pathLoad(2, p1);
+const p2 = solver.pathSum([2]);
+for (let q = 0; q < p1.length; q++) exact = exact && p2[q] === p1[q];
 let badLoad = false;
-try { solver.pathLoad(2, new Float32Array(3)); } catch (e) { badLoad = /Float32Array of ndof/.test(e.message); }
+try { solver.pathLoad(3, new Float32Array(3)); } catch (e) { badLoad = /Float64Array of ndof/.test(e.message); }
 solver.destroy();
 console.log(JSON.stringify({ name, ndof: one.length, converged: r0.converged && r1.converged, rel_err_sum: e1 / mx, f32ok, restore_exact: exact, badLoad }));

@@ -88,3 +88,37 @@ def test_path_run_sums_to_single_solves(variant):
     du = [model.path_du(k).astype(np.float64) for k in range(len(steps))]
     f32, _, _ = P.finish_path(steps, path, du, temps["all"])
     assert np.abs(f32[125] - fields[125]).max() <= 1e-6 * np.abs(fields[125]).max()
+
+
+def test_path_blocks_follow_the_worker_farm_rule():
+    # runSweep (src/worker.js:114-117): max(2, ceil(n / (3 workers))) consecutive steps per block
+    assert P.path_blocks(44, 3) == [list(range(k, min(44, k + 5))) for k in range(0, 44, 5)]
+    assert P.path_blocks(5, 4) == [[0, 1], [2, 3], [4]]
+    assert P.path_blocks(7, 1) == [[0, 1, 2], [3, 4, 5], [6]]
+
+
+@pytest.mark.parametrize("variant", ["hostemu", pytest.param("hip", marks=pytest.mark.gpu)])
+def test_path_over_model_instances(variant):
+    """The plan's blocks over two model instances (their own warm-start bases
+    and streams, one host thread each) sum to the fields of the one-model run,
+    and with one instance the run is the one-model run bit for bit; an FP64
+    increment handed to another model (the partial message) sums the same."""
+    spec, model, steps, path, temps, step_of = _plate_path(variant)
+    ref, _, _, _ = P.run_path(model, steps, path, temps["all"], step_of, tol=1e-12, precision=hs.PREC_FP64)
+    one, _, _, _ = P.run_path_blocks([model], steps, path, temps["all"], step_of, tol=1e-12,
+                                     precision=hs.PREC_FP64)
+    for T in ref:
+        assert np.array_equal(one[T], ref[T]), T
+    other = hs.Model(meshspec.build_mesh(spec), variant=variant)
+    two, _, _, recs = P.run_path_blocks([model, other], steps, path, temps["all"], step_of, tol=1e-12,
+                                        precision=hs.PREC_FP64)
+    assert all(r["converged"] for r in recs)
+    scale = max(np.abs(v).max() for v in ref.values())  # (the field at the stress-free 217 C is ~0)
+    for T in ref:
+        assert np.abs(two[T] - ref[T]).max() <= 1e-9 * scale, T
+    # step 0's increment as a message: FP64 from the model that kept it into a fresh one
+    fresh = hs.Model(meshspec.build_mesh(spec), variant=variant)
+    fresh.path_reset()
+    d0 = model.path_sum([0])
+    fresh.path_load(0, d0)
+    assert np.array_equal(fresh.path_sum([0]), d0)
