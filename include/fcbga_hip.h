@@ -16,6 +16,9 @@
  * All functions return 0 on success and a negative code on failure; the
  * message is available from fcbga_last_error (per calling thread).
  * Arrays are borrowed for the duration of the call only.
+ * Each model (and submodel) has its own stream and state: calls on different
+ * models may run on different host threads at once, calls on one model may
+ * not (fcbga_path_copy also reads its source model, which must then be idle).
  */
 #ifndef FCBGA_HIP_H
 #define FCBGA_HIP_H
