@@ -1993,6 +1993,41 @@ void zero_vec(T *v, int64_t n, be::stream_t s) {
   be::pfor(n, s, [=] __host__ __device__(int64_t i) { v[i] = T(0); });
 }
 
+// The fine x and the x of every shell level above the coarsest start a cycle
+// at zero, and each level's x is first read by its own smoothing sweep: one
+// launch clears them all when the cycle starts.
+template <typename T>
+struct ZeroSegs {
+  static constexpr int MAXV = 8;
+  T *p[MAXV];
+  int64_t end[MAXV];  // running ends of the segments
+  int n;
+};
+
+template <typename T>
+void zero_cycle_vectors(fcbga_model *M, T *xf, int64_t n3, bool withCoarse, be::stream_t s) {
+  ZeroSegs<T> sg{};
+  int64_t tot = 0;
+  const auto add = [&](T *p, int64_t n) {
+    if (sg.n == ZeroSegs<T>::MAXV) {
+      zero_vec<T>(p, n, s);
+      return;
+    }
+    sg.p[sg.n] = p;
+    tot += n;
+    sg.end[sg.n++] = tot;
+  };
+  add(xf, n3);
+  if (withCoarse)
+    for (auto &D : M->lv)
+      if (!D->coarsest) add(LevelVec<T>::x(*D), 6 * (int64_t)D->nagg);
+  be::pfor(tot, s, [=] __host__ __device__(int64_t i) {
+    int q = 0;
+    while (i >= sg.end[q]) q++;
+    sg.p[q][i - (q ? sg.end[q - 1] : 0)] = T(0);
+  });
+}
+
 template <typename T>
 void shell_cycle(fcbga_model *M, int l) {
   DevShell &D = *M->lv[l];
@@ -2004,7 +2039,7 @@ void shell_cycle(fcbga_model *M, int l) {
     dense_matvec<T>(D.ndense, D.inv64.p, b, x, s);
     return;
   }
-  zero_vec<T>(x, 6 * (int64_t)D.nagg, s);
+  // x is zero here: fine_cycle clears it with the fine x when the cycle starts
   {
     KScope ks(M, l == 0 ? FCBGA_KCLASS_SHELL1_SWEEP : FCBGA_KCLASS_SHELL_SWEEP);
     for (int sw = 0; sw < M->nsweeps; sw++)
@@ -2067,7 +2102,7 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
   be::stream_t s = M->stream;
   FineOps<T> o = fine_ops<T>(M);
   const int64_t n3 = 3 * M->g.G;
-  zero_vec<T>(x, n3, s);
+  zero_cycle_vectors<T>(M, x, n3, withCoarse, s);
   for (int sw = 0; sw < M->nsweeps; sw++)
     for (int c = 0; c < 4; c++) fine_color_sweep<T>(M, o, M->fineColors[c].p, M->nFineColor[c], b, x, r, dx, s);
   if (withCoarse) {
