@@ -2097,6 +2097,47 @@ void shell_cycle(fcbga_model *M, int l) {
 }
 
 // z = M^-1 r on the fine grid (T precision), r and z are T vectors of 3G
+#if FCBGA_GPU
+// Restriction of the fine residual to shell level 0, one 64-lane block per
+// grid column: lane k loads plane k's scaled residual (coalesced along the
+// column) into LDS, then one lane per aggregate of the column adds its nodes
+// in node order, with the per-node terms of the one-thread-per-aggregate loop.
+template <typename T>
+__global__ void __launch_bounds__(64) k_restrict_fine(const int32_t *__restrict__ cst,
+                                                      const int32_t *__restrict__ k0,
+                                                      const int32_t *__restrict__ k1,
+                                                      const int32_t *__restrict__ nodeAgg,
+                                                      const T *__restrict__ res, const T *__restrict__ sinv,
+                                                      const T *__restrict__ zp, int nz, T *__restrict__ bc) {
+  __shared__ T s0[FINE_MAXNZ], s1[FINE_MAXNZ], s2[FINE_MAXNZ], sz[FINE_MAXNZ];
+  __shared__ int32_t sa[FINE_MAXNZ];
+  const int64_t c = blockIdx.x;
+  const int k = threadIdx.x;
+  if (k < nz) {
+    const int64_t gi = c * nz + k;
+    sa[k] = nodeAgg[gi];
+    s0[k] = res[3 * gi] * sinv[3 * gi];
+    s1[k] = res[3 * gi + 1] * sinv[3 * gi + 1];
+    s2[k] = res[3 * gi + 2] * sinv[3 * gi + 2];
+    sz[k] = zp[gi];
+  }
+  __syncthreads();
+  const int32_t a0 = cst[c], na = cst[c + 1] - a0;
+  for (int32_t q = k; q < na; q += 64) {
+    const int32_t A = a0 + q;
+    T acc[6] = {0, 0, 0, 0, 0, 0};
+    const int ka = k0[A], kb = k1[A];
+    for (int kk = ka; kk <= kb; kk++) {
+      if (sa[kk] != A) continue;
+      const T r0 = s0[kk], r1 = s1[kk], r2 = s2[kk], z = sz[kk];
+      acc[0] += r0; acc[1] += r1; acc[2] += r2;
+      acc[3] -= z * r1; acc[4] += z * r0; acc[5] += z * r2;
+    }
+    for (int a = 0; a < 6; a++) bc[6 * (int64_t)A + a] = acc[a];
+  }
+}
+#endif
+
 template <typename T>
 void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) {
   be::stream_t s = M->stream;
@@ -2116,6 +2157,13 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
     T *bc = LevelVec<T>::b(C);
     const int nz = M->g.nz;
     const T *res = r;
+#if FCBGA_GPU
+    if ((int64_t)C.ncol * nz == M->g.G && nz <= FINE_MAXNZ) {
+      hipLaunchKernelGGL(k_restrict_fine<T>, dim3(C.ncol), dim3(64), 0, s, C.colStart.p, k0, k1, nodeAgg, res, sinv,
+                         zp, nz, bc);
+      BE_CHECK(hipGetLastError());
+    } else
+#endif
     be::pfor(C.nagg, s, [=] __host__ __device__(int64_t A) {
       T acc[6] = {0, 0, 0, 0, 0, 0};
       const int64_t c = aggCol[A];
