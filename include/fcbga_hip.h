@@ -275,6 +275,11 @@ int fcbga_submodel_loop_sample(fcbga_submodel *s, double T, double E, double fso
 int fcbga_submodel_section(fcbga_submodel *s, double E, double fsolder, float *out);
 /* displacement (3 per grid node) */
 int fcbga_submodel_displacement(fcbga_submodel *s, double *u);
+/* committed material-point states, 8 x FCBGA_ANAND_STATE doubles per cell in cell order, the
+ * cell's Gauss points in the element's point order (the states of AnandPoint: ep[6], s, ebar,
+ * W, sigEq, and the Newton warm start; zero for cells without solder): what a host that keeps
+ * the reference's BumpSubmodel instance reads back into its own points after a device step */
+int fcbga_submodel_points(fcbga_submodel *s, double *st);
 
 /* Thermal path on the device (Analysis.run / finishPath, src/core/11-analysis.js:212-261): the
  * increment of each solved step stays in device memory (FP64); fields are sums of increments.

@@ -81,10 +81,10 @@ struct Univ {
 // zeta stations -1, 0, 1, and their Simpson sum 1/3, 4/3, 1/3 through the
 // thickness (exact for the trilinear + incompatible-mode products; the UNIV
 // table of src/core/09-assembly.js:26-60).
-const Univ &univ() {
-  static Univ U;
-  static bool done = false;
-  if (done) return U;
+// Built once behind a function-local static (thread-safe initialisation):
+// models created on several host threads at once share the one table.
+Univ make_univ() {
+  Univ U;
   std::memset(&U, 0, sizeof U);
   const int XI[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, ETA[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
             ZE[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
@@ -116,7 +116,11 @@ const Univ &univ() {
         }
     }
   }
-  done = true;
+  return U;
+}
+
+const Univ &univ() {
+  static const Univ U = make_univ();
   return U;
 }
 
@@ -1041,7 +1045,18 @@ void build_hierarchy(fcbga_model *M) {
     for (int i = 0; i < L.nx; i++)
       for (int j = 0; j < L.ny; j++) cols[(i & 1) * 2 + (j & 1)].push_back(i * L.ny + j);
     for (int c = 0; c < 4; c++) { D->colors[c].upload(cols[c], s); D->ncolor[c] = (int32_t)cols[c].size(); }
+    // what the column-walking kernels (restriction, sweeps, Galerkin) assume:
+    // aggregates of a column are contiguous and in order, and an aggregate of
+    // level 1 spans planes k0..k1 of its column inside the grid
+    for (int32_t c = 0; c < D->ncol; c++)
+      if (L.colStart[c + 1] < L.colStart[c]) throw be::Error("shell level: column starts not monotone");
+    if (L.colStart[0] != 0 || L.colStart[D->ncol] != L.nagg) throw be::Error("shell level: column starts do not cover the aggregates");
     if (l == 0) {
+      if ((int32_t)L.k0.size() != L.nagg || (int32_t)L.k1.size() != L.nagg)
+        throw be::Error("shell level 1: plane ranges missing");
+      for (int32_t a = 0; a < L.nagg; a++)
+        if (L.k0[a] < 0 || L.k0[a] > L.k1[a] || L.k1[a] >= M->g.nz)
+          throw be::Error("shell level 1: aggregate plane range outside its column");
       std::vector<int32_t> aggCol(L.nagg);
       for (int32_t c = 0; c < D->ncol; c++)
         for (int32_t a = L.colStart[c]; a < L.colStart[c + 1]; a++) aggCol[a] = c;
@@ -2798,9 +2813,14 @@ void iteration_body(fcbga_model *M, const fcbga_solve_opts &o, bool flex) {
   });
 }
 
+// One PCG iteration, replayed from a captured HIP graph per key.  IC(0)
+// iterations run eagerly: their level-scheduled triangular solves are one
+// launch per dependency level (about 2,000 on the lidded mesh), and a graph
+// of that size gains nothing over eager launches of a latency-bound apply
+// while it was the one graph form a profiled run crashed in.
 void run_iteration(fcbga_model *M, const fcbga_solve_opts &o, bool flex, int gkey) {
 #if FCBGA_GPU
-  if (M->useGraphs && !M->timing) {
+  if (M->useGraphs && !M->timing && o.precond != FCBGA_PC_IC) {
     auto it = M->graphs.find(gkey);
     if (it == M->graphs.end()) {
       hipGraph_t graph;
@@ -3043,7 +3063,10 @@ int pcg_iterate(fcbga_model *M, const fcbga_solve_opts &o, double bnorm, double 
   be::dot_async(n3, r, z, M->partials.p, sc + SC_RZ, s);
   // the FP32 cycles need the flexible form; IC(0) runs in FP64 whatever the precision
   const bool flex = o.precision == FCBGA_PREC_MIXED && o.precond != FCBGA_PC_IC;
-  const int gkey = o.precision * 64 + o.precond * 8 + M->nsweeps;
+  // every host-side launch choice baked into a captured iteration is part of
+  // the key: precision, preconditioner, sweeps and the element-table form
+  // (compact mirror-symmetric or full), which assemble_step decides per step
+  const int gkey = (M->symTables ? 4096 : 0) + o.precision * 64 + o.precond * 8 + M->nsweeps;
   int it;
   for (it = 1; it <= o.maxit; it++) {
     run_iteration(M, o, flex, gkey);
@@ -4247,6 +4270,16 @@ int fcbga_submodel_section(fcbga_submodel *s, double E, double fsolder, float *o
   if (!(E > 0)) { g_err = "E > 0 required"; return FCBGA_ERR_ARG; }
   try {
     sub_section(s, E, fsolder, out);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_submodel_points(fcbga_submodel *s, double *st) {
+  if (!s || !st) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    be::d2h(st, s->st.p, sizeof(double) * 8 * FCBGA_ANAND_STATE * (size_t)s->ne, s->stream);
   } catch (const std::exception &e) {
     return fail(e, FCBGA_ERR_DEVICE);
   }

@@ -16,6 +16,11 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBS = {"hip": os.path.join(HERE, "libfcbga_hip.so"), "hostemu": os.path.join(HERE, "libfcbga_hostemu.so")}
+# FCBGA_HOSTEMU_SANITIZE=1 swaps the host-emulation build for the same build
+# under the address and undefined-behaviour sanitizers (the process must then
+# preload libasan; tests/test_sanitize.py runs the host-emulation tests so)
+if os.environ.get("FCBGA_HOSTEMU_SANITIZE") == "1":
+    LIBS["hostemu"] = os.path.join(HERE, "libfcbga_hostemu_asan.so")
 
 PREC_FP64, PREC_MIXED = 0, 1
 PC_MG, PC_COLUMN, PC_JACOBI, PC_IC = 0, 1, 2, 3
@@ -81,6 +86,7 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_submodel_create", "fcbga_submodel_destroy", "fcbga_submodel_nboundary", "fcbga_submodel_step",
            "fcbga_submodel_save", "fcbga_submodel_restore", "fcbga_submodel_layer_means",
            "fcbga_submodel_loop_sample", "fcbga_submodel_section", "fcbga_submodel_displacement",
+           "fcbga_submodel_points",
            "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
            "fcbga_path_load64", "fcbga_path_copy",
            "fcbga_mesh_fingerprint", "fcbga_last_error"]

@@ -142,160 +142,178 @@ class HipSensitivity {
 }
 
 /**
- * screenJoints (src/core/13-fatigue.js:375-455) with the per-joint loop on the
- * device: the sites, the bump-layer strain table, the cycle history and the
- * per-step material data come from the reference's own functions, passed in
- * `core` ({screeningSites, siteStrainTable, findMaterial, cycleHistory,
- * AnandPoint, materialTables, gridAt, findInterval, clamp, CONST}); every
- * joint's AnandPoint.stepColumn / stepSeriesColumn march through the cycle runs
- * in one device thread (hip.screenJoints).  Returns the reference's record.
+ * Bump screening with the joint loop on the device.  The reference's
+ * screenJoints (src/core/13-fatigue.js:375-455) marches every joint through the
+ * cycle on one thread; here the host only prepares what is shared by all joints
+ * and hip.screenJoints runs one device thread per joint.  `core` supplies the
+ * reference's own helpers: {screeningSites, siteStrainTable, findMaterial,
+ * cycleHistory, AnandPoint, materialTables, gridAt, findInterval, CONST}.
+ * The returned function has screenJoints' signature and record.
  */
 function makeScreenJoints(core) {
   const hip = load();
-  return function screenJoints(an, d, layer, solderId, pillar) {
-    const t0 = Date.now();
-    const { sites, subsampled } = core.screeningSites(d, pillar ? core.CONST.SCREEN_MAX_DDI : null);
-    const temps = an.temps.cycling;
-    const table = core.siteStrainTable(an, null, sites, temps, layer);
-    const solder = core.findMaterial(an.cfg.materials, solderId);
-    const hist = core.cycleHistory(an.cfg.cycling);
-    const n = sites.x.length, nsteps = hist.t.length;
-    const pt = new core.AnandPoint(solder);
-    const st = {
-      nsteps, cycles: hist.cycles, nsites: n, fs: pillar ? pillar.fs : 1,
-      T: Float64Array.from(hist.T), dt: new Float64Array(nsteps), E: new Float64Array(nsteps), Fth: new Float64Array(nsteps),
-      wt: new Float64Array(nsteps), wi: new Int32Array(nsteps), cycle: Int32Array.from(hist.cycle), same: new Uint8Array(nsteps),
+  const PARAMS = ['A', 'QR', 'xi', 'm', 'shat', 'n', 'h0', 'a', 's0', 'nu'];
+
+  // everything the device needs per history point, from the history and the
+  // cycling temperatures of the strain table
+  function historyArrays(hist, temps, point) {
+    const nsteps = hist.T.length, f25 = point.thermalStrain(25);
+    const arr = {
+      nsteps, cycles: hist.cycles, T: Float64Array.from(hist.T), cycle: Int32Array.from(hist.cycle),
+      dt: new Float64Array(nsteps), E: new Float64Array(nsteps), Fth: new Float64Array(nsteps),
+      wt: new Float64Array(nsteps), wi: new Int32Array(nsteps), same: new Uint8Array(nsteps),
     };
-    for (let k = 0; k < nsteps; k++) {
-      const i = core.findInterval(temps, hist.T[k]);
-      st.wi[k] = i;
-      st.wt[k] = core.clamp((hist.T[k] - temps[i]) / (temps[i + 1] - temps[i]), 0, 1);
-      st.Fth[k] = pt.thermalStrain(hist.T[k]) - pt.thermalStrain(25);
-      st.E[k] = pt.E(hist.T[k]);
-      st.dt[k] = k > 0 ? hist.t[k] - hist.t[k - 1] : 0;
-      st.same[k] = k > 0 && hist.phase[k] === hist.phase[k - 1] ? 1 : 0;
+    hist.T.forEach((T, k) => {
+      const lo = core.findInterval(temps, T), span = temps[lo + 1] - temps[lo];
+      arr.wi[k] = lo;
+      arr.wt[k] = Math.min(1, Math.max(0, (T - temps[lo]) / span));
+      arr.E[k] = point.E(T);
+      arr.Fth[k] = point.thermalStrain(T) - f25;
+      if (k === 0) return;
+      arr.dt[k] = hist.t[k] - hist.t[k - 1];
+      arr.same[k] = hist.phase[k] === hist.phase[k - 1] ? 1 : 0;
+    });
+    return arr;
+  }
+
+  // an elastic pillar below the solder cap: compliances (1/E, 1/G, 1/G) and
+  // thermal strain from 25 C at every history point
+  function pillarArrays(arr, tables) {
+    const n = arr.nsteps, Cp = new Float64Array(3 * n), Fp = new Float64Array(n);
+    const F25 = core.gridAt(tables.F, 25);
+    for (let k = 0; k < n; k++) {
+      const E = core.gridAt(tables.E, arr.T[k]), G = E / (2 + 2 * tables.nu);
+      Cp.set([1 / E, 1 / G, 1 / G], 3 * k);
+      Fp[k] = core.gridAt(tables.F, arr.T[k]) - F25;
     }
+    arr.Cp = Cp;
+    arr.Fp = Fp;
+  }
+
+  // indices of the finite entries of v, largest first
+  function ranked(v) {
+    const idx = [];
+    for (let q = 0; q < v.length; q++) if (Number.isFinite(v[q])) idx.push(q);
+    return idx.sort((p, q) => v[q] - v[p]);
+  }
+
+  return function screenJoints(an, d, layer, solderId, pillar) {
+    const started = Date.now();
+    const picked = core.screeningSites(d, pillar ? core.CONST.SCREEN_MAX_DDI : null);
+    const sites = picked.sites, temps = an.temps.cycling;
+    const table = Float64Array.from(core.siteStrainTable(an, null, sites, temps, layer));
+    const point = new core.AnandPoint(core.findMaterial(an.cfg.materials, solderId));
+    const hist = core.cycleHistory(an.cfg.cycling);
+    const arr = historyArrays(hist, temps, point);
+    arr.nsites = sites.x.length;
+    arr.fs = 1;
     if (pillar) {
-      const pm = core.findMaterial(an.cfg.materials, pillar.mat), tb = core.materialTables(pm);
-      st.Cp = new Float64Array(3 * nsteps); st.Fp = new Float64Array(nsteps);
-      for (let k = 0; k < nsteps; k++) {
-        const Ep = core.gridAt(tb.E, hist.T[k]), Gp = Ep / (2 * (1 + tb.nu));
-        st.Cp[3 * k] = 1 / Ep; st.Cp[3 * k + 1] = st.Cp[3 * k + 2] = 1 / Gp;
-        st.Fp[k] = core.gridAt(tb.F, hist.T[k]) - core.gridAt(tb.F, 25);
-      }
+      arr.fs = pillar.fs;
+      pillarArrays(arr, core.materialTables(core.findMaterial(an.cfg.materials, pillar.mat)));
     }
-    const params = { A: pt.A, QR: pt.QR, xi: pt.xi, m: pt.m, shat: pt.shat, n: pt.n, h0: pt.h0, a: pt.a, s0: pt.s0, nu: pt.nu };
-    const r = hip.screenJoints(params, st, Float64Array.from(table), temps.indexOf(25));
-    const order = Array.from({ length: n }, (_, q) => q).filter(q => isFinite(r.dW[q])).sort((a, b) => r.dW[b] - r.dW[a]);
-    return { die: d.name, x: sites.x, y: sites.y, ix: sites.ix, iy: sites.iy, dW: r.dW, dE: r.dE, stab: r.stab, subsampled, nSites: n,
-      top: order.slice(0, 20).map(q => ({ q, x: sites.x[q], y: sites.y[q], dW: r.dW[q], dE: r.dE[q] })), ms: Date.now() - t0, cycles: hist.cycles };
+    const params = {};
+    for (const k of PARAMS) params[k] = point[k];
+    const res = hip.screenJoints(params, arr, table, temps.indexOf(25));
+    const top = ranked(res.dW).slice(0, 20).map(q => ({ q, x: sites.x[q], y: sites.y[q], dW: res.dW[q], dE: res.dE[q] }));
+    return {
+      die: d.name, x: sites.x, y: sites.y, ix: sites.ix, iy: sites.iy, dW: res.dW, dE: res.dE, stab: res.stab,
+      subsampled: picked.subsampled, nSites: sites.x.length, top, ms: Date.now() - started, cycles: hist.cycles,
+    };
   };
 }
 
 /**
- * BumpSubmodel.run (src/core/13-fatigue.js:963-1090) with every time step on
- * the device engine (hip.submodelCreate / submodelCall): `sm` is the
- * reference's BumpSubmodel instance (its mesh, boundary, material tables and
- * global fields are set up by the reference's constructor), `core` the core's
- * {cycleHistory, AnandPoint, lifeEstimates, CONST}.  The step material data
- * come from the instance's own _elastic and the boundary from its prescribe;
- * the engine does stepTo, the commit, the cutback snapshot, the layer means,
- * the hysteresis samples and the section snapshots.  Returns the reference's
- * record.  Install with
- *   BumpSubmodel.prototype.run = function () { return fh.runSubmodel(this, core); };
- * (the homogenized-layer and elastic-solder check modes keep the JS run).
+ * The device engine behind one of the reference's BumpSubmodel instances
+ * (src/core/13-fatigue.js:514-1105).  The instance keeps its own run() (the
+ * cycle, adaptive sub-stepping, cutbacks, hysteresis samples, layer averages
+ * and the record); attachSubmodel replaces only the instance methods that do
+ * the numerical work with device calls:
+ *   stepTo(T, dt)          the Newton time step (:924-961): element pass with
+ *                          the Anand points, tangent, IC(0)-PCG, commit
+ *   _snapshot(), _restore  the committed state kept for a cutback (:1065-1075)
+ *   _stressSnapshot(T)     corner stresses of every element (:1036-1062)
+ * After each device step the committed displacement and material-point
+ * states are written back into the instance (sm.u and the AnandPoint objects
+ * of sm.anand), which is what run() reads.  The step material data come from
+ * the instance's own _elastic, the boundary values from its prescribe.
+ * Returns detach(), which frees the engine and puts the methods back.
+ *   const detach = fh.attachSubmodel(sm, core);   // core: {CONST}
+ *   try { rec = sm.run(); } finally { detach(); }
  */
-function runSubmodel(sm, core) {
+function attachSubmodel(sm, core) {
   const hip = load();
-  const t0 = Date.now();
-  const m = sm.mesh, MAT = sm.MAT, g = sm.an.mesh.geom, C = core.CONST;
-  const nx = m.xs.length, ny = m.ys.length, nz = m.zs.length, ncell = (nx - 1) * (ny - 1) * (nz - 1);
-  const cellMat = new Int8Array(ncell), cellLayer = new Uint8Array(ncell);
-  const kind = [MAT.SUB, MAT.SOLDER, MAT.UF, MAT.SI];
-  for (let e = 0; e < m.ne; e++) {
-    const i = m.eIJK[3 * e], j = m.eIJK[3 * e + 1], k = m.eIJK[3 * e + 2], c = (i * (ny - 1) + j) * (nz - 1) + k;
-    cellMat[c] = kind.indexOf(m.eGroup[e]);
-    if (m.eGroup[e] === MAT.SOLDER) {
-      const zc = 0.5 * (m.zs[k] + m.zs[k + 1]);
-      cellLayer[c] = (zc > g.zaf - C.SUB_T_AVG ? 1 : 0) | (zc < g.zt + C.SUB_T_AVG ? 2 : 0);
-    }
-  }
-  const pt = new core.AnandPoint(sm.solderMat);
-  const h = hip.submodelCreate({ nx, ny, nz, xs: Float64Array.from(m.xs), ys: Float64Array.from(m.ys), zs: Float64Array.from(m.zs), cellMat, cellLayer },
-    { A: pt.A, QR: pt.QR, xi: pt.xi, m: pt.m, shat: pt.shat, n: pt.n, h0: pt.h0, a: pt.a, s0: pt.s0, nu: pt.nu });
-  try {
-    const nb = sm.bNodes.length, u = new Float64Array(sm.ndof);
-    let Tn = 25;
-    const stepTo = (T, dt) => {
-      const elasD = new Float64Array(144), elasSth = new Float64Array(24);
-      for (let q = 0; q < 4; q++) sm._elastic(kind[q], Tn, T, elasD.subarray(36 * q, 36 * q + 36), elasSth.subarray(6 * q, 6 * q + 6));
-      sm.prescribe(T, u);
-      const ub = new Float64Array(3 * nb);
-      for (let q = 0; q < nb; q++) { const n = sm.bNodes[q]; ub[3 * q] = u[3 * n]; ub[3 * q + 1] = u[3 * n + 1]; ub[3 * q + 2] = u[3 * n + 2]; }
-      const r = hip.submodelCall(h, 'step', { T, dt, E: pt.E(T), fsolder: pt.thermalStrain(T) - pt.thermalStrain(25), elasD, elasSth, ub });
-      if (r.converged) Tn = T;
-      return r;
-    };
-    const hist = core.cycleHistory(sm.cfg.cycling), nsteps = hist.t.length, nc = hist.cycles;
-    const phi = Math.atan2(sm.site.y - sm.die.cy, sm.site.x - sm.die.cx), cphi = Math.cos(phi), sphi = Math.sin(phi);
-    const loop = { t: [], T: [], tau: [], gamma: [], tauSub: [], gammaSub: [] };
-    const Wd = new Float64Array(nc + 1), Ws = new Float64Array(nc + 1), Ed = new Float64Array(nc + 1), Es = new Float64Array(nc + 1);
-    const sample = (T, t) => {
-      const s = hip.submodelCall(h, 'loopSample', T, pt.E(T), pt.thermalStrain(T) - pt.thermalStrain(25), cphi, sphi);
-      loop.t.push(t); loop.T.push(T); loop.tau.push(s[0]); loop.gamma.push(s[1]); loop.tauSub.push(s[2]); loop.gammaSub.push(s[3]);
-    };
-    let totalNewton = 0, cutbacks = 0, steps = 0, nsubNext = 1;
-    const snaps = [], snapOn = !sm.opts.uniformMaterial;
-    sample(25, 0);
-    for (let k = 1; k < nsteps; k++) {
-      if (sm.opts.cancelled && sm.opts.cancelled()) throw new Error('cancelled');
-      const T0 = hist.T[k - 1], T1 = hist.T[k], t0s = hist.t[k - 1], t1s = hist.t[k];
-      let nsub = nsubNext, done = false, dLast = 0;
-      for (let attempt = 0; attempt < 8 && !done; attempt++) {
-        hip.submodelCall(h, 'save');
-        const TnSave = Tn;
-        let ok = true;
-        for (let s = 1; s <= nsub; s++) {
-          const res = stepTo(T0 + (T1 - T0) * s / nsub, (t1s - t0s) / nsub);
-          totalNewton += res.newton; steps++;
-          dLast = res.dMax;
-          if (!res.converged || res.dMax > C.INELASTIC_STEP_MAX) { ok = false; break; }
-        }
-        if (ok) done = true;
-        else { hip.submodelCall(h, 'restore'); Tn = TnSave; nsub *= 2; cutbacks++; }
-      }
-      nsubNext = dLast > 0.5 * C.INELASTIC_STEP_MAX ? Math.min(nsub * 2, 64) : (dLast < 0.2 * C.INELASTIC_STEP_MAX ? Math.max(1, nsub >> 1) : nsub);
-      if (!done) throw new Error('Submodel step failed to converge after cutbacks at T = ' + T1.toFixed(1) + ' °C');
-      sample(T1, t1s);
-      const c = hist.cycle[k];
-      if (snapOn && c === nc && (k === nsteps - 1 || hist.phase[k + 1] !== hist.phase[k]))
-        snaps.push({ T: T1, t: t1s, cycle: c, phase: hist.phase[k], sig: hip.submodelCall(h, 'section', pt.E(T1), pt.thermalStrain(T1) - pt.thermalStrain(25)) });
-      const lm = hip.submodelCall(h, 'layerMeans');
-      Wd[c] = lm[0]; Ed[c] = lm[1]; Ws[c] = lm[2]; Es[c] = lm[3];
-      if (sm.opts.progress) sm.opts.progress(k / (nsteps - 1), 'Submodel step ' + k + '/' + (nsteps - 1) + ' at ' + T1.toFixed(0) + ' °C');
-    }
-    const dWdie = Wd[nc] - Wd[nc - 1], dWsub = Ws[nc] - Ws[nc - 1], dEdie = Ed[nc] - Ed[nc - 1], dEsub = Es[nc] - Es[nc - 1];
-    const dW = Math.max(dWdie, dWsub), dE = dWdie >= dWsub ? dEdie : dEsub;
-    const prev = nc >= 2 ? Math.max(Wd[nc - 1] - Wd[nc - 2], Ws[nc - 1] - Ws[nc - 2]) : NaN;
-    const stabChange = isFinite(prev) && prev > 0 ? Math.abs(dW - prev) / prev : NaN;
-    const perCycle = [];
-    for (let c = 1; c <= nc; c++) perCycle.push({ cycle: c, dWdie: Wd[c] - Wd[c - 1], dWsub: Ws[c] - Ws[c - 1] });
-    const prof = sm.die.profile, dJoint = dWdie >= dWsub ? prof.rt * 2 : prof.rb * 2;
-    return {
-      die: sm.die.name, site: { x: sm.site.x, y: sm.site.y, ix: sm.site.ix, iy: sm.site.iy }, nxy: sm.opts.nxy, nz: sm.opts.nz, nElem: m.ne, ndof: sm.ndof,
-      dW, dWdie, dWsub, dE, dEdie, dEsub, critical: dWdie >= dWsub ? 'die side' : 'substrate side', stabChange, stabWarn: isFinite(stabChange) && stabChange > C.STABILIZATION_WARN,
-      perCycle, loop, life: core.lifeEstimates(dW, dE, sm.cfg.fatigue, dJoint), dJoint, steps, cutbacks, newtonPerStep: totalNewton / Math.max(steps, 1), ms: Date.now() - t0,
-      section: snapOn ? {
-        xs: Float64Array.from(m.xs), ys: Float64Array.from(m.ys), zs: Float64Array.from(m.zs), grp: Int8Array.from(m.eGroup), cx: sm.site.x, cy: sm.site.y, zt: g.zt, zaf: g.zaf,
-        dieCx: sm.die.cx, dieCy: sm.die.cy, radii: { top: prof.rt, bottom: prof.rb, max: prof.rm },
-        groups: [{ name: 'Substrate slice', material: 'top ' + Math.round(C.SUB_MARGIN * 1000) + ' µm, homogenized' }, { name: 'Solder', material: sm.solderMat.name },
-          { name: 'Underfill', material: sm.ufMat ? sm.ufMat.name : '' }, { name: 'Silicon', material: '' }],
-        snaps,
-      } : null,
-    };
-  } finally {
+  const msh = sm.mesh, nx = msh.xs.length, ny = msh.ys.length, nz = msh.zs.length;
+  const ncell = (nx - 1) * (ny - 1) * (nz - 1), STATE_PT = 11;
+  const cellOf = new Int32Array(msh.ne);
+  for (let e = 0; e < msh.ne; e++) cellOf[e] = (msh.eIJK[3 * e] * (ny - 1) + msh.eIJK[3 * e + 1]) * (nz - 1) + msh.eIJK[3 * e + 2];
+  // engine material codes (FCBGA_SUB_*) of the instance's groups
+  const groupOfCode = [sm.MAT.SUB, sm.MAT.SOLDER, sm.MAT.UF, sm.MAT.SI];
+  const cellMat = new Int8Array(ncell).fill(-1);
+  for (let e = 0; e < msh.ne; e++) cellMat[cellOf[e]] = groupOfCode.indexOf(msh.eGroup[e]);
+  if (cellMat.some(c => c < 0)) throw new Error('attachSubmodel: every cell needs one element of a known material');
+  const probe = sm.anand.find(p => p);
+  if (!probe) throw new Error('attachSubmodel: the submodel has no solder points');
+  const params = {};
+  for (const k of ['A', 'QR', 'xi', 'm', 'shat', 'n', 'h0', 'a', 's0', 'nu']) params[k] = probe[k];
+  const f25 = probe.thermalStrain(25);
+  const h = hip.submodelCreate({
+    nx, ny, nz, xs: Float64Array.from(msh.xs), ys: Float64Array.from(msh.ys), zs: Float64Array.from(msh.zs),
+    cellMat, cellLayer: new Uint8Array(ncell),
+  }, params);
+  const nb = sm.bNodes.length;
+  if (hip.submodelCall(h, 'nboundary') !== nb) {
     hip.submodelCall(h, 'destroy');
+    throw new Error('attachSubmodel: boundary node count differs from the engine\'s');
   }
+  const scratch = new Float64Array(sm.ndof), ub = new Float64Array(3 * nb);
+  const elasD = new Float64Array(4 * 36), elasSth = new Float64Array(4 * 6);
+
+  // committed device state -> the instance
+  const pull = () => {
+    sm.u.set(hip.submodelCall(h, 'displacement'));
+    if (sm.uN) sm.uN.set(sm.u);
+    const st = hip.submodelCall(h, 'points');
+    sm.anand.forEach((pt, idx) => {
+      if (!pt) return;
+      const o = (8 * cellOf[idx >> 3] + (idx & 7)) * STATE_PT;
+      pt.ep.set(st.subarray(o, o + 6));
+      pt.s = st[o + 6]; pt.ebar = st[o + 7]; pt.W = st[o + 8]; pt.sigEq = st[o + 9]; pt._lastD = st[o + 10];
+    });
+  };
+
+  const saved = { stepTo: sm.stepTo, _snapshot: sm._snapshot, _restore: sm._restore, _stressSnapshot: sm._stressSnapshot };
+  sm.stepTo = function (T, dt) {
+    for (let c = 0; c < 4; c++)
+      this._elastic(groupOfCode[c], this.Tn, T, elasD.subarray(36 * c, 36 * c + 36), elasSth.subarray(6 * c, 6 * c + 6));
+    scratch.set(this.u);
+    this.prescribe(T, scratch);
+    this.bNodes.forEach((n, q) => ub.set(scratch.subarray(3 * n, 3 * n + 3), 3 * q));
+    const r = hip.submodelCall(h, 'step', { T, dt, E: probe.E(T), fsolder: probe.thermalStrain(T) - f25, elasD, elasSth, ub });
+    if (r.converged) {
+      this.Tn = T;
+      pull();
+    }
+    return { converged: r.converged, newton: r.newton, dMax: r.dMax };
+  };
+  sm._snapshot = function () {
+    hip.submodelCall(h, 'save');
+    return { Tn: this.Tn };
+  };
+  sm._restore = function (snap) {
+    hip.submodelCall(h, 'restore');
+    this.Tn = snap.Tn;
+    pull();
+  };
+  sm._stressSnapshot = function (T) {
+    const dev = hip.submodelCall(h, 'section', probe.E(T), probe.thermalStrain(T) - f25);
+    const out = new Float32Array(48 * msh.ne);
+    for (let e = 0; e < msh.ne; e++) out.set(dev.subarray(48 * cellOf[e], 48 * cellOf[e] + 48), 48 * e);
+    return out;
+  };
+  return function detach() {
+    Object.assign(sm, saved);
+    hip.submodelCall(h, 'destroy');
+  };
 }
 
 function install(Analysis, opts) {
@@ -319,4 +337,4 @@ function install(Analysis, opts) {
   return original;
 }
 
-module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, makeScreenJoints, runSubmodel, stepFromGroups, install, STATE };
+module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, makeScreenJoints, attachSubmodel, stepFromGroups, install, STATE };

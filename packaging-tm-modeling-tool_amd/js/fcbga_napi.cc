@@ -45,7 +45,7 @@ constexpr uint32_t MODEL_TAG = 0x4d4f444c, SUBMODEL_TAG = 0x5355424d;
 struct ModelBox {
   uint32_t tag = MODEL_TAG;
   fcbga_model *m = nullptr;
-  int32_t ngroups = 0, ne = 0;
+  int32_t ngroups = 0, ne = 0, nsections = 0;
 };
 
 struct SubBox {
@@ -156,6 +156,15 @@ napi_value CreateModel(napi_env env, napi_callback_info info) {
     m.constraint_dofs = (const int32_t *)p;
     m.nconstraints = (int32_t)n;
   }
+  // layered sections (FEModel: mesh.sections from substrateBands, src/core/09-assembly.js:217-219, 309):
+  // eSection (-1 homogeneous) and the number of sections
+  int32_t nsec = 0;
+  if (get_int(env, mesh, "nsections", &nsec) && nsec > 0) {
+    if (!get_typed(env, mesh, "eSection", napi_int32_array, &p, &n) || (int)n != m.ne)
+      return throw_msg(env, "mesh.eSection: Int32Array(ne) with nsections > 0");
+    m.e_section = (const int32_t *)p;
+    m.nsections = nsec;
+  }
   if (get_typed(env, mesh, "featureX", napi_int32_array, &p, &n)) { m.feature_x = (const int32_t *)p; m.nfeature_x = (int32_t)n; }
   if (get_typed(env, mesh, "featureY", napi_int32_array, &p, &n)) { m.feature_y = (const int32_t *)p; m.nfeature_y = (int32_t)n; }
   fcbga_model *model = nullptr;
@@ -164,6 +173,7 @@ napi_value CreateModel(napi_env env, napi_callback_info info) {
   box->m = model;
   box->ngroups = m.ngroups;
   box->ne = m.ne;
+  box->nsections = m.nsections;
   napi_value ext;
   if (napi_create_external(env, box, finalize_model, nullptr, &ext) != napi_ok) {
     fcbga_model_destroy(model);
@@ -259,6 +269,7 @@ void set_pc_fields(napi_env env, napi_value rec, const fcbga_solve_stats &S) {
 }
 
 napi_value solve_record(napi_env env, const fcbga_solve_stats &S);
+bool read_step(napi_env env, napi_value v, const ModelBox *b, fcbga_step *st, std::string *err);
 
 napi_value SolveStep(napi_env env, napi_callback_info info) {
   size_t argc = 3;
@@ -268,18 +279,8 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
   ModelBox *b = unwrap(env, argv[0]);
   if (!b || !b->m) return throw_msg(env, "solveStep: not a live model");
   fcbga_step st;
-  memset(&st, 0, sizeof st);
-  size_t n;
-  void *p;
-  get_double(env, argv[1], "Ta", &st.Ta);
-  get_double(env, argv[1], "Tb", &st.Tb);
-  get_int(env, argv[1], "stage", &st.stage);
-  if (!get_typed(env, argv[1], "D", napi_float64_array, &p, &n)) return throw_msg(env, "step.D: Float64Array(ngroups*36)");
-  st.D = (const double *)p;
-  if (!get_typed(env, argv[1], "s0", napi_float64_array, &p, &n)) return throw_msg(env, "step.s0: Float64Array(ngroups*6)");
-  st.s0 = (const double *)p;
-  if (!get_typed(env, argv[1], "born", napi_uint8_array, &p, &n)) return throw_msg(env, "step.born: Uint8Array(ngroups)");
-  st.born = (const uint8_t *)p;
+  std::string err;
+  if (!read_step(env, argv[1], b, &st, &err)) return throw_msg(env, "solveStep: " + err);
   fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
   if (argc >= 3 && !read_opts(env, argv[2], &o)) return throw_msg(env, "solveStep: unknown opts.precond");
   const int64_t ndof = fcbga_model_ndof(b->m);
@@ -320,20 +321,49 @@ napi_value solve_record(napi_env env, const fcbga_solve_stats &S) {
   return rec;
 }
 
-// step record {Ta, Tb, stage, D, s0, born} -> fcbga_step (the arrays are borrowed)
-bool read_step(napi_env env, napi_value v, fcbga_step *st, std::string *err) {
+// step record {Ta, Tb, stage, D, s0, born[, secW, secV, secBorn]} -> fcbga_step (the arrays are
+// borrowed); every array's length is checked against the model's groups and sections, since the
+// library reads that many values from each
+bool read_step(napi_env env, napi_value v, const ModelBox *b, fcbga_step *st, std::string *err) {
   memset(st, 0, sizeof *st);
   size_t n;
   void *p;
+  const size_t ng = (size_t)b->ngroups, ns = (size_t)b->nsections;
   get_double(env, v, "Ta", &st->Ta);
   get_double(env, v, "Tb", &st->Tb);
   get_int(env, v, "stage", &st->stage);
-  if (!get_typed(env, v, "D", napi_float64_array, &p, &n)) { *err = "step.D: Float64Array(ngroups*36)"; return false; }
+  if (!get_typed(env, v, "D", napi_float64_array, &p, &n) || n != 36 * ng) {
+    *err = "step.D: Float64Array(ngroups*36)";
+    return false;
+  }
   st->D = (const double *)p;
-  if (!get_typed(env, v, "s0", napi_float64_array, &p, &n)) { *err = "step.s0: Float64Array(ngroups*6)"; return false; }
+  if (!get_typed(env, v, "s0", napi_float64_array, &p, &n) || n != 6 * ng) {
+    *err = "step.s0: Float64Array(ngroups*6)";
+    return false;
+  }
   st->s0 = (const double *)p;
-  if (!get_typed(env, v, "born", napi_uint8_array, &p, &n)) { *err = "step.born: Uint8Array(ngroups)"; return false; }
+  if (!get_typed(env, v, "born", napi_uint8_array, &p, &n) || n != ng) {
+    *err = "step.born: Uint8Array(ngroups)";
+    return false;
+  }
   st->born = (const uint8_t *)p;
+  if (ns > 0) {
+    if (!get_typed(env, v, "secW", napi_float64_array, &p, &n) || n != 108 * ns) {
+      *err = "step.secW: Float64Array(nsections*108)";
+      return false;
+    }
+    st->secW = (const double *)p;
+    if (!get_typed(env, v, "secV", napi_float64_array, &p, &n) || n != 18 * ns) {
+      *err = "step.secV: Float64Array(nsections*18)";
+      return false;
+    }
+    st->secV = (const double *)p;
+    if (!get_typed(env, v, "secBorn", napi_uint8_array, &p, &n) || n != ns) {
+      *err = "step.secBorn: Uint8Array(nsections)";
+      return false;
+    }
+    st->sec_born = (const uint8_t *)p;
+  }
   return true;
 }
 
@@ -361,7 +391,7 @@ napi_value StateAccumulate(napi_env env, napi_callback_info info) {
   if (!b || !b->m) return throw_msg(env, "stateAccumulate: not a live model");
   fcbga_step st;
   std::string err;
-  if (!read_step(env, argv[1], &st, &err)) return throw_msg(env, "stateAccumulate: " + err);
+  if (!read_step(env, argv[1], b, &st, &err)) return throw_msg(env, "stateAccumulate: " + err);
   void *du, *el;
   size_t ndu, ne;
   if (!arg_typed(env, argv[2], napi_float64_array, &du, &ndu) || (int64_t)ndu != fcbga_model_ndof(b->m))
@@ -444,7 +474,7 @@ napi_value AdjointForward(napi_env env, napi_callback_info info) {
   if (!b || !b->m) return throw_msg(env, "adjointForward: not a live model");
   fcbga_step st;
   std::string err;
-  if (!read_step(env, argv[1], &st, &err)) return throw_msg(env, "adjointForward: " + err);
+  if (!read_step(env, argv[1], b, &st, &err)) return throw_msg(env, "adjointForward: " + err);
   fcbga_objective obj;
   memset(&obj, 0, sizeof obj);
   obj.p = 8;
@@ -719,6 +749,13 @@ napi_value SubmodelCall(napi_env env, napi_callback_info info) {
     if (fcbga_submodel_section(b->s, E, f, (float *)data) != FCBGA_OK) return throw_msg(env, "submodel section: " + lib_error());
     return arr;
   }
+  if (!strcmp(op, "points")) {
+    double *o;
+    napi_value arr = make_f64(env, 8 * FCBGA_ANAND_STATE * (size_t)b->ne, &o);
+    if (!arr) return throw_msg(env, "submodel: out of memory");
+    if (fcbga_submodel_points(b->s, o) != FCBGA_OK) return throw_msg(env, "submodel points: " + lib_error());
+    return arr;
+  }
   if (!strcmp(op, "displacement")) {
     double *o;
     napi_value arr = make_f64(env, 3 * (size_t)b->nn, &o);
@@ -764,7 +801,7 @@ napi_value PathCall(napi_env env, napi_callback_info info) {
     if (argc < 4) return throw_msg(env, "pathCall(model, 'solve', k, step[, opts])");
     fcbga_step st;
     std::string err;
-    if (!read_step(env, argv[3], &st, &err)) return throw_msg(env, err);
+    if (!read_step(env, argv[3], b, &st, &err)) return throw_msg(env, err);
     fcbga_solve_opts so{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
     if (argc >= 5 && !read_opts(env, argv[4], &so)) return throw_msg(env, "pathCall solve: unknown opts.precond");
     fcbga_solve_stats S;

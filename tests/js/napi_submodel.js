@@ -1,13 +1,16 @@
 'use strict';
-// Bump submodel through the JS host path: runSubmodel (fcbga_hip.js) -> Node-API
-// addon -> C ABI -> device, for the committed submodel case, against the
-// reference's record (submodel_golden.json).  The reference core is not
-// available where this runs, so `sm` is a stand-in for the reference's
-// BumpSubmodel instance built from the case's set-up data (mesh, boundary
-// nodes, material grids, boundary samples; _elastic and prescribe restated on
-// those grids) and `core` holds stand-ins for cycleHistory, AnandPoint,
-// lifeEstimates and the constants.
-//   node tests/js/napi_submodel.js [steps]   -> one JSON line
+// Bump submodel through the JS host path: attachSubmodel (fcbga_hip.js) puts the
+// device engine behind a BumpSubmodel instance (stepTo, _snapshot, _restore,
+// _stressSnapshot -> Node-API addon -> C ABI -> device) for the committed
+// submodel case, checked against the reference's record (submodel_golden.json).
+// The reference core is not available where this runs, so `sm` is a stand-in
+// for the reference's instance built from the case's set-up data (mesh,
+// boundary nodes, material grids, boundary samples; _elastic and prescribe
+// restated on those grids), and the cycle is driven by this test's own driver
+// (`drive` below) in place of the instance's run().  The driver reads the
+// layer work densities from the instance's material points, which the
+// adapter refreshes from the device after every step.
+//   node tests/js/napi_submodel.js   -> one JSON line
 const fs = require('fs');
 const path = require('path');
 const zlib = require('zlib');
@@ -53,11 +56,17 @@ const tab = { sE: f64('submodel_solderE.f64.gz'), sF: f64('submodel_solderF.f64.
   subD: f64('submodel_subD.f64.gz'), subF: f64('submodel_subF.f64.gz'), siD: f64('submodel_siD.f64.gz'), siF: f64('submodel_siF.f64.gz') };
 const bvals = f64('submodel_boundary.f64.gz'), nb = S.bNodes.length;
 const rowSum = (D, r) => D[6 * r] + D[6 * r + 1] + D[6 * r + 2];
+class Point {
+  constructor() { Object.assign(this, S.solder); this.ep = new Float64Array(6); this.s = S.solder.s0; this.ebar = 0; this.W = 0; this.sigEq = 0; this._lastD = 0; }
+  E(T) { return gridAt(tab.sE, T); }
+  thermalStrain(T) { return gridAt(tab.sF, T); }
+}
+const anand = [];
+for (let e = 0; e < S.ne; e++) for (let g = 0; g < 8; g++) anand.push(S.eGroup[e] === 1 ? new Point() : null);
 const sm = {
   mesh: { xs: S.xs, ys: S.ys, zs: S.zs, ne: S.ne, eGroup: S.eGroup, eIJK: S.eIJK }, MAT: { SUB: 0, SOLDER: 1, UF: 2, SI: 3 },
   an: { mesh: { geom: { zt: S.zt, zaf: S.zaf } } }, bNodes: S.bNodes, ndof: 3 * S.nn, site: S.site,
-  die: { name: 'die', cx: S.die.cx, cy: S.die.cy, profile: { rt: S.die.rt, rb: S.die.rb, rm: S.die.rt } },
-  cfg: { cycling: S.cycling, fatigue: S.fatigue }, opts: { nxy: 8, nz: 8 }, solderMat: { name: 'solder' }, ufMat: { name: 'underfill' },
+  u: new Float64Array(3 * S.nn), uN: new Float64Array(3 * S.nn), Tn: 25, anand,
   _elastic(mat, Ta, Tb, D, sth) {
     const iso = (E, F, nu) => {
       const Em = mean(E, 1, 1, Ta, Tb, new Float64Array(1))[0];
@@ -101,18 +110,79 @@ const history = cyc => {
   for (let k = 1; k <= cyc.cycles; k++) { ramp(T[T.length - 1], cyc.Tmax, k); hold(cyc.Tmax, k, 'dwell hot'); ramp(cyc.Tmax, cyc.Tmin, k); hold(cyc.Tmin, k, 'dwell cold'); }
   return { t: Float64Array.from(t), T: Float64Array.from(T), cycle: Int32Array.from(cycle), phase, cycles: cyc.cycles };
 };
-class Point {
-  constructor() { Object.assign(this, S.solder); }
-  E(T) { return gridAt(tab.sE, T); }
-  thermalStrain(T) { return gridAt(tab.sF, T); }
+
+// volume-weighted means of W and ebar over the solder points of the die-side
+// and substrate-side averaging layers (25 um bands), read from sm.anand
+const BAND = 0.025;
+const layers = (() => {
+  const die = [], sub = [];
+  for (let e = 0; e < S.ne; e++) {
+    if (S.eGroup[e] !== 1) continue;
+    const [i, j, k] = [S.eIJK[3 * e], S.eIJK[3 * e + 1], S.eIJK[3 * e + 2]];
+    const w = (S.xs[i + 1] - S.xs[i]) * (S.ys[j + 1] - S.ys[j]) * (S.zs[k + 1] - S.zs[k]) / 8, zc = (S.zs[k] + S.zs[k + 1]) / 2;
+    for (let g = 0; g < 8; g++) {
+      if (zc > S.zaf - BAND) die.push([8 * e + g, w]);
+      if (zc < S.zt + BAND) sub.push([8 * e + g, w]);
+    }
+  }
+  return { die, sub };
+})();
+const layerMean = (list, key) => list.reduce((a, [q, w]) => a + w * sm.anand[q][key], 0) / list.reduce((a, [, w]) => a + w, 0);
+
+// test driver: every history interval is split into nsub equal sub-steps; a
+// sub-step that fails or exceeds the plastic-increment limit sends the interval
+// back to the saved state with twice the sub-steps (8 tries); the next
+// interval starts from nsub doubled (above half the limit, at most 64), halved
+// (below a fifth) or kept
+function drive(hist, LIM) {
+  const nc = hist.cycles, n = hist.T.length;
+  const cyc = { Wd: new Float64Array(nc + 1), Ws: new Float64Array(nc + 1), Ed: new Float64Array(nc + 1), Es: new Float64Array(nc + 1) };
+  let nsub = 1, steps = 0, cutbacks = 0, snaps = 0;
+  const interval = (k, parts) => {
+    const T0 = hist.T[k - 1], dT = hist.T[k] - T0, dt = (hist.t[k] - hist.t[k - 1]) / parts;
+    let last = 0;
+    for (let s = 1; s <= parts; s++) {
+      const r = sm.stepTo(T0 + dT * s / parts, dt);
+      steps++;
+      last = r.dMax;
+      if (!r.converged || r.dMax > LIM) return { ok: false, last };
+    }
+    return { ok: true, last };
+  };
+  for (let k = 1; k < n; k++) {
+    let res = { ok: false, last: 0 };
+    for (let attempt = 0; attempt < 8; attempt++) {
+      const snap = sm._snapshot();
+      res = interval(k, nsub);
+      if (res.ok) break;
+      sm._restore(snap);
+      nsub *= 2;
+      cutbacks++;
+    }
+    if (!res.ok) throw new Error('no convergence at history point ' + k);
+    if (res.last > 0.5 * LIM) nsub = Math.min(2 * nsub, 64);
+    else if (res.last < 0.2 * LIM) nsub = Math.max(1, nsub >> 1);
+    const c = hist.cycle[k];
+    if (c === nc && (k === n - 1 || hist.phase[k + 1] !== hist.phase[k])) {
+      const sig = sm._stressSnapshot(hist.T[k]);
+      if (sig.length !== 48 * S.ne || !sig.every(Number.isFinite)) throw new Error('bad section snapshot');
+      snaps++;
+    }
+    cyc.Wd[c] = layerMean(layers.die, 'W'); cyc.Ws[c] = layerMean(layers.sub, 'W');
+    cyc.Ed[c] = layerMean(layers.die, 'ebar'); cyc.Es[c] = layerMean(layers.sub, 'ebar');
+  }
+  const d = a => a[nc] - a[nc - 1];
+  const dWdie = d(cyc.Wd), dWsub = d(cyc.Ws), dW = Math.max(dWdie, dWsub);
+  const prev = Math.max(cyc.Wd[nc - 1] - cyc.Wd[nc - 2], cyc.Ws[nc - 1] - cyc.Ws[nc - 2]);
+  return { steps, cutbacks, snaps, dW, dWsub, dE: dWdie >= dWsub ? d(cyc.Ed) : d(cyc.Es), stabChange: Math.abs(dW - prev) / prev };
 }
-const core = {
-  cycleHistory: history, AnandPoint: Point,
-  lifeEstimates: (dW, dE, fat) => ({ syedEnergy: dW > 0 ? 1 / (fat.CW * dW) : Infinity, syedStrain: dE > 0 ? 1 / (fat.Cstrain * dE) : Infinity, darveaux: null }),
-  CONST: { SUB_T_AVG: 0.025, INELASTIC_STEP_MAX: 2e-4, STABILIZATION_WARN: 0.05, SUB_MARGIN: 0.05 },
-};
-const r = fh.runSubmodel(sm, core);
+
+const t0 = Date.now();
+const detach = fh.attachSubmodel(sm, { CONST: { SUB_T_AVG: BAND } });
+let r;
+try { r = drive(history(S.cycling), 2e-4); } finally { detach(); }
 const ref = G.result;
 const rel = k => Math.abs(r[k] - ref[k]) / Math.abs(ref[k]);
 console.log(JSON.stringify({ steps: r.steps, cutbacks: r.cutbacks, ref_steps: ref.steps, ref_cutbacks: ref.cutbacks, dW_rel_err: rel('dW'), dE_rel_err: rel('dE'),
-  dWsub_rel_err: rel('dWsub'), stab_rel_err: rel('stabChange'), snaps: r.section.snaps.length, ref_snaps: ref.snaps.length, ms: r.ms }));
+  dWsub_rel_err: rel('dWsub'), stab_rel_err: rel('stabChange'), snaps: r.snaps, ref_snaps: ref.snaps.length, ms: Date.now() - t0,
+  detached: sm.stepTo === undefined && sm._snapshot === undefined }));

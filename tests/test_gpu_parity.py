@@ -118,6 +118,25 @@ def test_full_element_tables_vs_oracle(oracle, lib, monkeypatch, prec):
     assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
 
 
+@pytest.mark.parametrize("prec", [hs.PREC_FP64, hs.PREC_MIXED])
+def test_table_form_flip_on_one_model(oracle, lib, prec):
+    """One model solves a mirror-symmetric step (compact tables), then a step
+    with normal/shear coupling (full tables), then the symmetric step again.
+    The captured iteration graphs bake the table form into their launches, so
+    each solve must replay a graph of its own form: every solution is checked
+    against the oracle."""
+    spec = golden_spec("small_box")
+    m = meshspec.build_mesh(spec)
+    model = hs.Model(m, variant="hip")
+    sym = meshspec.group_step_data(spec, spec["Ta"], spec["Tb"], 0)
+    cpl = coupled_step(spec)
+    for (D, s0, born), want_sym in ((sym, 1), (cpl, 0), (sym, 1), (cpl, 0)):
+        u, rec = model.solve_step(hs.Step(spec["Ta"], spec["Tb"], 0, D, s0, born), tol=1e-12, precision=prec)
+        assert rec["sym_tables"] == want_sym
+        uo, _ = oracle.System(m, D, s0, born).solve(tol=1e-13, maxit=3000)
+        assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
+
+
 def test_error_codes(lib):
     spec = golden_spec("small_box")
     m = meshspec.build_mesh(spec)

@@ -86,10 +86,11 @@ def test_gpu_submodel_two_cycles_match_reference():
 
 @pytest.mark.gpu
 def test_node_addon_submodel_run():
-    """The JS host path of the submodel: node -> runSubmodel -> Node-API addon ->
-    C ABI -> device, with the step material data and the boundary from the
-    instance's own _elastic and prescribe (stand-ins here), against the
-    reference's record."""
+    """The JS host path of the submodel: attachSubmodel puts the device engine
+    behind a BumpSubmodel instance (node -> Node-API addon -> C ABI -> device),
+    with the step material data and the boundary from the instance's own
+    _elastic and prescribe (stand-ins here) and the layer work densities read
+    back from the instance's material points, against the reference's record."""
     import json
     import os
     import shutil
@@ -104,6 +105,7 @@ def test_node_addon_submodel_run():
     assert (rec["steps"], rec["cutbacks"], rec["snaps"]) == (rec["ref_steps"], rec["ref_cutbacks"], rec["ref_snaps"]), rec
     for k in ("dW_rel_err", "dE_rel_err", "dWsub_rel_err", "stab_rel_err"):
         assert rec[k] <= 1e-6, rec
+    assert rec["detached"], rec
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
