@@ -226,6 +226,18 @@ typedef struct fcbga_screen_steps {
  * cycles); NaN for sites without data. */
 int fcbga_screen_joints(const fcbga_anand_params *solder, const fcbga_screen_steps *steps, int64_t nsites,
                         int32_t ntemps, int32_t i25, const double *table, double *dW, double *dE, double *stab);
+/* fcbga_screen_joints without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The caller passes the strain table and the three outputs in device memory
+ * (nsites x ntemps x 6 and nsites doubles each) and the HIP stream (NULL: the null stream); the
+ * history arrays of steps stay on the host and are staged into a static device array of the library
+ * (one per device, held until the process ends), so nsteps may not exceed
+ * FCBGA_SCREEN_NOALLOC_MAX_STEPS (FCBGA_ERR_ARG otherwise).  Calls that overlap on one device share
+ * that array and race.  The results are complete on return and equal fcbga_screen_joints' bit for bit. */
+#define FCBGA_SCREEN_NOALLOC_MAX_STEPS 4096
+int fcbga_screen_joints_noalloc(const fcbga_anand_params *solder, const fcbga_screen_steps *steps, int64_t nsites,
+                                int32_t ntemps, int32_t i25, const double *table_dev, double *dW_dev,
+                                double *dE_dev, double *stab_dev, void *stream);
 
 /* Bump submodel (BumpSubmodel, src/core/13-fatigue.js:514-1105): one pitch cell around a
  * bump on a structured grid of B-bar hexahedra (one element per cell), the nodes on the six

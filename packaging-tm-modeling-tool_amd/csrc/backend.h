@@ -28,6 +28,9 @@
 #ifndef __global__
 #define __global__
 #endif
+#ifndef __align__
+#define __align__(n) alignas(n)
+#endif
 #ifndef __forceinline__
 #define __forceinline__ inline
 #endif

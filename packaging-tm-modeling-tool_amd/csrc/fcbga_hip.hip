@@ -20,6 +20,7 @@
 #include <cmath>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -3673,6 +3674,99 @@ int fail(const std::exception &e, int code) {
 // C ABI
 // ---------------------------------------------------------------------------
 
+namespace {
+
+// Validation shared by the two screening entry points.
+void check_screen_inputs(const fcbga_anand_params *solder, const fcbga_screen_steps *st, int32_t ntemps,
+                         int32_t i25) {
+  const int32_t n = st->nsteps;
+  if (n < 2 || st->cycles < 1 || ntemps < 2 || i25 < -1 || i25 >= ntemps)
+    throw be::Error("screening needs nsteps >= 2, cycles >= 1, ntemps >= 2 and i25 in -1..ntemps-1");
+  if (!st->T || !st->dt || !st->E || !st->Fth || !st->wt || !st->wi || !st->cycle || !st->same)
+    throw be::Error("screening step arrays missing");
+  if ((st->Cp == nullptr) != (st->Fp == nullptr)) throw be::Error("pillar needs both Cp and Fp");
+  if (!(st->fs > 0 && st->fs <= 1)) throw be::Error("solder height fraction fs must be in (0, 1]");
+  if (!(solder->nu < 0.5)) throw be::Error("nu < 0.5 required");
+  for (int32_t k = 0; k < n; k++) {
+    if (st->wi[k] < 0 || st->wi[k] > ntemps - 2) throw be::Error("table interval wi out of range");
+    if (st->cycle[k] < 0 || st->cycle[k] > st->cycles) throw be::Error("cycle index out of range");
+    if (k > 0 && !(st->E[k] > 0 && st->dt[k] >= 0)) throw be::Error("E > 0 and dt >= 0 required");
+  }
+}
+
+fe::AnandParams screen_params(const fcbga_anand_params *p) {
+  return fe::AnandParams{p->A, p->QR, p->xi, p->m, p->shat, p->n, p->h0, p->a, p->s0, p->nu};
+}
+
+// One device thread per joint through the whole cycle history.
+void launch_screen(const fe::AnandParams P, const fe::ScreenSteps h, int32_t i25, const double *tab, int64_t nsites,
+                   int32_t ntemps, double *dW, double *dE, double *stab, be::stream_t s) {
+  const int64_t stride = (int64_t)ntemps * 6;
+  be::pfor(nsites, s, [=] __host__ __device__(int64_t q) {
+    double r[3];
+    fe::screen_site(P, h, i25, tab + q * stride, r);
+    dW[q] = r[0];
+    dE[q] = r[1];
+    stab[q] = r[2];
+  }, 64);
+}
+
+// Scratch of the allocation-free screening entry point: the per-step history
+// arrays staged for the kernel, FCBGA_SCREEN_NOALLOC_MAX_STEPS steps at most
+// (9 doubles, 2 int32 and 1 byte per step: 332 KB).  One copy per device,
+// held until the process ends; every call on a device reuses it, so calls
+// that overlap on one device race.
+constexpr size_t SCR_STEPS = FCBGA_SCREEN_NOALLOC_MAX_STEPS;
+constexpr size_t SCR_BYTES = SCR_STEPS * (9 * sizeof(double) + 2 * sizeof(int32_t) + 1);
+static __device__ __align__(256) unsigned char scratch[SCR_BYTES];
+
+struct ScreenScratch {
+  double *T, *dt, *E, *Fth, *wt, *Cp, *Fp;
+  int32_t *wi, *cycle;
+  uint8_t *same;
+};
+
+// The device address of this device's scratch array, looked up once per
+// device (with that device current) and kept: the array's name is a host-side
+// stand-in, never a device address.
+unsigned char *scratch_base() {
+#if FCBGA_GPU
+  constexpr int MAXDEV = 64;
+  static unsigned char *addr[MAXDEV] = {};
+  static std::mutex mu;
+  int dev = 0;
+  BE_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAXDEV) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+    addr[dev] = (unsigned char *)p;
+  }
+  return addr[dev];
+#else
+  return scratch;
+#endif
+}
+
+// The scratch array cut into the history arrays of n steps (each 8-byte aligned).
+ScreenScratch screen_scratch(int32_t n) {
+  unsigned char *b = scratch_base();
+  ScreenScratch w;
+  double *d = (double *)b;
+  w.T = d; w.dt = d + SCR_STEPS; w.E = d + 2 * SCR_STEPS; w.Fth = d + 3 * SCR_STEPS; w.wt = d + 4 * SCR_STEPS;
+  w.Cp = d + 5 * SCR_STEPS; w.Fp = d + 8 * SCR_STEPS;
+  int32_t *ii = (int32_t *)(d + 9 * SCR_STEPS);
+  w.wi = ii; w.cycle = ii + SCR_STEPS;
+  w.same = (uint8_t *)(ii + 2 * SCR_STEPS);
+  (void)n;
+  return w;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fcbga_model_create(const fcbga_mesh *mesh, fcbga_model **out) {
@@ -4102,19 +4196,8 @@ int fcbga_screen_joints(const fcbga_anand_params *solder, const fcbga_screen_ste
   if (!solder || !st || nsites < 0 || !table || !dW || !dE || !stab) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
   if (nsites == 0) return FCBGA_OK;
   try {
+    check_screen_inputs(solder, st, ntemps, i25);
     const int32_t n = st->nsteps;
-    if (n < 2 || st->cycles < 1 || ntemps < 2 || i25 < -1 || i25 >= ntemps)
-      throw be::Error("screening needs nsteps >= 2, cycles >= 1, ntemps >= 2 and i25 in -1..ntemps-1");
-    if (!st->T || !st->dt || !st->E || !st->Fth || !st->wt || !st->wi || !st->cycle || !st->same)
-      throw be::Error("screening step arrays missing");
-    if ((st->Cp == nullptr) != (st->Fp == nullptr)) throw be::Error("pillar needs both Cp and Fp");
-    if (!(st->fs > 0 && st->fs <= 1)) throw be::Error("solder height fraction fs must be in (0, 1]");
-    if (!(solder->nu < 0.5)) throw be::Error("nu < 0.5 required");
-    for (int32_t k = 0; k < n; k++) {
-      if (st->wi[k] < 0 || st->wi[k] > ntemps - 2) throw be::Error("table interval wi out of range");
-      if (st->cycle[k] < 0 || st->cycle[k] > st->cycles) throw be::Error("cycle index out of range");
-      if (k > 0 && !(st->E[k] > 0 && st->dt[k] >= 0)) throw be::Error("E > 0 and dt >= 0 required");
-    }
     be::stream_t s = nullptr;
 #if FCBGA_GPU
     BE_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -4129,24 +4212,55 @@ int fcbga_screen_joints(const fcbga_anand_params *solder, const fcbga_screen_ste
     dout.alloc(3 * (size_t)nsites);
     fe::ScreenSteps h{n, st->cycles, dT.p, ddt.p, dEm.p, dF.p, dwt.p, st->Cp ? dCp.p : nullptr,
                       st->Cp ? dFp.p : nullptr, dwi.p, dcy.p, dsame.p, st->fs};
-    const fe::AnandParams P{solder->A, solder->QR, solder->xi, solder->m, solder->shat, solder->n, solder->h0,
-                            solder->a, solder->s0, solder->nu};
-    const double *tab = dtab.p;
-    double *out = dout.p;
-    const int64_t stride = (int64_t)ntemps * 6;
-    be::pfor(nsites, s, [=] __host__ __device__(int64_t q) {
-      double r[3];
-      fe::screen_site(P, h, i25, tab + q * stride, r);
-      out[q] = r[0];
-      out[nsites + q] = r[1];
-      out[2 * nsites + q] = r[2];
-    }, 64);
-    be::d2h(dW, out, sizeof(double) * nsites, s);
-    be::d2h(dE, out + nsites, sizeof(double) * nsites, s);
-    be::d2h(stab, out + 2 * nsites, sizeof(double) * nsites, s);
+    launch_screen(screen_params(solder), h, i25, dtab.p, nsites, ntemps, dout.p, dout.p + nsites,
+                  dout.p + 2 * nsites, s);
+    be::d2h(dW, dout.p, sizeof(double) * nsites, s);
+    be::d2h(dE, dout.p + nsites, sizeof(double) * nsites, s);
+    be::d2h(stab, dout.p + 2 * nsites, sizeof(double) * nsites, s);
 #if FCBGA_GPU
     (void)hipStreamDestroy(s);
 #endif
+  } catch (const be::Error &e) {
+    return fail(e, FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_screen_joints_noalloc(const fcbga_anand_params *solder, const fcbga_screen_steps *st, int64_t nsites,
+                                int32_t ntemps, int32_t i25, const double *table_dev, double *dW_dev,
+                                double *dE_dev, double *stab_dev, void *stream) {
+  if (!solder || !st || nsites < 0 || !table_dev || !dW_dev || !dE_dev || !stab_dev) {
+    g_err = "NULL argument";
+    return FCBGA_ERR_ARG;
+  }
+  try {
+    check_screen_inputs(solder, st, ntemps, i25);
+    const int32_t n = st->nsteps;
+    if (n > FCBGA_SCREEN_NOALLOC_MAX_STEPS)
+      throw be::Error("fcbga_screen_joints_noalloc: nsteps exceeds FCBGA_SCREEN_NOALLOC_MAX_STEPS (" +
+                      std::to_string(FCBGA_SCREEN_NOALLOC_MAX_STEPS) + ")");
+    if (nsites == 0) return FCBGA_OK;
+    be::stream_t s = (be::stream_t)stream;
+    const ScreenScratch w = screen_scratch(n);
+    // stage the history arrays in the scratch array (copies only, no allocation)
+    be::h2d(w.T, st->T, sizeof(double) * n, s);
+    be::h2d(w.dt, st->dt, sizeof(double) * n, s);
+    be::h2d(w.E, st->E, sizeof(double) * n, s);
+    be::h2d(w.Fth, st->Fth, sizeof(double) * n, s);
+    be::h2d(w.wt, st->wt, sizeof(double) * n, s);
+    if (st->Cp) {
+      be::h2d(w.Cp, st->Cp, sizeof(double) * 3 * n, s);
+      be::h2d(w.Fp, st->Fp, sizeof(double) * n, s);
+    }
+    be::h2d(w.wi, st->wi, sizeof(int32_t) * n, s);
+    be::h2d(w.cycle, st->cycle, sizeof(int32_t) * n, s);
+    be::h2d(w.same, st->same, n, s);
+    fe::ScreenSteps h{n, st->cycles, w.T, w.dt, w.E, w.Fth, w.wt, st->Cp ? w.Cp : nullptr,
+                      st->Cp ? w.Fp : nullptr, w.wi, w.cycle, w.same, st->fs};
+    launch_screen(screen_params(solder), h, i25, table_dev, nsites, ntemps, dW_dev, dE_dev, stab_dev, s);
+    be::sync(s);
   } catch (const be::Error &e) {
     return fail(e, FCBGA_ERR_ARG);
   } catch (const std::exception &e) {

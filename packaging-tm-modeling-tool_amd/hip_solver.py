@@ -86,7 +86,7 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_submodel_create", "fcbga_submodel_destroy", "fcbga_submodel_nboundary", "fcbga_submodel_step",
            "fcbga_submodel_save", "fcbga_submodel_restore", "fcbga_submodel_layer_means",
            "fcbga_submodel_loop_sample", "fcbga_submodel_section", "fcbga_submodel_displacement",
-           "fcbga_submodel_points",
+           "fcbga_submodel_points", "fcbga_screen_joints_noalloc",
            "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
            "fcbga_path_load64", "fcbga_path_copy",
            "fcbga_mesh_fingerprint", "fcbga_last_error"]
@@ -158,6 +158,10 @@ def load(variant: str = "hip"):
     L.fcbga_screen_joints.argtypes = [C.POINTER(FcbgaAnandParams), C.POINTER(FcbgaScreenSteps), C.c_int64,
                                       C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _f64p]
     L.fcbga_screen_joints.restype = C.c_int
+    L.fcbga_screen_joints_noalloc.argtypes = [C.POINTER(FcbgaAnandParams), C.POINTER(FcbgaScreenSteps), C.c_int64,
+                                              C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p]
+    L.fcbga_screen_joints_noalloc.restype = C.c_int
     L.fcbga_path_reset.argtypes = [C.c_void_p]
     L.fcbga_path_reset.restype = C.c_int
     L.fcbga_path_solve.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FcbgaStep), C.POINTER(FcbgaSolveOpts),
@@ -489,16 +493,10 @@ def anand_update(params, eps, T, dt, E, state, tangent=True, variant="hip"):
     return trial, sig, D, dep
 
 
-def screen_joints(params, steps, table, i25, variant="hip"):
-    """Bump screening loop on the device (fcbga_screen_joints): params the
-    solder's FcbgaAnandParams, steps the per-step arrays of
-    screening.step_data, table (nsites, ntemps, 6) bump-layer strains, i25 the
-    row of 25 C (-1: none).  Returns per-site dW, dE, stab."""
-    L = load(variant)
-    tab = np.ascontiguousarray(table, np.float64)
-    nsites, ntemps = tab.shape[0], tab.shape[1]
-    keep = {}
+SCREEN_NOALLOC_MAX_STEPS = 4096
 
+
+def _screen_steps_struct(steps, keep):
     def ptr(k, dt, t):
         v = steps[k]
         if v is None:
@@ -513,6 +511,35 @@ def screen_joints(params, steps, table, i25, variant="hip"):
     h.wi = ptr("wi", np.int32, _i32p)
     h.cycle = ptr("cycle", np.int32, _i32p)
     h.same = ptr("same", np.uint8, C.POINTER(C.c_uint8))
+    return h
+
+
+def screen_joints_noalloc(params, steps, nsites, ntemps, i25, table_ptr, dW_ptr, dE_ptr, stab_ptr, stream=None,
+                          variant="hip"):
+    """fcbga_screen_joints_noalloc: the screening loop with the strain table
+    and the outputs already in device memory (addresses as ints; host
+    addresses for the host-emulation build) and no device allocation; the
+    history arrays of steps are staged in the library's static scratch."""
+    L = load(variant)
+    keep = {}
+    h = _screen_steps_struct(steps, keep)
+    rc = L.fcbga_screen_joints_noalloc(C.byref(params), C.byref(h), int(nsites), int(ntemps), int(i25),
+                                       C.c_void_p(table_ptr), C.c_void_p(dW_ptr), C.c_void_p(dE_ptr),
+                                       C.c_void_p(stab_ptr), C.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise SolverError(last_error(L))
+
+
+def screen_joints(params, steps, table, i25, variant="hip"):
+    """Bump screening loop on the device (fcbga_screen_joints): params the
+    solder's FcbgaAnandParams, steps the per-step arrays of
+    screening.step_data, table (nsites, ntemps, 6) bump-layer strains, i25 the
+    row of 25 C (-1: none).  Returns per-site dW, dE, stab."""
+    L = load(variant)
+    tab = np.ascontiguousarray(table, np.float64)
+    nsites, ntemps = tab.shape[0], tab.shape[1]
+    keep = {}
+    h = _screen_steps_struct(steps, keep)
     dW, dE, stab = np.zeros(nsites), np.zeros(nsites), np.zeros(nsites)
     rc = L.fcbga_screen_joints(C.byref(params), C.byref(h), nsites, ntemps, int(i25), tab.ctypes.data_as(_f64p),
                                dW.ctypes.data_as(_f64p), dE.ctypes.data_as(_f64p), stab.ctypes.data_as(_f64p))

@@ -254,3 +254,31 @@ def test_node_addon_screening(case):
     assert out.returncode == 0, out.stderr
     rec = json.loads(out.stdout.strip().splitlines()[-1])
     assert rec["dW_rel_err"] <= 1e-9 and rec["nan_ok"] and rec["top_ok"], rec
+
+
+def test_noalloc_entry_point_host_emulation():
+    """fcbga_screen_joints_noalloc in the host-emulation build (its 'device'
+    buffers are host arrays there): two calls in a row, the second smaller and
+    without the pillar, equal bit for bit to fcbga_screen_joints."""
+    from fcbga_amd import hip_solver as hs
+    g = golden()
+    P = anand_params(g)
+    cases = {c["name"]: c for c in g["cases"]}
+    _, s1, t1, i1 = case_inputs(g, cases["ddi2"])
+    _, s2, t2, i2 = case_inputs(g, cases["bump2"])
+    t2 = np.ascontiguousarray(t2[: t2.shape[0] // 2])
+    for steps, table, i25 in ((s1, t1, i1), (s2, t2, i2)):
+        ref = hs.screen_joints(P, steps, table, i25, variant="hostemu")
+        tab = np.ascontiguousarray(table)
+        out = [np.full(tab.shape[0], -1.0) for _ in range(3)]
+        hs.screen_joints_noalloc(P, steps, tab.shape[0], tab.shape[1], i25, tab.ctypes.data, out[0].ctypes.data,
+                                 out[1].ctypes.data, out[2].ctypes.data, variant="hostemu")
+        for a, b in zip(out, ref):
+            np.testing.assert_array_equal(a, b)
+    long_steps = dict(s2, nsteps=hs.SCREEN_NOALLOC_MAX_STEPS + 1)
+    for k in ("T", "dt", "E", "Fth", "wt", "wi", "cycle", "same"):
+        v = np.asarray(s2[k])
+        long_steps[k] = np.resize(v, hs.SCREEN_NOALLOC_MAX_STEPS + 1).astype(v.dtype)
+    with pytest.raises(hs.SolverError, match="FCBGA_SCREEN_NOALLOC_MAX_STEPS"):
+        hs.screen_joints_noalloc(P, long_steps, 1, t2.shape[1], i2, t2.ctypes.data, out[0].ctypes.data,
+                                 out[1].ctypes.data, out[2].ctypes.data, variant="hostemu")
