@@ -293,6 +293,59 @@ int fcbga_submodel_displacement(fcbga_submodel *s, double *u);
  * the reference's BumpSubmodel instance reads back into its own points after a device step */
 int fcbga_submodel_points(fcbga_submodel *s, double *st);
 
+/* Global viscoplastic Newton (the bump-array cycling workload): a package mesh whose solder group
+ * holds discrete bumps with Anand points at the 2x2x2 Gauss points of B-bar elements
+ * (AnandPoint.step, src/core/13-fatigue.js:67-143; bbarMatrix, :474-490), the rest of the mesh the
+ * step solve's elastic elements.  Each time step is a Newton solve for the total displacement
+ * (the time step of BumpSubmodel.stepTo, :924-961: residual and increment tolerances, inexact
+ * inner tolerance, a final pass at the converged displacement, then the points are committed) with
+ * the step solve's flexible PCG and multigrid built on the current tangent as the inner solver.
+ * The reference has no global model of this kind; its Anand points live in the bump submodel. */
+typedef struct fcbga_vp fcbga_vp;
+typedef struct fcbga_vp_solder {
+  int32_t group;              /* mesh group of the solder elements (one element per cell, scale 1) */
+  fcbga_anand_params anand;   /* Anand constants; nu is the solder's Poisson ratio */
+  int32_t nE;                 /* E(T) table of the solder: temperatures (C) and moduli (MPa), */
+  const double *E_T, *E_v;    /* linear between and beyond the points */
+  double cte;                 /* solder CTE (1/K) */
+  double Tref;                /* stress-free temperature of the solder and start of the history (C) */
+} fcbga_vp_solder;
+typedef struct fcbga_vp_opts {
+  fcbga_solve_opts inner;     /* inner PCG: precision, preconditioner, maxit (tol and warm are set per iteration) */
+  int32_t maxit;              /* Newton iterations (the reference's SUB_NEWTON_MAXIT is 25) */
+  double res_tol;             /* |r| / (|f_th| + |f_solder|) (SUB_RES_TOL 1e-6) */
+  double du_tol;              /* |du| / |u| (SUB_DU_TOL 1e-8) */
+  double inner_floor;         /* floor of the inexact inner tolerance (SUB_INNER_TOL 1e-8) */
+} fcbga_vp_opts;
+typedef struct fcbga_vp_stats {
+  int32_t converged, newton, pcg_iters;
+  int32_t nonfinite;          /* 1: a point's stress turned non-finite (the caller cuts the step back) */
+  double dmax;                /* largest equivalent plastic increment of the step */
+  double res;                 /* last relative residual */
+  double ms, ms_setup, ms_pcg;
+} fcbga_vp_stats;
+int fcbga_vp_create(const fcbga_mesh *mesh, const fcbga_vp_solder *solder, fcbga_vp **out);
+void fcbga_vp_destroy(fcbga_vp *v);
+int32_t fcbga_vp_nsolder(const fcbga_vp *v);
+/* One time step to temperature T over dt (s).  elastic: the step data of every group for the
+ * total load from the stress-free state (Ta = Tref, Tb = T; the solder group's D is its elastic
+ * stiffness and its s0 is ignored).  A step that does not converge leaves the points uncommitted;
+ * the caller restores the saved state and cuts the step. */
+int fcbga_vp_step(fcbga_vp *v, const fcbga_step *elastic, double T, double dt, const fcbga_vp_opts *opts,
+                  fcbga_vp_stats *stats);
+int fcbga_vp_save(fcbga_vp *v);
+int fcbga_vp_restore(fcbga_vp *v);
+/* committed point states, 8 x FCBGA_ANAND_STATE per solder element in the order of elems (the mesh
+ * element of each solder element, nsolder entries), the points of an element in corner order */
+int fcbga_vp_points(fcbga_vp *v, double *st, int32_t *elems);
+/* total displacement (3*nn, the reference's node order) */
+int fcbga_vp_displacement(fcbga_vp *v, double *u);
+/* test hooks: replace the displacement; evaluate at the current displacement without committing:
+ * the out-of-balance force r (3*nn) and, for x (3*nn, optional), y = K_t x with the tangent there */
+int fcbga_vp_set_displacement(fcbga_vp *v, const double *u);
+int fcbga_vp_residual(fcbga_vp *v, const fcbga_step *elastic, double T, double dt, double *r, const double *x,
+                      double *y);
+
 /* Thermal path on the device (Analysis.run / finishPath, src/core/11-analysis.js:212-261): the
  * increment of each solved step stays in device memory (FP64); fields are sums of increments.
  * fcbga_path_solve solves step k of the plan (as fcbga_step_solve, opts->warm carrying the

@@ -1704,8 +1704,20 @@ static double an_dsdd(const double *P, double d, double s, double dt, double exp
 
 /* P = {A, QR, xi, m, shat, n, h0, a, s0, nu}; st (11) committed state with the
  * warm start st[10] updated in place; tr (11) trial state; returns d. */
+/* robust = 0: the reference's iteration; robust = 1: a non-positive effective
+ * stress (an iterate at or past the bracket end d = sigma_trial / 3G) counts as
+ * no flow, g(d) = d > 0, instead of the NaN branch that raises the lower
+ * bracket (the rule of the global viscoplastic Newton) */
+static double an_step(const double *P, double E, double T, double dt, const double *eps, double *st, double *tr,
+                      double *sig, double *D, int robust);
+
 double or_anand_step(const double *P, double E, double T, double dt, const double *eps, double *st, double *tr,
                      double *sig, double *D) {
+  return an_step(P, E, T, dt, eps, st, tr, sig, D, 0);
+}
+
+static double an_step(const double *P, double E, double T, double dt, const double *eps, double *st, double *tr,
+                      double *sig, double *D, int robust) {
   const double A = P[0], QR = P[1], xi = P[2], m = P[3], nu = P[9];
   double G = E / (2 * (1 + nu)), K = E / (3 * (1 - 2 * nu));
   double expQ = exp(QR / (T + 273.15)), rate0 = dt * A / expQ;
@@ -1723,8 +1735,14 @@ double or_anand_step(const double *P, double E, double T, double dt, const doubl
     for (int it = 0; it < 60; it++) {
       rs = an_s(P, st[6], dd, dt, expQ);
       rx = xi * (sigTr - 3 * G * dd) / rs;
-      rsh = sinh(rx);
-      double g = dd - rate0 * pow(rsh, 1 / m);
+      double g;
+      if (robust && !(rx > 0)) {
+        rsh = 0;
+        g = dd;
+      } else {
+        rsh = sinh(rx);
+        g = dd - rate0 * pow(rsh, 1 / m);
+      }
       if (!isfinite(g)) { lo = dd; dd = 0.5 * (lo + hi); continue; }
       if (fabs(g) < 1e-11 * fmax(dmax, 1e-30)) break;
       if (g > 0) hi = dd; else lo = dd;
@@ -1759,6 +1777,12 @@ void or_anand_batch(const double *P, int64_t n, double E, double T, double dt, c
                     double *tr, double *sig, double *D, double *dep) {
   for (int64_t q = 0; q < n; q++)
     dep[q] = or_anand_step(P, E, T, dt, eps + 6 * q, st + 11 * q, tr + 11 * q, sig + 6 * q, D ? D + 36 * q : NULL);
+}
+
+void or_anand_batch_robust(const double *P, int64_t n, double E, double T, double dt, const double *eps, double *st,
+                           double *tr, double *sig, double *D, double *dep) {
+  for (int64_t q = 0; q < n; q++)
+    dep[q] = an_step(P, E, T, dt, eps + 6 * q, st + 11 * q, tr + 11 * q, sig + 6 * q, D ? D + 36 * q : NULL, 1);
 }
 
 /* ------------------------------------------------------------------------- */

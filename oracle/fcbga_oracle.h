@@ -115,6 +115,8 @@ double or_anand_step(const double *P, double E, double T, double dt, const doubl
                      double *sig, double *D);
 void or_anand_batch(const double *P, int64_t n, double E, double T, double dt, const double *eps, double *st,
                     double *tr, double *sig, double *D, double *dep);
+void or_anand_batch_robust(const double *P, int64_t n, double E, double T, double dt, const double *eps, double *st,
+                    double *tr, double *sig, double *D, double *dep); /* the global Newton's return map (bracket-end guard) */
 
 /* ---- bump screening (screenJoints, src/core/13-fatigue.js:375-455) ----
  * per-step data of the cycle history (step 0 is the start); Cp/Fp NULL without a pillar */

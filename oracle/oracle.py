@@ -95,6 +95,7 @@ def lib():
                                        C.POINTER(OrWarpage)]
         L.or_anand_batch.argtypes = [_f64p, C.c_int64, C.c_double, C.c_double, C.c_double, _f64p, _f64p, _f64p,
                                      _f64p, _f64p, _f64p]
+        L.or_anand_batch_robust.argtypes = L.or_anand_batch.argtypes
         L.or_screen_joints.argtypes = [_f64p, C.POINTER(OrScreenSteps), C.c_int64, C.c_int32, C.c_int32, _f64p,
                                        _f64p, _f64p, _f64p]
         _lib = L
@@ -267,16 +268,18 @@ def warpage_jeita(mesh, u, bga):
     return {f: getattr(out, f) for f, _ in OrWarpage._fields_}
 
 
-def anand_batch(P, eps, T, dt, E, state, tangent=True):
-    """Oracle AnandPoint.step over a batch; P = (A, QR, xi, m, shat, n, h0, a, s0, nu)."""
+def anand_batch(P, eps, T, dt, E, state, tangent=True, robust=False):
+    """Oracle AnandPoint.step over a batch; P = (A, QR, xi, m, shat, n, h0, a, s0, nu).
+    robust: the bracket-end guard of the global viscoplastic Newton."""
     Pa = f64(P)
     eps = f64(eps).reshape(-1, 6)
     n = eps.shape[0]
     assert state.shape == (n, 11) and state.flags.c_contiguous and state.dtype == np.float64
     tr = np.zeros((n, 11)); sig = np.zeros((n, 6)); dep = np.zeros(n)
     D = np.zeros((n, 36)) if tangent else None
-    lib().or_anand_batch(_p(Pa, _f64p), n, E, T, dt, _p(eps, _f64p), _p(state, _f64p), _p(tr, _f64p),
-                         _p(sig, _f64p), _p(D, _f64p), _p(dep, _f64p))
+    fn = lib().or_anand_batch_robust if robust else lib().or_anand_batch
+    fn(_p(Pa, _f64p), n, E, T, dt, _p(eps, _f64p), _p(state, _f64p), _p(tr, _f64p), _p(sig, _f64p), _p(D, _f64p),
+       _p(dep, _f64p))
     return tr, sig, D, dep
 
 

@@ -68,6 +68,148 @@ def cfg1_spec() -> dict:
     }
 
 
+# SAC305 solder of the bump-array workload: the Anand constants of the workload
+# statement; elastic E(T) and nu and the CTE of the reference's sac305 record
+# (materials-db.json: 57.4 / 46.5 / 38.2 / 30.0 GPa at -40 / 25 / 75 / 125 C,
+# linear, nu 0.40, 22.5 ppm/K)
+SAC305_ANAND = dict(A=5.87e6, QR=7460.0, xi=2.0, m=0.0942, shat=58.3, n=0.015, h0=9350.0, a=1.5, s0=45.9)
+SAC305 = dict(name="sac305", E=46500.0, nu=0.40, cte=22.5, part=PART["SOLDER"],
+              E_table={"T": [-40.0, 25.0, 75.0, 125.0], "v": [57400.0, 46500.0, 38200.0, 30000.0]},
+              anand=SAC305_ANAND)
+
+
+def graded_sizes(length: float, h_small: float, h_max: float, ratio: float = 1.25) -> list:
+    """Cell sizes covering `length`, smallest first: h_small growing by `ratio`
+    until h_max, then cells of about h_max; all scaled to fill the length."""
+    sizes, h, tot = [], h_small, 0.0
+    while h < h_max and tot + h <= length:
+        sizes.append(h)
+        tot += h
+        h *= ratio
+    n_uni = max(0, int(round((length - tot) / h_max)))
+    sizes += [h_max] * n_uni
+    f = length / sum(sizes)
+    return [v * f for v in sizes]
+
+
+def bump_array_lines(a0: float, pitch: float, n: int, d: float, per_bump: int) -> list:
+    """Grid lines over a bump array from a0: in every pitch cell the bump's
+    per_bump equal cells (width d, centred), the gaps one cell each."""
+    g = (pitch - d) / 2
+    lines = [a0]
+    for m in range(n):
+        b0 = a0 + m * pitch + g
+        lines += [b0 + d * q / per_bump for q in range(per_bump + 1)]
+    lines.append(a0 + n * pitch)
+    return lines
+
+
+def cfg3_lines(arr0: float, arr1: float, arr_lines: list, half: float = 20.0, die_half: float = 10.0,
+               n_out: int = 12, h: float = 40.0 / 48) -> list:
+    """Grid lines on [-half, half]: n_out uniform cells from -half to the die
+    edge -die_half, graded (ratio <= 1.25, cells up to h) from the bump array's
+    gap size toward the die edge on the left and toward the substrate edge on
+    the right of the array [arr0, arr1]."""
+    left = linspace_ref(-half, -die_half, n_out)
+    gap = arr_lines[1] - arr_lines[0]
+    s_left = graded_sizes(arr0 - (-die_half), 1.5 * gap, h)
+    mid = [arr0]
+    for v in s_left:
+        mid.append(mid[-1] - v)
+    mid = list(reversed(mid))
+    mid[0] = -die_half
+    s_right = graded_sizes(half - arr1, 1.5 * gap, h)
+    right = [arr1]
+    for v in s_right:
+        right.append(right[-1] + v)
+    right[-1] = half
+    return left[:-1] + mid[:-1] + arr_lines[:-1] + right
+
+
+def cfg3_spec() -> dict:
+    """Bare-die FCBGA of cfg1_spec with a 20 x 20 array of discrete SAC305
+    bumps at 150 um pitch, each 3 x 3 x 4 hexes, under the die corner.
+
+    The array covers [7, 10] x [7, 10] mm (its outer bumps at the die corner
+    x, y = 10, where the distance to the neutral point is largest); a bump is a
+    100 um square column (3 cells of 33.3 um), the gaps 50 um (one cell), 25 um
+    at the array's edges.  Outside the array the bump layer is the smeared
+    bump/underfill layer of cfg1 (between the bumps, underfill of the same
+    properties).  The bump layer is 0.08 mm in 4 layers; the in-plane lines are
+    cfg1's 0.833 mm spacing graded (ratio <= 1.25) down to the array.
+    Loading: stress-free at 25 C, 2 temperature cycles -40 <-> 125 C with 15 min
+    ramps and 15 min dwells, 15 one-minute steps per ramp or dwell (the first
+    ramp 25 -> 125 C), 120 steps.
+    """
+    bumps = {"x0": 7.0, "y0": 7.0, "pitch": 0.15, "n": 20, "d": 0.1}
+    arr = bump_array_lines(7.0, 0.15, 20, 0.1, 3)
+    xs = cfg3_lines(7.0, 10.0, arr)
+    zs = _stack([(0.2, 2), (0.8, 4), (0.2, 2), (0.08, 4), (0.775, 4)])
+    die = [-10.0, 10.0, -10.0, 10.0]
+    return {
+        "name": "cfg3_bare_die_bump_array_20x20",
+        "xs": xs, "ys": list(xs), "zs": zs,
+        "materials": [BUILDUP, CORE, BUMP_UF, SI, SAC305],
+        "cells": [
+            {"k0": 0, "k1": 2, "mat": 0},
+            {"k0": 2, "k1": 6, "mat": 1},
+            {"k0": 6, "k1": 8, "mat": 0},
+            {"k0": 8, "k1": 12, "mat": 4, "in": die, "bumps": bumps},
+            {"k0": 8, "k1": 12, "mat": 2, "in": die},
+            {"k0": 12, "k1": 16, "mat": 3, "in": die},
+        ],
+        "bumps": bumps, "bump_layer": [8, 12], "solder_group": 4,
+        "Tref": 25.0,
+        "cycle": {"Tmin": -40.0, "Tmax": 125.0, "cycles": 2, "steps_per_segment": 15, "segment_s": 900.0},
+        "Ta": 25.0, "Tb": 125.0, "stage": 0,
+        "constraint_span": 8.0,
+        "feature_x": [xs.index(-10.0), xs.index(10.0)], "feature_y": [xs.index(-10.0), xs.index(10.0)],
+        "bga": {"pitch": 1.0, "inset": 1.0, "sx": 40.0, "sy": 40.0},
+        "top_plane": 16,
+        "tol": 1e-10,
+    }
+
+
+def cfg3_small_spec(n: int = 2, solder_anand: dict = None, half: float = 1.0) -> dict:
+    """A small stand-in of the bump-array workload for the oracle: a substrate
+    of 2 half x 2 half mm (build-up 0.1 / core 0.4 / build-up 0.1 mm, 1/2/1
+    layers), the 0.08 mm bump layer (4 layers) and a 1 x 1 x 0.3 mm die (2
+    layers), with an n x n array of the same bumps (150 um pitch, 3 x 3 x 4
+    hexes) at the die corner [0.5 - 0.15 n, 0.5]^2; same materials and loading
+    as cfg3_spec."""
+    a0 = 0.5 - 0.15 * n
+    bumps = {"x0": a0, "y0": a0, "pitch": 0.15, "n": n, "d": 0.1}
+    arr = bump_array_lines(a0, 0.15, n, 0.1, 3)
+    xs = cfg3_lines(a0, 0.5, arr, half=half, die_half=0.5, n_out=2 if half > 0.75 else 1, h=0.25)
+    zs = _stack([(0.1, 1), (0.4, 2), (0.1, 1), (0.08, 4), (0.3, 2)])
+    die = [-0.5, 0.5, -0.5, 0.5]
+    sol = dict(SAC305)
+    if solder_anand is not None:
+        sol["anand"] = dict(SAC305_ANAND, **solder_anand)
+    return {
+        "name": f"cfg3_small_{n}x{n}",
+        "xs": xs, "ys": list(xs), "zs": zs,
+        "materials": [BUILDUP, CORE, BUMP_UF, SI, sol],
+        "cells": [
+            {"k0": 0, "k1": 1, "mat": 0},
+            {"k0": 1, "k1": 3, "mat": 1},
+            {"k0": 3, "k1": 4, "mat": 0},
+            {"k0": 4, "k1": 8, "mat": 4, "in": die, "bumps": bumps},
+            {"k0": 4, "k1": 8, "mat": 2, "in": die},
+            {"k0": 8, "k1": 10, "mat": 3, "in": die},
+        ],
+        "bumps": bumps, "bump_layer": [4, 8], "solder_group": 4,
+        "Tref": 25.0,
+        "cycle": {"Tmin": -40.0, "Tmax": 125.0, "cycles": 2, "steps_per_segment": 15, "segment_s": 900.0},
+        "Ta": 25.0, "Tb": 125.0, "stage": 0,
+        "constraint_span": 0.4,
+        "feature_x": [xs.index(-0.5), xs.index(0.5)], "feature_y": [xs.index(-0.5), xs.index(0.5)],
+        "bga": {"pitch": 0.5, "inset": 0.25, "sx": 2 * half, "sy": 2 * half},
+        "top_plane": 10,
+        "tol": 1e-10,
+    }
+
+
 def small_box_spec() -> dict:
     """A 6 x 5 x 4 box exercising every assembly branch: three materials, a void
     corner, an element scale, two superposed phases in one layer and a group born

@@ -239,6 +239,14 @@ struct fcbga_model {
   DBuf<float> R32;
   bool symTables = false;
   bool sym_allowed = true;  // FCBGA_FULL_TABLES=1 forces the full tables
+  // elements of group ownGroup (-1: none) each get a type of their own,
+  // types ownType0.. in the order of ownElems; when ownK64 is set (device,
+  // 576 FP64 per own type) assemble_step takes those types' tables from it
+  // and gives them no thermal load (the viscoplastic Newton's tangents)
+  int32_t ownGroup = -1, ownType0 = 0;
+  std::vector<int32_t> ownElems;
+  std::vector<int64_t> ownCells;  // cell of every own-type element
+  const double *ownK64 = nullptr;
   // generation of the prepared step (operator + preconditioner), bumped by every prepare_step
   uint64_t prepGen = 0;
   int prepPrecond = -1, prepPrecision = -1;
@@ -368,7 +376,7 @@ void check_mesh(const fcbga_mesh *m) {
   if (m->nsections > 0 && !m->e_section) throw be::Error("layered sections given without e_section");
 }
 
-void model_create(const fcbga_mesh *mm, fcbga_model *M) {
+void model_create(const fcbga_mesh *mm, fcbga_model *M, int32_t ownGroup = -1) {
   check_mesh(mm);
   fe::Grid &g = M->g;
   g.nx = mm->nx; g.ny = mm->ny; g.nz = mm->nz;
@@ -410,6 +418,9 @@ void model_create(const fcbga_mesh *mm, fcbga_model *M) {
   std::map<std::tuple<int32_t, int32_t, int32_t, int32_t>, int32_t> tmap;
   M->e_type.assign(M->ne, -1);
   std::vector<int32_t> cellType(2 * ncells, -1), cellElem(2 * ncells, -1);
+  if (ownGroup >= M->ngroups) throw be::Error("own-type group out of range");
+  M->ownGroup = ownGroup;
+  std::vector<int64_t> ownSlots;  // cell slot 2c+q of every own-type element, in grid order
   for (int i = 0; i < g.ncx; i++)
     for (int j = 0; j < g.ncy; j++)
       for (int k = 0; k < g.ncz; k++) {
@@ -418,6 +429,12 @@ void model_create(const fcbga_mesh *mm, fcbga_model *M) {
           int32_t e = M->cell_elems[2 * c + q];
           if (e < 0) continue;
           if (e >= M->ne) throw be::Error("cell_elems entry out of range");
+          if (ownGroup >= 0 && M->e_group[e] == ownGroup) {
+            if (M->e_section[e] >= 0) throw be::Error("own-type elements cannot be layered");
+            ownSlots.push_back(2 * c + q);
+            cellElem[2 * c + q] = e;
+            continue;
+          }
           int32_t slot = M->e_section[e] >= 0 ? M->ngroups + M->e_section[e] : M->e_group[e];
           auto key = std::make_tuple(slot, cx[i], cy[j], cz[k]);
           auto it = tmap.find(key);
@@ -440,6 +457,27 @@ void model_create(const fcbga_mesh *mm, fcbga_model *M) {
           }
         }
       }
+  // own-type elements after the shared types, in grid order of their cells
+  M->ownType0 = (int32_t)M->typeSlot.size();
+  M->ownElems.clear();
+  M->ownCells.clear();
+  for (int64_t cs : ownSlots) {
+    const int64_t c = cs / 2;
+    const int k = (int)(c % g.ncz), j = (int)((c / g.ncz) % g.ncy), i = (int)(c / ((int64_t)g.ncz * g.ncy));
+    const int32_t e = cellElem[cs], t = (int32_t)M->typeSlot.size();
+    M->typeSlot.push_back(M->e_group[e]);
+    M->typeH.push_back(M->xs[i + 1] - M->xs[i]);
+    M->typeH.push_back(M->ys[j + 1] - M->ys[j]);
+    M->typeH.push_back(M->zs[k + 1] - M->zs[k]);
+    M->e_type[e] = t;
+    cellType[cs] = t;
+    M->ownElems.push_back(e);
+    M->ownCells.push_back(c);
+    for (int a = 0; a < 8; a++) {
+      int64_t gn = ((int64_t)(i + fe::corner_li(a)) * g.ny + j + fe::corner_lj(a)) * g.nz + k + fe::corner_lk(a);
+      if (M->node_id[gn] < 0) throw be::Error("element corner node missing in node_id");
+    }
+  }
   M->ntypes = (int32_t)M->typeSlot.size();
   if (g.nz > FINE_MAXNZ) throw be::Error("more than 64 grid planes in z");
   // DOF mask and compact -> grid map
@@ -535,6 +573,22 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
       for (int q = 0; q < 576; q++) Kc32[576 * t + q] = (float)Kc[576 * t + q];
       tF[t] = sF[sl];
     }, 64);
+  }
+  // own-type elements: the tables come from the caller (viscoplastic tangents)
+  if (M->ownK64 && M->ntypes > M->ownType0) {
+    const int32_t t0 = M->ownType0;
+    const double *oK = M->ownK64;
+    double *Kc = M->Kc64.p, *fc = M->fc64.p;
+    float *Kc32 = M->Kc32.p;
+    uint8_t *tF = M->typeHasF.p;
+    be::pfor((int64_t)(M->ntypes - t0) * 576, s, [=] __host__ __device__(int64_t q) {
+      const int64_t t = t0 + q / 576;
+      const double v = oK[q];
+      Kc[576 * t0 + q] = v;
+      Kc32[576 * t0 + q] = (float)v;
+      if (q % 576 < 24) fc[24 * t + q % 576] = 0.0;
+      if (q % 576 == 0) tF[t] = 0;
+    });
   }
   // mirror-symmetric compact tables (all types must pass, else the full tables)
   M->R64.alloc((size_t)M->ntypes * fe::SYM_TABLE);
@@ -3669,6 +3723,7 @@ int fail(const std::exception &e, int code) {
 }  // namespace
 
 #include "submodel.h"
+#include "viscoplastic.h"
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -4384,6 +4439,151 @@ int fcbga_submodel_section(fcbga_submodel *s, double E, double fsolder, float *o
   if (!(E > 0)) { g_err = "E > 0 required"; return FCBGA_ERR_ARG; }
   try {
     sub_section(s, E, fsolder, out);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_vp_create(const fcbga_mesh *mesh, const fcbga_vp_solder *solder, fcbga_vp **out) {
+  if (!out || !mesh || !solder) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  *out = nullptr;
+  fcbga_vp *V = new fcbga_vp();
+  try {
+    vp_create(mesh, solder, V);
+  } catch (const be::Error &e) {
+    fcbga_vp_destroy(V);
+    return fail(e, std::string(e.what()).rfind("HIP", 0) == 0 ? FCBGA_ERR_DEVICE : FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    fcbga_vp_destroy(V);
+    return fail(e, FCBGA_ERR_INTERNAL);
+  }
+  *out = V;
+  return FCBGA_OK;
+}
+
+void fcbga_vp_destroy(fcbga_vp *v) {
+  if (!v) return;
+  fcbga_model *M = v->M;
+  v->M = nullptr;
+#if FCBGA_GPU
+  if (M && M->stream) (void)hipStreamSynchronize(M->stream);
+#endif
+  delete v;  // device buffers of the engine first, then the model and its stream
+  if (M) fcbga_model_destroy(M);
+}
+
+int32_t fcbga_vp_nsolder(const fcbga_vp *v) { return v ? v->ns : 0; }
+
+int fcbga_vp_step(fcbga_vp *v, const fcbga_step *elastic, double T, double dt, const fcbga_vp_opts *opts,
+                  fcbga_vp_stats *stats) {
+  if (!v || !elastic || !opts) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    if (!(dt > 0) || !std::isfinite(T)) throw be::Error("dt > 0 and a finite T required");
+    if (opts->maxit < 1 || !(opts->res_tol > 0) || !(opts->du_tol > 0) || !(opts->inner_floor > 0))
+      throw be::Error("Newton options out of range");
+    fcbga_vp_opts o = *opts;
+    o.inner = check_opts(&opts->inner);
+    if (o.inner.precond == FCBGA_PC_IC) throw be::Error("the viscoplastic Newton takes the multigrid inner solver");
+    check_step(v->M, elastic);
+    const fcbga_vp_stats R = vp_newton(v, elastic, T, dt, o);
+    if (stats) *stats = R;
+    if (!R.converged) {
+      g_err = R.nonfinite ? "viscoplastic step: non-finite material-point stress" : "viscoplastic step did not converge";
+      return FCBGA_ERR_NOT_CONVERGED;
+    }
+  } catch (const be::Error &e) {
+    return fail(e, FCBGA_ERR_ARG);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_vp_save(fcbga_vp *v) {
+  if (!v) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try { vp_save(v); } catch (const std::exception &e) { return fail(e, FCBGA_ERR_DEVICE); }
+  return FCBGA_OK;
+}
+
+int fcbga_vp_restore(fcbga_vp *v) {
+  if (!v) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try { vp_restore(v); } catch (const be::Error &e) { return fail(e, FCBGA_ERR_ARG); }
+  catch (const std::exception &e) { return fail(e, FCBGA_ERR_DEVICE); }
+  return FCBGA_OK;
+}
+
+int fcbga_vp_points(fcbga_vp *v, double *st, int32_t *elems) {
+  if (!v || !st) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    be::d2h(st, v->st.p, sizeof(double) * 88 * (size_t)v->ns, v->M->stream);
+    if (elems) std::copy(v->elemOf.begin(), v->elemOf.end(), elems);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_vp_displacement(fcbga_vp *v, double *u) {
+  if (!v || !u) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    DBuf<double> tmp;
+    tmp.alloc(3 * (size_t)v->M->nn);
+    vp_to_nodes(v, v->u.p, tmp.p);
+    be::d2h(u, tmp.p, sizeof(double) * tmp.n, v->M->stream);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_vp_set_displacement(fcbga_vp *v, const double *u) {
+  if (!v || !u) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    DBuf<double> tmp;
+    tmp.upload(u, 3 * (size_t)v->M->nn, v->M->stream);
+    vp_from_nodes(v, tmp.p, v->u.p);
+    be::sync(v->M->stream);
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_vp_residual(fcbga_vp *v, const fcbga_step *elastic, double T, double dt, double *r, const double *x,
+                      double *y) {
+  if (!v || !elastic || !r || (x && !y)) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    if (!(dt > 0)) throw be::Error("dt > 0 required");
+    check_step(v->M, elastic);
+    fcbga_model *M = v->M;
+    be::stream_t s = M->stream;
+    vp_elements(v, T, dt, 1);
+    fcbga_solve_opts o{1e-8, 1, FCBGA_PREC_FP64, FCBGA_PC_JACOBI, 0, 0};
+    fcbga_solve_stats S{};
+    vp_prepare(v, elastic, o, S);
+    vp_residual(v);
+    const size_t nn3 = 3 * (size_t)M->nn;
+    DBuf<double> tmp;
+    tmp.alloc(nn3);
+    vp_to_nodes(v, v->rg.p, tmp.p);
+    be::d2h(r, tmp.p, sizeof(double) * nn3, s);
+    if (x) {
+      DBuf<double> xg, xn;
+      xn.upload(x, nn3, s);
+      xg.alloc(3 * M->g.G);
+      vp_from_nodes(v, xn.p, xg.p);
+      const int64_t n3 = 3 * M->g.G;
+      const double *si = M->sinv64.p, *wv = M->w64.p;
+      double *xp = xg.p, *q = M->q.p;
+      be::pfor(n3, s, [=] __host__ __device__(int64_t i) { xp[i] = wv[i] != 0 ? xp[i] * si[i] : 0.0; });
+      fine_apply<double>(fine_ops<double>(M), xp, q, s);
+      be::pfor(n3, s, [=] __host__ __device__(int64_t i) { xp[i] = wv[i] != 0 ? q[i] * si[i] : 0.0; });
+      vp_to_nodes(v, xp, tmp.p);
+      be::d2h(y, tmp.p, sizeof(double) * nn3, s);
+    }
+  } catch (const be::Error &e) {
+    return fail(e, FCBGA_ERR_ARG);
   } catch (const std::exception &e) {
     return fail(e, FCBGA_ERR_DEVICE);
   }

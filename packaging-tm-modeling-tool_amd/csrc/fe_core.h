@@ -591,8 +591,16 @@ FE_HD double anand_dsdd(const AnandParams &P, double d, double s, double dt, dou
 // One trial step from the committed state st (st[10], the warm start, is
 // updated in place as the reference does); the trial state goes to out (11).
 // Returns the equivalent plastic strain increment.
+// robust = false is the reference's iteration, kept bit for bit where the
+// reference runs it (screening, submodel): there an iterate at the bracket end
+// d = sigma_trial / 3G, where sigma_trial - 3 G d rounds to a tiny negative,
+// makes sinh(x)^(1/m) NaN and the NaN branch raises the lower bracket to the
+// end, so the iteration stays on the spurious root d = dmax (sigma_eq = 0).
+// robust = true (the global viscoplastic Newton, which has no reference
+// counterpart) takes a non-positive effective stress as no flow: g(d) = d > 0,
+// an upper bracket, so the iteration returns to the interior root.
 FE_HD double anand_step(const AnandParams &P, double E, double T, double dt, const double eps[6], double *st,
-                        double *out, double sig[6], double *D) {
+                        double *out, double sig[6], double *D, bool robust = false) {
   FE_NO_CONTRACT
   const double G = E / (2 * (1 + P.nu)), K = E / (3 * (1 - 2 * P.nu));
   const double expQ = exp(P.QR / (T + KELVIN));
@@ -617,8 +625,13 @@ FE_HD double anand_step(const AnandParams &P, double E, double T, double dt, con
     for (int it = 0; it < ANAND_MAXIT; it++) {
       rs = anand_solve_s(P, sn, dd, dt, expQ);
       rx = P.xi * (sigTr - 3 * G * dd) / rs;
-      rsh = sinh(rx);
-      rg = dd - rate0 * pow(rsh, 1 / P.m);
+      if (robust && !(rx > 0)) {
+        rsh = 0;
+        rg = dd;
+      } else {
+        rsh = sinh(rx);
+        rg = dd - rate0 * pow(rsh, 1 / P.m);
+      }
       if (!isfinite(rg)) { lo = dd; dd = 0.5 * (lo + hi); continue; }
       if (fabs(rg) < ANAND_TOL * fmax(dmax, 1e-30)) break;
       if (rg > 0) hi = dd; else lo = dd;

@@ -87,6 +87,9 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_submodel_save", "fcbga_submodel_restore", "fcbga_submodel_layer_means",
            "fcbga_submodel_loop_sample", "fcbga_submodel_section", "fcbga_submodel_displacement",
            "fcbga_submodel_points", "fcbga_screen_joints_noalloc",
+           "fcbga_vp_create", "fcbga_vp_destroy", "fcbga_vp_nsolder", "fcbga_vp_step", "fcbga_vp_save",
+           "fcbga_vp_restore", "fcbga_vp_points", "fcbga_vp_displacement", "fcbga_vp_set_displacement",
+           "fcbga_vp_residual",
            "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
            "fcbga_path_load64", "fcbga_path_copy",
            "fcbga_mesh_fingerprint", "fcbga_last_error"]
@@ -101,6 +104,22 @@ class FcbgaObjective(C.Structure):
 
 class FcbgaAnandParams(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0", "nu")]
+
+
+class FcbgaVpSolder(C.Structure):
+    _fields_ = [("group", C.c_int32), ("anand", FcbgaAnandParams), ("nE", C.c_int32), ("E_T", _f64p),
+                ("E_v", _f64p), ("cte", C.c_double), ("Tref", C.c_double)]
+
+
+class FcbgaVpOpts(C.Structure):
+    _fields_ = [("inner", FcbgaSolveOpts), ("maxit", C.c_int32), ("res_tol", C.c_double), ("du_tol", C.c_double),
+                ("inner_floor", C.c_double)]
+
+
+class FcbgaVpStats(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("newton", C.c_int32), ("pcg_iters", C.c_int32), ("nonfinite", C.c_int32),
+                ("dmax", C.c_double), ("res", C.c_double), ("ms", C.c_double), ("ms_setup", C.c_double),
+                ("ms_pcg", C.c_double)]
 
 # kernel classes of fcbga_kernel_times (include/fcbga_hip.h)
 KCLASS = {"apply": 0, "fine_sweep": 1, "fine_resid": 2, "shell1_sweep": 3, "shell_sweep": 4, "coarsest": 5,
@@ -182,6 +201,26 @@ def load(variant: str = "hip"):
     L.fcbga_mesh_fingerprint.restype = C.c_int
     L.fcbga_warm_reset.argtypes = [C.c_void_p]
     L.fcbga_warm_reset.restype = C.c_int
+    L.fcbga_vp_create.argtypes = [C.POINTER(FcbgaMesh), C.POINTER(FcbgaVpSolder), C.POINTER(C.c_void_p)]
+    L.fcbga_vp_create.restype = C.c_int
+    L.fcbga_vp_destroy.argtypes = [C.c_void_p]
+    L.fcbga_vp_destroy.restype = None
+    L.fcbga_vp_nsolder.argtypes = [C.c_void_p]
+    L.fcbga_vp_nsolder.restype = C.c_int32
+    L.fcbga_vp_step.argtypes = [C.c_void_p, C.POINTER(FcbgaStep), C.c_double, C.c_double, C.POINTER(FcbgaVpOpts),
+                                C.POINTER(FcbgaVpStats)]
+    L.fcbga_vp_step.restype = C.c_int
+    for f in ("fcbga_vp_save", "fcbga_vp_restore"):
+        getattr(L, f).argtypes = [C.c_void_p]
+        getattr(L, f).restype = C.c_int
+    L.fcbga_vp_points.argtypes = [C.c_void_p, _f64p, _i32p]
+    L.fcbga_vp_points.restype = C.c_int
+    L.fcbga_vp_displacement.argtypes = [C.c_void_p, _f64p]
+    L.fcbga_vp_displacement.restype = C.c_int
+    L.fcbga_vp_set_displacement.argtypes = [C.c_void_p, _f64p]
+    L.fcbga_vp_set_displacement.restype = C.c_int
+    L.fcbga_vp_residual.argtypes = [C.c_void_p, C.POINTER(FcbgaStep), C.c_double, C.c_double, _f64p, _f64p, _f64p]
+    L.fcbga_vp_residual.restype = C.c_int
     L.fcbga_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.fcbga_last_error.restype = C.c_int
     _loaded[variant] = L
@@ -219,36 +258,41 @@ class Step:
         self.c = st
 
 
+def _mesh_struct(mesh):
+    """FcbgaMesh over the mesh's arrays; returns (struct, arrays to keep alive)."""
+    a = dict(
+        xs=np.ascontiguousarray(mesh.xs, np.float64), ys=np.ascontiguousarray(mesh.ys, np.float64),
+        zs=np.ascontiguousarray(mesh.zs, np.float64),
+        node_id=np.ascontiguousarray(mesh.node_id, np.int32),
+        cell_elems=np.ascontiguousarray(mesh.cell_elems, np.int32),
+        e_group=np.ascontiguousarray(mesh.e_group, np.int32),
+        e_part=np.ascontiguousarray(mesh.e_part, np.int8),
+        e_scale=np.ascontiguousarray(mesh.e_scale, np.float64),
+        cons=np.ascontiguousarray(mesh.constraint_dofs, np.int32),
+        fx=np.ascontiguousarray(mesh.feature_x, np.int32), fy=np.ascontiguousarray(mesh.feature_y, np.int32))
+    m = FcbgaMesh()
+    m.nx, m.ny, m.nz, m.nn, m.ne = mesh.nx, mesh.ny, mesh.nz, mesh.nn, mesh.ne
+    m.xs, m.ys, m.zs = _ptr(a["xs"], _f64p), _ptr(a["ys"], _f64p), _ptr(a["zs"], _f64p)
+    m.node_id = _ptr(a["node_id"], _i32p)
+    m.cell_elems = _ptr(a["cell_elems"], _i32p)
+    m.e_group = _ptr(a["e_group"], _i32p)
+    m.e_part = _ptr(a["e_part"], _i8p)
+    m.e_scale = _ptr(a["e_scale"], _f64p)
+    m.e_section = None
+    m.ngroups = len(mesh.groups)
+    m.nsections = 0
+    m.nconstraints = len(a["cons"])
+    m.constraint_dofs = _ptr(a["cons"], _i32p)
+    m.nfeature_x, m.nfeature_y = len(a["fx"]), len(a["fy"])
+    m.feature_x, m.feature_y = _ptr(a["fx"], _i32p), _ptr(a["fy"], _i32p)
+    return m, a
+
+
 class Model:
     def __init__(self, mesh, variant: str = "hip"):
         self.L = load(variant)
         self.mesh = mesh
-        a = dict(
-            xs=np.ascontiguousarray(mesh.xs, np.float64), ys=np.ascontiguousarray(mesh.ys, np.float64),
-            zs=np.ascontiguousarray(mesh.zs, np.float64),
-            node_id=np.ascontiguousarray(mesh.node_id, np.int32),
-            cell_elems=np.ascontiguousarray(mesh.cell_elems, np.int32),
-            e_group=np.ascontiguousarray(mesh.e_group, np.int32),
-            e_part=np.ascontiguousarray(mesh.e_part, np.int8),
-            e_scale=np.ascontiguousarray(mesh.e_scale, np.float64),
-            cons=np.ascontiguousarray(mesh.constraint_dofs, np.int32),
-            fx=np.ascontiguousarray(mesh.feature_x, np.int32), fy=np.ascontiguousarray(mesh.feature_y, np.int32))
-        self._keep = a
-        m = FcbgaMesh()
-        m.nx, m.ny, m.nz, m.nn, m.ne = mesh.nx, mesh.ny, mesh.nz, mesh.nn, mesh.ne
-        m.xs, m.ys, m.zs = _ptr(a["xs"], _f64p), _ptr(a["ys"], _f64p), _ptr(a["zs"], _f64p)
-        m.node_id = _ptr(a["node_id"], _i32p)
-        m.cell_elems = _ptr(a["cell_elems"], _i32p)
-        m.e_group = _ptr(a["e_group"], _i32p)
-        m.e_part = _ptr(a["e_part"], _i8p)
-        m.e_scale = _ptr(a["e_scale"], _f64p)
-        m.e_section = None
-        m.ngroups = len(mesh.groups)
-        m.nsections = 0
-        m.nconstraints = len(a["cons"])
-        m.constraint_dofs = _ptr(a["cons"], _i32p)
-        m.nfeature_x, m.nfeature_y = len(a["fx"]), len(a["fy"])
-        m.feature_x, m.feature_y = _ptr(a["fx"], _i32p), _ptr(a["fy"], _i32p)
+        m, self._keep = _mesh_struct(mesh)
         self._cmesh = m
         h = C.c_void_p()
         rc = self.L.fcbga_model_create(C.byref(m), C.byref(h))
@@ -546,3 +590,95 @@ def screen_joints(params, steps, table, i25, variant="hip"):
     if rc != 0:
         raise SolverError(last_error(L))
     return dW, dE, stab
+
+
+class ViscoplasticModel:
+    """fcbga_vp_*: the global viscoplastic Newton over a mesh whose solder
+    group carries Anand points (the bump-array cycling workload).  solder:
+    dict with group, anand (A QR xi m shat n h0 a s0), nu, E_table {T, v},
+    cte (ppm/K), Tref."""
+
+    def __init__(self, mesh, solder: dict, variant: str = "hip"):
+        self.L = load(variant)
+        self.mesh = mesh
+        m, self._keep = _mesh_struct(mesh)
+        self._cmesh = m
+        sp = FcbgaVpSolder()
+        sp.group = int(solder["group"])
+        for k, v in solder["anand"].items():
+            setattr(sp.anand, k, float(v))
+        sp.anand.nu = float(solder["nu"])
+        self._ET = np.ascontiguousarray(solder["E_table"]["T"], np.float64)
+        self._Ev = np.ascontiguousarray(solder["E_table"]["v"], np.float64)
+        sp.nE = len(self._ET)
+        sp.E_T, sp.E_v = _ptr(self._ET, _f64p), _ptr(self._Ev, _f64p)
+        sp.cte = float(solder["cte"]) * 1e-6
+        sp.Tref = float(solder["Tref"])
+        self._sp = sp
+        h = C.c_void_p()
+        rc = self.L.fcbga_vp_create(C.byref(m), C.byref(sp), C.byref(h))
+        if rc != 0:
+            raise SolverError(f"fcbga_vp_create failed ({rc}): {last_error(self.L)}")
+        self.h = h
+        self.nsolder = int(self.L.fcbga_vp_nsolder(h))
+        self.ndof = 3 * mesh.nn
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fcbga_vp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, elastic: Step, T, dt, precision=PREC_MIXED, maxit=25, res_tol=1e-6, du_tol=1e-8,
+             inner_floor=1e-8, inner_maxit=2000):
+        o = FcbgaVpOpts()
+        o.inner.tol, o.inner.maxit, o.inner.precision, o.inner.precond = 1e-8, int(inner_maxit), int(precision), PC_MG
+        o.inner.sweeps, o.inner.warm = 0, 0
+        o.maxit, o.res_tol, o.du_tol, o.inner_floor = int(maxit), float(res_tol), float(du_tol), float(inner_floor)
+        st = FcbgaVpStats()
+        rc = self.L.fcbga_vp_step(self.h, C.byref(elastic.c), float(T), float(dt), C.byref(o), C.byref(st))
+        rec = {k: getattr(st, k) for k, _ in FcbgaVpStats._fields_}
+        rec["converged"] = bool(rec["converged"])
+        if rc not in (0, -3):
+            raise SolverError(f"fcbga_vp_step failed ({rc}): {last_error(self.L)}")
+        return rec
+
+    def save(self):
+        if self.L.fcbga_vp_save(self.h) != 0:
+            raise SolverError(last_error(self.L))
+
+    def restore(self):
+        if self.L.fcbga_vp_restore(self.h) != 0:
+            raise SolverError(last_error(self.L))
+
+    def points(self):
+        """Committed point states (nsolder, 8, 11) and the mesh element of each solder element."""
+        st = np.empty(88 * self.nsolder)
+        el = np.empty(self.nsolder, np.int32)
+        if self.L.fcbga_vp_points(self.h, st.ctypes.data_as(_f64p), el.ctypes.data_as(_i32p)) != 0:
+            raise SolverError(last_error(self.L))
+        return st.reshape(self.nsolder, 8, ANAND_STATE), el
+
+    def displacement(self):
+        u = np.empty(self.ndof)
+        if self.L.fcbga_vp_displacement(self.h, u.ctypes.data_as(_f64p)) != 0:
+            raise SolverError(last_error(self.L))
+        return u
+
+    def set_displacement(self, u):
+        u = np.ascontiguousarray(u, np.float64)
+        if self.L.fcbga_vp_set_displacement(self.h, u.ctypes.data_as(_f64p)) != 0:
+            raise SolverError(last_error(self.L))
+
+    def residual(self, elastic: Step, T, dt, x=None):
+        """Out-of-balance force at the current displacement and, for x, K_t x."""
+        r = np.empty(self.ndof)
+        y = np.empty(self.ndof) if x is not None else None
+        xx = None if x is None else np.ascontiguousarray(x, np.float64)
+        rc = self.L.fcbga_vp_residual(self.h, C.byref(elastic.c), float(T), float(dt), r.ctypes.data_as(_f64p),
+                                      _ptr(xx, _f64p), _ptr(y, _f64p))
+        if rc != 0:
+            raise SolverError(last_error(self.L))
+        return (r, y) if x is not None else r

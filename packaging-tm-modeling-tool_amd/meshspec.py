@@ -15,12 +15,15 @@ A spec is a plain dict (JSON-serialisable):
                     "stage" (default 0), "column" (default false)}, ...],
      "cells": [{"k0", "k1", "mat", "in": [x0, x1, y0, y1] | null,
                 "out": [x0, x1, y0, y1] | null, "scale": 1.0,
-                "also": mat | null, "also_scale": 1.0}, ...],
+                "also": mat | null, "also_scale": 1.0,
+                "bumps": {"x0", "y0", "pitch", "n", "d"} | null}, ...],
      "Ta", "Tb", "stage", "constraint_span", "feature_x", "feature_y",
      "bga": {"pitch", "inset", "sx", "sy"}, "tol"}
 
 A cell (i, j, k) takes the first rule with k0 <= k < k1 whose region test
-passes at the cell centre (inside "in", outside "out"); no rule = void.
+passes at the cell centre (inside "in", outside "out", inside one of the
+n x n square bump footprints of side d centred in the pitch cells of a bump
+array starting at (x0, y0)); no rule = void.
 """
 from __future__ import annotations
 
@@ -121,6 +124,29 @@ def _inside(r, x, y):
     return r[0] <= x <= r[1] and r[2] <= y <= r[3]
 
 
+def in_bumps(b: dict, X, Y):
+    """Points (X, Y) inside a bump footprint of the array b: n x n pitch cells
+    from (x0, y0), each with a d x d square centred in it."""
+    def axis(V, v0):
+        t = (np.asarray(V) - v0) / b["pitch"]
+        m = np.floor(t)
+        off = (t - m) * b["pitch"]
+        lo, hi = (b["pitch"] - b["d"]) / 2, (b["pitch"] + b["d"]) / 2
+        return (m >= 0) & (m < b["n"]) & (off >= lo) & (off <= hi)
+    return axis(X, b["x0"]) & axis(Y, b["y0"])
+
+
+def bump_index(b: dict, x: float, y: float) -> int:
+    """Bump of the array b (row-major in x then y) whose footprint holds (x, y), or -1."""
+    mx = math.floor((x - b["x0"]) / b["pitch"])
+    my = math.floor((y - b["y0"]) / b["pitch"])
+    if not (0 <= mx < b["n"] and 0 <= my < b["n"]):
+        return -1
+    if not bool(in_bumps(b, np.array([x]), np.array([y]))[0]):
+        return -1
+    return int(mx * b["n"] + my)
+
+
 def cell_assignment(spec: dict) -> np.ndarray:
     """Per cell (i, j, k): up to two (material, scale) entries; returns int arrays
     mat[ncells, 2] (-1 none) and scale[ncells, 2]."""
@@ -146,6 +172,8 @@ def cell_assignment(spec: dict) -> np.ndarray:
             if r.get("out") is not None:
                 a = r["out"]
                 ok &= ~((X >= a[0]) & (X <= a[1]) & (Y >= a[2]) & (Y <= a[3]))
+            if r.get("bumps") is not None:
+                ok &= in_bumps(r["bumps"], X, Y)
             mat[ok, k, 0] = r["mat"]
             scl[ok, k, 0] = r.get("scale", 1.0)
             if r.get("also") is not None:
