@@ -4,20 +4,26 @@
 Workload: the lidded full-fidelity FCBGA (160 x 160 x 52 hexes, graded,
 3,093,159 DOF) solved cold from the 150 C stress-free state to each of six
 temperatures 25..260 C; one "step" is one complete step solve (device
-assembly of the condensed element tables and load, multigrid set-up, PCG to a
-1e-10 scaled relative residual) in mixed precision (FP32 multigrid cycle
-inside FP64 flexible PCG).  The mesh is uploaded before the timed region; the
-displacements stay in HBM.  The six steps are timed twice: one at a time on
-one model instance (solve_s_mixed, value_one_at_a_time), and with three
-solves in flight on three instances (each its own stream, HIP graphs and
-host thread), as the reference spreads the six temperatures over its workers;
-value = sum(DOF x PCG iterations) / timed seconds of the latter.
+assembly of the condensed element tables and load, multigrid set-up, PCG) in
+mixed precision (FP32 multigrid cycle inside FP64 flexible PCG), to the
+tolerance at which the JEITA warpage is within 1e-6 of the FP64 oracle (a
+tight FP64 solve, tol 1e-12, of the same step): an untimed calibration pass
+takes, for each precision, the loosest tolerance of 1e-4 .. 1e-10 that meets
+1e-6, and every timed solve's warpage error is reported.  The mesh is
+uploaded before the timed region; the displacements stay in HBM.  The six
+steps are timed one at a time on one model instance (solve_s_mixed,
+value_one_at_a_time) and with three solves in flight on three instances
+(each its own stream, HIP graphs and host thread), as the reference spreads
+the six temperatures over its workers; value = sum(DOF x PCG iterations) /
+timed seconds of the latter.
 
 Beside the timed mixed-precision solves the run also reports the plain FP64
-PCG-MG solve of the same steps (the comparator), the warpage error of each
-mixed solve against it, the dominant kernel's roofline from HIP-event timing,
-and the CPU oracle (a single-threaded C restatement of the reference's FP64
-PCG-MG) on a bounded sample of the same workload.
+PCG-MG solve of the same steps at its own 1e-6-warpage tolerance (the
+comparator), the dominant kernel's roofline and the composite per-iteration
+roofline from HIP-event timing, the bump-array viscoplastic cycling workload
+(120 steps of the global Newton), and the CPU oracle (a single-threaded C
+restatement of the reference's FP64 PCG-MG) on a bounded sample of the same
+workload, one process per temperature.
 
   python bench.py [--gpus 1] [--steps 6] [--warmup 1] [--config cfg2|cfg1] [--concurrent 3]
 """
@@ -104,6 +110,115 @@ def roofline_for(cls, ms_total, launches, mesh, dtype_cycle):
     return {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": None, "kernel": cls, "dtype": unit_dt,
             "avg_launch_us": avg_s * 1e6, "launches": launches, "flop_per_launch": flops}
+
+
+def composite_roofline(ktimes, ksolves, rec_iters, ms_pcg, mesh, ncycle_sweeps=4):
+    """Algorithmic time of one PCG iteration of the mixed solve over its
+    measured time: sum over the fine-level kernels of max(flop / peak,
+    bytes / HBM bandwidth) with the per-unit figures of the hot-path table
+    (outer FP64 apply: 1152 flop per element, 16 B per DOF; z-line sweeps,
+    2 pre + 2 post per cycle: 36 B per DOF each; FP32 residual apply: 1152 flop
+    per element, 8 B per DOF; restriction + prolongation: 56 B per DOF; PCG
+    vectors: 64 B per DOF, the fused minimum).  The shell levels and the
+    coarsest solve are left out of the algorithmic sum (they are counted in
+    the measured time), so the fraction is an upper bound of the fine
+    level's share."""
+    N, E = mesh.ndof, mesh.ne
+    terms = {
+        "outer_apply_fp64": max(1152.0 * E / PEAK["fp64"], 16.0 * N / PEAK["hbm"]),
+        "fine_sweeps": ncycle_sweeps * 36.0 * N / PEAK["hbm"],
+        "fine_residual_fp32": max(1152.0 * E / PEAK["fp32"], 8.0 * N / PEAK["hbm"]),
+        "transfers": 56.0 * N / PEAK["hbm"],
+        "pcg_vectors": 64.0 * N / PEAK["hbm"],
+    }
+    t_alg = sum(terms.values())
+    t_meas = ms_pcg / 1e3 / max(rec_iters, 1)
+    return {"alg_us_per_iter": 1e6 * t_alg, "measured_us_per_iter": 1e6 * t_meas, "frac": t_alg / t_meas,
+            "terms_us": {k: 1e6 * v for k, v in terms.items()},
+            "kernel_ms_per_iter": {k: v[0] / max(1, ksolves * rec_iters) for k, v in ktimes.items() if v[1]},
+            "note": "fine-level kernels only in the algorithmic sum; shell levels, coarsest solve and launch gaps "
+                    "are in the measured time"}
+
+
+def warpage_oracle(model, step, du, mesh, bga, post):
+    """FP64 oracle of a step: the same solve in FP64 to 1e-12."""
+    model.solve_step_device(step, du.data_ptr(), tol=1e-12, maxit=4000, precision=0, precond=0)
+    u = du.cpu().numpy().copy()
+    return u, post.warpage_jeita(mesh, u, bga)["signed"]
+
+
+def calibrate_tol(solve_fn, du, mesh, bga, post, w_ref, target=1e-6):
+    """Loosest tolerance of 1e-4 .. 1e-10 whose warpage is within target of w_ref."""
+    ladder = [1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10]
+    tried = []
+    for tol in ladder:
+        rec = solve_fn(tol)
+        u = du.cpu().numpy()
+        err = abs(post.warpage_jeita(mesh, u, bga)["signed"] - w_ref) / abs(w_ref)
+        tried.append({"tol": tol, "iters": rec["iters"], "warpage_err": err})
+        if err <= target:
+            return tol, tried
+    return ladder[-1], tried
+
+
+def cfg3_timing(hs, cases, meshspec, nsteps=None):
+    """The bump-array viscoplastic cycling workload: the global Newton through
+    the 120-step history (two cycles -40 <-> 125 C), mixed-precision inner
+    solves; per-bump inelastic work density of each cycle."""
+    from fcbga_amd import viscoplastic as vpm
+    spec = cases.cfg3_spec()
+    mesh = meshspec.build_mesh(spec)
+    V = hs.ViscoplasticModel(mesh, vpm.solder_of(spec), variant="hip")
+    rec = vpm.run(V, spec, nsteps=nsteps)
+    V.close()
+    out = {k: rec[k] for k in ("steps", "solves", "cutbacks", "newton", "pcg_iters", "s", "ms_per_step",
+                               "ms_setup", "ms_pcg")}
+    out.update({"workload": spec["name"], "ndof": mesh.ndof, "solder_elements": int((mesh.e_group ==
+                spec["solder_group"]).sum()), "points": 8 * int((mesh.e_group == spec["solder_group"]).sum()),
+                "newton_per_step": rec["newton"] / max(rec["solves"], 1),
+                "pcg_per_newton": rec["pcg_iters"] / max(rec["newton"], 1)})
+    pc = rec["per_cycle"]
+    if pc:
+        dW = pc[-1]["dW"]
+        b = spec["bumps"]["n"]
+        corner = (b - 1) * b + (b - 1)  # the bump nearest the die corner
+        out["dW_last_cycle"] = {"max": float(np.nanmax(dW)), "mean": float(np.nanmean(dW)),
+                                "corner_bump": float(dW[corner]), "argmax_bump": int(np.nanargmax(dW)),
+                                "per_cycle_max": [float(np.nanmax(c["dW"])) for c in pc], "unit": "MPa"}
+    return out
+
+
+def _cpu_sample_child(args):
+    """One process of the CPU baseline: the oracle's cold solve of one
+    temperature, `iters` PCG iterations."""
+    spec_name, T, iters = args
+    sys.path.insert(0, ROOT)
+    ge.package()
+    from oracle import oracle as orc
+    from fcbga_amd import cases, meshspec
+    spec = cases.cfg2_spec() if spec_name == "cfg2" else cases.cfg1_spec()
+    mesh = meshspec.build_mesh(spec)
+    D, s0, born = meshspec.group_step_data(spec, spec["Ta"], T, 0)
+    t0 = time.perf_counter()
+    sysm = orc.System(mesh, D, s0, born)
+    _, st = sysm.solve(tol=1e-30, maxit=iters)
+    return mesh.ndof, st["iters"], time.perf_counter() - t0
+
+
+def host_facts():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": ncpu}
 
 
 def cpu_baseline(spec, mesh, iters=5):
@@ -335,6 +450,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-path", action="store_true")
     ap.add_argument("--no-submodel", action="store_true")
+    ap.add_argument("--no-cfg3", action="store_true")
+    ap.add_argument("--fixed-tol", dest="tol_from_warpage", action="store_false",
+                    help="time at --tol instead of the tolerance calibrated to 1e-6 warpage error")
     ap.add_argument("--submodel-batch", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.submodel_batch:
@@ -353,17 +471,32 @@ def main():
     temps = spec.get("temperatures", [spec["Tb"]])
     mesh = meshspec.build_mesh(spec)
     log(f"{spec['name']}: {mesh.ndof} DOF, {mesh.ne} elements")
+    du = torch.empty(mesh.ndof, dtype=torch.float64, device="cuda")  # torch's device context first
     model = hs.Model(mesh, variant="hip")
-    du = torch.empty(mesh.ndof, dtype=torch.float64, device="cuda")
     steps = [hs.step_for_spec(spec, Ta=spec["Ta"], Tb=temps[k % len(temps)]) for k in range(max(args.steps, 1))]
+    bga = meshspec.bga_sites(**spec["bga"])
+    tols = {hs.PREC_MIXED: args.tol, hs.PREC_FP64: args.tol}
 
-    def solve(k, prec):
-        return model.solve_step_device(steps[k % len(steps)], du.data_ptr(), tol=args.tol, maxit=2000,
-                                       precision=prec, precond=hs.PC_MG, sweeps=args.sweeps)
+    def solve(k, prec, tol=None):
+        return model.solve_step_device(steps[k % len(steps)], du.data_ptr(), tol=tols[prec] if tol is None else tol,
+                                       maxit=2000, precision=prec, precond=hs.PC_MG, sweeps=args.sweeps)
 
     for w in range(args.warmup):
         rec = solve(w, hs.PREC_MIXED)
         log(f"warmup {w}: iters {rec['iters']} {rec['ms_total']:.1f} ms")
+    # untimed calibration: the FP64 oracle of every step (tol 1e-12) and, per
+    # precision, the loosest tolerance whose warpage is within 1e-6 of it
+    w_ref, cal = [], {}
+    for k in range(len(steps)):
+        w_ref.append(warpage_oracle(model, steps[k], du, mesh, bga, post)[1])
+    if args.tol_from_warpage:
+        for prec in (hs.PREC_MIXED, hs.PREC_FP64):
+            tols[prec], cal[prec] = calibrate_tol(lambda t: solve(0, prec, t), du, mesh, bga, post, w_ref[0])
+            log(f"precision {prec}: tolerance {tols[prec]:g} for warpage error <= 1e-6 ({cal[prec]})")
+    golden_w = None
+    if args.config == "cfg2":
+        with open(os.path.join(ROOT, "tests", "golden", "cfg2_golden.json")) as f:
+            golden_w = json.load(f)["warpage"]["signed"]  # the reference's JS solve, 25 C, tol 1e-12
     recs = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -371,6 +504,12 @@ def main():
         recs.append(solve(k, hs.PREC_MIXED))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # warpage error of the timed mixed solves (recomputed untimed, same tolerance, deterministic)
+    werr_mixed = []
+    for k in range(min(args.steps, len(steps))):
+        solve(k, hs.PREC_MIXED)
+        w = post.warpage_jeita(mesh, du.cpu().numpy(), bga)["signed"]
+        werr_mixed.append(abs(w - w_ref[k]) / abs(w_ref[k]))
     # throughput pass: the same steps with K solves in flight on K instances of
     # the model (each its own stream, graphs and host thread), as the reference
     # spreads the six temperatures over its workers
@@ -380,8 +519,9 @@ def main():
     dus_k = [du] + [torch.empty(mesh.ndof, dtype=torch.float64, device="cuda") for _ in extra]
 
     def solve_on(mi, k):
-        return models_k[mi].solve_step_device(steps[k % len(steps)], dus_k[mi].data_ptr(), tol=args.tol, maxit=2000,
-                                              precision=hs.PREC_MIXED, precond=hs.PC_MG, sweeps=args.sweeps)
+        return models_k[mi].solve_step_device(steps[k % len(steps)], dus_k[mi].data_ptr(), tol=tols[hs.PREC_MIXED],
+                                              maxit=2000, precision=hs.PREC_MIXED, precond=hs.PC_MG,
+                                              sweeps=args.sweeps)
 
     for mi in range(1, K):
         solve_on(mi, 0)  # warm-up: graph capture of each instance
@@ -438,40 +578,45 @@ def main():
         "ms_per_step": 1e3 * elapsed_conc / max(args.steps, 1), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": f"{spec['name']}: one cold thermoelastic step solve per step (150 C stress-free "
-                               f"to T in {temps}), PCG to {args.tol:g} scaled relative residual, {K} solves in "
-                               f"flight on {K} model instances",
+                               f"to T in {temps}), PCG to {tols[hs.PREC_MIXED]:g} scaled relative residual (the "
+                               f"loosest tolerance with JEITA warpage within 1e-6 of the FP64 oracle), {K} solves "
+                               f"in flight on {K} model instances",
                    "ndof": mesh.ndof, "elements": mesh.ne, "precision": "FP32 multigrid in FP64 flexible PCG"},
         "solves_in_flight": K,
+        "tol_mixed": tols[hs.PREC_MIXED], "tol_fp64": tols[hs.PREC_FP64],
+        "tol_calibration": {("mixed" if p == hs.PREC_MIXED else "fp64"): c for p, c in cal.items()},
+        "warpage_rel_err_mixed_vs_fp64_oracle": werr_mixed,
+        "warpage_fp64_oracle_vs_reference": (abs(w_ref[0] - golden_w) / abs(golden_w)) if golden_w else None,
         "value_one_at_a_time": dof_iters / elapsed,
         "solve_s_mixed": elapsed / max(args.steps, 1),
         "iters_mixed": [r["iters"] for r in recs],
         "iters_mixed_in_flight": [r["iters"] for r in recs_conc],
         "setup_ms_mixed": float(np.mean([r["ms_assemble"] + r["ms_setup"] for r in recs])),
         "roofline": roof,
+        "composite_roofline": composite_roofline(ktimes, min(args.steps, 2), int(np.mean([r["iters"] for r in recs])),
+                                                 float(np.mean([r["ms_pcg"] for r in recs])), mesh),
     }
 
     if not args.no_fp64:
-        # FP64 comparator on the same steps; warpage error of each mixed solve
-        bga = meshspec.bga_sites(**spec["bga"])
+        # FP64 comparator on the same steps at its own 1e-6-warpage tolerance
         t_fp64, it64, werr = [], [], []
+        solve(0, hs.PREC_FP64)  # graph capture of the FP64 iteration
         for k in range(args.steps):
-            rec = solve(k, hs.PREC_MIXED)
-            u_mixed = du.cpu().numpy().copy()
             torch.cuda.synchronize()
             ta = time.perf_counter()
             rec64 = solve(k, hs.PREC_FP64)
             torch.cuda.synchronize()
             t_fp64.append(time.perf_counter() - ta)
             it64.append(rec64["iters"])
-            u64 = du.cpu().numpy()
-            w_m = post.warpage_jeita(mesh, u_mixed, bga)["signed"]
-            w_d = post.warpage_jeita(mesh, u64, bga)["signed"]
-            werr.append(abs(w_m - w_d) / abs(w_d))
+            w_d = post.warpage_jeita(mesh, du.cpu().numpy(), bga)["signed"]
+            werr.append(abs(w_d - w_ref[k % len(steps)]) / abs(w_ref[k % len(steps)]))
         out["solve_s_fp64"] = float(np.mean(t_fp64))
         out["iters_fp64"] = it64
         out["speedup_mixed_vs_fp64"] = out["solve_s_fp64"] / out["solve_s_mixed"]
-        out["warpage_rel_err_mixed_vs_fp64_max"] = float(max(werr))
-        log(f"FP64 comparator: {out['solve_s_fp64']:.3f} s/step, warpage err max {max(werr):.2e}")
+        out["warpage_rel_err_fp64_vs_fp64_oracle"] = werr
+        out["warpage_rel_err_mixed_vs_fp64_max"] = float(max(werr_mixed))
+        log(f"FP64 comparator: {out['solve_s_fp64']:.3f} s/step at tol {tols[hs.PREC_FP64]:g}, mixed "
+            f"{out['solve_s_mixed']:.3f} s/step at tol {tols[hs.PREC_MIXED]:g}")
 
     if not args.no_adjoint and args.config == "cfg2":
         out["adjoint"] = adjoint_timing(torch, hs, cases, meshspec, args.tol)
@@ -497,6 +642,13 @@ def main():
             f"(dW vs reference {out['submodel']['dW_rel_err_vs_reference']:.1e}); "
             f"{b['submodels']} at once in {b['s']:.1f} s ({b['vs_one_at_a_time']:.1f}x one at a time)")
 
+    if not args.no_cfg3:
+        out["cfg3"] = cfg3_timing(hs, cases, meshspec)
+        c3 = out["cfg3"]
+        log(f"cfg3 bump array ({c3['ndof']} DOF, {c3['points']} Anand points): {c3['steps']} steps in "
+            f"{c3['s']:.1f} s ({c3['ms_per_step']:.0f} ms per step, {c3['newton_per_step']:.2f} Newton per step, "
+            f"{c3['pcg_per_newton']:.1f} PCG per Newton)")
+
     from fcbga_amd import screening as scr
     out["screening"] = screening_timing(hs, scr)
     log(f"screening: {out['screening']['joints']} joints x {out['screening']['steps']} steps in "
@@ -505,6 +657,20 @@ def main():
     if not args.no_cpu:
         log("CPU oracle sample ...")
         base = cpu_baseline(spec, mesh)
+        base.update(host_facts())
+        # the six-temperature sweep as the reference's worker farm runs it: one
+        # process per temperature, min(6, floor(nproc / 1.5)) at once, each the
+        # oracle's assembly + set-up + 5 PCG iterations
+        import multiprocessing as mp
+        nw = max(1, min(len(temps), int(base["nproc"] // 1.5)))
+        t0 = time.perf_counter()
+        with mp.get_context("spawn").Pool(nw) as pool:
+            res = pool.map(_cpu_sample_child, [(args.config, T, 5) for T in temps])
+        dts = time.perf_counter() - t0
+        base["sweep"] = {"processes": nw, "temperatures": len(temps), "seconds": dts,
+                         "value": sum(n * it for n, it, _ in res) / dts, "unit": "DOF*iter/s",
+                         "sample": "per temperature: oracle assembly + multigrid set-up + 5 PCG iterations, "
+                                   "FP64, 1 thread per process"}
         # the same screening loop in the oracle (C, 1 thread) on every 97th joint
         from oracle import oracle as orc
         steps, table, i25 = screening_case(scr)
