@@ -270,6 +270,9 @@ def _mesh_struct(mesh):
         e_scale=np.ascontiguousarray(mesh.e_scale, np.float64),
         cons=np.ascontiguousarray(mesh.constraint_dofs, np.int32),
         fx=np.ascontiguousarray(mesh.feature_x, np.int32), fy=np.ascontiguousarray(mesh.feature_y, np.int32))
+    nsec = int(getattr(mesh, "nsections", 0) or 0)
+    if nsec > 0:
+        a["e_section"] = np.ascontiguousarray(mesh.e_section, np.int32)
     m = FcbgaMesh()
     m.nx, m.ny, m.nz, m.nn, m.ne = mesh.nx, mesh.ny, mesh.nz, mesh.nn, mesh.ne
     m.xs, m.ys, m.zs = _ptr(a["xs"], _f64p), _ptr(a["ys"], _f64p), _ptr(a["zs"], _f64p)
@@ -278,9 +281,9 @@ def _mesh_struct(mesh):
     m.e_group = _ptr(a["e_group"], _i32p)
     m.e_part = _ptr(a["e_part"], _i8p)
     m.e_scale = _ptr(a["e_scale"], _f64p)
-    m.e_section = None
+    m.e_section = _ptr(a["e_section"], _i32p) if nsec > 0 else None
     m.ngroups = len(mesh.groups)
-    m.nsections = 0
+    m.nsections = nsec
     m.nconstraints = len(a["cons"])
     m.constraint_dofs = _ptr(a["cons"], _i32p)
     m.nfeature_x, m.nfeature_y = len(a["fx"]), len(a["fy"])

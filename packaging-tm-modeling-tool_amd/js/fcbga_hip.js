@@ -23,6 +23,50 @@ function load() {
   return addon;
 }
 
+/**
+ * Step data of a prepared FEModel step (FEModel.prepareStep, src/core/09-assembly.js:346-365): per
+ * group the mean stiffness _D, the thermal stress increment _s0 and the birth flag; per layered
+ * section (FEModel.sections, src/core/09-assembly.js:217-219) the zeta-station stiffness weights
+ * averaged over the step (sectionWmean, src/core/04-homogenize.js:206, the core's own function,
+ * passed in), its thermal stress weights _V and birth flag.  A layered group's elements read their
+ * section's data; its own flag is the group's birth (FEModel.isBorn), which scales its elements.
+ */
+function stepFromModel(model, Ta, Tb, stage, sectionWmean) {
+  const groups = model.groups, ng = groups.length, sections = model.sections || [], ns = sections.length;
+  const D = new Float64Array(ng * 36), s0 = new Float64Array(ng * 6), born = new Uint8Array(ng);
+  const isBorn = g => (typeof model.isBorn === 'function' ? model.isBorn(g, stage) : (g.stage || 0) <= stage);
+  groups.forEach((g, q) => {
+    if (g.kind === 'layered') { born[q] = isBorn(g) ? 1 : 0; return; }
+    D.set(g._D, 36 * q);
+    s0.set(g._s0, 6 * q);
+    born[q] = g._born ? 1 : 0;
+  });
+  const step = { Ta, Tb, stage, D, s0, born };
+  if (ns) {
+    if (typeof sectionWmean !== 'function') throw new Error('layered sections need the core\'s sectionWmean');
+    const secW = new Float64Array(ns * 108), secV = new Float64Array(ns * 18), secBorn = new Uint8Array(ns);
+    sections.forEach((sec, q) => {
+      sectionWmean(sec, Ta, Tb, secW.subarray(108 * q, 108 * q + 108));
+      secV.set(sec._V, 18 * q);
+      secBorn[q] = sec._born ? 1 : 0;
+    });
+    Object.assign(step, { secW, secV, secBorn });
+  }
+  return step;
+}
+
+/** The mesh contract for createModel from a FEModel: its mesh plus group count and element sections. */
+function meshForModel(mesh, model) {
+  const sections = model.sections || [];
+  const out = Object.assign({}, mesh, { ngroups: model.groups.length, nsections: sections.length });
+  if (sections.length) {
+    const eSection = new Int32Array(mesh.ne).fill(-1);
+    for (let e = 0; e < mesh.ne; e++) if (model.groups[mesh.eGroup[e]].kind === 'layered') eSection[e] = model.sectionIndex(e);
+    out.eSection = eSection;
+  }
+  return out;
+}
+
 /** Step data from per-group {D (36), s0 (6), born} records (the fields prepareStep sets: _D, _s0, _born). */
 function stepFromGroups(Ta, Tb, stage, groups) {
   const ng = groups.length;
@@ -318,17 +362,24 @@ function attachSubmodel(sm, core) {
 
 function install(Analysis, opts) {
   const original = Analysis.prototype.solveStep;
+  const o = Object.assign({}, opts || {});
+  const wmean = o.sectionWmean || (typeof globalThis.sectionWmean === 'function' ? globalThis.sectionWmean : null);
+  delete o.sectionWmean;
   Analysis.prototype.solveStep = function (k) {
-    if (this.model && this.model.sections && this.model.sections.length) return original.call(this, k);
+    const s = this.path.steps[k];
+    // stacked-module stages hold every node outside the born modules (per-stage constraint
+    // sets, src/core/09-assembly.js:392-424): those steps keep the JS solver
+    if (s.stage < 0 && this.model.moduleConstraints) return original.call(this, k);
+    const sections = this.model.sections || [];
+    if (sections.length && !wmean) return original.call(this, k);
     if (typeof this.checkCancel === 'function') this.checkCancel();
     if (!this._hip) {
-      const mesh = Object.assign({}, this.mesh, { ngroups: this.model.groups.length });
-      this._hip = new HipStepSolver(mesh, Object.assign({ tol: this.solver.opts.tol, maxit: this.solver.opts.maxit }, opts));
+      this._hip = new HipStepSolver(meshForModel(this.mesh, this.model),
+        Object.assign({ tol: this.solver.opts.tol, maxit: this.solver.opts.maxit }, o));
     }
-    const s = this.path.steps[k];
     this.model.prepareStep(s.Ta, s.Tb, s.stage);
     const label = 'step ' + k + ' (' + s.Ta + ' -> ' + s.Tb + ' C)';
-    const res = this._hip.solveStep(stepFromGroups(s.Ta, s.Tb, s.stage, this.model.groups), label);
+    const res = this._hip.solveStep(stepFromModel(this.model, s.Ta, s.Tb, s.stage, wmean), label);
     res.T = s.Tb; res.stage = s.stage; res.step = k;
     if (this.timings) this.timings.solve = (this.timings.solve || 0) + res.ms;
     if (!res.converged) throw new Error('Linear solver did not converge (' + label + '): relative residual ' + res.relres.toExponential(2));
@@ -337,4 +388,5 @@ function install(Analysis, opts) {
   return original;
 }
 
-module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, makeScreenJoints, attachSubmodel, stepFromGroups, install, STATE };
+module.exports = { load, HipStepSolver, HipStateView, HipSensitivity, makeScreenJoints, attachSubmodel, stepFromGroups, stepFromModel, meshForModel,
+  install, STATE };

@@ -100,6 +100,8 @@ class Mesh:
     feature_x: np.ndarray
     feature_y: np.ndarray
     groups: list = field(default_factory=list)
+    e_section: np.ndarray = None  # int32 [ne]: layered section of an element, -1 homogeneous (None: none)
+    nsections: int = 0
 
     @property
     def nx(self): return len(self.xs)
@@ -314,3 +316,38 @@ def save_spec(spec: dict, path: str) -> None:
 def load_spec(path: str) -> dict:
     with open(path) as f:
         return json.load(f)
+
+
+def mesh_from_arrays(xs, ys, zs, node_id, cell_elems, e_group, e_part, e_scale, constraint_dofs, feature_x=(),
+                     feature_y=(), ngroups=None, e_section=None, nsections=0) -> Mesh:
+    """A Mesh from the arrays of the reference's mesh contract (buildStructuredMesh /
+    buildPackageMesh, src/core/06-mesher.js:66-71, 500-545): the derived arrays (element
+    cells, connectivity, coordinates, column starts) are rebuilt from node_id and cell_elems."""
+    xs, ys, zs = (np.asarray(a, np.float64) for a in (xs, ys, zs))
+    nx, ny, nz = len(xs), len(ys), len(zs)
+    node_id = np.asarray(node_id, np.int32)
+    cell_elems = np.asarray(cell_elems, np.int32)
+    ne = len(e_group)
+    e_cell = np.empty(ne, np.int64)
+    slots = np.nonzero(cell_elems >= 0)[0]
+    e_cell[cell_elems[slots]] = slots // 2
+    ncz, ncy = nz - 1, ny - 1
+    ek, ej, ei = e_cell % ncz, (e_cell // ncz) % ncy, e_cell // (ncz * ncy)
+    e_ijk = np.stack([ei, ej, ek], axis=1).astype(np.int32)
+    conn = np.empty((ne, 8), np.int32)
+    for a in range(8):
+        li, lj, lk = HEX_LOC[a]
+        conn[:, a] = node_id[((ei + li) * ny + ej + lj) * nz + ek + lk]
+    exist = node_id.reshape(nx, ny, nz) >= 0
+    I, J, K = np.nonzero(exist)
+    coords = np.stack([xs[I], ys[J], zs[K]], axis=1)
+    col_counts = exist.reshape(nx * ny, nz).sum(axis=1)
+    col_start = np.concatenate([[0], np.cumsum(col_counts)]).astype(np.int32)
+    ng = int(ngroups if ngroups is not None else np.max(e_group) + 1)
+    return Mesh(xs=xs, ys=ys, zs=zs, node_id=node_id, cell_elems=cell_elems,
+                e_group=np.asarray(e_group, np.int32), e_part=np.asarray(e_part, np.int8),
+                e_scale=np.asarray(e_scale, np.float64), e_ijk=e_ijk, conn=conn, coords=coords,
+                col_start=col_start, constraint_dofs=np.asarray(constraint_dofs, np.int32), constraint_nodes=[],
+                feature_x=np.asarray(feature_x, np.int32), feature_y=np.asarray(feature_y, np.int32),
+                groups=[None] * ng,
+                e_section=None if e_section is None else np.asarray(e_section, np.int32), nsections=int(nsections))

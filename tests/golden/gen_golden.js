@@ -15,6 +15,7 @@
 //   screen_golden.json, screen_<case>_table.f64.gz (bump screening on synthetic strain tables, cycle histories, site subsampling)
 //   submodel_golden.json, submodel_*.f64.gz (a bump submodel of the bare-die preset: set-up data and result)
 //   path_golden.json                    (thermal path plans, evaluation temperatures and step sums)
+//   package_<bare|lidded>_golden.json.gz (reference-built Draft package meshes with layered sections, two steps)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -526,7 +527,62 @@ function resultsCases() {
   console.log('results: state', stBytes.length, 'bytes,', steps.length, 'steps; fingerprints', JSON.stringify(fingerprints));
 }
 
+// Reference-built package meshes (the drop-in's own inputs): the bare-die and the lidded
+// default presets at the Draft mesh, built by the reference's buildPackageMesh (graded
+// lines, fill fractions, layered substrate bands), with the FEModel's prepared data of two
+// steps of its thermal path (the first chain step and the last, at 25 C in the final stage)
+// and the reference LinearSolver's solution of each at tol 1e-12 with its JEITA warpage.
+// Written per preset: package_<name>_golden.json.gz (mesh arrays, group and section step
+// data, element sections, solutions and warpage).
+function packageCases() {
+  const db = JSON.parse(fs.readFileSync(path.join(REF, 'src', 'materials-db.json'), 'utf8'));
+  const presetConfig = get('presetConfig'), Analysis = get('Analysis'), sectionWmean = get('sectionWmean');
+  for (const [name, which] of [['bare', 'bare'], ['lidded', 'default']]) {
+    const t0 = Date.now();
+    const cfg = presetConfig(db, which);
+    cfg.mesh.preset = 'draft';
+    const an = new Analysis(cfg, { preset: 'draft', evaluations: 'full' });
+    an.build();
+    const steps = an.buildPath();
+    const m = an.mesh, model = an.model;
+    const eSection = new Int32Array(m.ne).fill(-1);
+    for (let e = 0; e < m.ne; e++) if (model.groups[m.eGroup[e]].kind === 'layered') eSection[e] = model.sectionIndex(e);
+    const chain = an.path.chain, pick = [chain[0], chain[chain.length - 1]];
+    const solver = new LinearSolver(model, { precond: 'mg', tol: 1e-12 });
+    const rhs = new Float64Array(model.ndof), W = new Float64Array(108);
+    const out = [];
+    for (const k of pick) {
+      const s = steps[k];
+      model.assembleStep(s.Ta, s.Tb, s.stage, rhs);
+      const gD = [], gs0 = [], gborn = [];
+      for (const g of model.groups) {
+        if (g.kind === 'layered') { gD.push(...new Float64Array(36)); gs0.push(...new Float64Array(6)); gborn.push(1); continue; }
+        gD.push(...g._D); gs0.push(...g._s0); gborn.push(g._born ? 1 : 0);
+      }
+      const sW = [], sV = [], sborn = [];
+      for (const sec of model.sections) { sectionWmean(sec, s.Ta, s.Tb, W); sW.push(...W); sV.push(...sec._V); sborn.push(sec._born ? 1 : 0); }
+      const rhsCopy = list(rhs);
+      solver.basis = []; solver.pc = null;
+      const res = solver.solve(rhs, 'package ' + name + ' step ' + k);
+      const w = warpageJEITA(m, m.geom, res.u, cfg);
+      out.push({ k, Ta: s.Ta, Tb: s.Tb, stage: s.stage, branch: s.branch, D: gD, s0: gs0, born: gborn, secW: sW, secV: sV,
+        secBorn: sborn, u: list(res.u), rhs: rhsCopy, iters: res.iters, relres: res.relres, warpage: w.signed, warpMag: w.mag });
+      console.log('package', name, 'step', k, s.Ta, '->', s.Tb, 'stage', s.stage, 'iters', res.iters, 'relres', res.relres, 'warpage', w.signed);
+    }
+    const bga = m.geom.bga;
+    const rec = { name, preset: which, nx: m.xs.length, ny: m.ys.length, nz: m.zs.length, nn: m.nn, ne: m.ne,
+      xs: list(m.xs), ys: list(m.ys), zs: list(m.zs), nodeId: list(m.nodeId), cellElems: list(m.cellElems),
+      eGroup: list(m.eGroup), ePart: list(m.ePart), eScale: list(m.eScale), eSection: list(eSection),
+      ngroups: model.groups.length, nsections: model.sections.length, groupKinds: model.groups.map(g => g.kind || 'homog'),
+      constraints: list(m.constraints.dofs), featureX: list(m.featureX || []), featureY: list(m.featureY || []),
+      bga: { x: list(bga.x), y: list(bga.y), zone: bga.zone || null }, steps: out, pathSteps: steps.length };
+    fs.writeFileSync(path.join(HERE, 'package_' + name + '_golden.json.gz'), zlib.gzipSync(Buffer.from(JSON.stringify(rec)), { level: 9 }));
+    console.log('package', name, m.nn * 3, 'DOF', m.ne, 'elements', model.sections.length, 'sections', Date.now() - t0, 'ms');
+  }
+}
+
 const what = process.argv.slice(3);
+if (what.includes('package')) packageCases();
 if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }
 if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
 if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(n);

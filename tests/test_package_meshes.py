@@ -1,0 +1,120 @@
+"""The step solve on the reference's own package meshes: the bare-die and the
+lidded default presets at the Draft mesh, built by the reference's
+buildPackageMesh (graded lines, fill fractions, layered substrate bands),
+with the FEModel's prepared group and section data of two thermal-path steps
+(tests/golden/package_<name>_golden.json.gz, gen_golden.js 'package').
+
+Bar: the assembled load against the reference's to 1e-13, nodal
+displacements within 1e-6 of the largest displacement of the reference's
+solution (tol 1e-12) and JEITA warpage within 0.1 %, in FP64 and mixed
+precision; the adjoint pair runs on the layered mesh and its group
+gradients match central differences."""
+import os
+import shutil
+import subprocess
+import json
+
+import numpy as np
+import pytest
+
+import package_case as pc
+from fcbga_amd import hip_solver as hs
+from fcbga_amd import post
+
+CASES = [("bare", 0), ("bare", 1), ("lidded", 0), ("lidded", 1)]
+
+
+def _check(g, m, q, u, rec):
+    s = g["steps"][q]
+    uref = np.array(s["u"])
+    assert rec["converged"]
+    assert np.abs(u - uref).max() <= 1e-6 * np.abs(uref).max()
+    w = post.warpage_jeita(m, u, pc.bga(g))["signed"]
+    assert abs(w - s["warpage"]) <= 1e-3 * abs(s["warpage"])
+
+
+@pytest.mark.parametrize("name,q", [("bare", 0), ("bare", 1)])
+def test_host_emulation_solves_reference_package(name, q):
+    g, m = pc.load(name)
+    assert m.nsections > 0 and (m.e_section >= 0).any()
+    M = hs.Model(m, variant="hostemu")
+    st = pc.step(g, q)
+    f = M.load(st)
+    rref = np.array(g["steps"][q]["rhs"])
+    assert np.abs(f - rref).max() <= 1e-13 * np.abs(rref).max()
+    u, rec = M.solve_step(st, tol=1e-10, precision=hs.PREC_FP64)
+    _check(g, m, q, u, rec)
+
+
+def test_section_step_data_required():
+    g, m = pc.load("bare")
+    M = hs.Model(m, variant="hostemu")
+    s = g["steps"][0]
+    bad = hs.Step(s["Ta"], s["Tb"], s["stage"], np.array(s["D"]), np.array(s["s0"]), np.array(s["born"], np.uint8))
+    with pytest.raises(hs.SolverError, match="layered sections"):
+        M.solve_step(bad, tol=1e-8)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,q", CASES)
+@pytest.mark.parametrize("prec", [hs.PREC_FP64, hs.PREC_MIXED])
+def test_gpu_solves_reference_package(name, q, prec):
+    g, m = pc.load(name)
+    M = hs.Model(m, variant="hip")
+    u, rec = M.solve_step(pc.step(g, q), tol=1e-10, precision=prec)
+    _check(g, m, q, u, rec)
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_on_layered_mesh():
+    """Group gradients of the warpage proxy on the layered bare-die mesh (the
+    layered substrate group included: its bands' stiffness and thermal load
+    scale together) against central differences of the forward objective."""
+    g, m = pc.load("bare")
+    M = hs.Model(m, variant="hip")
+    s = g["steps"][1]
+    kt = m.nz - 1
+    base = pc.step(g, 1)
+    J0, _, _ = M.adjoint_forward(base, kt, np.zeros(0, np.int32), 8.0, tol=1e-12, precision=hs.PREC_FP64)
+    dlnE, dlnA, _, _ = M.adjoint_backward(np.array([1.0, 0.0]))
+    D, s0 = np.array(s["D"]).reshape(-1, 36), np.array(s["s0"]).reshape(-1, 6)
+    W, V = np.array(s["secW"]).reshape(-1, 108), np.array(s["secV"]).reshape(-1, 18)
+    layered = [i for i, k in enumerate(g["groupKinds"]) if k == "layered"]
+    h = 1e-4
+    worst = 0.0
+    for grp in range(len(D)):
+        Jp = []
+        for sg in (1, -1):
+            f = np.exp(sg * h)
+            D2, s02, W2, V2 = D.copy(), s0.copy(), W.copy(), V.copy()
+            if grp in layered:
+                W2 *= f
+                V2 *= f
+            else:
+                D2[grp] *= f
+                s02[grp] *= f
+            st = hs.Step(s["Ta"], s["Tb"], s["stage"], D2, s02, np.array(s["born"], np.uint8), W2, V2,
+                         np.array(s["secBorn"], np.uint8))
+            J, _, _ = M.adjoint_forward(st, kt, np.zeros(0, np.int32), 8.0, tol=1e-12, precision=hs.PREC_FP64)
+            Jp.append(J[0])
+        fd = (Jp[0] - Jp[1]) / (2 * h)
+        worst = max(worst, abs(fd - dlnE[grp]) / max(np.abs(dlnE).max(), 1e-300))
+    assert worst <= 1e-4, worst
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["bare", "lidded"])
+def test_node_addon_install_on_reference_package(name):
+    """The JS drop-in on a reference-built mesh: install(Analysis) replaces
+    solveStep, the step data of the layered sections come from the model's
+    own sections (sectionWmean) and prepared fields, and both steps' du match
+    the reference's solution."""
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "js", "napi_package.js")
+    out = subprocess.run([node, script, name], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    for r in rec["steps"]:
+        assert r["converged"] and r["rel_err"] <= 1e-6, rec
