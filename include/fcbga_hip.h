@@ -35,6 +35,7 @@ extern "C" {
 #define FCBGA_ERR_DEVICE -2
 #define FCBGA_ERR_NOT_CONVERGED -3
 #define FCBGA_ERR_INTERNAL -4
+#define FCBGA_ERR_CANCELLED -5  /* a progress callback asked to stop (the reference's Error('cancelled')) */
 
 typedef struct fcbga_model fcbga_model;
 
@@ -117,7 +118,26 @@ typedef struct fcbga_solve_stats {
   double ic_shift;         /* Manteuffel shift of the IC(0) factor in use (ICPC.info.shift), 0 without IC */
 } fcbga_solve_stats;
 
+/* Live progress from inside the PCG (LIVE.post / liveReport, src/core/10-solver.js:811-849, 1032) and
+ * in-solve cancellation (the UI's cancel, src/ui/00-app.js:234-238; Analysis.checkCancel,
+ * src/core/11-analysis.js:99).  The callback runs on the calling thread, during the solve call,
+ * every `every` PCG iterations (and at the start and end of each PCG run); a non-zero return stops
+ * the solve, which then returns FCBGA_ERR_CANCELLED with the message "cancelled".  Per model
+ * (fn NULL: none); the state of the last iteration is complete when the callback runs. */
+typedef struct fcbga_progress {
+  int32_t event;     /* 0 start of a PCG run, 1 iteration, 2 end of the run */
+  int32_t iters;     /* iterations so far in this solve (across a fallback) */
+  int32_t maxit;
+  int32_t precond;   /* FCBGA_PC_* of the running PCG */
+  double relres;     /* relative residual in the scaled norm */
+  double r0;         /* relative residual of the initial guess */
+  double tol;
+  double ms;         /* since the start of the PCG run */
+} fcbga_progress;
+typedef int (*fcbga_progress_fn)(const fcbga_progress *p, void *user);
+
 int fcbga_model_create(const fcbga_mesh *mesh, fcbga_model **out);
+int fcbga_set_progress_cb(fcbga_model *m, fcbga_progress_fn fn, void *user, int32_t every);
 void fcbga_model_destroy(fcbga_model *m);
 /* Empties the warm-start basis (a new path, or a stage with another constraint set). */
 int fcbga_warm_reset(fcbga_model *m);

@@ -281,6 +281,13 @@ struct fcbga_model {
   bool colsFactored = false;  // fine column factors of the prepared step exist
   // kernel timing (measurement hook): event pairs per kernel class
   bool timing = false;
+  // live progress and cancellation (fcbga_set_progress_cb)
+  fcbga_progress_fn progFn = nullptr;
+  void *progUser = nullptr;
+  int32_t progEvery = 1;
+  bool cancelled = false;
+  double solveT0 = 0;
+  int32_t itersBase = 0;
   int nsweeps = 2;     // pre- and post-smoothing sweeps per level
   bool useGraphs = true;
 #if FCBGA_GPU
@@ -3108,11 +3115,21 @@ void residual_from_x(fcbga_model *M, const double *bl) {
 
 // PCG iterations from the current x with r = b - A x (_pcg, src/core/10-solver.js:1001-1042);
 // returns the iteration count, rnorm and conv updated
+// progress event to the model's callback; false when it asks to stop
+bool report(fcbga_model *M, int event, int it, const fcbga_solve_opts &o, double relres, double r0) {
+  if (!M->progFn) return true;
+  fcbga_progress p{event, M->itersBase + it, o.maxit, o.precond, relres, r0, o.tol, now_ms() - M->solveT0};
+  if (M->progFn(&p, M->progUser) != 0) M->cancelled = true;
+  return !M->cancelled;
+}
+
 int pcg_iterate(fcbga_model *M, const fcbga_solve_opts &o, double bnorm, double &rnorm, bool &conv) {
   be::stream_t s = M->stream;
   const int64_t n3 = 3 * M->g.G;
   double *r = M->r.p, *z = M->z.p, *p = M->p.p, *sc = M->scal.p;
-  if (rnorm / bnorm < o.tol) { conv = true; return 0; }
+  const double r0 = rnorm / bnorm;
+  if (!report(M, 0, 0, o, r0, r0)) return 0;
+  if (rnorm / bnorm < o.tol) { conv = true; report(M, 2, 0, o, r0, r0); return 0; }
   precondition(M, o.precision, o.precond);
   be::pfor(n3, s, [=] __host__ __device__(int64_t i) { p[i] = z[i]; });
   be::dot_async(n3, r, z, M->partials.p, sc + SC_RZ, s);
@@ -3129,9 +3146,13 @@ int pcg_iterate(fcbga_model *M, const fcbga_solve_opts &o, double bnorm, double 
     read_scalars(M, h);  // rr, breakdown flag
     if (h[1] != 0) break;  // p.Ap <= 0: stop before the update; the iteration counts, as in the reference
     rnorm = std::sqrt(h[0]);
-    if (rnorm / bnorm < o.tol) { conv = true; break; }
+    const bool done = rnorm / bnorm < o.tol;
+    if (M->progFn && (done || it % M->progEvery == 0) && !report(M, 1, it, o, rnorm / bnorm, r0)) break;
+    if (done) { conv = true; break; }
   }
-  return it > o.maxit ? o.maxit : it;
+  const int n = it > o.maxit ? o.maxit : it;
+  if (!M->cancelled) report(M, 2, n, o, rnorm / bnorm, r0);
+  return n;
 }
 
 // PCG on the prepared step: right-hand side the step's load, or rhs_grid
@@ -3143,6 +3164,8 @@ void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, 
              fcbga_solve_stats &S) {
   be::stream_t s = M->stream;
   const double t2 = now_ms();
+  M->cancelled = false;
+  M->solveT0 = t2;
   const int64_t n3 = 3 * M->g.G;
   double *x = M->x.p, *r = M->r.p;
   const double *bl = rhs_grid ? rhs_grid : M->bload.p, *sv = M->s64.p;
@@ -3165,8 +3188,9 @@ void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, 
     S.r0 = 0;
   } else {
     S.r0 = rnorm / bnorm;
+    M->itersBase = 0;
     it = pcg_iterate(M, o, bnorm, rnorm, conv);
-    if (!conv && (o.precond == FCBGA_PC_MG || o.precond == FCBGA_PC_IC)) {
+    if (!conv && !M->cancelled && (o.precond == FCBGA_PC_MG || o.precond == FCBGA_PC_IC)) {
       fcbga_solve_opts o2 = o;
       o2.precond = o.precond == FCBGA_PC_MG ? FCBGA_PC_IC : FCBGA_PC_MG;
       S.fallback_first = o.precond;
@@ -3176,6 +3200,7 @@ void pcg_run(fcbga_model *M, const fcbga_solve_opts &o, const double *rhs_grid, 
       residual_from_x(M, bl);
       rnorm = std::sqrt(dotd(M, r, r, SC_RR));
       S.r0 = rnorm / bnorm;  // the record's iters0 is that of the last run
+      M->itersBase = it;
       it += pcg_iterate(M, o2, bnorm, rnorm, conv);
       S.precond = o2.precond;
     }
@@ -3876,6 +3901,10 @@ static int solve_common(fcbga_model *m, const fcbga_step *step, const fcbga_solv
     fcbga_solve_stats S{};
     step_solve(m, step, opts, dst, du_host, &S);
     if (stats) *stats = S;
+    if (m->cancelled) {
+      g_err = "cancelled";
+      return FCBGA_ERR_CANCELLED;
+    }
     if (!S.converged) {
       char buf[160];
       snprintf(buf, sizeof buf, "Linear solver did not converge: relative residual %.2e after %d iterations",
@@ -3888,6 +3917,15 @@ static int solve_common(fcbga_model *m, const fcbga_step *step, const fcbga_solv
   } catch (const std::exception &e) {
     return fail(e, FCBGA_ERR_INTERNAL);
   }
+  return FCBGA_OK;
+}
+
+int fcbga_set_progress_cb(fcbga_model *m, fcbga_progress_fn fn, void *user, int32_t every) {
+  if (!m) { g_err = "model is NULL"; return FCBGA_ERR_ARG; }
+  if (every < 1) { g_err = "every must be >= 1"; return FCBGA_ERR_ARG; }
+  m->progFn = fn;
+  m->progUser = user;
+  m->progEvery = every;
   return FCBGA_OK;
 }
 
@@ -4067,6 +4105,10 @@ int fcbga_adjoint_forward(fcbga_model *m, const fcbga_step *step, const fcbga_so
     fcbga_solve_stats S{};
     adjoint_forward(m, step, opts, obj, J, du_out, &S);
     if (stats) *stats = S;
+    if (m->cancelled) {
+      g_err = "cancelled";
+      return FCBGA_ERR_CANCELLED;
+    }
     if (!S.converged) {
       g_err = "not converged";
       return FCBGA_ERR_NOT_CONVERGED;
@@ -4086,6 +4128,10 @@ int fcbga_adjoint_backward(fcbga_model *m, const double *gJ, double *dlnE, doubl
     fcbga_solve_stats S{};
     adjoint_backward(m, gJ, dlnE, dlnA, dlnA_elem, &S);
     if (stats) *stats = S;
+    if (m->cancelled) {
+      g_err = "cancelled";
+      return FCBGA_ERR_CANCELLED;
+    }
     if (!S.converged) {
       g_err = "adjoint solve not converged";
       return FCBGA_ERR_NOT_CONVERGED;

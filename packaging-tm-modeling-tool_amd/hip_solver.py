@@ -89,7 +89,7 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_submodel_points", "fcbga_screen_joints_noalloc",
            "fcbga_vp_create", "fcbga_vp_destroy", "fcbga_vp_nsolder", "fcbga_vp_step", "fcbga_vp_save",
            "fcbga_vp_restore", "fcbga_vp_points", "fcbga_vp_displacement", "fcbga_vp_set_displacement",
-           "fcbga_vp_residual",
+           "fcbga_vp_residual", "fcbga_set_progress_cb",
            "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
            "fcbga_path_load64", "fcbga_path_copy",
            "fcbga_mesh_fingerprint", "fcbga_last_error"]
@@ -104,6 +104,15 @@ class FcbgaObjective(C.Structure):
 
 class FcbgaAnandParams(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0", "nu")]
+
+
+class FcbgaProgress(C.Structure):
+    _fields_ = [("event", C.c_int32), ("iters", C.c_int32), ("maxit", C.c_int32), ("precond", C.c_int32),
+                ("relres", C.c_double), ("r0", C.c_double), ("tol", C.c_double), ("ms", C.c_double)]
+
+
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.POINTER(FcbgaProgress), C.c_void_p)
+ERR_CANCELLED = -5
 
 
 class FcbgaVpSolder(C.Structure):
@@ -201,6 +210,8 @@ def load(variant: str = "hip"):
     L.fcbga_mesh_fingerprint.restype = C.c_int
     L.fcbga_warm_reset.argtypes = [C.c_void_p]
     L.fcbga_warm_reset.restype = C.c_int
+    L.fcbga_set_progress_cb.argtypes = [C.c_void_p, PROGRESS_FN, C.c_void_p, C.c_int32]
+    L.fcbga_set_progress_cb.restype = C.c_int
     L.fcbga_vp_create.argtypes = [C.POINTER(FcbgaMesh), C.POINTER(FcbgaVpSolder), C.POINTER(C.c_void_p)]
     L.fcbga_vp_create.restype = C.c_int
     L.fcbga_vp_destroy.argtypes = [C.c_void_p]
@@ -335,6 +346,23 @@ class Model:
         if rc != 0 and (check or rc != -3):
             raise SolverError(f"fcbga_step_solve failed ({rc}): {last_error(self.L)}")
         return du, rec
+
+    def set_progress(self, fn=None, every=1):
+        """Live progress from inside the PCG: fn(record dict) on the calling
+        thread every `every` iterations; a true return stops the solve, which
+        then raises SolverError('cancelled').  fn None removes it."""
+        if fn is None:
+            self._progress = None
+            self.L.fcbga_set_progress_cb(self.h, PROGRESS_FN(), None, 1)
+            return
+
+        def tramp(p, _user):
+            rec = {k: getattr(p.contents, k) for k, _ in FcbgaProgress._fields_}
+            return 1 if fn(rec) else 0
+
+        self._progress = PROGRESS_FN(tramp)  # kept alive while installed
+        if self.L.fcbga_set_progress_cb(self.h, self._progress, None, int(every)) != 0:
+            raise SolverError(last_error(self.L))
 
     def solve_step_device(self, step: Step, du_dev_ptr: int, tol=1e-8, maxit=2000, precision=PREC_FP64,
                           precond=PC_MG, sweeps=0, warm=0):

@@ -87,8 +87,9 @@ class HipStepSolver {
     this.opts = Object.assign({ tol: 1e-8, maxit: 2000, precision: 'mixed', precond: 'mg' }, opts || {});
     this.log = [];
   }
-  solveStep(step, label) {
-    const rec = this.hip.solveStep(this.model, step, this.opts);
+  /** extra: per-call options, e.g. {progress: fn(p), progressEvery: n} (live progress, in-solve cancel). */
+  solveStep(step, label, extra) {
+    const rec = this.hip.solveStep(this.model, step, extra ? Object.assign({}, this.opts, extra) : this.opts);
     rec.label = label;
     this.log.push({ label, iters: rec.iters, relres: rec.relres, converged: rec.converged, ms: rec.ms, pcMs: rec.pcMs, precond: rec.precond, fallback: rec.fallback });
     return rec;
@@ -379,7 +380,16 @@ function install(Analysis, opts) {
     }
     this.model.prepareStep(s.Ta, s.Tb, s.stage);
     const label = 'step ' + k + ' (' + s.Ta + ' -> ' + s.Tb + ' C)';
-    const res = this._hip.solveStep(stepFromModel(this.model, s.Ta, s.Tb, s.stage, wmean), label);
+    // live progress from inside the PCG (liveReport, src/core/10-solver.js:811-849, 1032) and the
+    // in-solve cancel (checkCancel throws Error('cancelled'), which stops the device solve)
+    const self = this, live = typeof globalThis.liveReport === 'function' ? globalThis.liveReport : null;
+    const progress = p => {
+      if (typeof self.checkCancel === 'function') self.checkCancel();
+      if (!live) return;
+      if (p.event === 'start') live({ event: 'start', label, n: 3 * self.mesh.nn, precond: p.precond, building: true, warm: 0 });
+      else if (p.event === 'iter') live({ event: 'iter', label, iters: p.iters, it: p.iters, maxit: p.maxit, relres: p.relres, r0: p.r0, tol: p.tol, precond: p.precond }, true);
+    };
+    const res = this._hip.solveStep(stepFromModel(this.model, s.Ta, s.Tb, s.stage, wmean), label, { progress, progressEvery: 1 });
     res.T = s.Tb; res.stage = s.stage; res.step = k;
     if (this.timings) this.timings.solve = (this.timings.solve || 0) + res.ms;
     if (!res.converged) throw new Error('Linear solver did not converge (' + label + '): relative residual ' + res.relres.toExponential(2));

@@ -249,6 +249,71 @@ bool read_opts(napi_env env, napi_value v, fcbga_solve_opts *o) {
   return true;
 }
 
+// Live progress of a synchronous solve: opts.progress (a function) is called on
+// this thread with {event, iters, maxit, precond, relres, r0, tol, ms} every
+// opts.progressEvery iterations; a truthy return or a throw (the reference's
+// checkCancel throws Error('cancelled')) stops the solve.
+struct ProgressScope {
+  napi_env env;
+  fcbga_model *m;
+  napi_ref fn = nullptr;
+  bool active = false;
+  static int call(const fcbga_progress *p, void *user) {
+    ProgressScope *ps = static_cast<ProgressScope *>(user);
+    napi_env env = ps->env;
+    bool pending = false;
+    napi_is_exception_pending(env, &pending);
+    if (pending) return 1;
+    napi_value fn, rec, res, glob;
+    if (napi_get_reference_value(env, ps->fn, &fn) != napi_ok) return 0;
+    napi_create_object(env, &rec);
+    static const char *ev[3] = {"start", "iter", "end"};
+    napi_value e;
+    napi_create_string_utf8(env, ev[p->event < 0 || p->event > 2 ? 1 : p->event], NAPI_AUTO_LENGTH, &e);
+    napi_set_named_property(env, rec, "event", e);
+    set_num(env, rec, "iters", p->iters);
+    set_num(env, rec, "maxit", p->maxit);
+    set_num(env, rec, "relres", p->relres);
+    set_num(env, rec, "r0", p->r0);
+    set_num(env, rec, "tol", p->tol);
+    set_num(env, rec, "ms", p->ms);
+    napi_value pc;
+    napi_create_string_utf8(env, pc_name(p->precond), NAPI_AUTO_LENGTH, &pc);
+    napi_set_named_property(env, rec, "precond", pc);
+    napi_get_global(env, &glob);
+    if (napi_call_function(env, glob, fn, 1, &rec, &res) != napi_ok) return 1;  // threw: stop
+    bool stop = false;
+    napi_valuetype t;
+    napi_typeof(env, res, &t);
+    if (t == napi_boolean) napi_get_value_bool(env, res, &stop);
+    return stop ? 1 : 0;
+  }
+  ProgressScope(napi_env e, fcbga_model *model, napi_value opts) : env(e), m(model) {
+    napi_valuetype t;
+    if (!opts || napi_typeof(env, opts, &t) != napi_ok || t != napi_object) return;
+    napi_value f;
+    if (!get_prop(env, opts, "progress", &f)) return;
+    napi_typeof(env, f, &t);
+    if (t != napi_function) return;
+    int32_t every = 1;
+    get_int(env, opts, "progressEvery", &every);
+    if (every < 1) every = 1;
+    napi_create_reference(env, f, 1, &fn);
+    active = fcbga_set_progress_cb(m, &ProgressScope::call, this, every) == FCBGA_OK;
+  }
+  ~ProgressScope() {
+    if (active) fcbga_set_progress_cb(m, nullptr, nullptr, 1);
+    if (fn) napi_delete_reference(env, fn);
+  }
+  // after a stopped solve: the callback's own exception if it threw, else Error('cancelled')
+  napi_value cancelled() {
+    bool pending = false;
+    napi_is_exception_pending(env, &pending);
+    if (!pending) napi_throw_error(env, nullptr, "cancelled");
+    return nullptr;
+  }
+};
+
 // precond of the final run, fallback {first, iters, relres} or null, and the IC(0) shift
 void set_pc_fields(napi_env env, napi_value rec, const fcbga_solve_stats &S) {
   napi_value pc, fb;
@@ -289,7 +354,12 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_create_arraybuffer(env, sizeof(double) * (size_t)ndof, &udata, &ab));
   fcbga_solve_stats S;
   memset(&S, 0, sizeof S);
-  int rc = fcbga_step_solve(b->m, &st, &o, (double *)udata, &S);
+  int rc;
+  {
+    ProgressScope ps(env, b->m, argc >= 3 ? argv[2] : nullptr);
+    rc = fcbga_step_solve(b->m, &st, &o, (double *)udata, &S);
+    if (rc == FCBGA_ERR_CANCELLED) return ps.cancelled();
+  }
   if (rc != FCBGA_OK && rc != FCBGA_ERR_NOT_CONVERGED) return throw_msg(env, "solveStep: " + lib_error());
   napi_value u;
   NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)ndof, ab, 0, &u));
@@ -806,7 +876,12 @@ napi_value PathCall(napi_env env, napi_callback_info info) {
     if (argc >= 5 && !read_opts(env, argv[4], &so)) return throw_msg(env, "pathCall solve: unknown opts.precond");
     fcbga_solve_stats S;
     memset(&S, 0, sizeof S);
-    const int rc = fcbga_path_solve(b->m, k, &st, &so, &S);
+    int rc;
+    {
+      ProgressScope ps(env, b->m, argc >= 5 ? argv[4] : nullptr);
+      rc = fcbga_path_solve(b->m, k, &st, &so, &S);
+      if (rc == FCBGA_ERR_CANCELLED) return ps.cancelled();
+    }
     if (rc != FCBGA_OK && rc != FCBGA_ERR_NOT_CONVERGED) return throw_msg(env, "pathCall solve: " + lib_error());
     return solve_record(env, S);
   }

@@ -202,3 +202,23 @@ def test_node_addon_step_solve(lib, name, prec):
     assert out.returncode == 0, out.stderr
     rec = json.loads(out.stdout.strip().splitlines()[-1])
     assert rec["converged"] and rec["rel_err"] <= 1e-6, rec
+
+
+def test_node_addon_progress_and_cancel(lib):
+    """Live progress and cancel through the Node addon: one 'iter' event per
+    PCG iteration between 'start' and 'end', a callback that throws (the
+    reference's checkCancel) or returns true stops the solve, and the next
+    solve on the same model converges as before."""
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node not installed")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "js", "napi_progress.js")
+    out = subprocess.run([node, script, "cfg1"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    assert r["iterEvents"] == r["iters"] and r["monotone"] and r["first"] == "start" and r["last"] == "end", r
+    assert abs(r["lastRelres"] - r["relres"]) <= 1e-12 * r["relres"], r
+    assert r["cancelled"] == "cancelled" and r["seen"] == 3 and r["stopped"] == "cancelled", r
+    assert r["againConverged"] and r["againIters"] == r["iters"], r

@@ -130,3 +130,24 @@ def test_node_addon_loads_and_exports():
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip() == ("adjointBackward,adjointForward,createModel,destroyModel,pathCall,screenJoints,solveStep,"
                                   "stateAccumulate,stateStress,submodelCall,submodelCreate")
+
+
+def test_progress_callback_and_cancel():
+    """Live progress from inside the PCG and the in-solve cancel (the reference's
+    LIVE.post and checkCancel): one 'iter' event per iteration with the
+    running relative residual, start and end events, and a callback that
+    asks to stop ends the solve with 'cancelled'; the model stays usable."""
+    spec, m, M, st = _model("plate_16")
+    ev = []
+    M.set_progress(lambda p: ev.append(p) and False, every=1)
+    u, rec = M.solve_step(st, tol=1e-10)
+    iters = [p for p in ev if p["event"] == 1]
+    assert ev[0]["event"] == 0 and ev[-1]["event"] == 2
+    assert len(iters) == rec["iters"] and [p["iters"] for p in iters] == list(range(1, rec["iters"] + 1))
+    assert iters[-1]["relres"] == pytest.approx(rec["relres"], rel=1e-12)
+    M.set_progress(lambda p: p["event"] == 1 and p["iters"] >= 3)
+    with pytest.raises(hs.SolverError, match="cancelled"):
+        M.solve_step(st, tol=1e-10)
+    M.set_progress(None)
+    u2, rec2 = M.solve_step(st, tol=1e-10)
+    assert rec2["converged"] and np.array_equal(u2, u)
