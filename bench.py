@@ -21,7 +21,8 @@ Beside the timed mixed-precision solves the run also reports the plain FP64
 PCG-MG solve of the same steps at its own 1e-6-warpage tolerance (the
 comparator), the dominant kernel's roofline and the composite per-iteration
 roofline from HIP-event timing, the bump-array viscoplastic cycling workload
-(120 steps of the global Newton), and the CPU oracle (a single-threaded C
+(120 steps of the global Newton), a sample of the 256 batched bump
+submodels (60 steps each, several at once), and the CPU oracle (a single-threaded C
 restatement of the reference's FP64 PCG-MG) on a bounded sample of the same
 workload, one process per temperature.
 
@@ -186,6 +187,35 @@ def cfg3_timing(hs, cases, meshspec, nsteps=None):
                                 "corner_bump": float(dW[corner]), "argmax_bump": int(np.nanargmax(dW)),
                                 "per_cycle_max": [float(np.nanmax(c["dW"])) for c in pc], "unit": "MPa"}
     return out
+
+
+def cfg5_timing(hs, cases, nsub=8, workers=4):
+    """Batched bump submodels: the first nsub of the 256 seeded sphere-cap voxel
+    submodels (24^3 .. 56^3 voxels), each through its 60-step cycle, `workers`
+    at once (one host thread and device stream each).  Reports the sample's
+    per-submodel times (p50, p99), its aggregate DOF x PCG iterations per
+    second, and the 256-submodel time extrapolated from the sample in
+    proportion to DOF (labelled as extrapolated)."""
+    from fcbga_amd import viscoplastic as vpm
+    draws = cases.cfg5_draws()
+    specs = [cases.cfg5_spec(d) for d in draws[:nsub]]
+    t0 = time.perf_counter()
+    recs = vpm.run_submodel_batch(specs, workers=workers)
+    wall = time.perf_counter() - t0
+    times = np.array([r["wall_s"] for r in recs])
+    dof = np.array([r["ndof"] for r in recs], np.float64)
+    dof_all = float(sum(3 * (d["n"] + 1) ** 3 for d in draws))
+    return {"submodels_total": len(draws), "sample": nsub, "workers": workers, "steps": recs[0]["steps"],
+            "wall_s": wall, "p50_s": float(np.percentile(times, 50)), "p99_s": float(np.percentile(times, 99)),
+            "dof_sample": int(dof.sum()), "dof_all": int(dof_all),
+            "dof_iters_per_s": float(sum(r["ndof"] * r["pcg_iters"] for r in recs) / wall),
+            "gp_updates_per_s": float(sum(8 * r["solder_elements"] * r["newton"] for r in recs) / wall),
+            "newton_per_step": float(np.mean([r["newton"] / max(r["solves"], 1) for r in recs])),
+            "cutbacks": int(sum(r["cutbacks"] for r in recs)),
+            "dW_MPa": [r["dW"] for r in recs], "n": [r["n"] for r in recs],
+            "all_256_s_extrapolated": wall * dof_all / dof.sum(),
+            "note": "seeded std::mt19937_64 draws (seed 20261203); the 256-submodel time is extrapolated from the "
+                    "sample in proportion to DOF"}
 
 
 def _cpu_sample_child(args):
@@ -451,6 +481,8 @@ def main():
     ap.add_argument("--no-path", action="store_true")
     ap.add_argument("--no-submodel", action="store_true")
     ap.add_argument("--no-cfg3", action="store_true")
+    ap.add_argument("--no-cfg5", action="store_true")
+    ap.add_argument("--cfg5-sample", type=int, default=8, help="submodels of the 256 timed in the batched leg")
     ap.add_argument("--fixed-tol", dest="tol_from_warpage", action="store_false",
                     help="time at --tol instead of the tolerance calibrated to 1e-6 warpage error")
     ap.add_argument("--submodel-batch", type=int, default=0, help=argparse.SUPPRESS)
@@ -648,6 +680,13 @@ def main():
         log(f"cfg3 bump array ({c3['ndof']} DOF, {c3['points']} Anand points): {c3['steps']} steps in "
             f"{c3['s']:.1f} s ({c3['ms_per_step']:.0f} ms per step, {c3['newton_per_step']:.2f} Newton per step, "
             f"{c3['pcg_per_newton']:.1f} PCG per Newton)")
+
+    if not args.no_cfg5:
+        out["cfg5"] = cfg5_timing(hs, cases, nsub=args.cfg5_sample)
+        c5 = out["cfg5"]
+        log(f"cfg5 batched submodels: {c5['sample']} of {c5['submodels_total']} ({c5['dof_sample']} DOF) in "
+            f"{c5['wall_s']:.1f} s with {c5['workers']} in flight; p50 {c5['p50_s']:.1f} s p99 {c5['p99_s']:.1f} s; "
+            f"{c5['dof_iters_per_s']:.3e} DOF*iter/s; all 256 extrapolated {c5['all_256_s_extrapolated']:.0f} s")
 
     from fcbga_amd import screening as scr
     out["screening"] = screening_timing(hs, scr)

@@ -355,6 +355,10 @@ int fcbga_vp_step(fcbga_vp *v, const fcbga_step *elastic, double T, double dt, c
                   fcbga_vp_stats *stats);
 int fcbga_vp_save(fcbga_vp *v);
 int fcbga_vp_restore(fcbga_vp *v);
+/* prescribed displacement of the constrained DOFs (nconstraints values in the mesh's constraint_dofs
+ * order; zero until set), applied by every later step: the boundary of a bump submodel, whose six
+ * faces follow the global field (BumpSubmodel.prescribe, src/core/13-fatigue.js:646-693) */
+int fcbga_vp_set_prescribed(fcbga_vp *v, const double *values);
 /* committed point states, 8 x FCBGA_ANAND_STATE per solder element in the order of elems (the mesh
  * element of each solder element, nsolder entries), the points of an element in corner order */
 int fcbga_vp_points(fcbga_vp *v, double *st, int32_t *elems);

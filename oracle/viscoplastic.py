@@ -86,15 +86,17 @@ class VpOracle:
         m2 = _copy_mesh(mesh)
         m2.e_scale = mesh.e_scale.copy()
         m2.e_scale[self.solder_elems] = 0.0
+        m2.constraint_dofs = np.zeros(0, np.int32)  # unconstrained: prescribed values enter the free rows
         Tref = solder["Tref"]
         D, s0, born = meshspec.group_step_data(spec, Tref, Tref + 1.0, 0)
         S = orc.System(m2, D, s0, born)
         rp, col, val = S.csr()
         self.ndof = S.ndof
         self.K_el = _bcsr_to_csr(rp, col, val, S.nn)
-        self.f_unit = S.rhs()  # thermal load of the elastic elements per kelvin from Tref
         self.free = np.ones(self.ndof, bool)
         self.free[mesh.constraint_dofs] = False
+        self.f_unit = S.rhs() * self.free  # thermal load of the elastic elements per kelvin from Tref
+        self.bc = np.zeros(len(mesh.constraint_dofs))  # prescribed values of the constrained DOFs
         P = solder["anand"]
         self.P = [P[k] for k in ("A", "QR", "xi", "m", "shat", "n", "h0", "a", "s0")] + [solder["nu"]]
         self.nu = solder["nu"]
@@ -138,9 +140,10 @@ class VpOracle:
         fs = np.zeros(self.ndof)
         np.add.at(fs, self.dofs.reshape(-1), fe.reshape(-1))
         fth = self.f_unit * (T - self.sol["Tref"])
-        r = fth - self.K_el @ u - fs
+        ku = (self.K_el @ u) * self.free
+        r = fth - ku - fs
         r[~self.free] = 0.0
-        scale = np.linalg.norm(fth) + np.linalg.norm(fs)
+        scale = np.linalg.norm(fth) + np.linalg.norm(fs) + np.linalg.norm(ku)
         Kt = None
         if tangent:
             Dq = D.reshape(-1, 36).copy()
@@ -163,6 +166,7 @@ class VpOracle:
         """One time step from the committed state (the stepTo logic with exact
         inner solves); commits on convergence.  Returns the record."""
         u = self.u.copy()
+        u[self.mesh.constraint_dofs] = self.bc
         conv, it = False, 0
         for it in range(1, maxit + 1):
             r, scale, Kt, _, _ = self.evaluate(u, T, dt)

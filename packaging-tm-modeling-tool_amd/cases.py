@@ -11,6 +11,8 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
+
 from .meshspec import PART, linspace_ref
 
 # ---- materials (E in MPa, CTE in ppm/K) -----------------------------------------
@@ -208,6 +210,127 @@ def cfg3_small_spec(n: int = 2, solder_anand: dict = None, half: float = 1.0) ->
         "top_plane": 10,
         "tol": 1e-10,
     }
+
+
+class MT19937_64:
+    """The 64-bit Mersenne Twister (std::mt19937_64: Matsumoto and Nishimura's
+    MT19937-64 with the standard's default seeding)."""
+    NN, MM = 312, 156
+    MATRIX_A, UM, LM = 0xB5026F5AA96619E9, 0xFFFFFFFF80000000, 0x7FFFFFFF
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.mt = [seed & self.MASK]
+        for i in range(1, self.NN):
+            prev = self.mt[-1]
+            self.mt.append((6364136223846793005 * (prev ^ (prev >> 62)) + i) & self.MASK)
+        self.mti = self.NN
+
+    def next(self) -> int:
+        mt = self.mt
+        if self.mti >= self.NN:
+            for i in range(self.NN):
+                x = (mt[i] & self.UM) | (mt[(i + 1) % self.NN] & self.LM)
+                xa = x >> 1
+                if x & 1:
+                    xa ^= self.MATRIX_A
+                mt[i] = mt[(i + self.MM) % self.NN] ^ xa
+            self.mti = 0
+        x = mt[self.mti]
+        self.mti += 1
+        x ^= (x >> 29) & 0x5555555555555555
+        x ^= (x << 17) & 0x71D67FFFEDA60000
+        x ^= (x << 37) & 0xFFF7EEE000000000
+        x ^= x >> 43
+        return x & self.MASK
+
+    def uniform(self) -> float:
+        """A double in [0, 1) from the top 53 bits."""
+        return (self.next() >> 11) * 2.0 ** -53
+
+
+CFG5_SEED = 20261203
+UNDERFILL5 = dict(name="underfill", E=8000.0, nu=0.3, cte=30.0, part=PART["UF_BUMP"])
+
+
+def cfg5_draws(count: int = 256, seed: int = CFG5_SEED) -> list:
+    """The seeded parameters of the batched bump submodels: per submodel, from
+    one std::mt19937_64 stream in this order, the voxel count n = 24 + (r mod 33)
+    (uniform on 24..56), the sphere diameter as a fraction 0.6 + 0.3 u of the
+    cube edge, the submodel's position (x, y) = 4 + 6 (u1, u2) mm from the die
+    centre (corner and edge region of a 20 mm die), and 3 x 9 amplitudes (within
+    2e-8 mm/K) and phases of the boundary field's smooth part."""
+    g = MT19937_64(seed)
+    out = []
+    for q in range(count):
+        n = 24 + g.next() % 33
+        frac = 0.6 + 0.3 * g.uniform()
+        x0, y0 = 4.0 + 6.0 * g.uniform(), 4.0 + 6.0 * g.uniform()
+        amp = [[2e-8 * (2 * g.uniform() - 1) for _ in range(9)] for _ in range(3)]
+        ph = [[2 * math.pi * g.uniform() for _ in range(9)] for _ in range(3)]
+        out.append({"id": q, "n": int(n), "frac": frac, "x0": x0, "y0": y0, "amp": amp, "phase": ph})
+    return out
+
+
+def cfg5_spec(d: dict, edge: float = 0.15) -> dict:
+    """One batched bump submodel: a cube of edge 0.15 mm in n^3 voxels, the
+    bottom n/6 voxel layers the substrate (core of cfg1), the top n/6 silicon,
+    between them underfill (E 8 GPa, CTE 30 ppm/K) around a SAC305 sphere of
+    diameter frac x edge centred in the joint layer, cut by the two slabs into
+    a sphere cap; the cube's six faces follow the boundary field; one cycle of
+    the bump-array workload's history (60 steps)."""
+    n = d["n"]
+    lines = linspace_ref(0.0, edge, n)
+    slab = max(3, int(round(n / 6)))
+    zc = (lines[slab] + lines[n - slab]) / 2
+    sol = dict(SAC305)
+    return {
+        "name": f"cfg5_submodel_{d['id']}_n{n}",
+        "xs": lines, "ys": list(lines), "zs": list(lines),
+        "materials": [CORE, sol, UNDERFILL5, SI],
+        "cells": [
+            {"k0": 0, "k1": slab, "mat": 0},
+            {"k0": n - slab, "k1": n, "mat": 3},
+            {"k0": slab, "k1": n - slab, "mat": 1, "sphere": {"c": [edge / 2, edge / 2, zc], "r": d["frac"] * edge / 2}},
+            {"k0": slab, "k1": n - slab, "mat": 2},
+        ],
+        "constrain_faces": True,
+        "solder_group": 1, "Tref": 25.0, "bump_layer": [slab, n - slab], "draw": d,
+        "cycle": {"Tmin": -40.0, "Tmax": 125.0, "cycles": 1, "steps_per_segment": 15, "segment_s": 900.0},
+        "Ta": 25.0, "Tb": 125.0, "stage": 0, "feature_x": [], "feature_y": [],
+        "bga": {"pitch": 0.05, "inset": 0.025, "sx": edge, "sy": edge}, "top_plane": n, "tol": 1e-10,
+    }
+
+
+CFG5_SHEAR_SHARE = 0.15  # share of the die/substrate CTE mismatch the joint layer takes (package compliance)
+
+
+def cfg5_boundary(spec: dict, coords, T: float):
+    """Prescribed displacement (mm) at points coords (k, 3) of a submodel at T:
+    (T - Tref) g(x).  In-plane: the substrate's expansion (15 ppm/K) at the
+    submodel's position (x0, y0) from the die centre, plus, across the joint
+    layer (linearly in z from the substrate slab's top to the silicon slab's
+    bottom, constant in the slabs), CFG5_SHEAR_SHARE of the silicon/substrate
+    mismatch (2.6 - 15 ppm/K): the bump's shear.  Out of plane: the expansion
+    integrated through the layers (substrate 15, joint 30, silicon 2.6 ppm/K).
+    Plus the seeded smooth part, sum over k in {0,1,2}^2 of
+    a_k cos(pi (k1 x + k2 z) / edge + phase_k) per component."""
+    d = spec["draw"]
+    edge = spec["xs"][-1] - spec["xs"][0]
+    k0, k1 = spec["bump_layer"]
+    zb, zt = spec["zs"][k0], spec["zs"][k1]
+    a_sub, a_si, a_joint = 15e-6, 2.6e-6, 30e-6
+    c = np.asarray(coords, np.float64)
+    z = c[:, 2]
+    zeta = np.clip((z - zb) / (zt - zb), 0.0, 1.0)
+    a_in = a_sub + CFG5_SHEAR_SHARE * (a_si - a_sub) * zeta
+    uz = a_sub * np.minimum(z, zb) + a_joint * (np.clip(z, zb, zt) - zb) + a_si * np.maximum(z - zt, 0.0)
+    g = np.stack([a_in * (d["x0"] + c[:, 0]), a_in * (d["y0"] + c[:, 1]), uz], axis=1)
+    for comp in range(3):
+        for q in range(9):
+            q1, q2 = q // 3, q % 3
+            g[:, comp] += d["amp"][comp][q] * np.cos(np.pi * (q1 * c[:, 0] + q2 * c[:, 2]) / edge + d["phase"][comp][q])
+    return (T - spec["Tref"]) * g
 
 
 def small_box_spec() -> dict:

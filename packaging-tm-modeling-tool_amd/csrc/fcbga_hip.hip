@@ -4583,6 +4583,19 @@ int fcbga_vp_displacement(fcbga_vp *v, double *u) {
   return FCBGA_OK;
 }
 
+int fcbga_vp_set_prescribed(fcbga_vp *v, const double *values) {
+  if (!v || !values) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  try {
+    if (v->bcGrid.n) {
+      be::h2d(v->bcVal.p, values, sizeof(double) * v->bcVal.n, v->M->stream);
+      be::sync(v->M->stream);
+    }
+  } catch (const std::exception &e) {
+    return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
 int fcbga_vp_set_displacement(fcbga_vp *v, const double *u) {
   if (!v || !u) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
   try {

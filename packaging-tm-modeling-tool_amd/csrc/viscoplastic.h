@@ -39,9 +39,12 @@ struct fcbga_vp {
   DBuf<double> st, tr, snapSt; // per Gauss point, 11 each
   DBuf<double> Ke, fe, ce, dmaxE;
   DBuf<int32_t> badE;
+  DBuf<double> gpSig, gpD, gpDep;  // per Gauss point: stress (6), tangent (36), plastic increment
   DBuf<double> u, snapU;       // total displacement, grid layout (3 per grid node)
   DBuf<double> rg, fsv, xs_, duNode;
   std::vector<int32_t> elemOf; // mesh element of every solder element
+  DBuf<int64_t> bcGrid;        // grid DOF of every constrained DOF (the mesh's constraint order)
+  DBuf<double> bcVal;          // its prescribed displacement (zero unless set)
   double Tn = 25, snapTn = 25;
   bool haveSnap = false;
 };
@@ -107,18 +110,74 @@ FE_HD void upper_entry(int idx, int &a, int &b) {
   b = a + rem;
 }
 
+// B-bar strain (engineering shear) at a point of a box from its corner
+// displacements, without forming the 6 x 24 matrix: the normal rows share the
+// volumetric part of the centre's derivatives, sum_a ((c_a - d_a) / 3) . u_a,
+// and add their own derivative; the shear rows are the standard ones (the
+// rows of fe::bbar_matrix applied to u).
+FE_HD void bbar_strain(double hx, double hy, double hz, double xi, double eta, double zeta, const double *ue,
+                       double eps[6]) {
+  double dx[8], dy[8], dz[8], cx[8], cy[8], cz[8];
+  fe::box_corner_derivs(hx, hy, hz, xi, eta, zeta, dx, dy, dz);
+  fe::box_corner_derivs(hx, hy, hz, 0, 0, 0, cx, cy, cz);
+  double vol = 0, e0 = 0, e1 = 0, e2 = 0, g01 = 0, g12 = 0, g20 = 0;
+  for (int a = 0; a < 8; a++) {
+    const double ux = ue[3 * a], uy = ue[3 * a + 1], uz = ue[3 * a + 2];
+    vol += (cx[a] - dx[a]) / 3 * ux + (cy[a] - dy[a]) / 3 * uy + (cz[a] - dz[a]) / 3 * uz;
+    e0 += dx[a] * ux;
+    e1 += dy[a] * uy;
+    e2 += dz[a] * uz;
+    g01 += dy[a] * ux + dx[a] * uy;
+    g12 += dz[a] * uy + dy[a] * uz;
+    g20 += dz[a] * ux + dx[a] * uz;
+  }
+  eps[0] = e0 + vol; eps[1] = e1 + vol; eps[2] = e2 + vol;
+  eps[3] = g01; eps[4] = g12; eps[5] = g20;
+}
+
 #if FCBGA_GPU
-// One wavefront per solder element: the Anand trial states of its 8 points
-// (lanes 0..7) at the mechanical strain B u - F, the internal force
-// f = sum w B^T sigma, the tangent K = sum w B^T D B (300 upper entries over
-// the lanes, mirrored; only when wantK) and c = K u_e - f, the element's
-// term of the residual correction; the largest plastic increment and a flag
-// for non-finite stresses.
-__global__ void __launch_bounds__(64 * VP_EW) k_vp_elements(VpView v, fe::AnandParams P, double T, double dt,
-                                                           double E, double fsol, int wantK,
-                                                           const double *__restrict__ u, double *st, double *tr,
-                                                           double *Kout, double *fe_, double *ce, double *dmaxE,
-                                                           int32_t *badE) {
+// One thread per solder Gauss point: the point's B-bar rows at its own
+// position, the mechanical strain B u_e - F, the Anand trial state from the
+// committed one, the stress and (wantK) the consistent tangent (the secant
+// one where it is not finite).  Every lane of a wavefront runs a return map.
+__global__ void __launch_bounds__(256) k_vp_points(VpView v, fe::AnandParams P, double T, double dt, double E,
+                                                   double fsol, int wantK, const double *__restrict__ u, double *st,
+                                                   double *tr, double *__restrict__ sigOut, double *__restrict__ Dout,
+                                                   double *__restrict__ depOut) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 8 * (int64_t)v.ns) return;
+  const int64_t e = q >> 3;
+  const int gq = (int)(q & 7);
+  const double hx = v.h[3 * e], hy = v.h[3 * e + 1], hz = v.h[3 * e + 2];
+  double ue[24];
+  for (int a = 0; a < 8; a++) {
+    const int64_t gn = v.nodes[8 * e + a];
+    for (int c = 0; c < 3; c++) ue[3 * a + c] = u[3 * gn + c];
+  }
+  double xi, eta, zeta;
+  vp_gauss(gq, xi, eta, zeta);
+  double eps[6], sig[6], D[36];
+  bbar_strain(hx, hy, hz, xi, eta, zeta, ue, eps);
+  eps[0] -= fsol; eps[1] -= fsol; eps[2] -= fsol;
+  const double d = fe::anand_step(P, E, T, dt, eps, st + 11 * q, tr + 11 * q, sig, wantK ? D : nullptr, true);
+  if (wantK) {
+    if (!all_finite(D, 36)) vp_secant_tangent(P, E, d, tr[11 * q + 9], D);
+    for (int c = 0; c < 36; c++) Dout[36 * q + c] = D[c];
+  }
+  for (int r = 0; r < 6; r++) sigOut[6 * q + r] = sig[r];
+  depOut[q] = all_finite(sig, 6) ? d : -1.0;  // -1: non-finite stress
+}
+
+// One wavefront per solder element, from its points' stresses and tangents:
+// the internal force f = sum w B^T sigma, the tangent K = sum w B^T D B (300
+// upper entries over the lanes, mirrored; only when wantK) and c = K u_e - f,
+// the element's term of the residual correction; the largest plastic
+// increment and a flag for non-finite stresses.
+__global__ void __launch_bounds__(64 * VP_EW) k_vp_elements(VpView v, int wantK, const double *__restrict__ u,
+                                                           const double *__restrict__ gpSig,
+                                                           const double *__restrict__ gpD,
+                                                           const double *__restrict__ gpDep, double *Kout, double *fe_,
+                                                           double *ce, double *dmaxE, int32_t *badE) {
   __shared__ double Bs[VP_EW][8 * 144];
   __shared__ double Ds[VP_EW][8 * 36];
   __shared__ double Ss[VP_EW][8 * 6];
@@ -137,32 +196,22 @@ __global__ void __launch_bounds__(64 * VP_EW) k_vp_elements(VpView v, fe::AnandP
       double xi, eta, zeta;
       vp_gauss(lane, xi, eta, zeta);
       fe::bbar_matrix(hx, hy, hz, xi, eta, zeta, B + 144 * lane);
+      d = gpDep[8 * e + lane];
+      if (d < 0) bad = 1, d = 0;
     }
-  }
-  __syncthreads();
-  if (valid && lane < 8) {
-    const double *Bg = B + 144 * lane;
-    double eps[6], sig[6];
-    for (int r = 0; r < 6; r++) {
-      double s = 0;
-      for (int c = 0; c < 24; c++) s += Bg[r * 24 + c] * ue[c];
-      eps[r] = s;
-    }
-    eps[0] -= fsol; eps[1] -= fsol; eps[2] -= fsol;
-    const int64_t gq = 8 * e + lane;
-    double *Dg = Dl + 36 * lane;
-    d = fe::anand_step(P, E, T, dt, eps, st + 11 * gq, tr + 11 * gq, sig, wantK ? Dg : nullptr, true);
-    if (wantK && !all_finite(Dg, 36)) vp_secant_tangent(P, E, d, tr[11 * gq + 9], Dg);
-    if (!all_finite(sig, 6)) bad = 1;
-    for (int r = 0; r < 6; r++) Sg[6 * lane + r] = sig[r];
+    if (lane < 48) Sg[lane] = gpSig[48 * e + lane];
+    if (wantK)
+      for (int c = lane; c < 288; c += 64) Dl[c] = gpD[288 * e + c];
   }
   __syncthreads();
   double *Ke = Kout + 576 * e;
   if (valid && wantK) {
+#pragma unroll 1
     for (int idx = lane; idx < 300; idx += 64) {
       int a, bc;
       upper_entry(idx, a, bc);
       double acc = 0;
+#pragma unroll 1
       for (int gq = 0; gq < 8; gq++) {
         const double *Bg = B + 144 * gq, *Dg = Dl + 36 * gq;
         double s = 0;
@@ -218,13 +267,8 @@ FE_HD void vp_element_host(const VpView &v, const fe::AnandParams &P, int64_t e,
     double xi, eta, zeta;
     vp_gauss(gq, xi, eta, zeta);
     fe::bbar_matrix(hx, hy, hz, xi, eta, zeta, B + 144 * gq);
-    const double *Bg = B + 144 * gq;
     double eps[6], sig[6];
-    for (int r = 0; r < 6; r++) {
-      double s = 0;
-      for (int c = 0; c < 24; c++) s += Bg[r * 24 + c] * ue[c];
-      eps[r] = s;
-    }
+    bbar_strain(hx, hy, hz, xi, eta, zeta, ue, eps);
     eps[0] -= fsol; eps[1] -= fsol; eps[2] -= fsol;
     const int64_t g8 = 8 * e + gq;
     double *Dg = Dl + 36 * gq;
@@ -277,9 +321,13 @@ void vp_elements(fcbga_vp *V, double T, double dt, int wantK) {
   const double E = vp_E(V, T), fsol = V->cte * (T - V->Tref);
   if (V->ns == 0) return;
 #if FCBGA_GPU
-  hipLaunchKernelGGL(k_vp_elements, dim3((V->ns + VP_EW - 1) / VP_EW), dim3(64 * VP_EW), 0, M->stream, v, P, T, dt,
-                     E, fsol, wantK, (const double *)V->u.p, V->st.p, V->tr.p, V->Ke.p, V->fe.p, V->ce.p,
-                     V->dmaxE.p, V->badE.p);
+  const int64_t npt = 8 * (int64_t)V->ns;
+  hipLaunchKernelGGL(k_vp_points, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, M->stream, v, P, T, dt, E, fsol,
+                     wantK, (const double *)V->u.p, V->st.p, V->tr.p, V->gpSig.p, V->gpD.p, V->gpDep.p);
+  BE_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_vp_elements, dim3((V->ns + VP_EW - 1) / VP_EW), dim3(64 * VP_EW), 0, M->stream, v, wantK,
+                     (const double *)V->u.p, (const double *)V->gpSig.p, (const double *)V->gpD.p,
+                     (const double *)V->gpDep.p, V->Ke.p, V->fe.p, V->ce.p, V->dmaxE.p, V->badE.p);
   BE_CHECK(hipGetLastError());
 #else
   const double *u = V->u.p;
@@ -308,8 +356,11 @@ void vp_element_summary(fcbga_vp *V, double &dmax, bool &bad) {
 // Out-of-balance force r = f_th - K_el u - f_solder(u) on the grid (zero on
 // constrained and absent DOFs), from the prepared step: the operator applies
 // K_t = K_el + sum K_e (own types hold K_e), so r = bload - K_t u + sum (K_e u_e
-// - f_e), the last sum gathered at the solder nodes in a fixed order; fsv
-// receives the gathered solder forces sum f_e (for the force scale).
+// - f_e), the last sum gathered at the solder nodes in a fixed order.  u holds
+// the prescribed values on the constrained DOFs, which the unmasked apply
+// carries into the free rows.  For the force scale |f_th| + |f_solder| +
+// |K_el u|: fsv receives the gathered solder forces sum f_e, xs_ the elastic
+// internal force on the free DOFs.
 void vp_residual(fcbga_vp *V) {
   fcbga_model *M = V->M;
   be::stream_t s = M->stream;
@@ -317,11 +368,20 @@ void vp_residual(fcbga_vp *V) {
   const int64_t n3 = 3 * g.G;
   const double *u = V->u.p, *si = M->sinv64.p, *bl = M->bload.p, *wv = M->w64.p;
   double *xs = V->xs_.p, *q = M->q.p, *r = V->rg.p, *fs = V->fsv.p;
-  // K_t u through the scaled operator: A (u / s) = w (K_t u)
+  // K_t u through the scaled operator with the unmasked scale on both sides
+  // (s (K_t (s (u / s)))), so prescribed boundary values enter the free rows
   be::pfor(n3, s, [=] __host__ __device__(int64_t i) { xs[i] = u[i] * si[i]; });
-  fine_apply<double>(fine_ops<double>(M), xs, q, s);
+  {
+    FineOps<double> full = fine_ops<double>(M);
+    full.w = M->s64.p;
+    fine_apply<double>(full, xs, q, s);
+  }
+  // xs now receives the elastic elements' internal force K_el u on the free
+  // DOFs (the solder elements' K_e u_e is taken off at their nodes below)
   be::pfor(n3, s, [=] __host__ __device__(int64_t i) {
-    r[i] = wv[i] != 0 ? bl[i] - q[i] * si[i] : 0.0;
+    const double ku = q[i] * si[i];
+    r[i] = wv[i] != 0 ? bl[i] - ku : 0.0;
+    xs[i] = wv[i] != 0 ? ku : 0.0;
     fs[i] = 0.0;
   });
   const int32_t *cs = V->cellSolder.p, *sn = V->sNodes.p;
@@ -345,7 +405,10 @@ void vp_residual(fcbga_vp *V) {
       }
     }
     for (int p = 0; p < 3; p++) {
-      if (wv[3 * gi + p] != 0) r[3 * gi + p] += add[p];
+      if (wv[3 * gi + p] != 0) {
+        r[3 * gi + p] += add[p];
+        xs[3 * gi + p] -= add[p] + fadd[p];  // K_e u_e = c_e + f_e
+      }
       fs[3 * gi + p] = fadd[p];
     }
   });
@@ -373,6 +436,13 @@ fcbga_vp_stats vp_newton(fcbga_vp *V, const fcbga_step *st, double T, double dt,
   bool conv = false;
   int it;
   double rn = 0, fscale = 0;
+  // the prescribed displacement of the constrained DOFs at this step
+  if (V->bcGrid.n) {
+    double *u = V->u.p;
+    const int64_t *bg = V->bcGrid.p;
+    const double *bv = V->bcVal.p;
+    be::pfor((int64_t)V->bcGrid.n, s, [=] __host__ __device__(int64_t q) { u[bg[q]] = bv[q]; });
+  }
   for (it = 1; it <= vo.maxit; it++) {
     vp_elements(V, T, dt, 1);
     double dm;
@@ -385,7 +455,7 @@ fcbga_vp_stats vp_newton(fcbga_vp *V, const fcbga_step *st, double T, double dt,
     R.ms_setup += now_ms() - ts;
     vp_residual(V);
     rn = vp_norm(V, V->rg.p);
-    fscale = vp_norm(V, M->bload.p) + vp_norm(V, V->fsv.p);
+    fscale = vp_norm(V, M->bload.p) + vp_norm(V, V->fsv.p) + vp_norm(V, V->xs_.p);
     if (!std::isfinite(rn)) { R.nonfinite = 1; break; }
     if (rn <= vo.res_tol * std::max(fscale, 1e-300)) { conv = true; break; }
     // inexact Newton: the inner solve only has to beat the Newton residual by a margin
@@ -477,6 +547,12 @@ void vp_create(const fcbga_mesh *mesh, const fcbga_vp_solder *sol, fcbga_vp *V) 
   std::vector<int32_t> sn;
   for (int64_t gi = 0; gi < g.G; gi++)
     if (touch[gi]) sn.push_back((int32_t)gi);
+  // constrained DOFs on the grid (prescribed values, zero until set)
+  std::vector<int64_t> gridOfNode(M->nn), bc(M->constraints.size());
+  for (int64_t q = 0; q < g.G; q++)
+    if (M->node_id[q] >= 0) gridOfNode[M->node_id[q]] = q;
+  for (size_t q = 0; q < bc.size(); q++) bc[q] = 3 * gridOfNode[M->constraints[q] / 3] + M->constraints[q] % 3;
+  std::vector<double> bz(bc.size(), 0.0);
   if (g.G > INT32_MAX) throw be::Error("grid too large for the solder node list");
   be::stream_t s = M->stream;
   V->eNodes.upload(nodes, s);
@@ -492,8 +568,15 @@ void vp_create(const fcbga_mesh *mesh, const fcbga_vp_solder *sol, fcbga_vp *V) 
   V->ce.alloc(std::max<size_t>(1, 24 * (size_t)V->ns));
   V->dmaxE.alloc(std::max<size_t>(1, V->ns));
   V->badE.alloc(std::max<size_t>(1, V->ns));
+  V->gpSig.alloc(std::max<size_t>(1, 48 * (size_t)V->ns));
+  V->gpD.alloc(std::max<size_t>(1, 288 * (size_t)V->ns));
+  V->gpDep.alloc(std::max<size_t>(1, 8 * (size_t)V->ns));
   V->u.alloc(3 * g.G);
   V->u.zero(s);
+  if (!bc.empty()) {
+    V->bcGrid.upload(bc, s);
+    V->bcVal.upload(bz, s);
+  }
   V->rg.alloc(3 * g.G);
   V->fsv.alloc(3 * g.G);
   V->xs_.alloc(3 * g.G);

@@ -89,7 +89,7 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_submodel_points", "fcbga_screen_joints_noalloc",
            "fcbga_vp_create", "fcbga_vp_destroy", "fcbga_vp_nsolder", "fcbga_vp_step", "fcbga_vp_save",
            "fcbga_vp_restore", "fcbga_vp_points", "fcbga_vp_displacement", "fcbga_vp_set_displacement",
-           "fcbga_vp_residual", "fcbga_set_progress_cb",
+           "fcbga_vp_residual", "fcbga_set_progress_cb", "fcbga_vp_set_prescribed",
            "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
            "fcbga_path_load64", "fcbga_path_copy",
            "fcbga_mesh_fingerprint", "fcbga_last_error"]
@@ -228,6 +228,8 @@ def load(variant: str = "hip"):
     L.fcbga_vp_points.restype = C.c_int
     L.fcbga_vp_displacement.argtypes = [C.c_void_p, _f64p]
     L.fcbga_vp_displacement.restype = C.c_int
+    L.fcbga_vp_set_prescribed.argtypes = [C.c_void_p, _f64p]
+    L.fcbga_vp_set_prescribed.restype = C.c_int
     L.fcbga_vp_set_displacement.argtypes = [C.c_void_p, _f64p]
     L.fcbga_vp_set_displacement.restype = C.c_int
     L.fcbga_vp_residual.argtypes = [C.c_void_p, C.POINTER(FcbgaStep), C.c_double, C.c_double, _f64p, _f64p, _f64p]
@@ -697,6 +699,13 @@ class ViscoplasticModel:
         if self.L.fcbga_vp_displacement(self.h, u.ctypes.data_as(_f64p)) != 0:
             raise SolverError(last_error(self.L))
         return u
+
+    def set_prescribed(self, values):
+        """Prescribed displacement of the constrained DOFs (mesh.constraint_dofs order) for later steps."""
+        v = np.ascontiguousarray(values, np.float64)
+        assert v.shape == (len(self.mesh.constraint_dofs),)
+        if self.L.fcbga_vp_set_prescribed(self.h, v.ctypes.data_as(_f64p)) != 0:
+            raise SolverError(last_error(self.L))
 
     def set_displacement(self, u):
         u = np.ascontiguousarray(u, np.float64)

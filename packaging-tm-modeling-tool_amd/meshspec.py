@@ -23,7 +23,9 @@ A spec is a plain dict (JSON-serialisable):
 A cell (i, j, k) takes the first rule with k0 <= k < k1 whose region test
 passes at the cell centre (inside "in", outside "out", inside one of the
 n x n square bump footprints of side d centred in the pitch cells of a bump
-array starting at (x0, y0)); no rule = void.
+array starting at (x0, y0), inside the sphere {"c": [x, y, z], "r"}); no rule
+= void.  "constrain_faces": true holds every DOF of the six faces' nodes
+instead of the 3-2-1 support.
 """
 from __future__ import annotations
 
@@ -176,6 +178,10 @@ def cell_assignment(spec: dict) -> np.ndarray:
                 ok &= ~((X >= a[0]) & (X <= a[1]) & (Y >= a[2]) & (Y <= a[3]))
             if r.get("bumps") is not None:
                 ok &= in_bumps(r["bumps"], X, Y)
+            if r.get("sphere") is not None:
+                sc = r["sphere"]
+                zc = (zs[k] + zs[k + 1]) / 2
+                ok &= (X - sc["c"][0]) ** 2 + (Y - sc["c"][1]) ** 2 + (zc - sc["c"][2]) ** 2 <= sc["r"] ** 2
             mat[ok, k, 0] = r["mat"]
             scl[ok, k, 0] = r.get("scale", 1.0)
             if r.get("also") is not None:
@@ -240,6 +246,15 @@ def build_mesh(spec: dict) -> Mesh:
              col_start=col_start, constraint_dofs=np.zeros(0, np.int32), constraint_nodes=[],
              feature_x=np.asarray(spec.get("feature_x", []), np.int32),
              feature_y=np.asarray(spec.get("feature_y", []), np.int32), groups=list(mats))
+    if spec.get("constrain_faces"):
+        # every DOF of the nodes on the six faces (a submodel's prescribed boundary)
+        onf = np.zeros((nx, ny, nz), bool)
+        onf[0], onf[-1], onf[:, 0], onf[:, -1], onf[:, :, 0], onf[:, :, -1] = True, True, True, True, True, True
+        nid = node_id.reshape(nx, ny, nz)
+        nodes = np.sort(nid[onf & (nid >= 0)])
+        m.constraint_nodes = list(nodes)
+        m.constraint_dofs = (3 * nodes[:, None] + np.arange(3)[None, :]).reshape(-1).astype(np.int32)
+        return m
     span = spec.get("constraint_span")
     if span is None:
         span = CONSTRAINT_SPAN * min((xs[-1] - xs[0]) / 2, (ys[-1] - ys[0]) / 2)

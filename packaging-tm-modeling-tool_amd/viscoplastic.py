@@ -71,7 +71,7 @@ def bump_layers(mesh, spec: dict, elems: np.ndarray):
     """For every solder element (in the engine's order): its bump, whether it
     lies in the die-side or substrate-side averaging layer, and its point
     weight (volume / 8)."""
-    b = spec["bumps"]
+    b = spec.get("bumps")  # None: one joint (a submodel)
     k0, k1 = spec["bump_layer"]
     zt, zaf = mesh.zs[k0], mesh.zs[k1]
     bump = np.empty(len(elems), np.int64)
@@ -81,7 +81,7 @@ def bump_layers(mesh, spec: dict, elems: np.ndarray):
     for q, e in enumerate(elems):
         i, j, k = mesh.e_ijk[e]
         xc, yc, zc = [(a[v] + a[v + 1]) / 2 for a, v in ((mesh.xs, i), (mesh.ys, j), (mesh.zs, k))]
-        bump[q] = meshspec.bump_index(b, xc, yc)
+        bump[q] = meshspec.bump_index(b, xc, yc) if b else 0
         die[q] = zc > zaf - SUB_T_AVG
         sub[q] = zc < zt + SUB_T_AVG
         w[q] = (mesh.xs[i + 1] - mesh.xs[i]) * (mesh.ys[j + 1] - mesh.ys[j]) * (mesh.zs[k + 1] - mesh.zs[k]) / 8
@@ -102,15 +102,20 @@ def layer_means(points: np.ndarray, layers, nb: int, field: int = 8):
 
 
 def run(vp: hs.ViscoplasticModel, spec: dict, nsteps: int = None, precision: int = hs.PREC_MIXED,
-        max_depth: int = 4, dmax_limit: float = None, log=None, **newton):
+        max_depth: int = 4, dmax_limit: float = None, log=None, prescribe=None, field=None, **newton):
     """Step the model through the cycle history (its first nsteps steps);
     returns the record: per-step Newton counts and times, cutbacks, and the
     per-bump work densities per cycle.  A step is cut back when Newton fails
     and, with dmax_limit (the reference submodel's is INELASTIC_STEP_MAX),
-    when its largest plastic increment exceeds the limit."""
+    when its largest plastic increment exceeds the limit.  prescribe(T), when
+    given, returns the constrained DOFs' displacement at T (a submodel's
+    boundary), set before every (sub-)step; field(T), when given, the
+    boundary field at every node, whose increment over the step is added to
+    the whole displacement as the Newton's starting point (the boundary jump
+    then does not land on the boundary elements alone)."""
     hist = cycle_history(spec)
     n = len(hist["T"]) - 1 if nsteps is None else min(nsteps, len(hist["T"]) - 1)
-    nb = spec["bumps"]["n"] ** 2
+    nb = spec["bumps"]["n"] ** 2 if spec.get("bumps") else 1
     _, elems = vp.points()
     layers = bump_layers(vp.mesh, spec, elems)
     Wd = [np.zeros(nb)]
@@ -121,6 +126,10 @@ def run(vp: hs.ViscoplasticModel, spec: dict, nsteps: int = None, precision: int
 
     def advance(T0, T1, dt, depth):
         vp.save()
+        if prescribe is not None:
+            vp.set_prescribed(prescribe(T1))
+        if field is not None:
+            vp.set_displacement(vp.displacement() + field(T1) - field(T0))
         rec = vp.step(elastic_step(spec, T1), T1, dt, precision=precision, **newton)
         ok = rec["converged"] and (dmax_limit is None or rec["dmax"] <= dmax_limit)
         if ok:
@@ -158,3 +167,55 @@ def run(vp: hs.ViscoplasticModel, spec: dict, nsteps: int = None, precision: int
         "ms_setup": float(sum(s["ms_setup"] for s in steps)), "ms_pcg": float(sum(s["ms_pcg"] for s in steps)),
         "per_cycle": per_cycle, "step_records": steps,
     }
+
+
+def submodel_boundary_values(mesh, spec: dict):
+    """prescribe(T) for a batched submodel spec (cases.cfg5_spec): the seeded
+    boundary field at the coordinates of the constrained DOFs' nodes."""
+    from . import cases
+    nodes = mesh.constraint_dofs[0::3] // 3
+    shape = cases.cfg5_boundary(spec, mesh.coords[nodes], spec["Tref"] + 1.0).reshape(-1)  # linear in T - Tref
+
+    def prescribe(T):
+        return (T - spec["Tref"]) * shape
+    return prescribe
+
+
+def submodel_field(mesh, spec: dict):
+    """field(T): the boundary field at every node (the Newton's predictor)."""
+    from . import cases
+    shape = cases.cfg5_boundary(spec, mesh.coords, spec["Tref"] + 1.0).reshape(-1)  # linear in T - Tref
+
+    def field(T):
+        return (T - spec["Tref"]) * shape
+    return field
+
+
+def run_submodel(spec: dict, variant: str = "hip", precision: int = hs.PREC_MIXED, nsteps: int = None, **newton):
+    """One batched bump submodel through its cycle: mesh, engine, boundary,
+    steps; returns its record with the wall time, DOF and the joint's dW."""
+    t0 = time.perf_counter()
+    mesh = meshspec.build_mesh(spec)
+    vp = hs.ViscoplasticModel(mesh, solder_of(spec), variant=variant)
+    try:
+        rec = run(vp, spec, nsteps=nsteps, precision=precision, prescribe=submodel_boundary_values(mesh, spec),
+                  field=submodel_field(mesh, spec), **newton)
+    finally:
+        vp.close()
+    pc = rec.pop("per_cycle")
+    rec.pop("step_records")
+    rec.update({"id": spec["draw"]["id"], "n": spec["draw"]["n"], "ndof": mesh.ndof, "solder_elements": vp.nsolder,
+                "wall_s": time.perf_counter() - t0,
+                "dW": float(pc[-1]["dW"][0]) if pc else None, "dWdie": float(pc[-1]["dWdie"][0]) if pc else None,
+                "dWsub": float(pc[-1]["dWsub"][0]) if pc else None})
+    return rec
+
+
+def run_submodel_batch(specs, workers: int = 4, variant: str = "hip", **kw):
+    """Many submodels at once (the reference gives each BumpSubmodel to a helper
+    worker, src/worker.js:222-246): one host thread and one engine (its own
+    device stream) per submodel in flight, `workers` in flight; records in the
+    order of specs."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        return list(ex.map(lambda sp: run_submodel(sp, variant=variant, **kw), specs))
