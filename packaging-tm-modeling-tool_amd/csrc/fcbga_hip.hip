@@ -34,6 +34,8 @@ constexpr int SHELL_MAXM = 512;
 // table from its mirror-symmetric form for the compact tables to be used
 constexpr double SYM_TOL = 1e-11;  // 6 DOFs x up to 85 aggregates in one shell column
 constexpr int FINE_MAXNZ = 64;  // grid planes in z: one lane per plane in a fine column block
+// the fine operator's products on the matrix cores unless FCBGA_ELEM_OP says otherwise
+constexpr bool ELEM_OP_MFMA_DEFAULT = false;
 
 double now_ms() {
   using namespace std::chrono;
@@ -239,6 +241,14 @@ struct fcbga_model {
   DBuf<float> R32;
   bool symTables = false;
   bool sym_allowed = true;  // FCBGA_FULL_TABLES=1 forces the full tables
+  // fine operator products on the matrix cores (mop::k_elem_mfma, element
+  // groups of every tile) or by node rows on the vector units
+  // (k_fine_col_op); FCBGA_ELEM_OP=mfma|valu chooses, per model
+  bool mfmaOp = false;
+  int32_t mopTiles[3] = {0, 0, 0};  // tiles in x, in y, chunks in z
+  DBuf<int32_t> mopType, mopOff;
+  DBuf<int16_t> mopCell;
+  DBuf<uint32_t> mopMask;
   // elements of group ownGroup (-1: none) each get a type of their own,
   // types ownType0.. in the order of ownElems; when ownK64 is set (device,
   // 576 FP64 per own type) assemble_step takes those types' tables from it
@@ -383,6 +393,8 @@ void check_mesh(const fcbga_mesh *m) {
   if (m->nsections > 0 && !m->e_section) throw be::Error("layered sections given without e_section");
 }
 
+void mop_setup(fcbga_model *M, const std::vector<int32_t> &cellType);
+
 void model_create(const fcbga_mesh *mm, fcbga_model *M, int32_t ownGroup = -1) {
   check_mesh(mm);
   fe::Grid &g = M->g;
@@ -502,6 +514,11 @@ void model_create(const fcbga_mesh *mm, fcbga_model *M, int32_t ownGroup = -1) {
   be::stream_t s = M->stream;
   M->cellType.upload(cellType, s);
   M->cellElem.upload(cellElem, s);
+  {
+    const char *ev = getenv("FCBGA_ELEM_OP");
+    M->mfmaOp = ev ? std::string(ev) == "mfma" : ELEM_OP_MFMA_DEFAULT;
+    if (M->mfmaOp) mop_setup(M, cellType);
+  }
   M->d_egroup.upload(M->e_group, s);
   M->d_escale.upload(M->e_scale, s);
   M->d_typeSlot.upload(M->typeSlot, s);
@@ -722,6 +739,31 @@ void factor_fine_columns(fcbga_model *M) {
 // fine-level operator pieces
 // ---------------------------------------------------------------------------
 
+#if FCBGA_GPU
+// Block order across the 8 XCDs: the hardware hands out workgroups round-robin,
+// so block b runs on XCD b % 8.  Giving each XCD a contiguous run of columns
+// keeps neighbouring columns (whose neighbourhoods overlap) in one L2.
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+  const int per = (nblocks + 7) >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+#endif
+
+}  // namespace
+#include "mfma_op.h"
+namespace {
+
+// the element groups of the matrix-core operator, from the cells' types
+void mop_setup(fcbga_model *M, const std::vector<int32_t> &cellType) {
+  mop::Lists L;
+  mop::build_lists(M->g, cellType, L);
+  M->mopTiles[0] = L.ntx; M->mopTiles[1] = L.nty; M->mopTiles[2] = L.nchunk;
+  M->mopType.upload(L.type, M->stream);
+  M->mopOff.upload(L.off, M->stream);
+  M->mopCell.upload(L.cell, M->stream);
+  M->mopMask.upload(L.mask, M->stream);
+}
+
 template <typename T>
 struct FineOps {
   fe::Grid g;
@@ -732,6 +774,12 @@ struct FineOps {
   const T *zp;
   const T *R;  // compact mirror-symmetric tables (fe::SYM_TABLE per type) when sym
   bool sym;
+  // element groups of the matrix-core operator (mop::k_elem_mfma) when mfma
+  bool mfma;
+  int ntx, nty, nchunk;
+  const int32_t *gType, *gOff;
+  const int16_t *gCell;
+  const uint32_t *gMask;
 };
 
 template <typename T>
@@ -740,12 +788,14 @@ FineOps<T> fine_ops(fcbga_model *M);
 template <>
 FineOps<double> fine_ops<double>(fcbga_model *M) {
   return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->C64.p,
-                         M->nodeAgg.p, M->zp64.p, M->R64.p, M->symTables};
+                         M->nodeAgg.p, M->zp64.p, M->R64.p, M->symTables, M->mfmaOp, M->mopTiles[0],
+                         M->mopTiles[1], M->mopTiles[2], M->mopType.p, M->mopOff.p, M->mopCell.p, M->mopMask.p};
 }
 template <>
 FineOps<float> fine_ops<float>(fcbga_model *M) {
   return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->C32.p,
-                        M->nodeAgg.p, M->zp32.p, M->R32.p, M->symTables};
+                        M->nodeAgg.p, M->zp32.p, M->R32.p, M->symTables, M->mfmaOp, M->mopTiles[0],
+                        M->mopTiles[1], M->mopTiles[2], M->mopType.p, M->mopOff.p, M->mopCell.p, M->mopMask.p};
 }
 
 #if FCBGA_GPU
@@ -758,6 +808,13 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
 template <typename T, int MODE>
 void launch_col_op(const FineOps<T> &o, const T *x, const T *b, T *y, be::stream_t s) {
   const fe::Grid g = o.g;
+  if (o.mfma) {
+    const int nt = o.ntx * o.nty * o.nchunk;
+    hipLaunchKernelGGL((mop::k_elem_mfma<T, MODE>), dim3(8 * ((nt + 7) / 8)), dim3(mop::THREADS), 0, s, g, o.ntx,
+                       o.nty, o.nchunk, o.gType, o.gCell, o.gOff, o.gMask, o.cellScale, o.Kc, o.w, x, b, y);
+    BE_CHECK(hipGetLastError());
+    return;
+  }
   const dim3 grid(8 * ((g.nx * g.ny + 7) / 8));
   const size_t lds = (size_t)9 * (g.nz + 2) * 3 * sizeof(T);  // the 3 x 3 column neighbourhood
   if (o.sym)
@@ -780,6 +837,12 @@ void fine_apply(const FineOps<T> &o, const T *x, T *y, be::stream_t s) {
   (void)ct; (void)cs; (void)Kc; (void)w;
   launch_col_op<T, 0>(o, x, (const T *)nullptr, y, s);
 #else
+  if (o.mfma) {
+    be::sync(s);
+    mop::elem_op_host<T, 0>(g, o.ntx, o.nty, o.nchunk, o.gType, o.gCell, o.gOff, o.gMask, cs, Kc, w, x,
+                            (const T *)nullptr, y);
+    return;
+  }
   const T *R = o.R;
   const bool sym = o.sym;
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
@@ -806,6 +869,11 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
   (void)ct; (void)cs; (void)Kc; (void)w;
   launch_col_op<T, 1>(o, x, b, res, s);
 #else
+  if (o.mfma) {
+    be::sync(s);
+    mop::elem_op_host<T, 1>(g, o.ntx, o.nty, o.nchunk, o.gType, o.gCell, o.gOff, o.gMask, cs, Kc, w, x, b, res);
+    return;
+  }
   const T *R = o.R;
   const bool sym = o.sym;
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
@@ -881,14 +949,6 @@ __device__ __forceinline__ void load_scan_coefs<double>(const double *__restrict
 // lane k = grid plane k of the column (nz <= 64): the w*x neighbourhood of the
 // column is staged in LDS, then the residual row, both substitution
 // recurrences as wavefront scans in registers, then x += dx
-// Block order across the 8 XCDs: the hardware hands out workgroups round-robin,
-// so block b runs on XCD b % 8.  Giving each XCD a contiguous run of columns
-// keeps neighbouring columns (whose neighbourhoods overlap) in one L2.
-__device__ __forceinline__ int xcd_block(int b, int nblocks) {
-  const int per = (nblocks + 7) >> 3;
-  return (b & 7) * per + (b >> 3);
-}
-
 template <typename T, bool SYM>
 __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t *__restrict__ cols, int ncols,
                                                        const int32_t *__restrict__ ct, const T *__restrict__ cs,
@@ -4215,16 +4275,19 @@ int fcbga_step_apply(fcbga_model *m, const fcbga_step *step, const double *x, do
     be::pfor(m->nn, m->stream, [=] __host__ __device__(int64_t n) {
       for (int c = 0; c < 3; c++) xgp[3 * gof[n] + c] = xi[3 * n + c];
     });
-    const int32_t *ct = m->cellType.p;
-    const double *cs = m->cellScale64.p, *Kc = m->Kc64.p;
-    be::pfor(g.G, m->stream, [=] __host__ __device__(int64_t gi) {
-      int k = (int)(gi % g.nz);
-      int64_t cj = gi / g.nz;
-      int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
-      double v[3];
-      fe::node_apply_raw<double>(g, i, j, k, ct, cs, Kc, xgp, v);
-      for (int c = 0; c < 3; c++) ygp[3 * gi + c] = v[c];
+    // the solver's own operator kernel (full element tables) with unit
+    // scaling on every DOF of an existing node: y = K x, no constraints
+    DBuf<double> wone;
+    wone.alloc(3 * g.G);
+    wone.zero(m->stream);
+    double *wo = wone.p;
+    be::pfor(m->nn, m->stream, [=] __host__ __device__(int64_t n) {
+      for (int c = 0; c < 3; c++) wo[3 * gof[n] + c] = 1.0;
     });
+    FineOps<double> fo = fine_ops<double>(m);
+    fo.w = wone.p;
+    fo.sym = false;
+    fine_apply<double>(fo, xgp, ygp, m->stream);
     be::pfor(m->nn, m->stream, [=] __host__ __device__(int64_t n) {
       for (int c = 0; c < 3; c++) o[3 * n + c] = ygp[3 * gof[n] + c];
     });

@@ -106,36 +106,6 @@ FE_HD void node_apply(const Grid &g, int i, int j, int k, const int32_t *__restr
   }
 }
 
-// unscaled K x at a node (no constraints): y_i = sum_j K_ij x_j (test hook and residual check)
-template <typename T>
-FE_HD void node_apply_raw(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
-                          const T *__restrict__ cellScale, const T *__restrict__ Kc, const T *__restrict__ x,
-                          T out[3]) {
-  T a0 = 0, a1 = 0, a2 = 0;
-  for (int ci = 0; ci < 2; ci++)
-    for (int cj = 0; cj < 2; cj++)
-      for (int ck = 0; ck < 2; ck++) {
-        int ii = i - 1 + ci, jj = j - 1 + cj, kk = k - 1 + ck;
-        if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy || kk < 0 || kk >= g.ncz) continue;
-        int64_t c = ((int64_t)ii * g.ncy + jj) * g.ncz + kk;
-        const int a = corner_of(1 - ci, 1 - cj, 1 - ck);
-        for (int q = 0; q < 2; q++) {
-          int32_t t = cellType[2 * c + q];
-          if (t < 0) continue;
-          T sc = cellScale[2 * c + q];
-          const T *K = Kc + (int64_t)t * 576 + (3 * a) * 24;
-          for (int b = 0; b < 8; b++) {
-            int64_t gb = ((int64_t)(ii + corner_li(b)) * g.ny + jj + corner_lj(b)) * g.nz + kk + corner_lk(b);
-            const T *xv = x + 3 * gb;
-            a0 += sc * (K[3 * b] * xv[0] + K[3 * b + 1] * xv[1] + K[3 * b + 2] * xv[2]);
-            a1 += sc * (K[24 + 3 * b] * xv[0] + K[24 + 3 * b + 1] * xv[1] + K[24 + 3 * b + 2] * xv[2]);
-            a2 += sc * (K[48 + 3 * b] * xv[0] + K[48 + 3 * b + 1] * xv[1] + K[48 + 3 * b + 2] * xv[2]);
-          }
-        }
-      }
-  out[0] = a0; out[1] = a1; out[2] = a2;
-}
-
 // thermal load of a node: sum over its elements of scale * fc (src/core/09-assembly.js:416-420)
 FE_HD void node_load(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
                      const double *__restrict__ cellScale, const double *__restrict__ fc,
