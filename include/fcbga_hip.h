@@ -333,7 +333,8 @@ typedef struct fcbga_vp_solder {
 typedef struct fcbga_vp_opts {
   fcbga_solve_opts inner;     /* inner PCG: precision, preconditioner, maxit (tol and warm are set per iteration) */
   int32_t maxit;              /* Newton iterations (the reference's SUB_NEWTON_MAXIT is 25) */
-  double res_tol;             /* |r| / (|f_th| + |f_solder|) (SUB_RES_TOL 1e-6) */
+  double res_tol;             /* |r| / (|f_load| + |f_solder| + |K_el u|), K_el the elastic elements' part
+                               * of the operator (the reference's SUB_RES_TOL is 1e-6) */
   double du_tol;              /* |du| / |u| (SUB_DU_TOL 1e-8) */
   double inner_floor;         /* floor of the inexact inner tolerance (SUB_INNER_TOL 1e-8) */
 } fcbga_vp_opts;
