@@ -614,7 +614,8 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
       if (q % 576 == 0) tF[t] = 0;
     });
   }
-  // mirror-symmetric compact tables (all types must pass, else the full tables)
+  // mirror-symmetric compact tables (all shared types must pass, else the
+  // full tables for every type)
   M->R64.alloc((size_t)M->ntypes * fe::SYM_TABLE);
   M->R32.alloc((size_t)M->ntypes * fe::SYM_TABLE);
   M->symErr.alloc(M->ntypes);
@@ -628,8 +629,11 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
       fe::sym_compact(Rf, R + fe::SYM_TABLE * t);
       for (int q = 0; q < fe::SYM_TABLE; q++) R32[fe::SYM_TABLE * t + q] = (float)R[fe::SYM_TABLE * t + q];
     }, 64);
-    std::vector<double> e(M->ntypes);
-    be::d2h(e.data(), M->symErr.p, e.size() * sizeof(double), s);
+    // the own types (viscoplastic tangents) keep their full tables beside
+    // the compact ones, so only the shared types must pass
+    const int32_t nshared = std::min(M->ntypes, M->ownType0);
+    std::vector<double> e(nshared);
+    if (nshared) be::d2h(e.data(), M->symErr.p, e.size() * sizeof(double), s);
     be::sync(s);
     double emax = 0;
     for (double v : e) emax = v > emax ? v : emax;
@@ -774,6 +778,9 @@ struct FineOps {
   const T *zp;
   const T *R;  // compact mirror-symmetric tables (fe::SYM_TABLE per type) when sym
   bool sym;
+  // with the compact tables, the types from own0 on (the viscoplastic
+  // solder's own tangents; INT32_MAX: none) take the full tables
+  int own0;
   // element groups of the matrix-core operator (mop::k_elem_mfma) when mfma
   bool mfma;
   int ntx, nty, nchunk;
@@ -785,25 +792,42 @@ struct FineOps {
 template <typename T>
 FineOps<T> fine_ops(fcbga_model *M);
 
+// first own type of the model, INT32_MAX when it has none
+int own_from(const fcbga_model *M) { return M->ownType0 < M->ntypes ? M->ownType0 : INT32_MAX; }
+
 template <>
 FineOps<double> fine_ops<double>(fcbga_model *M) {
   return FineOps<double>{M->g, M->cellType.p, M->cellScale64.p, M->Kc64.p, M->w64.p, M->sinv64.p, M->C64.p,
-                         M->nodeAgg.p, M->zp64.p, M->R64.p, M->symTables, M->mfmaOp, M->mopTiles[0],
+                         M->nodeAgg.p, M->zp64.p, M->R64.p, M->symTables, own_from(M), M->mfmaOp, M->mopTiles[0],
                          M->mopTiles[1], M->mopTiles[2], M->mopType.p, M->mopOff.p, M->mopCell.p, M->mopMask.p};
 }
 template <>
 FineOps<float> fine_ops<float>(fcbga_model *M) {
   return FineOps<float>{M->g, M->cellType.p, M->cellScale32.p, M->Kc32.p, M->w32.p, M->sinv32.p, M->C32.p,
-                        M->nodeAgg.p, M->zp32.p, M->R32.p, M->symTables, M->mfmaOp, M->mopTiles[0],
+                        M->nodeAgg.p, M->zp32.p, M->R32.p, M->symTables, own_from(M), M->mfmaOp, M->mopTiles[0],
                         M->mopTiles[1], M->mopTiles[2], M->mopType.p, M->mopOff.p, M->mopCell.p, M->mopMask.p};
 }
 
 #if FCBGA_GPU
-template <typename T, int MODE, bool SYM>
+// FORM: 0 full tables (Kc), 1 compact tables (Kc = R), 2 compact tables with
+// the full tables Kf for the types from own0 on
+template <typename T, int MODE, int FORM, bool PACK>
 __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *__restrict__ ct,
                                                     const T *__restrict__ cs, const T *__restrict__ Kc,
-                                                    const T *__restrict__ w, const T *__restrict__ x,
-                                                    const T *__restrict__ bvec, T *__restrict__ y);
+                                                    const T *__restrict__ Kf, int own0, const T *__restrict__ w,
+                                                    const T *__restrict__ x, const T *__restrict__ bvec,
+                                                    T *__restrict__ y);
+
+// columns per wavefront of the column kernels: 64 / nz for grids of at most
+// 32 planes, else one
+inline int cols_per_wave(int nz) { return nz <= 32 ? 64 / nz : 1; }
+
+// the table form of an operator launch: 0 full, 1 compact, 2 compact with own full tables
+template <typename T>
+int table_form(const FineOps<T> &o) {
+  if (!o.sym) return 0;
+  return o.own0 < INT32_MAX ? 2 : 1;
+}
 
 template <typename T, int MODE>
 void launch_col_op(const FineOps<T> &o, const T *x, const T *b, T *y, be::stream_t s) {
@@ -815,14 +839,27 @@ void launch_col_op(const FineOps<T> &o, const T *x, const T *b, T *y, be::stream
     BE_CHECK(hipGetLastError());
     return;
   }
-  const dim3 grid(8 * ((g.nx * g.ny + 7) / 8));
-  const size_t lds = (size_t)9 * (g.nz + 2) * 3 * sizeof(T);  // the 3 x 3 column neighbourhood
-  if (o.sym)
-    hipLaunchKernelGGL((k_fine_col_op<T, MODE, true>), grid, dim3(64), lds, s, g, o.cellType, o.cellScale, o.R, o.w, x,
-                       b, y);
-  else
-    hipLaunchKernelGGL((k_fine_col_op<T, MODE, false>), grid, dim3(64), lds, s, g, o.cellType, o.cellScale, o.Kc, o.w,
-                       x, b, y);
+  const int P = cols_per_wave(g.nz);
+  const int nb = (g.nx * g.ny + P - 1) / P;
+  const dim3 grid(8 * ((nb + 7) / 8));
+  const size_t lds = (size_t)P * 9 * (g.nz + 2) * 3 * sizeof(T);  // the 3 x 3 column neighbourhoods
+  const T *K0 = table_form(o) ? o.R : o.Kc;
+  const auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(64), lds, s, g, o.cellType, o.cellScale, K0, o.Kc, o.own0, o.w, x, b, y);
+  };
+  if (P > 1) {
+    switch (table_form(o)) {
+      case 2: go(k_fine_col_op<T, MODE, 2, true>); break;
+      case 1: go(k_fine_col_op<T, MODE, 1, true>); break;
+      default: go(k_fine_col_op<T, MODE, 0, true>);
+    }
+  } else {
+    switch (table_form(o)) {
+      case 2: go(k_fine_col_op<T, MODE, 2, false>); break;
+      case 1: go(k_fine_col_op<T, MODE, 1, false>); break;
+      default: go(k_fine_col_op<T, MODE, 0, false>);
+    }
+  }
   BE_CHECK(hipGetLastError());
 }
 #endif
@@ -845,13 +882,14 @@ void fine_apply(const FineOps<T> &o, const T *x, T *y, be::stream_t s) {
   }
   const T *R = o.R;
   const bool sym = o.sym;
+  const int own0 = o.own0;
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
     int k = (int)(gi % g.nz);
     int64_t cj = gi / g.nz;
     int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
     T out[3];
     if (sym)
-      fe::node_apply_sym<T>(g, i, j, k, ct, cs, R, w, x, out);
+      fe::node_apply_sym<T>(g, i, j, k, ct, cs, fe::SymTabs<T>{R, Kc, own0}, w, x, out);
     else
       fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
     y[3 * gi] = out[0]; y[3 * gi + 1] = out[1]; y[3 * gi + 2] = out[2];
@@ -876,13 +914,14 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
   }
   const T *R = o.R;
   const bool sym = o.sym;
+  const int own0 = o.own0;
   be::pfor(g.G, s, [=] __host__ __device__(int64_t gi) {
     int k = (int)(gi % g.nz);
     int64_t cj = gi / g.nz;
     int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
     T out[3];
     if (sym)
-      fe::node_apply_sym<T>(g, i, j, k, ct, cs, R, w, x, out);
+      fe::node_apply_sym<T>(g, i, j, k, ct, cs, fe::SymTabs<T>{R, Kc, own0}, w, x, out);
     else
       fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
     for (int p = 0; p < 3; p++) res[3 * gi + p] = b[3 * gi + p] - out[p];
@@ -949,20 +988,32 @@ __device__ __forceinline__ void load_scan_coefs<double>(const double *__restrict
 // lane k = grid plane k of the column (nz <= 64): the w*x neighbourhood of the
 // column is staged in LDS, then the residual row, both substitution
 // recurrences as wavefront scans in registers, then x += dx
-template <typename T, bool SYM>
+// PACK (grids of at most 32 planes): 64 / nz columns per wavefront, lane
+// q * nz + k = plane k of the block's column q, each column's neighbourhood in
+// its own slice of the dynamic LDS (plane stride nz + 2); the scans stay
+// within a column because its end nodes carry zero maps (P at the bottom, Q
+// at the top), which the compositions across a column boundary multiply by.
+template <typename T, int FORM, bool PACK>
 __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t *__restrict__ cols, int ncols,
                                                        const int32_t *__restrict__ ct, const T *__restrict__ cs,
-                                                       const T *__restrict__ Kc, const T *__restrict__ w,
-                                                       const T *__restrict__ C, const T *__restrict__ bvec,
-                                                       T *__restrict__ x) {
-  __shared__ T nb[9 * fe::NZP * 3];  // fixed 64-plane stride
-  const int t = xcd_block(blockIdx.x, gridDim.x);
-  if (t >= ncols) return;
-  const int c = cols[t];
-  const int i = c / g.ny, j = c % g.ny;
+                                                       const T *__restrict__ Kc, const T *__restrict__ Kf, int own0,
+                                                       const T *__restrict__ w, const T *__restrict__ C,
+                                                       const T *__restrict__ bvec, T *__restrict__ x) {
+  constexpr bool SYM = FORM != 0;
+  __shared__ T nb_fix[PACK ? 1 : 9 * fe::NZP * 3];  // fixed 64-plane stride
+  extern __shared__ double nb_dyn[];
   const int nz = g.nz;
-  const int k = threadIdx.x;
-  const bool valid = k < nz;
+  const int P = PACK ? 64 / nz : 1;
+  const int nzp = PACK ? nz + 2 : fe::NZP;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blk * P >= ncols) return;
+  const int slot = PACK ? threadIdx.x / nz : 0;
+  const int k = PACK ? threadIdx.x % nz : threadIdx.x;
+  const int t = blk * P + slot;
+  const bool valid = PACK ? (slot < P && t < ncols) : k < nz;
+  const int c = cols[valid ? t : blk * P];
+  const int i = c / g.ny, j = c % g.ny;
+  T *nb = PACK ? reinterpret_cast<T *>(nb_dyn) + (slot < P ? slot : 0) * 9 * nzp * 3 : nb_fix;
   const int gi = c * nz + (valid ? k : 0);
   // this node's own data first: their latency overlaps the staging below
   // (the scan coefficients are loaded after the residual, which needs the
@@ -973,23 +1024,26 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
     xv[p] = x[3 * gi + p];
     bv[p] = bvec[3 * gi + p];
   }
-  for (int q = threadIdx.x; q < 9 * 2 * 3; q += 64) {  // padding levels -1 and nz
-    const int col = q / 6, side = (q / 3) % 2, p = q % 3;
-    nb[(col * fe::NZP + (side ? nz + 1 : 0)) * 3 + p] = T(0);
+  {
+    T *nb0 = PACK ? reinterpret_cast<T *>(nb_dyn) : nb_fix;
+    for (int q = threadIdx.x; q < P * 9 * 2 * 3; q += 64) {  // padding levels -1 and nz of every column
+      const int sl = q / 54, col = (q % 54) / 6, side = (q / 3) % 2, p = q % 3;
+      nb0[((sl * 9 + col) * nzp + (side ? nz + 1 : 0)) * 3 + p] = T(0);
+    }
   }
-  if (valid) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, fe::NZP);
+  if (valid) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, nzp);
   __syncthreads();
   T r[3] = {0, 0, 0};
   T acc[3] = {0, 0, 0};
   if (SYM) {
     T up[3] = {0, 0, 0};
-    if (valid) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up, fe::NZP);
+    if (valid) fe::node_rows_nb_sym<T, FORM == 2>(g, i, j, k, ct, cs, fe::SymTabs<T>{Kc, Kf, own0}, nb, acc, up, nzp);
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);
       acc[p] += k > 0 ? u : T(0);
     }
   } else if (valid) {
-    fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc, fe::NZP);
+    fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc, nzp);
   }
   if (valid)
     for (int p = 0; p < 3; p++) r[p] = bv[p] - (wv[p] != T(0) ? wv[p] * acc[p] : xv[p]);
@@ -1017,37 +1071,45 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
 
 // y = A x (mode 0) or y = b - A x (mode 1) over every node column, one 64-lane
 // block per column with the neighbourhood staged in LDS
-template <typename T, int MODE, bool SYM>
+// (with PACK, 64 / nz columns per block as in k_fine_col_sweep)
+template <typename T, int MODE, int FORM, bool PACK>
 __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *__restrict__ ct,
                                                     const T *__restrict__ cs, const T *__restrict__ Kc,
-                                                    const T *__restrict__ w, const T *__restrict__ x,
-                                                    const T *__restrict__ bvec, T *__restrict__ y) {
-  extern __shared__ double nb_lds[];  // 9 * (nz + 2) * 3 values of T
-  T *nb = reinterpret_cast<T *>(nb_lds);
-  const int c = xcd_block(blockIdx.x, gridDim.x);
-  if (c >= g.nx * g.ny) return;
-  const int i = c / g.ny, j = c % g.ny;
-  const int nz = g.nz;
-  const int k = threadIdx.x;
-  const int gi = c * nz + k;
-  for (int q = threadIdx.x; q < 9 * 2 * 3; q += 64) {
-    const int col = q / 6, side = (q / 3) % 2, p = q % 3;
-    nb[(col * (nz + 2) + (side ? nz + 1 : 0)) * 3 + p] = T(0);
+                                                    const T *__restrict__ Kf, int own0, const T *__restrict__ w,
+                                                    const T *__restrict__ x, const T *__restrict__ bvec,
+                                                    T *__restrict__ y) {
+  constexpr bool SYM = FORM != 0;
+  extern __shared__ double nb_lds[];  // P * 9 * (nz + 2) * 3 values of T
+  const int nz = g.nz, ncol = g.nx * g.ny;
+  const int P = PACK ? 64 / nz : 1;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blk * P >= ncol) return;
+  const int slot = PACK ? threadIdx.x / nz : 0;
+  const int k = PACK ? threadIdx.x % nz : threadIdx.x;
+  const int c = blk * P + slot;
+  const bool valid = PACK ? (slot < P && c < ncol) : k < nz;
+  const int cc = valid ? c : blk * P;
+  const int i = cc / g.ny, j = cc % g.ny;
+  T *nb = reinterpret_cast<T *>(nb_lds) + (slot < P ? slot : 0) * 9 * (nz + 2) * 3;
+  const int gi = cc * nz + k;
+  for (int q = threadIdx.x; q < P * 9 * 2 * 3; q += 64) {
+    const int sl = q / 54, col = (q % 54) / 6, side = (q / 3) % 2, p = q % 3;
+    reinterpret_cast<T *>(nb_lds)[((sl * 9 + col) * (nz + 2) + (side ? nz + 1 : 0)) * 3 + p] = T(0);
   }
-  if (k < nz) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, nz + 2);
+  if (valid) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, nz + 2);
   __syncthreads();
   T acc[3] = {0, 0, 0};
   if (SYM) {
     T up[3] = {0, 0, 0};
-    if (k < nz) fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Kc, nb, acc, up, nz + 2);
+    if (valid) fe::node_rows_nb_sym<T, FORM == 2>(g, i, j, k, ct, cs, fe::SymTabs<T>{Kc, Kf, own0}, nb, acc, up, nz + 2);
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);
       acc[p] += k > 0 ? u : T(0);
     }
-  } else if (k < nz) {
+  } else if (valid) {
     fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc, nz + 2);
   }
-  if (k >= nz) return;
+  if (!valid) return;
   for (int p = 0; p < 3; p++) {
     const T wi = w[3 * gi + p];
     const T out = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
@@ -1067,17 +1129,32 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   KScope ks(M, FCBGA_KCLASS_FINE_SWEEP);
 #if FCBGA_GPU
   (void)r; (void)dx;
-  const size_t lds = 0;
-  if (o.sym)
-    hipLaunchKernelGGL((k_fine_col_sweep<T, true>), dim3(8 * ((ncols + 7) / 8)), dim3(64), lds, s, g, cols, ncols, ct,
-                       cs, o.R, w, C, b, x);
-  else
-    hipLaunchKernelGGL((k_fine_col_sweep<T, false>), dim3(8 * ((ncols + 7) / 8)), dim3(64), lds, s, g, cols, ncols, ct,
-                       cs, Kc, w, C, b, x);
+  const int P = cols_per_wave(nz);
+  const int nblk = (ncols + P - 1) / P;
+  const dim3 grid(8 * ((nblk + 7) / 8));
+  const size_t lds = P > 1 ? (size_t)P * 9 * (nz + 2) * 3 * sizeof(T) : 0;
+  const T *K0 = table_form(o) ? o.R : Kc;
+  const auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(64), lds, s, g, cols, ncols, ct, cs, K0, Kc, o.own0, w, C, b, x);
+  };
+  if (P > 1) {
+    switch (table_form(o)) {
+      case 2: go(k_fine_col_sweep<T, 2, true>); break;
+      case 1: go(k_fine_col_sweep<T, 1, true>); break;
+      default: go(k_fine_col_sweep<T, 0, true>);
+    }
+  } else {
+    switch (table_form(o)) {
+      case 2: go(k_fine_col_sweep<T, 2, false>); break;
+      case 1: go(k_fine_col_sweep<T, 1, false>); break;
+      default: go(k_fine_col_sweep<T, 0, false>);
+    }
+  }
   BE_CHECK(hipGetLastError());
 #else
   const T *Rt = o.R;
   const bool symT = o.sym;
+  const int own0 = o.own0;
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     const int c = cols[t];
     const int i = c / g.ny, j = c % g.ny;
@@ -1090,7 +1167,7 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
       T acc[3];
       if (symT) {
         T up[3];
-        fe::node_rows_nb_sym<T>(g, i, j, k, ct, cs, Rt, nb, acc, up, fe::NZP);
+        fe::node_rows_nb_sym<T, true>(g, i, j, k, ct, cs, fe::SymTabs<T>{Rt, Kc, own0}, nb, acc, up, fe::NZP);
         for (int p = 0; p < 3; p++) { acc[p] += upPrev[p]; upPrev[p] = up[p]; }
       } else {
         fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc, fe::NZP);

@@ -968,12 +968,40 @@ FE_HD void cell_sym_pair(const T *__restrict__ R, const T (&xb)[8][3], T sc, T o
   }
 }
 
+// The same pair of rows from a full 24 x 24 table (an element type without
+// the mirror symmetry: the viscoplastic solder's own tangents)
+template <typename T, int MX, int MY>
+FE_HD void cell_full_pair(const T *__restrict__ K, const T (&xb)[8][3], T sc, T own[3], T up[3]) {
+  constexpr int ao = MX ? (MY ? 2 : 1) : (MY ? 3 : 0), au = ao + 4;  // corners (MX, MY, 0) and (MX, MY, 1)
+  for (int p = 0; p < 3; p++) {
+    const T *Ko = K + (3 * ao + p) * 24, *Ku = K + (3 * au + p) * 24;
+    T so = 0, su = 0;
+    for (int b = 0; b < 8; b++)
+      for (int q = 0; q < 3; q++) {
+        so += Ko[3 * b + q] * xb[b][q];
+        su += Ku[3 * b + q] * xb[b][q];
+      }
+    own[p] += sc * so;
+    up[p] += sc * su;
+  }
+}
+
+// Element tables of the compact row forms: R (SYM_TABLE per type) for the
+// types below own0, the full tables K (576 per type) from own0 on
+template <typename T>
+struct SymTabs {
+  const T *R;
+  const T *K;
+  int own0;
+};
+
 // Column form on the LDS neighbourhood: for node (i, j, k) the four cells of
 // level k (above the node).  own = their rows of node (i, j, k), up = their
-// rows of node (i, j, k+1); the node's full row is own_k + up_{k-1}.
-template <typename T, int CI, int CJ>
+// rows of node (i, j, k+1); the node's full row is own_k + up_{k-1}.  HYB:
+// some types (from tb.own0 on) take the full tables.
+template <typename T, int CI, int CJ, bool HYB>
 FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
-                       const T *__restrict__ cellScale, const T *__restrict__ R, const T *nb, T own[3], T up[3],
+                       const T *__restrict__ cellScale, const SymTabs<T> &tb, const T *nb, T own[3], T up[3],
                        int nzp) {
   const int ii = i - 1 + CI, jj = j - 1 + CJ;
   if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy) return;
@@ -991,27 +1019,30 @@ FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__rest
       const T *xv = nb + (col * nzp + k + corner_lk(b) + 1) * 3;
       xb[b][0] = xv[0]; xb[b][1] = xv[1]; xb[b][2] = xv[2];
     }
-    cell_sym_pair<T, 1 - CI, 1 - CJ>(R + SYM_TABLE * t, xb, sc, own, up);
+    if (HYB && t >= tb.own0)
+      cell_full_pair<T, 1 - CI, 1 - CJ>(tb.K + 576 * (int64_t)t, xb, sc, own, up);
+    else
+      cell_sym_pair<T, 1 - CI, 1 - CJ>(tb.R + SYM_TABLE * t, xb, sc, own, up);
   }
 }
 
-template <typename T>
+template <typename T, bool HYB = false>
 FE_HD void node_rows_nb_sym(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
-                            const T *__restrict__ cellScale, const T *__restrict__ R, const T *nb, T own[3],
+                            const T *__restrict__ cellScale, const SymTabs<T> &tb, const T *nb, T own[3],
                             T up[3], int nzp) {
   for (int p = 0; p < 3; p++) { own[p] = 0; up[p] = 0; }
   if (k >= g.ncz) return;
-  cell_sym_nb<T, 0, 0>(g, i, j, k, cellType, cellScale, R, nb, own, up, nzp);
-  cell_sym_nb<T, 0, 1>(g, i, j, k, cellType, cellScale, R, nb, own, up, nzp);
-  cell_sym_nb<T, 1, 0>(g, i, j, k, cellType, cellScale, R, nb, own, up, nzp);
-  cell_sym_nb<T, 1, 1>(g, i, j, k, cellType, cellScale, R, nb, own, up, nzp);
+  cell_sym_nb<T, 0, 0, HYB>(g, i, j, k, cellType, cellScale, tb, nb, own, up, nzp);
+  cell_sym_nb<T, 0, 1, HYB>(g, i, j, k, cellType, cellScale, tb, nb, own, up, nzp);
+  cell_sym_nb<T, 1, 0, HYB>(g, i, j, k, cellType, cellScale, tb, nb, own, up, nzp);
+  cell_sym_nb<T, 1, 1, HYB>(g, i, j, k, cellType, cellScale, tb, nb, own, up, nzp);
 }
 
 // Host form of one node row through the symmetric tables (same output rule as
 // node_apply: w * (A w x) on free DOFs, x on constrained ones).
 template <typename T, int CI, int CJ>
 FE_HD void cell_sym_direct(const Grid &g, int i, int j, int kk, bool wantUp, const int32_t *__restrict__ cellType,
-                           const T *__restrict__ cellScale, const T *__restrict__ R, const T *__restrict__ w,
+                           const T *__restrict__ cellScale, const SymTabs<T> &tb, const T *__restrict__ w,
                            const T *__restrict__ x, T acc[3]) {
   const int ii = i - 1 + CI, jj = j - 1 + CJ;
   if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy || kk < 0 || kk >= g.ncz) return;
@@ -1026,14 +1057,17 @@ FE_HD void cell_sym_direct(const Grid &g, int i, int j, int kk, bool wantUp, con
       for (int p = 0; p < 3; p++) xb[b][p] = w[3 * gn + p] * x[3 * gn + p];
     }
     T own[3] = {0, 0, 0}, up[3] = {0, 0, 0};
-    cell_sym_pair<T, 1 - CI, 1 - CJ>(R + SYM_TABLE * t, xb, sc, own, up);
+    if (t >= tb.own0)
+      cell_full_pair<T, 1 - CI, 1 - CJ>(tb.K + 576 * (int64_t)t, xb, sc, own, up);
+    else
+      cell_sym_pair<T, 1 - CI, 1 - CJ>(tb.R + SYM_TABLE * t, xb, sc, own, up);
     for (int p = 0; p < 3; p++) acc[p] += wantUp ? up[p] : own[p];
   }
 }
 
 template <typename T>
 FE_HD void node_apply_sym(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
-                          const T *__restrict__ cellScale, const T *__restrict__ R, const T *__restrict__ w,
+                          const T *__restrict__ cellScale, const SymTabs<T> &R, const T *__restrict__ w,
                           const T *__restrict__ x, T out[3]) {
   T own[3] = {0, 0, 0}, up[3] = {0, 0, 0};
   cell_sym_direct<T, 0, 0>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
