@@ -734,8 +734,13 @@ void factor_fine_columns(fcbga_model *M) {
   // sub-diagonal block of the node above)
   be::pfor(g.G, M->stream, [=] __host__ __device__(int64_t gi) {
     const int k = (int)(gi % nz);
-    fe::node_scan_coefs_at(k, nz, F + 18 * gi, C + NC * gi);
-    for (int q = 0; q < NC; q++) C32[NC * gi + q] = (float)C[NC * gi + q];
+    const int64_t c = gi / nz;
+    double v[NC];
+    fe::node_scan_coefs_at(k, nz, F + 18 * gi, v);
+    for (int q = 0; q < NC; q++) {
+      C[fe::scan_coef_index<double>(c, k, nz, q)] = v[q];
+      C32[fe::scan_coef_index<float>(c, k, nz, q)] = (float)v[q];
+    }
   });
 }
 
@@ -960,27 +965,33 @@ __device__ __forceinline__ void scan_round(T A[9], T b[3], int s, bool needA) {
   for (int p = 0; p < 3; p++) b[p] = rb[p];
 }
 
-// the SCAN_COEFS coefficients of one node (16-byte aligned records)
+// the SCAN_COEFS coefficients of node (column c, plane k) from the device
+// layout (fe::scan_coef_index): slot v of a column is one contiguous run of
+// 16-byte vectors over the planes, so each load instruction of a wavefront
+// reads one run
 template <typename T>
-__device__ __forceinline__ void load_scan_coefs(const T *__restrict__ C, int gi, bool valid, T out[fe::SCAN_COEFS]);
+__device__ __forceinline__ void load_scan_coefs(const T *__restrict__ C, int c, int k, int nz, bool valid,
+                                                T out[fe::SCAN_COEFS]);
 template <>
-__device__ __forceinline__ void load_scan_coefs<float>(const float *__restrict__ C, int gi, bool valid,
+__device__ __forceinline__ void load_scan_coefs<float>(const float *__restrict__ C, int c, int k, int nz, bool valid,
                                                         float out[fe::SCAN_COEFS]) {
-  const float4 *p = reinterpret_cast<const float4 *>(C + (int64_t)fe::SCAN_COEFS * (valid ? gi : 0));
+  constexpr int NV = fe::SCAN_COEFS / 4;
+  const float4 *p = reinterpret_cast<const float4 *>(C) + (int64_t)c * NV * nz + (valid ? k : 0);
 #pragma unroll
-  for (int v = 0; v < fe::SCAN_COEFS / 4; v++) {
-    const float4 q = p[v];
+  for (int v = 0; v < NV; v++) {
+    const float4 q = p[(int64_t)v * nz];
     out[4 * v] = valid ? q.x : 0.f; out[4 * v + 1] = valid ? q.y : 0.f;
     out[4 * v + 2] = valid ? q.z : 0.f; out[4 * v + 3] = valid ? q.w : 0.f;
   }
 }
 template <>
-__device__ __forceinline__ void load_scan_coefs<double>(const double *__restrict__ C, int gi, bool valid,
+__device__ __forceinline__ void load_scan_coefs<double>(const double *__restrict__ C, int c, int k, int nz, bool valid,
                                                          double out[fe::SCAN_COEFS]) {
-  const double2 *p = reinterpret_cast<const double2 *>(C + (int64_t)fe::SCAN_COEFS * (valid ? gi : 0));
+  constexpr int NV = fe::SCAN_COEFS / 2;
+  const double2 *p = reinterpret_cast<const double2 *>(C) + (int64_t)c * NV * nz + (valid ? k : 0);
 #pragma unroll
-  for (int v = 0; v < fe::SCAN_COEFS / 2; v++) {
-    const double2 q = p[v];
+  for (int v = 0; v < NV; v++) {
+    const double2 q = p[(int64_t)v * nz];
     out[2 * v] = valid ? q.x : 0.0; out[2 * v + 1] = valid ? q.y : 0.0;
   }
 }
@@ -1024,6 +1035,9 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
     xv[p] = x[3 * gi + p];
     bv[p] = bvec[3 * gi + p];
   }
+  // the types and scales of the node's four cells, with the staging loads
+  fe::CellQuad<T> cq;
+  if (SYM) fe::load_cell_quad<T>(g, i, j, k, valid, ct, cs, cq);
   {
     T *nb0 = PACK ? reinterpret_cast<T *>(nb_dyn) : nb_fix;
     for (int q = threadIdx.x; q < P * 9 * 2 * 3; q += 64) {  // padding levels -1 and nz of every column
@@ -1037,7 +1051,15 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
   T acc[3] = {0, 0, 0};
   if (SYM) {
     T up[3] = {0, 0, 0};
-    if (valid) fe::node_rows_nb_sym<T, FORM == 2>(g, i, j, k, ct, cs, fe::SymTabs<T>{Kc, Kf, own0}, nb, acc, up, nzp);
+    const fe::SymTabs<T> tb{Kc, Kf, own0};
+    // wavefronts whose nodes all see one table in their four cells load it
+    // once (the scan coefficients stay behind the residual: loading them here
+    // as well costs a wave per SIMD in registers, measured slower)
+    if (__all(!valid || fe::cell_quad_uniform<T, FORM == 2>(cq, tb))) {
+      if (valid) fe::node_rows_nb_sym_uni<T>(k, cq, tb, nb, acc, up, nzp);
+    } else if (valid) {
+      fe::node_rows_nb_sym_q<T, FORM == 2>(k, cq, tb, nb, acc, up, nzp);
+    }
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);
       acc[p] += k > 0 ? u : T(0);
@@ -1051,17 +1073,21 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
   // dx_k = Q_k dx_{k+1} + Li_k^T y_k, each as a scan over the 64 lanes in six
   // rounds (the coefficients of lanes past the column are zero; rounds at
   // distances beyond nz change nothing)
-  load_scan_coefs<T>(C, gi, valid, Ck);
-  const T *Li = Ck;
+  load_scan_coefs<T>(C, c, k, nz, valid, Ck);
+  // Li is lower triangular: Ck[0..5] = Li00, Li10, Li11, Li20, Li21, Li22
   T A[9], b[3];
-  for (int q = 0; q < 9; q++) A[q] = Ck[9 + q];
-  for (int p = 0; p < 3; p++) b[p] = Li[3 * p] * r[0] + Li[3 * p + 1] * r[1] + Li[3 * p + 2] * r[2];
+  for (int q = 0; q < 9; q++) A[q] = Ck[6 + q];
+  b[0] = Ck[0] * r[0];
+  b[1] = Ck[1] * r[0] + Ck[2] * r[1];
+  b[2] = Ck[3] * r[0] + Ck[4] * r[1] + Ck[5] * r[2];
 #pragma unroll
   for (int s = 1; s < 64; s <<= 1) scan_round<T, true>(A, b, s, s < 32);
   {
     const T y0 = b[0], y1 = b[1], y2 = b[2];
-    for (int q = 0; q < 9; q++) A[q] = Ck[18 + q];
-    for (int p = 0; p < 3; p++) b[p] = Li[p] * y0 + Li[3 + p] * y1 + Li[6 + p] * y2;
+    for (int q = 0; q < 9; q++) A[q] = Ck[15 + q];
+    b[0] = Ck[0] * y0 + Ck[1] * y1 + Ck[3] * y2;
+    b[1] = Ck[2] * y1 + Ck[4] * y2;
+    b[2] = Ck[5] * y2;
   }
 #pragma unroll
   for (int s = 1; s < 64; s <<= 1) scan_round<T, false>(A, b, s, s < 32);
@@ -1092,6 +1118,8 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
   const int i = cc / g.ny, j = cc % g.ny;
   T *nb = reinterpret_cast<T *>(nb_lds) + (slot < P ? slot : 0) * 9 * (nz + 2) * 3;
   const int gi = cc * nz + k;
+  fe::CellQuad<T> cq;
+  if (SYM) fe::load_cell_quad<T>(g, i, j, k, valid, ct, cs, cq);
   for (int q = threadIdx.x; q < P * 9 * 2 * 3; q += 64) {
     const int sl = q / 54, col = (q % 54) / 6, side = (q / 3) % 2, p = q % 3;
     reinterpret_cast<T *>(nb_lds)[((sl * 9 + col) * (nz + 2) + (side ? nz + 1 : 0)) * 3 + p] = T(0);
@@ -1101,7 +1129,12 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
   T acc[3] = {0, 0, 0};
   if (SYM) {
     T up[3] = {0, 0, 0};
-    if (valid) fe::node_rows_nb_sym<T, FORM == 2>(g, i, j, k, ct, cs, fe::SymTabs<T>{Kc, Kf, own0}, nb, acc, up, nz + 2);
+    const fe::SymTabs<T> tb{Kc, Kf, own0};
+    if (__all(!valid || fe::cell_quad_uniform<T, FORM == 2>(cq, tb))) {
+      if (valid) fe::node_rows_nb_sym_uni<T>(k, cq, tb, nb, acc, up, nz + 2);
+    } else if (valid) {
+      fe::node_rows_nb_sym_q<T, FORM == 2>(k, cq, tb, nb, acc, up, nz + 2);
+    }
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);
       acc[p] += k > 0 ? u : T(0);
@@ -1180,7 +1213,7 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
   });
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t];
-    fe::column_solve_scan<T, T>(nz, C + fe::SCAN_COEFS * c * nz, r + 3 * c * nz, dx + 3 * c * nz);
+    fe::column_solve_scan<T, T>(nz, C, c, r + 3 * c * nz, dx + 3 * c * nz);
     for (int q = 0; q < 3 * nz; q++) x[3 * c * nz + q] += dx[3 * c * nz + q];
   }, 64);
 #endif

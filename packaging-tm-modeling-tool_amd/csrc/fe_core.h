@@ -436,28 +436,55 @@ FE_HD void node_scan_coefs(const T *Fk, bool hasBelow, bool hasAbove, const T *L
 }
 
 // Scan coefficients of a node from the column's factor (18 per node),
-// SCAN_COEFS per node: Li (9), P (9), Q (9) and one zero pad value (so that a
-// node's record is 16-byte aligned).  Formed once per factorisation in FP64;
-// the sweeps read them instead of inverting the diagonal factors.
-constexpr int SCAN_COEFS = 28;
+// SCAN_COEFS per node: the lower triangle of Li by rows (Li00, Li10, Li11,
+// Li20, Li21, Li22; its upper entries are zero), then P (9) and Q (9): 24
+// values, so a node's record is 16-byte aligned (six float4 loads in FP32).
+// Formed once per factorisation in FP64; the sweeps read them instead of
+// inverting the diagonal factors.
+constexpr int SCAN_COEFS = 24;
 
 // node k of a column of nz nodes: Fk its factor record (the next node's
 // follows it), out its SCAN_COEFS coefficients
 FE_HD void node_scan_coefs_at(int k, int nz, const double *Fk, double *out) {
-  double Ld1[9];
+  double Ld1[9], full[27];
   for (int q = 0; q < 9; q++) Ld1[q] = k + 1 < nz ? Fk[18 + 9 + q] : 0.0;
-  node_scan_coefs<double>(Fk, k > 0, k + 1 < nz, Ld1, out);
-  out[27] = 0.0;
+  node_scan_coefs<double>(Fk, k > 0, k + 1 < nz, Ld1, full);
+  out[0] = full[0]; out[1] = full[3]; out[2] = full[4];
+  out[3] = full[6]; out[4] = full[7]; out[5] = full[8];
+  for (int q = 0; q < 18; q++) out[6 + q] = full[9 + q];
+}
+
+// Device layout of the records: per grid column, the records' 16-byte
+// vectors (four FP32 or two FP64 values) are stored plane-major within each
+// vector slot, so that the lanes of a wavefront (one plane each) read one
+// slot of a column as one contiguous run: value q of node (column c, plane k)
+// sits at ((c * NV + q / W) * nz + k) * W + q % W, W = 16 / sizeof(T),
+// NV = SCAN_COEFS / W.
+template <typename T>
+FE_HD int64_t scan_coef_index(int64_t c, int k, int nz, int q) {
+  constexpr int W = 16 / (int)sizeof(T), NV = SCAN_COEFS / W;
+  return ((c * NV + q / W) * nz + k) * W + q % W;
+}
+
+// the 27 coefficients (Li 9 with its zero upper triangle, P, Q) of node k of
+// column c from the device layout
+template <typename T, typename TF>
+FE_HD void unpack_scan_coefs(const TF *C, int64_t c, int k, int nz, T out[27]) {
+  TF v[SCAN_COEFS];
+  for (int q = 0; q < SCAN_COEFS; q++) v[q] = C[scan_coef_index<TF>(c, k, nz, q)];
+  out[0] = (T)v[0]; out[1] = 0; out[2] = 0;
+  out[3] = (T)v[1]; out[4] = (T)v[2]; out[5] = 0;
+  out[6] = (T)v[3]; out[7] = (T)v[4]; out[8] = (T)v[5];
+  for (int q = 0; q < 18; q++) out[9 + q] = (T)v[6 + q];
 }
 
 // Host form of the scan solve (the GPU kernel does the same rounds with lane
-// shuffles): x = (L L^T)^-1 r for one column from its scan coefficients C
-// (SCAN_COEFS per node), nz <= 128.
+// shuffles): x = (L L^T)^-1 r for column c from the scan coefficients Cd
+// (device layout), nz <= 128.
 template <typename T, typename TF>
-FE_HD void column_solve_scan(int nz, const TF *Cc, const T *r, T *x) {
+FE_HD void column_solve_scan(int nz, const TF *Cd, int64_t col, const T *r, T *x) {
   T A[128][9], b[128][3], A2[128][9], b2[128][3], C[128][27];
-  for (int k = 0; k < nz; k++)
-    for (int q = 0; q < 27; q++) C[k][q] = (T)Cc[SCAN_COEFS * k + q];
+  for (int k = 0; k < nz; k++) unpack_scan_coefs<T, TF>(Cd, col, k, nz, C[k]);
   // forward: maps y_k = P_k y_{k-1} + Li_k r_k
   for (int k = 0; k < nz; k++) {
     const T *o = C[k];
@@ -995,21 +1022,47 @@ struct SymTabs {
   int own0;
 };
 
-// Column form on the LDS neighbourhood: for node (i, j, k) the four cells of
-// level k (above the node).  own = their rows of node (i, j, k), up = their
-// rows of node (i, j, k+1); the node's full row is own_k + up_{k-1}.  HYB:
-// some types (from tb.own0 on) take the full tables.
+// The four cells of level k around node column (i, j) (quadrant q = 2 CI +
+// CJ, cell (i - 1 + CI, j - 1 + CJ, k)): both element slots' types and scales,
+// -1 where the cell is outside the grid (or k is the top plane).  The loads
+// are issued together, with a clamped index for absent cells, so that a
+// wavefront waits on them once instead of once per cell.
+template <typename T>
+struct CellQuad {
+  int32_t t[4][2];
+  T sc[4][2];
+};
+
+template <typename T>
+FE_HD void load_cell_quad(const Grid &g, int i, int j, int k, bool valid, const int32_t *__restrict__ cellType,
+                          const T *__restrict__ cellScale, CellQuad<T> &cq) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int q = 0; q < 4; q++) {
+    const int ii = i - 1 + (q >> 1), jj = j - 1 + (q & 1);
+    const bool in = valid && k < g.ncz && ii >= 0 && ii < g.ncx && jj >= 0 && jj < g.ncy;
+    const int64_t c = in ? ((int64_t)ii * g.ncy + jj) * g.ncz + k : 0;
+    const int32_t t0 = cellType[2 * c], t1 = cellType[2 * c + 1];
+    const T s0 = cellScale[2 * c], s1 = cellScale[2 * c + 1];
+    cq.t[q][0] = in ? t0 : -1;
+    cq.t[q][1] = in ? t1 : -1;
+    cq.sc[q][0] = s0;
+    cq.sc[q][1] = s1;
+  }
+}
+
+// Column form on the LDS neighbourhood: for node (i, j, k) the cell of
+// quadrant (CI, CJ) at level k (above the node).  own = its rows of node
+// (i, j, k), up = its rows of node (i, j, k+1); the node's full row is own_k +
+// up_{k-1}.  HYB: some types (from tb.own0 on) take the full tables.
 template <typename T, int CI, int CJ, bool HYB>
-FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
-                       const T *__restrict__ cellScale, const SymTabs<T> &tb, const T *nb, T own[3], T up[3],
-                       int nzp) {
-  const int ii = i - 1 + CI, jj = j - 1 + CJ;
-  if (ii < 0 || ii >= g.ncx || jj < 0 || jj >= g.ncy) return;
-  const int c = (ii * g.ncy + jj) * g.ncz + k;
+FE_HD void cell_sym_nb(int k, const CellQuad<T> &cq, const SymTabs<T> &tb, const T *nb, T own[3], T up[3], int nzp) {
+  constexpr int q4 = 2 * CI + CJ;
   for (int q = 0; q < 2; q++) {
-    const int32_t t = cellType[2 * c + q];
+    const int32_t t = cq.t[q4][q];
     if (t < 0) continue;
-    const T sc = cellScale[2 * c + q];
+    const T sc = cq.sc[q4][q];
     T xb[8][3];
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
@@ -1026,16 +1079,81 @@ FE_HD void cell_sym_nb(const Grid &g, int i, int j, int k, const int32_t *__rest
   }
 }
 
+// own and up rows of node (i, j, k) from the four cells of level k (cq from
+// load_cell_quad)
+template <typename T, bool HYB = false>
+FE_HD void node_rows_nb_sym_q(int k, const CellQuad<T> &cq, const SymTabs<T> &tb, const T *nb, T own[3], T up[3],
+                              int nzp) {
+  for (int p = 0; p < 3; p++) { own[p] = 0; up[p] = 0; }
+  cell_sym_nb<T, 0, 0, HYB>(k, cq, tb, nb, own, up, nzp);
+  cell_sym_nb<T, 0, 1, HYB>(k, cq, tb, nb, own, up, nzp);
+  cell_sym_nb<T, 1, 0, HYB>(k, cq, tb, nb, own, up, nzp);
+  cell_sym_nb<T, 1, 1, HYB>(k, cq, tb, nb, own, up, nzp);
+}
+
+// true when the four cells hold one element each, of one compact-table type
+// (scales may differ), or no element at all: their rows then share one table
+template <typename T, bool HYB = false>
+FE_HD bool cell_quad_uniform(const CellQuad<T> &cq, const SymTabs<T> &tb) {
+  const int32_t t = cq.t[0][0];
+  bool u = !(HYB && t >= tb.own0);
+  for (int q = 0; q < 4; q++) u = u && cq.t[q][0] == t && cq.t[q][1] < 0;
+  return u;
+}
+
+// node_rows_nb_sym_q for a uniform quad (cell_quad_uniform): the compact
+// table is loaded once for the four cells; the rows are summed in the same
+// order with the same terms
+// the compact table of the quad's type (zeros for an empty quad)
+template <typename T>
+FE_HD void load_quad_table(const CellQuad<T> &cq, const SymTabs<T> &tb, T R[SYM_TABLE]) {
+  const int32_t t = cq.t[0][0];
+  const T *Rg = tb.R + SYM_TABLE * (t < 0 ? 0 : t);
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int e = 0; e < SYM_TABLE; e++) R[e] = Rg[e];
+}
+
+// rows of a uniform quad from its table R (load_quad_table)
+template <typename T>
+FE_HD void node_rows_nb_sym_uni_R(int k, const CellQuad<T> &cq, const T R[SYM_TABLE], const T *nb, T own[3], T up[3],
+                                  int nzp) {
+  for (int p = 0; p < 3; p++) { own[p] = 0; up[p] = 0; }
+  if (cq.t[0][0] < 0) return;
+  T xb[8][3];
+#define FE_UNI_QUAD(CI, CJ)                                                      \
+  for (int b = 0; b < 8; b++) {                                                  \
+    const int col = (CI + corner_li(b)) * 3 + (CJ + corner_lj(b));               \
+    const T *xv = nb + (col * nzp + k + corner_lk(b) + 1) * 3;                   \
+    xb[b][0] = xv[0]; xb[b][1] = xv[1]; xb[b][2] = xv[2];                        \
+  }                                                                              \
+  cell_sym_pair<T, 1 - CI, 1 - CJ>(R, xb, cq.sc[2 * CI + CJ][0], own, up);
+  FE_UNI_QUAD(0, 0)
+  FE_UNI_QUAD(0, 1)
+  FE_UNI_QUAD(1, 0)
+  FE_UNI_QUAD(1, 1)
+#undef FE_UNI_QUAD
+}
+
+template <typename T>
+FE_HD void node_rows_nb_sym_uni(int k, const CellQuad<T> &cq, const SymTabs<T> &tb, const T *nb, T own[3], T up[3],
+                                int nzp) {
+  T R[SYM_TABLE];
+  load_quad_table<T>(cq, tb, R);
+  node_rows_nb_sym_uni_R<T>(k, cq, R, nb, own, up, nzp);
+}
+
 template <typename T, bool HYB = false>
 FE_HD void node_rows_nb_sym(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
                             const T *__restrict__ cellScale, const SymTabs<T> &tb, const T *nb, T own[3],
                             T up[3], int nzp) {
-  for (int p = 0; p < 3; p++) { own[p] = 0; up[p] = 0; }
-  if (k >= g.ncz) return;
-  cell_sym_nb<T, 0, 0, HYB>(g, i, j, k, cellType, cellScale, tb, nb, own, up, nzp);
-  cell_sym_nb<T, 0, 1, HYB>(g, i, j, k, cellType, cellScale, tb, nb, own, up, nzp);
-  cell_sym_nb<T, 1, 0, HYB>(g, i, j, k, cellType, cellScale, tb, nb, own, up, nzp);
-  cell_sym_nb<T, 1, 1, HYB>(g, i, j, k, cellType, cellScale, tb, nb, own, up, nzp);
+  CellQuad<T> cq;
+  load_cell_quad<T>(g, i, j, k, true, cellType, cellScale, cq);
+  if (cell_quad_uniform<T, HYB>(cq, tb))
+    node_rows_nb_sym_uni<T>(k, cq, tb, nb, own, up, nzp);
+  else
+    node_rows_nb_sym_q<T, HYB>(k, cq, tb, nb, own, up, nzp);
 }
 
 // Host form of one node row through the symmetric tables (same output rule as
