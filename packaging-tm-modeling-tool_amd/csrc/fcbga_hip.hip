@@ -1057,8 +1057,15 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
     // as well costs a wave per SIMD in registers, measured slower)
     if (__all(!valid || fe::cell_quad_uniform<T, FORM == 2>(cq, tb))) {
       if (valid) fe::node_rows_nb_sym_uni<T>(k, cq, tb, nb, acc, up, nzp);
-    } else if (valid) {
-      fe::node_rows_nb_sym_q<T, FORM == 2>(k, cq, tb, nb, acc, up, nzp);
+    } else {
+      const int pr = valid ? fe::cell_quad_pairs<T, FORM == 2>(cq, tb) : 1;
+      if (__all(pr == 1)) {
+        if (valid) fe::node_rows_nb_sym_pairs<T, 1>(k, cq, tb, nb, acc, up, nzp);
+      } else if (__all(!valid || pr == 2)) {
+        if (valid) fe::node_rows_nb_sym_pairs<T, 2>(k, cq, tb, nb, acc, up, nzp);
+      } else if (valid) {
+        fe::node_rows_nb_sym_q<T, FORM == 2>(k, cq, tb, nb, acc, up, nzp);
+      }
     }
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);
@@ -1132,8 +1139,15 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
     const fe::SymTabs<T> tb{Kc, Kf, own0};
     if (__all(!valid || fe::cell_quad_uniform<T, FORM == 2>(cq, tb))) {
       if (valid) fe::node_rows_nb_sym_uni<T>(k, cq, tb, nb, acc, up, nz + 2);
-    } else if (valid) {
-      fe::node_rows_nb_sym_q<T, FORM == 2>(k, cq, tb, nb, acc, up, nz + 2);
+    } else {
+      const int pr = valid ? fe::cell_quad_pairs<T, FORM == 2>(cq, tb) : 1;
+      if (__all(pr == 1)) {
+        if (valid) fe::node_rows_nb_sym_pairs<T, 1>(k, cq, tb, nb, acc, up, nz + 2);
+      } else if (__all(!valid || pr == 2)) {
+        if (valid) fe::node_rows_nb_sym_pairs<T, 2>(k, cq, tb, nb, acc, up, nz + 2);
+      } else if (valid) {
+        fe::node_rows_nb_sym_q<T, FORM == 2>(k, cq, tb, nb, acc, up, nz + 2);
+      }
     }
     for (int p = 0; p < 3; p++) {
       const T u = __shfl_up(up[p], 1, 64);

@@ -1144,14 +1144,77 @@ FE_HD void node_rows_nb_sym_uni(int k, const CellQuad<T> &cq, const SymTabs<T> &
   node_rows_nb_sym_uni_R<T>(k, cq, R, nb, own, up, nzp);
 }
 
+// Quads whose cells share their types in pairs: J (the two cells of one
+// column of cells, i.e. equal i - 1 + CI, hold one type each: quadrants 0, 1
+// and 2, 3) or I (equal j - 1 + CJ: quadrants 0, 2 and 1, 3); one element per
+// cell, compact tables.  Absent cells (type -1) pair with absent cells.
+template <typename T, bool HYB = false>
+FE_HD int cell_quad_pairs(const CellQuad<T> &cq, const SymTabs<T> &tb) {
+  bool ok = true;
+  for (int q = 0; q < 4; q++) ok = ok && cq.t[q][1] < 0 && !(HYB && cq.t[q][0] >= tb.own0);
+  if (!ok) return 0;
+  if (cq.t[0][0] == cq.t[1][0] && cq.t[2][0] == cq.t[3][0]) return 1;  // J
+  if (cq.t[0][0] == cq.t[2][0] && cq.t[1][0] == cq.t[3][0]) return 2;  // I
+  return 0;
+}
+
+// rows of a paired quad (cell_quad_pairs = PAIR): two table loads instead of
+// four.  J keeps the quadrant order of node_rows_nb_sym_q; I takes the
+// quadrants sharing a table together (0, 2, 1, 3), which changes the order in
+// which the four cells' terms are added (rounding only).
+template <typename T, int PAIR>
+FE_HD void node_rows_nb_sym_pairs(int k, const CellQuad<T> &cq, const SymTabs<T> &tb, const T *nb, T own[3],
+                                  T up[3], int nzp) {
+  for (int p = 0; p < 3; p++) { own[p] = 0; up[p] = 0; }
+  T R[SYM_TABLE];
+  T xb[8][3];
+#define FE_PAIR_QUAD(CI, CJ)                                                     \
+  if (cq.t[2 * CI + CJ][0] >= 0) {                                               \
+    for (int b = 0; b < 8; b++) {                                                \
+      const int col = (CI + corner_li(b)) * 3 + (CJ + corner_lj(b));             \
+      const T *xv = nb + (col * nzp + k + corner_lk(b) + 1) * 3;                 \
+      xb[b][0] = xv[0]; xb[b][1] = xv[1]; xb[b][2] = xv[2];                      \
+    }                                                                            \
+    cell_sym_pair<T, 1 - CI, 1 - CJ>(R, xb, cq.sc[2 * CI + CJ][0], own, up);     \
+  }
+#define FE_PAIR_TABLE(Q)                                                         \
+  {                                                                              \
+    const int32_t t = cq.t[Q][0];                                                \
+    const T *Rg = tb.R + SYM_TABLE * (t < 0 ? 0 : t);                            \
+    for (int e = 0; e < SYM_TABLE; e++) R[e] = Rg[e];                            \
+  }
+  if (PAIR == 1) {
+    FE_PAIR_TABLE(0)
+    FE_PAIR_QUAD(0, 0)
+    FE_PAIR_QUAD(0, 1)
+    FE_PAIR_TABLE(2)
+    FE_PAIR_QUAD(1, 0)
+    FE_PAIR_QUAD(1, 1)
+  } else {
+    FE_PAIR_TABLE(0)
+    FE_PAIR_QUAD(0, 0)
+    FE_PAIR_QUAD(1, 0)
+    FE_PAIR_TABLE(1)
+    FE_PAIR_QUAD(0, 1)
+    FE_PAIR_QUAD(1, 1)
+  }
+#undef FE_PAIR_QUAD
+#undef FE_PAIR_TABLE
+}
+
 template <typename T, bool HYB = false>
 FE_HD void node_rows_nb_sym(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
                             const T *__restrict__ cellScale, const SymTabs<T> &tb, const T *nb, T own[3],
                             T up[3], int nzp) {
   CellQuad<T> cq;
   load_cell_quad<T>(g, i, j, k, true, cellType, cellScale, cq);
+  const int pr = cell_quad_pairs<T, HYB>(cq, tb);
   if (cell_quad_uniform<T, HYB>(cq, tb))
     node_rows_nb_sym_uni<T>(k, cq, tb, nb, own, up, nzp);
+  else if (pr == 1)
+    node_rows_nb_sym_pairs<T, 1>(k, cq, tb, nb, own, up, nzp);
+  else if (pr == 2)
+    node_rows_nb_sym_pairs<T, 2>(k, cq, tb, nb, own, up, nzp);
   else
     node_rows_nb_sym_q<T, HYB>(k, cq, tb, nb, own, up, nzp);
 }
