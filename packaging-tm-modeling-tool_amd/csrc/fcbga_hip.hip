@@ -21,6 +21,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <cstring>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -30,6 +31,14 @@ namespace {
 thread_local std::string g_err;
 
 constexpr int SHELL_MAXM = 512;
+// Column descriptor of a shell level (one 48-byte record per column, read by
+// every lane of the column's wavefront in one load): [0] first aggregate n0,
+// [1] aggregates na, [2..3] offset of the column's inverse (int64), then the
+// block-row pointers rowPtr[n0 .. n0 + min(na, SHELL_DESC_ROWS)].  The sweep
+// of a column with at most SHELL_DESC_ROWS aggregates needs no other index
+// load before its blocks.
+constexpr int SHELL_DESC = 12;
+constexpr int SHELL_DESC_ROWS = SHELL_DESC - 5;
 // largest deviation (relative to the largest entry) of a condensed element
 // table from its mirror-symmetric form for the compact tables to be used
 constexpr double SYM_TOL = 1e-11;  // 6 DOFs x up to 85 aggregates in one shell column
@@ -172,6 +181,7 @@ struct DevShell {
   DBuf<float> cf32;
   DBuf<int32_t> colors[4];
   int32_t ncolor[4] = {0, 0, 0, 0};
+  DBuf<int32_t> colDesc;  // SHELL_DESC ints per column (shell_desc_fill)
   // level 1: aggregate column and k range
   DBuf<int32_t> aggCol, k0, k1;
   // transfer to the next coarser level
@@ -1285,6 +1295,17 @@ void build_hierarchy(fcbga_model *M) {
       D->maxm = m > D->maxm ? m : D->maxm;
     }
     D->cfPtr.upload(cfp, s);
+    {
+      std::vector<int32_t> desc((size_t)SHELL_DESC * D->ncol, 0);
+      for (int32_t c = 0; c < D->ncol; c++) {
+        int32_t *d = desc.data() + (size_t)SHELL_DESC * c;
+        const int32_t n0 = L.colStart[c], na = L.colStart[c + 1] - n0;
+        d[0] = n0; d[1] = na;
+        std::memcpy(d + 2, &cfp[c], sizeof(int64_t));
+        for (int r = 0; r <= std::min(na, SHELL_DESC_ROWS); r++) d[4 + r] = L.rowPtr[n0 + r];
+      }
+      D->colDesc.upload(desc, s);
+    }
     D->cf64.alloc(D->cfTotal);
     D->cf32.alloc(D->cfTotal);
     std::vector<int32_t> cols[4];
@@ -1844,21 +1865,29 @@ __global__ void __launch_bounds__(256) k_dense_matvec(int32_t n, const double *_
 // (the usual case: 1..4 aggregates per column).  The lanes are split into
 // S = 64/m groups: group g takes every S-th block of a row for the residual
 // and every S-th column of the inverse for the correction, and the partial
-// sums meet through lane shuffles (no LDS, no barrier).  The inverse is
-// stored by columns (read coalesced) and is loaded, with b, before the
-// residual so that their latency overlaps it.
+// sums meet through lane shuffles (no LDS, no barrier).  Everything the lanes
+// index by comes from the column's descriptor (dsc, the 12 ints of
+// SHELL_DESC, read once by the whole wavefront), so the inverse, b, the block
+// columns and values are all loaded after one round trip; the inverse is
+// stored by columns (read coalesced).
 template <typename T, int U>
-__device__ __forceinline__ void shell_column_wave(int32_t c, int lane, const int32_t *__restrict__ cst,
-                                                  const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
-                                                  const T *__restrict__ val, const T *__restrict__ inv,
-                                                  const int64_t *__restrict__ cfp, const T *__restrict__ b,
+__device__ __forceinline__ void shell_column_wave(const int32_t dsc[SHELL_DESC], int lane,
+                                                  const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                  const T *__restrict__ inv, const T *__restrict__ b,
                                                   T *__restrict__ x) {
-  const int32_t n0 = cst[c], n1 = cst[c + 1];
-  const int m = 6 * (n1 - n0);
+  const int32_t n0 = dsc[0];
+  const int m = 6 * dsc[1];
   if (m == 0) return;
-  const T *A = inv + cfp[c];
+  int64_t cfo;
+  __builtin_memcpy(&cfo, dsc + 2, sizeof(cfo));
+  const T *A = inv + cfo;
   const int S = 64 / m;
   const int i = lane % m, grp = lane / m;
+  const int r = i / 6;  // row of the column, r < na <= SHELL_DESC_ROWS
+  int32_t k0 = dsc[4], k1 = dsc[5];
+#pragma unroll
+  for (int q = 1; q < SHELL_DESC_ROWS; q++)
+    if (r == q) { k0 = dsc[4 + q]; k1 = dsc[5 + q]; }
   T ainv[16];
 #pragma unroll
   for (int u = 0; u < 16; u++) {
@@ -1867,10 +1896,7 @@ __device__ __forceinline__ void shell_column_wave(int32_t c, int lane, const int
   }
   const T bl = b[6 * (int64_t)n0 + i];
   T acc = 0;
-  if (grp < S) {
-    const int32_t n = n0 + i / 6;
-    row_sub_blocks<T, U>(acc, rp[n] + grp, rp[n + 1], S, i % 6, col, val, x);
-  }
+  if (grp < S) row_sub_blocks<T, U>(acc, k0 + grp, k1, S, i % 6, col, val, x);
   T rr = bl;  // every lane ends with r_i of its row i
   for (int g = 0; g < S; g++) rr += __shfl(acc, g * m + i, 64);
   T dx = 0;
@@ -1888,23 +1914,33 @@ __device__ __forceinline__ void shell_column_wave(int32_t c, int lane, const int
 // one colour of a shell-level column sweep, one 64-lane block per column;
 // colour (ci, cj) holds the columns (ci + 2u, cj + 2v), u-major
 template <typename T, int U>
-__global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, const int32_t *__restrict__ cst,
+__global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, const int32_t *__restrict__ desc,
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                     const T *__restrict__ val, const T *__restrict__ inv,
-                                                    const int64_t *__restrict__ cfp, const T *__restrict__ b,
-                                                    T *__restrict__ x) {
+                                                    const T *__restrict__ b, T *__restrict__ x) {
   __shared__ T r[SHELL_MAXM];
   const int nyc = (ny - cj + 1) >> 1;
   const int32_t c = (ci + 2 * (int)(blockIdx.x / nyc)) * ny + cj + 2 * (int)(blockIdx.x % nyc);
-  const int32_t n0 = cst[c], n1 = cst[c + 1];
-  const int m = 6 * (n1 - n0);
+  int32_t dsc[SHELL_DESC];
+  {
+    const int4 *dp = reinterpret_cast<const int4 *>(desc + (int64_t)SHELL_DESC * c);
+#pragma unroll
+    for (int v = 0; v < SHELL_DESC / 4; v++) {
+      const int4 q = dp[v];
+      dsc[4 * v] = q.x; dsc[4 * v + 1] = q.y; dsc[4 * v + 2] = q.z; dsc[4 * v + 3] = q.w;
+    }
+  }
+  const int32_t n0 = dsc[0];
+  const int m = 6 * dsc[1];
   const int lane = threadIdx.x;
   if (m <= 32) {
-    shell_column_wave<T, U>(c, lane, cst, rp, col, val, inv, cfp, b, x);
+    shell_column_wave<T, U>(dsc, lane, col, val, inv, b, x);
     return;
   }
   // larger columns: one lane per row
-  const T *A = inv + cfp[c];
+  int64_t cfo;
+  __builtin_memcpy(&cfo, dsc + 2, sizeof(cfo));
+  const T *A = inv + cfo;
   for (int i = lane; i < m; i += 64) {
     const int32_t n = n0 + i / 6;
     T acc = b[6 * (int64_t)n0 + i];
@@ -2192,39 +2228,45 @@ struct LevelVec<float> {
 
 template <typename T>
 void shell_sweep(DevShell &D, int color, be::stream_t s) {
-  const int32_t *cols = D.colors[color].p, *cst = D.colStart.p, *rp = D.rowPtr.p, *col = D.col.p;
-  const int64_t *cfp = D.cfPtr.p;
+  const int32_t *cols = D.colors[color].p, *desc = D.colDesc.p, *rp = D.rowPtr.p, *col = D.col.p;
   const T *val = LevelVec<T>::val(D), *inv = LevelVec<T>::cf(D), *b = LevelVec<T>::b(D);
   T *x = LevelVec<T>::x(D);
   const int32_t ncols = D.ncolor[color];
   if (!ncols) return;
 #if FCBGA_GPU
+  (void)cols;
   // entries in flight per lane: two where the colour's columns fill the chip
   // (more registers would cut the waves resident per SIMD), four elsewhere
   if (ncols > 4096)
-    hipLaunchKernelGGL((k_shell_sweep<T, 2>), dim3(ncols), dim3(64), 0, s, color >> 1, color & 1, D.ny, cst, rp, col,
-                       val, inv, cfp, b, x);
+    hipLaunchKernelGGL((k_shell_sweep<T, 2>), dim3(ncols), dim3(64), 0, s, color >> 1, color & 1, D.ny, desc, rp, col,
+                       val, inv, b, x);
   else
-    hipLaunchKernelGGL((k_shell_sweep<T, 4>), dim3(ncols), dim3(64), 0, s, color >> 1, color & 1, D.ny, cst, rp, col,
-                       val, inv, cfp, b, x);
+    hipLaunchKernelGGL((k_shell_sweep<T, 4>), dim3(ncols), dim3(64), 0, s, color >> 1, color & 1, D.ny, desc, rp, col,
+                       val, inv, b, x);
   BE_CHECK(hipGetLastError());
 #else
   (void)s;
   T r[SHELL_MAXM];
   for (int32_t t = 0; t < ncols; t++) {
-    int32_t c = cols[t], n0 = cst[c], n1 = cst[c + 1];
-    int m = 6 * (n1 - n0);
+    // the column's descriptor, as the device kernel reads it
+    const int32_t c = cols[t], *dsc = desc + (int64_t)SHELL_DESC * c;
+    const int32_t n0 = dsc[0], na = dsc[1];
+    int64_t cfo;
+    std::memcpy(&cfo, dsc + 2, sizeof(cfo));
+    const int m = 6 * na;
     for (int i = 0; i < m; i++) {
-      int32_t n = n0 + i / 6;
-      int p = i % 6;
+      const int32_t n = n0 + i / 6;
+      const int p = i % 6;
+      const int32_t k0 = na <= SHELL_DESC_ROWS ? dsc[4 + i / 6] : rp[n];
+      const int32_t k1 = na <= SHELL_DESC_ROWS ? dsc[5 + i / 6] : rp[n + 1];
       T acc = b[6 * (int64_t)n0 + i];
-      for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
+      for (int32_t k = k0; k < k1; k++) {
         const T *v = val + 36 * (int64_t)k + 6 * p, *xj = x + 6 * (int64_t)col[k];
         acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
       }
       r[i] = acc;
     }
-    const T *A = inv + cfp[c];
+    const T *A = inv + cfo;
     for (int i = 0; i < m; i++) {
       T acc = 0;
       for (int j = 0; j < m; j++) acc += A[(int64_t)j * m + i] * r[j];
