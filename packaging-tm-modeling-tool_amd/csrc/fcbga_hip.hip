@@ -2090,6 +2090,130 @@ __global__ void __launch_bounds__(256) k_band_trail(int32_t n, int bw, int w, do
   }
   A[(int64_t)i * w + (k - i + bw)] -= s;
 }
+
+// Banded Cholesky of the coarsest level in one launch of one workgroup
+// (BCH_THREADS threads), right-looking by blocks of BCH columns:
+//  1. wave 0 factors the block's diagonal triangle in registers (lane r holds
+//     row j0 + r; each pivot takes the jitter rule of factorCoarsest, and the
+//     scaled column reaches the other lanes by lane reads),
+//  2. every thread solves one row of the panel below the block (rows j1 ..
+//     j1 + bw - 1) against the triangle, staged in LDS,
+//  3. the threads subtract the panel's rank-BCH product from the trailing band
+//     window, 4 x 4 entries per thread, the panel staged in LDS.
+// The band stays in global memory (it is a few MB, L2-resident); one
+// workgroup keeps it coherent with barriers alone.  The pivots and column
+// entries are those of the left-looking factor up to the order in which the
+// earlier blocks' terms are subtracted.
+constexpr int BCH = 32;
+constexpr int BCH_LD = BCH + 1;
+constexpr int BCH_THREADS = 512;
+
+__host__ __device__ inline size_t band_chol_lds(int bw) { return sizeof(double) * (size_t)BCH_LD * (BCH + bw); }
+
+__global__ void __launch_bounds__(BCH_THREADS) k_band_chol(int32_t n, int bw, int w, double *__restrict__ A,
+                                                           double rel) {
+  extern __shared__ double sm[];
+  double *Dl = sm;                 // BCH x BCH_LD: the block's factored triangle
+  double *Pn = sm + BCH * BCH_LD;  // bw x BCH_LD: the panel rows below it
+  constexpr int NW = BCH_THREADS / 64;
+  __shared__ double red[NW];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double dm = 0;
+  for (int32_t i = tid; i < n; i += BCH_THREADS) dm = fmax(dm, fabs(A[(int64_t)i * w + bw]));
+  for (int off = 32; off > 0; off >>= 1) dm = fmax(dm, __shfl_xor(dm, off, 64));
+  if (lane == 0) red[wv] = dm;
+  __syncthreads();
+  double mx = 0;
+  for (int q = 0; q < NW; q++) mx = fmax(mx, red[q]);
+  const double jitter = rel * (mx != 0 ? mx : 1.0);
+  for (int32_t j0 = 0; j0 < n; j0 += BCH) {
+    const int nb = n - j0 < BCH ? (int)(n - j0) : BCH;
+    const int32_t j1 = j0 + nb;
+    const int R = (int)((j1 + bw < n ? j1 + bw : n) - j1);
+    if (wv == 0) {
+      double row[BCH];
+#pragma unroll
+      for (int c = 0; c < BCH; c++)
+        row[c] = (lane < nb && c <= lane && lane - c <= bw) ? A[(int64_t)(j0 + lane) * w + (c - lane + bw)] : 0.0;
+#pragma unroll
+      for (int c = 0; c < BCH; c++) {
+        if (c < nb) {
+          double d = __shfl(row[c], c, 64);
+          if (!(d > jitter)) d = fmax(fabs(d), jitter);
+          const double ljj = sqrt(d), inv = 1.0 / ljj;
+          const double l = lane == c ? ljj : (lane > c ? row[c] * inv : row[c]);
+          row[c] = l;
+#pragma unroll
+          for (int c2 = c + 1; c2 < BCH; c2++) {
+            const double lc = __shfl(l, c2, 64);
+            if (lane >= c2) row[c2] -= l * lc;
+          }
+        }
+      }
+      if (lane < nb) {
+#pragma unroll
+        for (int c = 0; c < BCH; c++) {
+          Dl[lane * BCH_LD + c] = c <= lane ? row[c] : 0.0;
+          if (c <= lane && lane - c <= bw) A[(int64_t)(j0 + lane) * w + (c - lane + bw)] = row[c];
+        }
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < R; t += BCH_THREADS) {
+      const int32_t i = j1 + t;
+      double *x = Pn + t * BCH_LD;  // the row's solution, built in place
+      for (int c = 0; c < BCH; c++) {
+        const bool in = c < nb && i - (j0 + c) <= bw;
+        x[c] = in ? A[(int64_t)i * w + (j0 + c - i + bw)] : 0.0;
+      }
+#pragma unroll 1
+      for (int c = 0; c < nb; c++) {
+        double a = x[c];
+#pragma unroll 4
+        for (int k = 0; k < c; k++) a -= x[k] * Dl[c * BCH_LD + k];
+        a /= Dl[c * BCH_LD + c];
+        x[c] = a;
+        if (i - (j0 + c) <= bw) A[(int64_t)i * w + (j0 + c - i + bw)] = a;
+      }
+    }
+    __syncthreads();
+    const int T = (R + 3) >> 2;
+    for (int q = tid; q < T * (T + 1) / 2; q += BCH_THREADS) {
+      // tile (ti, tk), tk <= ti, in row-major order of the lower triangle
+      int ti = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+      while ((ti + 1) * (ti + 2) / 2 <= q) ti++;
+      while (ti * (ti + 1) / 2 > q) ti--;
+      const int tk = q - ti * (ti + 1) / 2;
+      double acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+#pragma unroll 2
+      for (int c = 0; c < nb; c++) {
+        double pi[4], pk[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          const int ri = 4 * ti + a, rk = 4 * tk + a;
+          pi[a] = ri < R ? Pn[ri * BCH_LD + c] : 0.0;
+          pk[a] = rk < R ? Pn[rk * BCH_LD + c] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++) acc[a][b] += pi[a] * pk[b];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int ri = 4 * ti + a, rk = 4 * tk + b;
+          if (ri < R && rk <= ri) A[(int64_t)(j1 + ri) * w + (rk - ri + bw)] -= acc[a][b];
+        }
+    }
+    __syncthreads();
+  }
+}
 #endif
 
 // serial banded Cholesky (left-looking) with the jitter rule of factorCoarsest
@@ -2148,6 +2272,15 @@ void factor_coarsest(fcbga_model *M) {
   // columns) fits 64 KB of LDS, else 16, else the host factor below
   const auto lds_of = [&](int P) { return (size_t)(P + bw) * (P + 3) * sizeof(double); };
   const int P = lds_of(32) <= 64 * 1024 ? 32 : lds_of(16) <= 64 * 1024 ? 16 : 0;
+  if (band_chol_lds(bw) <= 144 * 1024) {
+    // the whole factor in one workgroup (the block and its panel fit LDS)
+    hipLaunchKernelGGL(k_band_chol, dim3(1), dim3(BCH_THREADS), band_chol_lds(bw), s, n, bw, w, A, 1e-12);
+    BE_CHECK(hipGetLastError());
+    D.inv64.alloc((size_t)n * n);
+    D.ndense = n;
+    band_inverse(n, bw, w, D.band.p, D.inv64.p, s);
+    return;
+  }
   if (P) {
     D.jit.alloc(1);
     hipLaunchKernelGGL(k_band_jitter, dim3(1), dim3(64), 0, s, n, bw, w, A, 1e-12, D.jit.p);
