@@ -455,6 +455,11 @@ def cfg2_spec() -> dict:
         "top_plane": 52,
         "temperatures": [25.0, 72.0, 119.0, 166.0, 213.0, 260.0],
         "tol": 1e-10,
+        # stress recovery checks: die top/bottom surface maps; joint loads of a
+        # 39 x 39 field at 0.5 mm pitch (0.1 mm joints) in the smeared bump layer
+        "stress_die": {"rect": die, "ztop": zs[38], "zb": zs[24], "gkey": "si"},
+        "stress_joints": {"nbx": 39, "nby": 39, "bx0": -9.5, "by0": -9.5, "px": 0.5, "py": 0.5,
+                          "zmid": 0.5 * (zs[20] + zs[24]), "part": PART["UF_BUMP"], "A": math.pi * 0.05 ** 2},
     }
 
 

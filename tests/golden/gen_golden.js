@@ -8,7 +8,8 @@
 //   <name>_golden.json                  (solve record, warpage, sizes)
 //   element_golden.json                 (condensed element tables on sampled boxes)
 //   anand_golden.json                   (Anand material-point step sequences)
-//   <name>_stress_golden.json           (element states, point stresses, die surface maps, joint loads)
+//   <name>_stress_golden.json           (element states, point stresses, die surface maps, joint loads;
+//                                        cfg2 with the argument stress_cfg2, a three-minute reference solve)
 //   <name>_warm_golden.json, <name>_warm_u.f64.gz (warm-started solves along a path with T-dependent moduli)
 //   <name>_ic_golden.json, <name>_ic_u.f64.gz     (IC(0)-preconditioned solves)
 //   fallback_golden.json                (the fallback to the other preconditioner)
@@ -586,6 +587,7 @@ if (what.includes('package')) packageCases();
 if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }
 if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
 if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(n);
+if (what.includes('stress_cfg2')) stressCases('cfg2');
 if (what.includes('anand')) anandCases();
 if (what.includes('screen')) screenCases();
 if (what.includes('submodel')) submodelCases();

@@ -143,13 +143,33 @@ def test_backward_without_forward_and_bad_objective():
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", [hs.PREC_FP64, hs.PREC_MIXED])
 def test_gpu_adjoint_small_box(prec):
+    """The device pair (adjoint solve in either precision) against central
+    differences of the device's own FP64 forward solves, which use no part of
+    the adjoint: every group's E and CTE sensitivity of both objectives, and
+    the per-element CTE sensitivities of two elements, each split off into a
+    group of its own and differenced there."""
     spec, mesh, model, data, obj = _case("hip")
-    emu = _case("hostemu")[2]
-    gJ = np.array([1.0, 0.01])
+    fE, fA = _fd(model, spec, data, obj, hs.PREC_FP64, 1e-13)
+    for comp in (0, 1):
+        gJ = np.eye(2)[comp]
+        dE, dA, _ = _adjoint(model, spec, data, obj, prec, 1e-12, gJ)
+        _check(dE, {g: v[comp] for g, v in fE.items()}, 1e-5)
+        _check(dA, {g: v[comp] for g, v in fA.items()}, 1e-5)
+    gJ = np.array([1.0, 0.02])
     dE, dA, de = _adjoint(model, spec, data, obj, prec, 1e-12, gJ)
-    rE, rA, re = _adjoint(emu, spec, data, obj, hs.PREC_FP64, 1e-13, gJ)
-    for a, r in ((dE, rE), (dA, rA), (de, re)):
-        assert np.abs(a - r).max() <= 1e-7 * np.abs(r).max()
+    for g in range(len(dA)):
+        assert abs(de[mesh.e_group == g].sum() - dA[g]) <= 1e-10 * np.abs(dA).max()
+    D, s0, born = data
+    for e in (int(obj[1][3]), int(np.nonzero(mesh.e_group == 1)[0][5])):
+        m2 = copy.deepcopy(mesh)
+        g = int(m2.e_group[e])
+        m2.e_group = m2.e_group.copy()
+        m2.e_group[e] = len(m2.groups)
+        m2.groups = list(m2.groups) + [dict(m2.groups[g], name=m2.groups[g]["name"] + "_split")]
+        data2 = (np.vstack([D, D[g]]), np.vstack([s0, s0[g]]), np.append(born, born[g]))
+        _, fe = _fd(hs.Model(m2, variant="hip"), spec, data2, obj, hs.PREC_FP64, 1e-13, groups=[len(D)])
+        fd = float(gJ @ fe[len(D)])
+        assert abs(de[e] - fd) <= 1e-5 * abs(fd) + 1e-9 * np.abs(de).max(), (e, de[e], fd)
 
 
 @pytest.mark.gpu
@@ -167,6 +187,12 @@ def test_gpu_cfg4_adjoint_vs_central_differences():
     gJ = np.array([1.0, 1e-3])
     dE, dA, de = _adjoint(model, spec, data, obj, hs.PREC_MIXED, 1e-10, gJ)
     assert np.isfinite(de).all() and len(de) == mesh.ne
+    # the 961,600 per-element CTE sensitivities add up, group by group, to the
+    # group CTE sensitivities checked against differences below (a single
+    # element moves J by less than the differences resolve at this size; the
+    # small-box tests difference single elements)
+    for g in range(len(dA)):
+        assert abs(de[mesh.e_group == g].sum() - dA[g]) <= 1e-9 * np.abs(dA).max(), g
     fE, fA = _fd(model, spec, data, obj, hs.PREC_FP64, 1e-12, h=4e-3, richardson=True)
     _check(dE, {g: float(gJ @ v) for g, v in fE.items()}, 1e-4, floor=1e-2)
     _check(dA, {g: float(gJ @ v) for g, v in fA.items()}, 1e-4, floor=1e-2)

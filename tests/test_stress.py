@@ -209,6 +209,43 @@ def test_gpu_solve_to_die_peak_within_tenth_percent(prec):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", [hs.PREC_FP64, hs.PREC_MIXED])
+def test_gpu_cfg2_solve_to_die_and_joint_peaks(prec):
+    """The lidded 3,093,159-DOF mesh end to end on the device: the step solve
+    (tolerance 1e-10), the device stress states of the die and bump-layer
+    elements and the reference's post functions restated in post.py, against
+    the reference's own solve and post on the same mesh
+    (cfg2_stress_golden.json): peak die and joint stresses within 0.1 %, the
+    die surface maps within 0.1 % of their range."""
+    spec = golden_spec("cfg2")
+    mesh = meshspec.build_mesh(spec)
+    step = hs.step_for_spec(spec)
+    model = hs.Model(mesh, variant="hip")
+    u, rec = model.solve_step(step, tol=spec["tol"], precision=prec)
+    assert rec["converged"]
+    g = stress_golden("cfg2")
+    view = post.StressView(model, [(step, u)], spec["Tb"])
+    sd = spec["stress_die"]
+    x0, x1, y0, y1 = sd["rect"]
+    geom = {"siParts": [{"name": "die", "rect": {"x0": x0, "x1": x1, "y0": y0, "y1": y1}, "ztop": sd["ztop"],
+                         "zb": sd["zb"], "gkey": sd["gkey"], "part": meshspec.PART["DIE"]}]}
+    for which in ("top", "bottom"):
+        r = post.die_surface_stress(view, geom, 0, which)
+        ref = g["die_" + which]
+        for key in ("peakInterior", "gpPeakInterior", "gpPeakAll"):
+            assert abs(r[key] - ref[key]) <= 1e-3 * abs(ref[key]), (which, key, r[key], ref[key])
+        for key in ("smax", "sxx", "syy"):
+            close(r[key], arr(ref[key]), 1e-3)
+    sj = spec["stress_joints"]
+    field = dict(nbx=sj["nbx"], nby=sj["nby"], bx0=sj["bx0"], by0=sj["by0"], px=sj["px"], py=sj["py"])
+    r = post.joint_loads(view, field, sj["zmid"], sj["part"], sj["A"])
+    ref = g["joints"]
+    assert abs(r["worstN"] - ref["worstN"]) <= 1e-3 * abs(ref["worstN"])
+    assert abs(r["worstV"] - ref["worstV"]) <= 1e-3 * abs(ref["worstV"])
+    close(r["N"], arr(ref["N"]), 1e-3)
+
+
+@pytest.mark.gpu
 def test_node_addon_state_view():
     """The JS host path of the stress histories: node -> HipStateView ->
     Node-API addon -> C ABI -> HIP, against the reference's element states and
