@@ -2095,7 +2095,7 @@ __global__ void __launch_bounds__(256) k_band_trail(int32_t n, int bw, int w, do
 // (BCH_THREADS threads), right-looking by blocks of BCH columns:
 //  1. wave 0 factors the block's diagonal triangle in registers (lane r holds
 //     row j0 + r; each pivot takes the jitter rule of factorCoarsest, and the
-//     scaled column reaches the other lanes by lane reads),
+//     pivot and the scaled column reach the other lanes through LDS),
 //  2. every thread solves one row of the panel below the block (rows j1 ..
 //     j1 + bw - 1) against the triangle, staged in LDS,
 //  3. the threads subtract the panel's rank-BCH product from the trailing band
@@ -2117,6 +2117,7 @@ __global__ void __launch_bounds__(BCH_THREADS) k_band_chol(int32_t n, int bw, in
   double *Pn = sm + BCH * BCH_LD;  // bw x BCH_LD: the panel rows below it
   constexpr int NW = BCH_THREADS / 64;
   __shared__ double red[NW];
+  __shared__ double colb[64];  // column broadcast of the diagonal triangle's factor
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double dm = 0;
   for (int32_t i = tid; i < n; i += BCH_THREADS) dm = fmax(dm, fabs(A[(int64_t)i * w + bw]));
@@ -2135,19 +2136,27 @@ __global__ void __launch_bounds__(BCH_THREADS) k_band_chol(int32_t n, int bw, in
 #pragma unroll
       for (int c = 0; c < BCH; c++)
         row[c] = (lane < nb && c <= lane && lane - c <= bw) ? A[(int64_t)(j0 + lane) * w + (c - lane + bw)] : 0.0;
+      // the pivot and the scaled column pass through LDS (one wave: its LDS
+      // operations run in program order, which the may-alias accesses keep)
 #pragma unroll
       for (int c = 0; c < BCH; c++) {
         if (c < nb) {
-          double d = __shfl(row[c], c, 64);
+          colb[lane] = row[c];
+          __builtin_amdgcn_wave_barrier();
+          double d = colb[c];
           if (!(d > jitter)) d = fmax(fabs(d), jitter);
           const double ljj = sqrt(d), inv = 1.0 / ljj;
           const double l = lane == c ? ljj : (lane > c ? row[c] * inv : row[c]);
           row[c] = l;
+          __builtin_amdgcn_wave_barrier();
+          colb[lane] = l;
+          __builtin_amdgcn_wave_barrier();
 #pragma unroll
           for (int c2 = c + 1; c2 < BCH; c2++) {
-            const double lc = __shfl(l, c2, 64);
+            const double lc = colb[c2];
             if (lane >= c2) row[c2] -= l * lc;
           }
+          __builtin_amdgcn_wave_barrier();
         }
       }
       if (lane < nb) {
