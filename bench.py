@@ -522,6 +522,9 @@ def main():
     ap.add_argument("--operator-compare", action="store_true",
                     help="only the matrix-core vs node-row operator comparison (eager launches), one JSON line")
     ap.add_argument("--no-operator-compare", action="store_true")
+    ap.add_argument("--counter-pass", type=int, default=0,
+                    help="only N mixed solves at the calibrated tolerance (1e-7) with every kernel launched "
+                         "eagerly, no captured graphs: the workload for hardware-counter runs")
     args = ap.parse_args()
     if args.submodel_batch:
         submodel_batch_child(args.submodel_batch)
@@ -543,6 +546,19 @@ def main():
     steps = [hs.step_for_spec(spec, Ta=spec["Ta"], Tb=temps[k % len(temps)]) for k in range(max(args.steps, 1))]
     if args.operator_compare:
         print(json.dumps({"operator_compare": operator_compare(hs, mesh, steps[0], 1e-7)}), flush=True)
+        return
+    if args.counter_pass:
+        m = hs.Model(mesh, variant="hip")
+        m.set_kernel_timing(True)  # eager launches, each bracketed by events
+        for k in range(args.counter_pass):
+            rec = m.solve_step_device(steps[k % len(steps)], du.data_ptr(), tol=1e-7, maxit=2000,
+                                      precision=hs.PREC_MIXED, precond=hs.PC_MG)
+            log(f"counter pass {k}: iters {rec['iters']}")
+        # a plain 256 MiB device copy: the reference point for scaling hardware byte counters
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda").fill_(1.0)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"counter_pass": args.counter_pass, "kernel_classes": m.kernel_times()}), flush=True)
         return
     model = hs.Model(mesh, variant="hip")
     bga = meshspec.bga_sites(**spec["bga"])
