@@ -522,6 +522,10 @@ def main():
     ap.add_argument("--operator-compare", action="store_true",
                     help="only the matrix-core vs node-row operator comparison (eager launches), one JSON line")
     ap.add_argument("--no-operator-compare", action="store_true")
+    ap.add_argument("--smoother", default="coarse-jacobi", choices=["coarse-jacobi", "gs"],
+                    help="multigrid smoother: damped column Jacobi from the third shell level down (one launch "
+                         "per sweep), or the reference's colour Gauss-Seidel on every level; the line also "
+                         "times the other one, one solve at a time")
     ap.add_argument("--counter-pass", type=int, default=0,
                     help="only N mixed solves at the calibrated tolerance (1e-7) with every kernel launched "
                          "eagerly, no captured graphs: the workload for hardware-counter runs")
@@ -552,7 +556,9 @@ def main():
         m.set_kernel_timing(True)  # eager launches, each bracketed by events
         for k in range(args.counter_pass):
             rec = m.solve_step_device(steps[k % len(steps)], du.data_ptr(), tol=1e-7, maxit=2000,
-                                      precision=hs.PREC_MIXED, precond=hs.PC_MG)
+                                      precision=hs.PREC_MIXED, precond=hs.PC_MG,
+                                      smoother=hs.SMOOTH_COARSE_JACOBI if args.smoother == "coarse-jacobi"
+                                      else hs.SMOOTH_GS)
             log(f"counter pass {k}: iters {rec['iters']}")
         # a plain 256 MiB device copy: the reference point for scaling hardware byte counters
         src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -564,9 +570,12 @@ def main():
     bga = meshspec.bga_sites(**spec["bga"])
     tols = {hs.PREC_MIXED: args.tol, hs.PREC_FP64: args.tol}
 
-    def solve(k, prec, tol=None):
+    SM = hs.SMOOTH_COARSE_JACOBI if args.smoother == "coarse-jacobi" else hs.SMOOTH_GS
+
+    def solve(k, prec, tol=None, smoother=None):
         return model.solve_step_device(steps[k % len(steps)], du.data_ptr(), tol=tols[prec] if tol is None else tol,
-                                       maxit=2000, precision=prec, precond=hs.PC_MG, sweeps=args.sweeps)
+                                       maxit=2000, precision=prec, precond=hs.PC_MG, sweeps=args.sweeps,
+                                       smoother=SM if smoother is None else smoother)
 
     for w in range(args.warmup):
         rec = solve(w, hs.PREC_MIXED)
@@ -608,7 +617,7 @@ def main():
     def solve_on(mi, k):
         return models_k[mi].solve_step_device(steps[k % len(steps)], dus_k[mi].data_ptr(), tol=tols[hs.PREC_MIXED],
                                               maxit=2000, precision=hs.PREC_MIXED, precond=hs.PC_MG,
-                                              sweeps=args.sweeps)
+                                              sweeps=args.sweeps, smoother=SM)
 
     for mi in range(1, K):
         solve_on(mi, 0)  # warm-up: graph capture of each instance
@@ -630,6 +639,14 @@ def main():
     torch.cuda.synchronize()
     ktimes = model.kernel_times()
     model.set_kernel_timing(False)
+    # the other smoother on the same steps, one solve at a time (its own captured graphs)
+    OTHER = hs.SMOOTH_GS if SM == hs.SMOOTH_COARSE_JACOBI else hs.SMOOTH_COARSE_JACOBI
+    solve(0, hs.PREC_MIXED, smoother=OTHER)
+    torch.cuda.synchronize()
+    to0 = time.perf_counter()
+    recs_other = [solve(k, hs.PREC_MIXED, smoother=OTHER) for k in range(args.steps)]
+    torch.cuda.synchronize()
+    to1 = time.perf_counter()
     # a plain 256 MiB device copy: the reference point for scaling hardware byte counters
     src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
@@ -641,7 +658,8 @@ def main():
     elapsed_conc = tc1 - tc0
     value = sum(mesh.ndof * r["iters"] for r in recs_conc) / elapsed_conc
     log(f"{K} solves in flight: {1e3 * elapsed_conc / max(args.steps, 1):.1f} ms per solve; one at a time "
-        f"{1e3 * elapsed / max(args.steps, 1):.1f} ms")
+        f"{1e3 * elapsed / max(args.steps, 1):.1f} ms ({args.smoother}); other smoother one at a time "
+        f"{1e3 * (to1 - to0) / max(args.steps, 1):.1f} ms, iters {recs_other[0]['iters']}")
     for k, r in enumerate(recs):
         log(f"step {k} T={temps[k % len(temps)]}: iters {r['iters']} relres {r['relres']:.2e} "
             f"asm {r['ms_assemble']:.1f} setup {r['ms_setup']:.1f} pcg {r['ms_pcg']:.1f} ms")
@@ -679,6 +697,14 @@ def main():
         "iters_mixed": [r["iters"] for r in recs],
         "iters_mixed_in_flight": [r["iters"] for r in recs_conc],
         "setup_ms_mixed": float(np.mean([r["ms_assemble"] + r["ms_setup"] for r in recs])),
+        "smoother": args.smoother,
+        "other_smoother": {"smoother": "gs" if OTHER == hs.SMOOTH_GS else "coarse-jacobi",
+                           "solve_s_mixed": (to1 - to0) / max(args.steps, 1),
+                           "iters_mixed": [r["iters"] for r in recs_other],
+                           "value_one_at_a_time": sum(mesh.ndof * r["iters"] for r in recs_other) / (to1 - to0),
+                           "note": "gs is the reference's colour Gauss-Seidel on every level; coarse-jacobi smooths "
+                                   "the shell levels from the third down with four damped column-Jacobi sweeps, "
+                                   "one launch each"},
         "roofline": roof,
         "composite_roofline": composite_roofline(ktimes, min(args.steps, 2), int(np.mean([r["iters"] for r in recs])),
                                                  float(np.mean([r["ms_pcg"] for r in recs])), mesh),

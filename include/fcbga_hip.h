@@ -93,7 +93,18 @@ typedef struct fcbga_solve_opts {
                          size of the basis of previous solutions the load is projected onto, and that a
                          converged solution joins (the reference's default is 20); 0 = cold, the basis is
                          neither used nor extended */
+  int32_t smoother;   /* FCBGA_SMOOTH_*: the multigrid's smoother on the coarse shell levels */
 } fcbga_solve_opts;
+
+/* Smoothers of the multigrid V-cycle.  GS: the reference's column Gauss-Seidel
+ * in four colours on every level (MultigridPC, src/core/10-solver.js:512-555,
+ * 620-715).  COARSE_JACOBI: the same on the fine level and the first two shell
+ * levels; from the third shell level down, 2 x sweeps damped (0.6) column
+ * Jacobi sweeps, each one launch over all columns instead of four colour
+ * launches (no counterpart in the reference; same solution to the tolerance,
+ * iteration counts within one of GS on the bundled workloads). */
+#define FCBGA_SMOOTH_GS 0
+#define FCBGA_SMOOTH_COARSE_JACOBI 1
 
 /* Solve record of LinearSolver.solve (src/core/10-solver.js:988-993) plus timings. */
 typedef struct fcbga_solve_stats {

@@ -309,6 +309,7 @@ struct fcbga_model {
   double solveT0 = 0;
   int32_t itersBase = 0;
   int nsweeps = 2;     // pre- and post-smoothing sweeps per level
+  int jacobiFrom = 1 << 20;  // first shell level (index into lv) smoothed by column Jacobi
   bool useGraphs = true;
 #if FCBGA_GPU
   std::map<int, hipGraphExec_t> graphs;
@@ -1955,6 +1956,80 @@ __global__ void __launch_bounds__(64) k_shell_sweep(int ci, int cj, int ny, cons
   }
 }
 
+// One damped column-Jacobi sweep over every column of a shell level:
+// xout = xin + omega * (column inverse) (b - A xin) on each column's rows, all
+// columns at once from xin (one launch per sweep, no colours).  The column
+// work is shell_column_wave's with the iterate split into input and output.
+template <typename T, int U>
+__global__ void __launch_bounds__(64) k_shell_jacobi(const int32_t *__restrict__ desc,
+                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                     const T *__restrict__ val, const T *__restrict__ inv,
+                                                     const T *__restrict__ b, const T *__restrict__ xin,
+                                                     T *__restrict__ xout, T omega) {
+  __shared__ T rs[SHELL_MAXM];
+  const int32_t c = blockIdx.x;
+  int32_t dsc[SHELL_DESC];
+  {
+    const int4 *dp = reinterpret_cast<const int4 *>(desc + (int64_t)SHELL_DESC * c);
+#pragma unroll
+    for (int v = 0; v < SHELL_DESC / 4; v++) {
+      const int4 q = dp[v];
+      dsc[4 * v] = q.x; dsc[4 * v + 1] = q.y; dsc[4 * v + 2] = q.z; dsc[4 * v + 3] = q.w;
+    }
+  }
+  const int lane = threadIdx.x;
+  const int32_t n0 = dsc[0];
+  const int m = 6 * dsc[1];
+  if (m == 0) return;
+  int64_t cfo;
+  __builtin_memcpy(&cfo, dsc + 2, sizeof(cfo));
+  const T *A = inv + cfo;
+  if (m <= 32) {
+    const int S = 64 / m;
+    const int i = lane % m, grp = lane / m;
+    const int r = i / 6;
+    int32_t k0 = dsc[4], k1 = dsc[5];
+#pragma unroll
+    for (int q = 1; q < SHELL_DESC_ROWS; q++)
+      if (r == q) { k0 = dsc[4 + q]; k1 = dsc[5 + q]; }
+    T ainv[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int j = grp + u * S;
+      ainv[u] = (grp < S && j < m) ? A[(int64_t)j * m + i] : T(0);
+    }
+    const T bl = b[6 * (int64_t)n0 + i];
+    T acc = 0;
+    if (grp < S) row_sub_blocks<T, U>(acc, k0 + grp, k1, S, i % 6, col, val, xin);
+    T rr = bl;
+    for (int g = 0; g < S; g++) rr += __shfl(acc, g * m + i, 64);
+    T dx = 0;
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int j = grp + u * S;
+      const T rj = __shfl(rr, j < m ? j : 0, 64);
+      if (grp < S && j < m) dx += ainv[u] * rj;
+    }
+    T d = 0;
+    for (int g = 0; g < S; g++) d += __shfl(dx, g * m + i, 64);
+    if (lane < m) xout[6 * (int64_t)n0 + lane] = xin[6 * (int64_t)n0 + lane] + omega * d;
+    return;
+  }
+  // larger columns: one lane per row
+  for (int i = lane; i < m; i += 64) {
+    const int32_t n = n0 + i / 6;
+    T acc = b[6 * (int64_t)n0 + i];
+    row_sub_blocks<T>(acc, rp[n], rp[n + 1], 1, i % 6, col, val, xin);
+    rs[i] = acc;
+  }
+  __syncthreads();
+  for (int i = lane; i < m; i += 64) {
+    T acc = 0;
+    for (int j = 0; j < m; j++) acc += A[(int64_t)j * m + i] * rs[j];
+    xout[6 * (int64_t)n0 + i] = xin[6 * (int64_t)n0 + i] + omega * acc;
+  }
+}
+
 #endif
 
 // host form: the same substitutions column by column
@@ -2418,6 +2493,53 @@ void shell_sweep(DevShell &D, int color, be::stream_t s) {
 #endif
 }
 
+// The coarse-level smoother of FCBGA_SMOOTH_COARSE_JACOBI: 2 x nsweeps damped
+// column-Jacobi sweeps, the iterate alternating between x and the level's
+// residual vector (free until the level's residual is formed), ending in x.
+constexpr double JACOBI_OMEGA = 0.6;
+
+template <typename T>
+void jacobi_smooth(DevShell &D, int nsweeps, be::stream_t s) {
+  T *x = LevelVec<T>::x(D), *t = LevelVec<T>::r(D);
+  const int32_t *desc = D.colDesc.p, *rp = D.rowPtr.p, *col = D.col.p;
+  const T *val = LevelVec<T>::val(D), *inv = LevelVec<T>::cf(D), *b = LevelVec<T>::b(D);
+  for (int q = 0; q < 2 * nsweeps; q++) {
+    const T *xi = (q & 1) ? t : x;
+    T *xo = (q & 1) ? x : t;
+#if FCBGA_GPU
+    hipLaunchKernelGGL((k_shell_jacobi<T, 4>), dim3(D.ncol), dim3(64), 0, s, desc, rp, col, val, inv, b, xi, xo,
+                       (T)JACOBI_OMEGA);
+    BE_CHECK(hipGetLastError());
+#else
+    (void)s;
+    T r[SHELL_MAXM];
+    for (int32_t c = 0; c < D.ncol; c++) {
+      const int32_t *dsc = desc + (int64_t)SHELL_DESC * c;
+      const int32_t n0 = dsc[0], na = dsc[1];
+      int64_t cfo;
+      std::memcpy(&cfo, dsc + 2, sizeof(cfo));
+      const int m = 6 * na;
+      for (int i = 0; i < m; i++) {
+        const int32_t n = n0 + i / 6;
+        const int p = i % 6;
+        T acc = b[6 * (int64_t)n0 + i];
+        for (int32_t k = rp[n]; k < rp[n + 1]; k++) {
+          const T *v = val + 36 * (int64_t)k + 6 * p, *xj = xi + 6 * (int64_t)col[k];
+          acc -= v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2] + v[3] * xj[3] + v[4] * xj[4] + v[5] * xj[5];
+        }
+        r[i] = acc;
+      }
+      const T *A = inv + cfo;
+      for (int i = 0; i < m; i++) {
+        T acc = 0;
+        for (int j = 0; j < m; j++) acc += A[(int64_t)j * m + i] * r[j];
+        xo[6 * (int64_t)n0 + i] = xi[6 * (int64_t)n0 + i] + (T)JACOBI_OMEGA * acc;
+      }
+    }
+#endif
+  }
+}
+
 template <typename T>
 void shell_residual(DevShell &D, be::stream_t s) {
   const int32_t *rp = D.rowPtr.p, *col = D.col.p;
@@ -2487,8 +2609,11 @@ void shell_cycle(fcbga_model *M, int l) {
   // x is zero here: fine_cycle clears it with the fine x when the cycle starts
   {
     KScope ks(M, l == 0 ? FCBGA_KCLASS_SHELL1_SWEEP : FCBGA_KCLASS_SHELL_SWEEP);
-    for (int sw = 0; sw < M->nsweeps; sw++)
-      for (int c = 0; c < 4; c++) shell_sweep<T>(D, c, s);
+    if (l >= M->jacobiFrom)
+      jacobi_smooth<T>(D, M->nsweeps, s);
+    else
+      for (int sw = 0; sw < M->nsweeps; sw++)
+        for (int c = 0; c < 4; c++) shell_sweep<T>(D, c, s);
   }
   shell_residual<T>(D, s);
   DevShell &C = *M->lv[l + 1];
@@ -2537,6 +2662,10 @@ void shell_cycle(fcbga_model *M, int l) {
     });
   }
   KScope ks(M, l == 0 ? FCBGA_KCLASS_SHELL1_SWEEP : FCBGA_KCLASS_SHELL_SWEEP);
+  if (l >= M->jacobiFrom) {
+    jacobi_smooth<T>(D, M->nsweeps, s);
+    return;
+  }
   for (int sw = 0; sw < M->nsweeps; sw++)
     for (int c = 3; c >= 0; c--) shell_sweep<T>(D, c, s);
 }
@@ -3420,13 +3549,14 @@ void store_solution(fcbga_model *M, int nmax) {
 }
 
 fcbga_solve_opts check_opts(const fcbga_solve_opts *op) {
-  fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_FP64, FCBGA_PC_MG, 0};
+  fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_FP64, FCBGA_PC_MG, 0, 0, FCBGA_SMOOTH_GS};
   if (op) o = *op;
   if (!(o.tol > 0) || o.maxit < 1) throw be::Error("tol must be > 0 and maxit >= 1");
   if (o.precision != FCBGA_PREC_FP64 && o.precision != FCBGA_PREC_MIXED) throw be::Error("unknown precision");
   if (o.precond < FCBGA_PC_MG || o.precond > FCBGA_PC_IC) throw be::Error("unknown preconditioner");
   if (o.sweeps < 0 || o.sweeps > 8) throw be::Error("sweeps must be 0 (default 2) .. 8");
   if (o.warm < 0 || o.warm > 64) throw be::Error("warm must be 0 .. 64");
+  if (o.smoother != FCBGA_SMOOTH_GS && o.smoother != FCBGA_SMOOTH_COARSE_JACOBI) throw be::Error("unknown smoother");
   return o;
 }
 
@@ -3454,6 +3584,8 @@ double build_pc(fcbga_model *M, int kind, fcbga_solve_stats &S) {
 void prepare_step(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opts &o, fcbga_solve_stats &S) {
   check_step(M, st);
   M->nsweeps = o.sweeps ? o.sweeps : 2;
+  // column Jacobi from the third shell level (lv[2]) on
+  M->jacobiFrom = o.smoother == FCBGA_SMOOTH_COARSE_JACOBI ? 2 : 1 << 20;
   S.ndof = 3 * (int64_t)M->nn;
   S.ntypes = M->ntypes;
   S.levels = 1;
@@ -3504,9 +3636,9 @@ int pcg_iterate(fcbga_model *M, const fcbga_solve_opts &o, double bnorm, double 
   // the FP32 cycles need the flexible form; IC(0) runs in FP64 whatever the precision
   const bool flex = o.precision == FCBGA_PREC_MIXED && o.precond != FCBGA_PC_IC;
   // every host-side launch choice baked into a captured iteration is part of
-  // the key: precision, preconditioner, sweeps and the element-table form
-  // (compact mirror-symmetric or full), which assemble_step decides per step
-  const int gkey = (M->symTables ? 4096 : 0) + o.precision * 64 + o.precond * 8 + M->nsweeps;
+  // the key: smoother, precision, preconditioner, sweeps and the element-table
+  // form (compact mirror-symmetric or full), which assemble_step decides per step
+  const int gkey = (o.smoother ? 8192 : 0) + (M->symTables ? 4096 : 0) + o.precision * 64 + o.precond * 8 + M->nsweeps;
   int it;
   for (it = 1; it <= o.maxit; it++) {
     run_iteration(M, o, flex, gkey);
@@ -4989,7 +5121,7 @@ int fcbga_vp_residual(fcbga_vp *v, const fcbga_step *elastic, double T, double d
     fcbga_model *M = v->M;
     be::stream_t s = M->stream;
     vp_elements(v, T, dt, 1);
-    fcbga_solve_opts o{1e-8, 1, FCBGA_PREC_FP64, FCBGA_PC_JACOBI, 0, 0};
+    fcbga_solve_opts o{1e-8, 1, FCBGA_PREC_FP64, FCBGA_PC_JACOBI, 0, 0, FCBGA_SMOOTH_GS};
     fcbga_solve_stats S{};
     vp_prepare(v, elastic, o, S);
     vp_residual(v);

@@ -23,6 +23,7 @@ if os.environ.get("FCBGA_HOSTEMU_SANITIZE") == "1":
     LIBS["hostemu"] = os.path.join(HERE, "libfcbga_hostemu_asan.so")
 
 PREC_FP64, PREC_MIXED = 0, 1
+SMOOTH_GS, SMOOTH_COARSE_JACOBI = 0, 1  # FCBGA_SMOOTH_*
 PC_MG, PC_COLUMN, PC_JACOBI, PC_IC = 0, 1, 2, 3
 PC_NAMES = {PC_MG: "mg", PC_COLUMN: "column", PC_JACOBI: "jacobi", PC_IC: "ic"}
 
@@ -48,7 +49,7 @@ class FcbgaStep(C.Structure):
 
 class FcbgaSolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxit", C.c_int32), ("precision", C.c_int32), ("precond", C.c_int32),
-                ("sweeps", C.c_int32), ("warm", C.c_int32)]
+                ("sweeps", C.c_int32), ("warm", C.c_int32), ("smoother", C.c_int32)]
 
 
 class FcbgaSolveStats(C.Structure):
@@ -326,9 +327,10 @@ class Model:
         self.close()
 
     @staticmethod
-    def _opts(tol, maxit, precision, precond, sweeps=0, warm=0):
+    def _opts(tol, maxit, precision, precond, sweeps=0, warm=0, smoother=0):
         o = FcbgaSolveOpts()
         o.tol, o.maxit, o.precision, o.precond, o.sweeps, o.warm = tol, maxit, precision, precond, sweeps, warm
+        o.smoother = smoother
         return o
 
     def warm_reset(self):
@@ -336,12 +338,13 @@ class Model:
         self.L.fcbga_warm_reset(self.h)
 
     def solve_step(self, step: Step, tol=1e-8, maxit=2000, precision=PREC_FP64, precond=PC_MG, check=True, sweeps=0,
-                   warm=0):
+                   warm=0, smoother=SMOOTH_GS):
         """du (3*nn, reference node order) and the solve record; warm > 0
-        projects the load onto (and keeps) up to `warm` previous solutions."""
+        projects the load onto (and keeps) up to `warm` previous solutions;
+        smoother: SMOOTH_GS (the reference's) or SMOOTH_COARSE_JACOBI."""
         du = np.zeros(self.ndof)
         st = FcbgaSolveStats()
-        o = self._opts(tol, maxit, precision, precond, sweeps, warm)
+        o = self._opts(tol, maxit, precision, precond, sweeps, warm, smoother)
         rc = self.L.fcbga_step_solve(self.h, C.byref(step.c), C.byref(o), du.ctypes.data_as(C.c_void_p),
                                      C.byref(st))
         rec = _record(st)
@@ -367,9 +370,9 @@ class Model:
             raise SolverError(last_error(self.L))
 
     def solve_step_device(self, step: Step, du_dev_ptr: int, tol=1e-8, maxit=2000, precision=PREC_FP64,
-                          precond=PC_MG, sweeps=0, warm=0):
+                          precond=PC_MG, sweeps=0, warm=0, smoother=SMOOTH_GS):
         st = FcbgaSolveStats()
-        o = self._opts(tol, maxit, precision, precond, sweeps, warm)
+        o = self._opts(tol, maxit, precision, precond, sweeps, warm, smoother)
         rc = self.L.fcbga_step_solve_device(self.h, C.byref(step.c), C.byref(o), C.c_void_p(du_dev_ptr),
                                             C.byref(st))
         if rc != 0:
@@ -379,10 +382,11 @@ class Model:
     def path_reset(self):
         self.L.fcbga_path_reset(self.h)
 
-    def path_solve(self, k, step: Step, tol=1e-8, maxit=2000, precision=PREC_MIXED, precond=PC_MG, warm=20):
+    def path_solve(self, k, step: Step, tol=1e-8, maxit=2000, precision=PREC_MIXED, precond=PC_MG, warm=20,
+                   smoother=SMOOTH_GS):
         """Solve step k of a thermal path, keeping its increment on the device."""
         st = FcbgaSolveStats()
-        o = self._opts(tol, maxit, precision, precond, 0, warm)
+        o = self._opts(tol, maxit, precision, precond, 0, warm, smoother)
         rc = self.L.fcbga_path_solve(self.h, int(k), C.byref(step.c), C.byref(o), C.byref(st))
         rec = _record(st)
         if rc != 0 and rc != -3:

@@ -224,8 +224,8 @@ const char *pc_name(int pc) {
   return "?";
 }
 
-// opts {tol, maxit, sweeps, warm, precision: 'mixed'|'fp64', precond: 'mg'|'ic'|'column'|'jacobi'};
-// false for an unknown precond name
+// opts {tol, maxit, sweeps, warm, precision: 'mixed'|'fp64', precond: 'mg'|'ic'|'column'|'jacobi',
+//       smoother: 'gs'|'coarse-jacobi'}; false for an unknown precond or smoother name
 bool read_opts(napi_env env, napi_value v, fcbga_solve_opts *o) {
   napi_valuetype t;
   napi_typeof(env, v, &t);
@@ -245,6 +245,11 @@ bool read_opts(napi_env env, napi_value v, fcbga_solve_opts *o) {
       if (strcmp(buf, pc_name(q)) == 0) pc = q;
     if (pc < 0) return false;
     o->precond = pc;
+  }
+  if (get_prop(env, v, "smoother", &x) && napi_get_value_string_utf8(env, x, buf, sizeof buf, &len) == napi_ok) {
+    if (strcmp(buf, "gs") == 0) o->smoother = FCBGA_SMOOTH_GS;
+    else if (strcmp(buf, "coarse-jacobi") == 0) o->smoother = FCBGA_SMOOTH_COARSE_JACOBI;
+    else return false;
   }
   return true;
 }
@@ -346,8 +351,8 @@ napi_value SolveStep(napi_env env, napi_callback_info info) {
   fcbga_step st;
   std::string err;
   if (!read_step(env, argv[1], b, &st, &err)) return throw_msg(env, "solveStep: " + err);
-  fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
-  if (argc >= 3 && !read_opts(env, argv[2], &o)) return throw_msg(env, "solveStep: unknown opts.precond");
+  fcbga_solve_opts o{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0, 0, FCBGA_SMOOTH_GS};
+  if (argc >= 3 && !read_opts(env, argv[2], &o)) return throw_msg(env, "solveStep: unknown opts.precond or opts.smoother");
   const int64_t ndof = fcbga_model_ndof(b->m);
   napi_value ab;
   void *udata = nullptr;
@@ -556,8 +561,8 @@ napi_value AdjointForward(napi_env env, napi_callback_info info) {
     obj.n_vm = (int32_t)n;
   }
   get_double(env, argv[2], "p", &obj.p);
-  fcbga_solve_opts o{1e-10, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
-  if (argc >= 4 && !read_opts(env, argv[3], &o)) return throw_msg(env, "adjointForward: unknown opts.precond");
+  fcbga_solve_opts o{1e-10, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0, 0, FCBGA_SMOOTH_GS};
+  if (argc >= 4 && !read_opts(env, argv[3], &o)) return throw_msg(env, "adjointForward: unknown opts.precond or opts.smoother");
   const int64_t ndof = fcbga_model_ndof(b->m);
   double *J, *u;
   napi_value jv = make_f64(env, 2, &J), uv = make_f64(env, (size_t)ndof, &u);
@@ -872,8 +877,8 @@ napi_value PathCall(napi_env env, napi_callback_info info) {
     fcbga_step st;
     std::string err;
     if (!read_step(env, argv[3], b, &st, &err)) return throw_msg(env, err);
-    fcbga_solve_opts so{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0};
-    if (argc >= 5 && !read_opts(env, argv[4], &so)) return throw_msg(env, "pathCall solve: unknown opts.precond");
+    fcbga_solve_opts so{1e-8, 2000, FCBGA_PREC_MIXED, FCBGA_PC_MG, 0, 0, FCBGA_SMOOTH_GS};
+    if (argc >= 5 && !read_opts(env, argv[4], &so)) return throw_msg(env, "pathCall solve: unknown opts.precond or opts.smoother");
     fcbga_solve_stats S;
     memset(&S, 0, sizeof S);
     int rc;

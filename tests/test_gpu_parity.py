@@ -99,6 +99,36 @@ def test_cfg2_full_size_vs_reference(lib, prec):
     assert np.abs(u2 - ratio * u).max() <= 1e-6 * np.abs(u2).max()
 
 
+@pytest.mark.parametrize("prec", [hs.PREC_MIXED, hs.PREC_FP64])
+def test_coarse_jacobi_smoother_vs_reference(lib, prec):
+    """FCBGA_SMOOTH_COARSE_JACOBI on the device (damped column Jacobi from the
+    third shell level of eight down): the lidded mesh's solution against the
+    reference's to 1e-6 and its warpage to 0.1 %, within two iterations of the
+    colour Gauss-Seidel V-cycle; then the bare-die mesh against its
+    reference solution."""
+    spec = cases.cfg2_spec()
+    m = meshspec.build_mesh(spec)
+    meta = golden_meta("cfg2")
+    model = hs.Model(m, variant="hip")
+    step = hs.step_for_spec(spec)
+    _, rgs = model.solve_step(step, tol=1e-10, precision=prec)
+    u, rec = model.solve_step(step, tol=1e-10, precision=prec, smoother=hs.SMOOTH_COARSE_JACOBI)
+    assert rec["converged"] and rec["levels"] == rgs["levels"] == 8
+    assert rec["iters"] <= rgs["iters"] + 2, (rec["iters"], rgs["iters"])
+    rows = golden_f64("cfg2_usample.f64.gz").reshape(-1, 4)
+    us = u.reshape(-1, 3)[rows[:, 0].astype(np.int64)]
+    assert np.abs(us - rows[:, 1:]).max() <= 1e-6 * np.abs(rows[:, 1:]).max()
+    w = post.warpage_jeita(m, u, meshspec.bga_sites(**spec["bga"]))
+    assert abs(w["signed"] - meta["warpage"]["signed"]) <= 1e-3 * abs(meta["warpage"]["signed"])
+    spec1 = golden_spec("cfg1")
+    m1 = meshspec.build_mesh(spec1)
+    u1, r1 = hs.Model(m1, variant="hip").solve_step(hs.step_for_spec(spec1), tol=1e-10, precision=prec,
+                                                    smoother=hs.SMOOTH_COARSE_JACOBI)
+    uref = golden_f64("cfg1_u.f64.gz")
+    assert r1["converged"] and r1["iters"] <= golden_meta("cfg1")["iters"] + 5
+    assert np.abs(u1 - uref).max() <= 1e-6 * np.abs(uref).max()
+
+
 @pytest.mark.parametrize("prec", [hs.PREC_FP64, hs.PREC_MIXED])
 def test_full_element_tables_vs_oracle(oracle, lib, monkeypatch, prec):
     """The full 576-value element tables (the path for elements without mirror

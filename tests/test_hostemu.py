@@ -151,3 +151,23 @@ def test_progress_callback_and_cancel():
     M.set_progress(None)
     u2, rec2 = M.solve_step(st, tol=1e-10)
     assert rec2["converged"] and np.array_equal(u2, u)
+
+
+@pytest.mark.parametrize("name", ["cfg1"])
+def test_coarse_jacobi_smoother_reaches_the_same_solution(name):
+    """FCBGA_SMOOTH_COARSE_JACOBI (damped column Jacobi from the third shell
+    level down) against the reference's colour Gauss-Seidel on every level:
+    the same FP64 solution to the tolerance, within two iterations of it, and
+    a symmetric preconditioner (the FP64 PCG needs one: its iterations would
+    stall otherwise)."""
+    spec, mesh, model, step = _model(name)
+    u_gs, r_gs = model.solve_step(step, tol=1e-11, precision=hs.PREC_FP64)
+    u_j, r_j = model.solve_step(step, tol=1e-11, precision=hs.PREC_FP64, smoother=hs.SMOOTH_COARSE_JACOBI)
+    assert r_gs["converged"] and r_j["converged"]
+    # solid level, three shell levels above the coarsest: the third takes Jacobi
+    assert r_j["levels"] == r_gs["levels"] >= 5
+    assert abs(r_j["iters"] - r_gs["iters"]) <= 2, (r_j["iters"], r_gs["iters"])
+    assert np.abs(u_j - u_gs).max() <= 1e-8 * np.abs(u_gs).max()
+    assert not np.array_equal(u_j, u_gs)  # a different preconditioner ran
+    with pytest.raises(hs.SolverError):
+        model.solve_step(step, tol=1e-8, smoother=7)
