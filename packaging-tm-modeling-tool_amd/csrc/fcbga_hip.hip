@@ -3237,7 +3237,9 @@ double dotd(fcbga_model *M, const double *a, const double *b, int slot) {
   return v;
 }
 
-void precondition(fcbga_model *M, int precision, int precond) {
+// r32_ready: the mixed-precision cycle's FP32 copy of r (M->b32v) is already
+// written (by the PCG update that formed r)
+void precondition(fcbga_model *M, int precision, int precond, bool r32_ready = false) {
   // z = M^-1 r, then z = 0 on constrained / absent DOFs
   const int64_t n3 = 3 * M->g.G;
   be::stream_t s = M->stream;
@@ -3261,9 +3263,10 @@ void precondition(fcbga_model *M, int precision, int precond) {
     fine_cycle<double>(M, r, z, M->tmp64.p, M->dx64.p, coarse);
   } else {
     float *b32 = M->b32v.p, *x32 = M->x32v.p;
-    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { b32[i] = (float)r[i]; });
+    if (!r32_ready) be::pfor(n3, s, [=] __host__ __device__(int64_t i) { b32[i] = (float)r[i]; });
     fine_cycle<float>(M, b32, x32, M->r32v.p, M->dx32.p, coarse);
-    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { z[i] = (double)x32[i]; });
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { z[i] = mask[i] ? (double)x32[i] : 0.0; });
+    return;
   }
   be::pfor(n3, s, [=] __host__ __device__(int64_t i) { if (!mask[i]) z[i] = 0.0; });
 }
@@ -3279,7 +3282,8 @@ __global__ void __launch_bounds__(be::RED_BLOCK) k_cg_update(int64_t n, const do
                                                             double *__restrict__ x, double *__restrict__ r,
                                                             const double *__restrict__ p,
                                                             const double *__restrict__ q,
-                                                            const double *__restrict__ z, double *__restrict__ part) {
+                                                            const double *__restrict__ z, double *__restrict__ part,
+                                                            float *__restrict__ r32) {
   __shared__ double s1[be::RED_BLOCK], s2[be::RED_BLOCK];
   const double a = sc[SC_ALPHA];
   double rr = 0, rz = 0;
@@ -3289,6 +3293,7 @@ __global__ void __launch_bounds__(be::RED_BLOCK) k_cg_update(int64_t n, const do
     const double ri = r[i];
     rr += ri * ri;
     if (z) rz += ri * z[i];
+    if (r32) r32[i] = (float)ri;
   }
   s1[threadIdx.x] = rr;
   s2[threadIdx.x] = rz;
@@ -3307,10 +3312,11 @@ __global__ void __launch_bounds__(be::RED_BLOCK) k_cg_update(int64_t n, const do
 }
 #endif
 
+// r32 (optional): the updated r rounded to FP32, the mixed-precision cycle's input
 void cg_update(int64_t n3, double *sc, double *x, double *r, const double *p, const double *q, const double *z,
-               double *part, be::stream_t s) {
+               double *part, float *r32, be::stream_t s) {
 #if FCBGA_GPU
-  hipLaunchKernelGGL(k_cg_update, dim3(be::RED_GRID), dim3(be::RED_BLOCK), 0, s, n3, sc, x, r, p, q, z, part);
+  hipLaunchKernelGGL(k_cg_update, dim3(be::RED_GRID), dim3(be::RED_BLOCK), 0, s, n3, sc, x, r, p, q, z, part, r32);
   BE_CHECK(hipGetLastError());
   hipLaunchKernelGGL(be::k_sum_partials, dim3(1), dim3(be::RED_BLOCK), 0, s, 1, be::RED_GRID, part, sc + SC_RR);
   BE_CHECK(hipGetLastError());
@@ -3328,6 +3334,7 @@ void cg_update(int64_t n3, double *sc, double *x, double *r, const double *p, co
     r[i] -= a * q[i];
     rr += r[i] * r[i];
     if (z) rz += r[i] * z[i];
+    if (r32) r32[i] = (float)r[i];
   }
   sc[SC_RR] = rr;
   if (z) sc[SC_RZO] = rz;
@@ -3358,8 +3365,10 @@ void iteration_body(fcbga_model *M, const fcbga_solve_opts &o, bool flex) {
   });
   // x += alpha p, r -= alpha q with r.r and (flexible) r.z_old of the new r
   // in the same pass, before the preconditioner overwrites z
-  cg_update(n3, sc, x, r, p, q, flex ? z : nullptr, part, s);
-  precondition(M, o.precision, o.precond);
+  // the mixed multigrid cycle's FP32 input is written by the same pass
+  const bool r32 = o.precision == FCBGA_PREC_MIXED && (o.precond == FCBGA_PC_MG || o.precond == FCBGA_PC_COLUMN);
+  cg_update(n3, sc, x, r, p, q, flex ? z : nullptr, part, r32 ? M->b32v.p : nullptr, s);
+  precondition(M, o.precision, o.precond, r32);
   be::dot_async(n3, r, z, part, sc + SC_RZN, s);
   be::pfor(1, s, [=] __host__ __device__(int64_t) {
     double rz = sc[SC_RZ], rzn = sc[SC_RZN];
