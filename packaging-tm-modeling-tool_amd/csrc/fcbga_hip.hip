@@ -937,9 +937,9 @@ void fine_residual(const FineOps<T> &o, const T *b, const T *x, T *res, be::stre
     int j = (int)(cj % g.ny), i = (int)(cj / g.ny);
     T out[3];
     if (sym)
-      fe::node_apply_sym<T>(g, i, j, k, ct, cs, fe::SymTabs<T>{R, Kc, own0}, w, x, out);
+      fe::node_apply_sym<T, true>(g, i, j, k, ct, cs, fe::SymTabs<T>{R, Kc, own0}, w, x, out);
     else
-      fe::node_apply<T, T>(g, i, j, k, ct, cs, Kc, w, x, out);
+      fe::node_apply<T, T, true>(g, i, j, k, ct, cs, Kc, w, x, out);
     for (int p = 0; p < 3; p++) res[3 * gi + p] = b[3 * gi + p] - out[p];
   });
 #endif
@@ -1056,7 +1056,7 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
       nb0[((sl * 9 + col) * nzp + (side ? nz + 1 : 0)) * 3 + p] = T(0);
     }
   }
-  if (valid) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, nzp);
+  if (valid) fe::fill_neighbourhood<T, true>(g, i, j, k, w, x, nb, nzp);
   __syncthreads();
   T r[3] = {0, 0, 0};
   T acc[3] = {0, 0, 0};
@@ -1086,7 +1086,7 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
     fe::node_rows_nb<T>(g, i, j, k, ct, cs, Kc, nb, acc, nzp);
   }
   if (valid)
-    for (int p = 0; p < 3; p++) r[p] = bv[p] - (wv[p] != T(0) ? wv[p] * acc[p] : xv[p]);
+    for (int p = 0; p < 3; p++) r[p] = bv[p] - wv[p] * acc[p];
   // forward recurrence y_k = P_k y_{k-1} + Li_k r_k, then the backward one
   // dx_k = Q_k dx_{k+1} + Li_k^T y_k, each as a scan over the 64 lanes in six
   // rounds (the coefficients of lanes past the column are zero; rounds at
@@ -1110,7 +1110,7 @@ __global__ void __launch_bounds__(64) k_fine_col_sweep(fe::Grid g, const int32_t
 #pragma unroll
   for (int s = 1; s < 64; s <<= 1) scan_round<T, false>(A, b, s, s < 32);
   if (valid)
-    for (int p = 0; p < 3; p++) x[3 * gi + p] = xv[p] + b[p];
+    for (int p = 0; p < 3; p++) x[3 * gi + p] = xv[p] + wv[p] * b[p];
 }
 
 // y = A x (mode 0) or y = b - A x (mode 1) over every node column, one 64-lane
@@ -1142,7 +1142,7 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
     const int sl = q / 54, col = (q % 54) / 6, side = (q / 3) % 2, p = q % 3;
     reinterpret_cast<T *>(nb_lds)[((sl * 9 + col) * (nz + 2) + (side ? nz + 1 : 0)) * 3 + p] = T(0);
   }
-  if (valid) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, nz + 2);
+  if (valid) fe::fill_neighbourhood<T, MODE == 1>(g, i, j, k, w, x, nb, nz + 2);
   __syncthreads();
   T acc[3] = {0, 0, 0};
   if (SYM) {
@@ -1170,7 +1170,7 @@ __global__ void __launch_bounds__(64) k_fine_col_op(fe::Grid g, const int32_t *_
   if (!valid) return;
   for (int p = 0; p < 3; p++) {
     const T wi = w[3 * gi + p];
-    const T out = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
+    const T out = wi != T(0) ? wi * acc[p] : (MODE ? T(0) : x[3 * gi + p]);
     y[3 * gi + p] = MODE ? bvec[3 * gi + p] - out : out;
   }
 }
@@ -1218,7 +1218,7 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
     const int i = c / g.ny, j = c % g.ny;
     T nb[9 * fe::NZP * 3];
     for (int q = 0; q < 9 * fe::NZP * 3; q++) nb[q] = T(0);
-    for (int k = 0; k < nz; k++) fe::fill_neighbourhood<T>(g, i, j, k, w, x, nb, fe::NZP);
+    for (int k = 0; k < nz; k++) fe::fill_neighbourhood<T, true>(g, i, j, k, w, x, nb, fe::NZP);
     T upPrev[3] = {0, 0, 0};
     for (int k = 0; k < nz; k++) {
       const int gi = c * nz + k;
@@ -1232,14 +1232,14 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
       }
       for (int p = 0; p < 3; p++) {
         const T wi = w[3 * gi + p];
-        r[3 * gi + p] = b[3 * gi + p] - (wi != T(0) ? wi * acc[p] : x[3 * gi + p]);
+        r[3 * gi + p] = b[3 * gi + p] - wi * acc[p];
       }
     }
   });
   be::pfor(ncols, s, [=] __host__ __device__(int64_t t) {
     int64_t c = cols[t];
     fe::column_solve_scan<T, T>(nz, C, c, r + 3 * c * nz, dx + 3 * c * nz);
-    for (int q = 0; q < 3 * nz; q++) x[3 * c * nz + q] += dx[3 * c * nz + q];
+    for (int q = 0; q < 3 * nz; q++) x[3 * c * nz + q] += w[3 * c * nz + q] * dx[3 * c * nz + q];
   }, 64);
 #endif
 }
@@ -2765,14 +2765,17 @@ void fine_cycle(fcbga_model *M, const T *b, T *x, T *r, T *dx, bool withCoarse) 
     });
     shell_cycle<T>(M, 0);
     const T *ec = LevelVec<T>::x(C);
+    // prolongation into the iterate y = w x: the scaled correction
+    // sinv (P e) times w is the unscaled (P e) on free DOFs
+    const uint8_t *mask = M->dofmask.p;
     be::pfor(M->g.G, s, [=] __host__ __device__(int64_t gi) {
       int32_t A = nodeAgg[gi];
       if (A < 0) return;
       const T *e = ec + 6 * (int64_t)A;
       T z = zp[gi];
-      x[3 * gi] += (e[0] + z * e[4]) * sinv[3 * gi];
-      x[3 * gi + 1] += (e[1] - z * e[3]) * sinv[3 * gi + 1];
-      x[3 * gi + 2] += (e[2] + z * e[5]) * sinv[3 * gi + 2];
+      if (mask[3 * gi]) x[3 * gi] += e[0] + z * e[4];
+      if (mask[3 * gi + 1]) x[3 * gi + 1] += e[1] - z * e[3];
+      if (mask[3 * gi + 2]) x[3 * gi + 2] += e[2] + z * e[5];
     });
   }
   for (int sw = 0; sw < M->nsweeps; sw++)
@@ -3265,10 +3268,13 @@ void precondition(fcbga_model *M, int precision, int precond, bool r32_ready = f
     float *b32 = M->b32v.p, *x32 = M->x32v.p;
     if (!r32_ready) be::pfor(n3, s, [=] __host__ __device__(int64_t i) { b32[i] = (float)r[i]; });
     fine_cycle<float>(M, b32, x32, M->r32v.p, M->dx32.p, coarse);
-    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { z[i] = mask[i] ? (double)x32[i] : 0.0; });
+    // the cycle's iterate is y = w z: z = y / s on free DOFs
+    const double *sinv = M->sinv64.p;
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { z[i] = mask[i] ? (double)x32[i] * sinv[i] : 0.0; });
     return;
   }
-  be::pfor(n3, s, [=] __host__ __device__(int64_t i) { if (!mask[i]) z[i] = 0.0; });
+  const double *sinv = M->sinv64.p;
+  be::pfor(n3, s, [=] __host__ __device__(int64_t i) { z[i] = mask[i] ? z[i] * sinv[i] : 0.0; });
 }
 
 // device scalar slots of the PCG

@@ -52,7 +52,9 @@ FE_HD int corner_lk(int a) { return a >= 4; }
 // without the assembled matrix: K is gathered from the condensed element
 // matrices of the (up to 8 x 2) elements around the node.
 // ---------------------------------------------------------------------------
-template <typename T, typename TK>
+// PRESCALED: x already holds w*x (the cycle iterate y of the fine level); the
+// row is then w * (A y) on free DOFs and 0 on constrained ones
+template <typename T, typename TK, bool PRESCALED = false>
 FE_HD void node_apply(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
                       const TK *__restrict__ cellScale, const TK *__restrict__ Kc, const T *__restrict__ w,
                       const T *__restrict__ x, T out[3]) {
@@ -67,7 +69,7 @@ FE_HD void node_apply(const Grid &g, int i, int j, int k, const int32_t *__restr
         int ii = i + di - 1, jj = j + dj - 1, kk = k + dk - 1;
         bool in = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
         int64_t gn = in ? ((int64_t)ii * g.ny + jj) * g.nz + kk : 0;
-        for (int p = 0; p < 3; p++) nb[di][dj][dk][p] = in ? w[3 * gn + p] * x[3 * gn + p] : T(0);
+        for (int p = 0; p < 3; p++) nb[di][dj][dk][p] = in ? (PRESCALED ? x[3 * gn + p] : w[3 * gn + p] * x[3 * gn + p]) : T(0);
       }
   T a0 = 0, a1 = 0, a2 = 0;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -102,7 +104,7 @@ FE_HD void node_apply(const Grid &g, int i, int j, int k, const int32_t *__restr
   T acc[3] = {a0, a1, a2};
   for (int p = 0; p < 3; p++) {
     T wi = w[3 * gi + p];
-    out[p] = wi != T(0) ? wi * acc[p] : x[3 * gi + p];
+    out[p] = wi != T(0) ? wi * acc[p] : (PRESCALED ? T(0) : x[3 * gi + p]);
   }
 }
 
@@ -878,7 +880,8 @@ FE_HD void screen_site(const AnandParams &P, const ScreenSteps &h, int32_t i25, 
 // ---------------------------------------------------------------------------
 constexpr int NZP = 66;  // largest stride: 64 planes + 2 padding levels
 
-template <typename T>
+// PRESCALED: x already holds w*x (w unused)
+template <typename T, bool PRESCALED = false>
 FE_HD void fill_neighbourhood(const Grid &g, int i, int j, int k, const T *__restrict__ w, const T *__restrict__ x,
                               T *nb, int nzp) {
   // every load is issued unconditionally (columns outside the grid read the
@@ -892,7 +895,7 @@ FE_HD void fill_neighbourhood(const Grid &g, int i, int j, int k, const T *__res
     const int ii = i + q / 3 - 1, jj = j + q % 3 - 1;
     in[q] = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny;
     const int gn = ((in[q] ? ii * g.ny + jj : i * g.ny + j) * g.nz + k) * 3;
-    for (int p = 0; p < 3; p++) v[q][p] = w[gn + p] * x[gn + p];
+    for (int p = 0; p < 3; p++) v[q][p] = PRESCALED ? x[gn + p] : w[gn + p] * x[gn + p];
   }
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
@@ -1221,7 +1224,7 @@ FE_HD void node_rows_nb_sym(const Grid &g, int i, int j, int k, const int32_t *_
 
 // Host form of one node row through the symmetric tables (same output rule as
 // node_apply: w * (A w x) on free DOFs, x on constrained ones).
-template <typename T, int CI, int CJ>
+template <typename T, int CI, int CJ, bool PRESCALED = false>
 FE_HD void cell_sym_direct(const Grid &g, int i, int j, int kk, bool wantUp, const int32_t *__restrict__ cellType,
                            const T *__restrict__ cellScale, const SymTabs<T> &tb, const T *__restrict__ w,
                            const T *__restrict__ x, T acc[3]) {
@@ -1235,7 +1238,7 @@ FE_HD void cell_sym_direct(const Grid &g, int i, int j, int kk, bool wantUp, con
     T xb[8][3];
     for (int b = 0; b < 8; b++) {
       const int64_t gn = ((int64_t)(ii + corner_li(b)) * g.ny + jj + corner_lj(b)) * g.nz + kk + corner_lk(b);
-      for (int p = 0; p < 3; p++) xb[b][p] = w[3 * gn + p] * x[3 * gn + p];
+      for (int p = 0; p < 3; p++) xb[b][p] = PRESCALED ? x[3 * gn + p] : w[3 * gn + p] * x[3 * gn + p];
     }
     T own[3] = {0, 0, 0}, up[3] = {0, 0, 0};
     if (t >= tb.own0)
@@ -1246,23 +1249,23 @@ FE_HD void cell_sym_direct(const Grid &g, int i, int j, int kk, bool wantUp, con
   }
 }
 
-template <typename T>
+template <typename T, bool PRESCALED = false>
 FE_HD void node_apply_sym(const Grid &g, int i, int j, int k, const int32_t *__restrict__ cellType,
                           const T *__restrict__ cellScale, const SymTabs<T> &R, const T *__restrict__ w,
                           const T *__restrict__ x, T out[3]) {
   T own[3] = {0, 0, 0}, up[3] = {0, 0, 0};
-  cell_sym_direct<T, 0, 0>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
-  cell_sym_direct<T, 0, 1>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
-  cell_sym_direct<T, 1, 0>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
-  cell_sym_direct<T, 1, 1>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
-  cell_sym_direct<T, 0, 0>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
-  cell_sym_direct<T, 0, 1>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
-  cell_sym_direct<T, 1, 0>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
-  cell_sym_direct<T, 1, 1>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
+  cell_sym_direct<T, 0, 0, PRESCALED>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
+  cell_sym_direct<T, 0, 1, PRESCALED>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
+  cell_sym_direct<T, 1, 0, PRESCALED>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
+  cell_sym_direct<T, 1, 1, PRESCALED>(g, i, j, k, false, cellType, cellScale, R, w, x, own);
+  cell_sym_direct<T, 0, 0, PRESCALED>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
+  cell_sym_direct<T, 0, 1, PRESCALED>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
+  cell_sym_direct<T, 1, 0, PRESCALED>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
+  cell_sym_direct<T, 1, 1, PRESCALED>(g, i, j, k - 1, true, cellType, cellScale, R, w, x, up);
   const int64_t gi = ((int64_t)i * g.ny + j) * g.nz + k;
   for (int p = 0; p < 3; p++) {
     const T wi = w[3 * gi + p];
-    out[p] = wi != T(0) ? wi * (own[p] + up[p]) : x[3 * gi + p];
+    out[p] = wi != T(0) ? wi * (own[p] + up[p]) : (PRESCALED ? T(0) : x[3 * gi + p]);
   }
 }
 

@@ -1,6 +1,8 @@
 // Fine operator on the matrix cores: y = A x (MODE 0) or y = b - A x (MODE 1)
 // over every grid node, A the Jacobi-scaled, constraint-reduced operator
 //   y_i = w_i sum_e sc_e (K_t(e) (w x)_e)_i   on free DOFs,  y_i = x_i on the others
+// (MODE 1 is the multigrid cycle's residual, whose iterate x already holds w x:
+// the staging takes x as it is and constrained rows give b_i)
 // (the element-by-element form of the assembled product of
 // src/core/09-assembly.js:96-108, 151-165, 445-464, with the condensed element
 // matrices of src/core/09-assembly.js:368-424 shared per element type).
@@ -265,7 +267,7 @@ __global__ void __launch_bounds__(THREADS) k_elem_mfma(fe::Grid g, int ntx, int 
     const int i = i0 - 1 + a, j = j0 - 1 + b, k = k0 - 1 + dn;
     const bool in = i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
     const int64_t gn = in ? (((int64_t)i * g.ny + j) * g.nz + k) * 3 : 0;
-    for (int p = 0; p < 3; p++) xs[q * 3 + p] = in ? w[gn + p] * x[gn + p] : T(0);
+    for (int p = 0; p < 3; p++) xs[q * 3 + p] = in ? (MODE ? x[gn + p] : w[gn + p] * x[gn + p]) : T(0);
   }
   __syncthreads();
   T acc[2] = {T(0), T(0)};
@@ -287,7 +289,7 @@ __global__ void __launch_bounds__(THREADS) k_elem_mfma(fe::Grid g, int ntx, int 
   }
   for (int h = 0; h < 2; h++) {
     if (gno[h] < 0) continue;
-    const T out = wo[h] != T(0) ? wo[h] * acc[h] : xo[h];
+    const T out = wo[h] != T(0) ? wo[h] * acc[h] : (MODE ? T(0) : xo[h]);
     y[gno[h]] = MODE ? bo[h] - out : out;
   }
 }
@@ -311,7 +313,7 @@ void elem_op_host(const fe::Grid &g, int ntx, int nty, int nchunk, const int32_t
       const int i = i0 - 1 + a, j = j0 - 1 + b, k = k0 - 1 + dn;
       const bool in = i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
       const int64_t gn = in ? (((int64_t)i * g.ny + j) * g.nz + k) * 3 : 0;
-      for (int p = 0; p < 3; p++) xs[chk(q * 3 + p, XSN * 3)] = in ? w[gn + p] * x[gn + p] : T(0);
+      for (int p = 0; p < 3; p++) xs[chk(q * 3 + p, XSN * 3)] = in ? (MODE ? x[gn + p] : w[gn + p] * x[gn + p]) : T(0);
     }
     std::fill(acc.begin(), acc.end(), T(0));
     for (int ph = 0; ph < 2; ph++) {
@@ -362,7 +364,7 @@ void elem_op_host(const fe::Grid &g, int ntx, int nty, int nchunk, const int32_t
       if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
       const int64_t gn = (((int64_t)i * g.ny + j) * g.nz + k) * 3 + p;
       const T wi = w[gn];
-      const T out = wi != T(0) ? wi * acc[it] : x[gn];
+      const T out = wi != T(0) ? wi * acc[it] : (MODE ? T(0) : x[gn]);
       y[gn] = MODE ? bvec[gn] - out : out;
     }
   }
