@@ -56,6 +56,14 @@ typedef struct fcbga_mesh {
   const int32_t *constraint_dofs; /* constrained DOFs 3*node+component */
   int32_t nfeature_x, nfeature_y;
   const int32_t *feature_x, *feature_y; /* grid-line indices kept on every coarse level */
+  /* Stacked-module stages (FEModel.moduleConstraints, src/core/09-assembly.js:227, and
+   * assembleStep/moduleStageSet, :392-440): the package constraints plus each module's 3-2-1
+   * support on its base.  With nmodule_constraints > 0 a step of stage < 0 is a module stage:
+   * only the elements of born groups are assembled (the others are dropped, not scaled by
+   * 1e-6), these DOFs and every DOF of a node that no born element touches are held, and the
+   * multigrid aggregates are built over the born elements only.  0 / NULL: no module stages. */
+  int32_t nmodule_constraints;
+  const int32_t *module_constraint_dofs;
 } fcbga_mesh;
 
 /* One step of the thermal path [Ta, Tb] in a stage: the per-group data of

@@ -234,7 +234,14 @@ struct fcbga_model {
   // device: mesh-constant
   DBuf<int32_t> cellType, cellElem, d_egroup, d_typeSlot;
   DBuf<double> d_escale, d_typeH;
+  // free DOFs of the current step (1 free, 0 held or absent); the package's
+  // set, or in a stacked-module stage the module stage's (assemble_step)
   DBuf<uint8_t> dofmask;
+  std::vector<uint8_t> maskAll;      // the package constraints held
+  std::vector<int32_t> modConstraints;  // held in every module stage (with the package's)
+  std::vector<int64_t> modGridDofs;     // the same as grid DOFs (3 * grid node + component)
+  std::vector<uint8_t> maskBorn;     // group birth flags of the module stage in dofmask (empty: maskAll)
+  std::vector<uint8_t> hierBorn;     // group birth flags the hierarchy was built for (empty: every element)
   DBuf<int64_t> gridOf;
   DBuf<int32_t> fineColors[4];
   int32_t nFineColor[4] = {0, 0, 0, 0};
@@ -425,6 +432,10 @@ void model_create(const fcbga_mesh *mm, fcbga_model *M, int32_t ownGroup = -1) {
   if (mm->e_section) M->e_section.assign(mm->e_section, mm->e_section + M->ne);
   else M->e_section.assign(M->ne, -1);
   M->constraints.assign(mm->constraint_dofs, mm->constraint_dofs + mm->nconstraints);
+  if (mm->nmodule_constraints > 0) {
+    if (!mm->module_constraint_dofs) throw be::Error("module_constraint_dofs missing");
+    M->modConstraints.assign(mm->module_constraint_dofs, mm->module_constraint_dofs + mm->nmodule_constraints);
+  }
   if (mm->nfeature_x) M->fx.assign(mm->feature_x, mm->feature_x + mm->nfeature_x);
   if (mm->nfeature_y) M->fy.assign(mm->feature_y, mm->feature_y + mm->nfeature_y);
   // validate
@@ -437,6 +448,8 @@ void model_create(const fcbga_mesh *mm, fcbga_model *M, int32_t ownGroup = -1) {
   if (cnt != M->nn) throw be::Error("node_id count differs from nn");
   for (int32_t d : M->constraints)
     if (d < 0 || d >= 3 * M->nn) throw be::Error("constraint DOF out of range");
+  for (int32_t d : M->modConstraints)
+    if (d < 0 || d >= 3 * M->nn) throw be::Error("module constraint DOF out of range");
   for (int32_t e = 0; e < M->ne; e++) {
     if (M->e_group[e] < 0 || M->e_group[e] >= M->ngroups) throw be::Error("element group out of range");
     if (M->e_section[e] >= M->nsec) throw be::Error("element section out of range");
@@ -519,6 +532,8 @@ void model_create(const fcbga_mesh *mm, fcbga_model *M, int32_t ownGroup = -1) {
       mask[3 * q] = mask[3 * q + 1] = mask[3 * q + 2] = 1;
     }
   for (int32_t d : M->constraints) mask[3 * gridOf[d / 3] + d % 3] = 0;
+  M->maskAll = mask;
+  for (int32_t d : M->modConstraints) M->modGridDofs.push_back(3 * gridOf[d / 3] + d % 3);
 #if FCBGA_GPU
   BE_CHECK(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
 #endif
@@ -562,10 +577,70 @@ void check_step(const fcbga_model *M, const fcbga_step *st) {
     throw be::Error("step data missing for layered sections (secW, secV, sec_born)");
 }
 
+// captured iteration graphs hold buffer addresses: dropped when buffers move
+void drop_graphs(fcbga_model *M) {
+#if FCBGA_GPU
+  for (auto &gr : M->graphs) (void)hipGraphExecDestroy(gr.second);
+  M->graphs.clear();
+#else
+  (void)M;
+#endif
+}
+
+// DOF mask of a step: the package's, or in a stacked-module stage
+// (FEModel.moduleStageSet, src/core/09-assembly.js:427-440) the DOFs of the
+// nodes that born elements touch, less the module constraints (the package's
+// 3-2-1 and each module's support).  Uploaded only when the stage's set changes.
+void stage_mask(fcbga_model *M, bool modStage, const std::vector<uint8_t> &born) {
+  static const std::vector<uint8_t> all;
+  const std::vector<uint8_t> &key = modStage ? born : all;
+  if (M->maskBorn == key) return;
+  if (!modStage) {
+    M->dofmask.upload(M->maskAll, M->stream);
+  } else {
+    const fe::Grid g = M->g;
+    std::vector<uint8_t> mask(M->maskAll.size(), 0);
+    for (int ci = 0; ci < g.ncx; ci++)
+      for (int cj = 0; cj < g.ncy; cj++)
+        for (int ck = 0; ck < g.ncz; ck++) {
+          const int64_t c = ((int64_t)ci * g.ncy + cj) * g.ncz + ck;
+          bool on = false;
+          for (int q = 0; q < 2; q++) {
+            const int32_t e = M->cell_elems[2 * c + q];
+            on = on || (e >= 0 && born[M->e_group[e]]);
+          }
+          if (!on) continue;
+          for (int a = 0; a < 8; a++) {
+            const int64_t gn = ((int64_t)(ci + fe::corner_li(a)) * g.ny + cj + fe::corner_lj(a)) * g.nz + ck + fe::corner_lk(a);
+            for (int p = 0; p < 3; p++) mask[3 * gn + p] = M->maskAll[3 * gn + p];
+          }
+        }
+    for (int64_t d : M->modGridDofs) mask[d] = 0;
+    M->dofmask.upload(mask, M->stream);
+  }
+  M->maskBorn = key;
+}
+
 void assemble_step(fcbga_model *M, const fcbga_step *st) {
   be::stream_t s = M->stream;
   const fe::Grid g = M->g;
   const int64_t ncells = (int64_t)g.ncx * g.ncy * g.ncz;
+  // stacked-module stage (FEModel.assembleStep, src/core/09-assembly.js:397-403, 423):
+  // unborn elements are dropped, the stage's own constraint set holds the rest,
+  // and the multigrid aggregates follow the born elements (a new hierarchy
+  // whenever that set changes, LinearSolver._buildPC, src/core/10-solver.js:868-883)
+  const bool modStage = st->stage < 0 && !M->modConstraints.empty();
+  {
+    const std::vector<uint8_t> born(st->born, st->born + M->ngroups);
+    stage_mask(M, modStage, born);
+    const std::vector<uint8_t> hk = modStage ? born : std::vector<uint8_t>();
+    if (M->hierBorn != hk) {
+      if (M->hierBuilt) drop_graphs(M);
+      M->hierBuilt = false;
+      M->hierBorn = hk;
+    }
+  }
+  const double unbornScale = modStage ? 0.0 : 1e-6;
   // material slots: shape matrices and loads
   std::vector<double> Mall((size_t)M->nslot * 6 * 1089), load((size_t)M->nslot * 18, 0.0);
   std::vector<uint8_t> hasF(M->nslot, 0), layered(M->nslot, 0);
@@ -650,7 +725,7 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
     for (double v : e) emax = v > emax ? v : emax;
     M->symTables = M->sym_allowed && emax <= SYM_TOL;
   }
-  // element scales of the step: eScale x (born ? 1 : 1e-6)
+  // element scales of the step: eScale x (born ? 1 : 1e-6), unborn 0 in a module stage
   M->cellScale64.alloc(2 * ncells);
   M->cellScale32.alloc(2 * ncells);
   {
@@ -661,7 +736,7 @@ void assemble_step(fcbga_model *M, const fcbga_step *st) {
     float *cs32 = M->cellScale32.p;
     be::pfor(2 * ncells, s, [=] __host__ __device__(int64_t q) {
       int32_t e = ce[q];
-      double v = e >= 0 ? es[e] * (bn[eg[e]] ? 1.0 : 1e-6) : 0.0;
+      double v = e >= 0 ? es[e] * (bn[eg[e]] ? 1.0 : unbornScale) : 0.0;
       cs[q] = v;
       cs32[q] = (float)v;
     });
@@ -1258,6 +1333,25 @@ void build_hierarchy(fcbga_model *M) {
   mv.cell_elems = M->cell_elems.data();
   mv.e_part = M->e_part.data();
   mv.feature_x = M->fx; mv.feature_y = M->fy;
+  if (!M->hierBorn.empty()) {
+    // stacked-module stage: the born elements and the nodes they touch
+    mv.elem_on.assign(M->ne, 0);
+    mv.node_on.assign(M->nn, 0);
+    const fe::Grid &g = M->g;
+    for (int ci = 0; ci < g.ncx; ci++)
+      for (int cj = 0; cj < g.ncy; cj++)
+        for (int ck = 0; ck < g.ncz; ck++) {
+          const int64_t c = ((int64_t)ci * g.ncy + cj) * g.ncz + ck;
+          for (int q = 0; q < 2; q++) {
+            const int32_t e = M->cell_elems[2 * c + q];
+            if (e < 0 || !M->hierBorn[M->e_group[e]]) continue;
+            mv.elem_on[e] = 1;
+            for (int a = 0; a < 8; a++)
+              mv.node_on[M->node_id[((int64_t)(ci + fe::corner_li(a)) * g.ny + cj + fe::corner_lj(a)) * g.nz + ck +
+                                    fe::corner_lk(a)]] = 1;
+          }
+        }
+  }
   mgs::build_hierarchy(mv, M->H);
   M->nodeAgg.upload(M->H.nodeAgg, s);
   M->zp64.upload(M->H.zp, s);
@@ -3444,8 +3538,7 @@ void alloc_vectors(fcbga_model *M, int precision) {
   // captured iteration graphs hold these addresses: drop them when a buffer moved
   for (int q = 0; q < 6; q++)
     if (before[q] && before[q] != after[q]) {
-      for (auto &g : M->graphs) (void)hipGraphExecDestroy(g.second);
-      M->graphs.clear();
+      drop_graphs(M);
       break;
     }
 #else

@@ -30,6 +30,9 @@ struct MeshView {
   const int32_t *cell_elems;  // 2 * ncells
   const int8_t *e_part;       // per element
   std::vector<int32_t> feature_x, feature_y;
+  // stacked-module stage (buildShellLevel's act, src/core/10-solver.js:260-266): elements
+  // and nodes (compact ids) in play; the others count as void.  Empty: all.
+  std::vector<uint8_t> elem_on, node_on;
 };
 
 struct Level {
@@ -94,13 +97,14 @@ inline void build_level1(const MeshView &m, Hierarchy &H) {
   lv.ys.assign(m.ys, m.ys + ny);
   lv.keepX = m.feature_x; lv.keepY = m.feature_y;
   lv.colStart.assign(ncol + 1, 0);
+  const bool act = !m.elem_on.empty();
   auto cls = [&](int i, int j, int kk) -> int {
     if (kk < 0 || kk >= ncz) return 0;
     int64_t cc = ((int64_t)i * ncy + j) * ncz + kk;
     int any = 0, stiff = 0;
     for (int q = 0; q < 2; q++) {
       int32_t e = cc < ncells ? m.cell_elems[2 * cc + q] : -1;
-      if (e < 0) continue;
+      if (e < 0 || (act && !m.elem_on[e])) continue;
       any = 1;
       if (!compliant_part(m.e_part[e])) stiff = 1;
     }
@@ -115,7 +119,8 @@ inline void build_level1(const MeshView &m, Hierarchy &H) {
     bool have = false;
     int prevK = -2, prevType = 0;
     for (int k = 0; k < nz; k++) {
-      if (m.node_id[(int64_t)c * nz + k] < 0) { have = false; prevK = -2; continue; }
+      const int32_t nd = m.node_id[(int64_t)c * nz + k];
+      if (nd < 0 || (act && !m.node_on[nd])) { have = false; prevK = -2; continue; }
       int below = cls(i, j, k - 1), above = cls(i, j, k);
       int type = (below == 2 || above == 2) ? 2 : 1;
       bool joined = false;

@@ -39,7 +39,8 @@ class FcbgaMesh(C.Structure):
                 ("e_group", _i32p), ("e_part", _i8p), ("e_scale", _f64p), ("e_section", _i32p),
                 ("ngroups", C.c_int32), ("nsections", C.c_int32), ("nconstraints", C.c_int32),
                 ("constraint_dofs", _i32p), ("nfeature_x", C.c_int32), ("nfeature_y", C.c_int32),
-                ("feature_x", _i32p), ("feature_y", _i32p)]
+                ("feature_x", _i32p), ("feature_y", _i32p), ("nmodule_constraints", C.c_int32),
+                ("module_constraint_dofs", _i32p)]
 
 
 class FcbgaStep(C.Structure):
@@ -302,6 +303,11 @@ def _mesh_struct(mesh):
     m.constraint_dofs = _ptr(a["cons"], _i32p)
     m.nfeature_x, m.nfeature_y = len(a["fx"]), len(a["fy"])
     m.feature_x, m.feature_y = _ptr(a["fx"], _i32p), _ptr(a["fy"], _i32p)
+    mod = getattr(mesh, "module_constraint_dofs", None)
+    if mod is not None and len(mod):
+        a["mod"] = np.ascontiguousarray(mod, np.int32)
+        m.nmodule_constraints = len(a["mod"])
+        m.module_constraint_dofs = _ptr(a["mod"], _i32p)
     return m, a
 
 

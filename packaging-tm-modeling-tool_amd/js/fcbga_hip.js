@@ -55,10 +55,14 @@ function stepFromModel(model, Ta, Tb, stage, sectionWmean) {
   return step;
 }
 
-/** The mesh contract for createModel from a FEModel: its mesh plus group count and element sections. */
+/**
+ * The mesh contract for createModel from a FEModel: its mesh plus group count, element sections and
+ * the module constraints of stacked-module stages (FEModel.moduleConstraints, src/core/09-assembly.js:227).
+ */
 function meshForModel(mesh, model) {
   const sections = model.sections || [];
   const out = Object.assign({}, mesh, { ngroups: model.groups.length, nsections: sections.length });
+  if (model.moduleConstraints) out.moduleConstraints = Int32Array.from(model.moduleConstraints);
   if (sections.length) {
     const eSection = new Int32Array(mesh.ne).fill(-1);
     for (let e = 0; e < mesh.ne; e++) if (model.groups[mesh.eGroup[e]].kind === 'layered') eSection[e] = model.sectionIndex(e);
@@ -368,9 +372,9 @@ function install(Analysis, opts) {
   delete o.sectionWmean;
   Analysis.prototype.solveStep = function (k) {
     const s = this.path.steps[k];
-    // stacked-module stages hold every node outside the born modules (per-stage constraint
-    // sets, src/core/09-assembly.js:392-424): those steps keep the JS solver
-    if (s.stage < 0 && this.model.moduleConstraints) return original.call(this, k);
+    // stacked-module stages (stage < 0) go to the device too: the library drops the unborn
+    // elements, holds the module stage's constraint set and builds the stage's aggregates
+    // from the module constraints passed with the mesh and the step's birth flags
     const sections = this.model.sections || [];
     if (sections.length && !wmean) return original.call(this, k);
     if (typeof this.checkCancel === 'function') this.checkCancel();

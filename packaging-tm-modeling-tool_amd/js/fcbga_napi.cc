@@ -156,6 +156,11 @@ napi_value CreateModel(napi_env env, napi_callback_info info) {
     m.constraint_dofs = (const int32_t *)p;
     m.nconstraints = (int32_t)n;
   }
+  // stacked-module stages: the FEModel's moduleConstraints (src/core/09-assembly.js:227)
+  if (get_typed(env, mesh, "moduleConstraints", napi_int32_array, &p, &n) && n > 0) {
+    m.module_constraint_dofs = (const int32_t *)p;
+    m.nmodule_constraints = (int32_t)n;
+  }
   // layered sections (FEModel: mesh.sections from substrateBands, src/core/09-assembly.js:217-219, 309):
   // eSection (-1 homogeneous) and the number of sections
   int32_t nsec = 0;

@@ -104,6 +104,9 @@ class Mesh:
     groups: list = field(default_factory=list)
     e_section: np.ndarray = None  # int32 [ne]: layered section of an element, -1 homogeneous (None: none)
     nsections: int = 0
+    # stacked-module stages: the FEModel's moduleConstraints (the package's 3-2-1 plus each
+    # module's support, src/core/09-assembly.js:227); None: no module stages
+    module_constraint_dofs: np.ndarray = None
 
     @property
     def nx(self): return len(self.xs)
@@ -334,7 +337,7 @@ def load_spec(path: str) -> dict:
 
 
 def mesh_from_arrays(xs, ys, zs, node_id, cell_elems, e_group, e_part, e_scale, constraint_dofs, feature_x=(),
-                     feature_y=(), ngroups=None, e_section=None, nsections=0) -> Mesh:
+                     feature_y=(), ngroups=None, e_section=None, nsections=0, module_constraint_dofs=None) -> Mesh:
     """A Mesh from the arrays of the reference's mesh contract (buildStructuredMesh /
     buildPackageMesh, src/core/06-mesher.js:66-71, 500-545): the derived arrays (element
     cells, connectivity, coordinates, column starts) are rebuilt from node_id and cell_elems."""
@@ -365,4 +368,6 @@ def mesh_from_arrays(xs, ys, zs, node_id, cell_elems, e_group, e_part, e_scale, 
                 col_start=col_start, constraint_dofs=np.asarray(constraint_dofs, np.int32), constraint_nodes=[],
                 feature_x=np.asarray(feature_x, np.int32), feature_y=np.asarray(feature_y, np.int32),
                 groups=[None] * ng,
-                e_section=None if e_section is None else np.asarray(e_section, np.int32), nsections=int(nsections))
+                e_section=None if e_section is None else np.asarray(e_section, np.int32), nsections=int(nsections),
+                module_constraint_dofs=None if module_constraint_dofs is None
+                else np.asarray(module_constraint_dofs, np.int32))

@@ -17,6 +17,7 @@
 //   submodel_golden.json, submodel_*.f64.gz (a bump submodel of the bare-die preset: set-up data and result)
 //   path_golden.json                    (thermal path plans, evaluation temperatures and step sums)
 //   package_<bare|lidded>_golden.json.gz (reference-built Draft package meshes with layered sections, two steps)
+//   package_modules_golden.json.gz      (argument package_modules: a stacked-module package, module-stage steps)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -582,8 +583,75 @@ function packageCases() {
   }
 }
 
+// A stacked-module package (the reference's own V18 check, src/core/15-verify.js:577-585: the bare
+// preset on a 14 mm substrate with one Si-base module of two chiplets, Draft mesh capped at 20000
+// nodes). Its thermal path starts with the module stages (chiplet join, wafer underfill, mold cure;
+// stage < 0), in which only the born module parts are assembled and every node they do not touch
+// is held besides the module's 3-2-1 support (FEModel.assembleStep, src/core/09-assembly.js:392-450),
+// and the multigrid aggregates follow the born parts (LinearSolver._buildPC, 10-solver.js:868-883).
+// Written: package_modules_golden.json.gz with the FEModel's module constraints, the first step of
+// every module stage, the first package step and the last chain step: prepared data, the held DOFs
+// of the step, the reference's multigrid level sizes and its solution at tol 1e-12.
+function modulesCase() {
+  const db = JSON.parse(fs.readFileSync(path.join(REF, 'src', 'materials-db.json'), 'utf8'));
+  const presetConfig = get('presetConfig'), Analysis = get('Analysis'), sectionWmean = get('sectionWmean');
+  const defaultModule = get('defaultModule'), defaultChiplet = get('defaultChiplet');
+  const t0 = Date.now();
+  const cfg = presetConfig(db, 'bare');
+  cfg.substrate.sx = cfg.substrate.sy = 14;
+  const mod = defaultModule('M1', 0, 0, 'si');
+  Object.assign(mod, { sx: 7, sy: 6 });
+  mod.chiplets = [defaultChiplet('C1', -1.75, 0, 3.0, 5.0), defaultChiplet('C2', 1.75, 0, 3.0, 5.0)];
+  cfg.dies = []; cfg.modules = [mod];
+  const an = new Analysis(cfg, { preset: 'draft', evaluations: 'full', cap: 20000 });
+  an.build();
+  const steps = an.buildPath();
+  const m = an.mesh, model = an.model;
+  const eSection = new Int32Array(m.ne).fill(-1);
+  for (let e = 0; e < m.ne; e++) if (model.groups[m.eGroup[e]].kind === 'layered') eSection[e] = model.sectionIndex(e);
+  const chain = an.path.chain, pick = [];
+  const seen = new Set();
+  for (const k of chain) if (steps[k].stage < 0 && !seen.has(steps[k].stage)) { seen.add(steps[k].stage); pick.push(k); }
+  pick.push(chain.find(k => steps[k].stage >= 0), chain[chain.length - 1]);
+  const solver = new LinearSolver(model, { precond: 'mg', tol: 1e-12 });
+  const rhs = new Float64Array(model.ndof), W = new Float64Array(108);
+  const out = [];
+  for (const k of pick) {
+    const s = steps[k];
+    model.assembleStep(s.Ta, s.Tb, s.stage, rhs);
+    const gD = [], gs0 = [], gborn = [];
+    for (const g of model.groups) {
+      // a layered group's birth (FEModel.isBorn; prepareStep keeps no _born for it) scales its elements
+      if (g.kind === 'layered') { gD.push(...new Float64Array(36)); gs0.push(...new Float64Array(6)); gborn.push(model.isBorn(g, s.stage) ? 1 : 0); continue; }
+      gD.push(...g._D); gs0.push(...g._s0); gborn.push(g._born ? 1 : 0);
+    }
+    const sW = [], sV = [], sborn = [];
+    for (const sec of model.sections) { sectionWmean(sec, s.Ta, s.Tb, W); sW.push(...W); sV.push(...sec._V); sborn.push(sec._born ? 1 : 0); }
+    const held = s.stage < 0 ? list(model.moduleStageConstraints(s.stage)) : list(model.constraints);
+    const rhsCopy = list(rhs);
+    solver.basis = []; solver.pc = null;
+    const res = solver.solve(rhs, 'modules step ' + k);
+    const levels = solver.pc.info.levels.map(l => l.dof);
+    out.push({ k, Ta: s.Ta, Tb: s.Tb, stage: s.stage, branch: s.branch, D: gD, s0: gs0, born: gborn, secW: sW, secV: sV,
+      secBorn: sborn, held, levels, u: list(res.u), rhs: rhsCopy, iters: res.iters, relres: res.relres });
+    console.log('modules step', k, s.Ta, '->', s.Tb, 'stage', s.stage, 'held', held.length, 'levels', levels.join(' '), 'iters', res.iters, 'relres', res.relres);
+  }
+  const bga = m.geom.bga;
+  const rec = { name: 'modules', nx: m.xs.length, ny: m.ys.length, nz: m.zs.length, nn: m.nn, ne: m.ne,
+    xs: list(m.xs), ys: list(m.ys), zs: list(m.zs), nodeId: list(m.nodeId), cellElems: list(m.cellElems),
+    eGroup: list(m.eGroup), ePart: list(m.ePart), eScale: list(m.eScale), eSection: list(eSection),
+    ngroups: model.groups.length, nsections: model.sections.length, groupKinds: model.groups.map(g => g.kind || 'homog'),
+    groupStages: model.groups.map(g => g.stage || 0),
+    constraints: list(m.constraints.dofs), moduleConstraints: list(model.moduleConstraints),
+    featureX: list(m.featureX || []), featureY: list(m.featureY || []),
+    bga: { x: list(bga.x), y: list(bga.y), zone: bga.zone || null }, steps: out, pathSteps: steps.length };
+  fs.writeFileSync(path.join(HERE, 'package_modules_golden.json.gz'), zlib.gzipSync(Buffer.from(JSON.stringify(rec)), { level: 9 }));
+  console.log('modules', m.nn * 3, 'DOF', m.ne, 'elements', model.sections.length, 'sections', Date.now() - t0, 'ms');
+}
+
 const what = process.argv.slice(3);
 if (what.includes('package')) packageCases();
+if (what.includes('package_modules')) modulesCase();
 if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }
 if (what.includes('warm')) for (const n of ['small_box', 'plate_16']) warmCases(n);
 if (what.includes('stress')) for (const n of ['small_box', 'cfg1']) stressCases(n);

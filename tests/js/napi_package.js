@@ -9,7 +9,9 @@
 // groups' _D/_s0/_born and the sections' _V/_born of the step, sectionIndex
 // reads the fixture's element sections, and sectionWmean gives the step's
 // section weights.
-//   node tests/js/napi_package.js <bare|lidded>   -> one JSON line
+// The stacked-module package (package_modules_golden.json.gz) adds the FEModel's
+// module constraints and steps of the module stages (stage < 0).
+//   node tests/js/napi_package.js <bare|lidded|modules>   -> one JSON line
 const fs = require('fs');
 const path = require('path');
 const zlib = require('zlib');
@@ -26,7 +28,8 @@ const mesh = {
 };
 let cur = null;  // the fixture step prepared last
 const model = {
-  groups: G.groupKinds.map(kind => ({ kind, stage: 0 })),
+  groups: G.groupKinds.map((kind, q) => ({ kind, stage: G.groupStages ? G.groupStages[q] : 0 })),
+  moduleConstraints: G.moduleConstraints ? Int32Array.from(G.moduleConstraints) : null,
   sections: Array.from({ length: G.nsections }, (_, q) => ({ q })),
   sectionIndex: e => G.eSection[e],
   isBorn: (g, stage) => (g.stage || 0) <= stage,

@@ -1,7 +1,10 @@
 """The reference-built package meshes of tests/golden/package_<name>_golden.json.gz
 (written by gen_golden.js 'package': the bare-die and lidded default presets at the
 Draft mesh with layered substrate sections, two steps of their thermal path with the
-FEModel's prepared group and section data and the reference's solution at tol 1e-12)."""
+FEModel's prepared group and section data and the reference's solution at tol 1e-12),
+and the stacked-module package of package_modules_golden.json.gz (gen_golden.js
+'package_modules': the FEModel's module constraints, steps of the module stages with
+their held DOFs and the reference's multigrid level sizes)."""
 import gzip
 import json
 import os
@@ -18,7 +21,7 @@ def load(name):
         g = json.load(f)
     m = meshspec.mesh_from_arrays(g["xs"], g["ys"], g["zs"], g["nodeId"], g["cellElems"], g["eGroup"], g["ePart"],
                                   g["eScale"], g["constraints"], g["featureX"], g["featureY"], g["ngroups"],
-                                  g["eSection"], g["nsections"])
+                                  g["eSection"], g["nsections"], g.get("moduleConstraints"))
     return g, m
 
 

@@ -2,7 +2,9 @@
 lidded default presets at the Draft mesh, built by the reference's
 buildPackageMesh (graded lines, fill fractions, layered substrate bands),
 with the FEModel's prepared group and section data of two thermal-path steps
-(tests/golden/package_<name>_golden.json.gz, gen_golden.js 'package').
+(tests/golden/package_<name>_golden.json.gz, gen_golden.js 'package'), and a
+stacked-module package with steps of its module stages
+(package_modules_golden.json.gz, gen_golden.js 'package_modules').
 
 Bar: the assembled load against the reference's to 1e-13, nodal
 displacements within 1e-6 of the largest displacement of the reference's
@@ -65,6 +67,68 @@ def test_gpu_solves_reference_package(name, q, prec):
     _check(g, m, q, u, rec)
 
 
+# the module package's steps on one model, module and package stages alternating, so that the
+# stage's constraint set and multigrid aggregates switch both ways
+MODULE_ORDER = (0, 3, 1, 4, 2)
+
+
+def _check_module_step(g, q, f, u, rec, rel):
+    s = g["steps"][q]
+    rref, uref = np.array(s["rhs"]), np.array(s["u"])
+    assert rec["converged"], (q, rec)
+    # the stage's load: only born parts, zero on the held DOFs
+    assert np.abs(f - rref).max() <= 1e-13 * np.abs(rref).max(), q
+    assert np.abs(u - uref).max() <= rel * np.abs(uref).max(), (q, np.abs(u - uref).max() / np.abs(uref).max())
+    # the module stage's held set: the module supports, the package 3-2-1 and every node no born element touches
+    assert np.abs(u[np.array(s["held"])]).max() == 0, q
+    # the aggregates follow the born parts: the reference's level count and coarsest size
+    assert rec["levels"] == len(s["levels"]) and rec["coarsest_dof"] == s["levels"][-1], (q, rec["levels"], s["levels"])
+
+
+def test_host_emulation_module_stages():
+    """The stacked-module package (the reference's V18 model: one Si-base module with two
+    chiplets on a 14 mm substrate): the first step of each module stage (chiplet join, wafer
+    underfill, mold cure), in which only the born module parts are assembled and everything
+    else is held, and two package steps, on one model.  Loads to 1e-13, displacements within
+    1e-9 of the reference's (both at tol 1e-12), held DOFs exactly zero, the reference's
+    multigrid levels, and its PCG iteration counts within one."""
+    g, m = pc.load("modules")
+    assert len(m.module_constraint_dofs) > len(m.constraint_dofs)
+    M = hs.Model(m, variant="hostemu")
+    stages = set()
+    for q in MODULE_ORDER:
+        st = pc.step(g, q)
+        f = M.load(st)
+        u, rec = M.solve_step(st, tol=1e-12, precision=hs.PREC_FP64)
+        _check_module_step(g, q, f, u, rec, 1e-9)
+        assert abs(rec["iters"] - g["steps"][q]["iters"]) <= 1, (q, rec["iters"], g["steps"][q]["iters"])
+        stages.add(g["steps"][q]["stage"])
+    assert stages == {-3, -2, -1, 1, 2}
+
+
+def test_module_constraints_validated():
+    g, m = pc.load("modules")
+    m.module_constraint_dofs = np.array([3 * m.nn], np.int32)
+    with pytest.raises(hs.SolverError, match="module constraint"):
+        hs.Model(m, variant="hostemu")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [hs.PREC_FP64, hs.PREC_MIXED])
+def test_gpu_module_stages(prec):
+    """The module package's steps on the GPU in both precisions, module and package stages
+    alternating on one model (the iteration graphs captured for one stage's aggregates are
+    dropped when the next stage builds its own)."""
+    g, m = pc.load("modules")
+    M = hs.Model(m, variant="hip")
+    for q in MODULE_ORDER:
+        print("module package step", q, "stage", g["steps"][q]["stage"], flush=True)
+        st = pc.step(g, q)
+        f = M.load(st)
+        u, rec = M.solve_step(st, tol=1e-10, precision=prec)
+        _check_module_step(g, q, f, u, rec, 1e-6)
+
+
 @pytest.mark.gpu
 def test_gpu_adjoint_on_layered_mesh():
     """Group gradients of the warpage proxy on the layered bare-die mesh (the
@@ -103,7 +167,7 @@ def test_gpu_adjoint_on_layered_mesh():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["bare", "lidded"])
+@pytest.mark.parametrize("name", ["bare", "lidded", "modules"])
 def test_node_addon_install_on_reference_package(name):
     """The JS drop-in on a reference-built mesh: install(Analysis) replaces
     solveStep, the step data of the layered sections come from the model's
