@@ -313,6 +313,7 @@ def _mesh_struct(mesh):
 
 class Model:
     def __init__(self, mesh, variant: str = "hip"):
+        self.variant = variant
         self.L = load(variant)
         self.mesh = mesh
         m, self._keep = _mesh_struct(mesh)

@@ -67,6 +67,16 @@ class MaterialTables:
         self.F = np.array([cte_ppm * 1e-6 * (T - T_STRAIN_ORIGIN) for T in Ts])
         self.nu = nu
 
+    @classmethod
+    def from_grids(cls, E, F, nu):
+        """Tables given on the 1 degC grid (E in MPa, F the thermal strain), as a
+        material's own tables hold them (materialTables, src/core/03-materials.js)."""
+        t = cls.__new__(cls)
+        t.E, t.F, t.nu = np.asarray(E, np.float64), np.asarray(F, np.float64), nu
+        if len(t.E) != TGRID_N or len(t.F) != TGRID_N:
+            raise ValueError("property tables must have one value per degree C of the grid")
+        return t
+
     @staticmethod
     def grid_at(arr, T):
         x = T - TGRID_MIN

@@ -18,6 +18,8 @@
 //   path_golden.json                    (thermal path plans, evaluation temperatures and step sums)
 //   package_<bare|lidded>_golden.json.gz (reference-built Draft package meshes with layered sections, two steps)
 //   package_modules_golden.json.gz      (argument package_modules: a stacked-module package, module-stage steps)
+//   tornado_golden.json.gz, tornado_blocks.f64.gz (arguments tornado [nproc], then tornado_screen: the
+//                                        reference's tornado cases with their plans, step data and meshes)
 // Only data leaves this script; no reference source is copied.
 'use strict';
 const fs = require('fs');
@@ -649,7 +651,163 @@ function modulesCase() {
   console.log('modules', m.nn * 3, 'DOF', m.ne, 'elements', model.sections.length, 'sections', Date.now() - t0, 'ms');
 }
 
+// The reference's tornado (runSensitivityCase / tornadoFromCases, src/core/14-sensitivity.js:8-172)
+// on the lidded default preset at the Draft mesh, for a set of its inputs spanning every kind
+// (datasheet ranges, +-10 % material and geometry factors, +-10 C process and Tg temperatures):
+// the base case and each input's low and high case run as the reference runs them (a full Draft
+// analysis with the sensitivity evaluation set), with its outputs (warpRT, warpPeak, dieStress, dW)
+// and the tornado table. For the device to run the same cases, each case keeps its thermal-path
+// plan, its step data (per step: every group's D, s0, birth; every section's W, V, birth) and its
+// mesh (cases whose mesh differs, the geometry inputs, carry their own). Step data blocks shared
+// by several cases are stored once.
+//   tornado_golden.json.gz  cases, plans, outputs, tornado rows, block indices, meshes
+//   tornado_blocks.f64.gz   the distinct blocks (group: D 36, s0 6, born; section: W 108, V 18, born)
+const TORNADO_IDS = ['coreCTE', 'coreE', 'abfA1', 'abfE', 'cuRatio', 'ufA1', 'ufE', 'ufTg', 'ufTsf', 'dieT', 'lidCTE',
+  'lidT', 'adhE', 'Tattach', 'srA1', 'timE'];
+
+function tornadoSetup() {
+  const db = JSON.parse(fs.readFileSync(path.join(REF, 'src', 'materials-db.json'), 'utf8'));
+  const cfg = get('presetConfig')(db, 'default');
+  const inputs = get('sensitivityInputs')(cfg).filter(i => TORNADO_IDS.includes(i.id));
+  const cases = [{ id: '__base', side: null, cfg }];
+  for (const inp of inputs) {
+    const b = get('sensitivityBounds')(cfg, inp);
+    for (const side of ['lo', 'hi']) cases.push({ id: inp.id, side, value: b[side], base: b.base, cfg: get('applySensitivity')(cfg, inp, b[side]) });
+  }
+  return { cfg, inputs, cases };
+}
+
+// one case: the reference's runSensitivityCase, unrolled to keep the plan and step data
+function tornadoCaseRecord(c) {
+  const Analysis = get('Analysis'), sensitivityTemps = get('sensitivityTemps'), sensitivityOutputs = get('sensitivityOutputs');
+  const sectionWmean = get('sectionWmean'), meshFingerprint = get('meshFingerprint');
+  const an = new Analysis(c.cfg, { preset: 'draft', evaluations: 'sensitivity' });
+  an.build();
+  const temps = sensitivityTemps(c.cfg);
+  an.evaluationTemperatures = () => temps;
+  an.run();
+  const outputs = sensitivityOutputs(an);
+  const m = an.mesh, model = an.model, P = an.path, W = new Float64Array(108);
+  const steps = [];
+  for (const s of P.steps) {
+    model.prepareStep(s.Ta, s.Tb, s.stage);
+    const groups = model.groups.map(g => g.kind === 'layered' ? [...new Float64Array(42), model.isBorn(g, s.stage) ? 1 : 0]
+      : [...g._D, ...g._s0, g._born ? 1 : 0]);
+    const sections = model.sections.map(sec => { sectionWmean(sec, s.Ta, s.Tb, W); return [...W, ...sec._V, sec._born ? 1 : 0]; });
+    steps.push({ Ta: s.Ta, Tb: s.Tb, stage: s.stage, branch: s.branch, groups, sections });
+  }
+  const eSection = new Int32Array(m.ne).fill(-1);
+  for (let e = 0; e < m.ne; e++) if (model.groups[m.eGroup[e]].kind === 'layered') eSection[e] = model.sectionIndex(e);
+  const g = m.geom;
+  const mesh = { nx: m.xs.length, ny: m.ys.length, nz: m.zs.length, nn: m.nn, ne: m.ne,
+    xs: list(m.xs), ys: list(m.ys), zs: list(m.zs), nodeId: list(m.nodeId), cellElems: list(m.cellElems),
+    eGroup: list(m.eGroup), ePart: list(m.ePart), eScale: list(m.eScale), eSection: list(eSection),
+    ngroups: model.groups.length, nsections: model.sections.length, groupKinds: model.groups.map(q => q.kind || 'homog'),
+    groupKeys: model.groups.map(q => q.key), constraints: list(m.constraints.dofs),
+    featureX: list(m.featureX || []), featureY: list(m.featureY || []),
+    bga: { x: list(g.bga.x), y: list(g.bga.y), zone: g.bga.zone || null },
+    siParts: g.siParts.map(p => ({ name: p.name, rect: p.rect, ztop: p.ztop, zb: p.zb, gkey: p.gkey, part: p.part })) };
+  return { id: c.id, side: c.side, value: c.value, base: c.base, fingerprint: meshFingerprint(m), mesh, steps,
+    chain: P.chain, up: P.up, down: P.down, temps, peak: c.cfg.reflow.peak, outputs,
+    iters: an.solver.log.map(l => l.iters) };
+}
+
+function tornadoPart(i, n, file) {
+  const { cases } = tornadoSetup();
+  const out = [];
+  for (let q = i; q < cases.length; q += n) {
+    const t0 = Date.now();
+    out.push(Object.assign({ index: q }, tornadoCaseRecord(cases[q])));
+    console.log('tornado case', q, cases[q].id, cases[q].side || '', Date.now() - t0, 'ms');
+  }
+  fs.writeFileSync(file, JSON.stringify(out));
+}
+
+function tornadoCases(nproc) {
+  const cp = require('child_process'), os = require('os');
+  const { inputs, cases } = tornadoSetup();
+  const tmp = fs.mkdtempSync(path.join(os.tmpdir(), 'tornado-'));
+  const files = [];
+  const kids = [];
+  for (let i = 0; i < nproc; i++) {
+    const f = path.join(tmp, 'part' + i + '.json');
+    files.push(f);
+    kids.push(new Promise((res, rej) => {
+      const k = cp.spawn(process.execPath, [__filename, REF, 'tornado_part', String(i), String(nproc), f], { stdio: 'inherit' });
+      k.on('exit', code => code === 0 ? res() : rej(new Error('tornado part ' + i + ' exited ' + code)));
+    }));
+  }
+  return Promise.all(kids).then(() => {
+    const recs = [];
+    for (const f of files) recs.push(...JSON.parse(fs.readFileSync(f, 'utf8')));
+    recs.sort((a, b) => a.index - b.index);
+    // distinct meshes and step data blocks
+    const meshes = {}, blocks = [], blockIdx = new Map();
+    const put = arr => {
+      const key = Buffer.from(Float64Array.from(arr).buffer).toString('base64');
+      if (!blockIdx.has(key)) { blockIdx.set(key, blocks.length); blocks.push(arr); }
+      return blockIdx.get(key);
+    };
+    const outCases = recs.map(r => {
+      if (!meshes[r.fingerprint]) meshes[r.fingerprint] = r.mesh;
+      const steps = r.steps.map(s => ({ Ta: s.Ta, Tb: s.Tb, stage: s.stage, branch: s.branch,
+        groups: s.groups.map(put), sections: s.sections.map(put) }));
+      return { id: r.id, side: r.side, value: r.value, base: r.base, mesh: r.fingerprint, steps, chain: r.chain, up: r.up,
+        down: r.down, temps: r.temps, peak: r.peak, outputs: r.outputs, iters: r.iters };
+    });
+    const base = outCases[0].outputs, byInput = {};
+    for (const inp of inputs) {
+      const lo = outCases.find(c => c.id === inp.id && c.side === 'lo'), hi = outCases.find(c => c.id === inp.id && c.side === 'hi');
+      byInput[inp.id] = { lo: lo.outputs, hi: hi.outputs, loValue: lo.value, hiValue: hi.value, baseValue: lo.base };
+    }
+    const tornado = get('tornadoFromCases')(inputs, base, byInput);
+    const offs = [0];
+    for (const b of blocks) offs.push(offs[offs.length - 1] + b.length);
+    const flat = new Float64Array(offs[offs.length - 1]);
+    blocks.forEach((b, q) => flat.set(b, offs[q]));
+    f64gz(flat, 'tornado_blocks.f64.gz');
+    const rec = { preset: 'default', inputs: inputs.map(i => ({ id: i.id, label: i.label, rule: i.rule, path: i.path })),
+      cases: outCases, tornado, meshes, blockOffsets: offs };
+    fs.writeFileSync(path.join(HERE, 'tornado_golden.json.gz'), zlib.gzipSync(Buffer.from(JSON.stringify(rec)), { level: 9 }));
+    console.log('tornado:', outCases.length, 'cases,', Object.keys(meshes).length, 'meshes,', blocks.length, 'blocks,', flat.length, 'values');
+    for (const k of Object.keys(tornado)) console.log(k, tornado[k].rows.map(r => r.id + ' ' + r.lo.toExponential(3) + ' ' + r.hi.toExponential(3)).join(' | '));
+  });
+}
+
+// What the worst-bump dW output of each tornado case needs beside its solved path (screenBumps /
+// screenJoints over the flip-chip bump layer, src/core/13-fatigue.js:359-364, 318-323): the bump
+// layer's mid-plane and the dies' bump fields of the case's geometry, the cycling profile, and
+// the solder's Anand constants and 1 C property tables (the same for every case here).
+// Added to tornado_golden.json.gz by the argument tornado_screen (after tornado).
+function tornadoScreen() {
+  const Analysis = get('Analysis'), findMaterial = get('findMaterial'), AnandPoint = get('AnandPoint');
+  const file = path.join(HERE, 'tornado_golden.json.gz');
+  const rec = JSON.parse(zlib.gunzipSync(fs.readFileSync(file)).toString());
+  const { cases } = tornadoSetup();
+  let solder = null;
+  rec.cases.forEach((rc, q) => {
+    const c = cases[q];
+    if (c.id !== rc.id || c.side !== rc.side) throw new Error('tornado case order changed');
+    const an = new Analysis(c.cfg, { preset: 'draft', evaluations: 'sensitivity' });
+    an.build();
+    const g = an.mesh.geom;
+    rc.screen = { zmid: g.zt + 0.5 * g.so, cycling: c.cfg.cycling,
+      dies: g.dies.map(d => ({ nbx: d.nbx, nby: d.nby, bx0: d.bx0, by0: d.by0, px: d.px, py: d.py, peripheral: !!d.peripheral, rows: d.rows || 0 })) };
+    const pt = new AnandPoint(findMaterial(c.cfg.materials, c.cfg.bump.solder));
+    const s = { params: { A: pt.A, QR: pt.QR, xi: pt.xi, m: pt.m, shat: pt.shat, n: pt.n, h0: pt.h0, a: pt.a, s0: pt.s0, nu: pt.nu },
+      E: list(pt.tables.E), F: list(pt.tables.F) };
+    if (!solder) solder = s;
+    else if (JSON.stringify(s) !== JSON.stringify(solder)) throw new Error('solder differs between tornado cases');
+  });
+  rec.solder = solder;
+  fs.writeFileSync(file, zlib.gzipSync(Buffer.from(JSON.stringify(rec)), { level: 9 }));
+  console.log('tornado screening data:', rec.cases.length, 'cases, dies per case', rec.cases[0].screen.dies.length);
+}
+
 const what = process.argv.slice(3);
+if (what[0] === 'tornado_part') tornadoPart(+what[1], +what[2], what[3]);
+if (what.includes('tornado_screen')) tornadoScreen();
+if (what.includes('tornado')) tornadoCases(+(what[what.indexOf('tornado') + 1]) || 4);
 if (what.includes('package')) packageCases();
 if (what.includes('package_modules')) modulesCase();
 if (what.includes('ic')) { for (const n of ['small_box', 'plate_16', 'cfg1']) icCases(n); fallbackCases(); }

@@ -1,0 +1,133 @@
+"""The reference's tornado (runSensitivityCase / sensitivityOutputs /
+tornadoFromCases, src/core/14-sensitivity.js:8-172) on the device: the cases of
+tests/golden/tornado_golden.json.gz (the lidded default preset at the Draft
+mesh: the base case and the low and high case of sixteen of its inputs, from
+datasheet ranges, +-10 % material, layer-ratio and geometry factors and +-10 C
+process and Tg temperatures), each run from its own plan, step data and mesh
+through the device path, give the reference's outputs, and the tornado built
+from them has the reference's signs and ranking.
+
+Bar: warpages within 1e-5 of the base warpage, die stresses and the
+worst-bump dW proxy within 1e-4 of the base value (the reference solves at its
+PCG tolerance of 1e-8, the device at 1e-10; what is left is the reference's
+own solver error); every tornado row's low and high change with the
+reference's sign wherever the reference's change exceeds twice that bar; the
+swings within four times the bar; the order of the rows by swing the
+reference's, except between rows whose reference swings differ by less than
+four times the bar."""
+import math
+
+import numpy as np
+import pytest
+
+import tornado_case as tc
+from fcbga_amd import hip_solver as hs
+from fcbga_amd import sensitivity as sens
+
+FIELD_KEYS = ("warpRT", "warpPeak", "dieStress", "dW")
+TOL = {"warpRT": 1e-5, "warpPeak": 1e-5, "dieStress": 1e-4, "dW": 1e-4}
+
+
+def test_tornado_table_restatement():
+    """tornado_from_cases on the reference's own case outputs gives the reference's table."""
+    g, _ = tc.load()
+    base = g["cases"][0]["outputs"]
+    by = {}
+    for inp in g["inputs"]:
+        lo = next(c for c in g["cases"] if c["id"] == inp["id"] and c["side"] == "lo")
+        hi = next(c for c in g["cases"] if c["id"] == inp["id"] and c["side"] == "hi")
+        by[inp["id"]] = {"lo": lo["outputs"], "hi": hi["outputs"], "loValue": lo["value"], "hiValue": hi["value"],
+                         "baseValue": lo["base"]}
+    t = sens.tornado_from_cases(g["inputs"], base, by)
+    for k in sens.OUTPUT_KEYS:
+        ref = g["tornado"][k]
+        assert t[k]["base"] == ref["base"]
+        assert [r["id"] for r in t[k]["all"]] == [r["id"] for r in ref["all"]], k
+        for a, b in zip(t[k]["all"], ref["all"]):
+            assert (a["lo"], a["hi"], a["swing"], a["loValue"], a["hiValue"]) == (b["lo"], b["hi"], b["swing"],
+                                                                                  b["loValue"], b["hiValue"])
+        assert [r["id"] for r in t[k]["rows"]] == [r["id"] for r in ref["rows"]]
+
+
+def test_fixture_cases_cover_every_kind():
+    g, _ = tc.load()
+    rules = {i["rule"] for i in g["inputs"]}
+    assert rules == {"range", "rel", "dT"}
+    assert len(g["cases"]) == 1 + 2 * len(g["inputs"])
+    # geometry inputs bring meshes of their own; the plans of Tg and process inputs differ from the base's
+    assert len(g["meshes"]) >= 3
+    base_plan = [(s["Ta"], s["Tb"], s["stage"]) for s in g["cases"][0]["steps"]]
+    moved = {c["id"] for c in g["cases"][1:] if [(s["Ta"], s["Tb"], s["stage"]) for s in c["steps"]] != base_plan}
+    assert {"ufTg", "ufTsf", "Tattach"} <= moved
+
+
+def _ranking_ok(mine, ref, tol):
+    """Row order by swing: the reference's, up to swaps of rows whose reference
+    swings differ by less than tol."""
+    pos = {r["id"]: q for q, r in enumerate(ref)}
+    sw = {r["id"]: r["swing"] for r in ref}
+    ids = [r["id"] for r in mine]
+    for a in range(len(ids)):
+        for b in range(a + 1, len(ids)):
+            if pos[ids[a]] > pos[ids[b]] and abs(sw[ids[a]] - sw[ids[b]]) > tol:
+                return False, (ids[a], ids[b])
+    return True, None
+
+
+def _run(variant, cases_sel, tol, precision):
+    g, blocks = tc.load()
+    models = {}
+    step_of = tc.step_of(g, blocks)
+    outs = {}
+    for q in cases_sel:
+        c = g["cases"][q]
+        key = c["mesh"]
+        if key not in models:
+            models[key] = (hs.Model(tc.mesh(g, key), variant=variant), tc.mesh(g, key))
+        M, m = models[key]
+        print("tornado case", q, c["id"], c["side"], len(c["steps"]), "steps", flush=True)
+        out, recs = sens.run_case(M, m, c["steps"], tc.plan(c), step_of, c["temps"]["all"], tc.bga(g, key),
+                                  tc.geom(g, key), c["peak"], tol=tol, precision=precision,
+                                  screen=tc.screen(g, c))
+        outs[q] = out
+    return g, outs
+
+
+def _check_outputs(g, outs):
+    base = g["cases"][0]["outputs"]
+    worst = {k: 0.0 for k in FIELD_KEYS}
+    for q, out in outs.items():
+        ref = g["cases"][q]["outputs"]
+        for k in FIELD_KEYS:
+            worst[k] = max(worst[k], abs(out[k] - ref[k]) / abs(base[k]))
+            assert abs(out[k] - ref[k]) <= TOL[k] * abs(base[k]), (q, g["cases"][q]["id"], k, out[k], ref[k])
+    print("largest deviation from the reference's outputs, relative to the base value:",
+          {k: f"{v:.1e}" for k, v in worst.items()}, flush=True)
+
+
+@pytest.mark.gpu
+def test_gpu_tornado_matches_reference():
+    """Every case on the device (mixed precision, tol 1e-10); outputs against the
+    reference's, then the tornado rows: signs and ranking."""
+    g, _ = tc.load()
+    g, outs = _run("hip", range(len(g["cases"])), 1e-10, hs.PREC_MIXED)
+    _check_outputs(g, outs)
+    base = outs[0]
+    by = {}
+    for inp in g["inputs"]:
+        ql = next(q for q, c in enumerate(g["cases"]) if c["id"] == inp["id"] and c["side"] == "lo")
+        qh = next(q for q, c in enumerate(g["cases"]) if c["id"] == inp["id"] and c["side"] == "hi")
+        by[inp["id"]] = {"lo": outs[ql], "hi": outs[qh]}
+    t = sens.tornado_from_cases(g["inputs"], base, by, keys=FIELD_KEYS)
+    rbase = g["cases"][0]["outputs"]
+    for k in FIELD_KEYS:
+        bar = TOL[k] * abs(rbase[k])
+        ref = {r["id"]: r for r in g["tornado"][k]["all"]}
+        for r in t[k]["all"]:
+            rr = ref[r["id"]]
+            for side in ("lo", "hi"):
+                if abs(rr[side]) > 2 * bar:
+                    assert math.copysign(1, r[side]) == math.copysign(1, rr[side]), (k, r["id"], side, r[side], rr[side])
+            assert abs(r["swing"] - rr["swing"]) <= 4 * bar, (k, r["id"], r["swing"], rr["swing"])
+        ok, pair = _ranking_ok(t[k]["all"], g["tornado"][k]["all"], 4 * bar)
+        assert ok, (k, pair)
