@@ -190,6 +190,15 @@ struct DevShell {
   DBuf<float> pW32;
   // children of this level's aggregates in the finer shell level
   DBuf<int32_t> chPtr, chIdx, chQ;
+  // two-phase Galerkin product from the finer shell level (gal2): T = A_f P
+  // by fine rows (tPtr: T blocks of each fine row, tsPtr/tSlot: the T block of
+  // every (fine block, parent) pair in walk order), then A = P^T T by coarse
+  // rows (csPtr/cSlot: the row's block of every (child, T block) pair, -1 when
+  // outside the pattern)
+  bool gal2 = false;
+  int galMaxT = 0;
+  DBuf<int32_t> tPtr, tsPtr, tSlot, csPtr, cSlot;
+  DBuf<double> galT;
   // vectors
   DBuf<double> b64, x64, r64;
   DBuf<float> b32, x32, r32;
@@ -1323,6 +1332,66 @@ void fine_color_sweep(fcbga_model *M, const FineOps<T> &o, const int32_t *cols, 
 // multigrid numeric set-up
 // ---------------------------------------------------------------------------
 
+// Index lists of the two-phase Galerkin product lv[l] -> lv[l+1] (see
+// DevShell::gal2).  T(f, J) = sum over the blocks (f, g) of row f and the
+// parents J of g of A_fg W_g,J; A(I, J) = sum over the children f of I of
+// W_f,I^T T(f, J), for J in row I's pattern.  The T blocks of a fine row are
+// the distinct parents of its columns in first-seen order.
+constexpr int GAL2_MAXT = 96;   // T blocks of one fine row held in LDS
+constexpr int GAL_MAXROW = 64;  // blocks of one coarse row held in LDS
+
+void build_gal2(fcbga_model *M, int l, be::stream_t s) {
+  const mgs::Level &F = M->H.L[l], &Cl = M->H.L[l + 1];
+  DevShell &C = *M->lv[l + 1];
+  C.gal2 = false;
+  if (C.maxrow > GAL_MAXROW) return;
+  const int32_t nf = F.nagg, nc = Cl.nagg;
+  std::vector<int32_t> tPtr(nf + 1, 0), tsPtr(nf + 1, 0), tCol, tSlot;
+  std::vector<int32_t> mark(nc, -1), pos(nc, 0);
+  int maxT = 0;
+  for (int32_t f = 0; f < nf; f++) {
+    const int32_t t0 = (int32_t)tCol.size();
+    for (int32_t k = F.rowPtr[f]; k < F.rowPtr[f + 1]; k++) {
+      const int32_t gg = F.col[k];
+      for (int32_t pp = F.pPtr[gg]; pp < F.pPtr[gg + 1]; pp++) {
+        const int32_t J = F.pIdx[pp];
+        if (mark[J] != f) {
+          mark[J] = f;
+          pos[J] = (int32_t)tCol.size() - t0;
+          tCol.push_back(J);
+        }
+        tSlot.push_back(pos[J]);
+      }
+    }
+    tPtr[f + 1] = (int32_t)tCol.size();
+    tsPtr[f + 1] = (int32_t)tSlot.size();
+    maxT = std::max(maxT, tPtr[f + 1] - t0);
+    if (tSlot.size() > (size_t)INT32_MAX / 2) return;
+  }
+  if (maxT > GAL2_MAXT) return;
+  std::vector<int32_t> csPtr(nc + 1, 0), cSlot, rmark(nc, -1), rpos(nc, 0);
+  for (int32_t I = 0; I < nc; I++) {
+    for (int32_t q = Cl.rowPtr[I]; q < Cl.rowPtr[I + 1]; q++) {
+      rmark[Cl.col[q]] = I;
+      rpos[Cl.col[q]] = q - Cl.rowPtr[I];
+    }
+    for (int32_t h = Cl.chPtr[I]; h < Cl.chPtr[I + 1]; h++) {
+      const int32_t f = Cl.chIdx[h];
+      for (int32_t t = tPtr[f]; t < tPtr[f + 1]; t++) cSlot.push_back(rmark[tCol[t]] == I ? rpos[tCol[t]] : -1);
+    }
+    csPtr[I + 1] = (int32_t)cSlot.size();
+    if (cSlot.size() > (size_t)INT32_MAX / 2) return;
+  }
+  C.tPtr.upload(tPtr, s);
+  C.tsPtr.upload(tsPtr, s);
+  C.tSlot.upload(tSlot, s);
+  C.csPtr.upload(csPtr, s);
+  C.cSlot.upload(cSlot, s);
+  C.galT.alloc((size_t)36 * tPtr[nf]);
+  C.galMaxT = maxT;
+  C.gal2 = true;
+}
+
 void build_hierarchy(fcbga_model *M) {
   if (M->hierBuilt) return;
   be::stream_t s = M->stream;
@@ -1445,6 +1514,7 @@ void build_hierarchy(fcbga_model *M) {
     D->b32.alloc(n); D->x32.alloc(n); D->r32.alloc(n);
     M->lv.push_back(std::move(D));
   }
+  for (int l = 0; l + 1 < nl; l++) build_gal2(M, l, s);
   be::sync(s);
   M->hierBuilt = true;
 }
@@ -1591,7 +1661,6 @@ void galerkin_level1(fcbga_model *M) {
 
 // Galerkin product between shell levels l (fine) and l+1 (coarse)
 #if FCBGA_GPU
-constexpr int GAL_MAXROW = 64;  // blocks of one coarse row held in LDS
 
 // One 64-lane block per coarse row I: the row's blocks accumulate in LDS
 // while the wave walks the (child f, fine block k, parent pp of col k)
@@ -1659,9 +1728,152 @@ __global__ void __launch_bounds__(64) k_galerkin_shell(
 }
 #endif
 
+// Two-phase Galerkin product (DevShell::gal2), one 64-lane block per row:
+// lane (r, c), r, c < 6, owns entry (r, c) of every block of the row's
+// accumulator in LDS, so the lanes never share an LDS word and need no barrier.
+// Phase 1, fine row f: T(f, .) = sum over (block k, parent pp) of A_k W_pp.
+// Phase 2, coarse row I: A(I, .) = sum over (child h, T block t) of W_h^T T_t.
+#if FCBGA_GPU
+__global__ void __launch_bounds__(64) k_gal_ap(const int32_t *__restrict__ frp, const int32_t *__restrict__ fcol,
+                                               const int32_t *__restrict__ pPtr, const double *__restrict__ W,
+                                               const double *__restrict__ fval, const int32_t *__restrict__ tPtr,
+                                               const int32_t *__restrict__ tsPtr, const int32_t *__restrict__ tSlot,
+                                               double *__restrict__ T) {
+  extern __shared__ double acc[];  // galMaxT x 36
+  const int32_t f = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (lane >= 36) return;
+  const int r = lane / 6, c = lane % 6;
+  const int32_t t0 = tPtr[f], nt = tPtr[f + 1] - t0;
+  for (int32_t q = 0; q < nt; q++) acc[36 * q + lane] = 0.0;
+  int32_t idx = tsPtr[f];
+  for (int32_t k = frp[f]; k < frp[f + 1]; k++) {
+    const int32_t gg = fcol[k];
+    const double *K = fval + 36 * (int64_t)k + 6 * r;
+    double kr[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) kr[q] = K[q];
+    for (int32_t pp = pPtr[gg]; pp < pPtr[gg + 1]; pp++) {
+      const int32_t sl = tSlot[idx++];
+      const double *Wg = W + 36 * (int64_t)pp + c;
+      double tm = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; q++) tm += kr[q] * Wg[6 * q];
+      acc[36 * sl + lane] += tm;
+    }
+  }
+  for (int32_t q = 0; q < nt; q++) T[36 * ((int64_t)t0 + q) + lane] = acc[36 * q + lane];
+}
+
+__global__ void __launch_bounds__(64) k_gal_ptap(const int32_t *__restrict__ crp, const int32_t *__restrict__ chPtr,
+                                                 const int32_t *__restrict__ chIdx, const int32_t *__restrict__ chQ,
+                                                 const double *__restrict__ W, const int32_t *__restrict__ tPtr,
+                                                 const double *__restrict__ T, const int32_t *__restrict__ csPtr,
+                                                 const int32_t *__restrict__ cSlot, double *__restrict__ val,
+                                                 float *__restrict__ val32) {
+  extern __shared__ double acc[];  // maxrow x 36
+  const int32_t I = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (lane >= 36) return;
+  const int a = lane / 6, c = lane % 6;
+  const int32_t r0 = crp[I], nblk = crp[I + 1] - r0;
+  for (int32_t q = 0; q < nblk; q++) acc[36 * q + lane] = 0.0;
+  int32_t idx = csPtr[I];
+  for (int32_t h = chPtr[I]; h < chPtr[I + 1]; h++) {
+    const int32_t f = chIdx[h];
+    const double *Wf = W + 36 * (int64_t)chQ[h] + a;
+    double wfa[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) wfa[q] = Wf[6 * q];  // W_f(q, a)
+    for (int32_t t = tPtr[f]; t < tPtr[f + 1]; t++) {
+      const int32_t sl = cSlot[idx++];
+      if (sl < 0) continue;
+      const double *Tt = T + 36 * (int64_t)t + c;
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; q++) v += wfa[q] * Tt[6 * q];
+      acc[36 * sl + lane] += v;
+    }
+  }
+  for (int32_t q = 0; q < nblk; q++) {
+    const double v = acc[36 * q + lane];
+    val[36 * ((int64_t)r0 + q) + lane] = v;
+    val32[36 * ((int64_t)r0 + q) + lane] = (float)v;
+  }
+}
+#endif
+
+void galerkin_shell_2phase(fcbga_model *M, int l) {
+  DevShell &F = *M->lv[l];
+  DevShell &C = *M->lv[l + 1];
+  const int32_t *frp = F.rowPtr.p, *fcol = F.col.p, *pPtr = F.pPtr.p;
+  const int32_t *tPtr = C.tPtr.p, *tsPtr = C.tsPtr.p, *tSlot = C.tSlot.p, *csPtr = C.csPtr.p, *cSlot = C.cSlot.p;
+  const int32_t *crp = C.rowPtr.p, *chPtr = C.chPtr.p, *chIdx = C.chIdx.p, *chQ = C.chQ.p;
+  const double *W = F.pW64.p, *fval = F.val64.p;
+  double *T = C.galT.p, *val = C.val64.p;
+  float *val32 = C.val32.p;
+#if FCBGA_GPU
+  hipLaunchKernelGGL(k_gal_ap, dim3((unsigned)F.nagg), dim3(64), (size_t)C.galMaxT * 36 * sizeof(double), M->stream,
+                     frp, fcol, pPtr, W, fval, tPtr, tsPtr, tSlot, T);
+  BE_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_gal_ptap, dim3((unsigned)C.nagg), dim3(64),
+                     (size_t)(C.maxrow > 0 ? C.maxrow : 1) * 36 * sizeof(double), M->stream, crp, chPtr, chIdx, chQ,
+                     W, tPtr, T, csPtr, cSlot, val, val32);
+  BE_CHECK(hipGetLastError());
+#else
+  // the same sums in the same order, one fine row / coarse row per task
+  be::pfor(F.nagg, M->stream, [=] __host__ __device__(int64_t f) {
+    const int32_t t0 = tPtr[f], nt = tPtr[f + 1] - t0;
+    double *Tf = T + 36 * (int64_t)t0;
+    for (int32_t q = 0; q < 36 * nt; q++) Tf[q] = 0.0;
+    int32_t idx = tsPtr[f];
+    for (int32_t k = frp[f]; k < frp[f + 1]; k++) {
+      const int32_t gg = fcol[k];
+      const double *K = fval + 36 * (int64_t)k;
+      for (int32_t pp = pPtr[gg]; pp < pPtr[gg + 1]; pp++) {
+        double *Ts = Tf + 36 * tSlot[idx++];
+        const double *Wg = W + 36 * (int64_t)pp;
+        for (int r = 0; r < 6; r++)
+          for (int c = 0; c < 6; c++) {
+            double tm = 0.0;
+            for (int q = 0; q < 6; q++) tm += K[6 * r + q] * Wg[6 * q + c];
+            Ts[6 * r + c] += tm;
+          }
+      }
+    }
+  });
+  be::pfor(C.nagg, M->stream, [=] __host__ __device__(int64_t I) {
+    const int32_t r0 = crp[I], nblk = crp[I + 1] - r0;
+    double *A = val + 36 * (int64_t)r0;
+    for (int32_t q = 0; q < 36 * nblk; q++) A[q] = 0.0;
+    int32_t idx = csPtr[I];
+    for (int32_t h = chPtr[I]; h < chPtr[I + 1]; h++) {
+      const int32_t f = chIdx[h];
+      const double *Wf = W + 36 * (int64_t)chQ[h];
+      for (int32_t t = tPtr[f]; t < tPtr[f + 1]; t++) {
+        const int32_t sl = cSlot[idx++];
+        if (sl < 0) continue;
+        const double *Tt = T + 36 * (int64_t)t;
+        for (int a = 0; a < 6; a++)
+          for (int c = 0; c < 6; c++) {
+            double v = 0.0;
+            for (int q = 0; q < 6; q++) v += Wf[6 * q + a] * Tt[6 * q + c];
+            A[36 * sl + 6 * a + c] += v;
+          }
+      }
+    }
+    for (int32_t q = 0; q < 36 * nblk; q++) val32[36 * (int64_t)r0 + q] = (float)A[q];
+  });
+#endif
+}
+
 void galerkin_shell(fcbga_model *M, int l) {
   DevShell &F = *M->lv[l];
   DevShell &C = *M->lv[l + 1];
+  if (C.gal2) {
+    galerkin_shell_2phase(M, l);
+    return;
+  }
   const int32_t *brow = C.blockRow.p, *bcol = C.col.p, *chPtr = C.chPtr.p, *chIdx = C.chIdx.p, *chQ = C.chQ.p;
   const int32_t *frp = F.rowPtr.p, *fcol = F.col.p, *pPtr = F.pPtr.p, *pIdx = F.pIdx.p;
   const double *fval = F.val64.p, *W = F.pW64.p;
