@@ -413,6 +413,36 @@ int fcbga_path_load(fcbga_model *m, int32_t k, const float *du);
 int fcbga_path_load64(fcbga_model *m, int32_t k, const double *du);
 int fcbga_path_copy(fcbga_model *dst, const fcbga_model *src, int32_t k);
 
+/* Step material data from 1 C property grids: FEModel.prepareStep (src/core/09-assembly.js:346-365)
+ * with gridMean / gridStressIncrement (src/core/03-materials.js:370-409) and sectionWmean /
+ * sectionVstep (src/core/04-homogenize.js:206-221).  A grid holds FCBGA_TGRID_N points, one per
+ * degree C from FCBGA_TGRID_MIN, linear between them and clamped outside.  Host only: it fills
+ * the fcbga_step arrays of a step [Ta, Tb] in `stage`. */
+#define FCBGA_TGRID_MIN (-80)
+#define FCBGA_TGRID_N 381
+typedef struct fcbga_group_grids {
+  int32_t stage;        /* birth stage: born in every stage >= stage (FEModel.isBorn) */
+  int32_t layered;      /* 1: a layered group (its elements read their sections; D, s0 stay 0) */
+  const double *Dgrid;  /* FCBGA_TGRID_N x 36 stiffness (MPa), or NULL for a constant one */
+  const double *Dconst; /* 36, when Dgrid is NULL */
+  const double *F;      /* FCBGA_TGRID_N x 3 normal thermal strains from the strain origin */
+} fcbga_group_grids;
+typedef struct fcbga_section_grids {
+  int32_t ns;           /* sub-layers */
+  int32_t group;        /* group whose birth the section follows, -1: always born */
+  const double *W;      /* FCBGA_TGRID_N x 108 zeta-station stiffness weights */
+  const double *D;      /* ns x FCBGA_TGRID_N x 36 sub-layer stiffness */
+  const double *F;      /* ns x FCBGA_TGRID_N x 3 sub-layer thermal strains */
+  const double *Ls;     /* ns x 3 Lagrange integrals of the sub-layer's thickness range */
+} fcbga_section_grids;
+/* D (ngroups*36), s0 (ngroups*6), born (ngroups), secW (nsections*108), secV (nsections*18) and
+ * sec_born (nsections) of the step: the mean stiffness (a constant one as it is), the thermal
+ * stress increment of born groups (0 for the unborn), and per section the mean weights, the
+ * thermal stress weights (0 when unborn) and the birth. */
+int fcbga_prepare_step(int32_t ngroups, const fcbga_group_grids *groups, int32_t nsections,
+                       const fcbga_section_grids *sections, double Ta, double Tb, int32_t stage, double *D,
+                       double *s0, uint8_t *born, double *secW, double *secV, uint8_t *sec_born);
+
 /* meshFingerprint (src/core/16-results-file.js:104-119): the 14-hex-digit hash a results file
  * carries of the mesh its increments belong to, over the bytes of xs, ys, zs (FP64), eGroup
  * (int32), ePart (int8) and eScale (FP64), seeded with nn and ne.  Host only; out receives 14

@@ -15,6 +15,7 @@
 #include "backend.h"
 #include "fe_core.h"
 #include "mg_setup.h"
+#include "step_data.h"
 
 #include <chrono>
 #include <cmath>
@@ -4884,6 +4885,57 @@ int fcbga_path_copy(fcbga_model *dst, const fcbga_model *src, int32_t k) {
     be::sync(dst->stream);
   } catch (const std::exception &e) {
     return fail(e, FCBGA_ERR_DEVICE);
+  }
+  return FCBGA_OK;
+}
+
+int fcbga_prepare_step(int32_t ngroups, const fcbga_group_grids *groups, int32_t nsections,
+                       const fcbga_section_grids *sections, double Ta, double Tb, int32_t stage, double *D,
+                       double *s0, uint8_t *born, double *secW, double *secV, uint8_t *sec_born) {
+  static_assert(FCBGA_TGRID_MIN == sd::TGRID_MIN && FCBGA_TGRID_N == sd::TGRID_N, "property grid");
+  if (ngroups < 0 || nsections < 0 || (ngroups && (!groups || !D || !s0 || !born)) ||
+      (nsections && (!sections || !secW || !secV || !sec_born))) {
+    g_err = "NULL argument or negative size";
+    return FCBGA_ERR_ARG;
+  }
+  if (!std::isfinite(Ta) || !std::isfinite(Tb)) {
+    g_err = "step temperatures must be finite";
+    return FCBGA_ERR_ARG;
+  }
+  for (int32_t q = 0; q < ngroups; q++) {
+    const fcbga_group_grids &g = groups[q];
+    if (!g.layered && (!g.F || (!g.Dgrid && !g.Dconst))) {
+      g_err = "group " + std::to_string(q) + ": thermal strain grid and a stiffness grid or constant required";
+      return FCBGA_ERR_ARG;
+    }
+  }
+  for (int32_t q = 0; q < nsections; q++) {
+    const fcbga_section_grids &c = sections[q];
+    if (c.ns < 0 || c.group < -1 || c.group >= ngroups || !c.W || (c.ns && (!c.D || !c.F || !c.Ls))) {
+      g_err = "section " + std::to_string(q) + ": grids missing, negative sub-layer count or group out of range";
+      return FCBGA_ERR_ARG;
+    }
+  }
+  for (int32_t q = 0; q < ngroups; q++) {
+    const fcbga_group_grids &g = groups[q];
+    double *Dq = D + 36 * (int64_t)q, *sq = s0 + 6 * (int64_t)q;
+    born[q] = g.stage <= stage ? 1 : 0;
+    for (int r = 0; r < 6; r++) sq[r] = 0;
+    if (g.layered) {
+      for (int r = 0; r < 36; r++) Dq[r] = 0;
+      continue;
+    }
+    if (g.Dgrid) sd::step_mean(g.Dgrid, 36, 36, Ta, Tb, Dq);
+    else for (int r = 0; r < 36; r++) Dq[r] = g.Dconst[r];
+    if (born[q]) sd::step_stress(g.Dgrid, g.Dconst, g.F, Ta, Tb, sq);
+  }
+  for (int32_t q = 0; q < nsections; q++) {
+    const fcbga_section_grids &c = sections[q];
+    sd::step_mean(c.W, 108, 108, Ta, Tb, secW + 108 * (int64_t)q);
+    sec_born[q] = c.group < 0 || groups[c.group].stage <= stage ? 1 : 0;
+    double *V = secV + 18 * (int64_t)q;
+    if (sec_born[q]) sd::section_stress_weights(c.ns, c.D, c.F, c.Ls, Ta, Tb, V);
+    else for (int r = 0; r < 18; r++) V[r] = 0;
   }
   return FCBGA_OK;
 }

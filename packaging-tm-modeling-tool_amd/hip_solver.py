@@ -48,6 +48,14 @@ class FcbgaStep(C.Structure):
                 ("born", _u8p), ("secW", _f64p), ("secV", _f64p), ("sec_born", _u8p)]
 
 
+class FcbgaGroupGrids(C.Structure):
+    _fields_ = [("stage", C.c_int32), ("layered", C.c_int32), ("Dgrid", _f64p), ("Dconst", _f64p), ("F", _f64p)]
+
+
+class FcbgaSectionGrids(C.Structure):
+    _fields_ = [("ns", C.c_int32), ("group", C.c_int32), ("W", _f64p), ("D", _f64p), ("F", _f64p), ("Ls", _f64p)]
+
+
 class FcbgaSolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxit", C.c_int32), ("precision", C.c_int32), ("precond", C.c_int32),
                 ("sweeps", C.c_int32), ("warm", C.c_int32), ("smoother", C.c_int32)]
@@ -94,7 +102,7 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_vp_residual", "fcbga_set_progress_cb", "fcbga_vp_set_prescribed",
            "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
            "fcbga_path_load64", "fcbga_path_copy",
-           "fcbga_mesh_fingerprint", "fcbga_last_error"]
+           "fcbga_mesh_fingerprint", "fcbga_prepare_step", "fcbga_last_error"]
 
 ANAND_STATE = 11
 STATE = 81  # stress-history state of one element (FCBGA_STATE)
@@ -210,6 +218,9 @@ def load(variant: str = "hip"):
     L.fcbga_mesh_fingerprint.argtypes = [C.c_int32, C.c_int32, _f64p, C.c_int32, _f64p, C.c_int32, _f64p, C.c_int32,
                                          _i32p, C.POINTER(C.c_int8), _f64p, C.c_char_p]
     L.fcbga_mesh_fingerprint.restype = C.c_int
+    L.fcbga_prepare_step.argtypes = [C.c_int32, C.POINTER(FcbgaGroupGrids), C.c_int32, C.POINTER(FcbgaSectionGrids),
+                                     C.c_double, C.c_double, C.c_int32, _f64p, _f64p, _u8p, _f64p, _f64p, _u8p]
+    L.fcbga_prepare_step.restype = C.c_int
     L.fcbga_warm_reset.argtypes = [C.c_void_p]
     L.fcbga_warm_reset.restype = C.c_int
     L.fcbga_set_progress_cb.argtypes = [C.c_void_p, PROGRESS_FN, C.c_void_p, C.c_int32]
@@ -537,6 +548,50 @@ class Model:
         if rc != 0:
             raise SolverError(last_error(self.L))
         return y
+
+
+TGRID_MIN, TGRID_N = -80, 381  # FCBGA_TGRID_MIN, FCBGA_TGRID_N
+
+
+def prepare_step(groups, sections, Ta, Tb, stage, variant="hip"):
+    """Step data of [Ta, Tb] in `stage` from 1 degC property grids (fcbga_prepare_step:
+    FEModel.prepareStep, src/core/09-assembly.js:346-365).  groups: dicts with stage,
+    layered and, for the others, Dgrid (TGRID_N x 36) or Dconst (36) and F (TGRID_N x 3);
+    sections: dicts with ns, group (-1: always born), W (TGRID_N x 108), D (ns x TGRID_N x 36),
+    F (ns x TGRID_N x 3) and Ls (ns x 3).  Returns a Step."""
+    L = load(variant)
+    keep = []
+
+    def arr(a, n):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, np.float64).reshape(-1)
+        if len(a) != n:
+            raise SolverError(f"grid of {len(a)} values where {n} are expected")
+        keep.append(a)
+        return _ptr(a, _f64p)
+    ng, ns = len(groups), len(sections)
+    G = (FcbgaGroupGrids * max(ng, 1))()
+    for q, g in enumerate(groups):
+        G[q].stage, G[q].layered = int(g["stage"]), 1 if g.get("layered") else 0
+        G[q].Dgrid = arr(g.get("Dgrid"), TGRID_N * 36)
+        G[q].Dconst = arr(g.get("Dconst"), 36)
+        G[q].F = arr(g.get("F"), TGRID_N * 3)
+    S = (FcbgaSectionGrids * max(ns, 1))()
+    for q, c in enumerate(sections):
+        n = int(c["ns"])
+        S[q].ns, S[q].group = n, int(c["group"])
+        S[q].W = arr(c["W"], TGRID_N * 108)
+        S[q].D, S[q].F, S[q].Ls = arr(c["D"], n * TGRID_N * 36), arr(c["F"], n * TGRID_N * 3), arr(c["Ls"], n * 3)
+    D, s0, born = np.zeros((ng, 36)), np.zeros((ng, 6)), np.zeros(ng, np.uint8)
+    W, V, sb = np.zeros((ns, 108)), np.zeros((ns, 18)), np.zeros(ns, np.uint8)
+    rc = L.fcbga_prepare_step(ng, G, ns, S, float(Ta), float(Tb), int(stage), _ptr(D, _f64p), _ptr(s0, _f64p),
+                              _ptr(born, _u8p), _ptr(W, _f64p), _ptr(V, _f64p), _ptr(sb, _u8p))
+    if rc != 0:
+        raise SolverError(last_error(L))
+    if ns:
+        return Step(Ta, Tb, stage, D, s0, born, W, V, sb)
+    return Step(Ta, Tb, stage, D, s0, born)
 
 
 def step_for_spec(spec, Ta=None, Tb=None, stage=None):

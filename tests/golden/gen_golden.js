@@ -804,8 +804,52 @@ function tornadoScreen() {
   console.log('tornado screening data:', rec.cases.length, 'cases, dies per case', rec.cases[0].screen.dies.length);
 }
 
+// Step material data from the 1 C property grids (FEModel.prepareStep, src/core/09-assembly.js:346-365,
+// with gridMean / gridStressIncrement, src/core/03-materials.js:370-409, and sectionWmean /
+// sectionVstep, src/core/04-homogenize.js:206-221) on the lidded default preset: every group's
+// grids (stiffness grid or constant, thermal strain grid, birth stage, kind) and every layered
+// section's (weight grid, sub-layer stiffness and strain grids, Lagrange integrals, owner group),
+// then the prepared data of every step of the full thermal path and of steps that probe the edges
+// (a zero-length step, reversed and fractional steps, steps leaving the grid at either end).
+//   prepare_grids.f64.gz   every grid, concatenated in the order prepare_golden.json lists
+//   prepare_golden.json    the layout, the steps and their prepared data
+function prepareCase() {
+  const db = JSON.parse(fs.readFileSync(path.join(REF, 'src', 'materials-db.json'), 'utf8'));
+  const Analysis = get('Analysis'), sectionWmean = get('sectionWmean');
+  const cfg = get('presetConfig')(db, 'default');
+  const an = new Analysis(cfg, { preset: 'draft', evaluations: 'full' });
+  an.build();
+  const steps = an.buildPath(), model = an.model;
+  const flat = [];
+  const put = arr => { const o = flat.length; for (let i = 0; i < arr.length; i++) flat.push(arr[i]); return [o, arr.length]; };
+  const groups = model.groups.map(g => ({ kind: g.kind || 'homog', stage: g.stage,
+    Dgrid: g.kind === 'layered' || g.Dconst ? null : put(g.Dgrid), Dconst: g.kind !== 'layered' && g.Dconst ? put(g.Dconst) : null,
+    F: g.kind === 'layered' ? null : put(g.F) }));
+  const sections = model.sections.map(s => ({ ns: s.ns, group: s.group == null ? -1 : s.group, W: put(s.W), D: put(s.D), F: put(s.F),
+    Ls: put([].concat(...s.Ls.map(l => Array.from(l)))) }));
+  const probes = steps.map(s => [s.Ta, s.Tb, s.stage]).concat([[25, 25, 3], [25, 260, 3], [260, 25, 2], [24.3, 187.7, 1],
+    [187.7, 24.3, 0], [-95, -60, 3], [-60, -95, 3], [280, 320, 3], [-81.5, 301.25, 3], [150.5, 150.5, 1], [100, 99.999999, 2]]);
+  const W = new Float64Array(108);
+  const out = probes.map(([Ta, Tb, stage]) => {
+    model._prepKey = null;
+    model.prepareStep(Ta, Tb, stage);
+    const D = [], s0 = [], born = [];
+    for (const g of model.groups) {
+      if (g.kind === 'layered') { D.push(...new Float64Array(36)); s0.push(...new Float64Array(6)); born.push(model.isBorn(g, stage) ? 1 : 0); continue; }
+      D.push(...g._D); s0.push(...g._s0); born.push(g._born ? 1 : 0);
+    }
+    const secW = [], secV = [], secBorn = [];
+    for (const sec of model.sections) { sectionWmean(sec, Ta, Tb, W); secW.push(...W); secV.push(...sec._V); secBorn.push(sec._born ? 1 : 0); }
+    return { Ta, Tb, stage, D, s0, born, secW, secV, secBorn };
+  });
+  f64gz(flat, 'prepare_grids.f64.gz');
+  fs.writeFileSync(path.join(HERE, 'prepare_golden.json'), JSON.stringify({ preset: 'default', tgrid: [-80, 300], groups, sections, steps: out }));
+  console.log('prepare:', groups.length, 'groups,', sections.length, 'sections (sub-layers', sections.map(s => s.ns).join(' '), '),', flat.length, 'grid values,', out.length, 'steps');
+}
+
 const what = process.argv.slice(3);
 if (what[0] === 'tornado_part') tornadoPart(+what[1], +what[2], what[3]);
+if (what.includes('prepare')) prepareCase();
 if (what.includes('tornado_screen')) tornadoScreen();
 if (what.includes('tornado')) tornadoCases(+(what[what.indexOf('tornado') + 1]) || 4);
 if (what.includes('package')) packageCases();
