@@ -169,10 +169,11 @@ class HipStateView {
  * forward(step, objective) solves the step and returns J = [warpage proxy on
  * plane kTop, von Mises p-norm over objective.vmElems], keeping the context on
  * the model; backward(gJ) returns the log-derivatives of gJ . J for every
- * group's E and CTE and every element's CTE from one adjoint solve.  It takes
- * the place of the finite-difference tornado of the sensitivity study
- * (runSensitivityCase / tornadoFromCases, src/core/14-sensitivity.js:8-172):
- * dJ/dE_g = dlnE[g] / E_g, dJ/dCTE_g = dlnCte[g] / CTE_g.
+ * group's E and CTE and every element's CTE from one adjoint solve:
+ * dJ/dE_g = dlnE[g] / E_g, dJ/dCTE_g = dlnCte[g] / CTE_g.  The reference's
+ * finite-difference tornado (runSensitivityCase / tornadoFromCases,
+ * src/core/14-sensitivity.js:8-172) runs beside it unchanged, each case's
+ * solves going through install's solveStep.
  */
 class HipSensitivity {
   constructor(solver) { this.solver = solver; this.saved = false; }
