@@ -81,6 +81,38 @@ def traffic_from_profile(kernel):
         return None
 
 
+def profile_values(name):
+    """key = value lines of a committed counter summary under profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v.strip())
+            except ValueError:
+                pass
+    return vals
+
+
+SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9  # MI355X: 256 CUs x 4 SIMD16, wave64 FP32 op = 4 cycles
+
+
+def valu_issue_roofline(avg_launch_us):
+    """The fine sweep against its VALU issue bound: the committed counter figure
+    of VALU instructions per launch (profiles/k_fine_col_sweep_cfg2_valu_issue.txt)
+    x 4 cycles / 1,024 SIMDs at 2.4 GHz, over the measured launch time."""
+    v = profile_values("k_fine_col_sweep_cfg2_valu_issue.txt")
+    if not v or "valu_insts_per_launch" not in v:
+        return None
+    bound_us = v["valu_insts_per_launch"] * VALU_CYCLES / SIMDS / CLOCK_HZ * 1e6
+    return {"bound": "valu_issue", "bound_us": bound_us, "avg_launch_us": avg_launch_us,
+            "frac": bound_us / avg_launch_us, "valu_insts_per_wave": v["valu_insts_per_launch"] / v["waves_per_launch"],
+            "note": "FP32 VALU instructions per launch from SQ_INSTS_VALU; packed FP32 is not used"}
+
+
 def roofline_for(cls, ms_total, launches, mesh, dtype_cycle):
     """Roofline of a kernel class from its HIP-event time per launch.
     fine_sweep (the z-line smoother, one launch per colour): HBM-bound, 36 B per
@@ -98,7 +130,8 @@ def roofline_for(cls, ms_total, launches, mesh, dtype_cycle):
         tr = traffic_from_profile("k_fine_col_sweep")
         return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
                 "frac": achieved / PEAK["hbm"], "traffic": tr, "kernel": cls, "dtype": dtype_cycle,
-                "avg_launch_us": avg_s * 1e6, "launches": launches, "bytes_per_launch": nbytes}
+                "avg_launch_us": avg_s * 1e6, "launches": launches, "bytes_per_launch": nbytes,
+                "issue": valu_issue_roofline(avg_s * 1e6)}
     if cls == "apply":
         flops = 144.0 * inc.sum()
         peak, unit_dt = PEAK["fp64"], "fp64"
