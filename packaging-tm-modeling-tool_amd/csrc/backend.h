@@ -143,43 +143,51 @@ inline void pfor(int64_t n, stream_t s, F f, int block = 256) {
 }
 
 // ---- FP64 dot products --------------------------------------------------------
-// Two-pass reduction in a fixed order (deterministic for a given n).
+// Two-pass reduction in a fixed order (deterministic for a given n): per
+// block, each wavefront sums its lanes by a butterfly of lane exchanges and
+// the block adds its four wavefront sums in order.
 
 constexpr int RED_BLOCK = 256;
 constexpr int RED_GRID = 1024;  // blocks of the first pass: partial sums per dot
 
 #if FCBGA_GPU
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// the block's sum of v (RED_BLOCK threads), valid in thread 0; sm: RED_BLOCK / 64 doubles
+__device__ __forceinline__ double block_sum(double v, double *sm) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < RED_BLOCK / 64; w++) t += sm[w];
+  return t;
+}
+
 template <typename TA, typename TB>
 __global__ void __launch_bounds__(RED_BLOCK) k_dot_partial(int64_t n, const TA *__restrict__ a,
                                                            const TB *__restrict__ b, double *__restrict__ part) {
-  __shared__ double sm[RED_BLOCK];
+  __shared__ double sm[RED_BLOCK / 64];
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += (int64_t)RED_GRID * RED_BLOCK)
     s += (double)a[i] * (double)b[i];
-  sm[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = RED_BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ void __launch_bounds__(RED_BLOCK) k_sum_partials(int nslot, int n, const double *__restrict__ part,
-                                                            double *__restrict__ out) {
-  // nslot independent sums of n partials each (slot q at part[q * n ...])
-  __shared__ double sm[RED_BLOCK];
-  int q = blockIdx.x;
+// one sum per block q: the n partials at part[q * n ...] into out[q * ostride]
+__global__ void __launch_bounds__(RED_BLOCK) k_sum_partials(int n, const double *__restrict__ part,
+                                                            double *__restrict__ out, int ostride) {
+  __shared__ double sm[RED_BLOCK / 64];
+  const int q = blockIdx.x;
   double s = 0;
   for (int i = threadIdx.x; i < n; i += RED_BLOCK) s += part[(int64_t)q * n + i];
-  sm[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = RED_BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[q] = sm[0];
-  (void)nslot;
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) out[(int64_t)q * ostride] = s;
 }
 #endif
 
@@ -189,7 +197,7 @@ inline void dot_async(int64_t n, const TA *a, const TB *b, double *partials, dou
 #if FCBGA_GPU
   hipLaunchKernelGGL((k_dot_partial<TA, TB>), dim3(RED_GRID), dim3(RED_BLOCK), 0, s, n, a, b, partials);
   BE_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, s, 1, RED_GRID, partials, result);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, s, RED_GRID, partials, result, 1);
   BE_CHECK(hipGetLastError());
 #else
   (void)partials; (void)s;

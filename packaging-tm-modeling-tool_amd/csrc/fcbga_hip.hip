@@ -3589,7 +3589,7 @@ enum { SC_RR = 0, SC_RZ = 1, SC_PQ = 2, SC_RZO = 3, SC_RZN = 4, SC_ALPHA = 5, SC
 
 #if FCBGA_GPU
 // x += alpha p, r -= alpha q, and per block the partial sums of r.r and
-// r.z over the updated r (the grid-stride order and block tree of
+// r.z over the updated r (the grid-stride order and block reduction of
 // be::dot_async, so the sums are the same as two separate dots)
 __global__ void __launch_bounds__(be::RED_BLOCK) k_cg_update(int64_t n, const double *__restrict__ sc,
                                                             double *__restrict__ x, double *__restrict__ r,
@@ -3597,7 +3597,7 @@ __global__ void __launch_bounds__(be::RED_BLOCK) k_cg_update(int64_t n, const do
                                                             const double *__restrict__ q,
                                                             const double *__restrict__ z, double *__restrict__ part,
                                                             float *__restrict__ r32) {
-  __shared__ double s1[be::RED_BLOCK], s2[be::RED_BLOCK];
+  __shared__ double s1[be::RED_BLOCK / 64], s2[be::RED_BLOCK / 64];
   const double a = sc[SC_ALPHA];
   double rr = 0, rz = 0;
   for (int64_t i = (int64_t)blockIdx.x * be::RED_BLOCK + threadIdx.x; i < n; i += (int64_t)be::RED_GRID * be::RED_BLOCK) {
@@ -3608,19 +3608,11 @@ __global__ void __launch_bounds__(be::RED_BLOCK) k_cg_update(int64_t n, const do
     if (z) rz += ri * z[i];
     if (r32) r32[i] = (float)ri;
   }
-  s1[threadIdx.x] = rr;
-  s2[threadIdx.x] = rz;
-  __syncthreads();
-  for (int w = be::RED_BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      s1[threadIdx.x] += s1[threadIdx.x + w];
-      s2[threadIdx.x] += s2[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
+  rr = be::block_sum(rr, s1);
+  rz = be::block_sum(rz, s2);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = s1[0];
-    part[be::RED_GRID + blockIdx.x] = s2[0];
+    part[blockIdx.x] = rr;
+    part[be::RED_GRID + blockIdx.x] = rz;
   }
 }
 #endif
@@ -3631,13 +3623,10 @@ void cg_update(int64_t n3, double *sc, double *x, double *r, const double *p, co
 #if FCBGA_GPU
   hipLaunchKernelGGL(k_cg_update, dim3(be::RED_GRID), dim3(be::RED_BLOCK), 0, s, n3, sc, x, r, p, q, z, part, r32);
   BE_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(be::k_sum_partials, dim3(1), dim3(be::RED_BLOCK), 0, s, 1, be::RED_GRID, part, sc + SC_RR);
+  // r.r into SC_RR and, with z, r.z into SC_RZO: one launch, a block per sum
+  hipLaunchKernelGGL(be::k_sum_partials, dim3(z ? 2 : 1), dim3(be::RED_BLOCK), 0, s, be::RED_GRID, part, sc + SC_RR,
+                     SC_RZO - SC_RR);
   BE_CHECK(hipGetLastError());
-  if (z) {
-    hipLaunchKernelGGL(be::k_sum_partials, dim3(1), dim3(be::RED_BLOCK), 0, s, 1, be::RED_GRID, part + be::RED_GRID,
-                       sc + SC_RZO);
-    BE_CHECK(hipGetLastError());
-  }
 #else
   (void)part; (void)s;
   const double a = sc[SC_ALPHA];
