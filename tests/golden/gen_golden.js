@@ -649,6 +649,35 @@ function modulesCase() {
     bga: { x: list(bga.x), y: list(bga.y), zone: bga.zone || null }, steps: out, pathSteps: steps.length };
   fs.writeFileSync(path.join(HERE, 'package_modules_golden.json.gz'), zlib.gzipSync(Buffer.from(JSON.stringify(rec)), { level: 9 }));
   console.log('modules', m.nn * 3, 'DOF', m.ne, 'elements', model.sections.length, 'sections', Date.now() - t0, 'ms');
+  // the whole thermal path of the same package as the reference's analysis runs it (preview
+  // evaluations: module stages, then the package stages; warm starts along the path), with every
+  // step's prepared data, the plan, the 25 C field and its JEITA warpage, and the chiplet-join state
+  const t1 = Date.now();
+  const ap = new Analysis(cfg, { preset: 'draft', evaluations: 'preview', cap: 20000 });
+  ap.build();
+  const psteps = ap.buildPath();
+  const pm = ap.model, P = ap.path;
+  ap.run();
+  const stepData = psteps.map(s => {
+    pm._prepKey = null;
+    pm.prepareStep(s.Ta, s.Tb, s.stage);
+    const gD = [], gs0 = [], gborn = [];
+    for (const g of pm.groups) {
+      if (g.kind === 'layered') { gD.push(...new Float64Array(36)); gs0.push(...new Float64Array(6)); gborn.push(pm.isBorn(g, s.stage) ? 1 : 0); continue; }
+      gD.push(...g._D); gs0.push(...g._s0); gborn.push(g._born ? 1 : 0);
+    }
+    const sW = [], sV = [], sborn = [];
+    for (const sec of pm.sections) { sectionWmean(sec, s.Ta, s.Tb, W); sW.push(...W); sV.push(...sec._V); sborn.push(sec._born ? 1 : 0); }
+    return { Ta: s.Ta, Tb: s.Tb, stage: s.stage, branch: s.branch, D: gD, s0: gs0, born: gborn, secW: sW, secV: sV, secBorn: sborn };
+  });
+  const u25 = ap.fields.get(25), w25 = warpageJEITA(ap.mesh, ap.mesh.geom, u25, ap.cfg);
+  const prec = { plan: { chain: P.chain, up: P.up, down: P.down, chainJoin: P.chainJoin, cpi: P.cpi, hasCPI: P.hasCPI,
+    ddiJoin: P.ddiJoin, ddi: P.ddi, hasDDI: P.hasDDI }, temps: ap.temps.all, steps: stepData,
+    iters: ap.solver.log.map(l => l.iters), u25: list(u25), warpage25: w25.signed, uDDI: ap.uDDI ? list(ap.uDDI) : null,
+    same_mesh: ap.mesh.nn === m.nn && ap.mesh.ne === m.ne };
+  fs.writeFileSync(path.join(HERE, 'package_modules_path.json.gz'), zlib.gzipSync(Buffer.from(JSON.stringify(prec)), { level: 9 }));
+  console.log('modules path:', psteps.length, 'steps,', P.chain.filter(k => psteps[k].stage < 0).length, 'module steps in the chain, warpage',
+    w25.signed, 'ddi', !!ap.uDDI, Date.now() - t1, 'ms');
 }
 
 // The reference's tornado (runSensitivityCase / tornadoFromCases, src/core/14-sensitivity.js:8-172)

@@ -4,13 +4,15 @@ buildPackageMesh (graded lines, fill fractions, layered substrate bands),
 with the FEModel's prepared group and section data of two thermal-path steps
 (tests/golden/package_<name>_golden.json.gz, gen_golden.js 'package'), and a
 stacked-module package with steps of its module stages
-(package_modules_golden.json.gz, gen_golden.js 'package_modules').
+(package_modules_golden.json.gz, gen_golden.js 'package_modules') and its whole
+thermal path with the reference's 25 C field (package_modules_path.json.gz).
 
 Bar: the assembled load against the reference's to 1e-13, nodal
 displacements within 1e-6 of the largest displacement of the reference's
 solution (tol 1e-12) and JEITA warpage within 0.1 %, in FP64 and mixed
 precision; the adjoint pair runs on the layered mesh and its group
 gradients match central differences."""
+import gzip
 import os
 import shutil
 import subprocess
@@ -127,6 +129,38 @@ def test_gpu_module_stages(prec):
         f = M.load(st)
         u, rec = M.solve_step(st, tol=1e-10, precision=prec)
         _check_module_step(g, q, f, u, rec, 1e-6)
+
+
+def _modules_path():
+    with gzip.open(os.path.join(pc.GOLDEN, "package_modules_path.json.gz")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("variant", [pytest.param("hip", marks=pytest.mark.gpu), "hostemu"])
+def test_module_package_thermal_path(variant):
+    """The module package's whole thermal path as the reference's analysis runs it (preview
+    evaluations: chiplet join, wafer underfill and mold cure as module stages, then the
+    package stages), solved step by step with warm starts along the path (the stored
+    solutions of the module stages carry into the package stages) and summed on the device:
+    the 25 C field within 1e-6 of the largest displacement of the reference's and its JEITA
+    warpage within 0.1 %."""
+    from fcbga_amd import path as pth
+    g, m = pc.load("modules")
+    P = _modules_path()
+    assert P["same_mesh"]
+    plan = P["plan"]
+    assert any(P["steps"][k]["stage"] < 0 for k in plan["chain"]) and any(P["steps"][k]["stage"] >= 0 for k in plan["chain"])
+    M = hs.Model(m, variant=variant)
+
+    def step_of(s):
+        return hs.Step(s["Ta"], s["Tb"], s["stage"], np.array(s["D"]), np.array(s["s0"]), np.array(s["born"], np.uint8),
+                       np.array(s["secW"]), np.array(s["secV"]), np.array(s["secBorn"], np.uint8))
+    prec = hs.PREC_MIXED if variant == "hip" else hs.PREC_FP64
+    fields, _, _, recs = pth.run_path(M, P["steps"], plan, P["temps"], step_of, tol=1e-10, precision=prec)
+    u, uref = fields[25], np.array(P["u25"])
+    assert np.abs(u - uref).max() <= 1e-6 * np.abs(uref).max(), np.abs(u - uref).max() / np.abs(uref).max()
+    w = post.warpage_jeita(m, u, pc.bga(g))["signed"]
+    assert abs(w - P["warpage25"]) <= 1e-3 * abs(P["warpage25"]), (w, P["warpage25"])
 
 
 @pytest.mark.gpu
