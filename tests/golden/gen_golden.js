@@ -17,7 +17,8 @@
 //   submodel_golden.json, submodel_*.f64.gz (a bump submodel of the bare-die preset: set-up data and result)
 //   path_golden.json                    (thermal path plans, evaluation temperatures and step sums)
 //   package_<bare|lidded>_golden.json.gz (reference-built Draft package meshes with layered sections, two steps)
-//   package_modules_golden.json.gz      (argument package_modules: a stacked-module package, module-stage steps)
+//   package_modules_golden.json.gz      (argument package_modules: a stacked-module package, module-stage steps;
+//   package_modules_path.json.gz         with its whole thermal path, step data and 25 C field)
 //   tornado_golden.json.gz, tornado_blocks.f64.gz (arguments tornado [nproc], then tornado_screen: the
 //                                        reference's tornado cases with their plans, step data and meshes)
 // Only data leaves this script; no reference source is copied.
