@@ -217,6 +217,12 @@ int fcbga_adjoint_forward(fcbga_model *m, const fcbga_step *step, const fcbga_so
                           const fcbga_objective *obj, double *J, double *du_out, fcbga_solve_stats *stats);
 int fcbga_adjoint_backward(fcbga_model *m, const double *gJ, double *dlnE, double *dlnA, double *dlnA_elem,
                            fcbga_solve_stats *stats);
+/* The same backward pass for a caller's own objective of the step solution: dJdu (3 nn, node
+ * order) is dJ/du, the adjoint load as it is (gJ of the built-in objective taken as 0).  Sums
+ * over the steps of a thermal path give the derivatives of an objective of a path field
+ * (a field is a sum of step increments: each step's adjoint takes the field's dJ/du). */
+int fcbga_adjoint_backward_u(fcbga_model *m, const double *dJdu, double *dlnE, double *dlnA, double *dlnA_elem,
+                             fcbga_solve_stats *stats);
 
 /* Anand viscoplastic material points, batched (AnandPoint.step + commit,
  * src/core/13-fatigue.js:67-143, 229-236).  Per point: mechanical strain eps (6,

@@ -224,7 +224,7 @@ struct AdjTape {
   fcbga_solve_opts opts{};
   double p = 8, wmean = 0, S = 0, J[2] = {0, 0};
   int32_t ntop = 0, nvm = 0;
-  DBuf<double> u, lam, rhs, rhsGrid, work, ones, states, statesTh, sig, sigTh, dJds, gvm, dE, dA, red;
+  DBuf<double> u, lam, rhs, rhsGrid, work, ones, states, statesTh, sig, sigTh, dJds, gvm, dE, dA, red, dJdu;
   DBuf<int32_t> topNodes, vmElems, vmSlot;
 };
 
@@ -4374,8 +4374,10 @@ void adjoint_forward(fcbga_model *M, const fcbga_step *st, const fcbga_solve_opt
   if (stats) *stats = S;
 }
 
+// dJdu (optional, host, 3 nn in node order): a gradient of the caller's own
+// objective with respect to the step solution, added to the adjoint load
 void adjoint_backward(fcbga_model *M, const double *gJ, double *dlnE, double *dlnA, double *dlnAe,
-                      fcbga_solve_stats *stats) {
+                      fcbga_solve_stats *stats, const double *dJdu = nullptr) {
   AdjTape &T = M->tape;
   if (!T.valid) throw be::Error("adjoint backward without a forward on this model");
   be::stream_t s = M->stream;
@@ -4474,6 +4476,12 @@ void adjoint_backward(fcbga_model *M, const double *gJ, double *dlnE, double *dl
         for (int p = 0; p < 3; p++) r[3 * nd + p] += acc[p];
       });
     }
+  }
+  if (dJdu) {
+    T.dJdu.upload(dJdu, (size_t)n3, s);
+    const double *gu = T.dJdu.p;
+    double *r = T.rhs.p;
+    be::pfor(n3, s, [=] __host__ __device__(int64_t i) { r[i] += gu[i]; });
   }
   // adjoint solve K lambda = dJ/du with the forward's operator and preconditioner
   {
@@ -4982,12 +4990,28 @@ int fcbga_adjoint_forward(fcbga_model *m, const fcbga_step *step, const fcbga_so
   return FCBGA_OK;
 }
 
+static int adjoint_backward_entry(fcbga_model *m, const double *gJ, const double *dJdu, double *dlnE,
+                                  double *dlnA, double *dlnA_elem, fcbga_solve_stats *stats);
+
+int fcbga_adjoint_backward_u(fcbga_model *m, const double *dJdu, double *dlnE, double *dlnA, double *dlnA_elem,
+                             fcbga_solve_stats *stats) {
+  if (!m || !dJdu) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  const double gJ[2] = {0.0, 0.0};
+  return adjoint_backward_entry(m, gJ, dJdu, dlnE, dlnA, dlnA_elem, stats);
+}
+
 int fcbga_adjoint_backward(fcbga_model *m, const double *gJ, double *dlnE, double *dlnA, double *dlnA_elem,
                            fcbga_solve_stats *stats) {
-  if (!m || !gJ || !dlnE || !dlnA) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  if (!m || !gJ) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
+  return adjoint_backward_entry(m, gJ, nullptr, dlnE, dlnA, dlnA_elem, stats);
+}
+
+static int adjoint_backward_entry(fcbga_model *m, const double *gJ, const double *dJdu, double *dlnE,
+                                  double *dlnA, double *dlnA_elem, fcbga_solve_stats *stats) {
+  if (!dlnE || !dlnA) { g_err = "NULL argument"; return FCBGA_ERR_ARG; }
   try {
     fcbga_solve_stats S{};
-    adjoint_backward(m, gJ, dlnE, dlnA, dlnA_elem, &S);
+    adjoint_backward(m, gJ, dlnE, dlnA, dlnA_elem, &S, dJdu);
     if (stats) *stats = S;
     if (m->cancelled) {
       g_err = "cancelled";

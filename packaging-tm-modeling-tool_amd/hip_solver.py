@@ -102,7 +102,7 @@ EXPORTS = ["fcbga_model_create", "fcbga_model_destroy", "fcbga_model_ndof", "fcb
            "fcbga_vp_residual", "fcbga_set_progress_cb", "fcbga_vp_set_prescribed",
            "fcbga_path_reset", "fcbga_path_solve", "fcbga_path_sum", "fcbga_path_du", "fcbga_path_load",
            "fcbga_path_load64", "fcbga_path_copy",
-           "fcbga_mesh_fingerprint", "fcbga_prepare_step", "fcbga_last_error"]
+           "fcbga_mesh_fingerprint", "fcbga_prepare_step", "fcbga_adjoint_backward_u", "fcbga_last_error"]
 
 ANAND_STATE = 11
 STATE = 81  # stress-history state of one element (FCBGA_STATE)
@@ -221,6 +221,8 @@ def load(variant: str = "hip"):
     L.fcbga_prepare_step.argtypes = [C.c_int32, C.POINTER(FcbgaGroupGrids), C.c_int32, C.POINTER(FcbgaSectionGrids),
                                      C.c_double, C.c_double, C.c_int32, _f64p, _f64p, _u8p, _f64p, _f64p, _u8p]
     L.fcbga_prepare_step.restype = C.c_int
+    L.fcbga_adjoint_backward_u.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p, C.POINTER(FcbgaSolveStats)]
+    L.fcbga_adjoint_backward_u.restype = C.c_int
     L.fcbga_warm_reset.argtypes = [C.c_void_p]
     L.fcbga_warm_reset.restype = C.c_int
     L.fcbga_set_progress_cb.argtypes = [C.c_void_p, PROGRESS_FN, C.c_void_p, C.c_int32]
@@ -539,6 +541,24 @@ class Model:
                                            de.ctypes.data_as(_f64p) if de is not None else None, C.byref(st))
         if rc != 0:
             raise SolverError(f"fcbga_adjoint_backward failed ({rc}): {last_error(self.L)}")
+        return dE, dA, de, _record(st)
+
+    def adjoint_backward_u(self, dJdu, elem_cte=False):
+        """Backward of the pair for the caller's own objective: dJdu (3 nn) is
+        dJ/du of the forward's step solution; returns (dlnE, dlnA, dlnA_elem or
+        None, solve record) as adjoint_backward."""
+        g = np.ascontiguousarray(dJdu, np.float64).reshape(-1)
+        if len(g) != self.ndof:
+            raise SolverError(f"dJdu has {len(g)} values, the model {self.ndof} DOFs")
+        ng = int(self.mesh.e_group.max()) + 1 if len(self.mesh.groups) == 0 else len(self.mesh.groups)
+        dE, dA = np.zeros(ng), np.zeros(ng)
+        de = np.zeros(self.mesh.ne) if elem_cte else None
+        st = FcbgaSolveStats()
+        rc = self.L.fcbga_adjoint_backward_u(self.h, g.ctypes.data_as(_f64p), dE.ctypes.data_as(_f64p),
+                                             dA.ctypes.data_as(_f64p),
+                                             de.ctypes.data_as(_f64p) if de is not None else None, C.byref(st))
+        if rc != 0:
+            raise SolverError(f"fcbga_adjoint_backward_u failed ({rc}): {last_error(self.L)}")
         return dE, dA, de, _record(st)
 
     def apply(self, step: Step, x):

@@ -60,6 +60,35 @@ def warpage_jeita(mesh, u, bga):
             "rmin": float(res.min()), "rmax": float(res.max())}
 
 
+def warpage_jeita_grad(mesh, u, bga):
+    """Signed JEITA warpage and its gradient with respect to u (3 nn).  Where
+    the sites of the largest and smallest residual and the sign stay the
+    same the warpage is linear in u: J = sign (res[imax] - res[imin]) with
+    res = (I - H) w, H the least-squares plane fit over the sites (symmetric)
+    and w the bilinear samples of u_z on plane 0, so dJ/dw = sign (I - H)
+    (e_imax - e_imin), taken back to the nodes through the sampling weights."""
+    bx, by, _ = bga
+    bx, by = np.asarray(bx, np.float64), np.asarray(by, np.float64)
+    r = warpage_jeita(mesh, u, bga)
+    w = sample_plane_w(mesh, u, bx, by)
+    c0, c1, c2 = fit_plane(bx, by, w)
+    res = w - (c0 + c1 * bx + c2 * by)
+    imax, imin = int(np.argmax(res)), int(np.argmin(res))
+    e = np.zeros(len(bx))
+    e[imax] += 1.0
+    e[imin] -= 1.0
+    X = np.stack([np.ones_like(bx), bx, by], 1)
+    dw = r["sign"] * (e - X @ np.linalg.solve(X.T @ X, X.T @ e))
+    g = np.zeros(3 * mesh.nn)
+    i, j = _interval(mesh.xs, bx), _interval(mesh.ys, by)
+    tx = (bx - mesh.xs[i]) / (mesh.xs[i + 1] - mesh.xs[i])
+    ty = (by - mesh.ys[j]) / (mesh.ys[j + 1] - mesh.ys[j])
+    for a, b, wt in ((0, 0, (1 - tx) * (1 - ty)), (1, 0, tx * (1 - ty)), (0, 1, (1 - tx) * ty), (1, 1, tx * ty)):
+        n = mesh.node_id[((i + a) * mesh.ny + j + b) * mesh.nz]
+        np.add.at(g, 3 * n + 2, wt * dw)
+    return r["signed"], g
+
+
 # ---------------------------------------------------------------------------
 # Stress post-processing on the device stress histories
 # ---------------------------------------------------------------------------

@@ -79,6 +79,28 @@ def run_case(model, mesh, steps, plan, step_of, temps_all, bga, geom, peak, tol=
     return out, recs
 
 
+def path_field_gradient(model, steps, ids, step_of, dJdu, tol=1e-10, precision=None):
+    """Log-derivatives with respect to every group's E and CTE of an objective of
+    a path field u_T = sum over the steps ids of their increments (the fields of
+    finishPath, src/core/11-analysis.js:236-247), given dJ/du_T: each step k
+    solves K_k du_k = f_k, so dJ/dtheta = sum_k lambda_k . (df_k/dtheta -
+    dK_k/dtheta du_k) with K_k lambda_k = dJ/du_T, one forward re-solve and one
+    adjoint solve per step (hip_solver.Model.adjoint_forward /
+    adjoint_backward_u).  Returns (dlnE, dlnA) per group: scaling a group's
+    stiffness (and with it its thermal stress) or its thermal strain by f
+    changes J by dlnE[g] ln f or dlnA[g] ln f to first order."""
+    from . import hip_solver as hs
+    precision = hs.PREC_FP64 if precision is None else precision
+    dE = dA = 0.0
+    for k in ids:
+        model.adjoint_forward(step_of(steps[k]), 0, np.zeros(0, np.int32), 8.0, tol=tol, precision=precision)
+        e, a, _, rec = model.adjoint_backward_u(dJdu)
+        if not rec["converged"]:
+            raise hs.SolverError(f"adjoint solve of step {k} did not converge")
+        dE, dA = dE + e, dA + a
+    return dE, dA
+
+
 def tornado_from_cases(inputs, base, cases, keys=OUTPUT_KEYS):
     """Tornado rows per output (tornadoFromCases, src/core/14-sensitivity.js:159-172):
     for every input with both cases, the low and high changes from the base, the

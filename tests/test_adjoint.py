@@ -214,3 +214,39 @@ def test_node_addon_sensitivity_pair():
     assert out.returncode == 0, out.stderr
     rec = json.loads(out.stdout.strip().splitlines()[-1])
     assert rec["rel_err"] <= 1e-5 and rec["elem_sum_rel_err"] <= 1e-10, rec
+
+
+def test_path_field_gradient_matches_central_differences():
+    """An objective of a path field (the sum of two step increments with
+    different step data, as finishPath sums them) through the per-step adjoint
+    of a caller-given dJ/du (fcbga_adjoint_backward_u): every group's ln E
+    derivative against central differences of the two re-solved steps."""
+    from fcbga_amd import sensitivity as sens
+    spec = golden_spec("small_box")
+    mesh = meshspec.build_mesh(spec)
+    model = hs.Model(mesh, variant="hostemu")
+    d1 = meshspec.group_step_data(spec, 150, 25, 0)
+    d2 = meshspec.group_step_data(spec, 25, -40, 0)
+    steps = [dict(Ta=150, Tb=25, data=d1), dict(Ta=25, Tb=-40, data=(2 * d2[0], 1.5 * d2[1], d2[2]))]
+
+    def step_of(s, grp=None, f=1.0):
+        D, s0, born = (a.copy() for a in s["data"])
+        if grp is not None:
+            D[grp] *= f
+            s0[grp] *= f
+        return hs.Step(s["Ta"], s["Tb"], 0, D, s0, born)
+    gvec = np.random.default_rng(0).normal(size=mesh.ndof)
+
+    def J(grp=None, f=1.0):
+        u = 0
+        for s in steps:
+            du, _ = model.solve_step(step_of(s, grp, f), tol=1e-13, precision=hs.PREC_FP64)
+            u = u + du
+        return gvec @ u
+    dE, _ = sens.path_field_gradient(model, steps, [0, 1], step_of, gvec, tol=1e-13)
+    h = 1e-4
+    fd = np.array([(J(g, np.exp(h)) - J(g, np.exp(-h))) / (2 * h) for g in range(len(d1[0]))])
+    big = np.abs(fd).max()
+    assert np.abs(dE - fd).max() <= 1e-5 * big, (dE, fd)
+    with pytest.raises(hs.SolverError):
+        model.adjoint_backward_u(np.zeros(3))
