@@ -169,7 +169,8 @@ class HipStateView {
  * forward(step, objective) solves the step and returns J = [warpage proxy on
  * plane kTop, von Mises p-norm over objective.vmElems], keeping the context on
  * the model; backward(gJ) returns the log-derivatives of gJ . J for every
- * group's E and CTE and every element's CTE from one adjoint solve:
+ * group's E and CTE and every element's CTE from one adjoint solve (gJ of 3 nn
+ * values instead: dJ/du of the caller's own objective of the step solution):
  * dJ/dE_g = dlnE[g] / E_g, dJ/dCTE_g = dlnCte[g] / CTE_g.  The reference's
  * finite-difference tornado (runSensitivityCase / tornadoFromCases,
  * src/core/14-sensitivity.js:8-172) runs beside it unchanged, each case's

@@ -590,6 +590,7 @@ napi_value AdjointForward(napi_env env, napi_callback_info info) {
 }
 
 // adjointBackward(model, gJ: Float64Array(2)[, elemCte]) -> {dlnE, dlnCte, dlnCteElem, iters, relres, converged}
+// with gJ a Float64Array(3 nn) instead: dJ/du of the caller's own objective (fcbga_adjoint_backward_u)
 napi_value AdjointBackward(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
@@ -599,8 +600,9 @@ napi_value AdjointBackward(napi_env env, napi_callback_info info) {
   if (!b || !b->m) return throw_msg(env, "adjointBackward: not a live model");
   void *g;
   size_t n;
-  if (!arg_typed(env, argv[1], napi_float64_array, &g, &n) || n != 2)
-    return throw_msg(env, "adjointBackward: gJ must be a Float64Array(2)");
+  const size_t ndof = (size_t)fcbga_model_ndof(b->m);
+  if (!arg_typed(env, argv[1], napi_float64_array, &g, &n) || (n != 2 && n != ndof))
+    return throw_msg(env, "adjointBackward: gJ must be a Float64Array(2) or dJ/du, a Float64Array(3 nn)");
   bool elem = true;
   if (argc >= 3) napi_get_value_bool(env, argv[2], &elem);
   const int32_t ng = b->ngroups, ne = b->ne;
@@ -609,7 +611,8 @@ napi_value AdjointBackward(napi_env env, napi_callback_info info) {
   if (elem) elv = make_f64(env, (size_t)ne, &de);
   fcbga_solve_stats S;
   memset(&S, 0, sizeof S);
-  const int rc = fcbga_adjoint_backward(b->m, (const double *)g, dE, dA, de, &S);
+  const int rc = n == 2 ? fcbga_adjoint_backward(b->m, (const double *)g, dE, dA, de, &S)
+                        : fcbga_adjoint_backward_u(b->m, (const double *)g, dE, dA, de, &S);
   if (rc != FCBGA_OK && rc != FCBGA_ERR_NOT_CONVERGED) return throw_msg(env, "adjointBackward: " + lib_error());
   napi_value rec, cv;
   NAPI_OK(napi_create_object(env, &rec));

@@ -37,5 +37,26 @@ for (let q = 0; q < groups.length; q++) {
 let sumElem = 0, sumGroup = 0;
 for (let e = 0; e < mesh.ne; e++) sumElem += g.dlnCteElem[e];
 for (let q = 0; q < groups.length; q++) sumGroup += g.dlnCte[q];
+// the same backward from the caller's dJ/du: the warpage proxy's gradient, 2 (w - mean w) on
+// the z components of plane kTop, gives what gJ = [1, 0] gives
+const fw = sens.forward(stepOf(groups), objective);
+const top = [];
+for (let i = 0; i < mesh.nx; i++) for (let j = 0; j < mesh.ny; j++) {
+  const n = mesh.nodeId[(i * mesh.ny + j) * mesh.nz + objective.kTop];
+  if (n >= 0) top.push(n);
+}
+let wm = 0;
+for (const n of top) wm += fw.u[3 * n + 2];
+wm /= top.length;
+const dJdu = new Float64Array(3 * mesh.nn);
+for (const n of top) dJdu[3 * n + 2] = 2 * (fw.u[3 * n + 2] - wm);
+const gu = sens.backward(dJdu, false);
+sens.forward(stepOf(groups), objective);
+const g1 = sens.backward([1, 0], false);
+let uerr = 0, ubig = 0;
+for (let q = 0; q < groups.length; q++) {
+  ubig = Math.max(ubig, Math.abs(g1.dlnE[q]), Math.abs(g1.dlnCte[q]));
+  uerr = Math.max(uerr, Math.abs(gu.dlnE[q] - g1.dlnE[q]), Math.abs(gu.dlnCte[q] - g1.dlnCte[q]));
+}
 solver.destroy();
-console.log(JSON.stringify({ name, groups: groups.length, rel_err: worst / big, elem_sum_rel_err: Math.abs(sumElem - sumGroup) / Math.abs(sumGroup), adj_iters: g.iters }));
+console.log(JSON.stringify({ name, groups: groups.length, rel_err: worst / big, elem_sum_rel_err: Math.abs(sumElem - sumGroup) / Math.abs(sumGroup), adj_iters: g.iters, dJdu_rel_err: uerr / ubig }));

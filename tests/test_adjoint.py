@@ -214,6 +214,8 @@ def test_node_addon_sensitivity_pair():
     assert out.returncode == 0, out.stderr
     rec = json.loads(out.stdout.strip().splitlines()[-1])
     assert rec["rel_err"] <= 1e-5 and rec["elem_sum_rel_err"] <= 1e-10, rec
+    # the backward from the caller's dJ/du (the proxy's own gradient) equals the built-in one
+    assert rec["dJdu_rel_err"] <= 1e-10, rec
 
 
 def test_path_field_gradient_matches_central_differences():
