@@ -373,20 +373,73 @@ def screening_timing(hs, scr, reps=5):
                     "includes the host copies of the strain table and results"}
 
 
+# the property curve each material of the lidded workload follows: the index of
+# the group of the reference's lidded default preset (its prepared grids,
+# tests/golden/prepare_golden.json) whose E(T) / E(25 C) and thermal strain
+# shape it takes: the underfill's glass transition for the polymers, the
+# silicon's rising CTE, the TIM's constant properties, the copper lid's CTE
+# and the lid adhesive's curve
+PATH_CURVE = {"buildup": 2, "core": 5, "bump_uf": 2, "si": 3, "tim": 4, "lid_cu": 5, "lid_adh": 6}
+
+
+def path_groups(spec):
+    """Temperature-dependent step data for the lidded workload's materials
+    (the inputs of fcbga_prepare_step): each material's modulus and CTE at
+    25 C scaled along the 1 C grid by the shape of the reference preset group
+    PATH_CURVE names, E(T) = E25 Eref(T) / Eref(25) and F(T) = Fref(T)
+    alpha25 / alpha_ref(25)."""
+    import gzip
+    from fcbga_amd import hip_solver as hs
+    from fcbga_amd.meshspec import column_d, iso_d
+    gold = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(gold, "prepare_golden.json")) as f:
+        lay = json.load(f)
+    with open(os.path.join(gold, "prepare_grids.f64.gz"), "rb") as f:
+        flat = np.frombuffer(gzip.decompress(f.read()), dtype="<f8")
+    n, i25 = hs.TGRID_N, 25 - hs.TGRID_MIN
+    groups = []
+    for mt in spec["materials"]:
+        ref = lay["groups"][PATH_CURVE[mt["name"]]]
+        F = flat[ref["F"][0]:ref["F"][0] + ref["F"][1]].reshape(n, 3)
+        a25 = 0.5 * (F[i25 + 1, 0] - F[i25 - 1, 0])
+        g = {"stage": mt.get("stage", 0), "F": F * (mt["cte"] * 1e-6 / a25)}
+        Dfn = column_d if mt.get("column") else iso_d
+        if ref["Dgrid"] is not None:
+            Dr = flat[ref["Dgrid"][0]:ref["Dgrid"][0] + ref["Dgrid"][1]].reshape(n, 36)
+            shape = Dr[:, 0] / Dr[i25, 0]
+            g["Dgrid"] = np.stack([Dfn(mt["E"] * f, mt["nu"]).reshape(-1) for f in shape])
+        else:
+            g["Dconst"] = Dfn(mt["E"], mt["nu"]).reshape(-1)
+        groups.append(g)
+    return groups
+
+
 def path_timing(torch, hs, model, spec, tol):
     """The lidded package's thermal path (Analysis.run): the reference's plan
     for a lidded package without Tg values (process 217 -> 165 -> 150 -> 25 C,
     sweeps to -40 and 260 C), every step solved on the device with one
     warm-start basis of 20 solutions along the path, fields summed in device
-    memory."""
+    memory.  Step data from temperature-dependent property grids
+    (path_groups through fcbga_prepare_step), so that each step's stiffness
+    and load shape differ; the same path with the 25 C properties throughout
+    is timed beside it."""
     from fcbga_amd import path as P
     cfg = {"packageType": "lid", "process": {"Tsub": 217, "Tjoin": 217, "Tuf": 165, "Tattach": 150},
            "reflow": {"peak": 260, "reportTemps": [25, 150, 220, 260]}, "cycling": {"Tmin": -40, "Tmax": 125}}
     steps, path, temps = P.build_path(cfg, [], {0: [], 1: [], 2: [], 3: []}, [217], "full")
+    groups = path_groups(spec)
+
+    def step_of(s):
+        return hs.prepare_step(groups, [], s["Ta"], s["Tb"], 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fields, _, _, recs = P.run_path(model, steps, path, temps["all"],
-                                    lambda s: hs.step_for_spec(spec, s["Ta"], s["Tb"], 0), tol=tol)
+    _, _, _, rc = P.run_path(model, steps, path, temps["all"],
+                              lambda s: hs.step_for_spec(spec, s["Ta"], s["Tb"], 0), tol=tol)
+    torch.cuda.synchronize()
+    dtc = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fields, _, _, recs = P.run_path(model, steps, path, temps["all"], step_of, tol=tol)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     iters = [r["iters"] for r in recs]
@@ -394,8 +447,7 @@ def path_timing(torch, hs, model, spec, tol):
     extra = [hs.Model(model.mesh, variant="hip") for _ in range(2)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fb, _, _, rb = P.run_path_blocks([model] + extra, steps, path, temps["all"],
-                                     lambda s: hs.step_for_spec(spec, s["Ta"], s["Tb"], 0), tol=tol)
+    fb, _, _, rb = P.run_path_blocks([model] + extra, steps, path, temps["all"], step_of, tol=tol)
     torch.cuda.synchronize()
     dtb = time.perf_counter() - t0
     scale = max(np.abs(v).max() for v in fields.values())
@@ -404,9 +456,13 @@ def path_timing(torch, hs, model, spec, tol):
     return {"steps": len(steps), "fields": len(fields), "s": dt, "s_per_step": dt / len(steps),
             "iters_total": int(sum(iters)), "iters_mean": float(np.mean(iters)),
             "iters0_mean_warm": float(np.mean([r["r0"] for r in recs[1:]])),
-            "note": "one warm-started mixed solve per step (basis of 20), fresh multigrid per step; with "
-                    "temperature-independent properties every increment is a multiple of one load shape, so the "
-                    "projection onto the stored solutions already meets the tolerance on most steps",
+            "materials": "temperature-dependent: each material's 25 C modulus and CTE along the property curve "
+                         "of a group of the reference's lidded default preset (glass transitions of the polymers)",
+            "note": "one warm-started mixed solve per step (basis of 20), fresh multigrid per step",
+            "constant_properties": {"s": dtc, "iters_total": int(sum(r["iters"] for r in rc)),
+                                    "note": "the same path with the 25 C properties throughout: every increment is "
+                                            "then a multiple of one load shape and the projection onto the stored "
+                                            "solutions already meets the tolerance on most steps"},
             "blocks": {"instances": 3, "s": dtb, "iters_total": int(sum(r["iters"] for r in rb)),
                        "fields_rel_diff_vs_one": float(diff),
                        "note": "blocks of consecutive steps from one queue over three model instances, each with "

@@ -120,3 +120,26 @@ def test_bad_input_refused():
         hs.prepare_step(groups, [dict(sections[0], group=len(groups))], 20.0, 30.0, 0, variant="hostemu")
     with pytest.raises(hs.SolverError):
         hs.prepare_step(groups, sections, float("nan"), 30.0, 0, variant="hostemu")
+
+
+def test_path_material_curves():
+    """The temperature-dependent materials of the benchmarked thermal path: at
+    25 C every material keeps its 25 C modulus and CTE, and the polymers that
+    follow the underfill's curve lose about 99 % of their stiffness and gain
+    about 3.7x in CTE above its glass transition."""
+    import bench
+    from fcbga_amd import cases
+    spec = cases.cfg2_spec()
+    groups = bench.path_groups(spec)
+    assert len(groups) == len(spec["materials"])
+    for mt, gr in zip(spec["materials"], groups):
+        st = hs.prepare_step([gr], [], 24.5, 25.5, 0, variant="hostemu")
+        ref = hs.step_for_spec(dict(spec, materials=[mt]), 24.5, 25.5, 0)
+        assert np.abs(st.D - ref.D).max() <= 1e-2 * np.abs(ref.D).max(), mt["name"]
+        assert np.abs(st.s0 - ref.s0).max() <= 1e-2 * np.abs(ref.s0).max(), mt["name"]
+    i = [m["name"] for m in spec["materials"]].index("bump_uf")
+    hot = hs.prepare_step([groups[i]], [], 240.0, 241.0, 0, variant="hostemu")
+    cold = hs.prepare_step([groups[i]], [], 24.5, 25.5, 0, variant="hostemu")
+    assert 0.005 < hot.D.reshape(-1)[0] / cold.D.reshape(-1)[0] < 0.02
+    fh, fc = hot.s0.reshape(-1)[0] / hot.D.reshape(-1)[0], cold.s0.reshape(-1)[0] / cold.D.reshape(-1)[0]
+    assert 3.0 < fh / fc < 4.5
