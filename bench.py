@@ -453,7 +453,7 @@ def path_timing(torch, hs, model, spec, tol):
     scale = max(np.abs(v).max() for v in fields.values())
     diff = max(np.abs(fb[T] - fields[T]).max() for T in fields) / scale
     del extra
-    return {"steps": len(steps), "fields": len(fields), "s": dt, "s_per_step": dt / len(steps),
+    return {"steps": len(steps), "fields": len(fields), "tol": tol, "s": dt, "s_per_step": dt / len(steps),
             "iters_total": int(sum(iters)), "iters_mean": float(np.mean(iters)),
             "iters0_mean_warm": float(np.mean([r["r0"] for r in recs[1:]])),
             "materials": "temperature-dependent: each material's 25 C modulus and CTE along the property curve "
@@ -836,7 +836,9 @@ def main():
         f"{out['anand']['ms_per_update']:.3f} ms per batch)")
 
     if not args.no_path and args.config == "cfg2":
-        out["path"] = path_timing(torch, hs, model, spec, args.tol)
+        # at the tolerance the reference's Analysis gives its solver (PCG_TOL,
+        # src/core/00-constants.js:60, src/core/11-analysis.js:110)
+        out["path"] = path_timing(torch, hs, model, spec, 1e-8)
         log(f"thermal path: {out['path']['steps']} steps in {out['path']['s']:.2f} s "
             f"({out['path']['iters_mean']:.1f} iterations per step, warm start); over 3 instances "
             f"{out['path']['blocks']['s']:.2f} s (fields differ by {out['path']['blocks']['fields_rel_diff_vs_one']:.1e})")
